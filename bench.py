@@ -1,0 +1,154 @@
+"""Benchmark: subset evaluations/s of the Gray-code coset sweep on one MI355X.
+
+Workload (default): 32 x 1024 bit matrix, Bernoulli(0.3) bits from
+XorShift64(0x5EED0002); for every target row t, all subsets of the pivot block
+rows [0, 24) minus t are visited in reflected-Gray order, one row XOR plus one
+popcount per step; per target the minimum weight, the first Gray step reaching
+it and the lowest block mask reaching it are produced.  Subset evaluations per
+pass = 8 * 2^24 + 24 * 2^23 = 335,544,320.
+
+A step is one full pass over all 32 targets with the matrix already resident
+in HBM.  Timing uses HIP events on the stream the kernels are launched on.
+
+Prints one JSON line (see the README section of DESIGN.md for the fields).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "subset evals/s (FWHT, Gray); GA time-to-1%-of-planted, learned vs plain"
+SEED_C2 = 0x5EED0002
+ROWS, COLS, P, BLOCK = 32, 1024, 0.3, 24
+GRAY_OPS_PER_EVAL = 66      # 32 v_xor_b32 + 32 v_bcnt_u32_b32 + 2 key minima
+VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12  # INT32 VALU, non-packed
+
+
+def c2_evals() -> int:
+    return sum((1 << (BLOCK - 1)) if t < BLOCK else (1 << BLOCK) for t in range(ROWS))
+
+
+def cpu_baseline(matrix: np.ndarray, budget_s: float = 12.0) -> dict:
+    """Reference CPU brute force (BruteforceSearch, compiled from the reference
+    sources into oracle/_ref) on windows [block rows..., target], k = 1: each
+    window visits all 2^25 masks.  Falls back to the project's C restatement
+    (oracle/liboracle.so) where the reference build is absent."""
+    import oracle  # checker / baseline only
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    ref = oracle.reference()
+    if ref is not None:
+        evals = 0
+        t0 = time.perf_counter()
+        done = 0
+        for target in range(BLOCK, ROWS):
+            win = np.concatenate([matrix[:BLOCK], matrix[target:target + 1]])
+            ref.bruteforce_search(win, COLS, 1)
+            evals += 1 << (BLOCK + 1)
+            done += 1
+            if time.perf_counter() - t0 > budget_s:
+                break
+        dt = time.perf_counter() - t0
+        return {"value": evals / dt, "unit": "subset evals/s", "cores": threads,
+                "kind": "reference",
+                "sample": f"reference BruteforceSearch(k=1) on {done} windows of 25 rows "
+                          f"(24-row pivot block + target) x 1024 cols, 2^25 masks each, "
+                          f"OpenMP over prefixes"}
+    ora = oracle.oracle()
+    t0 = time.perf_counter()
+    evals = 0
+    done = 0
+    for target in range(BLOCK, ROWS):
+        ora.gray_min(matrix, BLOCK, target)
+        evals += 1 << BLOCK
+        done += 1
+        if time.perf_counter() - t0 > budget_s:
+            break
+    dt = time.perf_counter() - t0
+    return {"value": evals / dt, "unit": "subset evals/s", "cores": 1, "kind": "port",
+            "sample": f"C restatement Gray sweep, {done} targets x 2^24 subsets"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--chunk-bits", type=int, default=0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 "
+                                   "(multi-GPU sharding is described, not built)"}))
+        return 2
+
+    import torch
+    from bmmpy_amd import _lib
+
+    dev = torch.device("cuda:0")
+    matrix = _lib.gen_bernoulli(SEED_C2, ROWS, COLS, P)
+    d_rows = torch.from_numpy(matrix.view(np.int64).copy()).to(dev)
+    targets = np.arange(ROWS, dtype=np.int32)
+    d_first = torch.empty(ROWS, dtype=torch.int64, device=dev)
+    d_mask = torch.empty(ROWS, dtype=torch.int64, device=dev)
+    stream = torch.cuda.current_stream(dev)
+    sp = stream.cuda_stream
+
+    def step():
+        _lib.call("bmm_hip_gray_coset_min_dev", d_rows.data_ptr(), ROWS, COLS // 64, BLOCK,
+                  targets, ROWS, args.chunk_bits, d_first.data_ptr(), d_mask.data_ptr(), sp)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    start.record(stream)
+    for _ in range(args.steps):
+        step()
+    end.record(stream)
+    torch.cuda.synchronize()
+    ms = start.elapsed_time(end) / args.steps
+
+    # Results of the last pass (checked against the oracle by the tests).
+    keys = d_first.cpu().numpy().view(np.uint64)
+    evals = c2_evals()
+    value = evals / (ms / 1e3)
+    achieved_tops = evals * GRAY_OPS_PER_EVAL / (ms / 1e3) / 1e12
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "subset evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic",
+        "config": {"workload": "gray_coset_sweep", "rows": ROWS, "cols": COLS, "p": P,
+                   "pivot_block": BLOCK, "targets": ROWS, "seed": hex(SEED_C2),
+                   "subset_evals_per_step": evals},
+        "roofline": {"bound": "valu", "achieved": achieved_tops, "peak": VALU_PEAK_TOPS,
+                     "unit": "TOPS (int32 VALU)", "frac": achieved_tops / VALU_PEAK_TOPS,
+                     "traffic": None,
+                     "ops_per_eval": GRAY_OPS_PER_EVAL},
+        "result_min_weights": [int(k >> 32) for k in keys],
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(matrix)
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

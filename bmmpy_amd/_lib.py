@@ -1,0 +1,228 @@
+"""ctypes binding of ``libbmmpy_hip.so`` (the C ABI in ``include/bmmpy_hip.h``).
+
+The library is required: importing a compute wrapper without it raises, there
+is no CPU fallback.  Status codes map onto the reference's exception types
+(invalid_argument -> ValueError, overflow_error -> OverflowError,
+out_of_range -> IndexError, runtime_error -> RuntimeError).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libbmmpy_hip.so")
+
+BMM_OK, BMM_ERR_INVALID, BMM_ERR_OVERFLOW, BMM_ERR_RANGE, BMM_ERR_DEVICE = range(5)
+TOPK_MAX = 4096
+
+_u64p = np.ctypeslib.ndpointer(np.uint64, flags="C_CONTIGUOUS")
+_u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
+_i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+_i64p = np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")
+_i16p = np.ctypeslib.ndpointer(np.int16, flags="C_CONTIGUOUS")
+_u16p = np.ctypeslib.ndpointer(np.uint16, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+_f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+_vp = C.c_void_p
+_sz = C.c_size_t
+
+
+class TopKEntry(C.Structure):
+    _fields_ = [("mask", C.c_uint64), ("weight", C.c_uint32), ("pad", C.c_uint32)]
+
+
+TOPK_DTYPE = np.dtype([("mask", np.uint64), ("weight", np.uint32), ("pad", np.uint32)])
+_topkp = np.ctypeslib.ndpointer(TOPK_DTYPE, flags="C_CONTIGUOUS")
+
+
+class DeviceInfo(C.Structure):
+    _fields_ = [("device_count", C.c_int), ("device", C.c_int), ("compute_units", C.c_int),
+                ("wave_size", C.c_int), ("lds_per_block", C.c_int), ("clock_khz", C.c_int),
+                ("arch", C.c_char * 64), ("name", C.c_char * 128)]
+
+
+# name -> (argtypes, restype)
+SIGNATURES = {
+    "bmm_hip_last_error": ([], C.c_char_p),
+    "bmm_hip_version": ([], C.c_char_p),
+    "bmm_hip_device_info": ([C.POINTER(DeviceInfo)], C.c_int),
+    "bmm_hip_fwht_i32": ([_i32p, _sz], C.c_int),
+    "bmm_hip_fwht_i16": ([_i16p, _sz], C.c_int),
+    "bmm_hip_fwht_i32_dev": ([_vp, _sz, _vp], C.c_int),
+    "bmm_hip_fwht_i16_dev": ([_vp, _sz, _vp], C.c_int),
+    "bmm_hip_fwht_topk": ([_u64p, _sz, _sz, _sz, _sz, _topkp, C.POINTER(_sz)], C.c_int),
+    "bmm_hip_mitm_topk": ([_u64p, _u64p, _i32p, _sz, _sz, _sz, C.c_int32, _sz, _topkp,
+                           C.POINTER(_sz)], C.c_int),
+    "bmm_hip_bruteforce_topk": ([_u64p, _sz, _sz, _sz, _sz, _topkp, C.POINTER(_sz)], C.c_int),
+    "bmm_hip_coset_min": ([_u64p, _sz, _sz, _sz, _u32p, _u64p], C.c_int),
+    "bmm_hip_coset_min_batch": ([_u64p, _i64p, _i32p, _i32p, _sz, _i64p, _u32p, _u64p],
+                                C.c_int),
+    "bmm_hip_gray_coset_min": ([_u64p, _sz, _sz, _sz, _i32p, _sz, _u32p, _u64p, _u64p],
+                               C.c_int),
+    "bmm_hip_gray_coset_min_dev": ([_vp, _sz, _sz, _sz, _i32p, _sz, _sz, _vp, _vp, _vp],
+                                   C.c_int),
+    "bmm_gen_bernoulli": ([C.c_uint64, _sz, _sz, C.c_double, _u64p], C.c_int),
+    "bmm_gen_planted": ([C.c_uint64, _sz, _sz, C.c_double, _u64p, _u64p, _u64p], C.c_int),
+    "bmm_gen_batch": ([C.c_uint64, _sz, _i32p, _i32p, _f64p, _vp, _sz, _i64p, C.POINTER(_sz)],
+                      C.c_int),
+}
+
+_lib = None
+
+
+def load() -> C.CDLL:
+    """Load libbmmpy_hip.so; raises ImportError when it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: build it with "
+                              "`python -c 'import __graft_entry__ as g; g.build()'`")
+        lib = C.CDLL(LIB_PATH)
+        for name, (argt, rest) in SIGNATURES.items():
+            fn = getattr(lib, name, None)
+            if fn is None:
+                continue
+            fn.argtypes = argt
+            fn.restype = rest
+        _lib = lib
+    return _lib
+
+
+def exported_symbols() -> list[str]:
+    lib = load()
+    return [n for n in SIGNATURES if hasattr(lib, n)]
+
+
+_EXC = {BMM_ERR_INVALID: ValueError, BMM_ERR_OVERFLOW: OverflowError, BMM_ERR_RANGE: IndexError,
+        BMM_ERR_DEVICE: RuntimeError}
+
+
+def check(rc: int) -> None:
+    if rc != BMM_OK:
+        msg = load().bmm_hip_last_error().decode()
+        raise _EXC.get(rc, RuntimeError)(msg)
+
+
+def call(name: str, *args) -> None:
+    check(getattr(load(), name)(*args))
+
+
+def device_info() -> DeviceInfo:
+    d = DeviceInfo()
+    call("bmm_hip_device_info", C.byref(d))
+    return d
+
+
+def _rows(m) -> np.ndarray:
+    return np.ascontiguousarray(m, dtype=np.uint64)
+
+
+# ---- thin typed wrappers ---------------------------------------------------
+def fwht_i32(x) -> np.ndarray:
+    a = np.array(x, dtype=np.int32, copy=True)
+    call("bmm_hip_fwht_i32", a, a.size)
+    return a
+
+
+def fwht_i16(x) -> np.ndarray:
+    a = np.array(x, dtype=np.int16, copy=True)
+    call("bmm_hip_fwht_i16", a, a.size)
+    return a
+
+
+def _entries(out, n):
+    return [(int(out["weight"][i]), int(out["mask"][i])) for i in range(n)]
+
+
+def fwht_topk(rows, cols: int, k: int):
+    rows = _rows(rows)
+    out = np.zeros(max(k, 1), dtype=TOPK_DTYPE)
+    n = _sz()
+    call("bmm_hip_fwht_topk", rows, rows.shape[0], rows.shape[1], cols, k, out, C.byref(n))
+    return _entries(out, n.value)
+
+
+def mitm_topk(q, r, mult, low_bits: int, high_bits: int, total: int, k: int):
+    q = np.ascontiguousarray(q, dtype=np.uint64)
+    r = np.ascontiguousarray(r, dtype=np.uint64)
+    mult = np.ascontiguousarray(mult, dtype=np.int32)
+    out = np.zeros(max(k, 1), dtype=TOPK_DTYPE)
+    n = _sz()
+    call("bmm_hip_mitm_topk", q, r, mult, q.size, low_bits, high_bits, total, k, out,
+         C.byref(n))
+    return _entries(out, n.value)
+
+
+def bruteforce_topk(rows, k: int, chunk_bits: int = 0):
+    rows = _rows(rows)
+    out = np.zeros(max(k, 1), dtype=TOPK_DTYPE)
+    n = _sz()
+    call("bmm_hip_bruteforce_topk", rows, rows.shape[0], rows.shape[1], chunk_bits, k, out,
+         C.byref(n))
+    return _entries(out, n.value)
+
+
+def coset_min(rows, cols: int):
+    rows = _rows(rows)
+    w = np.zeros(rows.shape[0], np.uint32)
+    i = np.zeros(rows.shape[0], np.uint64)
+    call("bmm_hip_coset_min", rows, rows.shape[0], rows.shape[1], cols, w, i)
+    return w, i
+
+
+def coset_min_batch(words, word_offset, rows, cols, out_offset=None):
+    words = np.ascontiguousarray(words, dtype=np.uint64)
+    word_offset = np.ascontiguousarray(word_offset, dtype=np.int64)
+    rows = np.ascontiguousarray(rows, dtype=np.int32)
+    cols = np.ascontiguousarray(cols, dtype=np.int32)
+    if out_offset is None:
+        out_offset = np.concatenate([[0], np.cumsum(rows)[:-1]]).astype(np.int64)
+    out_offset = np.ascontiguousarray(out_offset, dtype=np.int64)
+    n_slots = int((out_offset + rows).max()) if rows.size else 0
+    w = np.zeros(max(n_slots, 1), np.uint32)
+    i = np.zeros(max(n_slots, 1), np.uint64)
+    call("bmm_hip_coset_min_batch", words, word_offset, rows, cols, rows.size, out_offset, w, i)
+    return w[:n_slots], i[:n_slots]
+
+
+def gray_coset_min(rows, block_rows: int, targets):
+    rows = _rows(rows)
+    targets = np.ascontiguousarray(targets, dtype=np.int32)
+    n = targets.size
+    w = np.zeros(n, np.uint32)
+    f = np.zeros(n, np.uint64)
+    m = np.zeros(n, np.uint64)
+    call("bmm_hip_gray_coset_min", rows, rows.shape[0], rows.shape[1], block_rows, targets, n,
+         w, f, m)
+    return w, f, m
+
+
+def gen_bernoulli(seed: int, n_rows: int, cols: int, p: float) -> np.ndarray:
+    out = np.zeros((n_rows, (cols + 63) // 64), dtype=np.uint64)
+    call("bmm_gen_bernoulli", seed, n_rows, cols, p, out)
+    return out
+
+
+def gen_planted(seed: int, n_rows: int, cols: int, p: float):
+    w, wn = (cols + 63) // 64, (n_rows + 63) // 64
+    S = np.zeros((n_rows, w), np.uint64)
+    A = np.zeros((n_rows, w), np.uint64)
+    M = np.zeros((n_rows, wn), np.uint64)
+    call("bmm_gen_planted", seed, n_rows, cols, p, S, A, M)
+    return S, A, M
+
+
+def gen_batch(seed: int, n_mat: int):
+    rows = np.zeros(n_mat, np.int32)
+    cols = np.zeros(n_mat, np.int32)
+    dens = np.zeros(n_mat, np.float64)
+    offs = np.zeros(n_mat, np.int64)
+    nw = _sz()
+    cap = n_mat * 28 * 32
+    words = np.zeros(cap, np.uint64)
+    call("bmm_gen_batch", seed, n_mat, rows, cols, dens, words.ctypes.data, cap, offs,
+         C.byref(nw))
+    return rows, cols, dens, words[:nw.value].copy(), offs

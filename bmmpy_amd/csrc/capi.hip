@@ -1,0 +1,250 @@
+// extern "C" boundary of libbmmpy_hip.so (declared in include/bmmpy_hip.h).
+// Validation happens before any device query; device and host errors become
+// BMM_* status codes with a thread-local message.
+#include "common.hpp"
+
+#include <cstring>
+#include <string>
+#include <vector>
+
+namespace bmmhip {
+
+namespace {
+thread_local std::string g_last_error;
+}
+
+void set_last_error(const char* msg) { g_last_error = msg ? msg : ""; }
+
+__global__ void fill_u64_kernel(uint64_t* p, size_t n, uint64_t v) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+void fill_u64(uint64_t* p, size_t n, uint64_t v, hipStream_t s) {
+    if (n == 0)
+        return;
+    const unsigned grid = (unsigned)std::min<size_t>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(fill_u64_kernel, dim3(grid), dim3(256), 0, s, p, n, v);
+    BMM_HIP(hipGetLastError());
+}
+
+__global__ void fill_u32_kernel(uint32_t* p, size_t n, uint32_t v) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+void fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t s) {
+    if (n == 0)
+        return;
+    const unsigned grid = (unsigned)std::min<size_t>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(fill_u32_kernel, dim3(grid), dim3(256), 0, s, p, n, v);
+    BMM_HIP(hipGetLastError());
+}
+
+// Fails with BMM_ERR_DEVICE when no device is visible (checked after argument
+// validation, like the reference searchers).
+void require_device() {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0)
+        fail(BMM_ERR_DEVICE, "no HIP device is available");
+}
+
+// Implemented in the kernel translation units.
+template <typename T> void fwht_array_dev(T* d, size_t n, hipStream_t stream);
+void gray_coset_min_dev(const uint64_t* d_rows, size_t n_rows, size_t words_per_row,
+                        size_t block_rows, const int32_t* targets, size_t n_targets,
+                        size_t chunk_bits, uint64_t* d_first_key, uint64_t* d_mask_key,
+                        hipStream_t stream);
+void coset_min_batch_dev(const uint64_t* d_rows, const int64_t* word_offset, const int32_t* rows,
+                         const int32_t* cols, size_t n_mat, const int64_t* out_offset,
+                         uint32_t* d_out_w, uint64_t* d_out_idx, int persistent_wgs,
+                         hipStream_t stream);
+int default_persistent_wgs();
+
+namespace {
+struct CapiWorkspace {
+    DevBuf a, b, c, d;
+};
+CapiWorkspace& capi_ws() {
+    thread_local CapiWorkspace ws;
+    return ws;
+}
+
+template <typename T> void fwht_host(T* data, size_t n) {
+    if (n <= 1)
+        return;
+    if ((n & (n - 1)) != 0)
+        fail(BMM_ERR_INVALID, "fwht_inplace: size must be a power of two");
+    require_device();
+    T* d = capi_ws().a.get<T>(n);
+    BMM_HIP(hipMemcpy(d, data, n * sizeof(T), hipMemcpyHostToDevice));
+    fwht_array_dev<T>(d, n, nullptr);
+    BMM_HIP(hipMemcpy(data, d, n * sizeof(T), hipMemcpyDeviceToHost));
+}
+} // namespace
+
+} // namespace bmmhip
+
+using namespace bmmhip;
+
+extern "C" {
+
+const char* bmm_hip_last_error(void) { return g_last_error.c_str(); }
+
+const char* bmm_hip_version(void) { return "0.1.0"; }
+
+int bmm_hip_device_info(bmm_device_info* out) {
+    return guarded([&] {
+        if (!out)
+            fail(BMM_ERR_INVALID, "device_info: null output");
+        std::memset(out, 0, sizeof(*out));
+        int n = 0;
+        if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) {
+            (void)hipGetLastError();
+            out->device_count = 0;
+            out->device = -1;
+            return;
+        }
+        out->device_count = n;
+        int dev = 0;
+        BMM_HIP(hipGetDevice(&dev));
+        hipDeviceProp_t p{};
+        BMM_HIP(hipGetDeviceProperties(&p, dev));
+        out->device = dev;
+        out->compute_units = p.multiProcessorCount;
+        out->wave_size = p.warpSize;
+        out->lds_per_block = (int)p.sharedMemPerBlock;
+        out->clock_khz = p.clockRate;
+        std::strncpy(out->arch, p.gcnArchName, sizeof(out->arch) - 1);
+        std::strncpy(out->name, p.name, sizeof(out->name) - 1);
+    });
+}
+
+int bmm_hip_fwht_i32(int32_t* data, size_t n) {
+    return guarded([&] {
+        if (n > 1 && !data)
+            fail(BMM_ERR_INVALID, "fwht_inplace: null data");
+        fwht_host<int32_t>(data, n);
+    });
+}
+
+int bmm_hip_fwht_i16(int16_t* data, size_t n) {
+    return guarded([&] {
+        if (n > 1 && !data)
+            fail(BMM_ERR_INVALID, "fwht_inplace: null data");
+        fwht_host<int16_t>(data, n);
+    });
+}
+
+int bmm_hip_fwht_i32_dev(int32_t* d_data, size_t n, void* stream) {
+    return guarded([&] { fwht_array_dev<int32_t>(d_data, n, (hipStream_t)stream); });
+}
+
+int bmm_hip_fwht_i16_dev(int16_t* d_data, size_t n, void* stream) {
+    return guarded([&] { fwht_array_dev<int16_t>(d_data, n, (hipStream_t)stream); });
+}
+
+int bmm_hip_gray_coset_min_dev(const uint64_t* d_rows, size_t n_rows, size_t words_per_row,
+                               size_t block_rows, const int32_t* targets, size_t n_targets,
+                               size_t chunk_bits, uint64_t* d_first_key, uint64_t* d_mask_key,
+                               void* stream) {
+    return guarded([&] {
+        gray_coset_min_dev(d_rows, n_rows, words_per_row, block_rows, targets, n_targets,
+                           chunk_bits, d_first_key, d_mask_key, (hipStream_t)stream);
+    });
+}
+
+int bmm_hip_gray_coset_min(const uint64_t* row_words, size_t n_rows, size_t words_per_row,
+                           size_t block_rows, const int32_t* targets, size_t n_targets,
+                           uint32_t* out_weight, uint64_t* out_first, uint64_t* out_mask) {
+    return guarded([&] {
+        if (!row_words || !targets || !out_weight || !out_first || !out_mask)
+            fail(BMM_ERR_INVALID, "gray_coset_min: null pointer");
+        if (words_per_row == 0 || words_per_row > 32)
+            fail(BMM_ERR_INVALID, "gray_coset_min: words_per_row must be in [1, 32]");
+        require_device();
+        CapiWorkspace& ws = capi_ws();
+        uint64_t* d_rows = ws.a.get<uint64_t>(n_rows * words_per_row);
+        uint64_t* d_k = ws.b.get<uint64_t>(2 * n_targets);
+        BMM_HIP(hipMemcpy(d_rows, row_words, n_rows * words_per_row * 8, hipMemcpyHostToDevice));
+        gray_coset_min_dev(d_rows, n_rows, words_per_row, block_rows, targets, n_targets, 0,
+                           d_k, d_k + n_targets, nullptr);
+        std::vector<uint64_t> k(2 * n_targets);
+        BMM_HIP(hipMemcpy(k.data(), d_k, k.size() * 8, hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < n_targets; ++i) {
+            out_weight[i] = (uint32_t)(k[i] >> 32);
+            out_first[i] = k[i] & 0xffffffffull;
+            out_mask[i] = k[n_targets + i] & 0xffffffffull;
+        }
+    });
+}
+
+int bmm_hip_coset_min_batch(const uint64_t* row_words, const int64_t* word_offset,
+                            const int32_t* rows, const int32_t* cols, size_t n_mat,
+                            const int64_t* out_offset, uint32_t* out_weight,
+                            uint64_t* out_index) {
+    return guarded([&] {
+        if (n_mat == 0)
+            return;
+        if (!row_words || !word_offset || !rows || !cols || !out_offset || !out_weight ||
+            !out_index)
+            fail(BMM_ERR_INVALID, "coset_min_batch: null pointer");
+        size_t words = 0, slots = 0;
+        for (size_t m = 0; m < n_mat; ++m) {
+            if (rows[m] < 2 || rows[m] > 31)
+                fail(BMM_ERR_INVALID, "coset_min: rows must be in [2, 31]");
+            if (cols[m] < 0 || word_offset[m] < 0 || out_offset[m] < 0)
+                fail(BMM_ERR_INVALID, "coset_min: negative size or offset");
+            words = std::max(words, (size_t)word_offset[m] +
+                                        (size_t)rows[m] * (size_t)((cols[m] + 63) / 64));
+            slots = std::max(slots, (size_t)out_offset[m] + (size_t)rows[m]);
+        }
+        require_device();
+        CapiWorkspace& ws = capi_ws();
+        uint64_t* d_rows = ws.a.get<uint64_t>(words);
+        uint32_t* d_w = ws.b.get<uint32_t>(slots);
+        uint64_t* d_i = ws.c.get<uint64_t>(slots);
+        BMM_HIP(hipMemcpy(d_rows, row_words, words * 8, hipMemcpyHostToDevice));
+        coset_min_batch_dev(d_rows, word_offset, rows, cols, n_mat, out_offset, d_w, d_i,
+                            default_persistent_wgs(), nullptr);
+        BMM_HIP(hipMemcpy(out_weight, d_w, slots * 4, hipMemcpyDeviceToHost));
+        BMM_HIP(hipMemcpy(out_index, d_i, slots * 8, hipMemcpyDeviceToHost));
+    });
+}
+
+int bmm_hip_coset_min(const uint64_t* row_words, size_t n_rows, size_t words_per_row,
+                      size_t cols, uint32_t* out_weight, uint64_t* out_index) {
+    if (words_per_row != (cols + 63) / 64) {
+        set_last_error("coset_min: words_per_row must equal ceil(cols / 64)");
+        return BMM_ERR_INVALID;
+    }
+    const int64_t woff = 0, ooff = 0;
+    const int32_t r = (int32_t)n_rows, c = (int32_t)cols;
+    if (n_rows > 31 || cols > (size_t)INT32_MAX) {
+        set_last_error("coset_min: rows must be in [2, 31]");
+        return BMM_ERR_INVALID;
+    }
+    return bmm_hip_coset_min_batch(row_words, &woff, &r, &c, 1, &ooff, out_weight, out_index);
+}
+
+int bmm_hip_fwht_topk(const uint64_t*, size_t, size_t, size_t, size_t, bmm_topk_entry*,
+                      size_t*) {
+    set_last_error("fwht_topk: not built yet");
+    return BMM_ERR_DEVICE;
+}
+
+int bmm_hip_mitm_topk(const uint64_t*, const uint64_t*, const int32_t*, size_t, size_t, size_t,
+                      int32_t, size_t, bmm_topk_entry*, size_t*) {
+    set_last_error("mitm_topk: not built yet");
+    return BMM_ERR_DEVICE;
+}
+
+int bmm_hip_bruteforce_topk(const uint64_t*, size_t, size_t, size_t, size_t, bmm_topk_entry*,
+                            size_t*) {
+    set_last_error("bruteforce_topk: not built yet");
+    return BMM_ERR_DEVICE;
+}
+
+} // extern "C"

@@ -1,0 +1,146 @@
+// Shared host/device helpers for the bmmpy HIP library (gfx950 only).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+
+#include "../../include/bmmpy_hip.h"
+
+namespace bmmhip {
+
+constexpr int kWave = 64;
+
+// ---------------------------------------------------------------- errors --
+// Status-carrying exception thrown inside the library and turned into a
+// BMM_* status code at the C-ABI boundary.
+struct Error : std::runtime_error {
+    int status;
+    Error(int s, const std::string& msg) : std::runtime_error(msg), status(s) {}
+};
+
+[[noreturn]] inline void fail(int status, const std::string& msg) { throw Error(status, msg); }
+
+inline void check(hipError_t e, const char* what) {
+    if (e != hipSuccess)
+        fail(BMM_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define BMM_HIP(call) ::bmmhip::check((call), #call)
+
+void set_last_error(const char* msg);
+
+// Runs `fn`, mapping exceptions to status codes and the thread-local message.
+template <typename F> int guarded(F&& fn) {
+    try {
+        fn();
+        set_last_error("");
+        return BMM_OK;
+    } catch (const Error& e) {
+        set_last_error(e.what());
+        return e.status;
+    } catch (const std::bad_alloc&) {
+        set_last_error("host allocation failed");
+        return BMM_ERR_DEVICE;
+    } catch (const std::exception& e) {
+        set_last_error(e.what());
+        return BMM_ERR_DEVICE;
+    }
+}
+
+// --------------------------------------------------------- device buffers --
+// Grow-only device allocation owned by a per-thread, per-device workspace.
+struct DevBuf {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+    int device = -1;
+    ~DevBuf() { release(); }
+    void release() noexcept {
+        if (ptr) {
+            int cur = 0;
+            (void)hipGetDevice(&cur);
+            if (device >= 0 && device != cur)
+                (void)hipSetDevice(device);
+            (void)hipFree(ptr);
+            if (device >= 0 && device != cur)
+                (void)hipSetDevice(cur);
+        }
+        ptr = nullptr;
+        bytes = 0;
+    }
+    template <typename T> T* get(size_t count) {
+        size_t need = count * sizeof(T);
+        if (need == 0)
+            need = 256;
+        int cur = 0;
+        BMM_HIP(hipGetDevice(&cur));
+        if (need > bytes || cur != device) {
+            release();
+            size_t cap = 256;
+            while (cap < need)
+                cap <<= 1;
+            BMM_HIP(hipMalloc(&ptr, cap));
+            bytes = cap;
+            device = cur;
+        }
+        return static_cast<T*>(ptr);
+    }
+};
+
+// ------------------------------------------------------------ bit helpers --
+__host__ __device__ inline uint64_t gray_of(uint64_t s) { return s ^ (s >> 1); }
+
+__host__ __device__ inline uint64_t gray_inverse(uint64_t g) {
+    g ^= g >> 1;
+    g ^= g >> 2;
+    g ^= g >> 4;
+    g ^= g >> 8;
+    g ^= g >> 16;
+    g ^= g >> 32;
+    return g;
+}
+
+// popcount(x) + acc as a single VALU op.
+__device__ inline uint32_t bcnt_acc(uint32_t x, uint32_t acc) {
+    uint32_t r;
+    asm("v_bcnt_u32_b32 %0, %1, %2" : "=v"(r) : "v"(x), "v"(acc));
+    return r;
+}
+
+// Wave-wide minimum of a 64-bit value (lanes of one wavefront).
+__device__ inline uint64_t wave_min_u64(uint64_t v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        uint64_t o = __shfl_xor(v, off, kWave);
+        v = o < v ? o : v;
+    }
+    return v;
+}
+
+__device__ inline uint32_t wave_min_u32(uint32_t v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        uint32_t o = __shfl_xor(v, off, kWave);
+        v = o < v ? o : v;
+    }
+    return v;
+}
+
+__device__ inline uint32_t wave_sum_u32(uint32_t v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+        v += __shfl_xor(v, off, kWave);
+    return v;
+}
+
+// Fill kernel used instead of hipMemsetAsync so the same launches can sit in
+// a captured graph.
+__global__ void fill_u64_kernel(uint64_t* p, size_t n, uint64_t v);
+void fill_u64(uint64_t* p, size_t n, uint64_t v, hipStream_t s);
+__global__ void fill_u32_kernel(uint32_t* p, size_t n, uint32_t v);
+void fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t s);
+
+} // namespace bmmhip

@@ -1,0 +1,154 @@
+/*
+ * bmmpy_hip.h - C ABI of the MI355X (gfx950) row-XOR weight-minimization
+ * library `libbmmpy_hip.so`.
+ *
+ * Plain pointers and sizes only.  Every entry point returns a BMM_* status; on
+ * failure `bmm_hip_last_error()` (thread-local) holds the message.  Status
+ * codes map onto the reference's exception types:
+ *   BMM_ERR_INVALID  -> std::invalid_argument / ValueError
+ *   BMM_ERR_OVERFLOW -> std::overflow_error / OverflowError
+ *   BMM_ERR_RANGE    -> std::out_of_range / IndexError
+ *   BMM_ERR_DEVICE   -> std::runtime_error / RuntimeError
+ * Validation happens before any device query, as in the reference.
+ *
+ * Row data: `rows` rows of `words_per_row` uint64 words, contiguous and
+ * unpadded, bit c of a row at word c/64 bit c%64 (the reference BitMatrix
+ * layout minus its stride padding).  "host" entry points take host pointers
+ * and are synchronous (the reference device seams are); "_dev" entry points
+ * take device pointers, enqueue on `stream` (a hipStream_t, NULL = default)
+ * and return without synchronising.
+ *
+ * Result records are ordered ascending by (weight, mask); mask 0 is never
+ * reported.
+ */
+#ifndef BMMPY_HIP_H
+#define BMMPY_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BMM_OK 0
+#define BMM_ERR_INVALID 1
+#define BMM_ERR_OVERFLOW 2
+#define BMM_ERR_RANGE 3
+#define BMM_ERR_DEVICE 4
+
+/* Result record; same fields as the reference DeviceTopKEntry
+ * (include/bmmpy/search/cuda_launch_topk.cuh:11-14), padded to 16 bytes. */
+typedef struct bmm_topk_entry {
+    uint64_t mask;
+    uint32_t weight;
+    uint32_t pad;
+} bmm_topk_entry;
+
+typedef struct bmm_device_info {
+    int device_count;
+    int device;
+    int compute_units;
+    int wave_size;
+    int lds_per_block;
+    int clock_khz;
+    char arch[64];
+    char name[128];
+} bmm_device_info;
+
+/* Thread-local message of the last failing call ("" after a success). */
+const char *bmm_hip_last_error(void);
+
+/* Library version string. */
+const char *bmm_hip_version(void);
+
+/* Device discovery for RuntimeFeatures (src/stub.cpp:17-43). device_count is 0
+ * (and BMM_OK returned) when no device is visible. */
+int bmm_hip_device_info(bmm_device_info *out);
+
+/* ---- src/math: fwht_inplace (include/bmmpy/math/fwht.hpp:8-9) ----------- */
+/* In-place unnormalised Walsh-Hadamard transform, natural order, wrapping
+ * arithmetic; n must be a power of two (n <= 1 is a no-op). Host pointers. */
+int bmm_hip_fwht_i32(int32_t *data, size_t n);
+int bmm_hip_fwht_i16(int16_t *data, size_t n);
+/* Same on device memory, enqueued on `stream`. */
+int bmm_hip_fwht_i32_dev(int32_t *d_data, size_t n, void *stream);
+int bmm_hip_fwht_i16_dev(int16_t *d_data, size_t n, void *stream);
+
+/* ---- FwhtSearch::search inner loops (src/search/fwht_search.cpp:53-131) -
+ * Histogram of active-column patterns, FWHT, exact top-k over masks
+ * 1..2^rows-1.  Writes min(k, 2^rows-1) entries to `out` (capacity k) and the
+ * count to *n_out.  rows <= 40, k <= BMM_TOPK_MAX. */
+#define BMM_TOPK_MAX 4096
+int bmm_hip_fwht_topk(const uint64_t *row_words, size_t rows, size_t words_per_row,
+                      size_t cols, size_t k, bmm_topk_entry *out, size_t *n_out);
+
+/* ---- run_cuda_mitm_fwht_search (include/bmmpy/search/cuda_mitm_fwht_launch.hpp:
+ * 15-16) over a CompactSplitWindow (include/bmmpy/search/split_window_prep.hpp:
+ * 10-20): `unique` entries of (q, r, multiplicity), pattern = q << low_bits |
+ * r, total = sum of multiplicities.  Exact top-k over masks of
+ * low_bits + high_bits rows. */
+int bmm_hip_mitm_topk(const uint64_t *q, const uint64_t *r, const int32_t *mult, size_t unique,
+                      size_t low_bits, size_t high_bits, int32_t total, size_t k,
+                      bmm_topk_entry *out, size_t *n_out);
+
+/* ---- run_cuda_bruteforce_search (include/bmmpy/search/cuda_bruteforce_launch.hpp:
+ * 22-23): Gray-code sweep over all masks of `rows` rows, popcount over
+ * words_per_row words without tail masking (src/search/bruteforce_search.cpp:
+ * 157-169).  chunk_bits = 0 selects automatically. */
+int bmm_hip_bruteforce_topk(const uint64_t *row_words, size_t rows, size_t words_per_row,
+                            size_t chunk_bits, size_t k, bmm_topk_entry *out, size_t *n_out);
+
+/* ---- per-target coset minimum (no single reference function: the reference
+ * FwhtSearch ranking restricted to masks containing the target's bit,
+ * src/search/fwht_search.cpp:59-126).  For every row i of an n_rows x cols
+ * matrix: out_weight[i] = min over subsets S of the other rows of
+ * wt(row_i ^ XOR_S), out_index[i] = the lowest such S as a bit set over the
+ * other rows in ascending order.  n_rows in [2, 31]. */
+int bmm_hip_coset_min(const uint64_t *row_words, size_t n_rows, size_t words_per_row,
+                      size_t cols, uint32_t *out_weight, uint64_t *out_index);
+
+/* Batched form: n_mat matrices, matrix m has rows[m] rows and cols[m] columns,
+ * its words at row_words + word_offset[m] (rows[m] x ceil(cols[m]/64) words).
+ * Outputs are indexed by out_offset[m] + target.  One persistent launch fed by
+ * a device work queue. */
+int bmm_hip_coset_min_batch(const uint64_t *row_words, const int64_t *word_offset,
+                            const int32_t *rows, const int32_t *cols, size_t n_mat,
+                            const int64_t *out_offset, uint32_t *out_weight, uint64_t *out_index);
+
+/* ---- Gray-code coset sweep (config: pivot block of rows [0, block_rows)
+ * minus the target).  For each target t in targets[]: block = rows
+ * [0, block_rows) \ {t} in ascending order; step s visits block subset
+ * gray(s) = s ^ (s >> 1) and evaluates wt(row_t ^ XOR(subset)) over
+ * words_per_row words.  out_weight = minimum, out_first = first step s that
+ * reaches it, out_mask = lowest block subset that reaches it. */
+int bmm_hip_gray_coset_min(const uint64_t *row_words, size_t n_rows, size_t words_per_row,
+                           size_t block_rows, const int32_t *targets, size_t n_targets,
+                           uint32_t *out_weight, uint64_t *out_first, uint64_t *out_mask);
+/* Device-resident form: d_rows on the device, results written as packed keys
+ * d_first_key[i] = weight << 32 | first step, d_mask_key[i] = weight << 32 |
+ * lowest mask.  chunk_bits = 0 selects automatically. */
+int bmm_hip_gray_coset_min_dev(const uint64_t *d_rows, size_t n_rows, size_t words_per_row,
+                               size_t block_rows, const int32_t *targets, size_t n_targets,
+                               size_t chunk_bits, uint64_t *d_first_key, uint64_t *d_mask_key,
+                               void *stream);
+
+/* ---- seeded synthetic workloads (no reference counterpart: they define the
+ * benchmark inputs).  XorShift64 stream (include/bmmpy/core/detail/
+ * xorshift64.hpp:10-40); Bernoulli(p) = next_unit_double() < p, row-major. */
+int bmm_gen_bernoulli(uint64_t seed, size_t n_rows, size_t cols, double p, uint64_t *out);
+/* sparse = Bernoulli(p) n_rows x cols; transform = random invertible
+ * n_rows x n_rows (p = 0.5 draws, rejected until full rank); mixed =
+ * transform * sparse.  Row strides ceil(cols/64) and ceil(n_rows/64) words. */
+int bmm_gen_planted(uint64_t seed, size_t n_rows, size_t cols, double p, uint64_t *sparse,
+                    uint64_t *mixed, uint64_t *transform);
+/* Per matrix: rows = 8 + index(21), cols = 64 (1 + index(32)),
+ * density = 0.1 + 0.4 unit(), then the bits.  words may be NULL to size. */
+int bmm_gen_batch(uint64_t seed, size_t n_mat, int32_t *rows, int32_t *cols, double *density,
+                  uint64_t *words, size_t words_cap, int64_t *word_offset, size_t *n_words);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,195 @@
+// Checker-only shim: a C entry layer over the reference's own CPU library, so
+// that tests and fixture generation can call the reference search, FWHT and
+// apply routines through ctypes.  Built by oracle/Makefile (target `ref`) from
+// the reference sources where they lie; the output goes to oracle/_ref/ and is
+// never shipped or called by the product path.
+//
+// Every function takes rows as `t` contiguous rows of `W` uint64 words
+// (unpadded, LSB-first bit order, bit c at word c/64, bit c%64) plus `cols`.
+// Return value: number of results written (>= 0), or -1 on a C++ exception
+// whose message is copied into `err` (up to err_len bytes).
+
+#include "bmmpy/apply/greedy_applier.hpp"
+#include "bmmpy/core/bit_matrix.hpp"
+#include "bmmpy/core/row_window.hpp"
+#include "bmmpy/math/fwht.hpp"
+#include "bmmpy/search/bruteforce_search.hpp"
+#include "bmmpy/search/fwht_search.hpp"
+#include "bmmpy/search/mitm_fwht_search.hpp"
+#include "bmmpy/search/split_window_prep.hpp"
+
+#include <cstdint>
+#include <cstring>
+#include <exception>
+#include <numeric>
+#include <string>
+#include <vector>
+
+namespace {
+
+bmmpy::BitMatrix load(const std::uint64_t* rows, int t, int W, int cols) {
+    bmmpy::BitMatrix m(static_cast<std::size_t>(t), static_cast<std::size_t>(cols));
+    for (int r = 0; r < t; ++r) {
+        std::uint64_t* dst = m.row_words(static_cast<std::size_t>(r));
+        std::memcpy(dst, rows + static_cast<std::size_t>(r) * W, sizeof(std::uint64_t) * W);
+    }
+    return m;
+}
+
+void set_err(char* err, int err_len, const char* msg) {
+    if (err == nullptr || err_len <= 0)
+        return;
+    std::strncpy(err, msg, static_cast<std::size_t>(err_len - 1));
+    err[err_len - 1] = '\0';
+}
+
+std::vector<std::size_t> all_rows(int t) {
+    std::vector<std::size_t> v(static_cast<std::size_t>(t));
+    std::iota(v.begin(), v.end(), std::size_t{0});
+    return v;
+}
+
+int emit(const std::vector<bmmpy::Candidate>& c, std::uint64_t* out_mask, std::uint32_t* out_w,
+         std::size_t cap) {
+    std::size_t n = c.size() < cap ? c.size() : cap;
+    for (std::size_t i = 0; i < n; ++i) {
+        out_mask[i] = c[i].mask.empty() ? 0 : c[i].mask.front();
+        out_w[i] = c[i].weight;
+    }
+    return static_cast<int>(c.size());
+}
+
+} // namespace
+
+extern "C" {
+
+int ref_fwht_search(const std::uint64_t* rows, int t, int W, int cols, std::size_t max_rows,
+                    std::size_t k, std::uint64_t* out_mask, std::uint32_t* out_w, std::size_t cap,
+                    char* err, int err_len) {
+    try {
+        auto m = load(rows, t, W, cols);
+        auto win = m.row_window(all_rows(t));
+        bmmpy::FwhtSearch s(bmmpy::FwhtSearchConfig{max_rows, k});
+        return emit(s.search(win), out_mask, out_w, cap);
+    } catch (const std::exception& e) {
+        set_err(err, err_len, e.what());
+        return -1;
+    }
+}
+
+int ref_mitm_search(const std::uint64_t* rows, int t, int W, int cols, std::size_t k,
+                    std::uint64_t* out_mask, std::uint32_t* out_w, std::size_t cap, char* err,
+                    int err_len) {
+    try {
+        auto m = load(rows, t, W, cols);
+        auto win = m.row_window(all_rows(t));
+        bmmpy::MitmFwhtSearch s(bmmpy::MitmFwhtSearchConfig{1024, 20, std::size_t{1} << 16, k});
+        return emit(s.search(win), out_mask, out_w, cap);
+    } catch (const std::exception& e) {
+        set_err(err, err_len, e.what());
+        return -1;
+    }
+}
+
+int ref_bruteforce_search(const std::uint64_t* rows, int t, int W, int cols, std::size_t k,
+                          std::size_t chunk_bits, std::uint64_t* out_mask, std::uint32_t* out_w,
+                          std::size_t cap, char* err, int err_len) {
+    try {
+        auto m = load(rows, t, W, cols);
+        auto win = m.row_window(all_rows(t));
+        bmmpy::BruteforceSearch s(bmmpy::BruteforceSearchConfig{k, chunk_bits});
+        return emit(s.search(win), out_mask, out_w, cap);
+    } catch (const std::exception& e) {
+        set_err(err, err_len, e.what());
+        return -1;
+    }
+}
+
+int ref_fwht_i32(std::int32_t* data, std::size_t n, char* err, int err_len) {
+    try {
+        bmmpy::fwht_inplace(data, n);
+        return 0;
+    } catch (const std::exception& e) {
+        set_err(err, err_len, e.what());
+        return -1;
+    }
+}
+
+int ref_fwht_i16(std::int16_t* data, std::size_t n, char* err, int err_len) {
+    try {
+        bmmpy::fwht_inplace(data, n);
+        return 0;
+    } catch (const std::exception& e) {
+        set_err(err, err_len, e.what());
+        return -1;
+    }
+}
+
+// Writes up to `cap` (q, r, multiplicity) entries; returns the unique count and
+// stores total_weight / high_bits through the out pointers.
+int ref_compact_split(const std::uint64_t* rows, int t, int W, int cols, int low_bits,
+                      std::int32_t* total, int* high_bits, std::uint64_t* q, std::uint64_t* r,
+                      std::int32_t* mult, std::size_t cap, char* err, int err_len) {
+    try {
+        auto m = load(rows, t, W, cols);
+        auto win = m.row_window(all_rows(t));
+        auto p = bmmpy::build_compact_split_window(win, static_cast<std::size_t>(low_bits));
+        *total = p.total_weight;
+        *high_bits = static_cast<int>(p.high_bits);
+        for (std::size_t i = 0; i < p.q.size() && i < cap; ++i) {
+            q[i] = p.q[i];
+            r[i] = p.r[i];
+            mult[i] = p.multiplicity[i];
+        }
+        return static_cast<int>(p.q.size());
+    } catch (const std::exception& e) {
+        set_err(err, err_len, e.what());
+        return -1;
+    }
+}
+
+// Replays row-XOR ops (dst ^= src; no-op when dst == src) through
+// RowWindow::row_xor and returns the window weight; rows are updated in place.
+long long ref_replay(std::uint64_t* rows, int t, int W, int cols, const std::uint16_t* ops,
+                     std::size_t n_ops, char* err, int err_len) {
+    try {
+        auto m = load(rows, t, W, cols);
+        auto win = m.row_window(all_rows(t));
+        for (std::size_t i = 0; i < n_ops; ++i)
+            win.row_xor(ops[2 * i + 1], ops[2 * i]);
+        for (int rr = 0; rr < t; ++rr)
+            std::memcpy(rows + static_cast<std::size_t>(rr) * W, m.row_words(rr),
+                        sizeof(std::uint64_t) * W);
+        return static_cast<long long>(m.weight());
+    } catch (const std::exception& e) {
+        set_err(err, err_len, e.what());
+        return -1;
+    }
+}
+
+// GreedyApplier::apply over candidates (mask, weight); rows updated in place.
+int ref_greedy_apply(std::uint64_t* rows, int t, int W, int cols, const std::uint64_t* masks,
+                     const std::uint32_t* weights, std::size_t n_cand, std::uint64_t min_gain,
+                     int stochastic, std::uint64_t seed, std::uint64_t* out_applied,
+                     std::uint64_t* out_improvement, char* err, int err_len) {
+    try {
+        auto m = load(rows, t, W, cols);
+        auto win = m.row_window(all_rows(t));
+        std::vector<bmmpy::Candidate> cands;
+        for (std::size_t i = 0; i < n_cand; ++i)
+            cands.push_back(bmmpy::Candidate::from_u64(masks[i], weights[i]));
+        bmmpy::GreedyApplier ap(min_gain, stochastic != 0, seed);
+        auto res = ap.apply(win, cands);
+        *out_applied = res.applied_count;
+        *out_improvement = res.weight_improvement;
+        for (int rr = 0; rr < t; ++rr)
+            std::memcpy(rows + static_cast<std::size_t>(rr) * W, m.row_words(rr),
+                        sizeof(std::uint64_t) * W);
+        return 0;
+    } catch (const std::exception& e) {
+        set_err(err, err_len, e.what());
+        return -1;
+    }
+}
+
+} // extern "C"

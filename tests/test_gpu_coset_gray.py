@@ -1,0 +1,108 @@
+"""GPU parity: Gray coset sweep, coset minimum (single and batched) and the
+stand-alone FWHT, against the oracle on identical seeded inputs."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+SEED_C1, SEED_C2, SEED_C4 = 0x5EED0001, 0x5EED0002, 0x5EED0004
+
+
+def _say(*a):
+    print(*a, flush=True)
+
+
+def test_fwht_kats(lib):
+    _say("fwht kats")
+    assert lib.fwht_i32([1, 2, 3, 4]).tolist() == [10, -2, -4, 0]
+    assert lib.fwht_i16([32767, 1]).tolist() == [-32768, 32766]
+
+
+@pytest.mark.parametrize("n_bits", [1, 5, 12, 13, 18, 21])
+def test_fwht_i32_matches_oracle(lib, ora, n_bits):
+    _say("fwht_i32", n_bits)
+    rng = np.random.default_rng(n_bits)
+    x = rng.integers(-2**31, 2**31, size=1 << n_bits, dtype=np.int64).astype(np.int32)
+    assert np.array_equal(lib.fwht_i32(x), ora.fwht_i32(x))
+
+
+@pytest.mark.parametrize("n_bits", [3, 12, 16, 19])
+def test_fwht_i16_wraps_like_oracle(lib, ora, n_bits):
+    _say("fwht_i16", n_bits)
+    rng = np.random.default_rng(100 + n_bits)
+    x = rng.integers(-2**15, 2**15, size=1 << n_bits, dtype=np.int64).astype(np.int16)
+    assert np.array_equal(lib.fwht_i16(x), ora.fwht_i16(x))
+
+
+def test_gray_c2_full_config(lib, ora):
+    """32x1024 Bernoulli(0.3); every target against rows 0..23 minus itself."""
+    _say("gray c2 full")
+    m = lib.gen_bernoulli(SEED_C2, 32, 1024, 0.3)
+    targets = np.arange(32, dtype=np.int32)
+    w, first, low = lib.gray_coset_min(m, 24, targets)
+    for t in range(32):
+        ew, es, em = ora.gray_min(m, 24, t)
+        assert (int(w[t]), int(first[t]), int(low[t])) == (ew, es, em), t
+
+
+@pytest.mark.parametrize("rows,cols,p,block", [(20, 128, 0.5, 16), (18, 512, 0.2, 18),
+                                               (40, 2048, 0.1, 24)])
+def test_gray_other_widths(lib, ora, rows, cols, p, block):
+    _say("gray", rows, cols, block)
+    m = lib.gen_bernoulli(rows * 1000 + cols, rows, cols, p)
+    targets = np.array([0, block - 1, rows - 1], dtype=np.int32)
+    w, first, low = lib.gray_coset_min(m, block, targets)
+    for i, t in enumerate(targets):
+        assert (int(w[i]), int(first[i]), int(low[i])) == ora.gray_min(m, block, int(t))
+
+
+def test_coset_c1_full_config(lib, ora):
+    """20x128 Bernoulli(0.5): per-target min and argmin over the other 19 rows."""
+    _say("coset c1")
+    m = lib.gen_bernoulli(SEED_C1, 20, 128, 0.5)
+    w, idx = lib.coset_min(m, 128)
+    ew, eidx = ora.coset_min(m, 128)
+    assert np.array_equal(w, ew)
+    assert np.array_equal(idx, eidx)
+
+
+@pytest.mark.parametrize("rows,cols", [(2, 1), (3, 5), (8, 64), (13, 100), (14, 777),
+                                       (16, 2048), (22, 300)])
+def test_coset_edge_shapes(lib, ora, rows, cols):
+    _say("coset", rows, cols)
+    m = lib.gen_bernoulli(rows * 7 + cols, rows, cols, 0.4)
+    w, idx = lib.coset_min(m, cols)
+    ew, eidx = ora.coset_min(m, cols)
+    assert np.array_equal(w, ew) and np.array_equal(idx, eidx)
+
+
+def test_coset_zero_rows_and_duplicates(lib, ora):
+    _say("coset degenerate")
+    m = lib.gen_bernoulli(5, 10, 200, 0.3)
+    m[3] = 0
+    m[7] = m[2]
+    w, idx = lib.coset_min(m, 200)
+    ew, eidx = ora.coset_min(m, 200)
+    assert np.array_equal(w, ew) and np.array_equal(idx, eidx)
+    assert w[2] == 0 and w[7] == 0
+
+
+def test_coset_batch_matches_per_matrix(lib, ora):
+    """Batched uneven workload (C4 generator), matrices with <= 20 rows checked
+    one by one against the oracle."""
+    _say("coset batch")
+    rows, cols, dens, words, offs = lib.gen_batch(SEED_C4, 256)
+    w, idx = lib.coset_min_batch(words, offs, rows, cols)
+    out_off = np.concatenate([[0], np.cumsum(rows)[:-1]])
+    checked = 0
+    for m in range(256):
+        if rows[m] > 20:
+            continue
+        W = cols[m] // 64
+        mat = words[offs[m]: offs[m] + rows[m] * W].reshape(rows[m], W)
+        ew, eidx = ora.coset_min(mat, int(cols[m]))
+        o = out_off[m]
+        assert np.array_equal(w[o:o + rows[m]], ew), m
+        assert np.array_equal(idx[o:o + rows[m]], eidx), m
+        checked += 1
+    assert checked > 100
