@@ -1,0 +1,42 @@
+"""bmmpy_amd: the bmmpy row-XOR weight-minimization API on MI355X.
+
+Same public names, keyword arguments, defaults and exceptions as the
+reference's ``bmmpy`` package (bmmpy/__init__.py:34-105) for the search,
+apply and runtime surface; every search and replay runs as HIP kernels
+(libbmmpy_hip.so) driven by the C++ host layer in ``_bmmpy``.
+"""
+from __future__ import annotations
+
+from ._bmmpy import (  # noqa: F401
+    ApplyResult,
+    BitMatrix,
+    Candidate,
+    MatrixErr,
+    MatrixError,
+    RowWindow,
+    RuntimeFeatures,
+    add,
+    fwht_i16,
+    fwht_i32,
+    get_runtime_features,
+    get_version,
+)
+from .apply import GreedyApplier
+from .matrix import matrix_from_rows
+from .search import (
+    BruteforceSearch,
+    CudaBruteforceSearch,
+    CudaMitmFwhtSearch,
+    FwhtSearch,
+    MitmFwhtSearch,
+)
+from .workflows import search_apply
+
+__all__ = [
+    "ApplyResult", "BitMatrix", "BruteforceSearch", "Candidate", "CudaBruteforceSearch",
+    "CudaMitmFwhtSearch", "FwhtSearch", "GreedyApplier", "MatrixErr", "MatrixError",
+    "MitmFwhtSearch", "RowWindow", "RuntimeFeatures", "add", "fwht_i16", "fwht_i32",
+    "get_runtime_features", "get_version", "matrix_from_rows", "search_apply",
+]
+
+__version__ = get_version()

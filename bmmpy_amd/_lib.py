@@ -64,6 +64,7 @@ SIGNATURES = {
                                C.c_int),
     "bmm_hip_gray_coset_min_dev": ([_vp, _sz, _sz, _sz, _i32p, _sz, _sz, _vp, _vp, _vp],
                                    C.c_int),
+    "bmm_hip_replay": ([_u64p, _sz, _sz, _u16p, _sz, C.POINTER(C.c_uint64)], C.c_int),
     "bmm_gen_bernoulli": ([C.c_uint64, _sz, _sz, C.c_double, _u64p], C.c_int),
     "bmm_gen_planted": ([C.c_uint64, _sz, _sz, C.c_double, _u64p, _u64p, _u64p], C.c_int),
     "bmm_gen_batch": ([C.c_uint64, _sz, _i32p, _i32p, _f64p, _vp, _sz, _i64p, C.POINTER(_sz)],
@@ -198,6 +199,15 @@ def gray_coset_min(rows, block_rows: int, targets):
     call("bmm_hip_gray_coset_min", rows, rows.shape[0], rows.shape[1], block_rows, targets, n,
          w, f, m)
     return w, f, m
+
+
+def replay(rows, ops):
+    """Row-XOR replay on the GPU; returns (new rows, total weight)."""
+    rows = np.array(rows, dtype=np.uint64, copy=True, order="C")
+    ops = np.ascontiguousarray(ops, dtype=np.uint16).reshape(-1)
+    wt = C.c_uint64()
+    call("bmm_hip_replay", rows, rows.shape[0], rows.shape[1], ops, ops.size // 2, C.byref(wt))
+    return rows, wt.value
 
 
 def gen_bernoulli(seed: int, n_rows: int, cols: int, p: float) -> np.ndarray:

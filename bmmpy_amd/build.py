@@ -23,7 +23,7 @@ BUILD = os.path.join(HERE, "_build")
 HIP_LIB = os.path.join(HERE, "libbmmpy_hip.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-HIP_SOURCES = ["capi.hip", "gray.hip", "fwht.hip", "topk.hip", "mitm.hip", "bruteforce.hip",
+HIP_SOURCES = ["capi.hip", "gray.hip", "fwht.hip", "search.hip", "replay.hip", "mitm.hip", "bruteforce.hip",
                "batch.hip", "ga.hip", "mlp.hip", "workloads.hip"]
 HIP_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-Wall",
@@ -59,7 +59,7 @@ def build_hip(jobs: int = 8) -> str:
 
 
 HOST_SOURCES = ["host/bit_matrix.cpp", "host/search.cpp", "host/ga.cpp", "host/apply.cpp",
-                "host/workloads.cpp", "host/runtime.cpp"]
+                "host/runtime.cpp"]
 
 
 def build_host(jobs: int = 8) -> str | None:

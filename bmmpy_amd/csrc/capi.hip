@@ -3,6 +3,7 @@
 // BMM_* status codes with a thread-local message.
 #include "common.hpp"
 
+#include <algorithm>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -62,6 +63,15 @@ void coset_min_batch_dev(const uint64_t* d_rows, const int64_t* word_offset, con
                          uint32_t* d_out_w, uint64_t* d_out_idx, int persistent_wgs,
                          hipStream_t stream);
 int default_persistent_wgs();
+std::vector<bmm_topk_entry> fwht_topk_rows(const uint64_t* rows, size_t t, size_t words,
+                                           size_t cols, size_t k);
+std::vector<bmm_topk_entry> mitm_topk_split(const uint64_t* q, const uint64_t* r,
+                                            const int32_t* mult, size_t unique, size_t low_bits,
+                                            size_t high_bits, int32_t total, size_t k);
+void replay_dev(uint64_t* d_rows, int n_rows, int words, const uint16_t* d_ops, int n_ops,
+                unsigned long long* d_wt, hipStream_t s);
+void gray_check_args(size_t n_rows, size_t words_per_row, size_t block_rows,
+                     const int32_t* targets, size_t n_targets, size_t chunk_bits);
 
 namespace {
 struct CapiWorkspace {
@@ -162,8 +172,7 @@ int bmm_hip_gray_coset_min(const uint64_t* row_words, size_t n_rows, size_t word
     return guarded([&] {
         if (!row_words || !targets || !out_weight || !out_first || !out_mask)
             fail(BMM_ERR_INVALID, "gray_coset_min: null pointer");
-        if (words_per_row == 0 || words_per_row > 32)
-            fail(BMM_ERR_INVALID, "gray_coset_min: words_per_row must be in [1, 32]");
+        gray_check_args(n_rows, words_per_row, block_rows, targets, n_targets, 0);
         require_device();
         CapiWorkspace& ws = capi_ws();
         uint64_t* d_rows = ws.a.get<uint64_t>(n_rows * words_per_row);
@@ -229,22 +238,117 @@ int bmm_hip_coset_min(const uint64_t* row_words, size_t n_rows, size_t words_per
     return bmm_hip_coset_min_batch(row_words, &woff, &r, &c, 1, &ooff, out_weight, out_index);
 }
 
-int bmm_hip_fwht_topk(const uint64_t*, size_t, size_t, size_t, size_t, bmm_topk_entry*,
-                      size_t*) {
-    set_last_error("fwht_topk: not built yet");
-    return BMM_ERR_DEVICE;
+int bmm_hip_fwht_topk(const uint64_t* row_words, size_t rows, size_t words_per_row,
+                      size_t cols, size_t k, bmm_topk_entry* out, size_t* n_out) {
+    return guarded([&] {
+        if (!n_out || (k && !out))
+            fail(BMM_ERR_INVALID, "fwht_topk: null output");
+        *n_out = 0;
+        if (rows == 0 || k == 0 || words_per_row == 0)
+            return;
+        if (!row_words)
+            fail(BMM_ERR_INVALID, "fwht_topk: null rows");
+        if (rows > 48)
+            fail(BMM_ERR_INVALID, "fwht_topk: window rows must be <= 48");
+        if (k > BMM_TOPK_MAX)
+            fail(BMM_ERR_INVALID, "fwht_topk: max_candidates exceeds the GPU top-k capacity");
+        if (cols > words_per_row * 64)
+            fail(BMM_ERR_INVALID, "fwht_topk: cols exceeds words_per_row * 64");
+        require_device();
+        auto res = fwht_topk_rows(row_words, rows, words_per_row, cols, k);
+        std::copy(res.begin(), res.end(), out);
+        *n_out = res.size();
+    });
 }
 
-int bmm_hip_mitm_topk(const uint64_t*, const uint64_t*, const int32_t*, size_t, size_t, size_t,
-                      int32_t, size_t, bmm_topk_entry*, size_t*) {
-    set_last_error("mitm_topk: not built yet");
-    return BMM_ERR_DEVICE;
+int bmm_hip_mitm_topk(const uint64_t* q, const uint64_t* r, const int32_t* mult, size_t unique,
+                      size_t low_bits, size_t high_bits, int32_t total, size_t k,
+                      bmm_topk_entry* out, size_t* n_out) {
+    return guarded([&] {
+        if (!n_out || (k && !out))
+            fail(BMM_ERR_INVALID, "mitm_topk: null output");
+        *n_out = 0;
+        if (k == 0 || unique == 0)
+            return;
+        if (!q || !r || !mult)
+            fail(BMM_ERR_INVALID, "mitm_topk: null input");
+        const size_t t = low_bits + high_bits;
+        if (t == 0 || t > 48 || low_bits > 48 || high_bits > 48)
+            fail(BMM_ERR_INVALID, "mitm_topk: low_bits + high_bits must be in [1, 48]");
+        if (k > BMM_TOPK_MAX)
+            fail(BMM_ERR_INVALID, "mitm_topk: max_candidates exceeds the GPU top-k capacity");
+        int64_t sum = 0;
+        for (size_t i = 0; i < unique; ++i) {
+            if ((low_bits < 64 && r[i] >> low_bits) || (high_bits < 64 && q[i] >> high_bits))
+                fail(BMM_ERR_INVALID,
+                     "mitm_topk: pattern index is outside the selected split");
+            if (mult[i] <= 0)
+                fail(BMM_ERR_INVALID, "mitm_topk: multiplicity must be positive");
+            sum += mult[i];
+        }
+        if (sum != total)
+            fail(BMM_ERR_INVALID, "mitm_topk: total_weight differs from the multiplicity sum");
+        require_device();
+        auto res = mitm_topk_split(q, r, mult, unique, low_bits, high_bits, total, k);
+        std::copy(res.begin(), res.end(), out);
+        *n_out = res.size();
+    });
 }
 
-int bmm_hip_bruteforce_topk(const uint64_t*, size_t, size_t, size_t, size_t, bmm_topk_entry*,
-                            size_t*) {
-    set_last_error("bruteforce_topk: not built yet");
-    return BMM_ERR_DEVICE;
+int bmm_hip_bruteforce_topk(const uint64_t* row_words, size_t rows, size_t words_per_row,
+                            size_t chunk_bits, size_t k, bmm_topk_entry* out, size_t* n_out) {
+    return guarded([&] {
+        if (!n_out || (k && !out))
+            fail(BMM_ERR_INVALID, "bruteforce_topk: null output");
+        *n_out = 0;
+        if (rows == 0 || k == 0 || words_per_row == 0)
+            return;
+        if (!row_words)
+            fail(BMM_ERR_INVALID, "bruteforce_topk: null rows");
+        if (rows > 48)
+            fail(BMM_ERR_INVALID, "bruteforce_topk: window rows must be <= 48");
+        if (chunk_bits != 0 && (chunk_bits > rows || chunk_bits >= 64))
+            fail(BMM_ERR_INVALID, "bruteforce_topk: chunk_bits must be in [1, rows] or 0");
+        if (k > BMM_TOPK_MAX)
+            fail(BMM_ERR_INVALID, "bruteforce_topk: max_candidates exceeds the GPU top-k capacity");
+        require_device();
+        // Popcount over all words_per_row words, tail bits included.
+        auto res = fwht_topk_rows(row_words, rows, words_per_row, words_per_row * 64, k);
+        std::copy(res.begin(), res.end(), out);
+        *n_out = res.size();
+    });
+}
+
+int bmm_hip_replay(uint64_t* rows, size_t n_rows, size_t words_per_row, const uint16_t* ops,
+                   size_t n_ops, uint64_t* out_weight) {
+    return guarded([&] {
+        if (!out_weight || (n_ops && !ops) || (n_rows && words_per_row && !rows))
+            fail(BMM_ERR_INVALID, "replay: null pointer");
+        if (n_rows > 65536 || n_rows * words_per_row > ((size_t)1 << 34))
+            fail(BMM_ERR_INVALID, "replay: matrix too large");
+        if (n_ops > (size_t)INT32_MAX)
+            fail(BMM_ERR_INVALID, "replay: too many ops");
+        for (size_t i = 0; i < 2 * n_ops; ++i)
+            if (ops[i] >= n_rows)
+                fail(BMM_ERR_RANGE, "replay: row index out of range");
+        *out_weight = 0;
+        if (n_rows == 0 || words_per_row == 0)
+            return;
+        require_device();
+        CapiWorkspace& ws = capi_ws();
+        const size_t nw = n_rows * words_per_row;
+        uint64_t* d_rows = ws.a.get<uint64_t>(nw);
+        uint16_t* d_ops = ws.b.get<uint16_t>(std::max<size_t>(2 * n_ops, 1));
+        unsigned long long* d_wt = ws.c.get<unsigned long long>(1);
+        BMM_HIP(hipMemcpy(d_rows, rows, nw * 8, hipMemcpyHostToDevice));
+        if (n_ops)
+            BMM_HIP(hipMemcpy(d_ops, ops, 2 * n_ops * sizeof(uint16_t), hipMemcpyHostToDevice));
+        replay_dev(d_rows, (int)n_rows, (int)words_per_row, d_ops, (int)n_ops, d_wt, nullptr);
+        BMM_HIP(hipMemcpy(rows, d_rows, nw * 8, hipMemcpyDeviceToHost));
+        unsigned long long wt = 0;
+        BMM_HIP(hipMemcpy(&wt, d_wt, 8, hipMemcpyDeviceToHost));
+        *out_weight = wt;
+    });
 }
 
 } // extern "C"

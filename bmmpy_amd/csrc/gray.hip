@@ -196,14 +196,36 @@ int gray_auto_chunk(int min_block_bits) {
     return c;
 }
 
-void gray_coset_min_dev(const uint64_t* d_rows, size_t n_rows, size_t words_per_row,
-                        size_t block_rows, const int32_t* targets, size_t n_targets,
-                        size_t chunk_bits, uint64_t* d_first_key, uint64_t* d_mask_key,
-                        hipStream_t stream) {
+// Argument checks shared by the host and device entry points (run before any
+// device call).
+void gray_check_args(size_t n_rows, size_t words_per_row, size_t block_rows,
+                     const int32_t* targets, size_t n_targets, size_t chunk_bits) {
     if (words_per_row == 0 || words_per_row > 32)
         fail(BMM_ERR_INVALID, "gray_coset_min: words_per_row must be in [1, 32]");
     if (block_rows > n_rows || block_rows > 64)
         fail(BMM_ERR_INVALID, "gray_coset_min: block_rows must be <= n_rows");
+    int min_bits = 64;
+    for (size_t i = 0; i < n_targets; ++i) {
+        if (targets[i] < 0 || (size_t)targets[i] >= n_rows)
+            fail(BMM_ERR_RANGE, "gray_coset_min: target row out of range");
+        const int b = (int)block_rows - ((size_t)targets[i] < block_rows ? 1 : 0);
+        if (b > kGrayMaxBlock)
+            fail(BMM_ERR_INVALID, "gray_coset_min: block larger than 40 rows");
+        min_bits = std::min(min_bits, b);
+    }
+    if (n_targets == 0)
+        return;
+    const int c = chunk_bits ? (int)chunk_bits : gray_auto_chunk(min_bits);
+    if (min_bits - c < 8 || c < 4 || c > 20)
+        fail(BMM_ERR_INVALID,
+             "gray_coset_min: block must have >= chunk_bits + 8 rows and chunk_bits in [4, 20]");
+}
+
+void gray_coset_min_dev(const uint64_t* d_rows, size_t n_rows, size_t words_per_row,
+                        size_t block_rows, const int32_t* targets, size_t n_targets,
+                        size_t chunk_bits, uint64_t* d_first_key, uint64_t* d_mask_key,
+                        hipStream_t stream) {
+    gray_check_args(n_rows, words_per_row, block_rows, targets, n_targets, chunk_bits);
     if (n_targets == 0)
         return;
     std::vector<GrayTask> tasks(n_targets);

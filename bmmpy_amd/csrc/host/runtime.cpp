@@ -1,0 +1,35 @@
+// RuntimeFeatures / version (src/stub.cpp:17-43) and the host entry of the
+// GPU fwht_inplace (include/bmmpy/math/fwht.hpp:8-9).
+#include "bmmpy.hpp"
+
+#include "bmmpy_hip.h"
+
+#include <thread>
+
+namespace bmmpy {
+
+RuntimeFeatures get_runtime_features() {
+    RuntimeFeatures f;
+    f.max_threads = (int)std::max(1u, std::thread::hardware_concurrency());
+    bmm_device_info info{};
+    if (bmm_hip_device_info(&info) == BMM_OK && info.device_count > 0) {
+        f.hip_available = true;
+        f.device_count = info.device_count;
+        f.compute_units = info.compute_units;
+        f.device_name = info.name;
+        f.device_arch = info.arch;
+    }
+    return f;
+}
+
+const char* get_version() noexcept { return bmm_hip_version(); }
+
+void fwht_inplace(std::int32_t* data, std::size_t size) {
+    throw_status(bmm_hip_fwht_i32(data, size));
+}
+
+void fwht_inplace(std::int16_t* data, std::size_t size) {
+    throw_status(bmm_hip_fwht_i16(data, size));
+}
+
+} // namespace bmmpy

@@ -1,0 +1,243 @@
+// Exact top-k searches over a row window on gfx950: FwhtSearch
+// (src/search/fwht_search.cpp:34-132), the MITM seam over a CompactSplitWindow
+// (run_cuda_mitm_fwht_search, src/search/cuda_mitm_fwht_launch.cu:460-496)
+// and the brute-force seam for widths the Gray kernel does not hold in
+// registers.
+//
+// All three reduce to one engine: a multiset of (pattern, multiplicity) with
+// weight(m) = (total - H[m]) / 2, H = FWHT of the pattern histogram.  Masks are
+// split m = mh << L | ml; each persistent workgroup takes blocks mh from a
+// device counter, builds g_mh in LDS (sign (-1)^popcount((p >> L) & mh) per
+// pattern), transforms 2^L entries on chip and offers the packed keys
+// (weight << t | mask) to its exact top-K (topk.hpp).  A final one-workgroup
+// merge folds the per-workgroup lists.  The split point does not change the
+// result: the top-K under the strict (weight, mask) order is unique.
+#include "common.hpp"
+#include "fwht_block.hpp"
+#include "topk.hpp"
+
+#include <algorithm>
+#include <vector>
+
+namespace bmmhip {
+
+// g[pl] = sum_i mult[i] (-1)^popcount((pat[i] >> L) & mh) at pl = pat[i] mod 2^L.
+template <int L>
+__device__ inline void block_build_g64(int32_t* s, const uint64_t* __restrict__ pat,
+                                       const int32_t* __restrict__ mult, int n, uint64_t mh) {
+    constexpr int N = 1 << L;
+    for (int i = threadIdx.x; i < fwht_lds_words(L); i += blockDim.x)
+        s[i] = 0;
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        const uint64_t p = pat[i];
+        const int32_t m = mult[i];
+        const int neg = __builtin_popcountll((p >> L) & mh) & 1;
+        atomicAdd(&s[fwht_pad((int)(p & (N - 1)))], neg ? -m : m);
+    }
+}
+
+template <int L, int CAP>
+__global__ __launch_bounds__(256) void fwht_topk_kernel(const uint64_t* __restrict__ pat,
+                                                        const int32_t* __restrict__ mult,
+                                                        int n_pat, int t, int32_t total,
+                                                        uint64_t n_blocks, int K,
+                                                        unsigned long long* __restrict__ counter,
+                                                        uint64_t* __restrict__ lists) {
+    __shared__ int32_t s[fwht_lds_words(L)];
+    __shared__ TopkShared<CAP> T;
+    __shared__ unsigned long long blk_id;
+    constexpr int N = 1 << L;
+    constexpr int E = N >= 256 ? N / 256 : 1;
+    topk_init(T);
+    for (;;) {
+        if (threadIdx.x == 0)
+            blk_id = atomicAdd(counter, 1ull);
+        __syncthreads();
+        const uint64_t mh = blk_id;
+        if (mh >= n_blocks)
+            break;
+        block_build_g64<L>(s, pat, mult, n_pat, mh);
+        block_fwht<L>(s);
+        uint64_t keys[E];
+        uint32_t valid = 0;
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const int ml = e * 256 + (int)threadIdx.x;
+            keys[e] = ~0ull;
+            if (ml < N) {
+                const uint64_t mask = (mh << L) | (uint64_t)ml;
+                const int64_t w2 = (int64_t)total - (int64_t)s[fwht_pad(ml)];
+                if (mask != 0 && w2 >= 0 && (w2 & 1) == 0) {
+                    keys[e] = ((uint64_t)(w2 >> 1) << t) | mask;
+                    valid |= 1u << e;
+                }
+            }
+        }
+        topk_offer<CAP, E>(T, keys, valid, K);
+    }
+    topk_store(T, lists + (size_t)blockIdx.x * K, K);
+}
+
+// u64 column patterns of a t-row window (t <= 64) over `cols` columns.
+__global__ void column_patterns64_kernel(const uint64_t* __restrict__ rows, int t, int words,
+                                         int cols, uint64_t* __restrict__ pat,
+                                         int32_t* __restrict__ mult) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= cols)
+        return;
+    const int wi = c >> 6, bit = c & 63;
+    uint64_t p = 0;
+    for (int i = 0; i < t; ++i)
+        p |= ((rows[(size_t)i * words + wi] >> bit) & 1ull) << i;
+    pat[c] = p;
+    mult[c] = p != 0 ? 1 : 0;
+}
+
+__global__ void pattern_from_split_kernel(const uint64_t* __restrict__ q,
+                                          const uint64_t* __restrict__ r, int low_bits, int n,
+                                          uint64_t* __restrict__ pat) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        pat[i] = (low_bits >= 64 ? 0ull : (q[i] << low_bits)) | r[i];
+}
+
+namespace {
+struct SearchWorkspace {
+    DevBuf rows, pat, mult, q, r, counter, lists, out;
+};
+SearchWorkspace& search_ws() {
+    thread_local SearchWorkspace ws;
+    return ws;
+}
+
+constexpr int kTopkMaxL = 13;
+
+template <int L, int CAP>
+void launch_fwht_topk(const uint64_t* pat, const int32_t* mult, int n_pat, int t, int32_t total,
+                      uint64_t n_blocks, int K, unsigned long long* counter, uint64_t* lists,
+                      int grid, hipStream_t s) {
+    hipLaunchKernelGGL((fwht_topk_kernel<L, CAP>), dim3(grid), dim3(256), 0, s, pat, mult, n_pat,
+                       t, total, n_blocks, K, counter, lists);
+    BMM_HIP(hipGetLastError());
+}
+
+template <int CAP>
+void dispatch_fwht_topk(int L, const uint64_t* pat, const int32_t* mult, int n_pat, int t,
+                        int32_t total, uint64_t n_blocks, int K, unsigned long long* counter,
+                        uint64_t* lists, int grid, hipStream_t s) {
+    switch (L) {
+#define BMM_L(LL)                                                                              \
+    case LL:                                                                                   \
+        launch_fwht_topk<LL, CAP>(pat, mult, n_pat, t, total, n_blocks, K, counter, lists, grid, \
+                                  s);                                                          \
+        break;
+        BMM_L(1) BMM_L(2) BMM_L(3) BMM_L(4) BMM_L(5) BMM_L(6) BMM_L(7) BMM_L(8) BMM_L(9)
+        BMM_L(10) BMM_L(11) BMM_L(12) BMM_L(13)
+#undef BMM_L
+    default: fail(BMM_ERR_INVALID, "fwht_topk: bad split");
+    }
+}
+} // namespace
+
+int default_persistent_wgs();
+
+// Device-resident engine: d_pat/d_mult hold n_pat entries on the device.
+// Returns up to k (weight, mask) entries in ascending order.
+std::vector<bmm_topk_entry> fwht_topk_engine(const uint64_t* d_pat, const int32_t* d_mult,
+                                             int n_pat, int t, int64_t total, size_t k,
+                                             hipStream_t stream) {
+    if (t < 1 || t > 48)
+        fail(BMM_ERR_INVALID, "fwht_topk: window rows must be in [1, 48]");
+    if (k == 0)
+        return {};
+    if (k > BMM_TOPK_MAX)
+        fail(BMM_ERR_INVALID, "fwht_topk: max_candidates exceeds the GPU top-k capacity");
+    if (total < 0 || (uint64_t)total >= ((1ull << (64 - t)) - 1))
+        fail(BMM_ERR_OVERFLOW, "fwht_topk: total weight too large for packed keys");
+    const uint64_t n_masks = 1ull << t;
+    const int K = (int)std::min<uint64_t>(k, n_masks - 1);
+    const int L = std::min(t, kTopkMaxL);
+    const uint64_t n_blocks = n_masks >> L;
+    const int grid = (int)std::max<uint64_t>(
+        1, std::min<uint64_t>(n_blocks, (uint64_t)default_persistent_wgs() / (K > 512 ? 4 : 1)));
+    SearchWorkspace& ws = search_ws();
+    unsigned long long* d_counter = ws.counter.get<unsigned long long>(1);
+    uint64_t* d_lists = ws.lists.get<uint64_t>((size_t)grid * K);
+    uint64_t* d_out = ws.out.get<uint64_t>((size_t)K);
+    fill_u64(reinterpret_cast<uint64_t*>(d_counter), 1, 0, stream);
+    if (K <= 512) {
+        dispatch_fwht_topk<2048>(L, d_pat, d_mult, n_pat, t, (int32_t)total, n_blocks, K,
+                                 d_counter, d_lists, grid, stream);
+        hipLaunchKernelGGL(topk_merge_kernel<2048>, dim3(1), dim3(256), 0, stream, d_lists,
+                           (size_t)grid * K, K, d_out);
+    } else {
+        dispatch_fwht_topk<8192>(L, d_pat, d_mult, n_pat, t, (int32_t)total, n_blocks, K,
+                                 d_counter, d_lists, grid, stream);
+        hipLaunchKernelGGL(topk_merge_kernel<8192>, dim3(1), dim3(256), 0, stream, d_lists,
+                           (size_t)grid * K, K, d_out);
+    }
+    BMM_HIP(hipGetLastError());
+    std::vector<uint64_t> keys((size_t)K);
+    BMM_HIP(hipMemcpyAsync(keys.data(), d_out, keys.size() * 8, hipMemcpyDeviceToHost, stream));
+    BMM_HIP(hipStreamSynchronize(stream));
+    std::vector<bmm_topk_entry> out;
+    const uint64_t mmask = t >= 64 ? ~0ull : ((1ull << t) - 1);
+    for (uint64_t key : keys) {
+        if (key == ~0ull)
+            break;
+        out.push_back(bmm_topk_entry{key & mmask, (uint32_t)(key >> t), 0});
+    }
+    return out;
+}
+
+// FwhtSearch over host rows: patterns of the first `cols` columns (tail masked
+// as the reference does, src/search/fwht_search.cpp:64-65).
+std::vector<bmm_topk_entry> fwht_topk_rows(const uint64_t* rows, size_t t, size_t words,
+                                           size_t cols, size_t k) {
+    if (cols > (size_t)words * 64)
+        fail(BMM_ERR_INVALID, "fwht_topk: cols exceeds words_per_row * 64");
+    if (cols > (size_t)INT32_MAX)
+        fail(BMM_ERR_OVERFLOW, "fwht_topk: total weight exceeds int32 range");
+    SearchWorkspace& ws = search_ws();
+    uint64_t* d_rows = ws.rows.get<uint64_t>(t * words);
+    uint64_t* d_pat = ws.pat.get<uint64_t>(std::max<size_t>(cols, 1));
+    int32_t* d_mult = ws.mult.get<int32_t>(std::max<size_t>(cols, 1));
+    BMM_HIP(hipMemcpy(d_rows, rows, t * words * 8, hipMemcpyHostToDevice));
+    // Active-column count = the number of nonzero patterns.
+    int64_t total = 0;
+    if (cols) {
+        hipLaunchKernelGGL(column_patterns64_kernel, dim3((unsigned)((cols + 255) / 256)),
+                           dim3(256), 0, nullptr, d_rows, (int)t, (int)words, (int)cols, d_pat,
+                           d_mult);
+        BMM_HIP(hipGetLastError());
+        std::vector<int32_t> m(cols);
+        BMM_HIP(hipMemcpy(m.data(), d_mult, cols * 4, hipMemcpyDeviceToHost));
+        for (int32_t v : m)
+            total += v;
+    }
+    return fwht_topk_engine(d_pat, d_mult, (int)cols, (int)t, total, k, nullptr);
+}
+
+std::vector<bmm_topk_entry> mitm_topk_split(const uint64_t* q, const uint64_t* r,
+                                            const int32_t* mult, size_t unique, size_t low_bits,
+                                            size_t high_bits, int32_t total, size_t k) {
+    SearchWorkspace& ws = search_ws();
+    const size_t n = std::max<size_t>(unique, 1);
+    uint64_t* d_q = ws.q.get<uint64_t>(n);
+    uint64_t* d_r = ws.r.get<uint64_t>(n);
+    uint64_t* d_pat = ws.pat.get<uint64_t>(n);
+    int32_t* d_mult = ws.mult.get<int32_t>(n);
+    if (unique) {
+        BMM_HIP(hipMemcpy(d_q, q, unique * 8, hipMemcpyHostToDevice));
+        BMM_HIP(hipMemcpy(d_r, r, unique * 8, hipMemcpyHostToDevice));
+        BMM_HIP(hipMemcpy(d_mult, mult, unique * 4, hipMemcpyHostToDevice));
+        hipLaunchKernelGGL(pattern_from_split_kernel, dim3((unsigned)((unique + 255) / 256)),
+                           dim3(256), 0, nullptr, d_q, d_r, (int)low_bits, (int)unique, d_pat);
+        BMM_HIP(hipGetLastError());
+    }
+    return fwht_topk_engine(d_pat, d_mult, (int)unique, (int)(low_bits + high_bits), total, k,
+                            nullptr);
+}
+
+} // namespace bmmhip

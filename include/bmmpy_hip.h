@@ -133,6 +133,14 @@ int bmm_hip_gray_coset_min_dev(const uint64_t *d_rows, size_t n_rows, size_t wor
                                size_t chunk_bits, uint64_t *d_first_key, uint64_t *d_mask_key,
                                void *stream);
 
+/* ---- src/apply: replay of a row-XOR sequence (RowWindow::row_xor,
+ * src/core/bit_matrix_rows.cpp:12-20, as driven by GreedyApplier::apply,
+ * src/apply/greedy_applier.cpp:92-95).  ops holds n_ops (source, target)
+ * pairs of row indices; target ^= source, a no-op when they are equal.  Rows
+ * are updated in place (host memory); *out_weight = total popcount after. */
+int bmm_hip_replay(uint64_t *rows, size_t n_rows, size_t words_per_row, const uint16_t *ops,
+                   size_t n_ops, uint64_t *out_weight);
+
 /* ---- seeded synthetic workloads (no reference counterpart: they define the
  * benchmark inputs).  XorShift64 stream (include/bmmpy/core/detail/
  * xorshift64.hpp:10-40); Bernoulli(p) = next_unit_double() < p, row-major. */

@@ -1,0 +1,213 @@
+"""The Python surface, mirroring the reference's tests/python/test_search_api.py,
+test_cuda_api.py and test_public_api.py.  CPU tests cover names, argument
+validation (raised before any device query) and the data structures; GPU
+tests run the searchers and the applier against the oracle."""
+import random
+
+import numpy as np
+import pytest
+
+import bmmpy_amd as bmm
+import oracle
+
+ROWS_6X12 = ["110010101001", "101100101100", "011010011001", "111000010111", "000111100011",
+             "101011110000"]
+
+
+def make_search_matrix():
+    m = bmm.BitMatrix(2, 5)
+    for col in (0, 2, 4):
+        m[0, col] = True
+        m[1, col] = True
+    return m
+
+
+def pairs(cands):
+    return [(c.weight, c.mask_u64()) for c in cands]
+
+
+def words_of(m: "bmm.BitMatrix"):
+    return m.to_words()
+
+
+# ------------------------------------------------------------------ CPU ----
+def test_public_names():
+    for name in ("BitMatrix", "RowWindow", "Candidate", "FwhtSearch", "MitmFwhtSearch",
+                 "BruteforceSearch", "CudaMitmFwhtSearch", "CudaBruteforceSearch",
+                 "GreedyApplier", "ApplyResult", "search_apply", "matrix_from_rows",
+                 "get_runtime_features", "get_version", "fwht_i16", "fwht_i32", "MatrixError"):
+        assert hasattr(bmm, name), name
+    # config structs stay hidden (tests/python/test_public_api.py:9-26)
+    for hidden in ("FwhtSearchConfig", "MitmFwhtSearchConfig", "BruteforceSearchConfig"):
+        assert not hasattr(bmm, hidden)
+
+
+def test_searcher_names_and_repr():
+    assert bmm.FwhtSearch().name() == "fwht"
+    assert bmm.MitmFwhtSearch().name() == "mitm_fwht"
+    assert bmm.BruteforceSearch().name() == "bruteforce"
+    assert bmm.CudaMitmFwhtSearch(max_candidates=8, low_bits=0).name() == "cuda_mitm_fwht"
+    assert bmm.CudaBruteforceSearch(max_candidates=8, chunk_bits=0).name() == "cuda_bruteforce"
+    assert repr(bmm.FwhtSearch(max_rows=3, max_candidates=2)) == \
+        "FwhtSearch(max_rows=3, max_candidates=2)"
+
+
+def test_runtime_features():
+    f = bmm.get_runtime_features()
+    assert f.hip_compiled and not f.cuda_compiled and not f.cuda_available
+    assert f.hip_available == (f.device_count > 0)
+
+
+def test_bitmatrix_layout_and_ops():
+    m = bmm.BitMatrix(3, 130)
+    assert m.stride_words == 4 and m.words_per_row == 3 and m.shape == (3, 130)
+    m[0, 0] = True
+    m[0, 129] = True
+    m[1, 129] = True
+    m.row_xor(1, 0)
+    assert m.row_popcount(1) == 1 and m[1, 0] and not m[1, 129]
+    m.row_xor(1, 1)                          # no-op on itself
+    assert m.row_popcount(1) == 1
+    with pytest.raises(IndexError):
+        m[3, 0]
+    with pytest.raises(IndexError):
+        m.row_window([0, 7])
+    with pytest.raises(ValueError):
+        m.row_window([0, 0])
+    w = m.row_window([2, 0])
+    assert len(w) == 2 and w.rows == [2, 0] and w.cols == 130
+    assert w.materialize().to_rows()[1] == m.to_rows()[0]
+    assert bmm.matrix_from_rows(["101", "010"]).to_rows() == ["101", "010"]
+    with pytest.raises(ValueError):
+        bmm.matrix_from_rows(["10", "1"])
+    i3 = bmm.BitMatrix.identity(3)
+    assert i3.rank() == 3 and (i3 @ i3).to_rows() == i3.to_rows()
+
+
+def test_candidate_api():
+    c = bmm.Candidate.from_u64(0b1011, 7)
+    assert c.rows == [0, 1, 3] and len(c) == 3 and 3 in c and 2 not in c
+    big = bmm.Candidate([1, 2], 3)
+    with pytest.raises(RuntimeError):
+        big.mask_u64()
+
+
+def test_validation_before_device():
+    m = bmm.BitMatrix(3, 5)
+    w3 = m.row_window([0, 1, 2])
+    with pytest.raises(ValueError):
+        bmm.FwhtSearch(max_rows=2).search(w3)
+    assert bmm.FwhtSearch(max_candidates=0).search(w3) == []
+    assert bmm.MitmFwhtSearch(max_candidates=0).search(w3) == []
+    assert bmm.BruteforceSearch(max_candidates=0).search(w3) == []
+    with pytest.raises(ValueError):
+        bmm.BruteforceSearch(chunk_bits=4).search(w3)
+    # GPU-class limits (tests/cpp/search_cuda_tests.cpp:29-70, 113-135)
+    with pytest.raises(ValueError):
+        bmm.CudaMitmFwhtSearch().search(w3)
+    m28 = bmm.BitMatrix(28, 4)
+    m28[0, 0] = True
+    with pytest.raises(ValueError):
+        bmm.CudaMitmFwhtSearch(max_candidates=64, low_bits=28).search(m28.row_window(list(range(28))))
+    heavy = bmm.BitMatrix(28, 32768)
+    for c in range(32768):
+        heavy[0, c] = True
+    with pytest.raises(ValueError):
+        bmm.CudaMitmFwhtSearch().search(heavy.row_window(list(range(28))))
+    eq15 = bmm.BitMatrix.from_words(oracle.equivalence_matrix(15, 512), 512)
+    with pytest.raises(ValueError):
+        bmm.CudaBruteforceSearch().search(eq15.row_window(list(range(15))))
+    eq16 = bmm.BitMatrix.from_words(oracle.equivalence_matrix(16, 256), 256)
+    with pytest.raises(ValueError):
+        bmm.CudaBruteforceSearch().search(eq16.row_window(list(range(16))))
+    eq16w = bmm.BitMatrix.from_words(oracle.equivalence_matrix(16, 512), 512)
+    with pytest.raises(ValueError):
+        bmm.CudaBruteforceSearch(max_candidates=129).search(eq16w.row_window(list(range(16))))
+
+
+def test_no_device_raises_runtime_error():
+    if bmm.get_runtime_features().hip_available:
+        pytest.skip("a device is visible")
+    w = make_search_matrix().row_window([0, 1])
+    for s in (bmm.FwhtSearch(), bmm.MitmFwhtSearch(), bmm.BruteforceSearch()):
+        with pytest.raises(RuntimeError):
+            s.search(w)
+    with pytest.raises(RuntimeError):
+        bmm.fwht_i32([1, 2, 3, 4])
+
+
+def test_compact_split_window_kat():
+    """tests/cpp/search_cpu_tests.cpp:128-152."""
+    from bmmpy_amd import _bmmpy
+    m = bmm.matrix_from_rows(["11001", "00100", "01101", "01001"])
+    c = _bmmpy.build_compact_split_window(m.row_window([0, 1, 2, 3]), 2)
+    assert (c.t, c.low_bits, c.high_bits, c.total_weight) == (4, 2, 2, 4)
+    assert c.q == [0, 1, 3] and c.r == [1, 2, 1] and c.multiplicity == [1, 1, 2]
+
+
+# ------------------------------------------------------------------ GPU ----
+@pytest.mark.gpu
+def test_gpu_fwht_search_wrapper():
+    w = make_search_matrix().row_window([0, 1])
+    s = bmm.FwhtSearch(max_rows=16, max_candidates=1)
+    c = s.search(w)
+    assert len(c) == 1 and c[0].mask_u64() == 0x3 and c[0].weight == 0
+    assert len(bmm.FwhtSearch().search(w)) == 3
+
+
+@pytest.mark.gpu
+def test_gpu_mitm_and_brute_match_fwht():
+    m = bmm.matrix_from_rows(ROWS_6X12)
+    w = m.row_window([0, 1, 2, 3, 4, 5])
+    exp = pairs(bmm.FwhtSearch(max_rows=16, max_candidates=8).search(w))
+    assert pairs(bmm.MitmFwhtSearch(max_candidates=8).search(w)) == exp
+    assert pairs(bmm.BruteforceSearch(max_candidates=8, chunk_bits=0).search(w)) == exp
+
+
+@pytest.mark.gpu
+def test_gpu_search_apply():
+    m = make_search_matrix()
+    r = bmm.search_apply(m.row_window([0, 1]), searcher=bmm.FwhtSearch(max_rows=16,
+                                                                        max_candidates=1),
+                         applier=bmm.GreedyApplier(min_gain=1))
+    assert r.applied_count >= 1 and r.weight_improvement > 0
+    assert 0 in (m.row_popcount(0), m.row_popcount(1))
+
+
+@pytest.mark.gpu
+def test_gpu_cuda_named_searchers_match_cpu_semantics():
+    rng = random.Random(42)
+    m = bmm.BitMatrix(16, 512)
+    for r in range(16):
+        for c in range(512):
+            if rng.getrandbits(1):
+                m[r, c] = True
+    w = m.row_window(list(range(16)))
+    ora = oracle.oracle()
+    exp = ora.bruteforce_topk(m.to_words(), 512, 8)
+    assert pairs(bmm.CudaBruteforceSearch(max_candidates=8).search(w)) == exp
+    e28 = bmm.BitMatrix.from_words(oracle.equivalence_matrix(28, 96), 96)
+    w28 = e28.row_window(list(range(28)))
+    for k in (8, 128):
+        assert pairs(bmm.CudaMitmFwhtSearch(max_candidates=k).search(w28)) == \
+            ora.mitm_topk(e28.to_words(), 96, k)
+
+
+@pytest.mark.gpu
+def test_gpu_greedy_apply_matches_oracle():
+    ora = oracle.oracle()
+    base = ora.bernoulli(32, 12, 256, 0.3)
+    cands = ora.fwht_topk(base, 256, 40)
+    for mg, st, sd in ((0, False, 0), (1, False, 0), (3, True, 7)):
+        m = bmm.BitMatrix.from_words(base, 256)
+        res = bmm.GreedyApplier(mg, stochastic=st, seed=sd).apply(
+            m.row_window(list(range(12))), [bmm.Candidate.from_u64(mk, w) for w, mk in cands])
+        exp_rows, ap, imp = ora.greedy_apply(base, 256, cands, mg, st, sd)
+        assert (res.applied_count, res.weight_improvement) == (ap, imp)
+        assert np.array_equal(m.to_words(), exp_rows)
+
+
+@pytest.mark.gpu
+def test_gpu_fwht_math_wrappers():
+    assert bmm.fwht_i32([1, 2, 3, 4]).tolist() == [10, -2, -4, 0]
+    assert bmm.fwht_i16([32767, 1]).tolist() == [-32768, 32766]

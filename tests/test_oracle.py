@@ -1,0 +1,288 @@
+"""Pins the oracle (oracle/oracle.c) before it is trusted as the checker:
+against the reference's golden outputs (tests/golden, written from the
+reference's own compiled CPU code), against the reference library directly
+where it is built here, and against closed forms / brute force."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+@pytest.fixture(scope="module")
+def arrays():
+    return np.load(os.path.join(GOLD, "golden_arrays.npz"), allow_pickle=False)
+
+
+@pytest.fixture(scope="module")
+def gold():
+    with open(os.path.join(GOLD, "golden_searches.json")) as f:
+        return json.load(f)
+
+
+def rows_of(js):
+    return np.array([[int(v) for v in r] for r in js], dtype=np.uint64)
+
+
+def res_of(js):
+    return [(int(w), int(m)) for w, m in js]
+
+
+# ---------------------------------------------------------------- RNG ----
+def test_xorshift_recurrence(ora):
+    M = (1 << 64) - 1
+    x = 0x5EED0002
+    exp = []
+    for _ in range(8):
+        x ^= (x << 13) & M
+        x ^= x >> 7
+        x ^= (x << 17) & M
+        exp.append(x)
+    assert ora.xs_stream(0x5EED0002, 8).tolist() == exp
+    # seed 0 -> default state (xorshift64.hpp:8,15-16)
+    assert ora.xs_stream(0, 1)[0] == ora.xs_stream(0x9E3779B97F4A7C15, 1)[0]
+
+
+# --------------------------------------------------------------- FWHT ----
+def test_fwht_reference_kats(ora):
+    assert ora.fwht_i32([1, 2, 3, 4]).tolist() == [10, -2, -4, 0]   # fwht_tests.cpp:58-62
+    assert ora.fwht_i16([32767, 1]).tolist() == [-32768, 32766]     # fwht_tests.cpp:64-70
+
+
+def test_fwht_golden(ora, arrays):
+    for nb in (1, 4, 10, 14):
+        assert np.array_equal(ora.fwht_i32(arrays[f"fwht_i32_{nb}_in"]),
+                              arrays[f"fwht_i32_{nb}_out"])
+        assert np.array_equal(ora.fwht_i16(arrays[f"fwht_i16_{nb}_in"]),
+                              arrays[f"fwht_i16_{nb}_out"])
+
+
+def test_fwht_closed_forms(ora):
+    rng = np.random.default_rng(3)
+    n = 1 << 9
+    x = rng.integers(-50, 50, n).astype(np.int32)
+    h = ora.fwht_i32(x)
+    # brute-force Walsh sum H[m] = sum_p x[p] (-1)^popcount(p & m)
+    idx = np.arange(n)
+    par = np.array([[bin(int(p & m)).count("1") & 1 for p in idx] for m in idx[:64]])
+    assert np.array_equal(h[:64], ((1 - 2 * par) * x[None, :]).sum(1))
+    assert np.array_equal(ora.fwht_i32(h), n * x)                    # involution
+    assert int((h.astype(np.int64) ** 2).sum()) == n * int((x.astype(np.int64) ** 2).sum())
+    d = np.zeros(n, np.int32)
+    d[37] = 1                                                         # delta -> sign row
+    assert np.array_equal(ora.fwht_i32(d), 1 - 2 * par_row(37, n))
+    assert h[0] == x.sum()
+
+
+def par_row(p, n):
+    return np.array([bin(p & m).count("1") & 1 for m in range(n)], dtype=np.int32)
+
+
+def test_fwht_rejects_non_power_of_two(ora):
+    with pytest.raises(oracle.OracleError):
+        ora.fwht_i32([1, 2, 3])
+
+
+# ----------------------------------------------------------- histogram ---
+def test_weight_identity_every_mask(ora):
+    """(total - H[m]) / 2 equals popcount(XOR of rows in m) for every m."""
+    m = ora.bernoulli(11, 10, 150, 0.35)
+    b, total = ora.histogram(m, 150)
+    h = ora.fwht_i32(b)
+    assert b.sum() == total
+    for mask in range(1 << 10):
+        acc = np.zeros(m.shape[1], np.uint64)
+        for r in range(10):
+            if mask >> r & 1:
+                acc ^= m[r]
+        direct = sum(bin(int(v)).count("1") for v in acc)
+        assert (total - int(h[mask])) // 2 == direct
+
+
+# -------------------------------------------------------------- searches --
+def _run(ora, s):
+    m = rows_of(s["rows"])
+    if s["kind"] == "fwht":
+        return ora.fwht_topk(m, s["cols"], s["k"])
+    if s["kind"] == "mitm":
+        return ora.mitm_topk(m, s["cols"], s["k"])
+    return ora.bruteforce_topk(m, s["cols"], s["k"], s["chunk_bits"])
+
+
+def test_searches_match_reference_golden(ora, gold):
+    for s in gold["searches"]:
+        assert _run(ora, s) == res_of(s["result"]), s["name"]
+
+
+def test_search_kats():
+    """search_cpu_tests.cpp:18-52: 2x5 window -> best mask 0x3 weight 0, 3 results."""
+    o = oracle.oracle()
+    m = np.array([[0b10101], [0b10101]], dtype=np.uint64)
+    res = o.fwht_topk(m, 5, 64)
+    assert len(res) == 3 and res[0] == (0, 3)
+    assert o.fwht_topk(m, 5, 1) == [(0, 3)]
+    assert o.bruteforce_topk(m, 5, 1) == [(0, 3)]
+
+
+def test_topk_equals_sorted_brute_force(ora):
+    for seed in range(4):
+        t, cols = 8 + seed, 90 + 17 * seed
+        m = ora.bernoulli(seed + 50, t, cols, 0.3)
+        allw = []
+        for mask in range(1, 1 << t):
+            acc = np.zeros(m.shape[1], np.uint64)
+            for r in range(t):
+                if mask >> r & 1:
+                    acc ^= m[r]
+            allw.append((sum(bin(int(v)).count("1") for v in acc), mask))
+        allw.sort()
+        for k in (1, 7, 60):
+            exp = allw[:k]
+            assert ora.fwht_topk(m, cols, k) == exp
+            assert ora.mitm_topk(m, cols, k) == exp
+            assert ora.bruteforce_topk(m, cols, k) == exp
+
+
+def test_bruteforce_counts_tail_bits_fwht_does_not(ora, gold):
+    by = {s["name"]: s for s in gold["searches"]}
+    b = res_of(by["tail_12x70_bruteforce_k16"]["result"])
+    f = res_of(by["tail_12x70_fwht_k16"]["result"])
+    assert b != f   # the planted bit beyond cols changes brute-force weights only
+
+
+def test_split_window_golden(ora, gold):
+    for s in gold["splits"]:
+        tot, q, r, mult = ora.split_window(rows_of(s["rows"]), s["cols"], s["low_bits"])
+        assert tot == s["total"], s["name"]
+        assert q.tolist() == s["q"] and r.tolist() == s["r"] and mult.tolist() == s["mult"]
+
+
+# ------------------------------------------------------------- configs ----
+def test_coset_c1_golden(ora, arrays):
+    m = arrays["c1_rows"]
+    w, idx = ora.coset_min(m, 128)
+    assert np.array_equal(w, arrays["c1_weight"])
+    for i in range(20):
+        mk = int(arrays["c1_mask"][i])
+        below = mk & ((1 << i) - 1)
+        above = mk >> (i + 1)
+        assert int(idx[i]) == below | (above << i)
+
+
+def test_coset_matches_brute_force_small(ora):
+    m = ora.bernoulli(9, 9, 77, 0.45)
+    w, idx = ora.coset_min(m, 77)
+    for t in range(9):
+        others = [r for r in range(9) if r != t]
+        best = None
+        for s in range(1 << 8):
+            acc = m[t].copy()
+            for j, r in enumerate(others):
+                if s >> j & 1:
+                    acc ^= m[r]
+            wt = sum(bin(int(v)).count("1") for v in acc)
+            if best is None or wt < best[0]:
+                best = (wt, s)
+        assert (int(w[t]), int(idx[t])) == best
+
+
+def test_gray_c2_golden(ora, arrays):
+    m = arrays["c2_rows"]
+    for i, t in enumerate(arrays["c2_targets"]):
+        w, s, low = ora.gray_min(m, 24, int(t))
+        assert w == arrays["c2_weight"][i]
+        assert s == arrays["c2_first"][i]
+        assert low == arrays["c2_mask"][i]
+
+
+def test_gray_first_step_small_brute_force(ora):
+    m = ora.bernoulli(19, 12, 64, 0.5)
+    for t in (0, 4, 11):
+        blk = [r for r in range(10) if r != t]
+        seq = []
+        for s in range(1 << len(blk)):
+            g = s ^ (s >> 1)
+            acc = m[t].copy()
+            for j, r in enumerate(blk):
+                if g >> j & 1:
+                    acc ^= m[r]
+            seq.append((sum(bin(int(v)).count("1") for v in acc), s, g))
+        wmin = min(x[0] for x in seq)
+        first = min(x[1] for x in seq if x[0] == wmin)
+        low = min(x[2] for x in seq if x[0] == wmin)
+        assert ora.gray_min(m, 10, t) == (wmin, first, low)
+
+
+def test_c3_reduced_windows_golden(ora, gold):
+    S, A, M = ora.planted(0x5EED0003, 48, 4096, 0.02)
+    assert ora.rank(M, 48) == 48
+    for case in gold["c3"]:
+        others, t = case["others"], case["target"]
+        win = np.concatenate([A[others], A[t:t + 1]])
+        assert ora.mitm_topk(win, 4096, 64) == res_of(case["result"])
+
+
+def test_c4_small_matrices_golden(ora, gold):
+    rows, cols, dens, words, offs = ora.batch(0x5EED0004, 64)
+    for case in gold["c4"]:
+        mi = case["matrix"]
+        W = cols[mi] // 64
+        mat = words[offs[mi]: offs[mi] + rows[mi] * W].reshape(rows[mi], W)
+        w, idx = ora.coset_min(mat, int(cols[mi]))
+        assert w.tolist() == case["weight"], mi
+        for i, mk in enumerate(case["mask"]):
+            mk = int(mk)
+            assert int(idx[i]) == (mk & ((1 << i) - 1)) | ((mk >> (i + 1)) << i)
+
+
+def test_replay_golden(ora, arrays):
+    out, wt = ora.replay(arrays["replay_in"], 300, arrays["replay_ops"])
+    assert np.array_equal(out, arrays["replay_out"])
+    assert wt == int(arrays["replay_weight"][0])
+
+
+def test_replay_identity_and_noop(ora):
+    a = ora.bernoulli(3, 2, 200, 0.5)
+    pop = lambda r: sum(bin(int(v)).count("1") for v in r)   # noqa: E731
+    out, _ = ora.replay(a, 200, np.array([[0, 1]], np.uint16))
+    assert pop(out[1]) == pop(a[0]) + pop(a[1]) - 2 * pop(a[0] & a[1])
+    same, _ = ora.replay(a, 200, np.array([[1, 1]], np.uint16))
+    assert np.array_equal(same, a)
+
+
+def test_greedy_apply_golden(ora, gold):
+    g = gold["greedy"]
+    rows = rows_of(g["rows"])
+    cands = res_of(g["candidates"])
+    for case in g["cases"]:
+        out, ap, imp = ora.greedy_apply(rows, g["cols"], cands, case["min_gain"],
+                                        case["stochastic"], case["seed"])
+        assert (ap, imp) == (case["applied"], case["improvement"])
+        assert np.array_equal(out, rows_of(case["rows"]))
+
+
+# ------------------------------------------------- live reference (here) --
+def test_oracle_matches_live_reference_random(ora):
+    r = oracle.reference()
+    if r is None:
+        pytest.skip("reference CPU library not built here")
+    for seed in range(6):
+        t = 6 + 3 * seed
+        cols = 50 + 61 * seed
+        m = ora.bernoulli(900 + seed, t, cols, 0.2 + 0.05 * seed)
+        for k in (1, 33):
+            assert ora.fwht_topk(m, cols, k) == r.fwht_search(m, cols, k)
+            assert ora.mitm_topk(m, cols, k) == r.mitm_search(m, cols, k)
+            if t <= 18:
+                assert ora.bruteforce_topk(m, cols, k, 0) == r.bruteforce_search(m, cols, k, 0)
+                assert ora.bruteforce_topk(m, cols, k, 3) == r.bruteforce_search(m, cols, k, 3)
+
+
+def test_planted_generator_properties(ora):
+    S, A, M = ora.planted(0x5EED0005, 64, 512, 0.05)
+    assert ora.rank(M, 64) == 64
+    assert ora.rank(A, 512) == ora.rank(S, 512)
