@@ -7,6 +7,8 @@ apply and runtime surface; every search and replay runs as HIP kernels
 """
 from __future__ import annotations
 
+from . import _runtime  # noqa: F401  (must precede the native imports)
+
 from ._bmmpy import (  # noqa: F401
     ApplyResult,
     BitMatrix,

@@ -44,11 +44,20 @@ class DeviceInfo(C.Structure):
                 ("arch", C.c_char * 64), ("name", C.c_char * 128)]
 
 
+class BatchInfo(C.Structure):
+    _fields_ = [("n_matrices", C.c_size_t), ("n_slots", C.c_size_t),
+                ("total_blocks", C.c_size_t), ("persistent_wgs", C.c_int), ("chunk", C.c_int),
+                ("max_block_bits", C.c_int), ("subset_evals", C.c_double),
+                ("transform_elements", C.c_double)]
+
+
 # name -> (argtypes, restype)
 SIGNATURES = {
     "bmm_hip_last_error": ([], C.c_char_p),
     "bmm_hip_version": ([], C.c_char_p),
     "bmm_hip_device_info": ([C.POINTER(DeviceInfo)], C.c_int),
+    "bmm_hip_timing_begin": ([], C.c_int),
+    "bmm_hip_timing_end": ([C.POINTER(C.c_double), C.POINTER(_sz)], C.c_int),
     "bmm_hip_fwht_i32": ([_i32p, _sz], C.c_int),
     "bmm_hip_fwht_i16": ([_i16p, _sz], C.c_int),
     "bmm_hip_fwht_i32_dev": ([_vp, _sz, _vp], C.c_int),
@@ -60,6 +69,15 @@ SIGNATURES = {
     "bmm_hip_coset_min": ([_u64p, _sz, _sz, _sz, _u32p, _u64p], C.c_int),
     "bmm_hip_coset_min_batch": ([_u64p, _i64p, _i32p, _i32p, _sz, _i64p, _u32p, _u64p],
                                 C.c_int),
+    "bmm_hip_coset_min_batch_dev": ([_vp, _i64p, _i32p, _i32p, _sz, _i64p, _vp, _vp, _sz, _vp],
+                                    C.c_int),
+    "bmm_hip_batch_plan_create": ([_vp, _i64p, _i32p, _i32p, _sz, _i64p, _vp, _vp, _sz, C.c_int,
+                                   C.POINTER(C.c_void_p)], C.c_int),
+    "bmm_hip_batch_plan_run": ([_vp, C.c_int, _vp], C.c_int),
+    "bmm_hip_batch_plan_info": ([_vp, C.POINTER(BatchInfo)], C.c_int),
+    "bmm_hip_batch_plan_destroy": ([_vp], C.c_int),
+    "bmm_hip_window_coset_min": ([_u64p, _sz, _sz, _sz, _i32p, _sz, _i32p, _sz, _u32p, _u64p],
+                                 C.c_int),
     "bmm_hip_gray_coset_min": ([_u64p, _sz, _sz, _sz, _i32p, _sz, _u32p, _u64p, _u64p],
                                C.c_int),
     "bmm_hip_gray_coset_min_dev": ([_vp, _sz, _sz, _sz, _i32p, _sz, _sz, _vp, _vp, _vp],
@@ -78,6 +96,7 @@ def load() -> C.CDLL:
     """Load libbmmpy_hip.so; raises ImportError when it has not been built."""
     global _lib
     if _lib is None:
+        from . import _runtime  # noqa: F401
         if not os.path.exists(LIB_PATH):
             raise ImportError(f"{LIB_PATH} is missing: build it with "
                               "`python -c 'import __graft_entry__ as g; g.build()'`")
@@ -115,6 +134,17 @@ def device_info() -> DeviceInfo:
     d = DeviceInfo()
     call("bmm_hip_device_info", C.byref(d))
     return d
+
+
+def timing_begin() -> None:
+    call("bmm_hip_timing_begin")
+
+
+def timing_end() -> tuple[float, int]:
+    """(summed kernel milliseconds, launches) of the timed dominant kernels."""
+    ms, n = C.c_double(), _sz()
+    call("bmm_hip_timing_end", C.byref(ms), C.byref(n))
+    return ms.value, n.value
 
 
 def _rows(m) -> np.ndarray:
@@ -187,6 +217,20 @@ def coset_min_batch(words, word_offset, rows, cols, out_offset=None):
     i = np.zeros(max(n_slots, 1), np.uint64)
     call("bmm_hip_coset_min_batch", words, word_offset, rows, cols, rows.size, out_offset, w, i)
     return w[:n_slots], i[:n_slots]
+
+
+def window_coset_min(rows, cols: int, targets, windows):
+    """Per target t (targets[i]) and window rows windows[i]: min weight of
+    row_t ^ XOR(subset of window) and its lowest subset mask."""
+    rows = _rows(rows)
+    targets = np.ascontiguousarray(targets, dtype=np.int32)
+    windows = np.ascontiguousarray(windows, dtype=np.int32)
+    n, width = windows.shape
+    w = np.zeros(n, np.uint32)
+    m = np.zeros(n, np.uint64)
+    call("bmm_hip_window_coset_min", rows, rows.shape[0], rows.shape[1], cols,
+         windows.reshape(-1), width, targets, n, w, m)
+    return w, m
 
 
 def gray_coset_min(rows, block_rows: int, targets):

@@ -1,0 +1,480 @@
+// Batched per-target coset minimum (many matrices of uneven size, one
+// persistent launch fed by a device work queue).
+//
+// For a matrix of r rows every row is a target: min over subsets S of the other
+// rows of wt(row_i ^ XOR_S).  Equivalently, over the full-window ranking of
+// the reference FwhtSearch (src/search/fwht_search.cpp:59-126), the lowest
+// (weight, mask) among masks that contain bit i.  One full 2^r transform per
+// matrix serves all r targets: each block mh of 2^L masks is transformed on
+// chip (fwht_block.hpp) and scanned once, keeping per-target block minima:
+//   * target bits below L: per-thread minima over the elements with that bit
+//     (register bits) or the thread's own minimum (thread bits);
+//   * target bits at or above L: the block minimum when mh has the bit.
+// Each block folds (weight << 32 | mh) into the target's key with atomicMin,
+// so the key ends at the minimum weight and the lowest block reaching it;
+// phase 2 recomputes that one block per target and picks its lowest mask.
+//
+// Work queue: matrices are ordered by block size (largest first, LPT) and
+// their blocks numbered through a prefix sum; persistent workgroups take
+// `chunk` consecutive block numbers per atomicAdd and locate the matrix by
+// binary search over the prefix.  A plan (device descriptors, launch shape)
+// is built once; its whole launch sequence is captured in a hipGraph, and the
+// launch shape can be picked by timing candidates on the plan's own data.
+#include "common.hpp"
+#include "fwht_block.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <map>
+#include <memory>
+#include <tuple>
+#include <numeric>
+#include <vector>
+
+namespace bmmhip {
+
+struct CosetMatrix {
+    int64_t word_off;  // first word of the matrix in the caller's row buffer
+    int64_t pat_off;   // first column pattern
+    int64_t out_off;   // output slot of target 0
+    int32_t rows;
+    int32_t cols;
+    int32_t words;
+    int32_t total;     // active columns (set on the device)
+    int32_t L;         // transform bits per block
+    int32_t pad;
+};
+
+constexpr int kBatchMaxL = 13;
+
+// Column patterns (bit i = row i) and the active-column count; one workgroup
+// per matrix.
+__global__ __launch_bounds__(256) void batch_patterns_kernel(const uint64_t* __restrict__ words,
+                                                             CosetMatrix* __restrict__ mats,
+                                                             uint32_t* __restrict__ pats) {
+    CosetMatrix& M = mats[blockIdx.x];
+    const uint64_t* rows = words + M.word_off;
+    uint32_t cnt = 0;
+    for (int c = threadIdx.x; c < M.cols; c += blockDim.x) {
+        const int wi = c >> 6, bit = c & 63;
+        uint32_t p = 0;
+        for (int i = 0; i < M.rows; ++i)
+            p |= (uint32_t)((rows[(size_t)i * M.words + wi] >> bit) & 1ull) << i;
+        pats[M.pat_off + c] = p;
+        cnt += p != 0u;
+    }
+    cnt = wave_sum_u32(cnt);
+    __shared__ uint32_t red[4];
+    if (threadIdx.x % kWave == 0)
+        red[threadIdx.x / kWave] = cnt;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        M.total = (int32_t)(red[0] + red[1] + red[2] + red[3]);
+}
+
+template <int L>
+__device__ void coset_block_phase1(int32_t* s, const CosetMatrix& M, const uint32_t* pat,
+                                   uint32_t mh, unsigned long long* keys) {
+    block_build_g<L>(s, pat, M.cols, mh);
+    block_fwht<L>(s);
+    // Element ml = e * 256 + tid: ml bits >= 8 are register bits.
+    constexpr int N = 1 << L;
+    constexpr int E = N >= 256 ? N / 256 : 1;
+    constexpr int HB = L > 8 ? L - 8 : 0;
+    const int tid = threadIdx.x;
+    uint32_t overall = 0xffffffffu;
+    uint32_t mreg[HB > 0 ? HB : 1];
+#pragma unroll
+    for (int i = 0; i < (HB > 0 ? HB : 1); ++i)
+        mreg[i] = 0xffffffffu;
+    if (tid < N) {
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const int ml = e * 256 + tid;
+            const uint32_t w = (uint32_t)(M.total - s[fwht_pad(ml)]) >> 1;
+            overall = min(overall, w);
+#pragma unroll
+            for (int i = 0; i < HB; ++i)
+                if ((e >> i) & 1)
+                    mreg[i] = min(mreg[i], w);
+        }
+    }
+    __shared__ uint32_t red[32][4];
+    const int wv = tid / kWave, lane = tid % kWave;
+#pragma unroll
+    for (int i = 0; i < L; ++i) {
+        uint32_t v = i < 8 ? (((tid >> i) & 1) ? overall : 0xffffffffu) : mreg[i >= 8 ? i - 8 : 0];
+        v = wave_min_u32(v);
+        if (lane == 0)
+            red[i][wv] = v;
+    }
+    const uint32_t wover = wave_min_u32(overall);
+    for (int i = L; i < M.rows; ++i)
+        if (lane == 0)
+            red[i][wv] = ((mh >> (i - L)) & 1u) ? wover : 0xffffffffu;
+    __syncthreads();
+    if (tid < M.rows) {
+        const uint32_t v = min(min(red[tid][0], red[tid][1]), min(red[tid][2], red[tid][3]));
+        if (v != 0xffffffffu)
+            atomicMin(&keys[M.out_off + tid], ((unsigned long long)v << 32) | mh);
+    }
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(256) void coset_phase1_kernel(
+    const CosetMatrix* __restrict__ mats, const int64_t* __restrict__ prefix, int n_mat,
+    int64_t total_blocks, int chunk, const uint32_t* __restrict__ pats,
+    unsigned long long* __restrict__ counter, unsigned long long* __restrict__ keys) {
+    __shared__ int32_t s[fwht_lds_words(kBatchMaxL)];
+    __shared__ long long first;
+    for (;;) {
+        if (threadIdx.x == 0)
+            first = (long long)atomicAdd(counter, (unsigned long long)chunk);
+        __syncthreads();
+        const int64_t g0 = first;
+        __syncthreads();
+        if (g0 >= total_blocks)
+            break;
+        const int64_t g1 = min(g0 + (int64_t)chunk, total_blocks);
+        // Matrix holding block g0: last index with prefix[idx] <= g0.
+        int lo = 0, hi = n_mat - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (prefix[mid] <= g0)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+        int idx = lo;
+        for (int64_t g = g0; g < g1; ++g) {
+            while (prefix[idx + 1] <= g)
+                ++idx;
+            const CosetMatrix M = mats[idx];
+            const uint32_t mh = (uint32_t)(g - prefix[idx]);
+            const uint32_t* pat = pats + M.pat_off;
+            switch (M.L) {
+#define BMM_CASE(LL)                                                                           \
+    case LL: coset_block_phase1<LL>(s, M, pat, mh, keys); break;
+                BMM_CASE(1) BMM_CASE(2) BMM_CASE(3) BMM_CASE(4) BMM_CASE(5) BMM_CASE(6)
+                BMM_CASE(7) BMM_CASE(8) BMM_CASE(9) BMM_CASE(10) BMM_CASE(11) BMM_CASE(12)
+                BMM_CASE(13)
+#undef BMM_CASE
+            default: break;
+            }
+        }
+    }
+}
+
+template <int L>
+__device__ void coset_block_phase2(int32_t* s, const CosetMatrix& M, const uint32_t* pat,
+                                   int target, unsigned long long key, uint32_t* low_out) {
+    const uint32_t mh = (uint32_t)key;
+    const uint32_t wmin = (uint32_t)(key >> 32);
+    block_build_g<L>(s, pat, M.cols, mh);
+    block_fwht<L>(s);
+    constexpr int N = 1 << L;
+    __shared__ uint32_t best;
+    if (threadIdx.x == 0)
+        best = 0xffffffffu;
+    __syncthreads();
+    uint32_t mine = 0xffffffffu;
+    for (int ml = threadIdx.x; ml < N; ml += blockDim.x) {
+        const uint32_t w = (uint32_t)(M.total - s[fwht_pad(ml)]) >> 1;
+        const bool has = target < L ? ((ml >> target) & 1) : true;
+        if (has && w == wmin)
+            mine = min(mine, (uint32_t)ml);
+    }
+    mine = wave_min_u32(mine);
+    if ((threadIdx.x % kWave) == 0 && mine != 0xffffffffu)
+        atomicMin(&best, mine);
+    __syncthreads();
+    if (threadIdx.x == 0)
+        *low_out = best;
+    __syncthreads();
+}
+
+// One workgroup per output slot (matrix, target).
+__global__ __launch_bounds__(256) void coset_phase2_kernel(
+    const CosetMatrix* __restrict__ mats, const int32_t* __restrict__ slot_matrix,
+    const uint32_t* __restrict__ pats, const unsigned long long* __restrict__ keys,
+    uint32_t* __restrict__ out_w, uint64_t* __restrict__ out_idx) {
+    __shared__ int32_t s[fwht_lds_words(kBatchMaxL)];
+    __shared__ uint32_t low;
+    const int slot = blockIdx.x;
+    const CosetMatrix M = mats[slot_matrix[slot]];
+    const int target = (int)(slot - M.out_off);
+    const unsigned long long key = keys[slot];
+    const uint32_t* pat = pats + M.pat_off;
+    switch (M.L) {
+#define BMM_CASE(LL)                                                                           \
+    case LL: coset_block_phase2<LL>(s, M, pat, target, key, &low); break;
+        BMM_CASE(1) BMM_CASE(2) BMM_CASE(3) BMM_CASE(4) BMM_CASE(5) BMM_CASE(6) BMM_CASE(7)
+        BMM_CASE(8) BMM_CASE(9) BMM_CASE(10) BMM_CASE(11) BMM_CASE(12) BMM_CASE(13)
+#undef BMM_CASE
+    default: break;
+    }
+    if (threadIdx.x == 0) {
+        const uint64_t mask = ((uint64_t)(uint32_t)key << M.L) | low;
+        // Coset index over the other rows: drop the target bit, close the gap.
+        const uint64_t below = mask & ((1ull << target) - 1ull);
+        const uint64_t above = mask >> (target + 1);
+        out_w[slot] = (uint32_t)(key >> 32);
+        out_idx[slot] = below | (above << target);
+    }
+}
+
+int default_persistent_wgs();
+
+// ------------------------------------------------------------- plan ------
+struct BatchPlan {
+    int device = -1;
+    size_t n_mat = 0;
+    int64_t n_slots = 0;
+    int64_t total_blocks = 0;
+    double subset_evals = 0.0;   // sum of r * 2^(r-1)
+    double transform_elems = 0.0; // sum of 2^r
+    int grid = 0;
+    int chunk = 1;
+    int lmax = kBatchMaxL;
+    const uint64_t* d_words = nullptr;
+    uint32_t* d_out_w = nullptr;
+    uint64_t* d_out_idx = nullptr;
+    DevBuf mats, prefix, pats, keys, counter, slot_matrix;
+    std::vector<int32_t> rows, cols;
+    std::vector<int64_t> word_off, out_off;
+    hipStream_t cap_stream = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+
+    ~BatchPlan() {
+        if (exec)
+            (void)hipGraphExecDestroy(exec);
+        if (graph)
+            (void)hipGraphDestroy(graph);
+        if (cap_stream)
+            (void)hipStreamDestroy(cap_stream);
+    }
+};
+
+namespace {
+
+// Uploads descriptors for a given maximum block size.
+void plan_upload(BatchPlan& P, int lmax) {
+    P.lmax = lmax;
+    const size_t n = P.n_mat;
+    std::vector<CosetMatrix> mats(n);
+    int64_t pat_off = 0;
+    for (size_t m = 0; m < n; ++m) {
+        CosetMatrix& M = mats[m];
+        M.word_off = P.word_off[m];
+        M.pat_off = pat_off;
+        M.out_off = P.out_off[m];
+        M.rows = P.rows[m];
+        M.cols = P.cols[m];
+        M.words = (P.cols[m] + 63) / 64;
+        M.total = 0;
+        M.L = std::min(P.rows[m], lmax);
+        M.pad = 0;
+        pat_off += std::max(P.cols[m], 1);
+    }
+    // LPT order: biggest blocks first, then most blocks.
+    std::vector<int> order(n);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+        if (mats[a].L != mats[b].L)
+            return mats[a].L > mats[b].L;
+        return mats[a].rows > mats[b].rows;
+    });
+    std::vector<CosetMatrix> sorted(n);
+    std::vector<int64_t> prefix(n + 1, 0);
+    for (size_t i = 0; i < n; ++i) {
+        sorted[i] = mats[order[i]];
+        prefix[i + 1] = prefix[i] + (int64_t(1) << (sorted[i].rows - sorted[i].L));
+    }
+    P.total_blocks = prefix[n];
+    std::vector<int32_t> slot_matrix((size_t)P.n_slots, -1);
+    for (size_t i = 0; i < n; ++i)
+        for (int t = 0; t < sorted[i].rows; ++t) {
+            int32_t& sm = slot_matrix[(size_t)(sorted[i].out_off + t)];
+            if (sm != -1)
+                fail(BMM_ERR_INVALID, "coset_min: overlapping output offsets");
+            sm = (int32_t)i;
+        }
+    for (int32_t sm : slot_matrix)
+        if (sm < 0)
+            fail(BMM_ERR_INVALID, "coset_min: output offsets leave gaps");
+    CosetMatrix* d_m = P.mats.get<CosetMatrix>(n);
+    int64_t* d_p = P.prefix.get<int64_t>(n + 1);
+    int32_t* d_s = P.slot_matrix.get<int32_t>((size_t)P.n_slots);
+    P.pats.get<uint32_t>((size_t)std::max<int64_t>(pat_off, 1));
+    P.keys.get<unsigned long long>((size_t)P.n_slots);
+    P.counter.get<unsigned long long>(1);
+    BMM_HIP(hipMemcpy(d_m, sorted.data(), sizeof(CosetMatrix) * n, hipMemcpyHostToDevice));
+    BMM_HIP(hipMemcpy(d_p, prefix.data(), sizeof(int64_t) * (n + 1), hipMemcpyHostToDevice));
+    BMM_HIP(hipMemcpy(d_s, slot_matrix.data(), sizeof(int32_t) * slot_matrix.size(),
+                      hipMemcpyHostToDevice));
+}
+
+void plan_enqueue(BatchPlan& P, hipStream_t s, bool timed) {
+    CosetMatrix* d_m = static_cast<CosetMatrix*>(P.mats.ptr);
+    int64_t* d_p = static_cast<int64_t*>(P.prefix.ptr);
+    uint32_t* d_pat = static_cast<uint32_t*>(P.pats.ptr);
+    unsigned long long* d_k = static_cast<unsigned long long*>(P.keys.ptr);
+    unsigned long long* d_c = static_cast<unsigned long long*>(P.counter.ptr);
+    int32_t* d_s = static_cast<int32_t*>(P.slot_matrix.ptr);
+    fill_u64(reinterpret_cast<uint64_t*>(d_k), (size_t)P.n_slots, ~0ull, s);
+    fill_u64(reinterpret_cast<uint64_t*>(d_c), 1, 0ull, s);
+    hipLaunchKernelGGL(batch_patterns_kernel, dim3((unsigned)P.n_mat), dim3(256), 0, s, P.d_words,
+                       d_m, d_pat);
+    BMM_HIP(hipGetLastError());
+    {
+        ScopedKernelTimer timer(s, timed);
+        hipLaunchKernelGGL(coset_phase1_kernel, dim3(P.grid), dim3(256), 0, s, d_m, d_p,
+                           (int)P.n_mat, P.total_blocks, P.chunk, d_pat, d_c, d_k);
+    }
+    BMM_HIP(hipGetLastError());
+    hipLaunchKernelGGL(coset_phase2_kernel, dim3((unsigned)P.n_slots), dim3(256), 0, s, d_m, d_s,
+                       d_pat, d_k, P.d_out_w, P.d_out_idx);
+    BMM_HIP(hipGetLastError());
+}
+
+// Times the sequence for each candidate launch shape and keeps the fastest.
+void plan_autotune(BatchPlan& P) {
+    int cus = default_persistent_wgs() / 8;
+    struct Cand { int lmax, grid, chunk; };
+    std::vector<Cand> cands;
+    for (int lmax : {12, 13})
+        for (int per_cu : {4, 8})
+            for (int chunk : {1, 8})
+                cands.push_back({lmax, cus * per_cu, chunk});
+    hipStream_t s;
+    BMM_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    hipEvent_t a, b;
+    BMM_HIP(hipEventCreate(&a));
+    BMM_HIP(hipEventCreate(&b));
+    float best = 1e30f;
+    Cand bestc = {kBatchMaxL, cus * 8, 1};
+    int cur_lmax = -1;
+    for (const Cand& c : cands) {
+        if (c.lmax != cur_lmax) {
+            plan_upload(P, c.lmax);
+            cur_lmax = c.lmax;
+        }
+        P.grid = (int)std::max<int64_t>(1, std::min<int64_t>(c.grid, P.total_blocks));
+        P.chunk = c.chunk;
+        plan_enqueue(P, s, false);   // warm
+        BMM_HIP(hipEventRecord(a, s));
+        plan_enqueue(P, s, false);
+        BMM_HIP(hipEventRecord(b, s));
+        BMM_HIP(hipEventSynchronize(b));
+        float ms = 0.f;
+        BMM_HIP(hipEventElapsedTime(&ms, a, b));
+        if (ms < best) {
+            best = ms;
+            bestc = c;
+        }
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    BMM_HIP(hipStreamSynchronize(s));
+    (void)hipStreamDestroy(s);
+    if (bestc.lmax != cur_lmax)
+        plan_upload(P, bestc.lmax);
+    P.grid = (int)std::max<int64_t>(1, std::min<int64_t>(bestc.grid, P.total_blocks));
+    P.chunk = bestc.chunk;
+}
+
+// Launch shapes chosen by autotuning, keyed by the batch's shape signature.
+std::map<std::vector<int64_t>, std::tuple<int, int, int>>& tune_cache() {
+    static std::map<std::vector<int64_t>, std::tuple<int, int, int>> c;
+    return c;
+}
+
+} // namespace
+
+BatchPlan* batch_plan_create(const uint64_t* d_words, const int64_t* word_offset,
+                             const int32_t* rows, const int32_t* cols, size_t n_mat,
+                             const int64_t* out_offset, uint32_t* d_out_w, uint64_t* d_out_idx,
+                             int persistent_wgs, int autotune) {
+    if (n_mat == 0)
+        fail(BMM_ERR_INVALID, "coset_min: empty batch");
+    auto P = std::make_unique<BatchPlan>();
+    P->n_mat = n_mat;
+    P->d_words = d_words;
+    P->d_out_w = d_out_w;
+    P->d_out_idx = d_out_idx;
+    P->rows.assign(rows, rows + n_mat);
+    P->cols.assign(cols, cols + n_mat);
+    P->word_off.assign(word_offset, word_offset + n_mat);
+    P->out_off.assign(out_offset, out_offset + n_mat);
+    std::vector<int64_t> sig;
+    for (size_t m = 0; m < n_mat; ++m) {
+        if (rows[m] < 2 || rows[m] > 31)
+            fail(BMM_ERR_INVALID, "coset_min: rows must be in [2, 31]");
+        if (cols[m] < 0 || word_offset[m] < 0 || out_offset[m] < 0)
+            fail(BMM_ERR_INVALID, "coset_min: negative size or offset");
+        P->n_slots = std::max<int64_t>(P->n_slots, out_offset[m] + rows[m]);
+        P->subset_evals += (double)rows[m] * std::ldexp(1.0, rows[m] - 1);
+        P->transform_elems += std::ldexp(1.0, rows[m]);
+        sig.push_back(rows[m]);
+        sig.push_back(cols[m]);
+    }
+    BMM_HIP(hipGetDevice(&P->device));
+    auto it = tune_cache().find(sig);
+    if (autotune && it == tune_cache().end()) {
+        plan_autotune(*P);
+        tune_cache()[sig] = {P->lmax, P->grid, P->chunk};
+    } else {
+        int lmax = kBatchMaxL, grid = persistent_wgs ? persistent_wgs : default_persistent_wgs();
+        int chunk = 1;
+        if (it != tune_cache().end() && persistent_wgs == 0)
+            std::tie(lmax, grid, chunk) = it->second;
+        plan_upload(*P, lmax);
+        P->grid = (int)std::max<int64_t>(1, std::min<int64_t>(grid, P->total_blocks));
+        P->chunk = chunk;
+    }
+    return P.release();
+}
+
+void batch_plan_run(BatchPlan* P, hipStream_t stream, bool use_graph) {
+    if (!use_graph) {
+        plan_enqueue(*P, stream, true);
+        return;
+    }
+    if (!P->exec) {
+        BMM_HIP(hipStreamCreateWithFlags(&P->cap_stream, hipStreamNonBlocking));
+        BMM_HIP(hipStreamBeginCapture(P->cap_stream, hipStreamCaptureModeThreadLocal));
+        plan_enqueue(*P, P->cap_stream, false);
+        BMM_HIP(hipStreamEndCapture(P->cap_stream, &P->graph));
+        BMM_HIP(hipGraphInstantiate(&P->exec, P->graph, nullptr, nullptr, 0));
+    }
+    ScopedKernelTimer timer(stream);
+    BMM_HIP(hipGraphLaunch(P->exec, stream));
+}
+
+void batch_plan_destroy(BatchPlan* P) { delete P; }
+
+void batch_plan_info(const BatchPlan* P, bmm_batch_info* out) {
+    out->n_matrices = P->n_mat;
+    out->n_slots = (size_t)P->n_slots;
+    out->total_blocks = (size_t)P->total_blocks;
+    out->persistent_wgs = P->grid;
+    out->chunk = P->chunk;
+    out->max_block_bits = P->lmax;
+    out->subset_evals = P->subset_evals;
+    out->transform_elements = P->transform_elems;
+}
+
+// One-shot form used by the plain entry points.
+void coset_min_batch_dev(const uint64_t* d_rows, const int64_t* word_offset, const int32_t* rows,
+                         const int32_t* cols, size_t n_mat, const int64_t* out_offset,
+                         uint32_t* d_out_w, uint64_t* d_out_idx, int persistent_wgs,
+                         hipStream_t stream) {
+    std::unique_ptr<BatchPlan> P(batch_plan_create(d_rows, word_offset, rows, cols, n_mat,
+                                                   out_offset, d_out_w, d_out_idx,
+                                                   persistent_wgs, 0));
+    batch_plan_run(P.get(), stream, false);
+    BMM_HIP(hipStreamSynchronize(stream));
+}
+
+} // namespace bmmhip

@@ -16,6 +16,11 @@ thread_local std::string g_last_error;
 
 void set_last_error(const char* msg) { g_last_error = msg ? msg : ""; }
 
+KernelTimer& kernel_timer() {
+    static KernelTimer t;
+    return t;
+}
+
 __global__ void fill_u64_kernel(uint64_t* p, size_t n, uint64_t v) {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
          i += (size_t)gridDim.x * blockDim.x)
@@ -63,6 +68,17 @@ void coset_min_batch_dev(const uint64_t* d_rows, const int64_t* word_offset, con
                          uint32_t* d_out_w, uint64_t* d_out_idx, int persistent_wgs,
                          hipStream_t stream);
 int default_persistent_wgs();
+void window_coset_min(const uint64_t* rows, size_t n_rows, size_t words, size_t cols,
+                      const int32_t* window_rows, size_t width, const int32_t* targets,
+                      size_t n_targets, uint32_t* out_w, uint64_t* out_mask);
+struct BatchPlan;
+BatchPlan* batch_plan_create(const uint64_t* d_words, const int64_t* word_offset,
+                             const int32_t* rows, const int32_t* cols, size_t n_mat,
+                             const int64_t* out_offset, uint32_t* d_out_w, uint64_t* d_out_idx,
+                             int persistent_wgs, int autotune);
+void batch_plan_run(BatchPlan* P, hipStream_t stream, bool use_graph);
+void batch_plan_destroy(BatchPlan* P);
+void batch_plan_info(const BatchPlan* P, bmm_batch_info* out);
 std::vector<bmm_topk_entry> fwht_topk_rows(const uint64_t* rows, size_t t, size_t words,
                                            size_t cols, size_t k);
 std::vector<bmm_topk_entry> mitm_topk_split(const uint64_t* q, const uint64_t* r,
@@ -104,6 +120,39 @@ extern "C" {
 const char* bmm_hip_last_error(void) { return g_last_error.c_str(); }
 
 const char* bmm_hip_version(void) { return "0.1.0"; }
+
+int bmm_hip_timing_begin(void) {
+    return guarded([&] {
+        KernelTimer& t = kernel_timer();
+        for (auto& e : t.events) {
+            (void)hipEventDestroy(e.first);
+            (void)hipEventDestroy(e.second);
+        }
+        t.events.clear();
+        t.enabled = true;
+    });
+}
+
+int bmm_hip_timing_end(double* total_ms, size_t* launches) {
+    return guarded([&] {
+        if (!total_ms || !launches)
+            fail(BMM_ERR_INVALID, "timing_end: null output");
+        KernelTimer& t = kernel_timer();
+        t.enabled = false;
+        double sum = 0.0;
+        for (auto& e : t.events) {
+            BMM_HIP(hipEventSynchronize(e.second));
+            float ms = 0.f;
+            BMM_HIP(hipEventElapsedTime(&ms, e.first, e.second));
+            sum += ms;
+            (void)hipEventDestroy(e.first);
+            (void)hipEventDestroy(e.second);
+        }
+        *total_ms = sum;
+        *launches = t.events.size();
+        t.events.clear();
+    });
+}
 
 int bmm_hip_device_info(bmm_device_info* out) {
     return guarded([&] {
@@ -220,6 +269,85 @@ int bmm_hip_coset_min_batch(const uint64_t* row_words, const int64_t* word_offse
                             default_persistent_wgs(), nullptr);
         BMM_HIP(hipMemcpy(out_weight, d_w, slots * 4, hipMemcpyDeviceToHost));
         BMM_HIP(hipMemcpy(out_index, d_i, slots * 8, hipMemcpyDeviceToHost));
+    });
+}
+
+int bmm_hip_coset_min_batch_dev(const uint64_t* d_row_words, const int64_t* word_offset,
+                                const int32_t* rows, const int32_t* cols, size_t n_mat,
+                                const int64_t* out_offset, uint32_t* d_out_weight,
+                                uint64_t* d_out_index, size_t persistent_wgs, void* stream) {
+    return guarded([&] {
+        if (n_mat == 0)
+            return;
+        if (!d_row_words || !word_offset || !rows || !cols || !out_offset || !d_out_weight ||
+            !d_out_index)
+            fail(BMM_ERR_INVALID, "coset_min_batch: null pointer");
+        coset_min_batch_dev(d_row_words, word_offset, rows, cols, n_mat, out_offset, d_out_weight,
+                            d_out_index,
+                            persistent_wgs ? (int)persistent_wgs : default_persistent_wgs(),
+                            (hipStream_t)stream);
+    });
+}
+
+int bmm_hip_batch_plan_create(const uint64_t* d_row_words, const int64_t* word_offset,
+                              const int32_t* rows, const int32_t* cols, size_t n_mat,
+                              const int64_t* out_offset, uint32_t* d_out_weight,
+                              uint64_t* d_out_index, size_t persistent_wgs, int autotune,
+                              void** plan) {
+    return guarded([&] {
+        if (!plan)
+            fail(BMM_ERR_INVALID, "batch_plan_create: null handle");
+        *plan = nullptr;
+        if (!d_row_words || !word_offset || !rows || !cols || !out_offset || !d_out_weight ||
+            !d_out_index)
+            fail(BMM_ERR_INVALID, "batch_plan_create: null pointer");
+        *plan = batch_plan_create(d_row_words, word_offset, rows, cols, n_mat, out_offset,
+                                  d_out_weight, d_out_index, (int)persistent_wgs, autotune);
+    });
+}
+
+int bmm_hip_batch_plan_run(void* plan, int use_graph, void* stream) {
+    return guarded([&] {
+        if (!plan)
+            fail(BMM_ERR_INVALID, "batch_plan_run: null handle");
+        batch_plan_run(static_cast<BatchPlan*>(plan), (hipStream_t)stream, use_graph != 0);
+    });
+}
+
+int bmm_hip_batch_plan_info(const void* plan, bmm_batch_info* out) {
+    return guarded([&] {
+        if (!plan || !out)
+            fail(BMM_ERR_INVALID, "batch_plan_info: null pointer");
+        batch_plan_info(static_cast<const BatchPlan*>(plan), out);
+    });
+}
+
+int bmm_hip_batch_plan_destroy(void* plan) {
+    return guarded([&] { batch_plan_destroy(static_cast<BatchPlan*>(plan)); });
+}
+
+int bmm_hip_window_coset_min(const uint64_t* row_words, size_t n_rows, size_t words_per_row,
+                             size_t cols, const int32_t* window_rows, size_t width,
+                             const int32_t* targets, size_t n_targets, uint32_t* out_weight,
+                             uint64_t* out_mask) {
+    return guarded([&] {
+        if (n_targets == 0)
+            return;
+        if (!row_words || !window_rows || !targets || !out_weight || !out_mask)
+            fail(BMM_ERR_INVALID, "window_coset_min: null pointer");
+        if (width == 0 || width > 47)
+            fail(BMM_ERR_INVALID, "window_coset_min: window width must be in [1, 47]");
+        for (size_t i = 0; i < n_targets; ++i) {
+            if (targets[i] < 0 || (size_t)targets[i] >= n_rows)
+                fail(BMM_ERR_RANGE, "window_coset_min: target row out of range");
+            for (size_t j = 0; j < width; ++j)
+                if (window_rows[i * width + j] < 0 ||
+                    (size_t)window_rows[i * width + j] >= n_rows)
+                    fail(BMM_ERR_RANGE, "window_coset_min: window row out of range");
+        }
+        require_device();
+        window_coset_min(row_words, n_rows, words_per_row, cols, window_rows, width, targets,
+                         n_targets, out_weight, out_mask);
     });
 }
 

@@ -7,6 +7,8 @@
 #include <cstdint>
 #include <stdexcept>
 #include <string>
+#include <utility>
+#include <vector>
 
 #include "../../include/bmmpy_hip.h"
 
@@ -135,6 +137,36 @@ __device__ inline uint32_t wave_sum_u32(uint32_t v) {
         v += __shfl_xor(v, off, kWave);
     return v;
 }
+
+// Optional HIP-event timing of the dominant kernel of each entry point
+// (bmm_hip_timing_begin / bmm_hip_timing_end).  When enabled, a
+// ScopedKernelTimer records an event pair on the launch stream around the
+// kernel it wraps.
+struct KernelTimer {
+    bool enabled = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
+};
+KernelTimer& kernel_timer();
+
+struct ScopedKernelTimer {
+    hipStream_t stream;
+    hipEvent_t stop = nullptr;
+    explicit ScopedKernelTimer(hipStream_t s, bool active = true) : stream(s) {
+        KernelTimer& t = kernel_timer();
+        if (!t.enabled || !active)
+            return;
+        hipEvent_t a, b;
+        BMM_HIP(hipEventCreate(&a));
+        BMM_HIP(hipEventCreate(&b));
+        BMM_HIP(hipEventRecord(a, stream));
+        t.events.emplace_back(a, b);
+        stop = b;
+    }
+    ~ScopedKernelTimer() {
+        if (stop)
+            (void)hipEventRecord(stop, stream);
+    }
+};
 
 // Fill kernel used instead of hipMemsetAsync so the same launches can sit in
 // a captured graph.

@@ -180,6 +180,7 @@ GrayWorkspace& gray_ws() {
 template <int NW, int RES>
 void launch_gray(const uint32_t* rows, const GrayTask* d_tasks, const int32_t* d_wg_task,
                  int n_wg, int c, uint64_t* k1, uint64_t* k2, hipStream_t s) {
+    ScopedKernelTimer timer(s);
     hipLaunchKernelGGL((gray_coset_kernel<NW, RES>), dim3(n_wg), dim3(256), 0, s, rows, d_tasks,
                        d_wg_task, c, k1, k2);
     BMM_HIP(hipGetLastError());

@@ -1,0 +1,232 @@
+// Single-target coset minimum over wide windows (the planted meet-in-the-middle
+// workload): for target row t and window rows w_0..w_{n-1} (n <= 47), the
+// minimum over subsets S of the window of wt(row_t ^ XOR_S) and the lowest
+// such S (bit j = w_j).
+//
+// Signed histogram: every active column c (any of target/window set) has a
+// window pattern p_c and a sign s_c = -1 if the target bit is set, else +1;
+// b[p] = sum of s_c, total = active columns; then weight(m) = (total - H[m]) / 2
+// with H = FWHT(b) (the FwhtSearch identity, src/search/fwht_search.cpp:105-112,
+// applied to the window + target with the target bit forced).  The mask space
+// is split m = mh << L | ml exactly as in the MITM search
+// (src/search/mitm_fwht_search.cpp:164-265, which walks the low half and
+// transforms the high half): one workgroup block enumerates one mh, builds
+// g_mh[pl] = sum_i mult_i (-1)^popcount(ph_i & mh) in LDS from the unique
+// (pattern, signed multiplicity) list and transforms the 2^L low half on chip.
+// Each block folds (weight << n | mask) into the target's key with one
+// atomicMin, so the final key carries the minimum weight and its lowest mask.
+#include "common.hpp"
+#include "fwht_block.hpp"
+
+#include <algorithm>
+#include <vector>
+
+namespace bmmhip {
+
+struct WinTarget {
+    int64_t pat_off;
+    int32_t n_pat;
+    int32_t bits;     // window size n
+    int32_t total;
+    int32_t L;
+    int64_t first_block;  // global number of this target's block 0
+};
+
+constexpr int kWinMaxL = 13;
+
+template <int L>
+__device__ inline void block_build_g_signed(int32_t* s, const uint64_t* __restrict__ pat,
+                                            const int32_t* __restrict__ mult, int n,
+                                            uint64_t mh) {
+    for (int i = threadIdx.x; i < fwht_lds_words(L); i += blockDim.x)
+        s[i] = 0;
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        const uint64_t p = pat[i];
+        const int32_t m = mult[i];
+        const int neg = __builtin_popcountll((p >> L) & mh) & 1;
+        atomicAdd(&s[fwht_pad((int)(p & ((1u << L) - 1u)))], neg ? -m : m);
+    }
+}
+
+template <int L>
+__device__ void win_block(int32_t* s, const WinTarget& T, const uint64_t* pats,
+                          const int32_t* mults, uint64_t mh, unsigned long long* key) {
+    block_build_g_signed<L>(s, pats + T.pat_off, mults + T.pat_off, T.n_pat, mh);
+    block_fwht<L>(s);
+    constexpr int N = 1 << L;
+    constexpr int E = N >= 256 ? N / 256 : 1;
+    uint32_t bw = 0xffffffffu, bml = 0;
+    if ((int)threadIdx.x < N) {
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const int ml = e * 256 + (int)threadIdx.x;
+            const uint32_t w = (uint32_t)(T.total - s[fwht_pad(ml)]) >> 1;
+            if (w < bw) {   // elements visited in increasing ml: first minimum kept
+                bw = w;
+                bml = (uint32_t)ml;
+            }
+        }
+    }
+    unsigned long long k = bw == 0xffffffffu
+                               ? ~0ull
+                               : (((unsigned long long)bw << T.bits) | (mh << L) | bml);
+    k = wave_min_u64(k);
+    if ((threadIdx.x % kWave) == 0 && k != ~0ull)
+        atomicMin(key, k);
+}
+
+__global__ __launch_bounds__(256) void win_min_kernel(const WinTarget* __restrict__ tgts,
+                                                      int n_tgt, int64_t total_blocks,
+                                                      const uint64_t* __restrict__ pats,
+                                                      const int32_t* __restrict__ mults,
+                                                      int chunk,
+                                                      unsigned long long* __restrict__ counter,
+                                                      unsigned long long* __restrict__ keys) {
+    __shared__ int32_t s[fwht_lds_words(kWinMaxL)];
+    __shared__ long long first;
+    for (;;) {
+        if (threadIdx.x == 0)
+            first = (long long)atomicAdd(counter, (unsigned long long)chunk);
+        __syncthreads();
+        const int64_t g0 = first;
+        __syncthreads();
+        if (g0 >= total_blocks)
+            break;
+        const int64_t g1 = min(g0 + (int64_t)chunk, total_blocks);
+        int ti = 0;
+        while (ti + 1 < n_tgt && tgts[ti + 1].first_block <= g0)
+            ++ti;
+        for (int64_t g = g0; g < g1; ++g) {
+            while (ti + 1 < n_tgt && tgts[ti + 1].first_block <= g)
+                ++ti;
+            const WinTarget T = tgts[ti];
+            const uint64_t mh = (uint64_t)(g - T.first_block);
+            switch (T.L) {
+#define BMM_CASE(LL)                                                                           \
+    case LL: win_block<LL>(s, T, pats, mults, mh, &keys[ti]); break;
+                BMM_CASE(1) BMM_CASE(2) BMM_CASE(3) BMM_CASE(4) BMM_CASE(5) BMM_CASE(6)
+                BMM_CASE(7) BMM_CASE(8) BMM_CASE(9) BMM_CASE(10) BMM_CASE(11) BMM_CASE(12)
+                BMM_CASE(13)
+#undef BMM_CASE
+            default: break;
+            }
+            __syncthreads();
+        }
+    }
+}
+
+namespace {
+struct WinWorkspace {
+    DevBuf tgts, pats, mults, keys, counter;
+};
+WinWorkspace& win_ws() {
+    thread_local WinWorkspace ws;
+    return ws;
+}
+} // namespace
+
+int default_persistent_wgs();
+
+// Host: unique (pattern, signed multiplicity) per target, then one persistent
+// launch over every target's blocks.  Results: weight and lowest window mask.
+void window_coset_min(const uint64_t* rows, size_t n_rows, size_t words, size_t cols,
+                      const int32_t* window_rows, size_t width, const int32_t* targets,
+                      size_t n_targets, uint32_t* out_w, uint64_t* out_mask) {
+    if (width == 0 || width > 47)
+        fail(BMM_ERR_INVALID, "window_coset_min: window width must be in [1, 47]");
+    if (cols > words * 64)
+        fail(BMM_ERR_INVALID, "window_coset_min: cols exceeds words_per_row * 64");
+    if (cols >= (1ull << (63 - width)))
+        fail(BMM_ERR_OVERFLOW, "window_coset_min: too many columns for packed keys");
+    std::vector<WinTarget> tg(n_targets);
+    std::vector<uint64_t> all_p;
+    std::vector<int32_t> all_m;
+    int64_t blocks = 0;
+    const uint64_t tail = cols % 64 ? ((1ull << (cols % 64)) - 1) : ~0ull;
+    for (size_t i = 0; i < n_targets; ++i) {
+        const int t = targets[i];
+        if (t < 0 || (size_t)t >= n_rows)
+            fail(BMM_ERR_RANGE, "window_coset_min: target row out of range");
+        const int32_t* wr = window_rows + i * width;
+        for (size_t j = 0; j < width; ++j) {
+            if (wr[j] < 0 || (size_t)wr[j] >= n_rows)
+                fail(BMM_ERR_RANGE, "window_coset_min: window row out of range");
+            if (wr[j] == t)
+                fail(BMM_ERR_INVALID, "window_coset_min: target inside its own window");
+        }
+        // (pattern << 1 | target bit) per active column, sorted to aggregate.
+        std::vector<uint64_t> keyed;
+        int32_t total = 0;
+        for (size_t wi = 0; wi * 64 < cols; ++wi) {
+            uint64_t active = rows[(size_t)t * words + wi];
+            for (size_t j = 0; j < width; ++j)
+                active |= rows[(size_t)wr[j] * words + wi];
+            if ((wi + 1) * 64 >= cols)
+                active &= tail;
+            while (active) {
+                const int bit = __builtin_ctzll(active);
+                uint64_t p = 0;
+                for (size_t j = 0; j < width; ++j)
+                    p |= ((rows[(size_t)wr[j] * words + wi] >> bit) & 1ull) << j;
+                const uint64_t tb = (rows[(size_t)t * words + wi] >> bit) & 1ull;
+                keyed.push_back((p << 1) | tb);
+                ++total;
+                active &= active - 1;
+            }
+        }
+        std::sort(keyed.begin(), keyed.end());
+        WinTarget& T = tg[i];
+        T.pat_off = (int64_t)all_p.size();
+        for (size_t a = 0; a < keyed.size();) {
+            const uint64_t p = keyed[a] >> 1;
+            int32_t m = 0;
+            size_t b = a;
+            for (; b < keyed.size() && (keyed[b] >> 1) == p; ++b)
+                m += (keyed[b] & 1) ? -1 : 1;
+            if (m != 0) {
+                all_p.push_back(p);
+                all_m.push_back(m);
+            }
+            a = b;
+        }
+        T.n_pat = (int32_t)(all_p.size() - T.pat_off);
+        T.bits = (int32_t)width;
+        T.total = total;
+        T.L = (int32_t)std::min<size_t>(width, kWinMaxL);
+        T.first_block = blocks;
+        blocks += int64_t(1) << (width - T.L);
+    }
+    if (n_targets == 0)
+        return;
+    WinWorkspace& ws = win_ws();
+    WinTarget* d_t = ws.tgts.get<WinTarget>(n_targets);
+    uint64_t* d_p = ws.pats.get<uint64_t>(std::max<size_t>(all_p.size(), 1));
+    int32_t* d_m = ws.mults.get<int32_t>(std::max<size_t>(all_m.size(), 1));
+    unsigned long long* d_k = ws.keys.get<unsigned long long>(n_targets);
+    unsigned long long* d_c = ws.counter.get<unsigned long long>(1);
+    BMM_HIP(hipMemcpy(d_t, tg.data(), sizeof(WinTarget) * n_targets, hipMemcpyHostToDevice));
+    if (!all_p.empty()) {
+        BMM_HIP(hipMemcpy(d_p, all_p.data(), all_p.size() * 8, hipMemcpyHostToDevice));
+        BMM_HIP(hipMemcpy(d_m, all_m.data(), all_m.size() * 4, hipMemcpyHostToDevice));
+    }
+    fill_u64(reinterpret_cast<uint64_t*>(d_k), n_targets, ~0ull, nullptr);
+    fill_u64(reinterpret_cast<uint64_t*>(d_c), 1, 0ull, nullptr);
+    const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(blocks, default_persistent_wgs()));
+    const int chunk = blocks > (int64_t)grid * 64 ? 16 : 1;
+    {
+        ScopedKernelTimer timer(nullptr);
+        hipLaunchKernelGGL(win_min_kernel, dim3(grid), dim3(256), 0, nullptr, d_t,
+                           (int)n_targets, blocks, d_p, d_m, chunk, d_c, d_k);
+    }
+    BMM_HIP(hipGetLastError());
+    std::vector<unsigned long long> keys(n_targets);
+    BMM_HIP(hipMemcpy(keys.data(), d_k, n_targets * 8, hipMemcpyDeviceToHost));
+    const uint64_t mmask = (1ull << width) - 1;
+    for (size_t i = 0; i < n_targets; ++i) {
+        out_w[i] = (uint32_t)(keys[i] >> width);
+        out_mask[i] = keys[i] & mmask;
+    }
+}
+
+} // namespace bmmhip

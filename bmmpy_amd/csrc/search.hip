@@ -117,6 +117,7 @@ template <int L, int CAP>
 void launch_fwht_topk(const uint64_t* pat, const int32_t* mult, int n_pat, int t, int32_t total,
                       uint64_t n_blocks, int K, unsigned long long* counter, uint64_t* lists,
                       int grid, hipStream_t s) {
+    ScopedKernelTimer timer(s);
     hipLaunchKernelGGL((fwht_topk_kernel<L, CAP>), dim3(grid), dim3(256), 0, s, pat, mult, n_pat,
                        t, total, n_blocks, K, counter, lists);
     BMM_HIP(hipGetLastError());

@@ -62,6 +62,12 @@ const char *bmm_hip_last_error(void);
 /* Library version string. */
 const char *bmm_hip_version(void);
 
+/* HIP-event timing of the dominant kernel launched by each entry point
+ * (recorded on the launch stream).  timing_end synchronises on the recorded
+ * events and returns the summed kernel time and the number of launches. */
+int bmm_hip_timing_begin(void);
+int bmm_hip_timing_end(double *total_ms, size_t *launches);
+
 /* Device discovery for RuntimeFeatures (src/stub.cpp:17-43). device_count is 0
  * (and BMM_OK returned) when no device is visible. */
 int bmm_hip_device_info(bmm_device_info *out);
@@ -115,6 +121,49 @@ int bmm_hip_coset_min(const uint64_t *row_words, size_t n_rows, size_t words_per
 int bmm_hip_coset_min_batch(const uint64_t *row_words, const int64_t *word_offset,
                             const int32_t *rows, const int32_t *cols, size_t n_mat,
                             const int64_t *out_offset, uint32_t *out_weight, uint64_t *out_index);
+
+/* Device-resident form: d_row_words on the device (offsets/sizes on the
+ * host), outputs d_out_weight / d_out_index on the device, enqueued on
+ * `stream`.  persistent_wgs = 0 selects the launch width automatically. */
+int bmm_hip_coset_min_batch_dev(const uint64_t *d_row_words, const int64_t *word_offset,
+                                const int32_t *rows, const int32_t *cols, size_t n_mat,
+                                const int64_t *out_offset, uint32_t *d_out_weight,
+                                uint64_t *d_out_index, size_t persistent_wgs, void *stream);
+
+/* Reusable plan for repeated batched runs on device-resident data: the
+ * descriptors and work queue are built once (optionally autotuning the launch
+ * shape: workgroups, queue chunk, block size, timed on this batch), and
+ * run() enqueues the whole sequence, as one captured hipGraph when use_graph
+ * is set.  Pointers given at creation must stay valid until destroy. */
+typedef struct bmm_batch_info {
+    size_t n_matrices;
+    size_t n_slots;
+    size_t total_blocks;
+    int persistent_wgs;
+    int chunk;
+    int max_block_bits;
+    double subset_evals;        /* sum over matrices of rows * 2^(rows-1) */
+    double transform_elements;  /* sum over matrices of 2^rows */
+} bmm_batch_info;
+int bmm_hip_batch_plan_create(const uint64_t *d_row_words, const int64_t *word_offset,
+                              const int32_t *rows, const int32_t *cols, size_t n_mat,
+                              const int64_t *out_offset, uint32_t *d_out_weight,
+                              uint64_t *d_out_index, size_t persistent_wgs, int autotune,
+                              void **plan);
+int bmm_hip_batch_plan_run(void *plan, int use_graph, void *stream);
+int bmm_hip_batch_plan_info(const void *plan, bmm_batch_info *out);
+int bmm_hip_batch_plan_destroy(void *plan);
+
+/* ---- single-target coset minimum over a wide window (planted MITM
+ * workload; the split-transform enumeration of src/search/mitm_fwht_search.cpp:
+ * 164-265 with the target bit forced).  For each i < n_targets: window =
+ * window_rows[i*width .. i*width+width) (rows other than targets[i]),
+ * out_weight[i] = min over subsets S of the window of wt(row_t ^ XOR_S),
+ * out_mask[i] = the lowest such S (bit j = window row j).  width <= 47. */
+int bmm_hip_window_coset_min(const uint64_t *row_words, size_t n_rows, size_t words_per_row,
+                             size_t cols, const int32_t *window_rows, size_t width,
+                             const int32_t *targets, size_t n_targets, uint32_t *out_weight,
+                             uint64_t *out_mask);
 
 /* ---- Gray-code coset sweep (config: pivot block of rows [0, block_rows)
  * minus the target).  For each target t in targets[]: block = rows

@@ -7,6 +7,15 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
+# torch bundles its own HIP runtime under the same soname as ROCm's.  Loading
+# torch first makes libbmmpy_hip.so bind to that one runtime; loading our
+# library first would put two HIP runtimes in the process and torch would then
+# see no device.
+try:
+    import torch  # noqa: F401
+except ImportError:  # pragma: no cover
+    torch = None
+
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs a HIP device (MI355X / gfx950)")

@@ -106,3 +106,45 @@ def test_coset_batch_matches_per_matrix(lib, ora):
         assert np.array_equal(idx[o:o + rows[m]], eidx), m
         checked += 1
     assert checked > 100
+
+
+def test_batch_plan_graph_and_autotune(lib, ora):
+    """Plan with autotuned launch shape, replayed as a captured hipGraph three
+    times; every replay equals the one-shot result and the oracle."""
+    import ctypes as C
+    import torch
+    _say("batch plan graph")
+    rows, cols, dens, words, offs = lib.gen_batch(0x5EED0004, 96)
+    w1, i1 = lib.coset_min_batch(words, offs, rows, cols)
+    out_off = np.concatenate([[0], np.cumsum(rows)[:-1]]).astype(np.int64)
+    n = int(rows.sum())
+    dev = torch.device("cuda:0")
+    d_words = torch.from_numpy(words.view(np.int64).copy()).to(dev)
+    d_w = torch.empty(n, dtype=torch.int32, device=dev)
+    d_i = torch.empty(n, dtype=torch.int64, device=dev)
+    plan = C.c_void_p()
+    lib.call("bmm_hip_batch_plan_create", d_words.data_ptr(), offs, rows, cols, rows.size,
+             out_off, d_w.data_ptr(), d_i.data_ptr(), 0, 1, C.byref(plan))
+    try:
+        info = lib.BatchInfo()
+        lib.call("bmm_hip_batch_plan_info", plan, C.byref(info))
+        assert info.n_matrices == 96 and info.n_slots == n
+        assert info.subset_evals == float(sum(int(r) * 2 ** (int(r) - 1) for r in rows))
+        for rep in range(3):
+            d_w.fill_(-1)
+            torch.cuda.synchronize()
+            lib.call("bmm_hip_batch_plan_run", plan, 1, None)
+            torch.cuda.synchronize()
+            assert np.array_equal(d_w.cpu().numpy().view(np.uint32), w1), rep
+            assert np.array_equal(d_i.cpu().numpy().view(np.uint64), i1), rep
+    finally:
+        lib.call("bmm_hip_batch_plan_destroy", plan)
+    for m in range(0, 96, 7):
+        if rows[m] > 18:
+            continue
+        W = cols[m] // 64
+        mat = words[offs[m]: offs[m] + rows[m] * W].reshape(rows[m], W)
+        ew, ei = ora.coset_min(mat, int(cols[m]))
+        o = out_off[m]
+        assert np.array_equal(w1[o:o + rows[m]], ew)
+        assert np.array_equal(i1[o:o + rows[m]], ei)

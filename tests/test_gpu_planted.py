@@ -1,0 +1,68 @@
+"""GPU parity of the wide-window coset minimum (planted meet-in-the-middle
+workload: 48 x 4096 Bernoulli(0.02) mixed by a random invertible 48 x 48
+GF(2) matrix), against the oracle on reduced windows and against closed-form
+properties on wider ones."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+SEED_C3 = 0x5EED0003
+
+
+def _say(*a):
+    print(*a, flush=True)
+
+
+def _oracle_window(ora, A, cols, target, window):
+    sub = np.concatenate([A[list(window)], A[target:target + 1]])
+    w, idx = ora.coset_min(sub, cols)
+    return int(w[-1]), int(idx[-1])
+
+
+@pytest.mark.parametrize("width", [1, 5, 13, 14, 18, 21])
+def test_window_coset_vs_oracle(lib, ora, width):
+    _say("window coset", width)
+    S, A, M = lib.gen_planted(SEED_C3, 48, 4096, 0.02)
+    targets = [0, 17, 47]
+    windows = [[r for r in range(48) if r != t][:width] for t in targets]
+    w, m = lib.window_coset_min(A, 4096, targets, windows)
+    for i, t in enumerate(targets):
+        assert (int(w[i]), int(m[i])) == _oracle_window(ora, A, 4096, t, windows[i]), t
+
+
+def test_window_coset_unordered_and_narrow(lib, ora):
+    _say("window coset misc")
+    m = lib.gen_bernoulli(77, 30, 300, 0.3)
+    targets = [3, 29]
+    windows = [[20, 1, 7, 0, 12, 25, 9, 14, 2, 28, 22, 5], [0, 3, 6, 9, 12, 15, 18, 21, 24, 27, 1, 2]]
+    w, mk = lib.window_coset_min(m, 300, targets, windows)
+    for i, t in enumerate(targets):
+        assert (int(w[i]), int(mk[i])) == _oracle_window(ora, m, 300, t, windows[i])
+
+
+def test_window_coset_duplicate_rows_reach_zero(lib):
+    m = lib.gen_bernoulli(5, 20, 500, 0.3)
+    m[7] = m[3] ^ m[11] ^ m[15]
+    w, mk = lib.window_coset_min(m, 500, [7], [[r for r in range(20) if r != 7]])
+    assert w[0] == 0
+    win = [r for r in range(20) if r != 7]
+    assert int(mk[0]) == (1 << win.index(3)) | (1 << win.index(11)) | (1 << win.index(15))
+
+
+def test_planted_40_wide_properties(lib):
+    """Full-width windows (40 other rows) on two targets: the minimum is at most
+    the target's own weight (empty subset) and the returned mask reproduces
+    the returned weight."""
+    _say("planted 40 wide")
+    S, A, M = lib.gen_planted(SEED_C3, 48, 4096, 0.02)
+    targets = [0, 47]
+    windows = [[r for r in range(48) if r != t][:40] for t in targets]
+    w, mk = lib.window_coset_min(A, 4096, targets, windows)
+    for i, t in enumerate(targets):
+        acc = A[t].copy()
+        for j, r in enumerate(windows[i]):
+            if (int(mk[i]) >> j) & 1:
+                acc ^= A[r]
+        assert int(w[i]) == sum(bin(int(v)).count("1") for v in acc)
+        assert int(w[i]) <= sum(bin(int(v)).count("1") for v in A[t])
