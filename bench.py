@@ -136,6 +136,47 @@ def secondary(torch, _lib, dev, stream) -> dict:
     return res
 
 
+def ga_compare(budget_s: float, seed: int = 0x5EED0005) -> dict:
+    """Planted 256 x 8192 matrix (Bernoulli(0.01) sparse S, random invertible
+    256 x 256 transform M, A = M S); 64 islands x 256 individuals x 512 ops,
+    ring migration every 20 generations.  Plain and learned proposal runs get
+    the same wall-clock budget; reported: weight reached, time to reach
+    1.01 * wt(S) (null when not reached) and the trajectory."""
+    from bmmpy_amd import _lib
+
+    S, A, _ = _lib.gen_planted(seed, 256, 8192, 0.01)
+    planted = int(sum(bin(int(v)).count("1") for v in S.reshape(-1)))
+    start_w = int(sum(bin(int(v)).count("1") for v in A.reshape(-1)))
+    target = int(planted * 1.01)
+    res = {"planted_weight": planted, "start_weight": start_w, "target": target}
+    for name, learned in (("plain", 0), ("learned", 1)):
+        p = _lib.ga_default_params(learned=learned, seed=seed)
+        t0 = time.perf_counter()
+        ga = _lib.GpuGA(A, 8192, p)
+        setup = time.perf_counter() - t0
+        traj = []
+        reached_at = None
+        elapsed = 0.0
+        st = None
+        while elapsed < budget_s:
+            st = ga.run(20, target=target, budget_s=0.0)
+            elapsed += st.elapsed_s
+            traj.append([round(elapsed, 3), int(st.best_weight)])
+            if st.reached_target:
+                reached_at = elapsed
+                break
+        best, w = ga.best()
+        ga.close()
+        res[name] = {"best_weight": int(w), "ratio_to_planted": w / planted,
+                     "time_to_target_s": reached_at, "elapsed_s": elapsed,
+                     "setup_s": setup, "generations": int(st.generations),
+                     "evaluations": int(st.evaluations),
+                     "evals_per_s": st.evaluations / max(elapsed, 1e-9),
+                     "ms_per_generation": 1e3 * elapsed / max(int(st.generations), 1),
+                     "train_loss": st.train_loss, "trajectory": traj[-50:]}
+    return res
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -143,6 +184,10 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--chunk-bits", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--workload", default="gray", choices=["gray", "ga"],
+                    help="gray: the Gray coset sweep (default); ga: plain vs learned GA")
+    ap.add_argument("--ga-budget", type=float, default=60.0,
+                    help="wall-clock seconds per GA variant in --workload ga")
     ap.add_argument("--no-secondary", action="store_true",
                     help="skip the coset-minimum workloads reported beside the headline")
     args = ap.parse_args()
@@ -153,6 +198,23 @@ def main() -> int:
 
     import torch
     from bmmpy_amd import _lib
+
+    if args.workload == "ga":
+        r = ga_compare(args.ga_budget)
+        plain, learned = r["plain"], r["learned"]
+        ratio = None
+        if plain["time_to_target_s"] and learned["time_to_target_s"]:
+            ratio = learned["time_to_target_s"] / plain["time_to_target_s"]
+        print(json.dumps({
+            "metric": METRIC, "value": learned["best_weight"], "unit": "total weight reached",
+            "n_gpus": 1, "steps": learned["generations"], "warmup": 0,
+            "ms_per_step": learned["ms_per_generation"], "higher_is_better": False,
+            "scaling": "weak", "vs_baseline": None, "dtype": "u32", "data": "synthetic",
+            "config": {"workload": "ga_island_planted_256x8192", "islands": 64,
+                       "population": 256, "max_ops": 512, "migrate_every": 20,
+                       "budget_s": args.ga_budget},
+            "learned_over_plain_time_ratio": ratio, "ga": r}))
+        return 0
 
     dev = torch.device("cuda:0")
     matrix = _lib.gen_bernoulli(SEED_C2, ROWS, COLS, P)

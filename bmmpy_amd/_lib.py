@@ -51,6 +51,22 @@ class BatchInfo(C.Structure):
                 ("transform_elements", C.c_double)]
 
 
+class GaParams(C.Structure):
+    _fields_ = [("islands", C.c_int), ("population", C.c_int), ("max_ops", C.c_int),
+                ("tournament", C.c_int), ("elite", C.c_int), ("migrate_every", C.c_int),
+                ("mutations", C.c_int), ("learned", C.c_int), ("n_candidates", C.c_int),
+                ("train_steps", C.c_int), ("check_every", C.c_int), ("use_graph", C.c_int),
+                ("rows", C.c_int), ("p_append", C.c_float), ("p_replace", C.c_float),
+                ("p_delete", C.c_float), ("learning_rate", C.c_float), ("seed", C.c_uint64)]
+
+
+class GaStatus(C.Structure):
+    _fields_ = [("generations", C.c_size_t), ("ran", C.c_size_t), ("evaluations", C.c_uint64),
+                ("best_weight", C.c_uint64), ("best_island", C.c_int),
+                ("reached_target", C.c_int), ("elapsed_s", C.c_double),
+                ("train_loss", C.c_double)]
+
+
 # name -> (argtypes, restype)
 SIGNATURES = {
     "bmm_hip_last_error": ([], C.c_char_p),
@@ -83,6 +99,20 @@ SIGNATURES = {
     "bmm_hip_gray_coset_min_dev": ([_vp, _sz, _sz, _sz, _i32p, _sz, _sz, _vp, _vp, _vp],
                                    C.c_int),
     "bmm_hip_replay": ([_u64p, _sz, _sz, _u16p, _sz, C.POINTER(C.c_uint64)], C.c_int),
+    "bmm_hip_ga_default_params": ([C.POINTER(GaParams)], C.c_int),
+    "bmm_hip_ga_create": ([_u64p, _sz, _sz, C.POINTER(GaParams), C.POINTER(C.c_void_p)], C.c_int),
+    "bmm_hip_ga_run": ([_vp, _sz, C.c_uint64, C.c_double, C.POINTER(GaStatus)], C.c_int),
+    "bmm_hip_ga_best": ([_vp, _u64p, C.POINTER(C.c_uint64)], C.c_int),
+    "bmm_hip_ga_destroy": ([_vp], C.c_int),
+    "bmm_hip_mlp_num_params": ([], C.c_size_t),
+    "bmm_hip_mlp_create": ([C.c_uint64, C.POINTER(C.c_void_p)], C.c_int),
+    "bmm_hip_mlp_destroy": ([_vp], C.c_int),
+    "bmm_hip_mlp_set_params": ([_vp, _f32p], C.c_int),
+    "bmm_hip_mlp_get_params": ([_vp, _f32p], C.c_int),
+    "bmm_hip_mlp_forward": ([_vp, _f32p, _sz, _f32p], C.c_int),
+    "bmm_hip_mlp_gradients": ([_vp, _f32p, _f32p, _sz, _f32p, C.POINTER(C.c_float)], C.c_int),
+    "bmm_hip_mlp_train_step": ([_vp, _f32p, _f32p, _sz, C.c_float, C.POINTER(C.c_float)],
+                               C.c_int),
     "bmm_gen_bernoulli": ([C.c_uint64, _sz, _sz, C.c_double, _u64p], C.c_int),
     "bmm_gen_planted": ([C.c_uint64, _sz, _sz, C.c_double, _u64p, _u64p, _u64p], C.c_int),
     "bmm_gen_batch": ([C.c_uint64, _sz, _i32p, _i32p, _f64p, _vp, _sz, _i64p, C.POINTER(_sz)],
@@ -280,3 +310,95 @@ def gen_batch(seed: int, n_mat: int):
     call("bmm_gen_batch", seed, n_mat, rows, cols, dens, words.ctypes.data, cap, offs,
          C.byref(nw))
     return rows, cols, dens, words[:nw.value].copy(), offs
+
+
+# ---- GA / MLP ---------------------------------------------------------------
+def ga_default_params(**overrides) -> GaParams:
+    p = GaParams()
+    call("bmm_hip_ga_default_params", C.byref(p))
+    for k, v in overrides.items():
+        setattr(p, k, v)
+    return p
+
+
+class GpuGA:
+    """Island-model GA over row-XOR sequences (bmm_hip_ga_*)."""
+
+    def __init__(self, matrix, cols: int, params: GaParams):
+        m = _rows(matrix)
+        self.rows, self.cols = m.shape[0], cols
+        self.h = C.c_void_p()
+        call("bmm_hip_ga_create", m, m.shape[0], cols, C.byref(params), C.byref(self.h))
+
+    def run(self, generations: int, target: int = 0, budget_s: float = 0.0) -> GaStatus:
+        st = GaStatus()
+        call("bmm_hip_ga_run", self.h, generations, target, budget_s, C.byref(st))
+        return st
+
+    def best(self):
+        out = np.zeros((self.rows, self.cols // 64), np.uint64)
+        w = C.c_uint64()
+        call("bmm_hip_ga_best", self.h, out, C.byref(w))
+        return out, w.value
+
+    def close(self):
+        if self.h:
+            call("bmm_hip_ga_destroy", self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class GpuMlp:
+    """Proposal MLP with host-side features (bmm_hip_mlp_*)."""
+
+    def __init__(self, seed: int = 1):
+        self.h = C.c_void_p()
+        call("bmm_hip_mlp_create", seed, C.byref(self.h))
+        self.n = load().bmm_hip_mlp_num_params()
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        assert p.size == self.n
+        call("bmm_hip_mlp_set_params", self.h, p)
+
+    def get_params(self):
+        p = np.zeros(self.n, np.float32)
+        call("bmm_hip_mlp_get_params", self.h, p)
+        return p
+
+    def forward(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        y = np.zeros(x.shape[0], np.float32)
+        call("bmm_hip_mlp_forward", self.h, x, x.shape[0], y)
+        return y
+
+    def gradients(self, x, t):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        g = np.zeros(self.n, np.float32)
+        loss = C.c_float()
+        call("bmm_hip_mlp_gradients", self.h, x, t, x.shape[0], g, C.byref(loss))
+        return loss.value, g
+
+    def train_step(self, x, t, lr=1e-3):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        loss = C.c_float()
+        call("bmm_hip_mlp_train_step", self.h, x, t, x.shape[0], lr, C.byref(loss))
+        return loss.value
+
+    def close(self):
+        if self.h:
+            call("bmm_hip_mlp_destroy", self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

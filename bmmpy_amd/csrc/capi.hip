@@ -71,6 +71,23 @@ int default_persistent_wgs();
 void window_coset_min(const uint64_t* rows, size_t n_rows, size_t words, size_t cols,
                       const int32_t* window_rows, size_t width, const int32_t* targets,
                       size_t n_targets, uint32_t* out_w, uint64_t* out_mask);
+void replay_host(uint64_t* rows, size_t n_rows, size_t words, const uint16_t* ops, size_t n_ops,
+                 uint64_t* weight);
+struct Ga;
+Ga* ga_create(const uint64_t* matrix, size_t rows, size_t cols, const bmm_ga_params* params);
+void ga_destroy(Ga* G);
+void ga_run(Ga* G, size_t max_generations, uint64_t target, double time_budget_s,
+            bmm_ga_status* st);
+void ga_best(Ga* G, uint64_t* out, uint64_t* weight);
+struct Mlp;
+Mlp* mlp_create(uint64_t seed);
+void mlp_destroy(Mlp* M);
+void mlp_set_params(Mlp* M, const float* host);
+void mlp_get_params(Mlp* M, float* host);
+void mlp_forward_host(Mlp* M, const float* x, size_t B, float* y);
+float mlp_grads_host(Mlp* M, const float* x, const float* t, size_t B, float* grads);
+float mlp_train_host(Mlp* M, const float* x, const float* t, size_t B, float lr);
+size_t mlp_num_params();
 struct BatchPlan;
 BatchPlan* batch_plan_create(const uint64_t* d_words, const int64_t* word_offset,
                              const int32_t* rows, const int32_t* cols, size_t n_mat,
@@ -84,8 +101,6 @@ std::vector<bmm_topk_entry> fwht_topk_rows(const uint64_t* rows, size_t t, size_
 std::vector<bmm_topk_entry> mitm_topk_split(const uint64_t* q, const uint64_t* r,
                                             const int32_t* mult, size_t unique, size_t low_bits,
                                             size_t high_bits, int32_t total, size_t k);
-void replay_dev(uint64_t* d_rows, int n_rows, int words, const uint16_t* d_ops, int n_ops,
-                unsigned long long* d_wt, hipStream_t s);
 void gray_check_args(size_t n_rows, size_t words_per_row, size_t block_rows,
                      const int32_t* targets, size_t n_targets, size_t chunk_bits);
 
@@ -463,19 +478,123 @@ int bmm_hip_replay(uint64_t* rows, size_t n_rows, size_t words_per_row, const ui
         if (n_rows == 0 || words_per_row == 0)
             return;
         require_device();
-        CapiWorkspace& ws = capi_ws();
-        const size_t nw = n_rows * words_per_row;
-        uint64_t* d_rows = ws.a.get<uint64_t>(nw);
-        uint16_t* d_ops = ws.b.get<uint16_t>(std::max<size_t>(2 * n_ops, 1));
-        unsigned long long* d_wt = ws.c.get<unsigned long long>(1);
-        BMM_HIP(hipMemcpy(d_rows, rows, nw * 8, hipMemcpyHostToDevice));
-        if (n_ops)
-            BMM_HIP(hipMemcpy(d_ops, ops, 2 * n_ops * sizeof(uint16_t), hipMemcpyHostToDevice));
-        replay_dev(d_rows, (int)n_rows, (int)words_per_row, d_ops, (int)n_ops, d_wt, nullptr);
-        BMM_HIP(hipMemcpy(rows, d_rows, nw * 8, hipMemcpyDeviceToHost));
-        unsigned long long wt = 0;
-        BMM_HIP(hipMemcpy(&wt, d_wt, 8, hipMemcpyDeviceToHost));
-        *out_weight = wt;
+        replay_host(rows, n_rows, words_per_row, ops, n_ops, out_weight);
+    });
+}
+
+int bmm_hip_ga_default_params(bmm_ga_params* p) {
+    return guarded([&] {
+        if (!p)
+            fail(BMM_ERR_INVALID, "ga_default_params: null output");
+        std::memset(p, 0, sizeof(*p));
+        p->islands = 64;
+        p->population = 256;
+        p->max_ops = 512;
+        p->tournament = 3;
+        p->elite = 2;
+        p->migrate_every = 20;
+        p->mutations = 2;
+        p->learned = 0;
+        p->n_candidates = 64;
+        p->train_steps = 8;
+        p->check_every = 10;
+        p->use_graph = 1;
+        p->p_append = 0.6f;
+        p->p_replace = 0.25f;
+        p->p_delete = 0.15f;
+        p->learning_rate = 1e-3f;
+        p->seed = 0x5EED0005ull;
+    });
+}
+
+int bmm_hip_ga_create(const uint64_t* matrix, size_t rows, size_t cols,
+                      const bmm_ga_params* params, void** ga) {
+    return guarded([&] {
+        if (!matrix || !params || !ga)
+            fail(BMM_ERR_INVALID, "ga_create: null pointer");
+        *ga = nullptr;
+        if (rows < 2 || rows > 256 || cols == 0 || cols % 2048 != 0)
+            fail(BMM_ERR_INVALID, "ga_create: rows in [2, 256] and cols a multiple of 2048");
+        require_device();
+        *ga = ga_create(matrix, rows, cols, params);
+    });
+}
+
+int bmm_hip_ga_run(void* ga, size_t max_generations, uint64_t target_weight,
+                   double time_budget_s, bmm_ga_status* out) {
+    return guarded([&] {
+        if (!ga || !out)
+            fail(BMM_ERR_INVALID, "ga_run: null pointer");
+        ga_run(static_cast<Ga*>(ga), max_generations, target_weight, time_budget_s, out);
+    });
+}
+
+int bmm_hip_ga_best(void* ga, uint64_t* out_matrix, uint64_t* out_weight) {
+    return guarded([&] {
+        if (!ga || !out_matrix || !out_weight)
+            fail(BMM_ERR_INVALID, "ga_best: null pointer");
+        ga_best(static_cast<Ga*>(ga), out_matrix, out_weight);
+    });
+}
+
+int bmm_hip_ga_destroy(void* ga) {
+    return guarded([&] { ga_destroy(static_cast<Ga*>(ga)); });
+}
+
+size_t bmm_hip_mlp_num_params(void) { return mlp_num_params(); }
+
+int bmm_hip_mlp_create(uint64_t seed, void** mlp) {
+    return guarded([&] {
+        if (!mlp)
+            fail(BMM_ERR_INVALID, "mlp_create: null handle");
+        require_device();
+        *mlp = mlp_create(seed);
+    });
+}
+
+int bmm_hip_mlp_destroy(void* mlp) {
+    return guarded([&] { mlp_destroy(static_cast<Mlp*>(mlp)); });
+}
+
+int bmm_hip_mlp_set_params(void* mlp, const float* params) {
+    return guarded([&] {
+        if (!mlp || !params)
+            fail(BMM_ERR_INVALID, "mlp_set_params: null pointer");
+        mlp_set_params(static_cast<Mlp*>(mlp), params);
+    });
+}
+
+int bmm_hip_mlp_get_params(void* mlp, float* params) {
+    return guarded([&] {
+        if (!mlp || !params)
+            fail(BMM_ERR_INVALID, "mlp_get_params: null pointer");
+        mlp_get_params(static_cast<Mlp*>(mlp), params);
+    });
+}
+
+int bmm_hip_mlp_forward(void* mlp, const float* x, size_t batch, float* y) {
+    return guarded([&] {
+        if (!mlp || !x || !y || batch == 0)
+            fail(BMM_ERR_INVALID, "mlp_forward: null pointer or empty batch");
+        mlp_forward_host(static_cast<Mlp*>(mlp), x, batch, y);
+    });
+}
+
+int bmm_hip_mlp_gradients(void* mlp, const float* x, const float* target, size_t batch,
+                          float* grads, float* loss) {
+    return guarded([&] {
+        if (!mlp || !x || !target || !loss || batch == 0)
+            fail(BMM_ERR_INVALID, "mlp_gradients: null pointer or empty batch");
+        *loss = mlp_grads_host(static_cast<Mlp*>(mlp), x, target, batch, grads);
+    });
+}
+
+int bmm_hip_mlp_train_step(void* mlp, const float* x, const float* target, size_t batch,
+                           float learning_rate, float* loss) {
+    return guarded([&] {
+        if (!mlp || !x || !target || !loss || batch == 0)
+            fail(BMM_ERR_INVALID, "mlp_train_step: null pointer or empty batch");
+        *loss = mlp_train_host(static_cast<Mlp*>(mlp), x, target, batch, learning_rate);
     });
 }
 

@@ -1,0 +1,744 @@
+// Island-model genetic algorithm over row-XOR sequences, fully on the GPU.
+//
+// Reference semantics kept (src/ga): exact fitness = total Hamming weight of
+// the transformed matrix (genetic_algorithm_internal.hpp:85-103 sums candidate
+// weights; here the individual *is* the transform, as a sequence of elementary
+// row XORs, and fitness is the popcount after replaying it); elitism +
+// tournament selection (genetic_algorithm_operators.cpp:39-56); mutation as
+// row-combination moves (:100-129); islands exchanging their best every
+// interval (island.cpp:149-228), here as a ring.
+//
+// Representation: island i holds a base matrix B_i (rows x words, HBM) and a
+// population of op sequences (src, dst) relative to B_i, at most max_ops long;
+// replaying ops on B_i gives the individual's matrix.  Every migrate_every
+// generations the island folds its best sequence into B_i (so progress is not
+// capped by max_ops), and each island adopts its ring predecessor's base when
+// that one is lighter.
+//
+// Kernels per generation (captured in one hipGraph per buffer parity):
+//   ga_eval_kernel  - one wavefront per (island, individual, 2048-column
+//                     slice): the slice of B_i (rows x 64 u32) is staged in
+//                     LDS, the ops are applied lane-parallel across the 64
+//                     words, the popcount is atomically added to fitness;
+//   ga_breed_kernel - one workgroup per island, one thread per child: best
+//                     tracking, elitism, tournament, one-point crossover of the
+//                     op lists, mutation (append / replace / delete an op).
+// Proposals for new ops are uniform random pairs (plain) or drawn from a
+// per-island list of pairs ranked by the proposal MLP (learned).
+#include "common.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <memory>
+#include <vector>
+
+namespace bmmhip {
+
+struct Mlp;
+Mlp* mlp_create(uint64_t seed);
+void mlp_destroy(Mlp* M);
+float* mlp_forward_dev(Mlp* M, const __bf16* d_x, int B, hipStream_t s);
+float* mlp_backward_dev(Mlp* M, const __bf16* d_x, const float* d_t, int B, int valid,
+                        hipStream_t s);
+void mlp_adam_dev(Mlp* M, float lr, hipStream_t s);
+
+void replay_host(uint64_t* rows, size_t n_rows, size_t words, const uint16_t* ops, size_t n_ops,
+                 uint64_t* weight);
+
+__device__ inline uint64_t xs_next(uint64_t& s) {
+    s ^= s << 13;
+    s ^= s >> 7;
+    s ^= s << 17;
+    return s;
+}
+
+// ------------------------------------------------------------- eval ------
+// rows <= 256; u32 words per row = W32 (multiple of 64).
+__global__ __launch_bounds__(64) void ga_eval_kernel(const uint32_t* __restrict__ base,
+                                                     const uint16_t* __restrict__ ops,
+                                                     const int32_t* __restrict__ len, int pop,
+                                                     int rows, int W32, int max_ops,
+                                                     unsigned int* __restrict__ fit) {
+    extern __shared__ uint32_t cols[];   // rows x 64
+    const int slices = W32 / 64;
+    const int task = blockIdx.x;
+    const int slice = task % slices;
+    const int ind = (task / slices) % pop;
+    const int isl = task / (slices * pop);
+    const int lane = threadIdx.x;
+    const uint32_t* b = base + (size_t)isl * rows * W32 + (size_t)slice * 64 + lane;
+    for (int r = 0; r < rows; ++r)
+        cols[r * 64 + lane] = b[(size_t)r * W32];
+    const size_t id = (size_t)isl * pop + ind;
+    const uint16_t* o = ops + id * max_ops;
+    const int n = len[id];
+    for (int k = 0; k < n; ++k) {
+        const uint32_t op = o[k];
+        const int src = op & 0xff, dst = op >> 8;
+        if (src != dst && src < rows && dst < rows)
+            cols[dst * 64 + lane] ^= cols[src * 64 + lane];
+    }
+    uint32_t pc = 0;
+    for (int r = 0; r < rows; ++r)
+        pc += __builtin_popcount(cols[r * 64 + lane]);
+    pc = wave_sum_u32(pc);
+    if (lane == 0)
+        atomicAdd(&fit[id], pc);
+}
+
+// ------------------------------------------------------------ breed ------
+struct GaDev {
+    int islands, pop, rows, max_ops, tournament, elite, n_cand, learned;
+    float p_append, p_replace, p_delete;
+    int mutations;
+};
+
+__device__ inline uint16_t propose(const GaDev& P, uint64_t& rng, const uint16_t* cand) {
+    if (P.learned && P.n_cand > 0)
+        return cand[xs_next(rng) % (uint64_t)P.n_cand];
+    const int r = P.rows;
+    const int s = (int)(xs_next(rng) % (uint64_t)r);
+    int d = (int)(xs_next(rng) % (uint64_t)(r - 1));
+    if (d >= s)
+        ++d;
+    return (uint16_t)(s | (d << 8));
+}
+
+__global__ __launch_bounds__(1024) void ga_breed_kernel(
+    GaDev P, const unsigned int* __restrict__ fit, const uint16_t* __restrict__ ops_cur,
+    const int32_t* __restrict__ len_cur, uint16_t* __restrict__ ops_next,
+    int32_t* __restrict__ len_next, uint64_t* __restrict__ rng_state,
+    unsigned int* __restrict__ best_w, uint16_t* __restrict__ best_ops,
+    int32_t* __restrict__ best_len, const uint16_t* __restrict__ cand,
+    unsigned long long* __restrict__ evals) {
+    extern __shared__ unsigned int sf[];  // pop fitness values
+    __shared__ unsigned long long bestkey;
+    const int isl = blockIdx.x;
+    const int t = threadIdx.x;
+    const size_t base_id = (size_t)isl * P.pop;
+    if (t == 0)
+        bestkey = ~0ull;
+    __syncthreads();
+    if (t < P.pop) {
+        sf[t] = fit[base_id + t];
+        atomicMin(&bestkey, ((unsigned long long)sf[t] << 32) | (unsigned)t);
+    }
+    __syncthreads();
+    const int arg = (int)(bestkey & 0xffffffffu);
+    const unsigned int wbest = (unsigned int)(bestkey >> 32);
+    const bool improved = wbest < best_w[isl];
+    __syncthreads();
+    if (improved) {
+        const uint16_t* src = ops_cur + (base_id + arg) * P.max_ops;
+        for (int k = t; k < P.max_ops; k += blockDim.x)
+            best_ops[(size_t)isl * P.max_ops + k] = src[k];
+        if (t == 0) {
+            best_w[isl] = wbest;
+            best_len[isl] = len_cur[base_id + arg];
+        }
+    }
+    if (t == 0)
+        atomicAdd(evals, (unsigned long long)P.pop);
+    if (t >= P.pop)
+        return;
+    uint64_t rng = rng_state[base_id + t];
+    uint16_t* child = ops_next + (base_id + t) * P.max_ops;
+    int clen;
+    if (t < P.elite) {
+        // Elite: the generation's best (t == 0) or a tournament winner copy.
+        int pick = arg;
+        if (t > 0) {
+            pick = (int)(xs_next(rng) % (uint64_t)P.pop);
+            for (int k = 1; k < P.tournament; ++k) {
+                const int c = (int)(xs_next(rng) % (uint64_t)P.pop);
+                if (sf[c] < sf[pick])
+                    pick = c;
+            }
+        }
+        const uint16_t* src = ops_cur + (base_id + pick) * P.max_ops;
+        clen = len_cur[base_id + pick];
+        for (int k = 0; k < clen; ++k)
+            child[k] = src[k];
+    } else {
+        int pa = (int)(xs_next(rng) % (uint64_t)P.pop), pb = (int)(xs_next(rng) % (uint64_t)P.pop);
+        for (int k = 1; k < P.tournament; ++k) {
+            const int ca = (int)(xs_next(rng) % (uint64_t)P.pop);
+            const int cb = (int)(xs_next(rng) % (uint64_t)P.pop);
+            if (sf[ca] < sf[pa])
+                pa = ca;
+            if (sf[cb] < sf[pb])
+                pb = cb;
+        }
+        const int la = len_cur[base_id + pa], lb = len_cur[base_id + pb];
+        const int ca = (int)(xs_next(rng) % (uint64_t)(la + 1));
+        const int cb = (int)(xs_next(rng) % (uint64_t)(lb + 1));
+        const uint16_t* A = ops_cur + (base_id + pa) * P.max_ops;
+        const uint16_t* B = ops_cur + (base_id + pb) * P.max_ops;
+        clen = 0;
+        for (int k = 0; k < ca && clen < P.max_ops; ++k)
+            child[clen++] = A[k];
+        for (int k = cb; k < lb && clen < P.max_ops; ++k)
+            child[clen++] = B[k];
+        for (int m = 0; m < P.mutations; ++m) {
+            const float u = (float)(xs_next(rng) >> 40) * (1.0f / 16777216.0f);
+            if (u < P.p_append) {
+                if (clen < P.max_ops)
+                    child[clen++] = propose(P, rng, cand + (size_t)isl * P.n_cand);
+            } else if (u < P.p_append + P.p_replace) {
+                if (clen > 0)
+                    child[xs_next(rng) % (uint64_t)clen] =
+                        propose(P, rng, cand + (size_t)isl * P.n_cand);
+            } else if (u < P.p_append + P.p_replace + P.p_delete) {
+                if (clen > 0) {
+                    const int pos = (int)(xs_next(rng) % (uint64_t)clen);
+                    for (int k = pos; k + 1 < clen; ++k)
+                        child[k] = child[k + 1];
+                    --clen;
+                }
+            }
+        }
+    }
+    len_next[base_id + t] = clen;
+    rng_state[base_id + t] = rng;
+}
+
+// ------------------------------------------------------ epoch kernels ----
+// Fold each island's best sequence into its base (one thread per u64 word
+// column; the ops of one island run in order on every column).
+__global__ void ga_rebase_kernel(uint64_t* __restrict__ base, int rows, int words,
+                                 const uint16_t* __restrict__ best_ops,
+                                 const int32_t* __restrict__ best_len, int max_ops) {
+    const int isl = blockIdx.y;
+    const int w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= words)
+        return;
+    uint64_t* b = base + (size_t)isl * rows * words + w;
+    const uint16_t* o = best_ops + (size_t)isl * max_ops;
+    const int n = best_len[isl];
+    for (int k = 0; k < n; ++k) {
+        const int src = o[k] & 0xff, dst = o[k] >> 8;
+        if (src != dst && src < rows && dst < rows)
+            b[(size_t)dst * words] ^= b[(size_t)src * words];
+    }
+}
+
+// After a rebase the best sequence is empty; individual 0 of the current
+// population (the elite slot) becomes the empty sequence.
+__global__ void ga_after_rebase_kernel(int32_t* __restrict__ best_len, int32_t* __restrict__ len,
+                                       int pop, int islands) {
+    const int isl = blockIdx.x * blockDim.x + threadIdx.x;
+    if (isl < islands) {
+        best_len[isl] = 0;
+        len[(size_t)isl * pop] = 0;
+    }
+}
+
+// Ring migration: island i adopts island i-1's base when it is lighter.
+__global__ void ga_migrate_decide_kernel(const unsigned int* __restrict__ best_w, int islands,
+                                         int32_t* __restrict__ flag,
+                                         unsigned int* __restrict__ w_snap) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= islands)
+        return;
+    const int prev = (i + islands - 1) % islands;
+    flag[i] = best_w[prev] < best_w[i] ? 1 : 0;
+    w_snap[i] = best_w[prev];
+}
+
+__global__ void ga_migrate_copy_kernel(const uint64_t* __restrict__ src,
+                                       uint64_t* __restrict__ dst, size_t per_island,
+                                       const int32_t* __restrict__ flag, int islands, int shift) {
+    const int i = blockIdx.y;
+    if (!flag[i])
+        return;
+    const int from = (i + islands - shift) % islands;
+    for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < per_island;
+         k += (size_t)gridDim.x * blockDim.x)
+        dst[(size_t)i * per_island + k] = src[(size_t)from * per_island + k];
+}
+
+__global__ void ga_migrate_weights_kernel(unsigned int* __restrict__ best_w,
+                                          const unsigned int* __restrict__ w_snap,
+                                          const int32_t* __restrict__ flag, int islands) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < islands && flag[i])
+        best_w[i] = w_snap[i];
+}
+
+// Per island row: weight and 128-bucket sketch (bucket = one u64 word of the
+// 64-word-per-2 layout: bucket b covers words [b*W/128, (b+1)*W/128)); bit set
+// when the bucket holds more ones than the row's average bucket.
+__global__ void ga_sketch_kernel(const uint64_t* __restrict__ base, int rows, int words,
+                                 uint32_t* __restrict__ roww, uint64_t* __restrict__ sketch) {
+    const int row = blockIdx.x;   // global row index = isl * rows + r
+    const uint64_t* b = base + (size_t)row * words;
+    __shared__ uint32_t cnt[128];
+    __shared__ uint32_t total;
+    if (threadIdx.x == 0)
+        total = 0;
+    if (threadIdx.x < 128)
+        cnt[threadIdx.x] = 0;
+    __syncthreads();
+    const int per = words >= 128 ? words / 128 : 1;
+    for (int w = threadIdx.x; w < words; w += blockDim.x) {
+        const uint32_t c = __builtin_popcountll(b[w]);
+        atomicAdd(&cnt[min(w / per, 127)], c);
+        atomicAdd(&total, c);
+    }
+    __syncthreads();
+    if (threadIdx.x < 2) {
+        uint64_t bits = 0;
+        for (int k = 0; k < 64; ++k) {
+            const int bucket = threadIdx.x * 64 + k;
+            if ((uint64_t)cnt[bucket] * 128u > (uint64_t)total)
+                bits |= 1ull << k;
+        }
+        sketch[(size_t)row * 2 + threadIdx.x] = bits;
+    }
+    if (threadIdx.x == 0)
+        roww[row] = total;
+}
+
+// Features of pair (s -> d) on island isl: sketch(s) | sketch(d) as +-1 bf16.
+__global__ void ga_features_kernel(const uint64_t* __restrict__ sketch, int rows,
+                                   const uint16_t* __restrict__ pairs,
+                                   const int32_t* __restrict__ pair_isl, int n,
+                                   __bf16* __restrict__ x) {
+    const int i = blockIdx.x;
+    const int f = threadIdx.x;   // 0..255
+    if (i >= n) {
+        x[(size_t)i * 256 + f] = (__bf16)0.f;
+        return;
+    }
+    const int isl = pair_isl[i];
+    const int s = pairs[i] & 0xff, d = pairs[i] >> 8;
+    const int row = f < 128 ? s : d;
+    const int bit = f & 127;
+    const uint64_t w = sketch[((size_t)isl * rows + row) * 2 + (bit >> 6)];
+    x[(size_t)i * 256 + f] = (__bf16)(((w >> (bit & 63)) & 1u) ? 1.f : -1.f);
+}
+
+// Exact label of pair (s -> d): (wt(r_s ^ r_d) - wt(r_d)) / scale.
+__global__ void ga_labels_kernel(const uint64_t* __restrict__ base, int rows, int words,
+                                 const uint16_t* __restrict__ pairs,
+                                 const int32_t* __restrict__ pair_isl, int n, float scale,
+                                 float* __restrict__ t) {
+    const int i = blockIdx.x;
+    if (i >= n)
+        return;
+    const int isl = pair_isl[i];
+    const int s = pairs[i] & 0xff, d = pairs[i] >> 8;
+    const uint64_t* rs = base + ((size_t)isl * rows + s) * words;
+    const uint64_t* rd = base + ((size_t)isl * rows + d) * words;
+    int delta = 0;
+    for (int w = threadIdx.x; w < words; w += blockDim.x)
+        delta += __builtin_popcountll(rs[w] ^ rd[w]) - __builtin_popcountll(rd[w]);
+    delta = (int)wave_sum_u32((uint32_t)delta);
+    __shared__ int red[4];
+    if (threadIdx.x % kWave == 0)
+        red[threadIdx.x / kWave] = delta;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int sum = 0;
+        for (int k = 0; k < (int)(blockDim.x + kWave - 1) / kWave; ++k)
+            sum += red[k];
+        t[i] = (float)sum / scale;
+    }
+}
+
+__global__ void ga_random_pairs_kernel(uint64_t* __restrict__ rng, int rows, int per_island,
+                                       int islands, uint16_t* __restrict__ pairs,
+                                       int32_t* __restrict__ pair_isl) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= per_island * islands)
+        return;
+    uint64_t s = rng[i];
+    const int a = (int)(xs_next(s) % (uint64_t)rows);
+    int b = (int)(xs_next(s) % (uint64_t)(rows - 1));
+    if (b >= a)
+        ++b;
+    rng[i] = s;
+    pairs[i] = (uint16_t)(a | (b << 8));
+    pair_isl[i] = i / per_island;
+}
+
+// Keeps the n_keep lowest-scored of per_island scored pairs per island
+// (bitonic sort of (score, pair) in LDS; per_island a power of two <= 2048).
+__global__ void ga_select_kernel(const float* __restrict__ score,
+                                 const uint16_t* __restrict__ pairs, int per_island, int n_keep,
+                                 uint16_t* __restrict__ cand) {
+    __shared__ unsigned long long key[2048];
+    const int isl = blockIdx.x;
+    for (int i = threadIdx.x; i < per_island; i += blockDim.x) {
+        float sc = score[(size_t)isl * per_island + i];
+        uint32_t u = __float_as_uint(sc);
+        u = (u & 0x80000000u) ? ~u : (u | 0x80000000u);   // order-preserving
+        key[i] = ((unsigned long long)u << 32) | pairs[(size_t)isl * per_island + i];
+    }
+    __syncthreads();
+    for (int k = 2; k <= per_island; k <<= 1)
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int i = threadIdx.x; i < per_island; i += blockDim.x) {
+                const int ixj = i ^ j;
+                if (ixj > i) {
+                    const unsigned long long a = key[i], b = key[ixj];
+                    if ((a > b) == ((i & k) == 0)) {
+                        key[i] = b;
+                        key[ixj] = a;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    for (int i = threadIdx.x; i < n_keep; i += blockDim.x)
+        cand[(size_t)isl * n_keep + i] = (uint16_t)(key[i] & 0xffffu);
+}
+
+// --------------------------------------------------------------- host ----
+struct Ga {
+    bmm_ga_params P{};
+    int W32 = 0, words = 0;
+    DevBuf base, base_tmp, ops[2], len[2], fit, rng, best_w, best_ops, best_len, cand, flag,
+        w_snap, roww, sketch, evals;
+    // proposal model data
+    DevBuf pairs, pair_isl, feats, labels, prng;
+    Mlp* mlp = nullptr;
+    int cur = 0;
+    long generation = 0;
+    hipStream_t stream = nullptr;
+    hipGraphExec_t gen_exec[2] = {nullptr, nullptr};
+    hipGraph_t gen_graph[2] = {nullptr, nullptr};
+    double train_loss = 0.0;
+    ~Ga() {
+        for (int i = 0; i < 2; ++i) {
+            if (gen_exec[i])
+                (void)hipGraphExecDestroy(gen_exec[i]);
+            if (gen_graph[i])
+                (void)hipGraphDestroy(gen_graph[i]);
+        }
+        if (mlp)
+            mlp_destroy(mlp);
+        if (stream)
+            (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+GaDev dev_params(const Ga& G) {
+    GaDev d;
+    d.islands = G.P.islands;
+    d.pop = G.P.population;
+    d.rows = G.P.rows;
+    d.max_ops = G.P.max_ops;
+    d.tournament = G.P.tournament;
+    d.elite = G.P.elite;
+    d.n_cand = G.P.learned ? G.P.n_candidates : 0;
+    d.learned = G.P.learned;
+    d.p_append = G.P.p_append;
+    d.p_replace = G.P.p_replace;
+    d.p_delete = G.P.p_delete;
+    d.mutations = G.P.mutations;
+    return d;
+}
+
+void enqueue_generation(Ga& G, int parity, hipStream_t s) {
+    const int I = G.P.islands, Pp = G.P.population;
+    unsigned int* fit = static_cast<unsigned int*>(G.fit.ptr);
+    fill_u32(fit, (size_t)I * Pp, 0u, s);
+    const int slices = G.W32 / 64;
+    const size_t lds = (size_t)G.P.rows * 64 * 4;
+    hipLaunchKernelGGL(ga_eval_kernel, dim3((unsigned)(I * Pp * slices)), dim3(64), lds, s,
+                       static_cast<const uint32_t*>(G.base.ptr),
+                       static_cast<const uint16_t*>(G.ops[parity].ptr),
+                       static_cast<const int32_t*>(G.len[parity].ptr), Pp, G.P.rows, G.W32,
+                       G.P.max_ops, fit);
+    BMM_HIP(hipGetLastError());
+    hipLaunchKernelGGL(ga_breed_kernel, dim3(I), dim3(std::max(64, (Pp + 63) / 64 * 64)),
+                       (size_t)Pp * 4, s, dev_params(G), fit,
+                       static_cast<const uint16_t*>(G.ops[parity].ptr),
+                       static_cast<const int32_t*>(G.len[parity].ptr),
+                       static_cast<uint16_t*>(G.ops[1 - parity].ptr),
+                       static_cast<int32_t*>(G.len[1 - parity].ptr),
+                       static_cast<uint64_t*>(G.rng.ptr), static_cast<unsigned int*>(G.best_w.ptr),
+                       static_cast<uint16_t*>(G.best_ops.ptr),
+                       static_cast<int32_t*>(G.best_len.ptr),
+                       static_cast<const uint16_t*>(G.cand.ptr),
+                       static_cast<unsigned long long*>(G.evals.ptr));
+    BMM_HIP(hipGetLastError());
+}
+
+void run_generation(Ga& G) {
+    const int parity = G.cur;
+    if (G.P.use_graph) {
+        if (!G.gen_exec[parity]) {
+            hipStream_t cap;
+            BMM_HIP(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+            BMM_HIP(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
+            enqueue_generation(G, parity, cap);
+            BMM_HIP(hipStreamEndCapture(cap, &G.gen_graph[parity]));
+            BMM_HIP(hipGraphInstantiate(&G.gen_exec[parity], G.gen_graph[parity], nullptr,
+                                        nullptr, 0));
+            (void)hipStreamDestroy(cap);
+        }
+        BMM_HIP(hipGraphLaunch(G.gen_exec[parity], G.stream));
+    } else {
+        enqueue_generation(G, parity, G.stream);
+    }
+    G.cur = 1 - parity;
+    ++G.generation;
+}
+
+void refresh_sketches(Ga& G) {
+    const int I = G.P.islands;
+    hipLaunchKernelGGL(ga_sketch_kernel, dim3((unsigned)(I * G.P.rows)), dim3(128), 0, G.stream,
+                       static_cast<const uint64_t*>(G.base.ptr), G.P.rows, G.words,
+                       static_cast<uint32_t*>(G.roww.ptr), static_cast<uint64_t*>(G.sketch.ptr));
+    BMM_HIP(hipGetLastError());
+}
+
+// Samples n pairs per island into G.pairs / G.pair_isl.
+void sample_pairs(Ga& G, int per_island) {
+    const int n = per_island * G.P.islands;
+    hipLaunchKernelGGL(ga_random_pairs_kernel, dim3((n + 255) / 256), dim3(256), 0, G.stream,
+                       static_cast<uint64_t*>(G.prng.ptr), G.P.rows, per_island, G.P.islands,
+                       static_cast<uint16_t*>(G.pairs.ptr), static_cast<int32_t*>(G.pair_isl.ptr));
+    BMM_HIP(hipGetLastError());
+}
+
+void make_features(Ga& G, int n_padded, int n) {
+    hipLaunchKernelGGL(ga_features_kernel, dim3((unsigned)n_padded), dim3(256), 0, G.stream,
+                       static_cast<const uint64_t*>(G.sketch.ptr), G.P.rows,
+                       static_cast<const uint16_t*>(G.pairs.ptr),
+                       static_cast<const int32_t*>(G.pair_isl.ptr), n,
+                       static_cast<__bf16*>(G.feats.ptr));
+    BMM_HIP(hipGetLastError());
+}
+
+constexpr int kScorePerIsland = 1024;
+constexpr int kTrainPerIsland = 64;
+constexpr float kLabelScale = 64.f;
+
+// Learned proposal: train on exact deltas of fresh random pairs, then rank a
+// fresh pool per island and keep the lowest predicted deltas.
+void update_proposals(Ga& G) {
+    const int I = G.P.islands;
+    refresh_sketches(G);
+    if (!G.P.learned)
+        return;
+    // training batch
+    const int nt = kTrainPerIsland * I;
+    const int ntp = (nt + 31) / 32 * 32;
+    sample_pairs(G, kTrainPerIsland);
+    make_features(G, ntp, nt);
+    hipLaunchKernelGGL(ga_labels_kernel, dim3((unsigned)nt), dim3(128), 0, G.stream,
+                       static_cast<const uint64_t*>(G.base.ptr), G.P.rows, G.words,
+                       static_cast<const uint16_t*>(G.pairs.ptr),
+                       static_cast<const int32_t*>(G.pair_isl.ptr), nt, kLabelScale,
+                       static_cast<float*>(G.labels.ptr));
+    BMM_HIP(hipGetLastError());
+    float* loss = nullptr;
+    for (int it = 0; it < G.P.train_steps; ++it) {
+        loss = mlp_backward_dev(G.mlp, static_cast<const __bf16*>(G.feats.ptr),
+                                static_cast<const float*>(G.labels.ptr), ntp, nt, G.stream);
+        mlp_adam_dev(G.mlp, G.P.learning_rate, G.stream);
+    }
+    if (loss) {
+        float l = 0.f;
+        BMM_HIP(hipMemcpyAsync(&l, loss, 4, hipMemcpyDeviceToHost, G.stream));
+        BMM_HIP(hipStreamSynchronize(G.stream));
+        G.train_loss = l;
+    }
+    // score a pool per island
+    const int ns = kScorePerIsland * I;
+    sample_pairs(G, kScorePerIsland);
+    make_features(G, ns, ns);
+    float* y = mlp_forward_dev(G.mlp, static_cast<const __bf16*>(G.feats.ptr), ns, G.stream);
+    hipLaunchKernelGGL(ga_select_kernel, dim3(I), dim3(256), 0, G.stream, y,
+                       static_cast<const uint16_t*>(G.pairs.ptr), kScorePerIsland,
+                       G.P.n_candidates, static_cast<uint16_t*>(G.cand.ptr));
+    BMM_HIP(hipGetLastError());
+}
+
+void epoch(Ga& G) {
+    const int I = G.P.islands;
+    const int parity = G.cur;   // population about to be evaluated
+    hipLaunchKernelGGL(ga_rebase_kernel, dim3((G.words + 63) / 64, I), dim3(64), 0, G.stream,
+                       static_cast<uint64_t*>(G.base.ptr), G.P.rows, G.words,
+                       static_cast<const uint16_t*>(G.best_ops.ptr),
+                       static_cast<const int32_t*>(G.best_len.ptr), G.P.max_ops);
+    hipLaunchKernelGGL(ga_after_rebase_kernel, dim3((I + 63) / 64), dim3(64), 0, G.stream,
+                       static_cast<int32_t*>(G.best_len.ptr),
+                       static_cast<int32_t*>(G.len[parity].ptr), G.P.population, I);
+    const size_t per = (size_t)G.P.rows * G.words;
+    hipLaunchKernelGGL(ga_migrate_decide_kernel, dim3((I + 63) / 64), dim3(64), 0, G.stream,
+                       static_cast<const unsigned int*>(G.best_w.ptr), I,
+                       static_cast<int32_t*>(G.flag.ptr), static_cast<unsigned int*>(G.w_snap.ptr));
+    hipLaunchKernelGGL(ga_migrate_copy_kernel, dim3(64, I), dim3(256), 0, G.stream,
+                       static_cast<const uint64_t*>(G.base.ptr),
+                       static_cast<uint64_t*>(G.base_tmp.ptr), per,
+                       static_cast<const int32_t*>(G.flag.ptr), I, 1);
+    hipLaunchKernelGGL(ga_migrate_copy_kernel, dim3(64, I), dim3(256), 0, G.stream,
+                       static_cast<const uint64_t*>(G.base_tmp.ptr),
+                       static_cast<uint64_t*>(G.base.ptr), per,
+                       static_cast<const int32_t*>(G.flag.ptr), I, 0);
+    hipLaunchKernelGGL(ga_migrate_weights_kernel, dim3((I + 63) / 64), dim3(64), 0, G.stream,
+                       static_cast<unsigned int*>(G.best_w.ptr),
+                       static_cast<const unsigned int*>(G.w_snap.ptr),
+                       static_cast<const int32_t*>(G.flag.ptr), I);
+    BMM_HIP(hipGetLastError());
+    update_proposals(G);
+}
+
+} // namespace
+
+Ga* ga_create(const uint64_t* matrix, size_t rows, size_t cols, const bmm_ga_params* params) {
+    auto G = std::make_unique<Ga>();
+    G->P = *params;
+    bmm_ga_params& P = G->P;
+    if (rows < 2 || rows > 256)
+        fail(BMM_ERR_INVALID, "ga: rows must be in [2, 256]");
+    if (cols == 0 || cols % 2048 != 0)
+        fail(BMM_ERR_INVALID, "ga: cols must be a positive multiple of 2048");
+    if (P.islands < 1 || P.population < 2 || P.population > 1024 || P.max_ops < 1 ||
+        P.max_ops > 4096 || P.tournament < 1 || P.elite < 0 || P.elite > P.population ||
+        P.migrate_every < 1 || P.mutations < 0)
+        fail(BMM_ERR_INVALID, "ga: parameter out of range");
+    if (P.learned && (P.n_candidates < 1 || P.n_candidates > kScorePerIsland))
+        fail(BMM_ERR_INVALID, "ga: n_candidates must be in [1, 1024] for the learned proposal");
+    P.rows = (int)rows;
+    G->words = (int)(cols / 64);
+    G->W32 = (int)(cols / 32);
+    const int I = P.islands, Pp = P.population;
+    const size_t per = rows * (size_t)G->words;
+    BMM_HIP(hipStreamCreateWithFlags(&G->stream, hipStreamNonBlocking));
+    uint64_t* base = G->base.get<uint64_t>(per * I);
+    G->base_tmp.get<uint64_t>(per * I);
+    for (int i = 0; i < I; ++i)
+        BMM_HIP(hipMemcpy(base + per * i, matrix, per * 8, hipMemcpyHostToDevice));
+    for (int b = 0; b < 2; ++b) {
+        G->ops[b].get<uint16_t>((size_t)I * Pp * P.max_ops);
+        G->len[b].get<int32_t>((size_t)I * Pp);
+        BMM_HIP(hipMemset(G->len[b].ptr, 0, (size_t)I * Pp * 4));
+    }
+    G->fit.get<unsigned int>((size_t)I * Pp);
+    G->best_ops.get<uint16_t>((size_t)I * P.max_ops);
+    int32_t* bl = G->best_len.get<int32_t>(I);
+    BMM_HIP(hipMemset(bl, 0, (size_t)I * 4));
+    G->flag.get<int32_t>(I);
+    G->w_snap.get<unsigned int>(I);
+    G->roww.get<uint32_t>((size_t)I * rows);
+    G->sketch.get<uint64_t>((size_t)I * rows * 2);
+    G->cand.get<uint16_t>((size_t)I * std::max(1, P.n_candidates));
+    unsigned long long* ev = G->evals.get<unsigned long long>(1);
+    BMM_HIP(hipMemset(ev, 0, 8));
+    // base weight as the initial best
+    uint64_t w0 = 0;
+    for (size_t i = 0; i < per; ++i)
+        w0 += (uint64_t)__builtin_popcountll(matrix[i]);
+    if (w0 >= 0xffffffffull)
+        fail(BMM_ERR_OVERFLOW, "ga: matrix weight exceeds 32 bits");
+    std::vector<unsigned int> bw(I, (unsigned int)w0);
+    BMM_HIP(hipMemcpy(G->best_w.get<unsigned int>(I), bw.data(), (size_t)I * 4,
+                      hipMemcpyHostToDevice));
+    // RNG streams: one per individual, one per proposal slot.
+    const int n_pairs = std::max(kScorePerIsland, kTrainPerIsland) * I;
+    std::vector<uint64_t> seeds((size_t)I * Pp + n_pairs);
+    uint64_t s = P.seed == 0 ? 0x9E3779B97F4A7C15ull : P.seed;
+    for (auto& x : seeds) {
+        s ^= s << 13;
+        s ^= s >> 7;
+        s ^= s << 17;
+        x = s | 1ull;
+    }
+    BMM_HIP(hipMemcpy(G->rng.get<uint64_t>((size_t)I * Pp), seeds.data(), (size_t)I * Pp * 8,
+                      hipMemcpyHostToDevice));
+    BMM_HIP(hipMemcpy(G->prng.get<uint64_t>(n_pairs), seeds.data() + (size_t)I * Pp,
+                      (size_t)n_pairs * 8, hipMemcpyHostToDevice));
+    G->pairs.get<uint16_t>(n_pairs);
+    G->pair_isl.get<int32_t>(n_pairs);
+    G->feats.get<__bf16>((size_t)n_pairs * 256 + 32 * 256);
+    G->labels.get<float>(n_pairs + 32);
+    if (P.learned)
+        G->mlp = mlp_create(P.seed ^ 0xA5A5A5A5ull);
+    update_proposals(*G);
+    BMM_HIP(hipStreamSynchronize(G->stream));
+    return G.release();
+}
+
+void ga_destroy(Ga* G) { delete G; }
+
+void ga_run(Ga* G, size_t max_generations, uint64_t target, double time_budget_s,
+            bmm_ga_status* st) {
+    const auto t0 = std::chrono::steady_clock::now();
+    const int I = G->P.islands;
+    std::vector<unsigned int> bw(I);
+    size_t done = 0;
+    bool reached = false;
+    while (done < max_generations) {
+        run_generation(*G);
+        ++done;
+        if (G->generation % G->P.migrate_every == 0)
+            epoch(*G);
+        if (done % (size_t)G->P.check_every == 0 || done == max_generations) {
+            BMM_HIP(hipMemcpyAsync(bw.data(), G->best_w.ptr, (size_t)I * 4,
+                                   hipMemcpyDeviceToHost, G->stream));
+            BMM_HIP(hipStreamSynchronize(G->stream));
+            const unsigned int b = *std::min_element(bw.begin(), bw.end());
+            const double el =
+                std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            if (b <= target) {
+                reached = true;
+                break;
+            }
+            if (time_budget_s > 0 && el >= time_budget_s)
+                break;
+        }
+    }
+    BMM_HIP(hipMemcpyAsync(bw.data(), G->best_w.ptr, (size_t)I * 4, hipMemcpyDeviceToHost,
+                           G->stream));
+    unsigned long long ev = 0;
+    BMM_HIP(hipMemcpyAsync(&ev, G->evals.ptr, 8, hipMemcpyDeviceToHost, G->stream));
+    BMM_HIP(hipStreamSynchronize(G->stream));
+    st->generations = (size_t)G->generation;
+    st->ran = done;
+    st->evaluations = ev;
+    st->best_weight = *std::min_element(bw.begin(), bw.end());
+    st->best_island = (int)(std::min_element(bw.begin(), bw.end()) - bw.begin());
+    st->reached_target = reached ? 1 : 0;
+    st->elapsed_s =
+        std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    st->train_loss = G->train_loss;
+}
+
+// Best matrix: the best island's base with its best sequence replayed.
+void ga_best(Ga* G, uint64_t* out, uint64_t* weight) {
+    bmm_ga_status st{};
+    const int I = G->P.islands;
+    std::vector<unsigned int> bw(I);
+    BMM_HIP(hipMemcpy(bw.data(), G->best_w.ptr, (size_t)I * 4, hipMemcpyDeviceToHost));
+    const int isl = (int)(std::min_element(bw.begin(), bw.end()) - bw.begin());
+    (void)st;
+    const size_t per = (size_t)G->P.rows * G->words;
+    std::vector<uint64_t> m(per);
+    BMM_HIP(hipMemcpy(m.data(), static_cast<uint64_t*>(G->base.ptr) + per * isl, per * 8,
+                      hipMemcpyDeviceToHost));
+    std::vector<int32_t> bl(I);
+    BMM_HIP(hipMemcpy(bl.data(), G->best_len.ptr, (size_t)I * 4, hipMemcpyDeviceToHost));
+    std::vector<uint16_t> ops((size_t)G->P.max_ops);
+    BMM_HIP(hipMemcpy(ops.data(), static_cast<uint16_t*>(G->best_ops.ptr) + (size_t)isl * G->P.max_ops,
+                      ops.size() * 2, hipMemcpyDeviceToHost));
+    // Replay on the GPU through the same replay path as the appliers.
+    std::vector<uint16_t> pairs;
+    for (int k = 0; k < bl[isl]; ++k) {
+        pairs.push_back((uint16_t)(ops[k] & 0xff));
+        pairs.push_back((uint16_t)(ops[k] >> 8));
+    }
+    uint64_t w = 0;
+    replay_host(m.data(), (size_t)G->P.rows, (size_t)G->words, pairs.data(), pairs.size() / 2, &w);
+    std::memcpy(out, m.data(), per * 8);
+    *weight = w;
+}
+
+} // namespace bmmhip

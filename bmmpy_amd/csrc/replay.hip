@@ -9,6 +9,7 @@
 // wavefront.
 #include "common.hpp"
 
+#include <algorithm>
 #include <vector>
 
 namespace bmmhip {
@@ -42,6 +43,37 @@ void replay_dev(uint64_t* d_rows, int n_rows, int words, const uint16_t* d_ops, 
     hipLaunchKernelGGL(replay_columns_kernel, dim3((words + 255) / 256), dim3(256), 0, s, d_rows,
                        n_rows, words, d_ops, n_ops, d_wt);
     BMM_HIP(hipGetLastError());
+}
+
+namespace {
+struct ReplayWorkspace {
+    DevBuf rows, ops, wt;
+};
+ReplayWorkspace& replay_ws() {
+    thread_local ReplayWorkspace ws;
+    return ws;
+}
+} // namespace
+
+// Host rows in, replayed rows + weight out (ops as (source, target) pairs).
+void replay_host(uint64_t* rows, size_t n_rows, size_t words, const uint16_t* ops, size_t n_ops,
+                 uint64_t* weight) {
+    *weight = 0;
+    if (n_rows == 0 || words == 0)
+        return;
+    ReplayWorkspace& ws = replay_ws();
+    const size_t nw = n_rows * words;
+    uint64_t* d_rows = ws.rows.get<uint64_t>(nw);
+    uint16_t* d_ops = ws.ops.get<uint16_t>(std::max<size_t>(2 * n_ops, 1));
+    unsigned long long* d_wt = ws.wt.get<unsigned long long>(1);
+    BMM_HIP(hipMemcpy(d_rows, rows, nw * 8, hipMemcpyHostToDevice));
+    if (n_ops)
+        BMM_HIP(hipMemcpy(d_ops, ops, 2 * n_ops * sizeof(uint16_t), hipMemcpyHostToDevice));
+    replay_dev(d_rows, (int)n_rows, (int)words, d_ops, (int)n_ops, d_wt, nullptr);
+    BMM_HIP(hipMemcpy(rows, d_rows, nw * 8, hipMemcpyDeviceToHost));
+    unsigned long long wt = 0;
+    BMM_HIP(hipMemcpy(&wt, d_wt, 8, hipMemcpyDeviceToHost));
+    *weight = wt;
 }
 
 } // namespace bmmhip

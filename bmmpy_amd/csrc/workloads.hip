@@ -2,9 +2,13 @@
 //
 // Generator: XorShift64 (include/bmmpy/core/detail/xorshift64.hpp:10-40);
 // Bernoulli(p) bits are `next_unit_double() < p`, filled row-major with columns
-// inner; random invertible GF(2) matrices draw p = 0.5 bits and are rejected
-// until full rank; planted matrices are transform * sparse over GF(2)
-// (BitMatrix::mul semantics, src/core/bit_matrix_algebra.cpp:27-50).
+// inner.  Random invertible GF(2) matrices take each bit as the top bit of the
+// splitmix64 finalizer of the next XorShift64 output and are rejected until
+// full rank: XorShift64 is linear over GF(2), so bits read linearly off its
+// outputs (the top bit is what `next_unit_double() < 0.5` tests) make every
+// matrix a linear image of the 64-bit state, of rank at most about 64.
+// Planted matrices are transform * sparse over GF(2) (BitMatrix::mul
+// semantics, src/core/bit_matrix_algebra.cpp:27-50).
 #include "common.hpp"
 
 #include <cstring>
@@ -27,6 +31,25 @@ struct Xs64 {
     uint64_t index(uint64_t ub) { return ub == 0 ? 0 : next() % ub; }
     double unit() { return (double)(next() >> 11) * (1.0 / 9007199254740992.0); }
 };
+
+uint64_t mix64(uint64_t x) {
+    x ^= x >> 30;
+    x *= 0xbf58476d1ce4e5b9ull;
+    x ^= x >> 27;
+    x *= 0x94d049bb133111ebull;
+    x ^= x >> 31;
+    return x;
+}
+
+void fill_mixed_half(Xs64& g, uint64_t* rows, size_t n_rows, size_t stride, size_t cols) {
+    for (size_t r = 0; r < n_rows; ++r) {
+        uint64_t* row = rows + r * stride;
+        std::memset(row, 0, stride * 8);
+        for (size_t c = 0; c < cols; ++c)
+            if (mix64(g.next()) >> 63)
+                row[c >> 6] |= 1ull << (c & 63);
+    }
+}
 
 void fill(Xs64& g, uint64_t* rows, size_t n_rows, size_t stride, size_t cols, double p) {
     for (size_t r = 0; r < n_rows; ++r) {
@@ -84,7 +107,7 @@ int bmm_gen_planted(uint64_t seed, size_t n_rows, size_t cols, double p, uint64_
         fill(g, sparse, n_rows, w, cols, p);
         std::vector<uint64_t> m(n_rows * wn);
         do {
-            fill(g, m.data(), n_rows, wn, n_rows, 0.5);
+            fill_mixed_half(g, m.data(), n_rows, wn, n_rows);
         } while (gf2_rank(m, n_rows, wn, n_rows) != n_rows);
         std::memcpy(transform, m.data(), m.size() * 8);
         for (size_t i = 0; i < n_rows; ++i) {

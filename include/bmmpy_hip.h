@@ -190,13 +190,82 @@ int bmm_hip_gray_coset_min_dev(const uint64_t *d_rows, size_t n_rows, size_t wor
 int bmm_hip_replay(uint64_t *rows, size_t n_rows, size_t words_per_row, const uint16_t *ops,
                    size_t n_ops, uint64_t *out_weight);
 
+/* ---- src/ga on the GPU: island model over row-XOR sequences ---------------
+ * Individual = up to max_ops ops (src, dst) applied to the island's base
+ * matrix; fitness = exact total weight after replay; elitism + tournament +
+ * one-point crossover + mutation (append / replace / delete an op); every
+ * migrate_every generations each island folds its best sequence into its base
+ * and adopts its ring predecessor's base when that one is lighter.  learned = 1
+ * draws new ops from pairs ranked by the proposal MLP (trained online on exact
+ * pair deltas); learned = 0 draws uniform random pairs.  rows <= 256, cols a
+ * multiple of 2048. */
+typedef struct bmm_ga_params {
+    int islands;
+    int population;
+    int max_ops;
+    int tournament;
+    int elite;
+    int migrate_every;
+    int mutations;      /* mutation draws per child */
+    int learned;
+    int n_candidates;   /* ranked pairs kept per island (learned) */
+    int train_steps;    /* MLP steps per migration epoch (learned) */
+    int check_every;    /* generations between host checks of the best weight */
+    int use_graph;      /* replay each generation as a captured hipGraph */
+    int rows;           /* filled in by create */
+    float p_append;
+    float p_replace;
+    float p_delete;
+    float learning_rate;
+    uint64_t seed;
+} bmm_ga_params;
+
+typedef struct bmm_ga_status {
+    size_t generations;   /* total so far */
+    size_t ran;           /* in this call */
+    uint64_t evaluations; /* exact fitness evaluations so far */
+    uint64_t best_weight;
+    int best_island;
+    int reached_target;
+    double elapsed_s;
+    double train_loss;    /* last MLP training loss (learned) */
+} bmm_ga_status;
+
+int bmm_hip_ga_default_params(bmm_ga_params *out);
+int bmm_hip_ga_create(const uint64_t *matrix, size_t rows, size_t cols,
+                      const bmm_ga_params *params, void **ga);
+/* Runs until max_generations, best weight <= target_weight, or time budget
+ * (seconds, 0 = none), checking every check_every generations. */
+int bmm_hip_ga_run(void *ga, size_t max_generations, uint64_t target_weight,
+                   double time_budget_s, bmm_ga_status *out);
+/* Best matrix found (rows x cols/64 words, host) and its weight. */
+int bmm_hip_ga_best(void *ga, uint64_t *out_matrix, uint64_t *out_weight);
+int bmm_hip_ga_destroy(void *ga);
+
+/* ---- proposal MLP (256 -> 256 -> 256 -> 1, ReLU, bf16 products with fp32
+ * accumulation on MFMA, fp32 master weights, Adam).  Parameters flat in the
+ * order W1[256][256] (in x out), b1[256], W2[256][256], b2[256], w3[256],
+ * b3[1].  Features are host floats, batch x 256; loss = mean squared error. */
+size_t bmm_hip_mlp_num_params(void);
+int bmm_hip_mlp_create(uint64_t seed, void **mlp);
+int bmm_hip_mlp_destroy(void *mlp);
+int bmm_hip_mlp_set_params(void *mlp, const float *params);
+int bmm_hip_mlp_get_params(void *mlp, float *params);
+int bmm_hip_mlp_forward(void *mlp, const float *x, size_t batch, float *y);
+int bmm_hip_mlp_gradients(void *mlp, const float *x, const float *target, size_t batch,
+                          float *grads, float *loss);
+int bmm_hip_mlp_train_step(void *mlp, const float *x, const float *target, size_t batch,
+                           float learning_rate, float *loss);
+
 /* ---- seeded synthetic workloads (no reference counterpart: they define the
  * benchmark inputs).  XorShift64 stream (include/bmmpy/core/detail/
  * xorshift64.hpp:10-40); Bernoulli(p) = next_unit_double() < p, row-major. */
 int bmm_gen_bernoulli(uint64_t seed, size_t n_rows, size_t cols, double p, uint64_t *out);
 /* sparse = Bernoulli(p) n_rows x cols; transform = random invertible
- * n_rows x n_rows (p = 0.5 draws, rejected until full rank); mixed =
- * transform * sparse.  Row strides ceil(cols/64) and ceil(n_rows/64) words. */
+ * n_rows x n_rows (bit = top bit of splitmix64(next output), rejected until
+ * full rank: XorShift64 is GF(2)-linear, so linearly extracted bits would cap
+ * the rank near 64); mixed = transform * sparse.  Row strides ceil(cols/64)
+ * and ceil(n_rows/64) words. */
 int bmm_gen_planted(uint64_t seed, size_t n_rows, size_t cols, double p, uint64_t *sparse,
                     uint64_t *mixed, uint64_t *transform);
 /* Per matrix: rows = 8 + index(21), cols = 64 (1 + index(32)),

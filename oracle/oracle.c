@@ -28,7 +28,8 @@
  *   or_greedy_apply    reference GreedyApplier golden.
  *   generators         not reference behaviour (workload definitions); the
  *                      product generator must match them bit for bit, and
- *                      rank/inverse properties are checked directly.
+ *                      rank/inverse properties are checked directly (incl. a
+ *                      256 x 256 transform: full rank needs nonlinear bits).
  */
 #include "oracle.h"
 
@@ -124,13 +125,31 @@ void or_mul(const uint64_t *a, int a_rows, int a_stride, int a_cols, const uint6
     }
 }
 
-/* Bits drawn with p = 0.5 row-major; rejected until full rank. Returns the
- * number of draws. */
+/* splitmix64 finalizer: a nonlinear (over GF(2)) bijection of 64-bit words. */
+static uint64_t mix64(uint64_t x) {
+    x ^= x >> 30;
+    x *= 0xbf58476d1ce4e5b9ull;
+    x ^= x >> 27;
+    x *= 0x94d049bb133111ebull;
+    x ^= x >> 31;
+    return x;
+}
+
+/* Bit (r, c) = top bit of mix64(next output), row-major; rejected until full
+ * rank.  XorShift64 is GF(2)-linear, so bits taken linearly from its outputs
+ * (as `unit() < 0.5` does) would confine the matrix to rank about 64. Returns
+ * the number of draws. */
 int or_random_invertible(uint64_t *state, uint64_t *m, int n, int stride) {
     int draws = 0;
     for (;;) {
         ++draws;
-        or_fill_bernoulli(state, m, n, stride, n, 0.5);
+        for (int r = 0; r < n; ++r) {
+            uint64_t *row = m + (size_t)r * stride;
+            memset(row, 0, sizeof(uint64_t) * (size_t)stride);
+            for (int c = 0; c < n; ++c)
+                if (mix64(or_xs_next(state)) >> 63)
+                    row[c >> 6] |= 1ull << (c & 63);
+        }
         if (or_rank(m, n, stride, n) == n)
             return draws;
     }

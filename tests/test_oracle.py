@@ -286,3 +286,15 @@ def test_planted_generator_properties(ora):
     S, A, M = ora.planted(0x5EED0005, 64, 512, 0.05)
     assert ora.rank(M, 64) == 64
     assert ora.rank(A, 512) == ora.rank(S, 512)
+    # 256 x 256 transforms need nonlinear bit extraction (XorShift64 is
+    # GF(2)-linear); the planted 256 x 8192 instance must be full rank.
+    S, A, M = ora.planted(0x5EED0005, 256, 8192, 0.01)
+    assert ora.rank(M, 256) == 256
+    assert ora.rank(A, 8192) == ora.rank(S, 8192)
+
+
+def test_linear_bits_cap_rank():
+    """Why the transform bits are mixed: p = 0.5 Bernoulli bits read off
+    XorShift64 leave a 100 x 100 matrix far from full rank."""
+    o = oracle.oracle()
+    assert o.rank(o.bernoulli(5, 100, 100, 0.5), 100) <= 66
