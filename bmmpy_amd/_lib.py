@@ -56,15 +56,16 @@ class GaParams(C.Structure):
                 ("tournament", C.c_int), ("elite", C.c_int), ("migrate_every", C.c_int),
                 ("mutations", C.c_int), ("learned", C.c_int), ("n_candidates", C.c_int),
                 ("train_steps", C.c_int), ("check_every", C.c_int), ("use_graph", C.c_int),
-                ("rows", C.c_int), ("p_append", C.c_float), ("p_replace", C.c_float),
-                ("p_delete", C.c_float), ("learning_rate", C.c_float), ("seed", C.c_uint64)]
+                ("li_sweeps", C.c_int), ("rows", C.c_int), ("p_append", C.c_float),
+                ("p_replace", C.c_float), ("p_delete", C.c_float), ("learning_rate", C.c_float),
+                ("p_learned", C.c_float), ("seed", C.c_uint64)]
 
 
 class GaStatus(C.Structure):
     _fields_ = [("generations", C.c_size_t), ("ran", C.c_size_t), ("evaluations", C.c_uint64),
                 ("best_weight", C.c_uint64), ("best_island", C.c_int),
                 ("reached_target", C.c_int), ("elapsed_s", C.c_double),
-                ("train_loss", C.c_double)]
+                ("train_loss", C.c_double), ("li_moves", C.c_uint64)]
 
 
 # name -> (argtypes, restype)

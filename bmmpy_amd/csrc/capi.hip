@@ -495,10 +495,12 @@ int bmm_hip_ga_default_params(bmm_ga_params* p) {
         p->migrate_every = 20;
         p->mutations = 2;
         p->learned = 0;
-        p->n_candidates = 64;
+        p->n_candidates = 256;
         p->train_steps = 8;
         p->check_every = 10;
         p->use_graph = 1;
+        p->li_sweeps = 1;
+        p->p_learned = 0.5f;
         p->p_append = 0.6f;
         p->p_replace = 0.25f;
         p->p_delete = 0.15f;

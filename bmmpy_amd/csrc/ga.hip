@@ -90,12 +90,15 @@ __global__ __launch_bounds__(64) void ga_eval_kernel(const uint32_t* __restrict_
 // ------------------------------------------------------------ breed ------
 struct GaDev {
     int islands, pop, rows, max_ops, tournament, elite, n_cand, learned;
-    float p_append, p_replace, p_delete;
+    float p_append, p_replace, p_delete, p_learned;
     int mutations;
 };
 
 __device__ inline uint16_t propose(const GaDev& P, uint64_t& rng, const uint16_t* cand) {
-    if (P.learned && P.n_cand > 0)
+    // Learned: a ranked pair with probability p_learned, else uniform (the
+    // plain proposal stays the fallback).
+    if (P.learned && P.n_cand > 0 &&
+        (float)(xs_next(rng) >> 40) * (1.0f / 16777216.0f) < P.p_learned)
         return cand[xs_next(rng) % (uint64_t)P.n_cand];
     const int r = P.rows;
     const int s = (int)(xs_next(rng) % (uint64_t)r);
@@ -395,6 +398,101 @@ __global__ void ga_select_kernel(const float* __restrict__ score,
         cand[(size_t)isl * n_keep + i] = (uint16_t)(key[i] & 0xffffu);
 }
 
+// Local improvement on the island bases (genetic_algorithm_operators.cpp:
+// 131-192: replace row i by r_i ^ r_j for the j minimising
+// wt(r_i) + wt(r_j) - 2 popcount(r_i & r_j) when that beats wt(r_i)).  The
+// reference sweeps rows in order; here one round takes the rows whose index
+// bit `bit` equals `phase` as targets and the other rows as sources, so all
+// chosen moves are independent and apply in parallel; rounds over every index
+// bit and both phases cover every pair.  One workgroup per (island, row):
+// target rows score each source row by AND-popcount over the row words.
+__global__ __launch_bounds__(256) void ga_li_score_kernel(const uint64_t* __restrict__ base,
+                                                          int rows, int words,
+                                                          const uint32_t* __restrict__ roww,
+                                                          int bit, int phase,
+                                                          int32_t* __restrict__ move,
+                                                          uint32_t* __restrict__ neww) {
+    extern __shared__ uint64_t ri[];
+    __shared__ unsigned long long best;
+    const int isl = blockIdx.x / rows;
+    const int i = blockIdx.x % rows;
+    if (((i >> bit) & 1) != phase)
+        return;
+    const uint64_t* b = base + (size_t)isl * rows * words;
+    for (int w = threadIdx.x; w < words; w += blockDim.x)
+        ri[w] = b[(size_t)i * words + w];
+    if (threadIdx.x == 0)
+        best = ~0ull;
+    __syncthreads();
+    const uint32_t wi = roww[(size_t)isl * rows + i];
+    for (int j = threadIdx.x; j < rows; j += blockDim.x) {
+        if (((j >> bit) & 1) == phase)
+            continue;
+        const uint32_t wj = roww[(size_t)isl * rows + j];
+        if (wi == 0 || wj >= 2 * wi)
+            continue;
+        const uint64_t* rj = b + (size_t)j * words;
+        uint32_t ov = 0;
+        for (int w = 0; w < words; ++w)
+            ov += __builtin_popcountll(ri[w] & rj[w]);
+        const uint32_t cw = wi + wj - 2 * ov;
+        if (cw < wi)
+            atomicMin(&best, ((unsigned long long)cw << 32) | (unsigned)j);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const size_t id = (size_t)isl * rows + i;
+        if (best != ~0ull) {
+            move[id] = (int32_t)(best & 0xffffffffu);
+            neww[id] = (uint32_t)(best >> 32);
+        } else {
+            move[id] = -1;
+        }
+    }
+}
+
+__global__ void ga_li_apply_kernel(uint64_t* __restrict__ base, int rows, int words,
+                                   const int32_t* __restrict__ move,
+                                   const uint32_t* __restrict__ neww,
+                                   uint32_t* __restrict__ roww, int bit, int phase,
+                                   unsigned int* __restrict__ changed) {
+    const int isl = blockIdx.x / rows;
+    const int i = blockIdx.x % rows;
+    if (((i >> bit) & 1) != phase)
+        return;
+    const size_t id = (size_t)isl * rows + i;
+    const int j = move[id];
+    if (j < 0)
+        return;
+    uint64_t* b = base + (size_t)isl * rows * words;
+    for (int w = threadIdx.x; w < words; w += blockDim.x)
+        b[(size_t)i * words + w] ^= b[(size_t)j * words + w];
+    if (threadIdx.x == 0) {
+        roww[id] = neww[id];
+        atomicAdd(changed, 1u);
+    }
+}
+
+// best_w[isl] = total weight of the island base (after local improvement).
+__global__ void ga_base_weight_kernel(const uint32_t* __restrict__ roww, int rows,
+                                      unsigned int* __restrict__ best_w) {
+    const int isl = blockIdx.x;
+    uint32_t s = 0;
+    for (int r = threadIdx.x; r < rows; r += blockDim.x)
+        s += roww[(size_t)isl * rows + r];
+    s = wave_sum_u32(s);
+    __shared__ uint32_t red[4];
+    if (threadIdx.x % kWave == 0)
+        red[threadIdx.x / kWave] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t = 0;
+        for (int k = 0; k < (int)(blockDim.x + kWave - 1) / kWave; ++k)
+            t += red[k];
+        best_w[isl] = t;
+    }
+}
+
 // --------------------------------------------------------------- host ----
 struct Ga {
     bmm_ga_params P{};
@@ -403,6 +501,7 @@ struct Ga {
         w_snap, roww, sketch, evals;
     // proposal model data
     DevBuf pairs, pair_isl, feats, labels, prng;
+    DevBuf li_move, li_w, li_changed;
     Mlp* mlp = nullptr;
     int cur = 0;
     long generation = 0;
@@ -410,6 +509,7 @@ struct Ga {
     hipGraphExec_t gen_exec[2] = {nullptr, nullptr};
     hipGraph_t gen_graph[2] = {nullptr, nullptr};
     double train_loss = 0.0;
+    uint64_t li_moves = 0;
     ~Ga() {
         for (int i = 0; i < 2; ++i) {
             if (gen_exec[i])
@@ -440,6 +540,7 @@ GaDev dev_params(const Ga& G) {
     d.p_replace = G.P.p_replace;
     d.p_delete = G.P.p_delete;
     d.mutations = G.P.mutations;
+    d.p_learned = G.P.p_learned;
     return d;
 }
 
@@ -561,6 +662,40 @@ void update_proposals(Ga& G) {
     BMM_HIP(hipGetLastError());
 }
 
+// Rounds of parallel pairwise improvement on every island base; returns the
+// number of applied moves.  Needs up-to-date row weights (ga_sketch_kernel).
+unsigned int local_improvement(Ga& G) {
+    const int I = G.P.islands, R = G.P.rows;
+    int bits = 0;
+    while ((1 << bits) < R)
+        ++bits;
+    unsigned int* changed = static_cast<unsigned int*>(G.li_changed.ptr);
+    fill_u32(changed, 1, 0u, G.stream);
+    for (int sweep = 0; sweep < G.P.li_sweeps; ++sweep)
+        for (int bit = 0; bit < bits; ++bit)
+            for (int phase = 0; phase < 2; ++phase) {
+                hipLaunchKernelGGL(ga_li_score_kernel, dim3((unsigned)(I * R)), dim3(256),
+                                   (size_t)G.words * 8, G.stream,
+                                   static_cast<const uint64_t*>(G.base.ptr), R, G.words,
+                                   static_cast<const uint32_t*>(G.roww.ptr), bit, phase,
+                                   static_cast<int32_t*>(G.li_move.ptr),
+                                   static_cast<uint32_t*>(G.li_w.ptr));
+                hipLaunchKernelGGL(ga_li_apply_kernel, dim3((unsigned)(I * R)), dim3(128),
+                                   0, G.stream, static_cast<uint64_t*>(G.base.ptr), R, G.words,
+                                   static_cast<const int32_t*>(G.li_move.ptr),
+                                   static_cast<const uint32_t*>(G.li_w.ptr),
+                                   static_cast<uint32_t*>(G.roww.ptr), bit, phase, changed);
+            }
+    hipLaunchKernelGGL(ga_base_weight_kernel, dim3(I), dim3(256), 0, G.stream,
+                       static_cast<const uint32_t*>(G.roww.ptr), R,
+                       static_cast<unsigned int*>(G.best_w.ptr));
+    BMM_HIP(hipGetLastError());
+    unsigned int n = 0;
+    BMM_HIP(hipMemcpyAsync(&n, changed, 4, hipMemcpyDeviceToHost, G.stream));
+    BMM_HIP(hipStreamSynchronize(G.stream));
+    return n;
+}
+
 void epoch(Ga& G) {
     const int I = G.P.islands;
     const int parity = G.cur;   // population about to be evaluated
@@ -588,6 +723,10 @@ void epoch(Ga& G) {
                        static_cast<const unsigned int*>(G.w_snap.ptr),
                        static_cast<const int32_t*>(G.flag.ptr), I);
     BMM_HIP(hipGetLastError());
+    if (G.P.li_sweeps > 0) {
+        refresh_sketches(G);
+        G.li_moves += local_improvement(G);
+    }
     update_proposals(G);
 }
 
@@ -656,6 +795,9 @@ Ga* ga_create(const uint64_t* matrix, size_t rows, size_t cols, const bmm_ga_par
                       hipMemcpyHostToDevice));
     BMM_HIP(hipMemcpy(G->prng.get<uint64_t>(n_pairs), seeds.data() + (size_t)I * Pp,
                       (size_t)n_pairs * 8, hipMemcpyHostToDevice));
+    G->li_move.get<int32_t>((size_t)I * rows);
+    G->li_w.get<uint32_t>((size_t)I * rows);
+    G->li_changed.get<unsigned int>(1);
     G->pairs.get<uint16_t>(n_pairs);
     G->pair_isl.get<int32_t>(n_pairs);
     G->feats.get<__bf16>((size_t)n_pairs * 256 + 32 * 256);
@@ -710,6 +852,7 @@ void ga_run(Ga* G, size_t max_generations, uint64_t target, double time_budget_s
     st->elapsed_s =
         std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     st->train_loss = G->train_loss;
+    st->li_moves = G->li_moves;
 }
 
 // Best matrix: the best island's base with its best sequence replayed.

@@ -194,8 +194,10 @@ int bmm_hip_replay(uint64_t *rows, size_t n_rows, size_t words_per_row, const ui
  * Individual = up to max_ops ops (src, dst) applied to the island's base
  * matrix; fitness = exact total weight after replay; elitism + tournament +
  * one-point crossover + mutation (append / replace / delete an op); every
- * migrate_every generations each island folds its best sequence into its base
- * and adopts its ring predecessor's base when that one is lighter.  learned = 1
+ * migrate_every generations each island folds its best sequence into its base,
+ * adopts its ring predecessor's base when that one is lighter, and runs
+ * li_sweeps sweeps of pairwise local improvement on it (the reference GA's
+ * local_improvement rule, applied in parallel rounds).  learned = 1
  * draws new ops from pairs ranked by the proposal MLP (trained online on exact
  * pair deltas); learned = 0 draws uniform random pairs.  rows <= 256, cols a
  * multiple of 2048. */
@@ -212,11 +214,13 @@ typedef struct bmm_ga_params {
     int train_steps;    /* MLP steps per migration epoch (learned) */
     int check_every;    /* generations between host checks of the best weight */
     int use_graph;      /* replay each generation as a captured hipGraph */
+    int li_sweeps;      /* pairwise local-improvement sweeps per epoch (0 = off) */
     int rows;           /* filled in by create */
     float p_append;
     float p_replace;
     float p_delete;
     float learning_rate;
+    float p_learned;    /* share of new ops drawn from the ranked pairs */
     uint64_t seed;
 } bmm_ga_params;
 
@@ -229,6 +233,7 @@ typedef struct bmm_ga_status {
     int reached_target;
     double elapsed_s;
     double train_loss;    /* last MLP training loss (learned) */
+    uint64_t li_moves;    /* local-improvement moves applied so far */
 } bmm_ga_status;
 
 int bmm_hip_ga_default_params(bmm_ga_params *out);
