@@ -76,10 +76,8 @@ template <int L>
 __device__ void coset_block_phase1(int32_t* s, const CosetMatrix& M, const uint32_t* pat,
                                    uint32_t mh, unsigned long long* keys) {
     block_build_g<L>(s, pat, M.cols, mh);
-    block_fwht<L>(s);
     // Element ml = e * 256 + tid: ml bits >= 8 are register bits.
     constexpr int N = 1 << L;
-    constexpr int E = N >= 256 ? N / 256 : 1;
     constexpr int HB = L > 8 ? L - 8 : 0;
     const int tid = threadIdx.x;
     uint32_t overall = 0xffffffffu;
@@ -87,34 +85,57 @@ __device__ void coset_block_phase1(int32_t* s, const CosetMatrix& M, const uint3
 #pragma unroll
     for (int i = 0; i < (HB > 0 ? HB : 1); ++i)
         mreg[i] = 0xffffffffu;
-    if (tid < N) {
+    auto visit = [&](int e, int32_t h) {
+        const uint32_t w = (uint32_t)(M.total - h) >> 1;
+        overall = min(overall, w);
 #pragma unroll
-        for (int e = 0; e < E; ++e) {
-            const int ml = e * 256 + tid;
-            const uint32_t w = (uint32_t)(M.total - s[fwht_pad(ml)]) >> 1;
-            overall = min(overall, w);
+        for (int i = 0; i < HB; ++i)
+            if ((e >> i) & 1)
+                mreg[i] = min(mreg[i], w);
+    };
+    if constexpr (L >= 11) {
+        block_fwht_visit<L, 0>(s, visit);
+    } else {
+        constexpr int E = N >= 256 ? N / 256 : 1;
+        block_fwht<L>(s);
+        if (tid < N) {
 #pragma unroll
-            for (int i = 0; i < HB; ++i)
-                if ((e >> i) & 1)
-                    mreg[i] = min(mreg[i], w);
+            for (int e = 0; e < E; ++e)
+                visit(e, s[fwht_pad(e * 256 + tid)]);
         }
     }
-    __shared__ uint32_t red[32][4];
-    const int wv = tid / kWave, lane = tid % kWave;
+    // Per-target block minima through LDS: row 0 holds each thread's overall
+    // minimum, rows 1..HB its register-bit minima; one wavefront reduces each
+    // target's 256 (or 128) candidates.
+    uint32_t* red = reinterpret_cast<uint32_t*>(s);
+    __shared__ uint32_t res[kBatchMaxL + 1];
+    __syncthreads();
+    red[tid] = overall;
 #pragma unroll
-    for (int i = 0; i < L; ++i) {
-        uint32_t v = i < 8 ? (((tid >> i) & 1) ? overall : 0xffffffffu) : mreg[i >= 8 ? i - 8 : 0];
+    for (int i = 0; i < HB; ++i)
+        red[(1 + i) * 256 + tid] = mreg[i];
+    __syncthreads();
+    const int wv = tid / kWave, lane = tid % kWave;
+    for (int q = wv; q <= L; q += 4) {
+        uint32_t v = 0xffffffffu;
+        for (int x = lane; x < 256; x += kWave) {
+            uint32_t val;
+            if (q < L && q < 8)
+                val = ((x >> q) & 1) ? red[x] : 0xffffffffu;
+            else if (q < L)
+                val = red[(1 + q - 8) * 256 + x];
+            else
+                val = red[x];
+            v = min(v, val);
+        }
         v = wave_min_u32(v);
         if (lane == 0)
-            red[i][wv] = v;
+            res[q] = v;
     }
-    const uint32_t wover = wave_min_u32(overall);
-    for (int i = L; i < M.rows; ++i)
-        if (lane == 0)
-            red[i][wv] = ((mh >> (i - L)) & 1u) ? wover : 0xffffffffu;
     __syncthreads();
     if (tid < M.rows) {
-        const uint32_t v = min(min(red[tid][0], red[tid][1]), min(red[tid][2], red[tid][3]));
+        const uint32_t v = tid < L ? res[tid]
+                                   : (((mh >> (tid - L)) & 1u) ? res[L] : 0xffffffffu);
         if (v != 0xffffffffu)
             atomicMin(&keys[M.out_off + tid], ((unsigned long long)v << 32) | mh);
     }

@@ -54,6 +54,47 @@ template <int L, int LO = 0> __device__ inline void block_fwht(int32_t* s) {
     }
 }
 
+// Like block_fwht, but the last round hands its results to
+// visit(index, value) instead of storing them: thread t sees the entries
+// ml = index * 256 + t, index in [0, 2^L / 256).  Needs L >= 11 (the last
+// round then spans >= 256 groups, one or more per thread, with stride >= 256);
+// starts with a barrier and leaves LDS holding the pre-last-round values.
+template <int L, int LO, typename Visit> __device__ inline void block_fwht_visit(int32_t* s,
+                                                                                Visit&& visit) {
+    constexpr int R = (L - LO) < 5 ? (L - LO) : 5;
+    if constexpr (LO + R < L) {
+        __syncthreads();
+        fwht_round<L, LO, R>(s);
+        block_fwht_visit<L, LO + R>(s, visit);
+    } else {
+        static_assert(LO >= 8, "fused last round needs a stride of at least 256");
+        __syncthreads();
+        constexpr int G = 1 << LO;          // groups in the last round
+        constexpr int E = 1 << R;
+        constexpr int PER = G / 256;        // groups per thread
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            const int g = (int)threadIdx.x + 256 * q;
+            uint32_t v[E];
+#pragma unroll
+            for (int e = 0; e < E; ++e)
+                v[e] = (uint32_t)s[fwht_pad(g + (e << LO))];
+#pragma unroll
+            for (int len = 1; len < E; len <<= 1)
+#pragma unroll
+                for (int i = 0; i < E; ++i)
+                    if ((i & len) == 0) {
+                        const uint32_t a = v[i], b = v[i + len];
+                        v[i] = a + b;
+                        v[i + len] = a - b;
+                    }
+#pragma unroll
+            for (int e = 0; e < E; ++e)
+                visit(q + PER * e, (int32_t)v[e]);
+        }
+    }
+}
+
 // g[pl] = sum over active columns c of (-1)^popcount((pat[c] >> L) & mh) at
 // pl = pat[c] & (2^L - 1).  Column patterns carry row r at bit r.
 template <int L>
