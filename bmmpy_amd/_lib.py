@@ -92,6 +92,7 @@ SIGNATURES = {
                                    C.POINTER(C.c_void_p)], C.c_int),
     "bmm_hip_batch_plan_run": ([_vp, C.c_int, _vp], C.c_int),
     "bmm_hip_batch_plan_info": ([_vp, C.POINTER(BatchInfo)], C.c_int),
+    "bmm_hip_batch_plan_set_shape": ([_vp, C.c_int, _sz, _sz], C.c_int),
     "bmm_hip_batch_plan_destroy": ([_vp], C.c_int),
     "bmm_hip_window_coset_min": ([_u64p, _sz, _sz, _sz, _i32p, _sz, _i32p, _sz, _u32p, _u64p],
                                  C.c_int),

@@ -22,6 +22,7 @@
 // launch shape can be picked by timing candidates on the plan's own data.
 #include "common.hpp"
 #include "fwht_block.hpp"
+#include "fwht_pk16.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -72,43 +73,20 @@ __global__ __launch_bounds__(256) void batch_patterns_kernel(const uint64_t* __r
         M.total = (int32_t)(red[0] + red[1] + red[2] + red[3]);
 }
 
+// Per-target block maxima of H from per-thread values, folded into the keys
+// as (weight << 32 | mh).  Thread t's elements have ml bits 0..7 = t;
+// `overall` is the thread's maximum, mreg[j] its maximum over the elements
+// with ml bit 8 + j set (j < L - 8).  Target bits below 8 reduce row 0 over
+// the threads with that bit, bits 8..L-1 reduce their rows, bits >= L take
+// the block maximum when mh has the bit.
 template <int L>
-__device__ void coset_block_phase1(int32_t* s, const CosetMatrix& M, const uint32_t* pat,
-                                   uint32_t mh, unsigned long long* keys) {
-    block_build_g<L>(s, pat, M.cols, mh);
-    // Element ml = e * 256 + tid: ml bits >= 8 are register bits.
-    constexpr int N = 1 << L;
+__device__ void coset_block_finish(int32_t* s, const CosetMatrix& M, uint32_t mh,
+                                   unsigned long long* keys, int32_t overall,
+                                   const int32_t* mreg) {
     constexpr int HB = L > 8 ? L - 8 : 0;
     const int tid = threadIdx.x;
-    uint32_t overall = 0xffffffffu;
-    uint32_t mreg[HB > 0 ? HB : 1];
-#pragma unroll
-    for (int i = 0; i < (HB > 0 ? HB : 1); ++i)
-        mreg[i] = 0xffffffffu;
-    auto visit = [&](int e, int32_t h) {
-        const uint32_t w = (uint32_t)(M.total - h) >> 1;
-        overall = min(overall, w);
-#pragma unroll
-        for (int i = 0; i < HB; ++i)
-            if ((e >> i) & 1)
-                mreg[i] = min(mreg[i], w);
-    };
-    if constexpr (L >= 11) {
-        block_fwht_visit<L, 0>(s, visit);
-    } else {
-        constexpr int E = N >= 256 ? N / 256 : 1;
-        block_fwht<L>(s);
-        if (tid < N) {
-#pragma unroll
-            for (int e = 0; e < E; ++e)
-                visit(e, s[fwht_pad(e * 256 + tid)]);
-        }
-    }
-    // Per-target block minima through LDS: row 0 holds each thread's overall
-    // minimum, rows 1..HB its register-bit minima; one wavefront reduces each
-    // target's 256 (or 128) candidates.
-    uint32_t* red = reinterpret_cast<uint32_t*>(s);
-    __shared__ uint32_t res[kBatchMaxL + 1];
+    int32_t* red = s;
+    __shared__ int32_t res[kPkL + 1];
     __syncthreads();
     red[tid] = overall;
 #pragma unroll
@@ -117,37 +95,157 @@ __device__ void coset_block_phase1(int32_t* s, const CosetMatrix& M, const uint3
     __syncthreads();
     const int wv = tid / kWave, lane = tid % kWave;
     for (int q = wv; q <= L; q += 4) {
-        uint32_t v = 0xffffffffu;
+        int32_t v = INT32_MIN;
         for (int x = lane; x < 256; x += kWave) {
-            uint32_t val;
+            int32_t val;
             if (q < L && q < 8)
-                val = ((x >> q) & 1) ? red[x] : 0xffffffffu;
+                val = ((x >> q) & 1) ? red[x] : INT32_MIN;
             else if (q < L)
                 val = red[(1 + q - 8) * 256 + x];
             else
                 val = red[x];
-            v = min(v, val);
+            v = max(v, val);
         }
-        v = wave_min_u32(v);
+        v = wave_max_i32(v);
         if (lane == 0)
             res[q] = v;
     }
     __syncthreads();
     if (tid < M.rows) {
-        const uint32_t v = tid < L ? res[tid]
-                                   : (((mh >> (tid - L)) & 1u) ? res[L] : 0xffffffffu);
-        if (v != 0xffffffffu)
-            atomicMin(&keys[M.out_off + tid], ((unsigned long long)v << 32) | mh);
+        const bool has = tid < L || ((mh >> (tid - L)) & 1u);
+        const int32_t h = res[tid < L ? tid : L];
+        if (has && h != INT32_MIN) {
+            const uint32_t w = (uint32_t)(M.total - h) >> 1;
+            atomicMin(&keys[M.out_off + tid], ((unsigned long long)w << 32) | mh);
+        }
     }
     __syncthreads();
 }
 
-__global__ __launch_bounds__(256) void coset_phase1_kernel(
-    const CosetMatrix* __restrict__ mats, const int64_t* __restrict__ prefix, int n_mat,
+// Transform of block mh (built into LDS by `build`), int32 entries: thread t
+// holds elements ml = i * 256 + t, so ml bits 8..L-1 index its register
+// values; maximising H minimises the weight (weight = (total - H) / 2).
+template <int L, typename Build>
+__device__ void coset_block_phase1(int32_t* s, const CosetMatrix& M, uint32_t mh,
+                                   unsigned long long* keys, Build&& build) {
+    build(s);
+    constexpr int N = 1 << L;
+    constexpr int HB = L > 8 ? L - 8 : 0;
+    const int tid = threadIdx.x;
+    int32_t overall = INT32_MIN;
+    int32_t mreg[HB > 0 ? HB : 1];
+#pragma unroll
+    for (int i = 0; i < (HB > 0 ? HB : 1); ++i)
+        mreg[i] = INT32_MIN;
+    // v[e] is register index q + PER * e: subset maxima over e, then the bits
+    // of q (fixed per call) take the group maximum.
+    auto visit = [&](int q, const auto& v) {
+        constexpr int E = sizeof(v) / sizeof(v[0]);
+        constexpr int LE = ilog2c(E);
+        constexpr int LP = HB - LE;
+        int32_t x[E], bm[LE > 0 ? LE : 1];
+#pragma unroll
+        for (int e = 0; e < E; ++e)
+            x[e] = (int32_t)v[e];
+        const int32_t all = subset_max<E>(x, bm);
+#pragma unroll
+        for (int j = 0; j < LE; ++j)
+            mreg[LP + j] = max(mreg[LP + j], bm[j]);
+#pragma unroll
+        for (int i = 0; i < LP; ++i)
+            if ((q >> i) & 1)
+                mreg[i] = max(mreg[i], all);
+        overall = max(overall, all);
+    };
+    if constexpr (L >= 11) {
+        block_fwht_visit<L, 0>(s, visit);
+    } else {
+        constexpr int E = N >= 256 ? N / 256 : 1;
+        block_fwht<L>(s);
+        if (tid < N) {
+            uint32_t v[E];
+#pragma unroll
+            for (int e = 0; e < E; ++e)
+                v[e] = (uint32_t)s[fwht_pad(e * 256 + tid)];
+            visit(0, v);
+        }
+    }
+    coset_block_finish<L>(s, M, mh, keys, overall, mreg);
+}
+
+// Packed-int16 block of 2^14 entries (matrices of >= 14 rows and at most
+// 32767 columns).  Word w = (q + 4 e) * 256 + t holds entries w and
+// w + 2^13: ml bits 8..12 are the register index q + 4 e, bit 13 the half.
+template <typename Build>
+__device__ void coset_block_pk16(uint32_t* s, const CosetMatrix& M, uint32_t mh,
+                                 unsigned long long* keys, Build&& build) {
+    build(s);
+    const s16x2 lowest = {INT16_MIN, INT16_MIN};
+    s16x2 overall = lowest;
+    s16x2 mreg[5];
+#pragma unroll
+    for (int i = 0; i < 5; ++i)
+        mreg[i] = lowest;
+    pk_block_fwht_visit(s, [&](int q, const uint32_t (&v)[8]) {
+        s16x2 x[8], bm[3];
+#pragma unroll
+        for (int e = 0; e < 8; ++e)
+            x[e] = pk_as(v[e]);
+        const s16x2 all = pk_subset_max<8>(x, bm);
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            mreg[2 + j] = pk_max(mreg[2 + j], bm[j]);
+        if (q & 1)
+            mreg[0] = pk_max(mreg[0], all);
+        if (q & 2)
+            mreg[1] = pk_max(mreg[1], all);
+        overall = pk_max(overall, all);
+    });
+    int32_t m[6];
+#pragma unroll
+    for (int i = 0; i < 5; ++i)
+        m[i] = max(pk_lo(mreg[i]), pk_hi(mreg[i]));
+    m[5] = pk_hi(overall);
+    coset_block_finish<kPkL>(reinterpret_cast<int32_t*>(s), M, mh, keys,
+                             max(pk_lo(overall), pk_hi(overall)), m);
+}
+
+// Column patterns of the current matrix kept in registers across its blocks
+// (up to kPatRegs * 256 columns; wider matrices read them from memory).
+constexpr int kPatRegs = 8;
+
+template <int L>
+__device__ inline void build_from_regs(int32_t* s, const uint32_t (&preg)[kPatRegs], uint32_t mh) {
+    constexpr int N = 1 << L;
+    int4* s4 = reinterpret_cast<int4*>(s);
+    for (int i = threadIdx.x; i < (fwht_lds_words(L) + 3) / 4; i += blockDim.x)
+        s4[i] = make_int4(0, 0, 0, 0);
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kPatRegs; ++k) {
+        const uint32_t p = preg[k];
+        if (p != 0u) {
+            const int neg = __builtin_popcount((p >> L) & mh) & 1;
+            atomicAdd(&s[fwht_pad((int)(p & (N - 1)))], neg ? -1 : 1);
+        }
+    }
+}
+
+// Blocks of the matrices with at least `L` rows (all of them 2^L wide):
+// persistent workgroups take `chunk` consecutive block numbers per atomicAdd.
+// The matrix of a block is found by galloping forward from the previous
+// one (block numbers handed to a workgroup only grow), then bisecting.
+template <int L>
+__global__ __launch_bounds__(256) void coset_full_kernel(
+    const CosetMatrix* __restrict__ mats, const int64_t* __restrict__ prefix, int n_full,
     int64_t total_blocks, int chunk, const uint32_t* __restrict__ pats,
     unsigned long long* __restrict__ counter, unsigned long long* __restrict__ keys) {
-    __shared__ int32_t s[fwht_lds_words(kBatchMaxL)];
+    constexpr int kWords = L == kPkL ? pk_lds_words() : fwht_lds_words(L);
+    __shared__ __align__(16) int32_t s[(kWords > 7 * 256 ? kWords : 7 * 256) + 4];
     __shared__ long long first;
+    int idx = 0, cached = -1;
+    uint32_t preg[kPatRegs];
+    CosetMatrix M{};
     for (;;) {
         if (threadIdx.x == 0)
             first = (long long)atomicAdd(counter, (unsigned long long)chunk);
@@ -157,32 +255,76 @@ __global__ __launch_bounds__(256) void coset_phase1_kernel(
         if (g0 >= total_blocks)
             break;
         const int64_t g1 = min(g0 + (int64_t)chunk, total_blocks);
-        // Matrix holding block g0: last index with prefix[idx] <= g0.
-        int lo = 0, hi = n_mat - 1;
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (prefix[mid] <= g0)
-                lo = mid;
-            else
-                hi = mid - 1;
+        if (prefix[idx + 1] <= g0) {
+            int step = 1, lo = idx + 1, hi = idx + 1;
+            while (hi < n_full - 1 && prefix[hi + 1] <= g0) {
+                lo = hi + 1;
+                hi = min(hi + step, n_full - 1);
+                step *= 2;
+            }
+            // prefix[lo] <= g0 < prefix[hi + 1]
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (prefix[mid] <= g0)
+                    lo = mid;
+                else
+                    hi = mid - 1;
+            }
+            idx = lo;
         }
-        int idx = lo;
         for (int64_t g = g0; g < g1; ++g) {
             while (prefix[idx + 1] <= g)
                 ++idx;
-            const CosetMatrix M = mats[idx];
+            if (idx != cached) {
+                M = mats[idx];
+                const uint32_t* pat = pats + M.pat_off;
+#pragma unroll
+                for (int k = 0; k < kPatRegs; ++k) {
+                    const int c = (int)threadIdx.x + 256 * k;
+                    preg[k] = c < M.cols ? pat[c] : 0u;
+                }
+                cached = idx;
+            }
             const uint32_t mh = (uint32_t)(g - prefix[idx]);
-            const uint32_t* pat = pats + M.pat_off;
-            switch (M.L) {
-#define BMM_CASE(LL)                                                                           \
-    case LL: coset_block_phase1<LL>(s, M, pat, mh, keys); break;
-                BMM_CASE(1) BMM_CASE(2) BMM_CASE(3) BMM_CASE(4) BMM_CASE(5) BMM_CASE(6)
-                BMM_CASE(7) BMM_CASE(8) BMM_CASE(9) BMM_CASE(10) BMM_CASE(11) BMM_CASE(12)
-                BMM_CASE(13)
-#undef BMM_CASE
-            default: break;
+            if constexpr (L == kPkL) {
+                uint32_t* su = reinterpret_cast<uint32_t*>(s);
+                if (M.cols <= kPatRegs * 256)
+                    coset_block_pk16(su, M, mh, keys,
+                                     [&](uint32_t* sp) { pk_build(sp, preg, mh); });
+                else
+                    coset_block_pk16(su, M, mh, keys, [&](uint32_t* sp) {
+                        pk_build_mem(sp, pats + M.pat_off, M.cols, mh);
+                    });
+            } else if (M.cols <= kPatRegs * 256) {
+                coset_block_phase1<L>(s, M, mh, keys,
+                                      [&](int32_t* sp) { build_from_regs<L>(sp, preg, mh); });
+            } else {
+                coset_block_phase1<L>(s, M, mh, keys, [&](int32_t* sp) {
+                    block_build_g<L>(sp, pats + M.pat_off, M.cols, mh);
+                });
             }
         }
+    }
+}
+
+// Matrices with fewer rows than the block size: one workgroup each, one
+// block of 2^rows masks.
+__global__ __launch_bounds__(256) void coset_small_kernel(const CosetMatrix* __restrict__ mats,
+                                                         const uint32_t* __restrict__ pats,
+                                                         unsigned long long* __restrict__ keys) {
+    __shared__ __align__(16) int32_t s[fwht_lds_words(kBatchMaxL) + 4];
+    const CosetMatrix M = mats[blockIdx.x];
+    const uint32_t* pat = pats + M.pat_off;
+    switch (M.L) {
+#define BMM_CASE(LL)                                                                           \
+    case LL:                                                                                   \
+        coset_block_phase1<LL>(s, M, 0u, keys,                                                 \
+                               [&](int32_t* sp) { block_build_g<LL>(sp, pat, M.cols, 0u); });  \
+        break;
+        BMM_CASE(1) BMM_CASE(2) BMM_CASE(3) BMM_CASE(4) BMM_CASE(5) BMM_CASE(6) BMM_CASE(7)
+        BMM_CASE(8) BMM_CASE(9) BMM_CASE(10) BMM_CASE(11) BMM_CASE(12) BMM_CASE(13)
+#undef BMM_CASE
+    default: break;
     }
 }
 
@@ -214,12 +356,48 @@ __device__ void coset_block_phase2(int32_t* s, const CosetMatrix& M, const uint3
     __syncthreads();
 }
 
+// Phase 2 for a packed-int16 block of 2^14 entries.
+__device__ void coset_block_phase2_pk16(uint32_t* s, const CosetMatrix& M, const uint32_t* pat,
+                                        int target, unsigned long long key,
+                                        uint32_t* low_out) {
+    const uint32_t mh = (uint32_t)key;
+    const int32_t want = M.total - 2 * (int32_t)(uint32_t)(key >> 32);  // H of the minimum
+    __shared__ uint32_t best;
+    if (threadIdx.x == 0)
+        best = 0xffffffffu;
+    pk_build_mem(s, pat, M.cols, mh);
+    uint32_t mine = 0xffffffffu;
+    pk_block_fwht_visit(s, [&](int q, const uint32_t (&v)[8]) {
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const uint32_t w = (uint32_t)((q + 4 * e) * 256 + (int)threadIdx.x);
+            const s16x2 x = pk_as(v[e]);
+            const uint32_t ml_lo = w, ml_hi = w + (1u << kPkW);
+            const bool has_lo = target < kPkL ? ((ml_lo >> target) & 1u) : true;
+            const bool has_hi = target < kPkL ? ((ml_hi >> target) & 1u) : true;
+            if (has_lo && pk_lo(x) == want)
+                mine = min(mine, ml_lo);
+            if (has_hi && pk_hi(x) == want)
+                mine = min(mine, ml_hi);
+        }
+    });
+    mine = wave_min_u32(mine);
+    if ((threadIdx.x % kWave) == 0 && mine != 0xffffffffu)
+        atomicMin(&best, mine);
+    __syncthreads();
+    if (threadIdx.x == 0)
+        *low_out = best;
+    __syncthreads();
+}
+
 // One workgroup per output slot (matrix, target).
 __global__ __launch_bounds__(256) void coset_phase2_kernel(
     const CosetMatrix* __restrict__ mats, const int32_t* __restrict__ slot_matrix,
     const uint32_t* __restrict__ pats, const unsigned long long* __restrict__ keys,
     uint32_t* __restrict__ out_w, uint64_t* __restrict__ out_idx) {
-    __shared__ int32_t s[fwht_lds_words(kBatchMaxL)];
+    constexpr int kWords = fwht_lds_words(kBatchMaxL) > pk_lds_words() ? fwht_lds_words(kBatchMaxL)
+                                                                         : pk_lds_words();
+    __shared__ __align__(16) int32_t s[kWords + 4];
     __shared__ uint32_t low;
     const int slot = blockIdx.x;
     const CosetMatrix M = mats[slot_matrix[slot]];
@@ -232,6 +410,9 @@ __global__ __launch_bounds__(256) void coset_phase2_kernel(
         BMM_CASE(1) BMM_CASE(2) BMM_CASE(3) BMM_CASE(4) BMM_CASE(5) BMM_CASE(6) BMM_CASE(7)
         BMM_CASE(8) BMM_CASE(9) BMM_CASE(10) BMM_CASE(11) BMM_CASE(12) BMM_CASE(13)
 #undef BMM_CASE
+    case kPkL:
+        coset_block_phase2_pk16(reinterpret_cast<uint32_t*>(s), M, pat, target, key, &low);
+        break;
     default: break;
     }
     if (threadIdx.x == 0) {
@@ -251,7 +432,9 @@ struct BatchPlan {
     int device = -1;
     size_t n_mat = 0;
     int64_t n_slots = 0;
-    int64_t total_blocks = 0;
+    int64_t total_blocks = 0;   // blocks of all matrices
+    int64_t full_blocks = 0;    // blocks of the matrices with >= lmax rows
+    int n_full = 0;             // those matrices (first in the sorted order)
     double subset_evals = 0.0;   // sum of r * 2^(r-1)
     double transform_elems = 0.0; // sum of 2^r
     int grid = 0;
@@ -308,10 +491,15 @@ void plan_upload(BatchPlan& P, int lmax) {
     });
     std::vector<CosetMatrix> sorted(n);
     std::vector<int64_t> prefix(n + 1, 0);
+    int n_full = 0;
     for (size_t i = 0; i < n; ++i) {
         sorted[i] = mats[order[i]];
         prefix[i + 1] = prefix[i] + (int64_t(1) << (sorted[i].rows - sorted[i].L));
+        if (sorted[i].L == lmax)
+            n_full = (int)i + 1;
     }
+    P.n_full = n_full;
+    P.full_blocks = prefix[n_full];
     P.total_blocks = prefix[n];
     std::vector<int32_t> slot_matrix((size_t)P.n_slots, -1);
     for (size_t i = 0; i < n; ++i)
@@ -348,12 +536,19 @@ void plan_enqueue(BatchPlan& P, hipStream_t s, bool timed) {
     hipLaunchKernelGGL(batch_patterns_kernel, dim3((unsigned)P.n_mat), dim3(256), 0, s, P.d_words,
                        d_m, d_pat);
     BMM_HIP(hipGetLastError());
-    {
+    if (P.full_blocks > 0) {
         ScopedKernelTimer timer(s, timed);
-        hipLaunchKernelGGL(coset_phase1_kernel, dim3(P.grid), dim3(256), 0, s, d_m, d_p,
-                           (int)P.n_mat, P.total_blocks, P.chunk, d_pat, d_c, d_k);
+        auto kern = P.lmax == kPkL ? coset_full_kernel<kPkL>
+                    : P.lmax == 13 ? coset_full_kernel<13> : coset_full_kernel<12>;
+        hipLaunchKernelGGL(kern, dim3(P.grid), dim3(256), 0, s, d_m, d_p, P.n_full,
+                           P.full_blocks, P.chunk, d_pat, d_c, d_k);
+        BMM_HIP(hipGetLastError());
     }
-    BMM_HIP(hipGetLastError());
+    if ((int64_t)P.n_mat > P.n_full) {
+        hipLaunchKernelGGL(coset_small_kernel, dim3((unsigned)(P.n_mat - P.n_full)), dim3(256), 0,
+                           s, d_m + P.n_full, d_pat, d_k);
+        BMM_HIP(hipGetLastError());
+    }
     hipLaunchKernelGGL(coset_phase2_kernel, dim3((unsigned)P.n_slots), dim3(256), 0, s, d_m, d_s,
                        d_pat, d_k, P.d_out_w, P.d_out_idx);
     BMM_HIP(hipGetLastError());
@@ -364,7 +559,11 @@ void plan_autotune(BatchPlan& P) {
     int cus = default_persistent_wgs() / 8;
     struct Cand { int lmax, grid, chunk; };
     std::vector<Cand> cands;
-    for (int lmax : {12, 13})
+    const bool pk = *std::max_element(P.cols.begin(), P.cols.end()) <= kPkMaxTotal;
+    std::vector<int> lmaxes = {12, 13};
+    if (pk)
+        lmaxes.push_back(kPkL);
+    for (int lmax : lmaxes)
         for (int per_cu : {4, 8})
             for (int chunk : {1, 8})
                 cands.push_back({lmax, cus * per_cu, chunk});
@@ -381,7 +580,7 @@ void plan_autotune(BatchPlan& P) {
             plan_upload(P, c.lmax);
             cur_lmax = c.lmax;
         }
-        P.grid = (int)std::max<int64_t>(1, std::min<int64_t>(c.grid, P.total_blocks));
+        P.grid = (int)std::max<int64_t>(1, std::min<int64_t>(c.grid, P.full_blocks));
         P.chunk = c.chunk;
         plan_enqueue(P, s, false);   // warm
         BMM_HIP(hipEventRecord(a, s));
@@ -401,7 +600,7 @@ void plan_autotune(BatchPlan& P) {
     (void)hipStreamDestroy(s);
     if (bestc.lmax != cur_lmax)
         plan_upload(P, bestc.lmax);
-    P.grid = (int)std::max<int64_t>(1, std::min<int64_t>(bestc.grid, P.total_blocks));
+    P.grid = (int)std::max<int64_t>(1, std::min<int64_t>(bestc.grid, P.full_blocks));
     P.chunk = bestc.chunk;
 }
 
@@ -451,7 +650,9 @@ BatchPlan* batch_plan_create(const uint64_t* d_words, const int64_t* word_offset
         if (it != tune_cache().end() && persistent_wgs == 0)
             std::tie(lmax, grid, chunk) = it->second;
         plan_upload(*P, lmax);
-        P->grid = (int)std::max<int64_t>(1, std::min<int64_t>(grid, P->total_blocks));
+        P->grid = (int)std::max<int64_t>(1, std::min<int64_t>(grid, P->full_blocks));
+        if (it == tune_cache().end() || persistent_wgs != 0)
+            chunk = P->full_blocks > (int64_t)P->grid * 64 ? 8 : 1;
         P->chunk = chunk;
     }
     return P.release();
@@ -471,6 +672,28 @@ void batch_plan_run(BatchPlan* P, hipStream_t stream, bool use_graph) {
     }
     ScopedKernelTimer timer(stream);
     BMM_HIP(hipGraphLaunch(P->exec, stream));
+}
+
+void batch_plan_set_shape(BatchPlan* P, int max_block_bits, int persistent_wgs, int chunk) {
+    if (max_block_bits != 12 && max_block_bits != 13 && max_block_bits != kPkL)
+        fail(BMM_ERR_INVALID, "batch_plan_set_shape: max_block_bits must be 12, 13 or 14");
+    if (max_block_bits == kPkL &&
+        *std::max_element(P->cols.begin(), P->cols.end()) > kPkMaxTotal)
+        fail(BMM_ERR_INVALID,
+             "batch_plan_set_shape: 14-bit (int16) blocks need at most 32767 columns");
+    if (persistent_wgs < 0 || chunk < 0)
+        fail(BMM_ERR_INVALID, "batch_plan_set_shape: negative launch size");
+    if (P->exec) {
+        BMM_HIP(hipStreamSynchronize(P->cap_stream));
+        BMM_HIP(hipGraphExecDestroy(P->exec));
+        BMM_HIP(hipGraphDestroy(P->graph));
+        P->exec = nullptr;
+        P->graph = nullptr;
+    }
+    plan_upload(*P, max_block_bits);
+    const int grid = persistent_wgs ? persistent_wgs : default_persistent_wgs();
+    P->grid = (int)std::max<int64_t>(1, std::min<int64_t>(grid, P->full_blocks));
+    P->chunk = chunk ? chunk : (P->full_blocks > (int64_t)P->grid * 64 ? 8 : 1);
 }
 
 void batch_plan_destroy(BatchPlan* P) { delete P; }

@@ -95,6 +95,7 @@ BatchPlan* batch_plan_create(const uint64_t* d_words, const int64_t* word_offset
                              int persistent_wgs, int autotune);
 void batch_plan_run(BatchPlan* P, hipStream_t stream, bool use_graph);
 void batch_plan_destroy(BatchPlan* P);
+void batch_plan_set_shape(BatchPlan* P, int max_block_bits, int persistent_wgs, int chunk);
 void batch_plan_info(const BatchPlan* P, bmm_batch_info* out);
 std::vector<bmm_topk_entry> fwht_topk_rows(const uint64_t* rows, size_t t, size_t words,
                                            size_t cols, size_t k);
@@ -334,6 +335,18 @@ int bmm_hip_batch_plan_info(const void* plan, bmm_batch_info* out) {
         if (!plan || !out)
             fail(BMM_ERR_INVALID, "batch_plan_info: null pointer");
         batch_plan_info(static_cast<const BatchPlan*>(plan), out);
+    });
+}
+
+int bmm_hip_batch_plan_set_shape(void* plan, int max_block_bits, size_t persistent_wgs,
+                                 size_t chunk) {
+    return guarded([&] {
+        if (!plan)
+            fail(BMM_ERR_INVALID, "batch_plan_set_shape: null handle");
+        if (persistent_wgs > (1u << 20) || chunk > (1u << 20))
+            fail(BMM_ERR_INVALID, "batch_plan_set_shape: launch size too large");
+        batch_plan_set_shape(static_cast<BatchPlan*>(plan), max_block_bits, (int)persistent_wgs,
+                             (int)chunk);
     });
 }
 

@@ -131,6 +131,13 @@ __device__ inline uint32_t wave_min_u32(uint32_t v) {
     return v;
 }
 
+__device__ inline int32_t wave_max_i32(int32_t v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+        v = max(v, __shfl_xor(v, off, kWave));
+    return v;
+}
+
 __device__ inline uint32_t wave_sum_u32(uint32_t v) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1)

@@ -13,20 +13,32 @@
 
 namespace bmmhip {
 
+__host__ __device__ constexpr int ilog2c(int n) { return n <= 1 ? 0 : 1 + ilog2c(n / 2); }
 __host__ __device__ constexpr int fwht_pad(int x) { return x + (x >> 5); }
 __host__ __device__ constexpr int fwht_lds_words(int L) { return (1 << L) + ((1 << L) >> 5) + 1; }
+
+// Padded distance between the entries one round combines.  Rounds start at
+// LO in {0, 5, 10, 15}: for LO >= 5, adding e << LO moves the padded index by
+// e * (2^LO + 2^(LO-5)); for LO = 0 the group base is a multiple of 32 (or 0)
+// and e < 32, so the padded index moves by e.  One base address per group then
+// serves all 2^R entries through immediate offsets.
+template <int LO> __host__ __device__ constexpr int fwht_pad_stride() {
+    static_assert(LO % 5 == 0, "rounds start at multiples of 5 bits");
+    return (1 << LO) + ((1 << LO) >> 5);
+}
 
 template <int L, int LO, int R> __device__ inline void fwht_round(int32_t* s) {
     constexpr int G = 1 << (L - R);
     constexpr int E = 1 << R;
+    constexpr int ST = fwht_pad_stride<LO>();
     for (int g = threadIdx.x; g < G; g += blockDim.x) {
         const int low = g & ((1 << LO) - 1);
         const int high = g >> LO;
-        const int base = (high << (LO + R)) | low;
+        int32_t* p = s + fwht_pad((high << (LO + R)) | low);
         uint32_t v[E];
 #pragma unroll
         for (int e = 0; e < E; ++e)
-            v[e] = (uint32_t)s[fwht_pad(base + (e << LO))];
+            v[e] = (uint32_t)p[e * ST];
 #pragma unroll
         for (int len = 1; len < E; len <<= 1)
 #pragma unroll
@@ -38,7 +50,7 @@ template <int L, int LO, int R> __device__ inline void fwht_round(int32_t* s) {
                 }
 #pragma unroll
         for (int e = 0; e < E; ++e)
-            s[fwht_pad(base + (e << LO))] = (int32_t)v[e];
+            p[e * ST] = (int32_t)v[e];
     }
 }
 
@@ -54,9 +66,9 @@ template <int L, int LO = 0> __device__ inline void block_fwht(int32_t* s) {
     }
 }
 
-// Like block_fwht, but the last round hands its results to
-// visit(index, value) instead of storing them: thread t sees the entries
-// ml = index * 256 + t, index in [0, 2^L / 256).  Needs L >= 11 (the last
+// Like block_fwht, but the last round hands its results to visit(q, v)
+// instead of storing them: v[e] (e < 2^R) is entry ml = (q + P * e) * 256 + t
+// for thread t, with P = 2^(L-R-8) and q in [0, P).  Needs L >= 11 (the last
 // round then spans >= 256 groups, one or more per thread, with stride >= 256);
 // starts with a barrier and leaves LDS holding the pre-last-round values.
 template <int L, int LO, typename Visit> __device__ inline void block_fwht_visit(int32_t* s,
@@ -74,11 +86,11 @@ template <int L, int LO, typename Visit> __device__ inline void block_fwht_visit
         constexpr int PER = G / 256;        // groups per thread
 #pragma unroll
         for (int q = 0; q < PER; ++q) {
-            const int g = (int)threadIdx.x + 256 * q;
+            const int32_t* p = s + fwht_pad((int)threadIdx.x + 256 * q);
             uint32_t v[E];
 #pragma unroll
             for (int e = 0; e < E; ++e)
-                v[e] = (uint32_t)s[fwht_pad(g + (e << LO))];
+                v[e] = (uint32_t)p[e * fwht_pad_stride<LO>()];
 #pragma unroll
             for (int len = 1; len < E; len <<= 1)
 #pragma unroll
@@ -88,10 +100,28 @@ template <int L, int LO, typename Visit> __device__ inline void block_fwht_visit
                         v[i] = a + b;
                         v[i + len] = a - b;
                     }
-#pragma unroll
-            for (int e = 0; e < E; ++e)
-                visit(q + PER * e, (int32_t)v[e]);
+            visit(q, v);
         }
+    }
+}
+
+// Subset maxima of x[0..N): returns the maximum of all entries and sets
+// bm[j] = max of x[i] over the i with bit j set (j < log2 N), in
+// 2N - log2(N) - 2 max operations; x is overwritten.
+template <int N> __device__ inline int32_t subset_max(int32_t* x, int32_t* bm) {
+    if constexpr (N == 1) {
+        return x[0];
+    } else {
+        constexpr int H = N / 2;
+        int32_t top = x[H];
+#pragma unroll
+        for (int i = H + 1; i < N; ++i)
+            top = max(top, x[i]);
+        bm[ilog2c(H)] = top;
+#pragma unroll
+        for (int i = 0; i < H; ++i)
+            x[i] = max(x[i], x[i + H]);
+        return subset_max<H>(x, bm);
     }
 }
 

@@ -28,8 +28,10 @@ struct WinTarget {
     int32_t n_pat;
     int32_t bits;     // window size n
     int32_t total;
-    int32_t L;
-    int64_t first_block;  // global number of this target's block 0
+    int32_t L;        // (total + 1) << L < 2^31 (packed block keys below)
+    int64_t first_block;  // number of this target's block 0 in its kernel's queue
+    int32_t slot;     // output index
+    int32_t pad;
 };
 
 constexpr int kWinMaxL = 13;
@@ -49,42 +51,67 @@ __device__ inline void block_build_g_signed(int32_t* s, const uint64_t* __restri
     }
 }
 
+// One block mh: transform, then the block's best element as the packed key
+// K = H * 2^L - ml (maximum H first, then lowest ml; |H| <= total keeps it in
+// int32), and one atomicMin of (weight << n | mh << L | ml) into the key.
 template <int L>
 __device__ void win_block(int32_t* s, const WinTarget& T, const uint64_t* pats,
                           const int32_t* mults, uint64_t mh, unsigned long long* key) {
     block_build_g_signed<L>(s, pats + T.pat_off, mults + T.pat_off, T.n_pat, mh);
-    block_fwht<L>(s);
     constexpr int N = 1 << L;
-    constexpr int E = N >= 256 ? N / 256 : 1;
-    uint32_t bw = 0xffffffffu, bml = 0;
-    if ((int)threadIdx.x < N) {
+    constexpr int NE = N >= 256 ? N / 256 : 1;   // register values per thread
+    int32_t best = INT32_MIN;
+    // v[e] is register index q + PER * e, i.e. ml = (q + PER * e) * 256 + tid.
+    auto visit = [&](int q, const auto& v) {
+        constexpr int E = sizeof(v) / sizeof(v[0]);
+        constexpr int PER = NE / E;
 #pragma unroll
-        for (int e = 0; e < E; ++e) {
-            const int ml = e * 256 + (int)threadIdx.x;
-            const uint32_t w = (uint32_t)(T.total - s[fwht_pad(ml)]) >> 1;
-            if (w < bw) {   // elements visited in increasing ml: first minimum kept
-                bw = w;
-                bml = (uint32_t)ml;
-            }
+        for (int e = 0; e < E; ++e)
+            best = max(best, (int32_t)(v[e] << L) - 256 * (q + PER * e));
+    };
+    if constexpr (L >= 11) {
+        block_fwht_visit<L, 0>(s, visit);
+    } else {
+        block_fwht<L>(s);
+        if ((int)threadIdx.x < N) {
+            uint32_t v[NE];
+#pragma unroll
+            for (int e = 0; e < NE; ++e)
+                v[e] = (uint32_t)s[fwht_pad(e * 256 + (int)threadIdx.x)];
+            visit(0, v);
         }
     }
-    unsigned long long k = bw == 0xffffffffu
-                               ? ~0ull
-                               : (((unsigned long long)bw << T.bits) | (mh << L) | bml);
-    k = wave_min_u64(k);
-    if ((threadIdx.x % kWave) == 0 && k != ~0ull)
-        atomicMin(key, k);
+    if (best != INT32_MIN)
+        best -= (int)threadIdx.x;
+    best = wave_max_i32(best);
+    __shared__ int32_t wbest[4];
+    if (threadIdx.x % kWave == 0)
+        wbest[threadIdx.x / kWave] = best;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int32_t k = max(max(wbest[0], wbest[1]), max(wbest[2], wbest[3]));
+        // k = H * 2^L - ml with 0 <= ml < 2^L: H = ceil(k / 2^L).
+        const int32_t h = -((-k) >> L);
+        const uint32_t ml = (uint32_t)((h << L) - k);
+        const uint32_t w = (uint32_t)(T.total - h) >> 1;
+        atomicMin(key, ((unsigned long long)w << T.bits) | (mh << L) | ml);
+    }
+    __syncthreads();
 }
 
-__global__ __launch_bounds__(256) void win_min_kernel(const WinTarget* __restrict__ tgts,
-                                                      int n_tgt, int64_t total_blocks,
-                                                      const uint64_t* __restrict__ pats,
-                                                      const int32_t* __restrict__ mults,
-                                                      int chunk,
-                                                      unsigned long long* __restrict__ counter,
-                                                      unsigned long long* __restrict__ keys) {
-    __shared__ int32_t s[fwht_lds_words(kWinMaxL)];
+// Blocks of the targets whose blocks are 2^L wide (the persistent queue of
+// one kernel); targets in queue order, found by scanning forward.
+template <int L>
+__global__ __launch_bounds__(256) void win_full_kernel(const WinTarget* __restrict__ tgts,
+                                                       int n_tgt, int64_t total_blocks,
+                                                       const uint64_t* __restrict__ pats,
+                                                       const int32_t* __restrict__ mults,
+                                                       int chunk,
+                                                       unsigned long long* __restrict__ counter,
+                                                       unsigned long long* __restrict__ keys) {
+    __shared__ __align__(16) int32_t s[fwht_lds_words(kWinMaxL) + 4];
     __shared__ long long first;
+    int ti = 0;
     for (;;) {
         if (threadIdx.x == 0)
             first = (long long)atomicAdd(counter, (unsigned long long)chunk);
@@ -94,9 +121,35 @@ __global__ __launch_bounds__(256) void win_min_kernel(const WinTarget* __restric
         if (g0 >= total_blocks)
             break;
         const int64_t g1 = min(g0 + (int64_t)chunk, total_blocks);
-        int ti = 0;
-        while (ti + 1 < n_tgt && tgts[ti + 1].first_block <= g0)
-            ++ti;
+        for (int64_t g = g0; g < g1; ++g) {
+            while (ti + 1 < n_tgt && tgts[ti + 1].first_block <= g)
+                ++ti;
+            const WinTarget T = tgts[ti];
+            win_block<L>(s, T, pats, mults, (uint64_t)(g - T.first_block), &keys[T.slot]);
+        }
+    }
+}
+
+// Targets with narrower blocks (small windows or very many columns).
+__global__ __launch_bounds__(256) void win_min_kernel(const WinTarget* __restrict__ tgts,
+                                                      int n_tgt, int64_t total_blocks,
+                                                      const uint64_t* __restrict__ pats,
+                                                      const int32_t* __restrict__ mults,
+                                                      int chunk,
+                                                      unsigned long long* __restrict__ counter,
+                                                      unsigned long long* __restrict__ keys) {
+    __shared__ __align__(16) int32_t s[fwht_lds_words(kWinMaxL) + 4];
+    __shared__ long long first;
+    int ti = 0;
+    for (;;) {
+        if (threadIdx.x == 0)
+            first = (long long)atomicAdd(counter, (unsigned long long)chunk);
+        __syncthreads();
+        const int64_t g0 = first;
+        __syncthreads();
+        if (g0 >= total_blocks)
+            break;
+        const int64_t g1 = min(g0 + (int64_t)chunk, total_blocks);
         for (int64_t g = g0; g < g1; ++g) {
             while (ti + 1 < n_tgt && tgts[ti + 1].first_block <= g)
                 ++ti;
@@ -104,14 +157,12 @@ __global__ __launch_bounds__(256) void win_min_kernel(const WinTarget* __restric
             const uint64_t mh = (uint64_t)(g - T.first_block);
             switch (T.L) {
 #define BMM_CASE(LL)                                                                           \
-    case LL: win_block<LL>(s, T, pats, mults, mh, &keys[ti]); break;
+    case LL: win_block<LL>(s, T, pats, mults, mh, &keys[T.slot]); break;
                 BMM_CASE(1) BMM_CASE(2) BMM_CASE(3) BMM_CASE(4) BMM_CASE(5) BMM_CASE(6)
                 BMM_CASE(7) BMM_CASE(8) BMM_CASE(9) BMM_CASE(10) BMM_CASE(11) BMM_CASE(12)
-                BMM_CASE(13)
 #undef BMM_CASE
             default: break;
             }
-            __syncthreads();
         }
     }
 }
@@ -142,7 +193,6 @@ void window_coset_min(const uint64_t* rows, size_t n_rows, size_t words, size_t 
     std::vector<WinTarget> tg(n_targets);
     std::vector<uint64_t> all_p;
     std::vector<int32_t> all_m;
-    int64_t blocks = 0;
     const uint64_t tail = cols % 64 ? ((1ull << (cols % 64)) - 1) : ~0ull;
     for (size_t i = 0; i < n_targets; ++i) {
         const int t = targets[i];
@@ -193,33 +243,58 @@ void window_coset_min(const uint64_t* rows, size_t n_rows, size_t words, size_t 
         T.n_pat = (int32_t)(all_p.size() - T.pat_off);
         T.bits = (int32_t)width;
         T.total = total;
-        T.L = (int32_t)std::min<size_t>(width, kWinMaxL);
-        T.first_block = blocks;
-        blocks += int64_t(1) << (width - T.L);
+        int L = (int)std::min<size_t>(width, kWinMaxL);
+        while (L > 1 && ((int64_t)total + 1) << L >= (int64_t(1) << 31))
+            --L;
+        T.L = L;
+        T.slot = (int32_t)i;
+        T.pad = 0;
     }
     if (n_targets == 0)
         return;
+    // Queue order: targets with 2^kWinMaxL-wide blocks (one kernel), then the
+    // rest (another); block numbers restart per queue.
+    std::stable_partition(tg.begin(), tg.end(),
+                          [](const WinTarget& T) { return T.L == kWinMaxL; });
+    size_t n_full = 0;
+    int64_t full_blocks = 0, rest_blocks = 0;
+    for (WinTarget& T : tg) {
+        int64_t& q = T.L == kWinMaxL ? full_blocks : rest_blocks;
+        n_full += T.L == kWinMaxL;
+        T.first_block = q;
+        q += int64_t(1) << (T.bits - T.L);
+    }
     WinWorkspace& ws = win_ws();
     WinTarget* d_t = ws.tgts.get<WinTarget>(n_targets);
     uint64_t* d_p = ws.pats.get<uint64_t>(std::max<size_t>(all_p.size(), 1));
     int32_t* d_m = ws.mults.get<int32_t>(std::max<size_t>(all_m.size(), 1));
     unsigned long long* d_k = ws.keys.get<unsigned long long>(n_targets);
-    unsigned long long* d_c = ws.counter.get<unsigned long long>(1);
+    unsigned long long* d_c = ws.counter.get<unsigned long long>(2);
     BMM_HIP(hipMemcpy(d_t, tg.data(), sizeof(WinTarget) * n_targets, hipMemcpyHostToDevice));
     if (!all_p.empty()) {
         BMM_HIP(hipMemcpy(d_p, all_p.data(), all_p.size() * 8, hipMemcpyHostToDevice));
         BMM_HIP(hipMemcpy(d_m, all_m.data(), all_m.size() * 4, hipMemcpyHostToDevice));
     }
     fill_u64(reinterpret_cast<uint64_t*>(d_k), n_targets, ~0ull, nullptr);
-    fill_u64(reinterpret_cast<uint64_t*>(d_c), 1, 0ull, nullptr);
-    const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(blocks, default_persistent_wgs()));
-    const int chunk = blocks > (int64_t)grid * 64 ? 16 : 1;
-    {
+    fill_u64(reinterpret_cast<uint64_t*>(d_c), 2, 0ull, nullptr);
+    auto shape = [](int64_t blocks, int& grid, int& chunk) {
+        grid = (int)std::max<int64_t>(1, std::min<int64_t>(blocks, default_persistent_wgs()));
+        chunk = blocks > (int64_t)grid * 64 ? 16 : 1;
+    };
+    int grid, chunk;
+    if (full_blocks > 0) {
+        shape(full_blocks, grid, chunk);
         ScopedKernelTimer timer(nullptr);
-        hipLaunchKernelGGL(win_min_kernel, dim3(grid), dim3(256), 0, nullptr, d_t,
-                           (int)n_targets, blocks, d_p, d_m, chunk, d_c, d_k);
+        hipLaunchKernelGGL(win_full_kernel<kWinMaxL>, dim3(grid), dim3(256), 0, nullptr, d_t,
+                           (int)n_full, full_blocks, d_p, d_m, chunk, d_c, d_k);
+        BMM_HIP(hipGetLastError());
     }
-    BMM_HIP(hipGetLastError());
+    if (rest_blocks > 0) {
+        shape(rest_blocks, grid, chunk);
+        hipLaunchKernelGGL(win_min_kernel, dim3(grid), dim3(256), 0, nullptr, d_t + n_full,
+                           (int)(n_targets - n_full), rest_blocks, d_p, d_m, chunk, d_c + 1, d_k);
+        BMM_HIP(hipGetLastError());
+    }
     std::vector<unsigned long long> keys(n_targets);
     BMM_HIP(hipMemcpy(keys.data(), d_k, n_targets * 8, hipMemcpyDeviceToHost));
     const uint64_t mmask = (1ull << width) - 1;

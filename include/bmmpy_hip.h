@@ -152,6 +152,12 @@ int bmm_hip_batch_plan_create(const uint64_t *d_row_words, const int64_t *word_o
                               void **plan);
 int bmm_hip_batch_plan_run(void *plan, int use_graph, void *stream);
 int bmm_hip_batch_plan_info(const void *plan, bmm_batch_info *out);
+/* Fixes the launch shape instead of the autotuned one: max_block_bits = 12
+ * or 13 (int32 blocks of up to 2^bits masks) or 14 (packed int16 blocks,
+ * every matrix at most 32767 columns); persistent_wgs / chunk = 0 pick
+ * defaults.  Drops a captured graph (recaptured on the next graph run). */
+int bmm_hip_batch_plan_set_shape(void *plan, int max_block_bits, size_t persistent_wgs,
+                                 size_t chunk);
 int bmm_hip_batch_plan_destroy(void *plan);
 
 /* ---- single-target coset minimum over a wide window (planted MITM

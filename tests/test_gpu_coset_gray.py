@@ -148,3 +148,96 @@ def test_batch_plan_graph_and_autotune(lib, ora):
         o = out_off[m]
         assert np.array_equal(w1[o:o + rows[m]], ew)
         assert np.array_equal(i1[o:o + rows[m]], ei)
+
+
+def _plan_run(lib, words, offs, rows, cols, bits, graph):
+    """Plan with a fixed block size; returns (weights, indices)."""
+    import ctypes as C
+    import torch
+    out_off = np.concatenate([[0], np.cumsum(rows)[:-1]]).astype(np.int64)
+    n = int(rows.sum())
+    dev = torch.device("cuda:0")
+    d_words = torch.from_numpy(words.view(np.int64).copy()).to(dev)
+    d_w = torch.full((n,), -1, dtype=torch.int32, device=dev)
+    d_i = torch.full((n,), -1, dtype=torch.int64, device=dev)
+    plan = C.c_void_p()
+    lib.call("bmm_hip_batch_plan_create", d_words.data_ptr(), offs, rows, cols, rows.size,
+             out_off, d_w.data_ptr(), d_i.data_ptr(), 0, 0, C.byref(plan))
+    try:
+        lib.call("bmm_hip_batch_plan_set_shape", plan, bits, 0, 0)
+        info = lib.BatchInfo()
+        lib.call("bmm_hip_batch_plan_info", plan, C.byref(info))
+        assert info.max_block_bits == bits
+        for _ in range(2 if graph else 1):
+            lib.call("bmm_hip_batch_plan_run", plan, int(graph), None)
+        torch.cuda.synchronize()
+        return d_w.cpu().numpy().view(np.uint32), d_i.cpu().numpy().view(np.uint64)
+    finally:
+        lib.call("bmm_hip_batch_plan_destroy", plan)
+
+
+def _check_small(ora, words, offs, rows, cols, w, idx, max_rows):
+    out_off = np.concatenate([[0], np.cumsum(rows)[:-1]])
+    checked = 0
+    for m in range(rows.size):
+        if rows[m] > max_rows:
+            continue
+        W = (int(cols[m]) + 63) // 64
+        mat = words[offs[m]: offs[m] + rows[m] * W].reshape(rows[m], W)
+        ew, ei = ora.coset_min(mat, int(cols[m]))
+        o = out_off[m]
+        assert np.array_equal(w[o:o + rows[m]], ew), (m, rows[m], cols[m])
+        assert np.array_equal(idx[o:o + rows[m]], ei), (m, rows[m], cols[m])
+        checked += 1
+    return checked
+
+
+@pytest.mark.parametrize("bits", [12, 13, 14])
+def test_batch_plan_block_sizes(lib, ora, bits):
+    """Every block size (int32 2^12 / 2^13 blocks, packed-int16 2^14 blocks)
+    against the oracle on the matrices of <= 20 rows, and against the 2^13
+    result on all matrices (up to 28 rows), eager and graph replay."""
+    _say("batch block size", bits)
+    rows, cols, dens, words, offs = lib.gen_batch(SEED_C4, 160)
+    w, i = _plan_run(lib, words, offs, rows, cols, bits, graph=bits == 14)
+    assert _check_small(ora, words, offs, rows, cols, w, i, 20) > 60
+    w13, i13 = _plan_run(lib, words, offs, rows, cols, 13, graph=False)
+    assert np.array_equal(w, w13) and np.array_equal(i, i13)
+
+
+@pytest.mark.parametrize("bits", [13, 14])
+def test_batch_wide_matrices_read_patterns_from_memory(lib, ora, bits):
+    """Matrices wider than 2048 columns (column patterns read from memory
+    instead of the per-thread cache), including a tail word and a matrix
+    exactly at the block size."""
+    _say("batch wide", bits)
+    shapes = [(16, 4096, 0.3), (15, 2561, 0.5), (14, 3000, 0.1), (17, 2049, 0.45), (9, 5000, 0.2)]
+    mats = [lib.gen_bernoulli(0x5EED0100 + k, r, c, p) for k, (r, c, p) in enumerate(shapes)]
+    rows = np.array([s[0] for s in shapes], np.int32)
+    cols = np.array([s[1] for s in shapes], np.int32)
+    offs = np.concatenate([[0], np.cumsum([m.size for m in mats])[:-1]]).astype(np.int64)
+    words = np.concatenate([m.reshape(-1) for m in mats])
+    w, i = _plan_run(lib, words, offs, rows, cols, bits, graph=False)
+    assert _check_small(ora, words, offs, rows, cols, w, i, 17) == len(shapes)
+
+
+def test_batch_int16_blocks_refuse_wide_matrices(lib):
+    import ctypes as C
+    import torch
+    rows = np.array([14], np.int32)
+    cols = np.array([32768], np.int32)
+    words = lib.gen_bernoulli(7, 14, 32768, 0.01).reshape(-1)
+    d_words = torch.from_numpy(words.view(np.int64).copy()).to("cuda:0")
+    d_w = torch.empty(14, dtype=torch.int32, device="cuda:0")
+    d_i = torch.empty(14, dtype=torch.int64, device="cuda:0")
+    plan = C.c_void_p()
+    lib.call("bmm_hip_batch_plan_create", d_words.data_ptr(), np.array([0], np.int64), rows,
+             cols, 1, np.array([0], np.int64), d_w.data_ptr(), d_i.data_ptr(), 0, 0,
+             C.byref(plan))
+    try:
+        with pytest.raises(ValueError, match="32767"):
+            lib.call("bmm_hip_batch_plan_set_shape", plan, 14, 0, 0)
+        with pytest.raises(ValueError):
+            lib.call("bmm_hip_batch_plan_set_shape", plan, 11, 0, 0)
+    finally:
+        lib.call("bmm_hip_batch_plan_destroy", plan)
