@@ -115,8 +115,12 @@ __device__ void coset_block_finish(int32_t* s, const CosetMatrix& M, uint32_t mh
         const bool has = tid < L || ((mh >> (tid - L)) & 1u);
         const int32_t h = res[tid < L ? tid : L];
         if (has && h != INT32_MIN) {
+            // Keys only decrease, so a stale read never skips an improvement.
             const uint32_t w = (uint32_t)(M.total - h) >> 1;
-            atomicMin(&keys[M.out_off + tid], ((unsigned long long)w << 32) | mh);
+            const unsigned long long k = ((unsigned long long)w << 32) | mh;
+            unsigned long long* dst = &keys[M.out_off + tid];
+            if (k < __atomic_load_n(dst, __ATOMIC_RELAXED))
+                atomicMin(dst, k);
         }
     }
     __syncthreads();
@@ -604,6 +608,25 @@ void plan_autotune(BatchPlan& P) {
     P.chunk = bestc.chunk;
 }
 
+// Untuned block size: the largest of 14 (packed int16, when every matrix has
+// at most 32767 columns), 13, 12 that still gives at least two blocks per
+// CU; else 12.
+int default_block_bits(const BatchPlan& P) {
+    const int64_t want = 2 * (default_persistent_wgs() / 8);
+    const bool pk = *std::max_element(P.cols.begin(), P.cols.end()) <= kPkMaxTotal;
+    for (int bits : {kPkL, 13, 12}) {
+        if (bits == kPkL && !pk)
+            continue;
+        int64_t blocks = 0;
+        for (size_t m = 0; m < P.n_mat; ++m)
+            if (P.rows[m] >= bits)
+                blocks += int64_t(1) << (P.rows[m] - bits);
+        if (blocks >= want)
+            return bits;
+    }
+    return 12;
+}
+
 // Launch shapes chosen by autotuning, keyed by the batch's shape signature.
 std::map<std::vector<int64_t>, std::tuple<int, int, int>>& tune_cache() {
     static std::map<std::vector<int64_t>, std::tuple<int, int, int>> c;
@@ -645,7 +668,8 @@ BatchPlan* batch_plan_create(const uint64_t* d_words, const int64_t* word_offset
         plan_autotune(*P);
         tune_cache()[sig] = {P->lmax, P->grid, P->chunk};
     } else {
-        int lmax = kBatchMaxL, grid = persistent_wgs ? persistent_wgs : default_persistent_wgs();
+        int lmax = default_block_bits(*P), grid = persistent_wgs ? persistent_wgs
+                                                                 : default_persistent_wgs();
         int chunk = 1;
         if (it != tune_cache().end() && persistent_wgs == 0)
             std::tie(lmax, grid, chunk) = it->second;

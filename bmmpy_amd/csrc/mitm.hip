@@ -17,6 +17,7 @@
 // atomicMin, so the final key carries the minimum weight and its lowest mask.
 #include "common.hpp"
 #include "fwht_block.hpp"
+#include "fwht_pk16.hpp"
 
 #include <algorithm>
 #include <vector>
@@ -94,7 +95,69 @@ __device__ void win_block(int32_t* s, const WinTarget& T, const uint64_t* pats,
         const int32_t h = -((-k) >> L);
         const uint32_t ml = (uint32_t)((h << L) - k);
         const uint32_t w = (uint32_t)(T.total - h) >> 1;
-        atomicMin(key, ((unsigned long long)w << T.bits) | (mh << L) | ml);
+        const unsigned long long nk = ((unsigned long long)w << T.bits) | (mh << L) | ml;
+        if (nk < __atomic_load_n(key, __ATOMIC_RELAXED))  // keys only decrease
+            atomicMin(key, nk);
+    }
+    __syncthreads();
+}
+
+// Packed-int16 block of 2^14 masks (windows of >= 14 rows, at most 32767
+// active columns).  Word w = (q + 4 e) * 256 + t holds masks w and
+// w + 2^13.  Each entry becomes the key H * 2^16 | (65535 - 4 m') with
+// m' = ml - t (one v_lshl_or / v_and_or), so one max keeps the largest H and
+// the lowest ml among the thread's entries.
+__device__ void win_block_pk16(uint32_t* s, const WinTarget& T, const uint64_t* __restrict__ pats,
+                               const int32_t* __restrict__ mults, uint64_t mh,
+                               unsigned long long* key) {
+    {
+        int4* s4 = reinterpret_cast<int4*>(s);
+        const int b = (int)kPkBias;
+        for (int i = threadIdx.x; i < (pk_lds_words() + 3) / 4; i += blockDim.x)
+            s4[i] = make_int4(b, b, b, b);
+        __syncthreads();
+        const uint64_t* pat = pats + T.pat_off;
+        const int32_t* mult = mults + T.pat_off;
+        for (int i = threadIdx.x; i < T.n_pat; i += blockDim.x) {
+            const uint64_t p = pat[i];
+            const int32_t m = mult[i];
+            const uint32_t pl = (uint32_t)p & ((1u << kPkL) - 1u);
+            const int neg = __builtin_popcountll((p >> kPkL) & mh) & 1;
+            const uint32_t v = (uint32_t)(neg ? -m : m);
+            atomicAdd(&s[fwht_pad((int)(pl & ((1u << kPkW) - 1u)))],
+                      (pl >> kPkW) ? v << 16 : v);
+        }
+    }
+    int32_t best = INT32_MIN;
+    pk_block_fwht_visit(s, [&](int q, const uint32_t (&v)[8]) {
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const uint32_t m_lo = (uint32_t)(256 * (q + 4 * e));
+            const uint32_t c_lo = 65535u - 4u * m_lo;
+            const uint32_t c_hi = 65535u - 4u * (m_lo + (1u << kPkW));
+            best = max(best, (int32_t)((v[e] << 16) | c_lo));
+            best = max(best, (int32_t)((v[e] & 0xffff0000u) | c_hi));
+        }
+    });
+    // Thread key -> (H, ml) -> block key H * 2^16 | (65535 - 4 ml).
+    {
+        const int32_t h = best >> 16;
+        const uint32_t ml = ((65535u - ((uint32_t)best & 0xffffu)) >> 2) + threadIdx.x;
+        best = (int32_t)(((uint32_t)h << 16) | (65535u - 4u * ml));
+    }
+    best = wave_max_i32(best);
+    __shared__ int32_t wbest[4];
+    if (threadIdx.x % kWave == 0)
+        wbest[threadIdx.x / kWave] = best;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int32_t k = max(max(wbest[0], wbest[1]), max(wbest[2], wbest[3]));
+        const int32_t h = k >> 16;
+        const uint32_t ml = (65535u - ((uint32_t)k & 0xffffu)) >> 2;
+        const uint32_t w = (uint32_t)(T.total - h) >> 1;
+        const unsigned long long nk = ((unsigned long long)w << T.bits) | (mh << kPkL) | ml;
+        if (nk < __atomic_load_n(key, __ATOMIC_RELAXED))  // keys only decrease
+            atomicMin(key, nk);
     }
     __syncthreads();
 }
@@ -109,7 +172,8 @@ __global__ __launch_bounds__(256) void win_full_kernel(const WinTarget* __restri
                                                        int chunk,
                                                        unsigned long long* __restrict__ counter,
                                                        unsigned long long* __restrict__ keys) {
-    __shared__ __align__(16) int32_t s[fwht_lds_words(kWinMaxL) + 4];
+    constexpr int kWords = L == kPkL ? pk_lds_words() : fwht_lds_words(L);
+    __shared__ __align__(16) int32_t s[kWords + 4];
     __shared__ long long first;
     int ti = 0;
     for (;;) {
@@ -125,7 +189,12 @@ __global__ __launch_bounds__(256) void win_full_kernel(const WinTarget* __restri
             while (ti + 1 < n_tgt && tgts[ti + 1].first_block <= g)
                 ++ti;
             const WinTarget T = tgts[ti];
-            win_block<L>(s, T, pats, mults, (uint64_t)(g - T.first_block), &keys[T.slot]);
+            const uint64_t mh = (uint64_t)(g - T.first_block);
+            if constexpr (L == kPkL)
+                win_block_pk16(reinterpret_cast<uint32_t*>(s), T, pats, mults, mh,
+                               &keys[T.slot]);
+            else
+                win_block<L>(s, T, pats, mults, mh, &keys[T.slot]);
         }
     }
 }
@@ -246,54 +315,61 @@ void window_coset_min(const uint64_t* rows, size_t n_rows, size_t words, size_t 
         int L = (int)std::min<size_t>(width, kWinMaxL);
         while (L > 1 && ((int64_t)total + 1) << L >= (int64_t(1) << 31))
             --L;
+        if (width >= (size_t)kPkL && total <= kPkMaxTotal)
+            L = kPkL;
         T.L = L;
         T.slot = (int32_t)i;
         T.pad = 0;
     }
     if (n_targets == 0)
         return;
-    // Queue order: targets with 2^kWinMaxL-wide blocks (one kernel), then the
-    // rest (another); block numbers restart per queue.
-    std::stable_partition(tg.begin(), tg.end(),
-                          [](const WinTarget& T) { return T.L == kWinMaxL; });
-    size_t n_full = 0;
-    int64_t full_blocks = 0, rest_blocks = 0;
+    // Three queues, one kernel each: packed 2^14 blocks, int32 2^13 blocks,
+    // narrower blocks; block numbers restart per queue.
+    auto queue_of = [](const WinTarget& T) { return T.L == kPkL ? 0 : T.L == kWinMaxL ? 1 : 2; };
+    std::stable_sort(tg.begin(), tg.end(), [&](const WinTarget& a, const WinTarget& b) {
+        return queue_of(a) < queue_of(b);
+    });
+    size_t n_in[3] = {0, 0, 0};
+    int64_t q_blocks[3] = {0, 0, 0};
     for (WinTarget& T : tg) {
-        int64_t& q = T.L == kWinMaxL ? full_blocks : rest_blocks;
-        n_full += T.L == kWinMaxL;
-        T.first_block = q;
-        q += int64_t(1) << (T.bits - T.L);
+        const int qi = queue_of(T);
+        ++n_in[qi];
+        T.first_block = q_blocks[qi];
+        q_blocks[qi] += int64_t(1) << (T.bits - T.L);
     }
     WinWorkspace& ws = win_ws();
     WinTarget* d_t = ws.tgts.get<WinTarget>(n_targets);
     uint64_t* d_p = ws.pats.get<uint64_t>(std::max<size_t>(all_p.size(), 1));
     int32_t* d_m = ws.mults.get<int32_t>(std::max<size_t>(all_m.size(), 1));
     unsigned long long* d_k = ws.keys.get<unsigned long long>(n_targets);
-    unsigned long long* d_c = ws.counter.get<unsigned long long>(2);
+    unsigned long long* d_c = ws.counter.get<unsigned long long>(3);
     BMM_HIP(hipMemcpy(d_t, tg.data(), sizeof(WinTarget) * n_targets, hipMemcpyHostToDevice));
     if (!all_p.empty()) {
         BMM_HIP(hipMemcpy(d_p, all_p.data(), all_p.size() * 8, hipMemcpyHostToDevice));
         BMM_HIP(hipMemcpy(d_m, all_m.data(), all_m.size() * 4, hipMemcpyHostToDevice));
     }
     fill_u64(reinterpret_cast<uint64_t*>(d_k), n_targets, ~0ull, nullptr);
-    fill_u64(reinterpret_cast<uint64_t*>(d_c), 2, 0ull, nullptr);
+    fill_u64(reinterpret_cast<uint64_t*>(d_c), 3, 0ull, nullptr);
     auto shape = [](int64_t blocks, int& grid, int& chunk) {
         grid = (int)std::max<int64_t>(1, std::min<int64_t>(blocks, default_persistent_wgs()));
         chunk = blocks > (int64_t)grid * 64 ? 16 : 1;
     };
     int grid, chunk;
-    if (full_blocks > 0) {
-        shape(full_blocks, grid, chunk);
-        ScopedKernelTimer timer(nullptr);
-        hipLaunchKernelGGL(win_full_kernel<kWinMaxL>, dim3(grid), dim3(256), 0, nullptr, d_t,
-                           (int)n_full, full_blocks, d_p, d_m, chunk, d_c, d_k);
-        BMM_HIP(hipGetLastError());
-    }
-    if (rest_blocks > 0) {
-        shape(rest_blocks, grid, chunk);
-        hipLaunchKernelGGL(win_min_kernel, dim3(grid), dim3(256), 0, nullptr, d_t + n_full,
-                           (int)(n_targets - n_full), rest_blocks, d_p, d_m, chunk, d_c + 1, d_k);
-        BMM_HIP(hipGetLastError());
+    const WinTarget* d_q[3] = {d_t, d_t + n_in[0], d_t + n_in[0] + n_in[1]};
+    {
+        // The timed kernel is the one holding the most blocks.
+        const int timed = (int)(std::max_element(q_blocks, q_blocks + 3) - q_blocks);
+        for (int qi = 0; qi < 3; ++qi) {
+            if (q_blocks[qi] == 0)
+                continue;
+            shape(q_blocks[qi], grid, chunk);
+            ScopedKernelTimer timer(nullptr, qi == timed);
+            auto kern = qi == 0 ? win_full_kernel<kPkL>
+                        : qi == 1 ? win_full_kernel<kWinMaxL> : win_min_kernel;
+            hipLaunchKernelGGL(kern, dim3(grid), dim3(256), 0, nullptr, d_q[qi], (int)n_in[qi],
+                               q_blocks[qi], d_p, d_m, chunk, d_c + qi, d_k);
+            BMM_HIP(hipGetLastError());
+        }
     }
     std::vector<unsigned long long> keys(n_targets);
     BMM_HIP(hipMemcpy(keys.data(), d_k, n_targets * 8, hipMemcpyDeviceToHost));

@@ -66,3 +66,22 @@ def test_planted_40_wide_properties(lib):
                 acc ^= A[r]
         assert int(w[i]) == sum(bin(int(v)).count("1") for v in acc)
         assert int(w[i]) <= sum(bin(int(v)).count("1") for v in A[t])
+
+
+def test_window_queues_mixed_in_one_call(lib, ora):
+    """One call whose targets land in different kernels: a dense 16-row window
+    over 40000 columns (more than 32767 active columns: int32 blocks) and a
+    sparse one (packed int16 blocks), plus a 10-row window (narrow blocks)."""
+    _say("window mixed queues")
+    dense = lib.gen_bernoulli(0x5EED0200, 17, 40000, 0.3)
+    sparse = lib.gen_bernoulli(0x5EED0201, 17, 40000, 0.001)
+    A = np.concatenate([dense, sparse])
+    targets = [0, 17, 5]
+    windows = [list(range(1, 17)), list(range(18, 34)),
+               [1, 2, 3, 4, 6, 7, 8, 9, 10, 11] + [None] * 6]
+    # The C ABI takes one width per call: run the 10-row window separately.
+    w, m = lib.window_coset_min(A, 40000, targets[:2], windows[:2])
+    for i, t in enumerate(targets[:2]):
+        assert (int(w[i]), int(m[i])) == _oracle_window(ora, A, 40000, t, windows[i]), t
+    w3, m3 = lib.window_coset_min(A, 40000, [5], [windows[2][:10]])
+    assert (int(w3[0]), int(m3[0])) == _oracle_window(ora, A, 40000, 5, windows[2][:10])
