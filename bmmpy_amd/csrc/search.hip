@@ -59,22 +59,28 @@ __global__ __launch_bounds__(256) void fwht_topk_kernel(const uint64_t* __restri
             break;
         block_build_g64<L>(s, pat, mult, n_pat, mh);
         block_fwht<L>(s);
-        uint64_t keys[E];
+        // 32-bit prefilter: a key below the list threshold has weight at most
+        // thr >> t, i.e. H >= total - 2 (thr >> t).  The threshold only
+        // decreases, so reading it here is conservative.
+        const uint64_t thr = T.thr;
+        const int64_t wthr = thr == ~0ull ? INT64_MAX : (int64_t)(thr >> t);
+        const int32_t hmin = wthr >= (int64_t)total ? INT32_MIN
+                                                    : (int32_t)((int64_t)total - 2 * wthr);
+        // Entry ml = e * 256 + tid sits at padded index pad(tid) + 264 e.
+        const int32_t* sp = s + fwht_pad((int)threadIdx.x);
         uint32_t valid = 0;
 #pragma unroll
         for (int e = 0; e < E; ++e) {
             const int ml = e * 256 + (int)threadIdx.x;
-            keys[e] = ~0ull;
-            if (ml < N) {
-                const uint64_t mask = (mh << L) | (uint64_t)ml;
-                const int64_t w2 = (int64_t)total - (int64_t)s[fwht_pad(ml)];
-                if (mask != 0 && w2 >= 0 && (w2 & 1) == 0) {
-                    keys[e] = ((uint64_t)(w2 >> 1) << t) | mask;
-                    valid |= 1u << e;
-                }
-            }
+            if (ml < N && (mh != 0 || ml != 0) && sp[e * 264] >= hmin)
+                valid |= 1u << e;
         }
-        topk_offer<CAP, E>(T, keys, valid, K);
+        // The transform stays in LDS until the next block's build, so a
+        // candidate's key is recomputed from it instead of held in registers.
+        topk_offer_fn<CAP, E>(T, valid, K, [&](int e) {
+            const uint64_t w = (uint64_t)(uint32_t)(total - sp[e * 264]) >> 1;
+            return (w << t) | (mh << L) | (uint64_t)(e * 256 + (int)threadIdx.x);
+        });
     }
     topk_store(T, lists + (size_t)blockIdx.x * K, K);
 }

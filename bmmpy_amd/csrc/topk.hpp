@@ -54,11 +54,11 @@ template <int CAP> __device__ inline void topk_sort(TopkShared<CAP>& S, int n) {
         }
 }
 
-// Offers E keys per thread (bit e of `valid` marks keys[e] as present).
-// Must be called by every thread of the workgroup.
-template <int CAP, int E>
-__device__ inline void topk_offer(TopkShared<CAP>& S, const uint64_t (&keys)[E], uint32_t valid,
-                                  int K) {
+// Offers up to E keys per thread: bit e of `valid` marks key_of(e) as
+// present (key_of computes the packed key, so callers can keep narrower
+// values in registers).  Must be called by every thread of the workgroup.
+template <int CAP, int E, typename KeyOf>
+__device__ inline void topk_offer_fn(TopkShared<CAP>& S, uint32_t valid, int K, KeyOf&& key_of) {
     uint32_t inserted = 0;
     for (;;) {
         __syncthreads();
@@ -68,7 +68,7 @@ __device__ inline void topk_offer(TopkShared<CAP>& S, const uint64_t (&keys)[E],
 #pragma unroll
         for (int e = 0; e < E; ++e)
             if (((valid & ~inserted) >> e) & 1u)
-                if (keys[e] < thr)
+                if (key_of(e) < thr)
                     pend |= 1u << e;
         __syncthreads();
         if (threadIdx.x == 0)
@@ -82,7 +82,7 @@ __device__ inline void topk_offer(TopkShared<CAP>& S, const uint64_t (&keys)[E],
             for (int e = 0; e < E; ++e)
                 if ((pend >> e) & 1u) {
                     if (pos < cap) {
-                        S.buf[base + pos] = keys[e];
+                        S.buf[base + pos] = key_of(e);
                         inserted |= 1u << e;
                     }
                     ++pos;
@@ -109,6 +109,12 @@ __device__ inline void topk_offer(TopkShared<CAP>& S, const uint64_t (&keys)[E],
         if (took == req)
             return;
     }
+}
+
+template <int CAP, int E>
+__device__ inline void topk_offer(TopkShared<CAP>& S, const uint64_t (&keys)[E], uint32_t valid,
+                                  int K) {
+    topk_offer_fn<CAP, E>(S, valid, K, [&](int e) { return keys[e]; });
 }
 
 // Writes the list (padded with ~0) to out[0, K).
