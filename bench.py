@@ -45,6 +45,16 @@ GRAY_OPS_PER_EVAL = 66      # 32 v_xor_b32 + 32 v_bcnt_u32_b32 + 2 key minima
 VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12  # INT32 VALU, non-packed
 VALU_PK16_PEAK_TOPS = 2 * VALU_PEAK_TOPS   # packed int16 (v_pk_add_u16: 2 ops per lane)
 
+# HBM bytes per launch of each workload's dominant kernel, from rocprofv3
+# FETCH_SIZE and WRITE_SIZE passes (KiB per dispatch), each scaled by what the
+# same counter reads for a plain 256 MiB device copy (FETCH_SIZE x 2.0,
+# WRITE_SIZE x 1.0).  Re-measure when the kernel changes.
+TRAFFIC_BYTES = {
+    "gray_coset_sweep": (192.5 * 2.0 + 160.0) * 1024,
+    "coset_batch_4096": (53729.0 * 2.0 + 29953.0) * 1024,
+    "window_coset_planted_48x4096": (118.4 * 2.0 + 131137.0) * 1024,
+}
+
 
 def c2_evals() -> int:
     return sum((1 << (BLOCK - 1)) if t < BLOCK else (1 << BLOCK) for t in range(ROWS))
@@ -180,7 +190,9 @@ def run_batch_plan(torch, _lib, dev, stream, name: str, steps: int, warmup: int,
                    "plan_create_s": plan_s},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": peak,
                      "unit": "TOPS (packed int16 VALU)" if packed else "TOPS (int32 VALU)",
-                     "frac": achieved / peak, "traffic": None, "ops_per_pass": ops,
+                     "frac": achieved / peak,
+                     "traffic": TRAFFIC_BYTES.get("coset_batch_4096") if name == "batch" else None,
+                     "ops_per_pass": ops,
                      "kernel": f"coset_full_kernel<{info.max_block_bits}>",
                      "kernel_avg_ms": kernel_avg_ms, "kernel_launches": nk},
     }
@@ -259,7 +271,8 @@ def run_window(torch, _lib, steps: int, warmup: int) -> dict:
                    "subset_evals_per_step": evals, "targets_timed": steps},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": VALU_PK16_PEAK_TOPS,
                      "unit": "TOPS (packed int16 VALU)", "frac": achieved / VALU_PK16_PEAK_TOPS,
-                     "traffic": None, "ops_per_eval": 14, "kernel": "win_full_kernel<14>",
+                     "traffic": TRAFFIC_BYTES["window_coset_planted_48x4096"],
+                     "ops_per_eval": 14, "kernel": "win_full_kernel<14>",
                      "kernel_avg_ms": kernel_avg_ms, "kernel_launches": nk},
     }
 
@@ -490,7 +503,8 @@ def main() -> int:
                    "subset_evals_per_step": evals},
         "roofline": {"bound": "valu", "achieved": achieved_tops, "peak": VALU_PEAK_TOPS,
                      "unit": "TOPS (int32 VALU)", "frac": achieved_tops / VALU_PEAK_TOPS,
-                     "traffic": None, "ops_per_eval": GRAY_OPS_PER_EVAL,
+                     "traffic": TRAFFIC_BYTES["gray_coset_sweep"],
+                     "ops_per_eval": GRAY_OPS_PER_EVAL,
                      "kernel": "gray_coset_kernel<32,3>", "kernel_avg_ms": kernel_avg_ms,
                      "kernel_launches": launches},
         "result_min_weights": [int(k >> 32) for k in keys],
