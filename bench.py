@@ -210,20 +210,21 @@ def batch_cpu_baseline(_lib, name: str, budget_s: float = 10.0) -> dict:
     evals = 0.0
     done = 0
     t0 = time.perf_counter()
-    for i in range(rows.size):
-        r, c = int(rows[i]), int(cols[i])
-        if r > 22:
-            continue
-        w = (c + 63) // 64
-        m = words[offs[i]:offs[i] + r * w].reshape(r, w)
-        ora.coset_min(m, c)
-        evals += r * 2.0 ** (r - 1)
-        done += 1
-        if time.perf_counter() - t0 > budget_s:
-            break
+    while time.perf_counter() - t0 <= budget_s:   # the 20 x 128 case repeats
+        for i in range(rows.size):
+            r, c = int(rows[i]), int(cols[i])
+            if r > 22:
+                continue
+            w = (c + 63) // 64
+            m = words[offs[i]:offs[i] + r * w].reshape(r, w)
+            ora.coset_min(m, c)
+            evals += r * 2.0 ** (r - 1)
+            done += 1
+            if time.perf_counter() - t0 > budget_s:
+                break
     dt = time.perf_counter() - t0
     return {"value": evals / dt, "unit": "subset evals/s", "cores": 1, "kind": "port",
-            "sample": f"C restatement coset minimum on {done} matrices "
+            "sample": f"C restatement coset minimum on {done} matrix passes "
                       f"({'20 x 128' if name == 'coset' else 'rows <= 22 of the batch'}), "
                       f"per target histogram + FWHT + scan, one thread"}
 

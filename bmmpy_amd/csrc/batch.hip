@@ -587,12 +587,20 @@ void plan_autotune(BatchPlan& P) {
         P.grid = (int)std::max<int64_t>(1, std::min<int64_t>(c.grid, P.full_blocks));
         P.chunk = c.chunk;
         plan_enqueue(P, s, false);   // warm
-        BMM_HIP(hipEventRecord(a, s));
-        plan_enqueue(P, s, false);
-        BMM_HIP(hipEventRecord(b, s));
-        BMM_HIP(hipEventSynchronize(b));
-        float ms = 0.f;
-        BMM_HIP(hipEventElapsedTime(&ms, a, b));
+        // Best of several timed passes (one pass for passes of >= 5 ms, up to
+        // 16 for short ones, where launch jitter dominates a single sample).
+        float ms = 1e30f;
+        for (int rep = 0, reps = 1; rep < reps; ++rep) {
+            BMM_HIP(hipEventRecord(a, s));
+            plan_enqueue(P, s, false);
+            BMM_HIP(hipEventRecord(b, s));
+            BMM_HIP(hipEventSynchronize(b));
+            float t = 0.f;
+            BMM_HIP(hipEventElapsedTime(&t, a, b));
+            ms = std::min(ms, t);
+            if (rep == 0 && t < 5.f)
+                reps = 16;
+        }
         if (ms < best) {
             best = ms;
             bestc = c;
