@@ -37,6 +37,15 @@ struct WinTarget {
 
 constexpr int kWinMaxL = 13;
 
+// A window pattern prepared for packed-int16 blocks: the sign for block mh is
+// the parity of hi & mh; val (the signed multiplicity, shifted into the high
+// half for entries >= 2^13) is added to LDS word `addr` (padded index).
+struct WinPat {
+    uint64_t hi;    // pattern >> 14
+    uint32_t addr;  // fwht_pad(pattern mod 2^13)
+    int32_t val;    // mult or mult << 16
+};
+
 template <int L>
 __device__ inline void block_build_g_signed(int32_t* s, const uint64_t* __restrict__ pat,
                                             const int32_t* __restrict__ mult, int n,
@@ -107,25 +116,19 @@ __device__ void win_block(int32_t* s, const WinTarget& T, const uint64_t* pats,
 // w + 2^13.  Each entry becomes the key H * 2^16 | (65535 - 4 m') with
 // m' = ml - t (one v_lshl_or / v_and_or), so one max keeps the largest H and
 // the lowest ml among the thread's entries.
-__device__ void win_block_pk16(uint32_t* s, const WinTarget& T, const uint64_t* __restrict__ pats,
-                               const int32_t* __restrict__ mults, uint64_t mh,
-                               unsigned long long* key) {
+__device__ void win_block_pk16(uint32_t* s, const WinTarget& T, const WinPat* __restrict__ wpats,
+                               uint64_t mh, unsigned long long* key) {
     {
         int4* s4 = reinterpret_cast<int4*>(s);
         const int b = (int)kPkBias;
         for (int i = threadIdx.x; i < (pk_lds_words() + 3) / 4; i += blockDim.x)
             s4[i] = make_int4(b, b, b, b);
         __syncthreads();
-        const uint64_t* pat = pats + T.pat_off;
-        const int32_t* mult = mults + T.pat_off;
+        const WinPat* wp = wpats + T.pat_off;
         for (int i = threadIdx.x; i < T.n_pat; i += blockDim.x) {
-            const uint64_t p = pat[i];
-            const int32_t m = mult[i];
-            const uint32_t pl = (uint32_t)p & ((1u << kPkL) - 1u);
-            const int neg = __builtin_popcountll((p >> kPkL) & mh) & 1;
-            const uint32_t v = (uint32_t)(neg ? -m : m);
-            atomicAdd(&s[fwht_pad((int)(pl & ((1u << kPkW) - 1u)))],
-                      (pl >> kPkW) ? v << 16 : v);
+            const WinPat P = wp[i];
+            const uint32_t v = (uint32_t)P.val;
+            atomicAdd(&s[P.addr], (__builtin_popcountll(P.hi & mh) & 1) ? 0u - v : v);
         }
     }
     int32_t best = INT32_MIN;
@@ -169,6 +172,7 @@ __global__ __launch_bounds__(256) void win_full_kernel(const WinTarget* __restri
                                                        int n_tgt, int64_t total_blocks,
                                                        const uint64_t* __restrict__ pats,
                                                        const int32_t* __restrict__ mults,
+                                                       const WinPat* __restrict__ wpats,
                                                        int chunk,
                                                        unsigned long long* __restrict__ counter,
                                                        unsigned long long* __restrict__ keys) {
@@ -191,8 +195,7 @@ __global__ __launch_bounds__(256) void win_full_kernel(const WinTarget* __restri
             const WinTarget T = tgts[ti];
             const uint64_t mh = (uint64_t)(g - T.first_block);
             if constexpr (L == kPkL)
-                win_block_pk16(reinterpret_cast<uint32_t*>(s), T, pats, mults, mh,
-                               &keys[T.slot]);
+                win_block_pk16(reinterpret_cast<uint32_t*>(s), T, wpats, mh, &keys[T.slot]);
             else
                 win_block<L>(s, T, pats, mults, mh, &keys[T.slot]);
         }
@@ -204,6 +207,7 @@ __global__ __launch_bounds__(256) void win_min_kernel(const WinTarget* __restric
                                                       int n_tgt, int64_t total_blocks,
                                                       const uint64_t* __restrict__ pats,
                                                       const int32_t* __restrict__ mults,
+                                                      const WinPat* __restrict__ /*wpats*/,
                                                       int chunk,
                                                       unsigned long long* __restrict__ counter,
                                                       unsigned long long* __restrict__ keys) {
@@ -238,7 +242,7 @@ __global__ __launch_bounds__(256) void win_min_kernel(const WinTarget* __restric
 
 namespace {
 struct WinWorkspace {
-    DevBuf tgts, pats, mults, keys, counter;
+    DevBuf tgts, pats, mults, wpats, keys, counter;
 };
 WinWorkspace& win_ws() {
     thread_local WinWorkspace ws;
@@ -262,6 +266,7 @@ void window_coset_min(const uint64_t* rows, size_t n_rows, size_t words, size_t 
     std::vector<WinTarget> tg(n_targets);
     std::vector<uint64_t> all_p;
     std::vector<int32_t> all_m;
+    std::vector<WinPat> all_wp;   // same indexing, for packed-int16 targets
     const uint64_t tail = cols % 64 ? ((1ull << (cols % 64)) - 1) : ~0ull;
     for (size_t i = 0; i < n_targets; ++i) {
         const int t = targets[i];
@@ -306,6 +311,10 @@ void window_coset_min(const uint64_t* rows, size_t n_rows, size_t words, size_t 
             if (m != 0) {
                 all_p.push_back(p);
                 all_m.push_back(m);
+                const uint32_t pl = (uint32_t)p & ((1u << kPkL) - 1u);
+                all_wp.push_back(WinPat{p >> kPkL,
+                                        (uint32_t)fwht_pad((int)(pl & ((1u << kPkW) - 1u))),
+                                        (pl >> kPkW) ? (int32_t)((uint32_t)m << 16) : m});
             }
             a = b;
         }
@@ -341,12 +350,15 @@ void window_coset_min(const uint64_t* rows, size_t n_rows, size_t words, size_t 
     WinTarget* d_t = ws.tgts.get<WinTarget>(n_targets);
     uint64_t* d_p = ws.pats.get<uint64_t>(std::max<size_t>(all_p.size(), 1));
     int32_t* d_m = ws.mults.get<int32_t>(std::max<size_t>(all_m.size(), 1));
+    WinPat* d_wp = ws.wpats.get<WinPat>(std::max<size_t>(all_wp.size(), 1));
     unsigned long long* d_k = ws.keys.get<unsigned long long>(n_targets);
     unsigned long long* d_c = ws.counter.get<unsigned long long>(3);
     BMM_HIP(hipMemcpy(d_t, tg.data(), sizeof(WinTarget) * n_targets, hipMemcpyHostToDevice));
     if (!all_p.empty()) {
         BMM_HIP(hipMemcpy(d_p, all_p.data(), all_p.size() * 8, hipMemcpyHostToDevice));
         BMM_HIP(hipMemcpy(d_m, all_m.data(), all_m.size() * 4, hipMemcpyHostToDevice));
+        BMM_HIP(hipMemcpy(d_wp, all_wp.data(), all_wp.size() * sizeof(WinPat),
+                          hipMemcpyHostToDevice));
     }
     fill_u64(reinterpret_cast<uint64_t*>(d_k), n_targets, ~0ull, nullptr);
     fill_u64(reinterpret_cast<uint64_t*>(d_c), 3, 0ull, nullptr);
@@ -367,7 +379,7 @@ void window_coset_min(const uint64_t* rows, size_t n_rows, size_t words, size_t 
             auto kern = qi == 0 ? win_full_kernel<kPkL>
                         : qi == 1 ? win_full_kernel<kWinMaxL> : win_min_kernel;
             hipLaunchKernelGGL(kern, dim3(grid), dim3(256), 0, nullptr, d_q[qi], (int)n_in[qi],
-                               q_blocks[qi], d_p, d_m, chunk, d_c + qi, d_k);
+                               q_blocks[qi], d_p, d_m, d_wp, chunk, d_c + qi, d_k);
             BMM_HIP(hipGetLastError());
         }
     }
