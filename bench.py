@@ -380,7 +380,8 @@ def ga_compare(budget_s: float, seed: int = 0x5EED0005) -> dict:
                      "evaluations": int(st.evaluations),
                      "evals_per_s": st.evaluations / max(elapsed, 1e-9),
                      "ms_per_generation": 1e3 * elapsed / max(int(st.generations), 1),
-                     "train_loss": st.train_loss, "trajectory": traj[-50:]}
+                     "train_loss": st.train_loss, "li_moves": int(st.li_moves),
+                     "window_moves": int(st.win_moves), "trajectory": traj[-50:]}
     return res
 
 
@@ -438,6 +439,7 @@ def main() -> int:
             "scaling": "weak", "vs_baseline": None, "dtype": "u32", "data": "synthetic",
             "config": {"workload": "ga_island_planted_256x8192", "islands": 64,
                        "population": 256, "max_ops": 512, "migrate_every": 20,
+                       "window_rows": 16, "window_rounds_per_epoch": 2,
                        "budget_s": args.ga_budget},
             "learned_over_plain_time_ratio": ratio, "ga": r}))
         return 0

@@ -58,14 +58,16 @@ class GaParams(C.Structure):
                 ("train_steps", C.c_int), ("check_every", C.c_int), ("use_graph", C.c_int),
                 ("li_sweeps", C.c_int), ("rows", C.c_int), ("p_append", C.c_float),
                 ("p_replace", C.c_float), ("p_delete", C.c_float), ("learning_rate", C.c_float),
-                ("p_learned", C.c_float), ("seed", C.c_uint64)]
+                ("p_learned", C.c_float), ("seed", C.c_uint64), ("win_rows", C.c_int),
+                ("win_rounds", C.c_int), ("win_candidates", C.c_int), ("reserved", C.c_int)]
 
 
 class GaStatus(C.Structure):
     _fields_ = [("generations", C.c_size_t), ("ran", C.c_size_t), ("evaluations", C.c_uint64),
                 ("best_weight", C.c_uint64), ("best_island", C.c_int),
                 ("reached_target", C.c_int), ("elapsed_s", C.c_double),
-                ("train_loss", C.c_double), ("li_moves", C.c_uint64)]
+                ("train_loss", C.c_double), ("li_moves", C.c_uint64),
+                ("win_moves", C.c_uint64)]
 
 
 # name -> (argtypes, restype)
@@ -112,6 +114,7 @@ SIGNATURES = {
     "bmm_hip_mlp_set_params": ([_vp, _f32p], C.c_int),
     "bmm_hip_mlp_get_params": ([_vp, _f32p], C.c_int),
     "bmm_hip_mlp_forward": ([_vp, _f32p, _sz, _f32p], C.c_int),
+    "bmm_hip_mlp_score_pairs": ([_vp, _u64p, _sz, _u16p, _sz, _f32p], C.c_int),
     "bmm_hip_mlp_gradients": ([_vp, _f32p, _f32p, _sz, _f32p, C.POINTER(C.c_float)], C.c_int),
     "bmm_hip_mlp_train_step": ([_vp, _f32p, _f32p, _sz, C.c_float, C.POINTER(C.c_float)],
                                C.c_int),
@@ -377,6 +380,15 @@ class GpuMlp:
         x = np.ascontiguousarray(x, dtype=np.float32)
         y = np.zeros(x.shape[0], np.float32)
         call("bmm_hip_mlp_forward", self.h, x, x.shape[0], y)
+        return y
+
+    def score_pairs(self, sketch, pairs):
+        """sketch: rows x 2 uint64 (128-bit row sketches); pairs: uint16
+        src | dst << 8.  Returns the model output per pair (float32)."""
+        sketch = np.ascontiguousarray(sketch, dtype=np.uint64).reshape(-1, 2)
+        pairs = np.ascontiguousarray(pairs, dtype=np.uint16).reshape(-1)
+        y = np.zeros(pairs.size, np.float32)
+        call("bmm_hip_mlp_score_pairs", self.h, sketch, sketch.shape[0], pairs, pairs.size, y)
         return y
 
     def gradients(self, x, t):

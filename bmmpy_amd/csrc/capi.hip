@@ -85,6 +85,8 @@ void mlp_destroy(Mlp* M);
 void mlp_set_params(Mlp* M, const float* host);
 void mlp_get_params(Mlp* M, float* host);
 void mlp_forward_host(Mlp* M, const float* x, size_t B, float* y);
+void mlp_score_pairs_host(Mlp* M, const uint64_t* sketch, size_t rows, const uint16_t* pairs,
+                          size_t n, float* y);
 float mlp_grads_host(Mlp* M, const float* x, const float* t, size_t B, float* grads);
 float mlp_train_host(Mlp* M, const float* x, const float* t, size_t B, float lr);
 size_t mlp_num_params();
@@ -519,6 +521,9 @@ int bmm_hip_ga_default_params(bmm_ga_params* p) {
         p->p_delete = 0.15f;
         p->learning_rate = 1e-3f;
         p->seed = 0x5EED0005ull;
+        p->win_rows = 16;
+        p->win_rounds = 2;
+        p->win_candidates = 32;
     });
 }
 
@@ -592,6 +597,20 @@ int bmm_hip_mlp_forward(void* mlp, const float* x, size_t batch, float* y) {
         if (!mlp || !x || !y || batch == 0)
             fail(BMM_ERR_INVALID, "mlp_forward: null pointer or empty batch");
         mlp_forward_host(static_cast<Mlp*>(mlp), x, batch, y);
+    });
+}
+
+int bmm_hip_mlp_score_pairs(void* mlp, const uint64_t* sketch, size_t rows,
+                            const uint16_t* pairs, size_t n, float* y) {
+    return guarded([&] {
+        if (!mlp || !sketch || !pairs || !y || n == 0)
+            fail(BMM_ERR_INVALID, "mlp_score_pairs: null pointer or empty batch");
+        if (rows < 1 || rows > 256)
+            fail(BMM_ERR_INVALID, "mlp_score_pairs: rows must be in [1, 256]");
+        for (size_t i = 0; i < n; ++i)
+            if ((size_t)(pairs[i] & 0xff) >= rows || (size_t)(pairs[i] >> 8) >= rows)
+                fail(BMM_ERR_RANGE, "mlp_score_pairs: pair row out of range");
+        mlp_score_pairs_host(static_cast<Mlp*>(mlp), sketch, rows, pairs, n, y);
     });
 }
 

@@ -26,6 +26,7 @@
 // Proposals for new ops are uniform random pairs (plain) or drawn from a
 // per-island list of pairs ranked by the proposal MLP (learned).
 #include "common.hpp"
+#include "fwht_pk16.hpp"
 
 #include <algorithm>
 #include <chrono>
@@ -39,6 +40,8 @@ struct Mlp;
 Mlp* mlp_create(uint64_t seed);
 void mlp_destroy(Mlp* M);
 float* mlp_forward_dev(Mlp* M, const __bf16* d_x, int B, hipStream_t s);
+float* mlp_score_pairs_dev(Mlp* M, const uint64_t* d_sketch, int rows, const uint16_t* d_pairs,
+                           const int32_t* d_pair_isl, int n, hipStream_t s);
 float* mlp_backward_dev(Mlp* M, const __bf16* d_x, const float* d_t, int B, int valid,
                         hipStream_t s);
 void mlp_adam_dev(Mlp* M, float lr, hipStream_t s);
@@ -473,6 +476,266 @@ __global__ void ga_li_apply_kernel(uint64_t* __restrict__ base, int rows, int wo
     }
 }
 
+// Window local improvement: for each target row i of a round (index bit `bit`
+// equal to `phase`) and a window of W rows from the other half, the exact
+// minimum over the 2^W subsets S of wt(r_i ^ XOR_S) (the coset minimum of the
+// FWHT engine: signed column-pattern histogram, packed-int16 blocks of 2^14
+// masks, fwht_pk16.hpp), applied when it beats wt(r_i).  Targets never serve
+// as sources within a round, so all moves are independent row operations.
+
+__device__ inline uint64_t ga_mix64(uint64_t x) {
+    x ^= x >> 30;
+    x *= 0xbf58476d1ce4e5b9ull;
+    x ^= x >> 27;
+    x *= 0x94d049bb133111ebull;
+    x ^= x >> 31;
+    return x;
+}
+
+// Candidate sources per target: C distinct rows of the other half (all of
+// them, in index order, when the half has at most C rows; -1 pads), one
+// thread per (island, row).
+__global__ void ga_win_pick_kernel(int islands, int rows, int bit, int phase, int C,
+                                   uint64_t seed, int16_t* __restrict__ cand) {
+    const int id = blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= islands * rows)
+        return;
+    const int i = id % rows;
+    int16_t* out = cand + (size_t)id * C;
+    if (((i >> bit) & 1) != phase)
+        return;
+    int nsrc = 0;
+    for (int j = 0; j < rows; ++j)
+        nsrc += ((j >> bit) & 1) != phase;
+    if (nsrc <= C) {
+        int c = 0;
+        for (int j = 0; j < rows; ++j)
+            if (((j >> bit) & 1) != phase)
+                out[c++] = (int16_t)j;
+        for (; c < C; ++c)
+            out[c] = -1;
+        return;
+    }
+    uint32_t used[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    uint64_t st = ga_mix64(seed ^ ((uint64_t)id << 1 | 1ull));
+    for (int c = 0; c < C;) {
+        st = ga_mix64(st + 0x9E3779B97F4A7C15ull);
+        const int j = (int)((st >> 32) % (uint64_t)rows);
+        if (((j >> bit) & 1) == phase || ((used[j >> 5] >> (j & 31)) & 1u))
+            continue;
+        used[j >> 5] |= 1u << (j & 31);
+        out[c++] = (int16_t)j;
+    }
+}
+
+// Pairs (candidate -> target) for the proposal model, one per (island, row,
+// candidate); non-target rows and padding get the pair (0 -> 0).
+__global__ void ga_win_pairs_kernel(int islands, int rows, int C, const int16_t* __restrict__ cand,
+                                    int bit, int phase, uint16_t* __restrict__ pairs,
+                                    int32_t* __restrict__ pair_isl) {
+    const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= (size_t)islands * rows * C)
+        return;
+    const size_t id = k / C;
+    const int i = (int)(id % rows);
+    const int s = cand[k];
+    const bool ok = ((i >> bit) & 1) == phase && s >= 0;
+    pairs[k] = ok ? (uint16_t)(s | (i << 8)) : (uint16_t)0;
+    pair_isl[k] = (int32_t)(id / rows);
+}
+
+// Window = the W candidates with the lowest score (learned: predicted delta;
+// score == nullptr: the first W candidates).  One thread per (island, row).
+__global__ void ga_win_rank_kernel(int islands, int rows, int C, int W, int bit, int phase,
+                                   const int16_t* __restrict__ cand,
+                                   const float* __restrict__ score,
+                                   int16_t* __restrict__ win) {
+    const int id = blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= islands * rows)
+        return;
+    const int i = id % rows;
+    if (((i >> bit) & 1) != phase)
+        return;
+    const int16_t* c = cand + (size_t)id * C;
+    int16_t* w = win + (size_t)id * W;
+    if (!score) {
+        for (int k = 0; k < W; ++k)
+            w[k] = c[k];
+        return;
+    }
+    const float* sc = score + (size_t)id * C;
+    uint64_t taken[2] = {0, 0};   // C <= 128
+    for (int k = 0; k < W; ++k) {
+        int best = -1;
+        float bv = 0.f;
+        for (int q = 0; q < C; ++q) {
+            if (c[q] < 0 || ((taken[q >> 6] >> (q & 63)) & 1u))
+                continue;
+            if (best < 0 || sc[q] < bv) {
+                best = q;
+                bv = sc[q];
+            }
+        }
+        if (best < 0) {
+            w[k] = -1;
+            continue;
+        }
+        taken[best >> 6] |= 1ull << (best & 63);
+        w[k] = c[best];
+    }
+}
+
+// One workgroup per (island, row); target rows only.  Columns: thread t holds
+// the CP = cols / 256 columns [t CP, (t + 1) CP) (cols <= 8192 and a multiple
+// of 2048, so CP is 8, 16, 24 or 32)
+// as W-bit window patterns with the target bit at bit 31.  Blocks mh of 2^14
+// masks (2^(W-14) of them) are visited in order; the first block holding the
+// largest H keeps the lowest mask.  move[id] = the subset mask (0: none).
+__global__ __launch_bounds__(256) void ga_win_kernel(const uint64_t* __restrict__ base, int rows,
+                                                     int words, int W, int bit, int phase,
+                                                     const int16_t* __restrict__ win,
+                                                     const uint32_t* __restrict__ roww,
+                                                     uint32_t* __restrict__ move,
+                                                     uint32_t* __restrict__ neww) {
+    __shared__ __align__(16) uint32_t s[pk_lds_words() + 4];
+    __shared__ int32_t wbest[4];
+    __shared__ uint32_t wcount[4];
+    __shared__ int32_t wins[32];
+    const int isl = blockIdx.x / rows;
+    const int i = blockIdx.x % rows;
+    if (((i >> bit) & 1) != phase)
+        return;
+    const size_t id = (size_t)isl * rows + i;
+    const uint64_t* b = base + (size_t)isl * rows * words;
+    if (threadIdx.x < W)
+        wins[threadIdx.x] = win[id * W + threadIdx.x];
+    __syncthreads();
+    bool complete = true;
+    for (int k = 0; k < W; ++k)
+        complete &= wins[k] >= 0;
+    if (!complete) {   // fewer than W sources: no move
+        if (threadIdx.x == 0)
+            move[id] = 0;
+        return;
+    }
+    const int CP = words * 64 / 256;
+    const int c0 = (int)threadIdx.x * CP;
+    const uint32_t cmask = CP >= 32 ? 0xffffffffu : ((1u << CP) - 1u);
+    auto chunk = [&](int r) -> uint32_t {
+        const uint64_t* row = b + (size_t)r * words;
+        const int w0 = c0 >> 6, off = c0 & 63;
+        uint64_t v = row[w0] >> off;
+        if (off + CP > 64)   // the thread's columns straddle two words (CP = 24)
+            v |= row[w0 + 1] << (64 - off);
+        return (uint32_t)v & cmask;
+    };
+    uint32_t pat[32];
+    {
+        const uint32_t xt = chunk(i);
+#pragma unroll
+        for (int k = 0; k < 32; ++k)
+            pat[k] = ((xt >> k) & 1u) << 31;
+    }
+    for (int r = 0; r < W; ++r) {
+        const uint32_t x = chunk(wins[r]);
+#pragma unroll
+        for (int k = 0; k < 32; ++k)
+            pat[k] |= ((x >> k) & 1u) << r;
+    }
+    uint32_t cnt = 0;
+#pragma unroll
+    for (int k = 0; k < 32; ++k)
+        cnt += pat[k] != 0u;
+    cnt = wave_sum_u32(cnt);
+    if (threadIdx.x % kWave == 0)
+        wcount[threadIdx.x / kWave] = cnt;
+    __syncthreads();
+    const int32_t total = (int32_t)(wcount[0] + wcount[1] + wcount[2] + wcount[3]);
+    int32_t best_h = INT32_MIN;
+    uint32_t best_mask = 0;
+    const uint32_t hi_mask = (1u << (W - kPkL)) - 1u;
+    for (uint32_t mh = 0; mh <= hi_mask; ++mh) {
+        int4* s4 = reinterpret_cast<int4*>(s);
+        const int bias = (int)kPkBias;
+        for (int q = threadIdx.x; q < (pk_lds_words() + 3) / 4; q += blockDim.x)
+            s4[q] = make_int4(bias, bias, bias, bias);
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < 32; ++k) {
+            const uint32_t p = pat[k];
+            if (p != 0u) {
+                const uint32_t pl = p & ((1u << kPkL) - 1u);
+                const uint32_t one = (pl >> kPkW) ? 0x10000u : 1u;
+                const uint32_t neg = ((p >> 31) ^ __builtin_popcount((p >> kPkL) & hi_mask & mh)) & 1u;
+                atomicAdd(&s[fwht_pad((int)(pl & ((1u << kPkW) - 1u)))], neg ? 0u - one : one);
+            }
+        }
+        int32_t best = INT32_MIN;
+        pk_block_fwht_visit(s, [&](int q, const uint32_t (&v)[8]) {
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const uint32_t m_lo = (uint32_t)(256 * (q + 4 * e));
+                best = max(best, (int32_t)((v[e] << 16) | (65535u - 4u * m_lo)));
+                best = max(best, (int32_t)((v[e] & 0xffff0000u) |
+                                           (65535u - 4u * (m_lo + (1u << kPkW)))));
+            }
+        });
+        {
+            const int32_t h = best >> 16;
+            const uint32_t ml = ((65535u - ((uint32_t)best & 0xffffu)) >> 2) + threadIdx.x;
+            best = (int32_t)(((uint32_t)h << 16) | (65535u - 4u * ml));
+        }
+        best = wave_max_i32(best);
+        if (threadIdx.x % kWave == 0)
+            wbest[threadIdx.x / kWave] = best;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const int32_t k = max(max(wbest[0], wbest[1]), max(wbest[2], wbest[3]));
+            const int32_t h = k >> 16;
+            if (h > best_h) {
+                best_h = h;
+                best_mask = (mh << kPkL) | ((65535u - ((uint32_t)k & 0xffffu)) >> 2);
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const uint32_t w = (uint32_t)(total - best_h) >> 1;
+        const bool better = w < roww[id];
+        move[id] = better ? best_mask : 0u;
+        neww[id] = w;
+    }
+}
+
+__global__ void ga_win_apply_kernel(uint64_t* __restrict__ base, int rows, int words, int W,
+                                    int bit, int phase, const int16_t* __restrict__ win,
+                                    const uint32_t* __restrict__ move,
+                                    const uint32_t* __restrict__ neww,
+                                    uint32_t* __restrict__ roww,
+                                    unsigned int* __restrict__ changed) {
+    const int isl = blockIdx.x / rows;
+    const int i = blockIdx.x % rows;
+    if (((i >> bit) & 1) != phase)
+        return;
+    const size_t id = (size_t)isl * rows + i;
+    const uint32_t m = move[id];
+    if (m == 0u)
+        return;
+    uint64_t* b = base + (size_t)isl * rows * words;
+    const int16_t* w = win + id * W;
+    for (int x = threadIdx.x; x < words; x += blockDim.x) {
+        uint64_t acc = b[(size_t)i * words + x];
+        for (int k = 0; k < W; ++k)
+            if ((m >> k) & 1u)
+                acc ^= b[(size_t)w[k] * words + x];
+        b[(size_t)i * words + x] = acc;
+    }
+    if (threadIdx.x == 0) {
+        roww[id] = neww[id];
+        atomicAdd(changed, 1u);
+    }
+}
+
 // best_w[isl] = total weight of the island base (after local improvement).
 __global__ void ga_base_weight_kernel(const uint32_t* __restrict__ roww, int rows,
                                       unsigned int* __restrict__ best_w) {
@@ -502,6 +765,9 @@ struct Ga {
     // proposal model data
     DevBuf pairs, pair_isl, feats, labels, prng;
     DevBuf li_move, li_w, li_changed;
+    DevBuf win_cand, win, win_move, win_w;
+    uint64_t win_round = 0;
+    uint64_t win_moves = 0;
     Mlp* mlp = nullptr;
     int cur = 0;
     long generation = 0;
@@ -654,8 +920,9 @@ void update_proposals(Ga& G) {
     // score a pool per island
     const int ns = kScorePerIsland * I;
     sample_pairs(G, kScorePerIsland);
-    make_features(G, ns, ns);
-    float* y = mlp_forward_dev(G.mlp, static_cast<const __bf16*>(G.feats.ptr), ns, G.stream);
+    float* y = mlp_score_pairs_dev(G.mlp, static_cast<const uint64_t*>(G.sketch.ptr), G.P.rows,
+                                   static_cast<const uint16_t*>(G.pairs.ptr),
+                                   static_cast<const int32_t*>(G.pair_isl.ptr), ns, G.stream);
     hipLaunchKernelGGL(ga_select_kernel, dim3(I), dim3(256), 0, G.stream, y,
                        static_cast<const uint16_t*>(G.pairs.ptr), kScorePerIsland,
                        G.P.n_candidates, static_cast<uint16_t*>(G.cand.ptr));
@@ -686,6 +953,64 @@ unsigned int local_improvement(Ga& G) {
                                    static_cast<const uint32_t*>(G.li_w.ptr),
                                    static_cast<uint32_t*>(G.roww.ptr), bit, phase, changed);
             }
+    hipLaunchKernelGGL(ga_base_weight_kernel, dim3(I), dim3(256), 0, G.stream,
+                       static_cast<const uint32_t*>(G.roww.ptr), R,
+                       static_cast<unsigned int*>(G.best_w.ptr));
+    BMM_HIP(hipGetLastError());
+    unsigned int n = 0;
+    BMM_HIP(hipMemcpyAsync(&n, changed, 4, hipMemcpyDeviceToHost, G.stream));
+    BMM_HIP(hipStreamSynchronize(G.stream));
+    return n;
+}
+
+// Rounds of window local improvement on every island base (windows chosen at
+// random, or the learned proposal's top-W of C random candidates per
+// target); returns the number of applied moves.
+unsigned int window_improvement(Ga& G) {
+    const int I = G.P.islands, R = G.P.rows, W = G.P.win_rows;
+    const int C = G.P.learned ? G.P.win_candidates : W;
+    int bits = 0;
+    while ((1 << bits) < R)
+        ++bits;
+    unsigned int* changed = static_cast<unsigned int*>(G.li_changed.ptr);
+    fill_u32(changed, 1, 0u, G.stream);
+    int16_t* cand = static_cast<int16_t*>(G.win_cand.ptr);
+    int16_t* win = static_cast<int16_t*>(G.win.ptr);
+    const int n_rows = I * R;
+    for (int round = 0; round < G.P.win_rounds; ++round) {
+        const uint64_t r = G.win_round++;
+        const int bit = (int)(r % (uint64_t)bits);
+        const int phase = (int)((r / (uint64_t)bits) & 1u);
+        refresh_sketches(G);
+        hipLaunchKernelGGL(ga_win_pick_kernel, dim3((n_rows + 255) / 256), dim3(256), 0,
+                           G.stream, I, R, bit, phase, C,
+                           (uint64_t)G.P.seed ^ (0xD1B54A32D192ED03ull * (r + 1)), cand);
+        const float* score = nullptr;
+        if (G.P.learned) {
+            const int n = n_rows * C;
+            hipLaunchKernelGGL(ga_win_pairs_kernel, dim3((n + 255) / 256), dim3(256), 0,
+                               G.stream, I, R, C, cand, bit, phase,
+                               static_cast<uint16_t*>(G.pairs.ptr),
+                               static_cast<int32_t*>(G.pair_isl.ptr));
+            score = mlp_score_pairs_dev(G.mlp, static_cast<const uint64_t*>(G.sketch.ptr), R,
+                                        static_cast<const uint16_t*>(G.pairs.ptr),
+                                        static_cast<const int32_t*>(G.pair_isl.ptr), n,
+                                        G.stream);
+        }
+        hipLaunchKernelGGL(ga_win_rank_kernel, dim3((n_rows + 255) / 256), dim3(256), 0,
+                           G.stream, I, R, C, W, bit, phase, cand, score, win);
+        hipLaunchKernelGGL(ga_win_kernel, dim3((unsigned)n_rows), dim3(256), 0, G.stream,
+                           static_cast<const uint64_t*>(G.base.ptr), R, G.words, W, bit, phase,
+                           win, static_cast<const uint32_t*>(G.roww.ptr),
+                           static_cast<uint32_t*>(G.win_move.ptr),
+                           static_cast<uint32_t*>(G.win_w.ptr));
+        hipLaunchKernelGGL(ga_win_apply_kernel, dim3((unsigned)n_rows), dim3(128), 0, G.stream,
+                           static_cast<uint64_t*>(G.base.ptr), R, G.words, W, bit, phase, win,
+                           static_cast<const uint32_t*>(G.win_move.ptr),
+                           static_cast<const uint32_t*>(G.win_w.ptr),
+                           static_cast<uint32_t*>(G.roww.ptr), changed);
+        BMM_HIP(hipGetLastError());
+    }
     hipLaunchKernelGGL(ga_base_weight_kernel, dim3(I), dim3(256), 0, G.stream,
                        static_cast<const uint32_t*>(G.roww.ptr), R,
                        static_cast<unsigned int*>(G.best_w.ptr));
@@ -727,6 +1052,8 @@ void epoch(Ga& G) {
         refresh_sketches(G);
         G.li_moves += local_improvement(G);
     }
+    if (G.P.win_rows > 0 && G.P.win_rounds > 0)
+        G.win_moves += window_improvement(G);
     update_proposals(G);
 }
 
@@ -746,6 +1073,16 @@ Ga* ga_create(const uint64_t* matrix, size_t rows, size_t cols, const bmm_ga_par
         fail(BMM_ERR_INVALID, "ga: parameter out of range");
     if (P.learned && (P.n_candidates < 1 || P.n_candidates > kScorePerIsland))
         fail(BMM_ERR_INVALID, "ga: n_candidates must be in [1, 1024] for the learned proposal");
+    if (P.win_rows != 0) {
+        if (P.win_rows < kPkL || P.win_rows > 20)
+            fail(BMM_ERR_INVALID, "ga: win_rows must be 0 or in [14, 20]");
+        if (P.win_rounds < 0)
+            fail(BMM_ERR_INVALID, "ga: win_rounds must be >= 0");
+        if (P.learned && (P.win_candidates < P.win_rows || P.win_candidates > 128))
+            fail(BMM_ERR_INVALID, "ga: win_candidates must be in [win_rows, 128]");
+        if (cols > 8192)
+            P.win_rows = 0;   // one thread holds cols / 256 <= 32 column patterns
+    }
     P.rows = (int)rows;
     G->words = (int)(cols / 64);
     G->W32 = (int)(cols / 32);
@@ -798,10 +1135,19 @@ Ga* ga_create(const uint64_t* matrix, size_t rows, size_t cols, const bmm_ga_par
     G->li_move.get<int32_t>((size_t)I * rows);
     G->li_w.get<uint32_t>((size_t)I * rows);
     G->li_changed.get<unsigned int>(1);
-    G->pairs.get<uint16_t>(n_pairs);
-    G->pair_isl.get<int32_t>(n_pairs);
-    G->feats.get<__bf16>((size_t)n_pairs * 256 + 32 * 256);
+    const size_t n_win = P.win_rows && P.learned ? (size_t)I * rows * P.win_candidates : 0;
+    const size_t n_pair_buf = std::max((size_t)n_pairs, n_win);
+    G->pairs.get<uint16_t>(n_pair_buf + 32);
+    G->pair_isl.get<int32_t>(n_pair_buf + 32);
+    G->feats.get<__bf16>((size_t)(kTrainPerIsland * I + 32) * 256);   // training batch
     G->labels.get<float>(n_pairs + 32);
+    if (P.win_rows) {
+        const int C = P.learned ? P.win_candidates : P.win_rows;
+        G->win_cand.get<int16_t>((size_t)I * rows * C);
+        G->win.get<int16_t>((size_t)I * rows * P.win_rows);
+        G->win_move.get<uint32_t>((size_t)I * rows);
+        G->win_w.get<uint32_t>((size_t)I * rows);
+    }
     if (P.learned)
         G->mlp = mlp_create(P.seed ^ 0xA5A5A5A5ull);
     update_proposals(*G);
@@ -853,6 +1199,7 @@ void ga_run(Ga* G, size_t max_generations, uint64_t target, double time_budget_s
         std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     st->train_loss = G->train_loss;
     st->li_moves = G->li_moves;
+    st->win_moves = G->win_moves;
 }
 
 // Best matrix: the best island's base with its best sequence replayed.

@@ -82,6 +82,105 @@ __global__ void f32_to_bf16_kernel(const float* __restrict__ x, __bf16* __restri
         y[i] = to_bf16(x[i]);
 }
 
+// y[n][k] = bf16(x[k][n]) for a 256 x 256 fp32 matrix.
+__global__ void transpose_bf16_kernel(const float* __restrict__ x, __bf16* __restrict__ y) {
+    const int n = blockIdx.x, k = threadIdx.x;
+    y[(size_t)n * kHid + k] = to_bf16(x[(size_t)k * kHid + n]);
+}
+
+// Fused scoring of (src -> dst) row pairs of the GA: the pair's features
+// (sketch(src) | sketch(dst), one bit per feature, as +-1 bf16) are built in
+// LDS; both hidden layers run on v_mfma_f32_32x32x16_bf16 with the 64 x 256
+// activations kept in LDS; weights come from the transposed bf16 copies
+// (lane l's fragment of W^T row n is 16 contiguous bytes); then the head.
+// One workgroup (4 wavefronts) per 64 pairs; wavefront v computes output
+// columns [64 v, 64 v + 64) of every layer as 2 x 2 tiles of 32 x 32.
+constexpr int kSB = 64;            // pairs per workgroup
+constexpr int kLdA = kHid + 8;     // LDS row stride (bf16): rows 16 bytes apart in bank space
+
+typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
+
+__device__ inline void score_layer(const __bf16* __restrict__ in, const __bf16* __restrict__ Wt,
+                                   const float* __restrict__ bias, __bf16* __restrict__ out) {
+    const int lane = threadIdx.x % kWave, wv = threadIdx.x / kWave;
+    const int r = lane % 32, half = lane / 32;
+    f32x16 acc[2][2] = {};
+    for (int k0 = 0; k0 < kHid; k0 += 16) {
+        bf16x8 a[2], b[2];
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+            a[t] = *reinterpret_cast<const bf16x8*>(&in[(32 * t + r) * kLdA + k0 + 8 * half]);
+            b[t] = *reinterpret_cast<const bf16x8*>(
+                &Wt[(size_t)(64 * wv + 32 * t + r) * kHid + k0 + 8 * half]);
+        }
+#pragma unroll
+        for (int tm = 0; tm < 2; ++tm)
+#pragma unroll
+            for (int tn = 0; tn < 2; ++tn)
+                acc[tm][tn] =
+                    __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[tm], b[tn], acc[tm][tn], 0, 0, 0);
+    }
+#pragma unroll
+    for (int tn = 0; tn < 2; ++tn) {
+        const int col = 64 * wv + 32 * tn + r;
+        const float bc = bias[col];
+#pragma unroll
+        for (int tm = 0; tm < 2; ++tm)
+#pragma unroll
+            for (int j = 0; j < 16; ++j) {
+                const int row = 32 * tm + 8 * (j / 4) + 4 * half + (j % 4);
+                const float v = acc[tm][tn][j] + bc;
+                out[row * kLdA + col] = to_bf16(v > 0.f ? v : 0.f);
+            }
+    }
+}
+
+__global__ __launch_bounds__(256) void mlp_score_kernel(
+    const uint64_t* __restrict__ sketch, int rows, const uint16_t* __restrict__ pairs,
+    const int32_t* __restrict__ pair_isl, int n, const __bf16* __restrict__ W1t,
+    const float* __restrict__ b1, const __bf16* __restrict__ W2t, const float* __restrict__ b2,
+    const __bf16* __restrict__ w3, const float* __restrict__ b3, float* __restrict__ y) {
+    __shared__ __align__(16) __bf16 X[kSB * kLdA];
+    __shared__ __align__(16) __bf16 H[kSB * kLdA];
+    const int p0 = blockIdx.x * kSB;
+    {
+        // Thread t: pair t / 4, features [64 q, 64 q + 64) for q = t % 4
+        // (src sketch words 0, 1, then dst sketch words 0, 1).
+        const int bi = threadIdx.x / 4, q = threadIdx.x % 4;
+        const int i = p0 + bi;
+        uint64_t w = 0;
+        if (i < n) {
+            const int isl = pair_isl[i];
+            const int row = q < 2 ? (pairs[i] & 0xff) : (pairs[i] >> 8);
+            w = sketch[((size_t)isl * rows + row) * 2 + (q & 1)];
+        }
+        u16x8* dst = reinterpret_cast<u16x8*>(&X[bi * kLdA + 64 * q]);
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            u16x8 v;
+#pragma unroll
+            for (int f = 0; f < 8; ++f)   // bf16 +1 = 0x3f80, -1 = 0xbf80
+                v[f] = (unsigned short)(((w >> (8 * c + f)) & 1u) ? 0x3f80u : 0xbf80u);
+            dst[c] = v;
+        }
+    }
+    __syncthreads();
+    score_layer(X, W1t, b1, H);
+    __syncthreads();
+    score_layer(H, W2t, b2, X);
+    __syncthreads();
+    {
+        const int bi = threadIdx.x / 4, q = threadIdx.x % 4;
+        float acc = 0.f;
+        for (int h = 64 * q; h < 64 * q + 64; ++h)
+            acc += (float)X[bi * kLdA + h] * (float)w3[h];
+        acc += __shfl_xor(acc, 1, kWave);
+        acc += __shfl_xor(acc, 2, kWave);
+        if (q == 0 && p0 + bi < n)
+            y[p0 + bi] = acc + b3[0];
+    }
+}
+
 // Output head (N = 1): y[b] = sum_h A2[b][h] w3[h] + b3, and the loss terms.
 __global__ void head_kernel(const __bf16* __restrict__ A2, const __bf16* __restrict__ w3,
                             const float* __restrict__ b3, int B, int valid,
@@ -190,6 +289,8 @@ struct Mlp {
     // fp32 master parameters (flat: W1, b1, W2, b2, w3, b3) + Adam moments.
     DevBuf p, g, m, v;
     DevBuf wb;    // bf16 copies of W1, W2, w3 (W1 | W2 | w3)
+    DevBuf wbt;   // bf16 transposes W1^T | W2^T ([out][in]) for the scoring kernel
+    DevBuf score; // outputs of mlp_score_pairs_dev
     DevBuf act;   // activations / scratch for the current batch
     int cap_batch = 0;
     long step = 0;
@@ -232,6 +333,15 @@ Acts acts_of(Mlp& M, int B) {
     return a;
 }
 
+void sync_transposes(Mlp& M, hipStream_t s) {
+    const float* p = static_cast<const float*>(M.p.ptr);
+    __bf16* t = static_cast<__bf16*>(M.wbt.ptr);
+    hipLaunchKernelGGL(transpose_bf16_kernel, dim3(kHid), dim3(kHid), 0, s, p + kOff.W1, t);
+    hipLaunchKernelGGL(transpose_bf16_kernel, dim3(kHid), dim3(kHid), 0, s, p + kOff.W2,
+                       t + (size_t)kIn * kHid);
+    BMM_HIP(hipGetLastError());
+}
+
 void sync_bf16(Mlp& M, hipStream_t s) {
     float* p = static_cast<float*>(M.p.ptr);
     __bf16* wb = static_cast<__bf16*>(M.wb.ptr);
@@ -241,7 +351,7 @@ void sync_bf16(Mlp& M, hipStream_t s) {
                        wb + (size_t)kIn * kHid, (size_t)kHid * kHid);
     hipLaunchKernelGGL(f32_to_bf16_kernel, dim3(1), dim3(256), 0, s, p + kOff.w3,
                        wb + (size_t)kIn * kHid + (size_t)kHid * kHid, (size_t)kHid);
-    BMM_HIP(hipGetLastError());
+    sync_transposes(M, s);
 }
 } // namespace
 
@@ -252,6 +362,7 @@ Mlp* mlp_create(uint64_t seed) {
     M->m.get<float>(kNumParams);
     M->v.get<float>(kNumParams);
     M->wb.get<__bf16>((size_t)kIn * kHid + (size_t)kHid * kHid + kHid);
+    M->wbt.get<__bf16>((size_t)kIn * kHid + (size_t)kHid * kHid);
     // He-uniform init from XorShift64.
     std::vector<float> h(kNumParams, 0.f);
     uint64_t s = seed == 0 ? 0x9E3779B97F4A7C15ull : seed;
@@ -308,6 +419,24 @@ float* mlp_forward_dev(Mlp* M, const __bf16* d_x, int B, hipStream_t s) {
     return a.y;
 }
 
+// Scores n (src -> dst) pairs from per-row sketches (2 u64 words per row,
+// row-major per island); returns y on the device (valid until the next call).
+float* mlp_score_pairs_dev(Mlp* M, const uint64_t* d_sketch, int rows, const uint16_t* d_pairs,
+                           const int32_t* d_pair_isl, int n, hipStream_t s) {
+    float* y = M->score.get<float>((size_t)std::max(n, 1));
+    if (n == 0)
+        return y;
+    const float* p = static_cast<const float*>(M->p.ptr);
+    const __bf16* wb = static_cast<const __bf16*>(M->wb.ptr);
+    const __bf16* t = static_cast<const __bf16*>(M->wbt.ptr);
+    hipLaunchKernelGGL(mlp_score_kernel, dim3((unsigned)((n + kSB - 1) / kSB)), dim3(256), 0, s,
+                       d_sketch, rows, d_pairs, d_pair_isl, n, t, p + kOff.b1,
+                       t + (size_t)kIn * kHid, p + kOff.b2,
+                       wb + (size_t)kIn * kHid + (size_t)kHid * kHid, p + kOff.b3, y);
+    BMM_HIP(hipGetLastError());
+    return y;
+}
+
 // Forward + backward for MSE against d_t (first `valid` rows count); leaves
 // gradients in M->g and the loss in the activation buffer.
 float* mlp_backward_dev(Mlp* M, const __bf16* d_x, const float* d_t, int B, int valid,
@@ -361,7 +490,7 @@ void mlp_adam_dev(Mlp* M, float lr, hipStream_t s) {
 namespace {
 // Host features (float, B x 256) -> padded bf16 device buffer.
 struct MlpIo {
-    DevBuf xf, xb, t;
+    DevBuf xf, xb, t, sk, pr, is;
 };
 MlpIo& mlp_io() {
     thread_local MlpIo io;
@@ -387,6 +516,19 @@ void mlp_forward_host(Mlp* M, const float* x, size_t B, float* y) {
     __bf16* xb = upload_features(x, (int)B, Bp);
     float* d_y = mlp_forward_dev(M, xb, Bp, nullptr);
     BMM_HIP(hipMemcpy(y, d_y, B * 4, hipMemcpyDeviceToHost));
+}
+
+void mlp_score_pairs_host(Mlp* M, const uint64_t* sketch, size_t rows, const uint16_t* pairs,
+                          size_t n, float* y) {
+    MlpIo& io = mlp_io();
+    uint64_t* d_sk = io.sk.get<uint64_t>(rows * 2);
+    uint16_t* d_pr = io.pr.get<uint16_t>(n);
+    int32_t* d_is = io.is.get<int32_t>(n);
+    BMM_HIP(hipMemcpy(d_sk, sketch, rows * 16, hipMemcpyHostToDevice));
+    BMM_HIP(hipMemcpy(d_pr, pairs, n * 2, hipMemcpyHostToDevice));
+    BMM_HIP(hipMemset(d_is, 0, n * 4));
+    float* d_y = mlp_score_pairs_dev(M, d_sk, (int)rows, d_pr, d_is, (int)n, nullptr);
+    BMM_HIP(hipMemcpy(y, d_y, n * 4, hipMemcpyDeviceToHost));
 }
 
 float mlp_grads_host(Mlp* M, const float* x, const float* t, size_t B, float* grads) {

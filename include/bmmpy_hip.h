@@ -228,6 +228,11 @@ typedef struct bmm_ga_params {
     float learning_rate;
     float p_learned;    /* share of new ops drawn from the ranked pairs */
     uint64_t seed;
+    int win_rows;       /* window local improvement: rows per window (0 = off, 14..20;
+                           off for matrices wider than 8192 columns) */
+    int win_rounds;     /* window rounds per epoch */
+    int win_candidates; /* learned: candidates scored per target, best win_rows kept */
+    int reserved;
 } bmm_ga_params;
 
 typedef struct bmm_ga_status {
@@ -240,6 +245,7 @@ typedef struct bmm_ga_status {
     double elapsed_s;
     double train_loss;    /* last MLP training loss (learned) */
     uint64_t li_moves;    /* local-improvement moves applied so far */
+    uint64_t win_moves;   /* window-improvement moves applied so far */
 } bmm_ga_status;
 
 int bmm_hip_ga_default_params(bmm_ga_params *out);
@@ -263,6 +269,11 @@ int bmm_hip_mlp_destroy(void *mlp);
 int bmm_hip_mlp_set_params(void *mlp, const float *params);
 int bmm_hip_mlp_get_params(void *mlp, float *params);
 int bmm_hip_mlp_forward(void *mlp, const float *x, size_t batch, float *y);
+/* Fused scoring of row pairs: sketch = 2 u64 words per row (128 bits),
+ * pairs[i] = src | dst << 8; y[i] = model output for the features
+ * sketch(src) | sketch(dst) as +-1 (the same value mlp_forward gives). */
+int bmm_hip_mlp_score_pairs(void *mlp, const uint64_t *sketch, size_t rows,
+                            const uint16_t *pairs, size_t n, float *y);
 int bmm_hip_mlp_gradients(void *mlp, const float *x, const float *target, size_t batch,
                           float *grads, float *loss);
 int bmm_hip_mlp_train_step(void *mlp, const float *x, const float *target, size_t batch,

@@ -119,7 +119,7 @@ def test_ga_local_improvement_matches_restatement(lib):
     _say("ga local improvement")
     S, A, M = lib.gen_planted(0x5EED0005, 32, 2048, 0.05)
     p = lib.ga_default_params(islands=1, population=4, max_ops=8, migrate_every=1,
-                              mutations=0, li_sweeps=1, check_every=1, learned=0)
+                              mutations=0, li_sweeps=1, check_every=1, learned=0, win_rows=0)
     ga = lib.GpuGA(A, 2048, p)
     st = ga.run(1)
     best, w = ga.best()
@@ -128,3 +128,54 @@ def test_ga_local_improvement_matches_restatement(lib):
     assert st.li_moves > 0
     assert np.array_equal(best, exp)
     assert w == st.best_weight == _weight(exp)
+
+
+def _win_reference(ora, A, rounds_bits):
+    """Window improvement restated: for each round (bit, phase), every row i
+    with index bit `bit` == phase takes its exact coset minimum over the
+    subsets of the other half (all of it as the window, in index order; the
+    oracle's coset minimum, lowest mask on ties) when that beats wt(r_i)."""
+    rows = A.copy()
+    n = rows.shape[0]
+    for bit, phase in rounds_bits:
+        src = [j for j in range(n) if (j >> bit) & 1 != phase]
+        new = rows.copy()
+        for i in range(n):
+            if (i >> bit) & 1 != phase:
+                continue
+            sub = np.concatenate([rows[src], rows[i:i + 1]])
+            w, idx = ora.coset_min(sub, rows.shape[1] * 64)
+            if int(w[-1]) < _weight(rows[i:i + 1]):
+                acc = rows[i].copy()
+                for k, j in enumerate(src):
+                    if (int(idx[-1]) >> k) & 1:
+                        acc ^= rows[j]
+                new[i] = acc
+        rows = new
+    return rows
+
+
+@pytest.mark.parametrize("learned,cols", [(0, 2048), (1, 2048), (0, 6144)])
+def test_ga_window_improvement_matches_restatement(lib, ora, learned, cols):
+    """One epoch of window improvement alone (no mutation, no pairwise
+    improvement) on 32 x 2048: with 16-row halves a 16-row window is the whole
+    other half, so the exact move is determined (plain: in index order, equal
+    rows; learned: windows in model order, so equal weights, not rows)."""
+    _say("ga window improvement", learned, cols)
+    S, A, M = lib.gen_planted(0x5EED0006, 32, cols, 0.05)
+    rounds = 1 if learned else 2   # ties may pick other rows in model order
+    p = lib.ga_default_params(islands=2, population=4, max_ops=8, migrate_every=1,
+                              mutations=0, li_sweeps=0, check_every=1, learned=learned,
+                              win_rows=16, win_rounds=rounds, win_candidates=32)
+    ga = lib.GpuGA(A, cols, p)
+    st = ga.run(1)
+    best, w = ga.best()
+    ga.close()
+    exp = _win_reference(ora, A, [(0, 0), (1, 0)][:rounds])
+    assert st.win_moves > 0
+    assert w == st.best_weight == _weight(exp)
+    if not learned:
+        assert np.array_equal(best, exp)
+    # Row operations only: same row space (full rank of [A; best] equals rank A).
+    both = np.concatenate([A, best])
+    assert ora.rank(both, cols) == ora.rank(A, cols) == 32

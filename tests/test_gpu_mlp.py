@@ -99,3 +99,26 @@ def test_training_fits_pair_deltas(lib):
     top = np.argsort(pred)[:128]
     assert tv[top].mean() < tv.mean() - 0.5 * tv.std(), (tv[top].mean(), tv.mean(), tv.std())
     m.close()
+
+
+@pytest.mark.parametrize("n", [1, 63, 1000])
+def test_fused_pair_scoring_matches_forward(lib, model, n):
+    """The GA's fused scorer (features from 128-bit row sketches built in LDS,
+    both layers on MFMA in one kernel) against the float64 oracle on the
+    same +-1 features and against the generic forward path."""
+    _say("fused scoring", n)
+    p = mlp.init_params(21).astype(np.float32)
+    model.set_params(p)
+    rng = np.random.default_rng(n)
+    rows = 40
+    sketch = rng.integers(0, 2**63, size=(rows, 2), dtype=np.uint64) ^ \
+        (rng.integers(0, 2, size=(rows, 2), dtype=np.uint64) << np.uint64(63))
+    src = rng.integers(0, rows, size=n)
+    dst = rng.integers(0, rows, size=n)
+    pairs = (src | (dst << 8)).astype(np.uint16)
+    bits = np.unpackbits(sketch.view(np.uint8), bitorder="little").reshape(rows, 128)
+    x = np.concatenate([bits[src], bits[dst]], axis=1).astype(np.float32) * 2 - 1
+    y = model.score_pairs(sketch, pairs)
+    ref, _ = mlp.forward(p.astype(np.float64), x.astype(np.float64), bf16=True)
+    assert np.all(np.abs(y - ref) <= 1e-3 * (1 + np.abs(ref)))
+    assert np.allclose(y, model.forward(x), rtol=1e-4, atol=1e-4)
