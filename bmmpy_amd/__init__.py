@@ -33,12 +33,13 @@ from .search import (
     MitmFwhtSearch,
 )
 from .workflows import search_apply
+from . import solver  # noqa: E402  (bmm.solver.ExactBasisSolver, as in the reference)
 
 __all__ = [
     "ApplyResult", "BitMatrix", "BruteforceSearch", "Candidate", "CudaBruteforceSearch",
     "CudaMitmFwhtSearch", "FwhtSearch", "GreedyApplier", "MatrixErr", "MatrixError",
     "MitmFwhtSearch", "RowWindow", "RuntimeFeatures", "add", "fwht_i16", "fwht_i32",
-    "get_runtime_features", "get_version", "matrix_from_rows", "search_apply",
+    "get_runtime_features", "get_version", "matrix_from_rows", "search_apply", "solver",
 ]
 
 __version__ = get_version()

@@ -115,6 +115,7 @@ SIGNATURES = {
     "bmm_hip_mlp_get_params": ([_vp, _f32p], C.c_int),
     "bmm_hip_mlp_forward": ([_vp, _f32p, _sz, _f32p], C.c_int),
     "bmm_hip_mlp_score_pairs": ([_vp, _u64p, _sz, _u16p, _sz, _f32p], C.c_int),
+    "bmm_hip_weight_order": ([_u64p, _sz, _sz, _sz, _sz, _u32p, _u32p, C.POINTER(_sz)], C.c_int),
     "bmm_hip_mlp_gradients": ([_vp, _f32p, _f32p, _sz, _f32p, C.POINTER(C.c_float)], C.c_int),
     "bmm_hip_mlp_train_step": ([_vp, _f32p, _f32p, _sz, C.c_float, C.POINTER(C.c_float)],
                                C.c_int),
@@ -266,6 +267,17 @@ def window_coset_min(rows, cols: int, targets, windows):
     call("bmm_hip_window_coset_min", rows, rows.shape[0], rows.shape[1], cols,
          windows.reshape(-1), width, targets, n, w, m)
     return w, m
+
+
+def weight_order(rows, cols: int, k: int):
+    """First k nonzero masks of the window `rows` in (weight, Gray index)
+    order: (masks, weights)."""
+    rows = _rows(rows)
+    m = np.zeros(max(k, 1), np.uint32)
+    w = np.zeros(max(k, 1), np.uint32)
+    n = _sz()
+    call("bmm_hip_weight_order", rows, rows.shape[0], rows.shape[1], cols, k, m, w, C.byref(n))
+    return m[:n.value], w[:n.value]
 
 
 def gray_coset_min(rows, block_rows: int, targets):

@@ -85,6 +85,8 @@ void mlp_destroy(Mlp* M);
 void mlp_set_params(Mlp* M, const float* host);
 void mlp_get_params(Mlp* M, float* host);
 void mlp_forward_host(Mlp* M, const float* x, size_t B, float* y);
+size_t weight_order(const uint64_t* rows, size_t t, size_t words, size_t cols, size_t k,
+                    uint32_t* out_mask, uint32_t* out_weight);
 void mlp_score_pairs_host(Mlp* M, const uint64_t* sketch, size_t rows, const uint16_t* pairs,
                           size_t n, float* y);
 float mlp_grads_host(Mlp* M, const float* x, const float* t, size_t B, float* grads);
@@ -521,7 +523,7 @@ int bmm_hip_ga_default_params(bmm_ga_params* p) {
         p->p_delete = 0.15f;
         p->learning_rate = 1e-3f;
         p->seed = 0x5EED0005ull;
-        p->win_rows = 16;
+        p->win_rows = 20;
         p->win_rounds = 2;
         p->win_candidates = 32;
     });
@@ -597,6 +599,25 @@ int bmm_hip_mlp_forward(void* mlp, const float* x, size_t batch, float* y) {
         if (!mlp || !x || !y || batch == 0)
             fail(BMM_ERR_INVALID, "mlp_forward: null pointer or empty batch");
         mlp_forward_host(static_cast<Mlp*>(mlp), x, batch, y);
+    });
+}
+
+int bmm_hip_weight_order(const uint64_t* row_words, size_t t, size_t words, size_t cols,
+                         size_t k, uint32_t* out_mask, uint32_t* out_weight, size_t* n_out) {
+    return guarded([&] {
+        if (!n_out || (t > 0 && words > 0 && !row_words) || (k > 0 && (!out_mask || !out_weight)))
+            fail(BMM_ERR_INVALID, "weight_order: null pointer");
+        *n_out = 0;
+        if (t > 28)
+            fail(BMM_ERR_INVALID, "weight_order: at most 28 rows");
+        if (cols > words * 64)
+            fail(BMM_ERR_INVALID, "weight_order: cols exceeds words_per_row * 64");
+        if (cols >= (size_t)1 << 31)
+            fail(BMM_ERR_OVERFLOW, "weight_order: too many columns");
+        if (t == 0 || k == 0)
+            return;
+        require_device();
+        *n_out = weight_order(row_words, t, words, cols, k, out_mask, out_weight);
     });
 }
 

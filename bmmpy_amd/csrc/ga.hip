@@ -600,7 +600,7 @@ __global__ __launch_bounds__(256) void ga_win_kernel(const uint64_t* __restrict_
     __shared__ __align__(16) uint32_t s[pk_lds_words() + 4];
     __shared__ int32_t wbest[4];
     __shared__ uint32_t wcount[4];
-    __shared__ int32_t wins[32];
+    __shared__ int32_t wins[32];   // win_rows <= 24
     const int isl = blockIdx.x / rows;
     const int i = blockIdx.x % rows;
     if (((i >> bit) & 1) != phase)
@@ -1074,8 +1074,8 @@ Ga* ga_create(const uint64_t* matrix, size_t rows, size_t cols, const bmm_ga_par
     if (P.learned && (P.n_candidates < 1 || P.n_candidates > kScorePerIsland))
         fail(BMM_ERR_INVALID, "ga: n_candidates must be in [1, 1024] for the learned proposal");
     if (P.win_rows != 0) {
-        if (P.win_rows < kPkL || P.win_rows > 20)
-            fail(BMM_ERR_INVALID, "ga: win_rows must be 0 or in [14, 20]");
+        if (P.win_rows < kPkL || P.win_rows > 24)
+            fail(BMM_ERR_INVALID, "ga: win_rows must be 0 or in [14, 24]");
         if (P.win_rounds < 0)
             fail(BMM_ERR_INVALID, "ga: win_rounds must be >= 0");
         if (P.learned && (P.win_candidates < P.win_rows || P.win_candidates > 128))

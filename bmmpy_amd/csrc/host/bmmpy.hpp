@@ -302,6 +302,41 @@ private:
 // GPU; returns the window weight afterwards.
 std::uint64_t replay_row_xors(RowWindow& window, const std::vector<std::uint16_t>& ops);
 
+// ------------------------------------------------------------ solver -----
+struct ExactBasisSolverConfig {
+    std::size_t max_rows = 24;
+    std::uint64_t max_states = (std::uint64_t{1} << 24) - 1;
+    std::size_t max_storage_bytes = 128u * 1024u * 1024u;
+};
+
+struct ExactBasisResult {
+    std::size_t input_rows = 0;
+    std::size_t cols = 0;
+    std::size_t rank = 0;
+    std::uint64_t enumerated_states = 0;
+    std::uint64_t total_weight = 0;
+    std::vector<std::uint32_t> basis_masks;
+    std::vector<std::uint32_t> basis_weights;
+    BitMatrix transform_matrix;
+    BitMatrix basis_matrix;
+};
+
+// Minimum-total-weight basis of a window's row space
+// (src/solver/exact_basis_solver.cpp:58-157): the stream of nonzero row
+// combinations in (weight, Gray index) order comes from the GPU (one
+// transform of the column-pattern histogram gives every weight); the greedy
+// pivot-basis selection over that stream runs on the host.
+class ExactBasisSolver {
+public:
+    explicit ExactBasisSolver(ExactBasisSolverConfig config = {}) : _config(config) {}
+    const char* name() const noexcept { return "exact_basis"; }
+    ExactBasisResult solve(const RowWindow& window) const;
+    const ExactBasisSolverConfig& config() const noexcept { return _config; }
+
+private:
+    ExactBasisSolverConfig _config;
+};
+
 // ----------------------------------------------------------- runtime -----
 struct RuntimeFeatures {
     bool avx2_compiled = false;

@@ -295,6 +295,27 @@ PYBIND11_MODULE(_bmmpy, m) {
         .def("apply", &GreedyApplier::apply, py::arg("window"), py::arg("candidates"));
     m.def("replay_row_xors", &replay_row_xors, py::arg("window"), py::arg("ops"));
 
+    py::class_<ExactBasisSolverConfig>(m, "ExactBasisSolverConfig")
+        .def(py::init<>())
+        .def_readwrite("max_rows", &ExactBasisSolverConfig::max_rows)
+        .def_readwrite("max_states", &ExactBasisSolverConfig::max_states)
+        .def_readwrite("max_storage_bytes", &ExactBasisSolverConfig::max_storage_bytes);
+    py::class_<ExactBasisResult>(m, "ExactBasisResult")
+        .def(py::init<>())
+        .def_readwrite("input_rows", &ExactBasisResult::input_rows)
+        .def_readwrite("cols", &ExactBasisResult::cols)
+        .def_readwrite("rank", &ExactBasisResult::rank)
+        .def_readwrite("enumerated_states", &ExactBasisResult::enumerated_states)
+        .def_readwrite("total_weight", &ExactBasisResult::total_weight)
+        .def_readwrite("basis_masks", &ExactBasisResult::basis_masks)
+        .def_readwrite("basis_weights", &ExactBasisResult::basis_weights)
+        .def_readwrite("transform_matrix", &ExactBasisResult::transform_matrix)
+        .def_readwrite("basis_matrix", &ExactBasisResult::basis_matrix);
+    py::class_<ExactBasisSolver>(m, "ExactBasisSolver")
+        .def(py::init<ExactBasisSolverConfig>(), py::arg("config") = ExactBasisSolverConfig{})
+        .def("name", &ExactBasisSolver::name)
+        .def("solve", &ExactBasisSolver::solve, py::arg("window"));
+
     py::class_<RuntimeFeatures>(m, "RuntimeFeatures")
         .def_readonly("avx2_compiled", &RuntimeFeatures::avx2_compiled)
         .def_readonly("avx2_supported", &RuntimeFeatures::avx2_supported)

@@ -171,6 +171,15 @@ int bmm_hip_window_coset_min(const uint64_t *row_words, size_t n_rows, size_t wo
                              const int32_t *targets, size_t n_targets, uint32_t *out_weight,
                              uint64_t *out_mask);
 
+/* ---- candidate stream of ExactBasisSolver (src/solver/exact_basis_solver.cpp:
+ * 29-124): the first min(k, 2^t - 1) nonzero masks of a t-row window
+ * (t <= 28) in the order (weight, Gray index), i.e. the reference's Gray
+ * enumeration counting-sorted by weight; mask bit r = row r.
+ * *n_out = number written. */
+int bmm_hip_weight_order(const uint64_t *row_words, size_t t, size_t words_per_row,
+                         size_t cols, size_t k, uint32_t *out_mask, uint32_t *out_weight,
+                         size_t *n_out);
+
 /* ---- Gray-code coset sweep (config: pivot block of rows [0, block_rows)
  * minus the target).  For each target t in targets[]: block = rows
  * [0, block_rows) \ {t} in ascending order; step s visits block subset
@@ -228,7 +237,7 @@ typedef struct bmm_ga_params {
     float learning_rate;
     float p_learned;    /* share of new ops drawn from the ranked pairs */
     uint64_t seed;
-    int win_rows;       /* window local improvement: rows per window (0 = off, 14..20;
+    int win_rows;       /* window local improvement: rows per window (0 = off, 14..24;
                            off for matrices wider than 8192 columns) */
     int win_rounds;     /* window rounds per epoch */
     int win_candidates; /* learned: candidates scored per target, best win_rows kept */

@@ -20,6 +20,10 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 ORACLE_SO = os.path.join(HERE, "liboracle.so")
 REF_SO = os.path.join(HERE, "_ref", "libbmmref.so")
+# The same reference sources without the optional AVX2 backend (its
+# ExactBasisSolver passes 16-byte-aligned std::vector buffers to 32-byte
+# aligned AVX2 loads and crashes on rows of >= 4 words).
+REF_SCALAR_SO = os.path.join(HERE, "_ref", "libbmmref_scalar.so")
 REF_TREE = "/root/reference"
 
 _u64p = np.ctypeslib.ndpointer(np.uint64, flags="C_CONTIGUOUS")
@@ -83,6 +87,11 @@ class Oracle:
                                       C.POINTER(C.c_int32), _u64p, _u64p, _i32p, C.c_size_t]
         L.or_split_window.restype = C.c_longlong
         L.or_coset_min.argtypes = [_u64p, C.c_int, C.c_int, C.c_int, _u32p, _u64p]
+        L.or_exact_basis.argtypes = [_u64p, C.c_int, C.c_int, _i32p, C.c_int, _u32p, _u32p,
+                                     C.c_size_t, C.POINTER(C.c_uint64)]
+        L.or_weight_order.argtypes = [_u64p, C.c_int, C.c_int, _i32p, C.c_int, C.c_size_t,
+                                      _u32p, _u32p]
+        L.or_weight_order.restype = C.c_longlong
         L.or_gray_min.argtypes = [_u64p, C.c_int, C.c_int, C.c_int, C.c_int,
                                   C.POINTER(C.c_uint32), C.POINTER(C.c_uint64),
                                   C.POINTER(C.c_uint64)]
@@ -190,6 +199,27 @@ class Oracle:
         _check(self.lib.or_coset_min(m, m.shape[1], cols, n, w, a), "coset_min")
         return w, a
 
+    def exact_basis(self, m, cols):
+        """ExactBasisSolver on all rows of m: (masks, weights, total, rank)."""
+        m = np.ascontiguousarray(m, dtype=np.uint64)
+        t = m.shape[0]
+        idx = np.arange(t, dtype=np.int32)
+        masks, ws = np.zeros(t, np.uint32), np.zeros(t, np.uint32)
+        tot = C.c_uint64()
+        rank = _check(self.lib.or_exact_basis(m, m.shape[1], cols, idx, t, masks, ws, t,
+                                              C.byref(tot)), "exact_basis")
+        return masks[:rank], ws[:rank], tot.value, rank
+
+    def weight_order(self, m, cols, k):
+        """First k nonzero masks in (weight, Gray index) order."""
+        m = np.ascontiguousarray(m, dtype=np.uint64)
+        t = m.shape[0]
+        idx = np.arange(t, dtype=np.int32)
+        masks, ws = np.zeros(max(k, 1), np.uint32), np.zeros(max(k, 1), np.uint32)
+        n = _check(self.lib.or_weight_order(m, m.shape[1], cols, idx, t, k, masks, ws),
+                   "weight_order")
+        return masks[:n], ws[:n]
+
     def gray_min(self, m, block_rows, target, words=None):
         m = np.ascontiguousarray(m, dtype=np.uint64)
         words = m.shape[1] if words is None else words
@@ -246,6 +276,8 @@ class Reference:
         L.ref_greedy_apply.argtypes = [_u64p, C.c_int, C.c_int, C.c_int, _u64p, _u32p,
                                        C.c_size_t, C.c_uint64, C.c_int, C.c_uint64,
                                        C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)] + ep
+        L.ref_exact_basis.argtypes = [_u64p, C.c_int, C.c_int, C.c_int, _u32p, _u32p,
+                                      C.c_size_t, C.POINTER(C.c_uint64), _u64p] + ep
 
     @staticmethod
     def _err():
@@ -261,6 +293,17 @@ class Reference:
     @staticmethod
     def _rows(m):
         return np.ascontiguousarray(m, dtype=np.uint64)
+
+    def exact_basis(self, m, cols):
+        """Reference ExactBasisSolver: (masks, weights, total, rank, basis rows)."""
+        m = self._rows(m)
+        t, W = m.shape
+        masks, ws = np.zeros(max(t, 1), np.uint32), np.zeros(max(t, 1), np.uint32)
+        basis = np.zeros((max(t, 1), W), np.uint64)
+        tot = C.c_uint64()
+        rank = self._run(self.lib.ref_exact_basis, m, t, W, cols, masks, ws, t, C.byref(tot),
+                         basis)
+        return masks[:rank], ws[:rank], tot.value, rank, basis[:rank]
 
     def fwht_search(self, m, cols, k, max_rows=64, cap=None):
         m = self._rows(m)
@@ -341,9 +384,11 @@ def oracle() -> Oracle:
     return Oracle()
 
 
-def reference() -> Reference | None:
-    """The compiled reference, or None where it was not built."""
-    return Reference() if os.path.exists(REF_SO) else None
+def reference(scalar: bool = False) -> Reference | None:
+    """The compiled reference (scalar: without its AVX2 backend), or None
+    where it was not built."""
+    path = REF_SCALAR_SO if scalar else REF_SO
+    return Reference(path) if os.path.exists(path) else None
 
 
 # -- small pure-numpy helpers shared by tests ------------------------------

@@ -26,6 +26,11 @@
  *   or_replay          reference RowWindow::row_xor replay golden; weight
  *                      identity wt(a^b) = wt(a)+wt(b)-2 pop(a&b).
  *   or_greedy_apply    reference GreedyApplier golden.
+ *   or_exact_basis,    reference ExactBasisSolver KAT (tests/python/
+ *   or_weight_order    test_matrix_api.py: 3x3, rank 2, total 3) and the live
+ *                      reference on random and rank-deficient windows;
+ *                      the order against a sort of all masks by
+ *                      (weight, inverse Gray index).
  *   generators         not reference behaviour (workload definitions); the
  *                      product generator must match them bit for bit, and
  *                      rank/inverse properties are checked directly (incl. a
@@ -791,4 +796,146 @@ int or_greedy_apply(uint64_t *rows, int n_rows, int stride, int cols, const uint
     *out_applied = applied;
     *out_improvement = improvement;
     return 0;
+}
+
+/* ----------------------------------------------------- exact basis --- */
+/* src/solver/exact_basis_solver.cpp:29-54: Gray enumeration of all nonzero
+ * masks, weight = popcount of the running XOR (whole words; padding bits are
+ * zero).  Fills w[i - 1] = weight(gray(i)). */
+static int eb_weights(const uint64_t *base, int stride, int cols, const int *idx, int t,
+                      uint32_t *w) {
+    const int words = words_for(cols);
+    uint64_t *cur = calloc((size_t)(words > 0 ? words : 1), sizeof(uint64_t));
+    if (!cur)
+        return OR_ENOMEM;
+    const uint64_t states = ((uint64_t)1 << t) - 1;
+    uint32_t prev = 0;
+    for (uint64_t i = 1; i <= states; ++i) {
+        const uint32_t g = (uint32_t)(i ^ (i >> 1));
+        const int r = __builtin_ctz(g ^ prev);
+        const uint64_t *row = base + (size_t)idx[r] * stride;
+        uint32_t pc = 0;
+        for (int k = 0; k < words; ++k) {
+            cur[k] ^= row[k];
+            pc += (uint32_t)__builtin_popcountll(cur[k]);
+        }
+        w[i - 1] = pc;
+        prev = g;
+    }
+    free(cur);
+    return 0;
+}
+
+/* :105-124: stable counting sort of the Gray stream by weight. */
+static long long eb_order(const uint32_t *w, uint64_t states, int cols, uint32_t *out_idx) {
+    size_t *off = calloc((size_t)cols + 2, sizeof(size_t));
+    if (!off)
+        return OR_ENOMEM;
+    for (uint64_t i = 0; i < states; ++i)
+        ++off[w[i] + 1];
+    for (int q = 0; q <= cols; ++q)
+        off[q + 1] += off[q];
+    for (uint64_t i = 0; i < states; ++i)
+        out_idx[off[w[i]]++] = (uint32_t)(i + 1);
+    free(off);
+    return (long long)states;
+}
+
+long long or_weight_order(const uint64_t *base, int stride, int cols, const int *idx, int t,
+                          size_t k, uint32_t *out_masks, uint32_t *out_w) {
+    if (t < 1 || t > 24 || cols < 0)
+        return OR_EINVAL;
+    const uint64_t states = ((uint64_t)1 << t) - 1;
+    uint32_t *w = malloc(sizeof(uint32_t) * states);
+    uint32_t *order = malloc(sizeof(uint32_t) * states);
+    if (!w || !order) {
+        free(w);
+        free(order);
+        return OR_ENOMEM;
+    }
+    int rc = eb_weights(base, stride, cols, idx, t, w);
+    if (rc == 0 && eb_order(w, states, cols, order) < 0)
+        rc = OR_ENOMEM;
+    long long n = 0;
+    if (rc == 0) {
+        n = (long long)(k < states ? k : states);
+        for (long long q = 0; q < n; ++q) {
+            const uint32_t i = order[q];
+            out_masks[q] = i ^ (i >> 1);
+            out_w[q] = w[i - 1];
+        }
+    }
+    free(w);
+    free(order);
+    return rc ? rc : n;
+}
+
+/* :58-157: greedy pivot-basis selection over the ordered stream. */
+int or_exact_basis(const uint64_t *base, int stride, int cols, const int *idx, int t,
+                   uint32_t *out_masks, uint32_t *out_w, size_t cap, uint64_t *out_total) {
+    if (t < 1 || t > 24 || cols < 0)
+        return OR_EINVAL;
+    const int words = words_for(cols) > 0 ? words_for(cols) : 1;
+    const uint64_t states = ((uint64_t)1 << t) - 1;
+    uint32_t *w = malloc(sizeof(uint32_t) * states);
+    uint32_t *order = malloc(sizeof(uint32_t) * states);
+    uint64_t *piv = calloc((size_t)t * words, sizeof(uint64_t)); /* reduced rows by pivot */
+    int *piv_bit = malloc(sizeof(int) * (size_t)t);
+    uint64_t *v = malloc(sizeof(uint64_t) * words);
+    uint64_t *sub = malloc(sizeof(uint64_t) * (size_t)t * words);
+    int rc = (!w || !order || !piv || !piv_bit || !v || !sub) ? OR_ENOMEM : 0;
+    if (rc == 0)
+        rc = eb_weights(base, stride, cols, idx, t, w);
+    if (rc == 0 && eb_order(w, states, cols, order) < 0)
+        rc = OR_ENOMEM;
+    int rank = 0, n_piv = 0;
+    uint64_t total = 0;
+    if (rc == 0) {
+        for (int r = 0; r < t; ++r)
+            memcpy(sub + (size_t)r * words, base + (size_t)idx[r] * stride, sizeof(uint64_t) * words);
+        rank = or_rank(sub, t, words, cols);
+        for (uint64_t q = 0; q < states && n_piv < rank; ++q) {
+            const uint32_t i = order[q];
+            const uint32_t mask = i ^ (i >> 1);
+            memset(v, 0, sizeof(uint64_t) * words);
+            for (int r = 0; r < t; ++r)
+                if ((mask >> r) & 1u)
+                    for (int k2 = 0; k2 < words; ++k2)
+                        v[k2] ^= sub[(size_t)r * words + k2];
+            /* reduce by existing pivots (highest set bit of each) */
+            for (int p = 0; p < n_piv; ++p)
+                if ((v[piv_bit[p] >> 6] >> (piv_bit[p] & 63)) & 1u)
+                    for (int k2 = 0; k2 < words; ++k2)
+                        v[k2] ^= piv[(size_t)p * words + k2];
+            int hb = -1;
+            for (int k2 = words - 1; k2 >= 0 && hb < 0; --k2)
+                if (v[k2])
+                    hb = k2 * 64 + 63 - __builtin_clzll(v[k2]);
+            if (hb < 0)
+                continue;
+            /* keep pivots fully reduced against the new one */
+            for (int p = 0; p < n_piv; ++p)
+                if ((piv[(size_t)p * words + (hb >> 6)] >> (hb & 63)) & 1u)
+                    for (int k2 = 0; k2 < words; ++k2)
+                        piv[(size_t)p * words + k2] ^= v[k2];
+            memcpy(piv + (size_t)n_piv * words, v, sizeof(uint64_t) * words);
+            piv_bit[n_piv] = hb;
+            if ((size_t)n_piv < cap) {
+                out_masks[n_piv] = mask;
+                out_w[n_piv] = w[i - 1];
+            }
+            total += w[i - 1];
+            ++n_piv;
+        }
+    }
+    free(w);
+    free(order);
+    free(piv);
+    free(piv_bit);
+    free(v);
+    free(sub);
+    if (rc)
+        return rc;
+    *out_total = total;
+    return n_piv;
 }

@@ -89,6 +89,18 @@ int or_greedy_apply(uint64_t *rows, int n_rows, int stride, int cols, const uint
                     const uint32_t *weights, size_t n_cand, uint64_t min_gain, int stochastic,
                     uint64_t seed, uint64_t *out_applied, uint64_t *out_improvement);
 
+/* ExactBasisSolver::solve (src/solver/exact_basis_solver.cpp:29-157): masks
+ * gray(i), i = 1 .. 2^t - 1, counting-sorted by weight (stable), inserted
+ * greedily into a GF(2) pivot basis until the rank is reached.  Writes the
+ * chosen masks / weights (selection order, up to cap) and the total; returns
+ * the rank (t <= 24). */
+int or_exact_basis(const uint64_t *base, int stride, int cols, const int *idx, int t,
+                   uint32_t *out_masks, uint32_t *out_w, size_t cap, uint64_t *out_total);
+
+/* The first min(k, 2^t - 1) masks of that order (the GPU candidate stream). */
+long long or_weight_order(const uint64_t *base, int stride, int cols, const int *idx, int t,
+                          size_t k, uint32_t *out_masks, uint32_t *out_w);
+
 #ifdef __cplusplus
 }
 #endif

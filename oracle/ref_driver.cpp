@@ -17,7 +17,9 @@
 #include "bmmpy/search/fwht_search.hpp"
 #include "bmmpy/search/mitm_fwht_search.hpp"
 #include "bmmpy/search/split_window_prep.hpp"
+#include "bmmpy/solver/exact_basis_solver.hpp"
 
+#include <algorithm>
 #include <cstdint>
 #include <cstring>
 #include <exception>
@@ -186,6 +188,32 @@ int ref_greedy_apply(std::uint64_t* rows, int t, int W, int cols, const std::uin
             std::memcpy(rows + static_cast<std::size_t>(rr) * W, m.row_words(rr),
                         sizeof(std::uint64_t) * W);
         return 0;
+    } catch (const std::exception& e) {
+        set_err(err, err_len, e.what());
+        return -1;
+    }
+}
+
+// ExactBasisSolver (default caps) on all t rows: writes up to `cap` basis
+// masks / weights in selection order, the total weight and the basis matrix
+// rows (rank x W words); returns the rank.
+int ref_exact_basis(const std::uint64_t* rows, int t, int W, int cols, std::uint32_t* out_masks,
+                    std::uint32_t* out_w, std::size_t cap, std::uint64_t* total,
+                    std::uint64_t* basis_rows, char* err, int err_len) {
+    try {
+        auto m = load(rows, t, W, cols);
+        auto win = m.row_window(all_rows(t));
+        bmmpy::ExactBasisSolver s;
+        const auto r = s.solve(win);
+        const std::size_t n = std::min(cap, r.basis_masks.size());
+        for (std::size_t i = 0; i < n; ++i) {
+            out_masks[i] = r.basis_masks[i];
+            out_w[i] = r.basis_weights[i];
+            std::memcpy(basis_rows + i * static_cast<std::size_t>(W),
+                        r.basis_matrix.row_words(i), sizeof(std::uint64_t) * W);
+        }
+        *total = r.total_weight;
+        return static_cast<int>(r.rank);
     } catch (const std::exception& e) {
         set_err(err, err_len, e.what());
         return -1;

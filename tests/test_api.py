@@ -211,3 +211,21 @@ def test_gpu_greedy_apply_matches_oracle():
 def test_gpu_fwht_math_wrappers():
     assert bmm.fwht_i32([1, 2, 3, 4]).tolist() == [10, -2, -4, 0]
     assert bmm.fwht_i16([32767, 1]).tolist() == [-32768, 32766]
+
+
+def test_exact_basis_caps_and_empty():
+    """ExactBasisSolver's caps and early returns (src/solver/exact_basis_solver.cpp:
+    58-96) are decided before any device work."""
+    mat = bmm.matrix_from_rows(["1" * 10])
+    empty = bmm.solver.ExactBasisSolver().solve(mat.row_window([]))
+    assert empty.rank == 0 and empty.input_rows == 0 and empty.basis_matrix.shape == (0, 10)
+    zero = bmm.matrix_from_rows(["000", "000"])
+    r0 = bmm.solver.ExactBasisSolver().solve(zero.row_window([0, 1]))
+    assert r0.rank == 0 and r0.enumerated_states == 3 and r0.transform_matrix.shape == (0, 2)
+    big = bmm.matrix_from_rows(["1"] * 5)
+    with pytest.raises(ValueError, match="max_rows"):
+        bmm.solver.ExactBasisSolver(max_rows=4).solve(big.row_window(list(range(5))))
+    with pytest.raises(ValueError, match="max_states"):
+        bmm.solver.ExactBasisSolver(max_states=30).solve(big.row_window(list(range(5))))
+    with pytest.raises(ValueError, match="storage"):
+        bmm.solver.ExactBasisSolver(max_storage_bytes=100).solve(big.row_window(list(range(5))))

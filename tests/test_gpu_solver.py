@@ -1,0 +1,64 @@
+"""GPU parity of ExactBasisSolver (the next row of the hot path: a minimum-
+total-weight basis from the (weight, Gray index) candidate stream) against
+the oracle, which is pinned to the reference solver (tests/test_oracle.py)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print(*a, flush=True)
+
+
+@pytest.mark.parametrize("t,cols,p,k", [(1, 10, 0.5, 5), (3, 64, 0.4, 7), (8, 130, 0.3, 255),
+                                        (12, 200, 0.2, 4095), (16, 96, 0.5, 3000),
+                                        (20, 1024, 0.1, 20000), (24, 256, 0.3, 500)])
+def test_weight_order_matches_oracle(lib, ora, t, cols, p, k):
+    _say("weight order", t, cols, k)
+    m = ora.bernoulli(7000 + t, t, cols, p)
+    gm, gw = lib.weight_order(m, cols, k)
+    om, ow = ora.weight_order(m, cols, k)
+    assert np.array_equal(gm, om) and np.array_equal(gw, ow)
+
+
+def _matrix(bmm, m, cols):
+    rows = []
+    for r in m:
+        bits = "".join("1" if (int(r[c // 64]) >> (c % 64)) & 1 else "0" for c in range(cols))
+        rows.append(bits)
+    return bmm.matrix_from_rows(rows)
+
+
+@pytest.mark.parametrize("t,cols,p,mode", [(3, 3, 0.0, "kat"), (6, 70, 0.3, "rand"),
+                                           (12, 150, 0.3, "deficient"), (16, 300, 0.1, "rand"),
+                                           (20, 128, 0.4, "rand"), (24, 200, 0.2, "rand")])
+def test_exact_basis_matches_oracle(lib, ora, t, cols, p, mode):
+    import bmmpy_amd as bmm
+    _say("exact basis", t, cols, mode)
+    if mode == "kat":   # the reference's own example (tests/python/test_matrix_api.py:45-65)
+        mat = bmm.matrix_from_rows(["111", "110", "001"])
+        import oracle
+        m, cols = oracle.matrix_from_rows(["111", "110", "001"])
+    else:
+        m = ora.bernoulli(7100 + t, t, cols, p)
+        if mode == "deficient":
+            m[3] = m[7]
+            m[5] = 0
+            m[9] = m[1] ^ m[2] ^ m[4]
+        mat = _matrix(bmm, m, cols)
+    res = bmm.solver.ExactBasisSolver().solve(mat.row_window(list(range(t))))
+    om, ow, ot, orank = ora.exact_basis(m, cols)
+    assert res.input_rows == t and res.cols == cols
+    assert res.rank == orank and res.total_weight == ot
+    assert res.enumerated_states == (1 << t) - 1
+    assert list(res.basis_masks) == om.tolist() and list(res.basis_weights) == ow.tolist()
+    assert res.transform_matrix.shape == (orank, t)
+    assert res.basis_matrix.shape == (orank, cols)
+    assert res.basis_matrix.rank() == orank
+    assert res.basis_matrix.weight() == ot
+    for q, mask in enumerate(om):
+        for r in range(t):
+            assert res.transform_matrix[q, r] == bool((int(mask) >> r) & 1)
+    if mode == "kat":
+        assert res.rank == 2 and res.total_weight == 3 and sorted(res.basis_weights) == [1, 2]

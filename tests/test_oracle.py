@@ -298,3 +298,74 @@ def test_linear_bits_cap_rank():
     XorShift64 leave a 100 x 100 matrix far from full rank."""
     o = oracle.oracle()
     assert o.rank(o.bernoulli(5, 100, 100, 0.5), 100) <= 66
+
+
+# ------------------------------------------------------- exact basis -----
+def _ginv(m):
+    g = 0
+    while m:
+        g ^= m
+        m >>= 1
+    return g
+
+
+def test_exact_basis_kat(ora):
+    """tests/python/test_matrix_api.py:45-65 of the reference: rows 111, 110,
+    001 -> rank 2, total weight 3, basis weights {1, 2}."""
+    m, cols = oracle.matrix_from_rows(["111", "110", "001"])
+    masks, ws, total, rank = ora.exact_basis(m, cols)
+    assert rank == 2 and total == 3 and sorted(ws.tolist()) == [1, 2]
+    r = oracle.reference(scalar=True)
+    if r is not None:
+        rm, rw, rt, rr, rb = r.exact_basis(m, cols)
+        assert (rr, rt) == (2, 3)
+        assert rm.tolist() == masks.tolist() and rw.tolist() == ws.tolist()
+
+
+def test_weight_order_is_weight_then_gray_index(ora):
+    """The candidate stream against a direct sort of every nonzero mask by
+    (weight, inverse Gray index), weights by explicit XOR + popcount."""
+    for seed, (t, cols, p) in enumerate([(5, 70, 0.4), (9, 130, 0.2), (12, 64, 0.5)]):
+        m = ora.bernoulli(4000 + seed, t, cols, p)
+        keys = []
+        for mask in range(1, 1 << t):
+            acc = np.zeros(m.shape[1], np.uint64)
+            for r in range(t):
+                if (mask >> r) & 1:
+                    acc ^= m[r]
+            keys.append((sum(bin(int(v)).count("1") for v in acc), _ginv(mask), mask))
+        keys.sort()
+        k = min(len(keys), 300)
+        masks, ws = ora.weight_order(m, cols, k)
+        assert masks.tolist() == [x[2] for x in keys[:k]]
+        assert ws.tolist() == [x[0] for x in keys[:k]]
+
+
+def test_exact_basis_matches_live_reference(ora):
+    """Random, rank-deficient (duplicate / zero / dependent rows) and tail-bit
+    windows against the reference ExactBasisSolver compiled here (its scalar
+    build: the AVX2 build of this solver crashes on rows of >= 4 words)."""
+    r = oracle.reference(scalar=True)
+    if r is None:
+        pytest.skip("reference CPU library not built here")
+    cases = []
+    for seed, (t, cols, p) in enumerate([(1, 10, 0.5), (4, 65, 0.3), (10, 200, 0.1),
+                                         (14, 96, 0.5), (18, 300, 0.05)]):
+        cases.append((ora.bernoulli(5000 + seed, t, cols, p), cols))
+    m = ora.bernoulli(5100, 12, 150, 0.3)
+    m[3] = m[7]
+    m[5] = 0
+    m[9] = m[1] ^ m[2] ^ m[4]
+    cases.append((m, 150))
+    for m, cols in cases:
+        masks, ws, total, rank = ora.exact_basis(m, cols)
+        rm, rw, rt, rr, rb = r.exact_basis(m, cols)
+        assert rank == rr and total == rt
+        assert masks.tolist() == rm.tolist() and ws.tolist() == rw.tolist()
+        # basis rows = transform x window rows
+        for q, mask in enumerate(masks):
+            acc = np.zeros(m.shape[1], np.uint64)
+            for row in range(m.shape[0]):
+                if (int(mask) >> row) & 1:
+                    acc ^= m[row]
+            assert np.array_equal(acc, rb[q])
