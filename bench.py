@@ -17,6 +17,9 @@ Other workloads (one JSON line each, same fields):
                      invertible 48 x 48, seed 0x5EED0003): per target, the
                      target plus the 40 lowest-index other rows (2^40 subsets);
                      a step is one target.
+  --workload basis   ExactBasisSolver on a 24 x 1024 Bernoulli(0.3) window
+                     (seed 0x5EED0007): all 2^24 - 1 row combinations weighed,
+                     minimum-total-weight basis; a step is one solve.
   --workload ga      plain vs learned island GA on the planted 256 x 8192.
 
 A step is one full pass of the workload with the inputs already resident in
@@ -308,6 +311,74 @@ def window_cpu_baseline(_lib, budget_s: float = 20.0) -> dict:
                       f"k=1 on {done} windows of 30 rows x 4096 cols (2^30 masks each), serial"}
 
 
+BASIS_SEED, BASIS_ROWS, BASIS_COLS, BASIS_P = 0x5EED0007, 24, 1024, 0.3
+
+
+def run_basis(torch, steps: int, warmup: int) -> dict:
+    """ExactBasisSolver on a 24 x 1024 Bernoulli(0.3) window: every one of the
+    2^24 - 1 nonzero row combinations is weighed (one FWHT of 2^24 int32 in
+    HBM), the (weight, Gray index) stream is selected on the GPU and the
+    basis assembled on the host.  A step is one solve."""
+    import bmmpy_amd as bmm
+    from bmmpy_amd import _lib
+
+    m = _lib.gen_bernoulli(BASIS_SEED, BASIS_ROWS, BASIS_COLS, BASIS_P)
+    rows = ["".join("1" if (int(r[c // 64]) >> (c % 64)) & 1 else "0" for c in range(BASIS_COLS))
+            for r in m]
+    mat = bmm.matrix_from_rows(rows)
+    win = mat.row_window(list(range(BASIS_ROWS)))
+    solver = bmm.solver.ExactBasisSolver()
+    for _ in range(warmup):
+        solver.solve(win)
+    _lib.timing_begin()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        res = solver.solve(win)
+    dt = (time.perf_counter() - t0) / steps
+    kms, nk = _lib.timing_end()
+    kernel_avg_ms = kms / max(nk, 1)
+    n = 1 << BASIS_ROWS
+    # Array FWHT passes over 2^24 int32: first pass 12 index bits, then 6 per
+    # pass (3 passes), each a read and a write of 4 bytes per element.
+    fwht_bytes = 3 * 8.0 * n
+    achieved = fwht_bytes / (kernel_avg_ms / 1e3) / 1e9
+    return {
+        "ms": 1e3 * dt, "evals": float(n - 1),
+        "results": [{"rank": int(res.rank), "total_weight": int(res.total_weight),
+                     "basis_weights": [int(x) for x in res.basis_weights]}],
+        "config": {"workload": "exact_basis_24x1024", "rows": BASIS_ROWS, "cols": BASIS_COLS,
+                   "p": BASIS_P, "seed": hex(BASIS_SEED), "subset_evals_per_step": n - 1},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": 8000.0, "unit": "GB/s",
+                     "frac": achieved / 8000.0, "traffic": None,
+                     "bytes_per_launch": fwht_bytes, "kernel": "fwht_chunk_kernel<int,13> + fwht_stride_kernel<int,6|5>",
+                     "kernel_avg_ms": kernel_avg_ms, "kernel_launches": nk,
+                     "solve_ms": 1e3 * dt},
+    }
+
+
+def basis_cpu_baseline(budget_s: float = 15.0) -> dict:
+    """The reference ExactBasisSolver (its scalar build) on the same window."""
+    import oracle  # checker / baseline only
+    from bmmpy_amd import _lib
+
+    m = _lib.gen_bernoulli(BASIS_SEED, BASIS_ROWS, BASIS_COLS, BASIS_P)
+    ref = oracle.reference(scalar=True)
+    done = 0
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < budget_s:
+        if ref is not None:
+            ref.exact_basis(m, BASIS_COLS)
+        else:
+            oracle.oracle().exact_basis(m, BASIS_COLS)
+        done += 1
+    dt = time.perf_counter() - t0
+    return {"value": done * float((1 << BASIS_ROWS) - 1) / dt, "unit": "subset evals/s",
+            "cores": 1, "kind": "reference" if ref is not None else "port",
+            "sample": f"{'reference' if ref is not None else 'C restatement'} ExactBasisSolver "
+                      f"on the 24 x 1024 window, {done} solves (two Gray enumerations of "
+                      f"2^24 - 1 masks each)"}
+
+
 def secondary(torch, _lib, dev, stream) -> dict:
     """The FWHT coset-minimum workloads, timed beside the headline (graph
     replay, autotuned launch shape)."""
@@ -393,7 +464,7 @@ def main() -> int:
     ap.add_argument("--chunk-bits", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--workload", default="gray",
-                    choices=["gray", "coset", "batch", "window", "ga"],
+                    choices=["gray", "coset", "batch", "window", "basis", "ga"],
                     help="gray: the Gray coset sweep (default); coset / batch / window: the "
                          "FWHT coset-minimum workloads; ga: plain vs learned GA")
     ap.add_argument("--ga-budget", type=float, default=60.0,
@@ -445,9 +516,12 @@ def main() -> int:
         return 0
 
     dev = torch.device("cuda:0")
-    if args.workload in ("coset", "batch", "window"):
+    if args.workload in ("coset", "batch", "window", "basis"):
         stream = torch.cuda.current_stream(dev)
-        if args.workload == "window":
+        if args.workload == "basis":
+            r = run_basis(torch, args.steps, args.warmup)
+            cpu = None if args.no_cpu_baseline else basis_cpu_baseline()
+        elif args.workload == "window":
             r = run_window(torch, _lib, args.steps, args.warmup)
             cpu = None if args.no_cpu_baseline else window_cpu_baseline(_lib)
         else:

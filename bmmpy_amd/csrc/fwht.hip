@@ -1,10 +1,9 @@
 // Walsh-Hadamard kernels on gfx950.
 //
-// 1. fwht_pass_kernel: the stand-alone in-place transform of a device array
-//    (fwht_inplace, src/math/fwht.cpp:28-57), as passes of up to 12 index
-//    bits each: the first pass transforms contiguous 4096-element chunks in
-//    LDS, later passes transform 64-column tiles strided by 2^lo, so every
-//    global access is a coalesced 256-byte row.
+// 1. fwht_chunk_kernel / fwht_stride_kernel: the stand-alone in-place
+//    transform of a device array (fwht_inplace, src/math/fwht.cpp:28-57):
+//    one LDS pass over the low 13 index bits, then register passes of 5-6
+//    bits with coalesced 256-byte wavefront accesses.
 //
 // 2. column_patterns_kernel: per-column row patterns of a small matrix.
 //
@@ -22,49 +21,102 @@
 namespace bmmhip {
 
 // ------------------------------------------------------ array transform --
+// fwht_inplace on a device array of 2^nb elements, wrapping arithmetic: the
+// radix-2 stages of the different index bits commute, so the transform is
+// applied as
+//   * one LDS pass over index bits [0, min(nb, 13)): each workgroup owns a
+//     contiguous chunk, loaded and stored as 16-byte vectors, transformed by
+//     block_fwht (radix-32 register rounds between LDS exchanges);
+//   * register passes over the remaining bits, 5 or 6 at a time: lane c of a
+//     wavefront owns column c of a 2^R x 64 tile at row stride 2^lo, so each
+//     of its 2^R loads and stores is one coalesced 256-byte wavefront access,
+//     and the R stages run in registers with no LDS or barriers.
+// Each pass reads and writes the array once (8 bytes per int32 element).
+// int16 arrays are computed in 32-bit lanes and truncated on store: the
+// transform is a ring homomorphism, so the result equals int16 wrapping.
 template <typename T> struct Unsigned;
 template <> struct Unsigned<int32_t> { using type = uint32_t; };
 template <> struct Unsigned<int16_t> { using type = uint16_t; };
 
-// Transforms index bits [lo, lo + pbits) of a 2^n array.  lo == 0: each
-// workgroup owns one contiguous chunk of 2^pbits elements.  lo > 0: each
-// workgroup owns a tile of 2^pbits rows (stride 2^lo) x cw adjacent columns.
-template <typename T>
-__global__ __launch_bounds__(256) void fwht_pass_kernel(T* __restrict__ data, int n, int lo,
-                                                        int pbits) {
-    using U = typename Unsigned<T>::type;
-    extern __shared__ unsigned char smem_raw[];
-    U* s = reinterpret_cast<U*>(smem_raw);
-    const int rows = 1 << pbits;
-    const int cw = lo == 0 ? 1 : min(64, 1 << lo);
-    const int tile = rows * cw;
-    // Tile origin: column group within the low part, then the high part.
-    const size_t low_groups = lo == 0 ? 1 : ((size_t)1 << lo) / cw;
-    const size_t gid = blockIdx.x;
-    const size_t col0 = (gid % low_groups) * cw;
-    const size_t hi = gid / low_groups;
-    const size_t origin = (hi << (lo + pbits)) + col0;
-    const size_t rstride = (size_t)1 << lo;
-    for (int i = threadIdx.x; i < tile; i += blockDim.x) {
-        const int r = i / cw, c = i % cw;
-        s[i] = (U)data[origin + (size_t)r * rstride + c];
-    }
-    __syncthreads();
-    for (int len = 1; len < rows; len <<= 1) {
-        for (int i = threadIdx.x; i < tile / 2; i += blockDim.x) {
-            const int pair = i / cw, c = i % cw;
-            const int grp = pair / len, off = pair % len;
-            const int r0 = grp * 2 * len + off, r1 = r0 + len;
-            const U a = s[r0 * cw + c], b = s[r1 * cw + c];
-            s[r0 * cw + c] = (U)(a + b);
-            s[r1 * cw + c] = (U)(a - b);
+constexpr int kArrBlockBits = 13;
+
+template <typename T, int L>
+__global__ __launch_bounds__(256) void fwht_chunk_kernel(T* __restrict__ data, int vec) {
+    __shared__ __align__(16) int32_t s[fwht_lds_words(L) + 4];
+    constexpr int N = 1 << L;
+    T* chunk = data + (size_t)blockIdx.x * N;
+    constexpr int V = 16 / sizeof(T);   // elements per 16-byte vector
+    if (N >= V * 256 && vec) {
+        for (int i = threadIdx.x; i < N / V; i += blockDim.x) {
+            const int4 q = reinterpret_cast<const int4*>(chunk)[i];
+            const T* e = reinterpret_cast<const T*>(&q);
+#pragma unroll
+            for (int k = 0; k < V; ++k)
+                s[fwht_pad(i * V + k)] = (int32_t)e[k];
         }
-        __syncthreads();
+    } else {
+        for (int i = threadIdx.x; i < N; i += blockDim.x)
+            s[fwht_pad(i)] = (int32_t)chunk[i];
     }
-    for (int i = threadIdx.x; i < tile; i += blockDim.x) {
-        const int r = i / cw, c = i % cw;
-        data[origin + (size_t)r * rstride + c] = (T)s[i];
+    block_fwht<L>(s);
+    if (N >= V * 256 && vec) {
+        for (int i = threadIdx.x; i < N / V; i += blockDim.x) {
+            int4 q;
+            T* e = reinterpret_cast<T*>(&q);
+#pragma unroll
+            for (int k = 0; k < V; ++k)
+                e[k] = (T)s[fwht_pad(i * V + k)];
+            reinterpret_cast<int4*>(chunk)[i] = q;
+        }
+    } else {
+        for (int i = threadIdx.x; i < N; i += blockDim.x)
+            chunk[i] = (T)s[fwht_pad(i)];
     }
+}
+
+// Index bits [lo, lo + R) with lo >= 13: thread (tile, c) transforms the 2^R
+// elements h 2^(lo+R) + e 2^lo + c, e < 2^R; 256 consecutive columns per
+// workgroup.
+template <typename T, int R>
+__global__ __launch_bounds__(256) void fwht_stride_kernel(T* __restrict__ data, int lo) {
+    const size_t cols = (size_t)1 << lo;
+    const size_t col = (size_t)blockIdx.x * 256 + threadIdx.x;   // over (h, c)
+    const size_t h = col >> lo, c = col & (cols - 1);
+    T* p = data + (h << (lo + R)) + c;
+    constexpr int E = 1 << R;
+    uint32_t v[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e)
+        v[e] = (uint32_t)(int32_t)p[(size_t)e << lo];
+#pragma unroll
+    for (int len = 1; len < E; len <<= 1)
+#pragma unroll
+        for (int i = 0; i < E; ++i)
+            if ((i & len) == 0) {
+                const uint32_t a = v[i], b = v[i + len];
+                v[i] = a + b;
+                v[i + len] = a - b;
+            }
+#pragma unroll
+    for (int e = 0; e < E; ++e)
+        p[(size_t)e << lo] = (T)(int32_t)v[e];
+}
+
+template <typename T> void launch_chunk_pass(T* d, size_t n, int L, hipStream_t stream) {
+    const unsigned grid = (unsigned)(n >> L);
+    const int vec = ((uintptr_t)d % 16) == 0;   // 16-byte vectors only on aligned arrays
+    switch (L) {
+#define BMM_L(LL)                                                                              \
+    case LL:                                                                                   \
+        hipLaunchKernelGGL((fwht_chunk_kernel<T, LL>), dim3(grid), dim3(256), 0, stream, d,   \
+                           vec);                                                               \
+        break;
+        BMM_L(1) BMM_L(2) BMM_L(3) BMM_L(4) BMM_L(5) BMM_L(6) BMM_L(7) BMM_L(8) BMM_L(9)
+        BMM_L(10) BMM_L(11) BMM_L(12) BMM_L(13)
+#undef BMM_L
+    default: fail(BMM_ERR_INVALID, "fwht_inplace: bad chunk size");
+    }
+    BMM_HIP(hipGetLastError());
 }
 
 template <typename T> void fwht_array_dev(T* d, size_t n, hipStream_t stream) {
@@ -75,16 +127,27 @@ template <typename T> void fwht_array_dev(T* d, size_t n, hipStream_t stream) {
     int nb = 0;
     while (((size_t)1 << nb) < n)
         ++nb;
-    int lo = 0;
+    const int L = std::min(nb, kArrBlockBits);
+    launch_chunk_pass(d, n, L, stream);
+    // Remaining bits in the fewest register passes of at most 6 bits, split
+    // evenly (11 bits: 6 + 5).
+    int lo = L;
     while (lo < nb) {
-        const int pbits = lo == 0 ? std::min(nb, 12) : std::min(nb - lo, 6);
-        const int cw = lo == 0 ? 1 : std::min(64, 1 << lo);
-        const size_t tile = ((size_t)1 << pbits) * cw;
-        const size_t grid = n / tile;
-        hipLaunchKernelGGL(fwht_pass_kernel<T>, dim3((unsigned)grid), dim3(256),
-                           tile * sizeof(T), stream, d, nb, lo, pbits);
+        const int rem = nb - lo;
+        const int passes = (rem + 5) / 6;
+        const int R = (rem + passes - 1) / passes;
+        const unsigned grid = (unsigned)((n >> R) / 256);
+        switch (R) {
+#define BMM_R(RR)                                                                              \
+    case RR:                                                                                   \
+        hipLaunchKernelGGL((fwht_stride_kernel<T, RR>), dim3(grid), dim3(256), 0, stream, d, lo); \
+        break;
+            BMM_R(1) BMM_R(2) BMM_R(3) BMM_R(4) BMM_R(5) BMM_R(6)
+#undef BMM_R
+        default: fail(BMM_ERR_INVALID, "fwht_inplace: bad stride pass");
+        }
         BMM_HIP(hipGetLastError());
-        lo += pbits;
+        lo += R;
     }
 }
 

@@ -152,8 +152,10 @@ size_t weight_order(const uint64_t* rows, size_t t, size_t words, size_t cols, s
         hipLaunchKernelGGL(order_hist_kernel, dim3((unsigned)((cols + 255) / 256)), dim3(256), 0,
                            nullptr, d_rows, (int)t, (int)words, (int)cols, d_h, d_total);
     BMM_HIP(hipGetLastError());
-    if (t > 0)
+    if (t > 0) {
+        ScopedKernelTimer timer(nullptr);
         fwht_array_dev<int32_t>(d_h, (size_t)n, nullptr);
+    }
     unsigned int total = 0;
     BMM_HIP(hipMemcpy(&total, d_total, 4, hipMemcpyDeviceToHost));
     const unsigned grid = (unsigned)std::min<uint64_t>((n + 255) / 256, 2048);

@@ -18,7 +18,7 @@ def test_fwht_kats(lib):
     assert lib.fwht_i16([32767, 1]).tolist() == [-32768, 32766]
 
 
-@pytest.mark.parametrize("n_bits", [1, 5, 12, 13, 18, 21])
+@pytest.mark.parametrize("n_bits", [1, 5, 12, 13, 14, 18, 19, 21, 24, 25])
 def test_fwht_i32_matches_oracle(lib, ora, n_bits):
     _say("fwht_i32", n_bits)
     rng = np.random.default_rng(n_bits)
@@ -26,12 +26,30 @@ def test_fwht_i32_matches_oracle(lib, ora, n_bits):
     assert np.array_equal(lib.fwht_i32(x), ora.fwht_i32(x))
 
 
-@pytest.mark.parametrize("n_bits", [3, 12, 16, 19])
+@pytest.mark.parametrize("n_bits", [3, 11, 12, 16, 19, 24])
 def test_fwht_i16_wraps_like_oracle(lib, ora, n_bits):
     _say("fwht_i16", n_bits)
     rng = np.random.default_rng(100 + n_bits)
     x = rng.integers(-2**15, 2**15, size=1 << n_bits, dtype=np.int64).astype(np.int16)
     assert np.array_equal(lib.fwht_i16(x), ora.fwht_i16(x))
+
+
+@pytest.mark.parametrize("n_bits", [12, 20])
+def test_fwht_dev_unaligned_array(lib, ora, n_bits):
+    """A device array 4 bytes off a 16-byte boundary takes the element-wise
+    load path of the first pass."""
+    import torch
+    _say("fwht unaligned", n_bits)
+    rng = np.random.default_rng(50 + n_bits)
+    x = rng.integers(-2**31, 2**31, size=1 << n_bits, dtype=np.int64).astype(np.int32)
+    buf = torch.zeros((1 << n_bits) + 1, dtype=torch.int32, device="cuda:0")
+    view = buf[1:]
+    view.copy_(torch.from_numpy(x))
+    torch.cuda.synchronize()
+    assert view.data_ptr() % 16 != 0
+    lib.call("bmm_hip_fwht_i32_dev", view.data_ptr(), x.size, None)
+    torch.cuda.synchronize()
+    assert np.array_equal(view.cpu().numpy(), ora.fwht_i32(x))
 
 
 def test_gray_c2_full_config(lib, ora):
