@@ -31,6 +31,8 @@ from .search import (
     CudaMitmFwhtSearch,
     FwhtSearch,
     MitmFwhtSearch,
+    SASelectionResult,
+    SASelector,
 )
 from .workflows import search_apply
 from . import solver  # noqa: E402  (bmm.solver.ExactBasisSolver, as in the reference)
@@ -38,7 +40,7 @@ from . import solver  # noqa: E402  (bmm.solver.ExactBasisSolver, as in the refe
 __all__ = [
     "ApplyResult", "BitMatrix", "BruteforceSearch", "Candidate", "CudaBruteforceSearch",
     "CudaMitmFwhtSearch", "FwhtSearch", "GreedyApplier", "MatrixErr", "MatrixError",
-    "MitmFwhtSearch", "RowWindow", "RuntimeFeatures", "add", "fwht_i16", "fwht_i32",
+    "MitmFwhtSearch", "RowWindow", "SASelectionResult", "SASelector", "RuntimeFeatures", "add", "fwht_i16", "fwht_i32",
     "get_runtime_features", "get_version", "matrix_from_rows", "search_apply", "solver",
 ]
 

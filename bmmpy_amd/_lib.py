@@ -115,6 +115,7 @@ SIGNATURES = {
     "bmm_hip_mlp_get_params": ([_vp, _f32p], C.c_int),
     "bmm_hip_mlp_forward": ([_vp, _f32p, _sz, _f32p], C.c_int),
     "bmm_hip_mlp_score_pairs": ([_vp, _u64p, _sz, _u16p, _sz, _f32p], C.c_int),
+    "bmm_hip_pair_overlaps": ([_u64p, _sz, _sz, _u32p], C.c_int),
     "bmm_hip_weight_order": ([_u64p, _sz, _sz, _sz, _sz, _u32p, _u32p, C.POINTER(_sz)], C.c_int),
     "bmm_hip_mlp_gradients": ([_vp, _f32p, _f32p, _sz, _f32p, C.POINTER(C.c_float)], C.c_int),
     "bmm_hip_mlp_train_step": ([_vp, _f32p, _f32p, _sz, C.c_float, C.POINTER(C.c_float)],
@@ -267,6 +268,15 @@ def window_coset_min(rows, cols: int, targets, windows):
     call("bmm_hip_window_coset_min", rows, rows.shape[0], rows.shape[1], cols,
          windows.reshape(-1), width, targets, n, w, m)
     return w, m
+
+
+def pair_overlaps(rows):
+    """popcount(row_i & row_j) for every pair (0 on the diagonal)."""
+    rows = _rows(rows)
+    n = rows.shape[0]
+    out = np.zeros((n, n), np.uint32)
+    call("bmm_hip_pair_overlaps", rows, n, rows.shape[1], out.reshape(-1))
+    return out
 
 
 def weight_order(rows, cols: int, k: int):

@@ -24,7 +24,8 @@ HIP_LIB = os.path.join(HERE, "libbmmpy_hip.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 HIP_SOURCES = ["capi.hip", "gray.hip", "fwht.hip", "search.hip", "replay.hip", "mitm.hip", "bruteforce.hip",
-               "batch.hip", "ga.hip", "mlp.hip", "workloads.hip", "order.hip"]
+               "batch.hip", "ga.hip", "mlp.hip", "workloads.hip", "order.hip",
+               "overlap.hip"]
 HIP_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-Wall",
     "-Wno-unused-function",
@@ -59,7 +60,7 @@ def build_hip(jobs: int = 8) -> str:
 
 
 HOST_SOURCES = ["host/bit_matrix.cpp", "host/search.cpp", "host/ga.cpp", "host/apply.cpp",
-                "host/runtime.cpp", "host/solver.cpp"]
+                "host/runtime.cpp", "host/solver.cpp", "host/sa_selector.cpp"]
 
 
 def build_host(jobs: int = 8) -> str | None:

@@ -87,6 +87,7 @@ void mlp_get_params(Mlp* M, float* host);
 void mlp_forward_host(Mlp* M, const float* x, size_t B, float* y);
 size_t weight_order(const uint64_t* rows, size_t t, size_t words, size_t cols, size_t k,
                     uint32_t* out_mask, uint32_t* out_weight);
+void pair_overlaps(const uint64_t* rows, size_t n, size_t words, uint32_t* out);
 void mlp_score_pairs_host(Mlp* M, const uint64_t* sketch, size_t rows, const uint16_t* pairs,
                           size_t n, float* y);
 float mlp_grads_host(Mlp* M, const float* x, const float* t, size_t B, float* grads);
@@ -599,6 +600,21 @@ int bmm_hip_mlp_forward(void* mlp, const float* x, size_t batch, float* y) {
         if (!mlp || !x || !y || batch == 0)
             fail(BMM_ERR_INVALID, "mlp_forward: null pointer or empty batch");
         mlp_forward_host(static_cast<Mlp*>(mlp), x, batch, y);
+    });
+}
+
+int bmm_hip_pair_overlaps(const uint64_t* row_words, size_t n, size_t words, uint32_t* out) {
+    return guarded([&] {
+        if ((n > 0 && !out) || (n > 0 && words > 0 && !row_words))
+            fail(BMM_ERR_INVALID, "pair_overlaps: null pointer");
+        if (n > 65536)
+            fail(BMM_ERR_INVALID, "pair_overlaps: at most 65536 rows");
+        if (words >= ((size_t)1 << 25))
+            fail(BMM_ERR_OVERFLOW, "pair_overlaps: overlaps would exceed 31 bits");
+        if (n == 0)
+            return;
+        require_device();
+        pair_overlaps(row_words, n, words, out);
     });
 }
 

@@ -295,6 +295,43 @@ PYBIND11_MODULE(_bmmpy, m) {
         .def("apply", &GreedyApplier::apply, py::arg("window"), py::arg("candidates"));
     m.def("replay_row_xors", &replay_row_xors, py::arg("window"), py::arg("ops"));
 
+    py::enum_<WindowScorePolicyKind>(m, "WindowScorePolicyKind")
+        .value("PairwiseSynergy", WindowScorePolicyKind::PairwiseSynergy)
+        .value("HigherOrderSynergy", WindowScorePolicyKind::HigherOrderSynergy);
+    py::enum_<CoolingPolicyKind>(m, "CoolingPolicyKind")
+        .value("AdaptiveGeometric", CoolingPolicyKind::AdaptiveGeometric);
+    py::class_<SASelectorConfig>(m, "SASelectorConfig")
+        .def(py::init<>())
+        .def_readwrite("iterations", &SASelectorConfig::iterations)
+        .def_readwrite("restarts", &SASelectorConfig::restarts)
+        .def_readwrite("seed", &SASelectorConfig::seed)
+        .def_readwrite("score_policy", &SASelectorConfig::score_policy)
+        .def_readwrite("cooling_policy", &SASelectorConfig::cooling_policy)
+        .def_readwrite("temperature_probe_samples", &SASelectorConfig::temperature_probe_samples)
+        .def_readwrite("initial_acceptance_probability",
+                       &SASelectorConfig::initial_acceptance_probability)
+        .def_readwrite("cooling_rate", &SASelectorConfig::cooling_rate)
+        .def_readwrite("min_temperature", &SASelectorConfig::min_temperature);
+    py::class_<SASelectionResult>(m, "SASelectionResult")
+        .def(py::init<>())
+        .def_readwrite("rows", &SASelectionResult::rows)
+        .def_readwrite("score", &SASelectionResult::score)
+        .def_readwrite("accepted_moves", &SASelectionResult::accepted_moves)
+        .def_readwrite("iterations_run", &SASelectionResult::iterations_run)
+        .def_readwrite("best_iteration", &SASelectionResult::best_iteration)
+        .def_readwrite("restart_index", &SASelectionResult::restart_index)
+        .def_readwrite("seed", &SASelectionResult::seed);
+    py::class_<SASelector>(m, "SASelector")
+        .def(py::init<SASelectorConfig>(), py::arg("config") = SASelectorConfig{})
+        .def("name", &SASelector::name)
+        .def("select", &SASelector::select, py::arg("matrix"), py::arg("window_size"))
+        .def(
+            "select_window",
+            [](const SASelector& s, BitMatrix& matrix, std::size_t window_size) {
+                return s.select_window(matrix, window_size);
+            },
+            py::arg("matrix"), py::arg("window_size"), py::keep_alive<0, 2>());
+
     py::class_<ExactBasisSolverConfig>(m, "ExactBasisSolverConfig")
         .def(py::init<>())
         .def_readwrite("max_rows", &ExactBasisSolverConfig::max_rows)

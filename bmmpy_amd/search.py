@@ -18,6 +18,11 @@ from ._bmmpy import (
     MitmFwhtSearch as _NativeMitmFwhtSearch,
     MitmFwhtSearchConfig as _NativeMitmFwhtSearchConfig,
     RowWindow,
+    CoolingPolicyKind as _NativeCoolingPolicyKind,
+    SASelectionResult,
+    SASelector as _NativeSASelector,
+    SASelectorConfig as _NativeSASelectorConfig,
+    WindowScorePolicyKind as _NativeWindowScorePolicyKind,
 )
 
 
@@ -147,5 +152,75 @@ class CudaBruteforceSearch:
         return self._impl.search(window)
 
 
+def _resolve_score_policy(name: str):
+    if name == "pairwise_synergy":
+        return _NativeWindowScorePolicyKind.PairwiseSynergy
+    if name == "higher_order_synergy":
+        return _NativeWindowScorePolicyKind.HigherOrderSynergy
+    raise ValueError(f"Unsupported score_policy: {name!r}")
+
+
+def _resolve_cooling_policy(name: str):
+    if name == "adaptive_geometric":
+        return _NativeCoolingPolicyKind.AdaptiveGeometric
+    raise ValueError(f"Unsupported cooling_policy: {name!r}")
+
+
+class SASelector:
+    """Select a row window by simulated annealing (bmmpy/search.py:416-538 of
+    the reference: same keyword arguments, policies and results).  The
+    pairwise synergy table is built on the GPU; the annealing runs on the
+    host with the reference's random stream, so a seed gives the same window.
+    """
+
+    __slots__ = ("iterations", "restarts", "seed", "score_policy", "cooling_policy",
+                 "temperature_probe_samples", "initial_acceptance_probability", "cooling_rate",
+                 "min_temperature", "_impl")
+
+    def __init__(self, *, iterations: int = 10000, restarts: int = 8, seed: int = 0,
+                 score_policy: str = "pairwise_synergy",
+                 cooling_policy: str = "adaptive_geometric",
+                 temperature_probe_samples: int = 64,
+                 initial_acceptance_probability: float = 0.8, cooling_rate: float = 0.99,
+                 min_temperature: float = 1e-6) -> None:
+        config = _NativeSASelectorConfig()
+        config.iterations = iterations
+        config.restarts = restarts
+        config.seed = seed
+        config.score_policy = _resolve_score_policy(score_policy)
+        config.cooling_policy = _resolve_cooling_policy(cooling_policy)
+        config.temperature_probe_samples = temperature_probe_samples
+        config.initial_acceptance_probability = initial_acceptance_probability
+        config.cooling_rate = cooling_rate
+        config.min_temperature = min_temperature
+        self.iterations = iterations
+        self.restarts = restarts
+        self.seed = seed
+        self.score_policy = score_policy
+        self.cooling_policy = cooling_policy
+        self.temperature_probe_samples = temperature_probe_samples
+        self.initial_acceptance_probability = initial_acceptance_probability
+        self.cooling_rate = cooling_rate
+        self.min_temperature = min_temperature
+        self._impl = _NativeSASelector(config)
+
+    def __repr__(self) -> str:
+        return (f"SASelector(iterations={self.iterations}, restarts={self.restarts}, "
+                f"seed={self.seed}, score_policy={self.score_policy!r}, "
+                f"cooling_policy={self.cooling_policy!r}, "
+                f"temperature_probe_samples={self.temperature_probe_samples}, "
+                f"initial_acceptance_probability={self.initial_acceptance_probability}, "
+                f"cooling_rate={self.cooling_rate}, min_temperature={self.min_temperature})")
+
+    def name(self) -> str:
+        return self._impl.name()
+
+    def select(self, matrix, window_size: int) -> SASelectionResult:
+        return self._impl.select(matrix, window_size)
+
+    def select_window(self, matrix, window_size: int) -> RowWindow:
+        return self._impl.select_window(matrix, window_size)
+
+
 __all__ = ["BruteforceSearch", "CudaBruteforceSearch", "CudaMitmFwhtSearch", "FwhtSearch",
-           "MitmFwhtSearch"]
+           "MitmFwhtSearch", "SASelectionResult", "SASelector"]

@@ -171,6 +171,13 @@ int bmm_hip_window_coset_min(const uint64_t *row_words, size_t n_rows, size_t wo
                              const int32_t *targets, size_t n_targets, uint32_t *out_weight,
                              uint64_t *out_mask);
 
+/* ---- pairwise row overlaps (the synergy table of SASelector,
+ * src/search/sa_selector_pairwise.cpp:21-52): out[i * n_rows + j] =
+ * popcount(row_i & row_j) for i != j, 0 on the diagonal; rows of
+ * words_per_row words with zero bits beyond the columns. */
+int bmm_hip_pair_overlaps(const uint64_t *row_words, size_t n_rows, size_t words_per_row,
+                          uint32_t *out);
+
 /* ---- candidate stream of ExactBasisSolver (src/solver/exact_basis_solver.cpp:
  * 29-124): the first min(k, 2^t - 1) nonzero masks of a t-row window
  * (t <= 28) in the order (weight, Gray index), i.e. the reference's Gray

@@ -278,6 +278,10 @@ class Reference:
                                        C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)] + ep
         L.ref_exact_basis.argtypes = [_u64p, C.c_int, C.c_int, C.c_int, _u32p, _u32p,
                                       C.c_size_t, C.POINTER(C.c_uint64), _u64p] + ep
+        L.ref_sa_select.argtypes = [_u64p, C.c_int, C.c_int, C.c_int, C.c_size_t, C.c_size_t,
+                                    C.c_size_t, C.c_uint64, C.c_int, C.c_size_t, C.c_double,
+                                    C.c_double, C.c_double, _u64p, _u64p,
+                                    C.POINTER(C.c_longlong)] + ep
 
     @staticmethod
     def _err():
@@ -293,6 +297,23 @@ class Reference:
     @staticmethod
     def _rows(m):
         return np.ascontiguousarray(m, dtype=np.uint64)
+
+    def sa_select(self, m, cols, window_size, iterations=10000, restarts=8, seed=0,
+                  policy="pairwise_synergy", probe_samples=64, p0=0.8, cooling=0.99,
+                  min_t=1e-6):
+        """Reference SASelector::select: dict of the result fields."""
+        m = self._rows(m)
+        t, W = m.shape
+        rows = np.zeros(max(window_size, 1), np.uint64)
+        stats = np.zeros(5, np.uint64)
+        score = C.c_longlong()
+        n = self._run(self.lib.ref_sa_select, m, t, W, cols, window_size, iterations, restarts,
+                      seed, 1 if policy == "higher_order_synergy" else 0, probe_samples, p0,
+                      cooling, min_t, rows, stats, C.byref(score))
+        return {"rows": [int(x) for x in rows[:n]], "score": score.value,
+                "accepted_moves": int(stats[0]), "iterations_run": int(stats[1]),
+                "best_iteration": int(stats[2]), "restart_index": int(stats[3]),
+                "seed": int(stats[4])}
 
     def exact_basis(self, m, cols):
         """Reference ExactBasisSolver: (masks, weights, total, rank, basis rows)."""

@@ -16,6 +16,7 @@
 #include "bmmpy/search/bruteforce_search.hpp"
 #include "bmmpy/search/fwht_search.hpp"
 #include "bmmpy/search/mitm_fwht_search.hpp"
+#include "bmmpy/search/sa_selector.hpp"
 #include "bmmpy/search/split_window_prep.hpp"
 #include "bmmpy/solver/exact_basis_solver.hpp"
 
@@ -188,6 +189,42 @@ int ref_greedy_apply(std::uint64_t* rows, int t, int W, int cols, const std::uin
             std::memcpy(rows + static_cast<std::size_t>(rr) * W, m.row_words(rr),
                         sizeof(std::uint64_t) * W);
         return 0;
+    } catch (const std::exception& e) {
+        set_err(err, err_len, e.what());
+        return -1;
+    }
+}
+
+// SASelector::select on a t-row matrix.  stats = {accepted_moves,
+// iterations_run, best_iteration, restart_index, seed}; returns the number
+// of selected rows written to out_rows.
+int ref_sa_select(const std::uint64_t* rows, int t, int W, int cols, std::size_t window_size,
+                  std::size_t iterations, std::size_t restarts, std::uint64_t seed, int policy,
+                  std::size_t probe_samples, double p0, double cooling, double min_t,
+                  std::uint64_t* out_rows, std::uint64_t* stats, long long* score, char* err,
+                  int err_len) {
+    try {
+        auto m = load(rows, t, W, cols);
+        bmmpy::SASelectorConfig c;
+        c.iterations = iterations;
+        c.restarts = restarts;
+        c.seed = seed;
+        c.score_policy = policy ? bmmpy::WindowScorePolicyKind::HigherOrderSynergy
+                                : bmmpy::WindowScorePolicyKind::PairwiseSynergy;
+        c.temperature_probe_samples = probe_samples;
+        c.initial_acceptance_probability = p0;
+        c.cooling_rate = cooling;
+        c.min_temperature = min_t;
+        const auto r = bmmpy::SASelector(c).select(m, window_size);
+        for (std::size_t i = 0; i < r.rows.size(); ++i)
+            out_rows[i] = r.rows[i];
+        stats[0] = r.accepted_moves;
+        stats[1] = r.iterations_run;
+        stats[2] = r.best_iteration;
+        stats[3] = r.restart_index;
+        stats[4] = r.seed;
+        *score = r.score;
+        return static_cast<int>(r.rows.size());
     } catch (const std::exception& e) {
         set_err(err, err_len, e.what());
         return -1;

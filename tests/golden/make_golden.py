@@ -199,6 +199,24 @@ def main():
                    "greedy": {"rows": [[str(int(v)) for v in r] for r in ga_in],
                               "cols": 256, "candidates": [[w, str(mk)] for w, mk in cands],
                               "cases": ga_cases}}, f)
+    # --- SASelector (src/search/sa_selector*.cpp; the scalar reference build) ---
+    refs = oracle.reference(scalar=True)
+    sa_cases = []
+    doc, doc_cols = oracle.matrix_from_rows(["111111000000"] * 4 + ["000000000000"] * 2)
+    specs = [(doc, doc_cols, 4, 256, 4, 7), (doc, doc_cols, 0, 256, 4, 7),
+             (doc, doc_cols, 6, 100, 2, 3),
+             (ora.bernoulli(41, 40, 300, 0.2), 300, 8, 2000, 3, 11),
+             (ora.bernoulli(42, 100, 512, 0.1), 512, 16, 5000, 2, 0),
+             (ora.bernoulli(43, 70, 130, 0.35), 130, 20, 3000, 2, 5)]
+    for mat, cols, k, iters, restarts, seed in specs:
+        for policy in ("pairwise_synergy", "higher_order_synergy"):
+            res = refs.sa_select(mat, cols, k, iterations=iters, restarts=restarts, seed=seed,
+                                 policy=policy)
+            sa_cases.append({"rows": [[str(int(v)) for v in r] for r in mat], "cols": cols,
+                             "window_size": k, "iterations": iters, "restarts": restarts,
+                             "seed": seed, "policy": policy, "expect": res})
+    with open(os.path.join(HERE, "golden_sa.json"), "w") as f:
+        json.dump({"cases": sa_cases}, f)
     print("wrote fixtures", flush=True)
 
 

@@ -229,3 +229,51 @@ def test_exact_basis_caps_and_empty():
         bmm.solver.ExactBasisSolver(max_states=30).solve(big.row_window(list(range(5))))
     with pytest.raises(ValueError, match="storage"):
         bmm.solver.ExactBasisSolver(max_storage_bytes=100).solve(big.row_window(list(range(5))))
+
+
+# ---------------------------------------------------------- SASelector ---
+def _sa_cases(policy):
+    import json
+    import os
+    path = os.path.join(os.path.dirname(__file__), "golden", "golden_sa.json")
+    with open(path) as f:
+        return [c for c in json.load(f)["cases"] if c["policy"] == policy]
+
+
+def _bitmatrix(rows_u64, cols):
+    strs = ["".join("1" if (int(r[c // 64]) >> (c % 64)) & 1 else "0" for c in range(cols))
+            for r in rows_u64]
+    return bmm.matrix_from_rows(strs)
+
+
+def test_sa_selector_higher_order_matches_reference_golden():
+    """The higher-order synergy policy (column counts, host annealing) against
+    the reference SASelector's outputs for the same seeds (golden_sa.json)."""
+    for case in _sa_cases("higher_order_synergy"):
+        rows = [[int(v) for v in r] for r in case["rows"]]
+        mat = _bitmatrix(rows, case["cols"])
+        sel = bmm.SASelector(iterations=case["iterations"], restarts=case["restarts"],
+                             seed=case["seed"], score_policy="higher_order_synergy")
+        res = sel.select(mat, case["window_size"])
+        e = case["expect"]
+        assert list(res.rows) == e["rows"] and res.score == e["score"]
+        assert (res.accepted_moves, res.iterations_run, res.best_iteration, res.restart_index,
+                res.seed) == (e["accepted_moves"], e["iterations_run"], e["best_iteration"],
+                              e["restart_index"], e["seed"])
+
+
+def test_sa_selector_validation():
+    mat = bmm.matrix_from_rows(["1100", "0110", "0011"])
+    for kw, msg in ((dict(restarts=0), "restarts"),
+                    (dict(initial_acceptance_probability=1.0), "initial_acceptance_probability"),
+                    (dict(cooling_rate=0.0), "cooling_rate"),
+                    (dict(min_temperature=0.0), "min_temperature")):
+        with pytest.raises(ValueError, match=msg):
+            bmm.SASelector(**kw).select(mat, 2)
+    with pytest.raises(ValueError, match="window_size"):
+        bmm.SASelector().select(mat, 4)
+    with pytest.raises(ValueError, match="score_policy"):
+        bmm.SASelector(score_policy="nope")
+    r = bmm.SASelector(seed=0).select(mat, 0)
+    assert list(r.rows) == [] and r.seed == 0x9E3779B97F4A7C15
+    assert bmm.SASelector().name() == "sa"
