@@ -23,7 +23,7 @@ from ._bmmpy import (  # noqa: F401
     get_runtime_features,
     get_version,
 )
-from .apply import GreedyApplier
+from .apply import GlobalGreedyApplier, GreedyApplier
 from .matrix import matrix_from_rows
 from .search import (
     BruteforceSearch,
@@ -39,7 +39,7 @@ from . import solver  # noqa: E402  (bmm.solver.ExactBasisSolver, as in the refe
 
 __all__ = [
     "ApplyResult", "BitMatrix", "BruteforceSearch", "Candidate", "CudaBruteforceSearch",
-    "CudaMitmFwhtSearch", "FwhtSearch", "GreedyApplier", "MatrixErr", "MatrixError",
+    "CudaMitmFwhtSearch", "FwhtSearch", "GlobalGreedyApplier", "GreedyApplier", "MatrixErr", "MatrixError",
     "MitmFwhtSearch", "RowWindow", "SASelectionResult", "SASelector", "RuntimeFeatures", "add", "fwht_i16", "fwht_i32",
     "get_runtime_features", "get_version", "matrix_from_rows", "search_apply", "solver",
 ]

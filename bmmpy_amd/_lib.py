@@ -117,6 +117,10 @@ SIGNATURES = {
     "bmm_hip_mlp_score_pairs": ([_vp, _u64p, _sz, _u16p, _sz, _f32p], C.c_int),
     "bmm_hip_pair_overlaps": ([_u64p, _sz, _sz, _u32p], C.c_int),
     "bmm_hip_weight_order": ([_u64p, _sz, _sz, _sz, _sz, _u32p, _u32p, C.POINTER(_sz)], C.c_int),
+    "bmm_hip_row_combine": ([_u64p, _sz, _sz, _u64p, _sz, _sz, _u64p, C.POINTER(C.c_uint64)],
+                            C.c_int),
+    "bmm_hip_scores_order_i32": ([_i32p, _sz, C.c_int32, _i32p, _i32p, _i32p, _i32p], C.c_int),
+    "bmm_hip_scores_order_i16": ([_i16p, _sz, C.c_int32, _i16p, _i32p, _i32p, _i32p], C.c_int),
     "bmm_hip_mlp_gradients": ([_vp, _f32p, _f32p, _sz, _f32p, C.POINTER(C.c_float)], C.c_int),
     "bmm_hip_mlp_train_step": ([_vp, _f32p, _f32p, _sz, C.c_float, C.POINTER(C.c_float)],
                                C.c_int),
@@ -288,6 +292,36 @@ def weight_order(rows, cols: int, k: int):
     n = _sz()
     call("bmm_hip_weight_order", rows, rows.shape[0], rows.shape[1], cols, k, m, w, C.byref(n))
     return m[:n.value], w[:n.value]
+
+
+def row_combine(rows, masks):
+    """GF(2) product: out row i = XOR of the rows selected by masks[i]
+    (uint64 words).  Returns (out rows, total weight)."""
+    rows = _rows(rows)
+    masks = np.ascontiguousarray(masks, dtype=np.uint64)
+    if masks.ndim == 1:
+        masks = masks.reshape(-1, 1)
+    out = np.zeros((masks.shape[0], rows.shape[1]), np.uint64)
+    wt = C.c_uint64()
+    call("bmm_hip_row_combine", rows.reshape(-1), rows.shape[0], rows.shape[1],
+         masks.reshape(-1), masks.shape[1], masks.shape[0], out.reshape(-1), C.byref(wt))
+    return out, wt.value
+
+
+def calc_scores_and_order(h, n: int):
+    """calc_scores_and_order on the dtype of h (int32 or int16):
+    (s_by_mask, order, cnt, off)."""
+    h = np.ascontiguousarray(h)
+    if h.dtype not in (np.int32, np.int16):
+        raise TypeError("h must be int32 or int16")
+    m = h.size
+    s = np.zeros(max(m, 1), h.dtype)
+    order = np.zeros(max(m - 1, 1), np.int32)
+    cnt = np.zeros(max(n + 1, 1), np.int32)
+    off = np.zeros(max(n + 1, 1), np.int32)
+    name = "bmm_hip_scores_order_i32" if h.dtype == np.int32 else "bmm_hip_scores_order_i16"
+    call(name, h if m else np.zeros(1, h.dtype), m, n, s, order, cnt, off)
+    return s[:m], order[:max(m - 1, 0)], cnt[:n + 1], off[:n + 1]
 
 
 def gray_coset_min(rows, block_rows: int, targets):

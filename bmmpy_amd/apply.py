@@ -1,9 +1,10 @@
-"""GreedyApplier wrapper (bmmpy/apply.py:46-131)."""
+"""GreedyApplier and GlobalGreedyApplier wrappers (bmmpy/apply.py:46-202)."""
 from __future__ import annotations
 
 from typing import Iterable
 
 from ._bmmpy import ApplyResult, Candidate, RowWindow
+from ._bmmpy import GlobalGreedyApplier as _NativeGlobalGreedyApplier
 from ._bmmpy import GreedyApplier as _NativeGreedyApplier
 
 
@@ -27,4 +28,26 @@ class GreedyApplier:
         return self._impl.apply(window, list(candidates))
 
 
-__all__ = ["GreedyApplier"]
+class GlobalGreedyApplier:
+    """Assemble one full invertible transform from the candidates plus the
+    window's unit rows (stably sorted by weight, greedy pivot basis over the
+    masks) and apply it in one GPU row combination when the total weight does
+    not grow (strictly drops when ``require_improvement``)."""
+
+    __slots__ = ("require_improvement", "_impl")
+
+    def __init__(self, *, require_improvement: bool = True) -> None:
+        self.require_improvement = require_improvement
+        self._impl = _NativeGlobalGreedyApplier(require_improvement)
+
+    def __repr__(self) -> str:
+        return f"GlobalGreedyApplier(require_improvement={self.require_improvement})"
+
+    def name(self) -> str:
+        return self._impl.name()
+
+    def apply(self, window: RowWindow, candidates: Iterable[Candidate]) -> ApplyResult:
+        return self._impl.apply(window, list(candidates))
+
+
+__all__ = ["GreedyApplier", "GlobalGreedyApplier"]

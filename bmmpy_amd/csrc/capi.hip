@@ -88,6 +88,11 @@ void mlp_forward_host(Mlp* M, const float* x, size_t B, float* y);
 size_t weight_order(const uint64_t* rows, size_t t, size_t words, size_t cols, size_t k,
                     uint32_t* out_mask, uint32_t* out_weight);
 void pair_overlaps(const uint64_t* rows, size_t n, size_t words, uint32_t* out);
+void row_combine(const uint64_t* rows, size_t n_rows, size_t words, const uint64_t* masks,
+                 size_t mask_words, size_t n_out, uint64_t* out, uint64_t* out_weight);
+template <typename S>
+void scores_order(const S* h, size_t m, int32_t n, S* s_by_mask, int32_t* order, int32_t* cnt,
+                  int32_t* off);
 void mlp_score_pairs_host(Mlp* M, const uint64_t* sketch, size_t rows, const uint16_t* pairs,
                           size_t n, float* y);
 float mlp_grads_host(Mlp* M, const float* x, const float* t, size_t B, float* grads);
@@ -129,6 +134,28 @@ template <typename T> void fwht_host(T* data, size_t n) {
     BMM_HIP(hipMemcpy(d, data, n * sizeof(T), hipMemcpyHostToDevice));
     fwht_array_dev<T>(d, n, nullptr);
     BMM_HIP(hipMemcpy(data, d, n * sizeof(T), hipMemcpyDeviceToHost));
+}
+
+template <typename S>
+int scores_order_entry(const S* h, size_t m, int32_t n, S* s_by_mask, int32_t* order,
+                       int32_t* cnt, int32_t* off) {
+    return guarded([&] {
+        if (n < 0)
+            fail(BMM_ERR_INVALID, "calc_scores_and_order: n must be non-negative");
+        if (!cnt || !off || (m > 0 && (!h || !s_by_mask)) || (m > 1 && !order))
+            fail(BMM_ERR_INVALID, "calc_scores_and_order: null pointer");
+        if (n > (1 << 28))
+            fail(BMM_ERR_INVALID, "calc_scores_and_order: n must be at most 2^28");
+        if (m > ((size_t)1 << 31))
+            fail(BMM_ERR_INVALID, "calc_scores_and_order: m must be at most 2^31");
+        if (m == 0) {   // empty spectrum: zero counts and offsets (fwht.cpp:71-76)
+            std::fill(cnt, cnt + (size_t)n + 1, 0);
+            std::fill(off, off + (size_t)n + 1, 0);
+            return;
+        }
+        require_device();
+        scores_order<S>(h, m, n, s_by_mask, order, cnt, off);
+    });
 }
 } // namespace
 
@@ -616,6 +643,45 @@ int bmm_hip_pair_overlaps(const uint64_t* row_words, size_t n, size_t words, uin
         require_device();
         pair_overlaps(row_words, n, words, out);
     });
+}
+
+int bmm_hip_row_combine(const uint64_t* row_words, size_t n_rows, size_t words,
+                        const uint64_t* masks, size_t mask_words, size_t n_out,
+                        uint64_t* out_words, uint64_t* out_weight) {
+    return guarded([&] {
+        if (!out_weight || (n_rows * words > 0 && !row_words) ||
+            (n_out * mask_words > 0 && !masks) || (n_out * words > 0 && !out_words))
+            fail(BMM_ERR_INVALID, "row_combine: null pointer");
+        *out_weight = 0;
+        if (words >= ((size_t)1 << 31) || n_out >= ((size_t)1 << 31) ||
+            n_rows >= ((size_t)1 << 31) || mask_words >= ((size_t)1 << 25))
+            fail(BMM_ERR_INVALID, "row_combine: dimensions too large");
+        if (n_out * words >= ((size_t)1 << 34))
+            fail(BMM_ERR_INVALID, "row_combine: output larger than 2^34 words");
+        for (size_t i = 0; i < n_out * mask_words; ++i) {
+            const size_t w = i % mask_words;
+            const uint64_t allowed = w * 64 >= n_rows
+                                         ? 0ull
+                                         : (n_rows - w * 64 >= 64 ? ~0ull
+                                                                  : (1ull << (n_rows - w * 64)) - 1);
+            if (masks[i] & ~allowed)
+                fail(BMM_ERR_RANGE, "row_combine: mask selects a row beyond n_rows");
+        }
+        if (n_out == 0 || words == 0)
+            return;
+        require_device();
+        row_combine(row_words, n_rows, words, masks, mask_words, n_out, out_words, out_weight);
+    });
+}
+
+int bmm_hip_scores_order_i32(const int32_t* h, size_t m, int32_t n, int32_t* s_by_mask,
+                             int32_t* order, int32_t* cnt, int32_t* off) {
+    return scores_order_entry<int32_t>(h, m, n, s_by_mask, order, cnt, off);
+}
+
+int bmm_hip_scores_order_i16(const int16_t* h, size_t m, int32_t n, int16_t* s_by_mask,
+                             int32_t* order, int32_t* cnt, int32_t* off) {
+    return scores_order_entry<int16_t>(h, m, n, s_by_mask, order, cnt, off);
 }
 
 int bmm_hip_weight_order(const uint64_t* row_words, size_t t, size_t words, size_t cols,

@@ -1,14 +1,19 @@
-// GreedyApplier (src/apply/greedy_applier.cpp:13-109) and row-XOR replay.
+// GreedyApplier (src/apply/greedy_applier.cpp:13-109), row-XOR replay and
+// GlobalGreedyApplier (src/apply/global_greedy_applier.cpp:15-105).
 //
-// The applier's choices depend only on candidate weights and the initial row
-// popcounts, so the host decides the whole row-XOR sequence first and the GPU
-// replays it over the window's rows in one launch.
+// GreedyApplier's choices depend only on candidate weights and the initial
+// row popcounts, so the host decides the whole row-XOR sequence first and the
+// GPU replays it over the window's rows in one launch.  GlobalGreedyApplier's
+// choice depends only on the masks and weights as well; its transform is
+// applied on the GPU as one row combination.
 #include "bmmpy.hpp"
 
 #include "bmmpy_hip.h"
+#include "pivot.hpp"
 
 #include <algorithm>
 #include <array>
+#include <limits>
 
 namespace bmmpy {
 
@@ -100,6 +105,88 @@ ApplyResult GreedyApplier::apply(RowWindow& window, const std::vector<Candidate>
     }
     if (!ops.empty())
         replay_row_xors(window, ops);
+    return result;
+}
+
+namespace {
+
+struct PoolEntry {
+    std::vector<std::uint64_t> mask;
+    std::uint32_t weight;
+};
+
+// Candidate mask words widened or trimmed to `words`; a set bit at or beyond
+// row `bits` is rejected (include/bmmpy/core/detail/xor_basis.hpp:73-110).
+std::vector<std::uint64_t> normalized_mask(const std::vector<std::uint64_t>& mask,
+                                           std::size_t bits, std::size_t words) {
+    for (std::size_t w = 0; w < mask.size(); ++w) {
+        const std::uint64_t allowed =
+            w * 64 >= bits ? 0ull : (bits - w * 64 >= 64 ? ~0ull : (1ull << (bits - w * 64)) - 1);
+        if (mask[w] & ~allowed)
+            throw std::invalid_argument("mask contains bits outside the supported row range");
+    }
+    std::vector<std::uint64_t> out(words, 0);
+    std::copy_n(mask.begin(), std::min(words, mask.size()), out.begin());
+    return out;
+}
+
+} // namespace
+
+ApplyResult GlobalGreedyApplier::apply(RowWindow& window,
+                                       const std::vector<Candidate>& candidates) const {
+    ApplyResult result;
+    const std::size_t t = window.size();
+    if (t == 0)
+        return result;
+    const std::size_t mw = (t + 63) / 64;
+
+    // Pool: candidates in input order, then the unit rows; stable by weight
+    // (global_greedy_applier.cpp:31-54).
+    std::vector<PoolEntry> pool;
+    pool.reserve(candidates.size() + t);
+    for (const Candidate& c : candidates)
+        pool.push_back(PoolEntry{normalized_mask(c.mask, t, mw), c.weight});
+    for (std::size_t r = 0; r < t; ++r) {
+        const std::uint64_t pc = window.row_popcount(r);
+        if (pc > std::numeric_limits<std::uint32_t>::max())
+            throw std::invalid_argument(
+                "GlobalGreedyApplier: row weight exceeds supported u32 candidate range");
+        std::vector<std::uint64_t> unit(mw, 0);
+        unit[r / 64] = 1ull << (r % 64);
+        pool.push_back(PoolEntry{std::move(unit), (std::uint32_t)pc});
+    }
+    std::stable_sort(pool.begin(), pool.end(),
+                     [](const PoolEntry& a, const PoolEntry& b) { return a.weight < b.weight; });
+
+    // Greedy basis in coefficient space (global_greedy_applier.cpp:72-86).
+    PivotRows basis(mw);
+    std::vector<std::uint64_t> chosen;   // t x mw mask words, in pick order
+    chosen.reserve(t * mw);
+    std::size_t n_chosen = 0;
+    for (const PoolEntry& e : pool) {
+        if (!basis.try_insert(e.mask))
+            continue;
+        chosen.insert(chosen.end(), e.mask.begin(), e.mask.end());
+        if (++n_chosen == t)
+            break;
+    }
+    if (n_chosen != t)
+        throw std::runtime_error(
+            "GlobalGreedyApplier: failed to assemble a full invertible transform");
+
+    // after = T * before on the GPU (global_greedy_applier.cpp:88-96).
+    const std::vector<std::uint64_t> before = window.packed_rows();
+    const std::size_t words = window.words_per_row();
+    const std::uint64_t old_weight = window.weight();
+    std::vector<std::uint64_t> after(t * words);
+    std::uint64_t new_weight = 0;
+    throw_status(bmm_hip_row_combine(before.data(), t, words, chosen.data(), mw, t, after.data(),
+                                     &new_weight));
+    if (new_weight > old_weight || (_require_improvement && new_weight == old_weight))
+        return result;
+    window.unpack_rows(after);
+    result.applied_count = 1;
+    result.weight_improvement = old_weight - new_weight;
     return result;
 }
 

@@ -195,14 +195,26 @@ std::size_t BitMatrix::rank() const {
     return rank;
 }
 
+// GF(2) product on the GPU (src/core/bit_matrix_algebra.cpp:27-50): row i of
+// the result is the XOR of the rows of `other` selected by row i of this
+// matrix, which bmm_hip_row_combine takes as its mask words.
 BitMatrix BitMatrix::mul(const BitMatrix& other) const {
     if (_cols != other._rows)
         throw MatrixError(MatrixErr::DimensionMismatch);
     BitMatrix out(_rows, other._cols);
+    const std::size_t mw = words_per_row(), ow = other.words_per_row();
+    if (_rows == 0 || ow == 0)
+        return out;
+    std::vector<std::uint64_t> masks(_rows * mw), rows(other._rows * ow), prod(_rows * ow);
     for (std::size_t i = 0; i < _rows; ++i)
-        for (std::size_t k = 0; k < _cols; ++k)
-            if (get(i, k))
-                out.row_xor_from(i, other, k);
+        std::memcpy(masks.data() + i * mw, row_words(i), mw * 8);
+    for (std::size_t k = 0; k < other._rows; ++k)
+        std::memcpy(rows.data() + k * ow, other.row_words(k), ow * 8);
+    std::uint64_t wt = 0;
+    throw_status(bmm_hip_row_combine(rows.data(), other._rows, ow, masks.data(), mw, _rows,
+                                     prod.data(), &wt));
+    for (std::size_t i = 0; i < _rows; ++i)
+        std::memcpy(out.row_words(i), prod.data() + i * ow, ow * 8);
     return out;
 }
 

@@ -274,6 +274,13 @@ CompactSplitWindow build_compact_split_window(const RowWindow& window, std::size
 // ------------------------------------------------------------ FWHT -------
 void fwht_inplace(std::int32_t* data, std::size_t size);
 void fwht_inplace(std::int16_t* data, std::size_t size);
+// include/bmmpy/math/fwht.hpp:11-25 (bmm_hip_scores_order_i16 / _i32).
+void calc_scores_and_order(const std::int16_t* h, std::size_t m, std::int32_t n,
+                           std::int16_t* s_by_mask, std::int32_t* order, std::int32_t* cnt,
+                           std::int32_t* off);
+void calc_scores_and_order(const std::int32_t* h, std::size_t m, std::int32_t n,
+                           std::int32_t* s_by_mask, std::int32_t* order, std::int32_t* cnt,
+                           std::int32_t* off);
 
 // ---------------------------------------------------------- appliers -----
 struct ApplyResult {
@@ -296,6 +303,23 @@ private:
     std::uint64_t _min_gain;
     bool _stochastic;
     std::uint64_t _prng_state;
+};
+
+// src/apply/global_greedy_applier.cpp:59-105: the pool (candidates plus unit
+// rows weighted by their popcounts, stably sorted by weight) and the pivot
+// greedy run on the host over the mask words; the chosen square transform is
+// applied to the window's rows on the GPU (bmm_hip_row_combine), which also
+// returns the new weight.
+class GlobalGreedyApplier {
+public:
+    explicit GlobalGreedyApplier(bool require_improvement = true) noexcept
+        : _require_improvement(require_improvement) {}
+    const char* name() const noexcept { return "global_greedy"; }
+    bool require_improvement() const noexcept { return _require_improvement; }
+    ApplyResult apply(RowWindow& window, const std::vector<Candidate>& candidates) const;
+
+private:
+    bool _require_improvement;
 };
 
 // Replays row-XOR ops (ops[2i] = source, ops[2i+1] = target, local rows) on the

@@ -293,6 +293,10 @@ PYBIND11_MODULE(_bmmpy, m) {
         .def(py::init<std::uint64_t, bool, std::uint64_t>(), py::arg("min_gain"),
              py::arg("stochastic") = false, py::arg("seed") = 0)
         .def("apply", &GreedyApplier::apply, py::arg("window"), py::arg("candidates"));
+    py::class_<GlobalGreedyApplier>(m, "GlobalGreedyApplier")
+        .def(py::init<bool>(), py::arg("require_improvement") = true)
+        .def("name", &GlobalGreedyApplier::name)
+        .def("apply", &GlobalGreedyApplier::apply, py::arg("window"), py::arg("candidates"));
     m.def("replay_row_xors", &replay_row_xors, py::arg("window"), py::arg("ops"));
 
     py::enum_<WindowScorePolicyKind>(m, "WindowScorePolicyKind")

@@ -32,4 +32,16 @@ void fwht_inplace(std::int16_t* data, std::size_t size) {
     throw_status(bmm_hip_fwht_i16(data, size));
 }
 
+void calc_scores_and_order(const std::int16_t* h, std::size_t m, std::int32_t n,
+                           std::int16_t* s_by_mask, std::int32_t* order, std::int32_t* cnt,
+                           std::int32_t* off) {
+    throw_status(bmm_hip_scores_order_i16(h, m, n, s_by_mask, order, cnt, off));
+}
+
+void calc_scores_and_order(const std::int32_t* h, std::size_t m, std::int32_t n,
+                           std::int32_t* s_by_mask, std::int32_t* order, std::int32_t* cnt,
+                           std::int32_t* off) {
+    throw_status(bmm_hip_scores_order_i32(h, m, n, s_by_mask, order, cnt, off));
+}
+
 } // namespace bmmpy

@@ -178,6 +178,31 @@ int bmm_hip_window_coset_min(const uint64_t *row_words, size_t n_rows, size_t wo
 int bmm_hip_pair_overlaps(const uint64_t *row_words, size_t n_rows, size_t words_per_row,
                           uint32_t *out);
 
+/* ---- GF(2) row combination: out row i = XOR of the input rows selected by
+ * mask i (mask_words words per mask, bit j of word q = input row 64 q + j).
+ * Replaces BitMatrix::mul (src/core/bit_matrix_algebra.cpp:27-50), as used
+ * by GlobalGreedyApplier::apply to apply its transform
+ * (src/apply/global_greedy_applier.cpp:93-96).  *out_weight = total popcount
+ * of the n_out output rows.  A mask bit at or beyond n_rows is
+ * BMM_ERR_RANGE. */
+int bmm_hip_row_combine(const uint64_t *row_words, size_t n_rows, size_t words_per_row,
+                        const uint64_t *masks, size_t mask_words, size_t n_out,
+                        uint64_t *out_words, uint64_t *out_weight);
+
+/* ---- calc_scores_and_order (include/bmmpy/math/fwht.hpp:11-25,
+ * src/math/fwht.cpp:59-126): s_by_mask[u] = clamp((n - h[u]) >> 1, 0, n)
+ * for u >= 1, s_by_mask[0] = 0; cnt[v] = number of masks u in 1 .. m-1 with
+ * score v; order[0 .. m-2] = those masks sorted by score, ascending u within
+ * a score; off[v] = end of bucket v in order (inclusive prefix of cnt).
+ * cnt and off hold n + 1 entries.  n < 0 is BMM_ERR_INVALID; m == 0 zeroes
+ * cnt and off only.  Limits: n <= 2^28, m <= 2^31; the int16 form fails with
+ * BMM_ERR_OVERFLOW when a score exceeds 32767 (the reference indexes out of
+ * its offsets there). */
+int bmm_hip_scores_order_i32(const int32_t *h, size_t m, int32_t n, int32_t *s_by_mask,
+                             int32_t *order, int32_t *cnt, int32_t *off);
+int bmm_hip_scores_order_i16(const int16_t *h, size_t m, int32_t n, int16_t *s_by_mask,
+                             int32_t *order, int32_t *cnt, int32_t *off);
+
 /* ---- candidate stream of ExactBasisSolver (src/solver/exact_basis_solver.cpp:
  * 29-124): the first min(k, 2^t - 1) nonzero masks of a t-row window
  * (t <= 28) in the order (weight, Gray index), i.e. the reference's Gray

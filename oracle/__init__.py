@@ -67,6 +67,7 @@ class Oracle:
         L.or_fill_bernoulli.argtypes = [C.POINTER(C.c_uint64), _u64p, C.c_int, C.c_int, C.c_int,
                                         C.c_double]
         L.or_rank.argtypes = [_u64p, C.c_int, C.c_int, C.c_int]
+        L.or_mul.argtypes = [_u64p, C.c_int, C.c_int, C.c_int, _u64p, C.c_int, C.c_int, _u64p]
         L.or_gen_planted.argtypes = [C.c_uint64, C.c_int, C.c_int, C.c_double, _u64p, _u64p,
                                      _u64p]
         L.or_gen_batch.argtypes = [C.c_uint64, C.c_int, _i32p, _i32p, _f64p, _u64p, C.c_size_t,
@@ -100,6 +101,13 @@ class Oracle:
         L.or_greedy_apply.argtypes = [_u64p, C.c_int, C.c_int, C.c_int, _u64p, _u32p, C.c_size_t,
                                       C.c_uint64, C.c_int, C.c_uint64, C.POINTER(C.c_uint64),
                                       C.POINTER(C.c_uint64)]
+        L.or_scores_order_i32.argtypes = [_i32p, C.c_size_t, C.c_int32, _i32p, _i32p, _i32p,
+                                          _i32p]
+        L.or_scores_order_i16.argtypes = [_i16p, C.c_size_t, C.c_int32, _i16p, _i32p, _i32p,
+                                          _i32p]
+        L.or_global_greedy.argtypes = [_u64p, C.c_int, C.c_int, C.c_int, _u64p, C.c_int, _u32p,
+                                       C.c_size_t, C.c_int, C.POINTER(C.c_uint64),
+                                       C.POINTER(C.c_uint64)]
 
     # -- RNG / generators -------------------------------------------------
     def xs_stream(self, seed: int, n: int) -> np.ndarray:
@@ -149,6 +157,14 @@ class Oracle:
         a = np.array(x, dtype=np.int16)
         _check(self.lib.or_fwht_i16(a, a.size), "fwht_i16")
         return a
+
+    def mul(self, a, a_cols: int, b, b_cols: int) -> np.ndarray:
+        """GF(2) product a * b (rows of words; a has a_cols columns)."""
+        a = np.ascontiguousarray(a, dtype=np.uint64)
+        b = np.ascontiguousarray(b, dtype=np.uint64)
+        out = np.zeros((a.shape[0], b.shape[1]), np.uint64)
+        self.lib.or_mul(a, a.shape[0], a.shape[1], a_cols, b, b.shape[1], b_cols, out)
+        return out
 
     def histogram(self, m: np.ndarray, cols: int, idx=None):
         m = np.ascontiguousarray(m, dtype=np.uint64)
@@ -247,6 +263,22 @@ class Oracle:
                                         C.byref(imp)), "greedy_apply")
         return m, ap.value, imp.value
 
+    def scores_order(self, h, n):
+        """calc_scores_and_order on int32 or int16 h: (s, order, cnt, off)."""
+        return _scores_order_call(self.lib.or_scores_order_i32, self.lib.or_scores_order_i16,
+                                  h, n, lambda f, *a: _check(f(*a), "scores_order"))
+
+    def global_greedy(self, m, cols, masks, weights, require_improvement=True):
+        """GlobalGreedyApplier on all rows of m; masks: (n_cand, mask_words)
+        uint64.  Returns (rows, applied, improvement)."""
+        m = np.array(m, dtype=np.uint64, copy=True, order="C")
+        masks, ws = _cand_arrays(masks, weights)
+        ap, imp = C.c_uint64(), C.c_uint64()
+        _check(self.lib.or_global_greedy(m, m.shape[0], m.shape[1], cols, masks.reshape(-1),
+                                         masks.shape[1], ws, ws.size, int(require_improvement),
+                                         C.byref(ap), C.byref(imp)), "global_greedy")
+        return m, ap.value, imp.value
+
 
 class RefError(RuntimeError):
     pass
@@ -282,6 +314,13 @@ class Reference:
                                     C.c_size_t, C.c_uint64, C.c_int, C.c_size_t, C.c_double,
                                     C.c_double, C.c_double, _u64p, _u64p,
                                     C.POINTER(C.c_longlong)] + ep
+        L.ref_scores_order_i32.argtypes = [_i32p, C.c_size_t, C.c_int32, _i32p, _i32p, _i32p,
+                                           _i32p] + ep
+        L.ref_scores_order_i16.argtypes = [_i16p, C.c_size_t, C.c_int32, _i16p, _i32p, _i32p,
+                                           _i32p] + ep
+        L.ref_global_greedy.argtypes = [_u64p, C.c_int, C.c_int, C.c_int, _u64p, C.c_int, _u32p,
+                                        C.c_size_t, C.c_int, C.POINTER(C.c_uint64),
+                                        C.POINTER(C.c_uint64)] + ep
 
     @staticmethod
     def _err():
@@ -399,6 +438,41 @@ class Reference:
         self._run(self.lib.ref_greedy_apply, m, m.shape[0], m.shape[1], cols, masks, ws,
                   len(cands), min_gain, int(stochastic), seed, C.byref(ap), C.byref(imp))
         return m, ap.value, imp.value
+
+    def scores_order(self, h, n):
+        return _scores_order_call(self.lib.ref_scores_order_i32, self.lib.ref_scores_order_i16,
+                                  h, n, self._run)
+
+    def global_greedy(self, m, cols, masks, weights, require_improvement=True):
+        m = np.array(m, dtype=np.uint64, copy=True, order="C")
+        masks, ws = _cand_arrays(masks, weights)
+        ap, imp = C.c_uint64(), C.c_uint64()
+        self._run(self.lib.ref_global_greedy, m, m.shape[0], m.shape[1], cols, masks.reshape(-1),
+                  masks.shape[1], ws, ws.size, int(require_improvement), C.byref(ap),
+                  C.byref(imp))
+        return m, ap.value, imp.value
+
+
+def _scores_order_call(f32, f16, h, n, run):
+    h = np.ascontiguousarray(h)
+    f = {np.dtype(np.int32): f32, np.dtype(np.int16): f16}[h.dtype]
+    m = h.size
+    s = np.zeros(max(m, 1), h.dtype)
+    order = np.zeros(max(m - 1, 1), np.int32)
+    cnt = np.zeros(max(n + 1, 1), np.int32)
+    off = np.zeros(max(n + 1, 1), np.int32)
+    run(f, h if m else np.zeros(1, h.dtype), m, n, s, order, cnt, off)
+    return s[:m], order[:max(m - 1, 0)], cnt[:n + 1], off[:n + 1]
+
+
+def _cand_arrays(masks, weights):
+    masks = np.ascontiguousarray(masks, dtype=np.uint64)
+    if masks.ndim == 1:
+        masks = masks.reshape(-1, 1)
+    if masks.size == 0:
+        masks = np.zeros((0, 1), np.uint64)
+    ws = np.ascontiguousarray(weights, dtype=np.uint32)
+    return np.ascontiguousarray(masks), ws
 
 
 def oracle() -> Oracle:

@@ -26,6 +26,12 @@
  *   or_replay          reference RowWindow::row_xor replay golden; weight
  *                      identity wt(a^b) = wt(a)+wt(b)-2 pop(a&b).
  *   or_greedy_apply    reference GreedyApplier golden.
+ *   or_scores_order_*  reference KATs fwht_tests.cpp:72-107 (i32 and i16); the
+ *                      live reference calc_scores_and_order via oracle/_ref on
+ *                      random spectra; order checked as a stable sort.
+ *   or_global_greedy   reference KAT test_matrix_api.py:66-105 (applies, keeps
+ *                      the rank; identity untouched) and the live reference
+ *                      GlobalGreedyApplier on random windows and candidates.
  *   or_exact_basis,    reference ExactBasisSolver KAT (tests/python/
  *   or_weight_order    test_matrix_api.py: 3x3, rank 2, total 3) and the live
  *                      reference on random and rank-deficient windows;
@@ -44,6 +50,7 @@
 #define OR_EINVAL (-1)
 #define OR_EOVERFLOW (-2)
 #define OR_ENOMEM (-3)
+#define OR_ERANK (-4)
 
 /* ---------------------------------------------------------------- RNG --- */
 /* include/bmmpy/core/detail/xorshift64.hpp:10-40 */
@@ -938,4 +945,176 @@ int or_exact_basis(const uint64_t *base, int stride, int cols, const int *idx, i
         return rc;
     *out_total = total;
     return n_piv;
+}
+
+/* ------------------------------------------- scores and weight order --- */
+/* calc_scores_and_order (src/math/fwht.cpp:59-100): score of mask u >= 1 =
+ * clamp((n - h[u]) >> 1, 0, n) (floor shift), s[0] = 0; cnt = histogram of
+ * the scores of masks 1 .. m-1; order = counting sort of those masks by
+ * score, stable; off[v] ends as the end of bucket v.  The difference is
+ * taken in 64 bits (the reference's 32-bit one agrees wherever it does not
+ * overflow).  The int16 form returns OR_EOVERFLOW for a score above 32767,
+ * where the reference would index outside its offsets. */
+static int scores_order_core(const int32_t *h32, const int16_t *h16, size_t m, int32_t n,
+                             int32_t *s32, int16_t *s16, int32_t *order, int32_t *cnt,
+                             int32_t *off) {
+    if (n < 0)
+        return OR_EINVAL;
+    for (int64_t v = 0; v <= n; ++v)
+        cnt[v] = off[v] = 0;
+    if (m == 0)
+        return 0;
+    if (s32)
+        s32[0] = 0;
+    else
+        s16[0] = 0;
+    for (size_t u = 1; u < m; ++u) {
+        const int64_t hu = h32 ? h32[u] : h16[u];
+        int64_t su = ((int64_t)n - hu) >> 1;
+        su = su < 0 ? 0 : (su > n ? n : su);
+        if (s32) {
+            s32[u] = (int32_t)su;
+        } else {
+            if (su > 32767)
+                return OR_EOVERFLOW;
+            s16[u] = (int16_t)su;
+        }
+        cnt[su] += 1;
+    }
+    for (int64_t v = 0; v < n; ++v)
+        off[v + 1] = off[v] + cnt[v];
+    for (size_t u = 1; u < m; ++u) {
+        const int32_t su = s32 ? s32[u] : s16[u];
+        order[off[su]++] = (int32_t)u;
+    }
+    return 0;
+}
+
+int or_scores_order_i32(const int32_t *h, size_t m, int32_t n, int32_t *s_by_mask,
+                        int32_t *order, int32_t *cnt, int32_t *off) {
+    return scores_order_core(h, NULL, m, n, s_by_mask, NULL, order, cnt, off);
+}
+
+int or_scores_order_i16(const int16_t *h, size_t m, int32_t n, int16_t *s_by_mask,
+                        int32_t *order, int32_t *cnt, int32_t *off) {
+    return scores_order_core(NULL, h, m, n, NULL, s_by_mask, order, cnt, off);
+}
+
+/* ------------------------------------------------ global greedy apply --- */
+typedef struct {
+    const uint64_t *mask;
+    uint32_t weight;
+    size_t order;
+} or_pool;
+
+static int pool_weight_stable(const void *pa, const void *pb) {
+    const or_pool *a = pa, *b = pb;
+    if (a->weight != b->weight)
+        return a->weight < b->weight ? -1 : 1;
+    return a->order < b->order ? -1 : (a->order > b->order);
+}
+
+static int highest_bit_words(const uint64_t *v, int words) {
+    for (int w = words - 1; w >= 0; --w)
+        if (v[w])
+            return w * 64 + 63 - __builtin_clzll(v[w]);
+    return -1;
+}
+
+/* GlobalGreedyApplier::apply (src/apply/global_greedy_applier.cpp:15-105):
+ * the pool is the candidates (mask words over t bits) followed by the unit
+ * rows weighted by their popcounts, stably sorted by weight; masks enter a
+ * pivot basis (highest set bit, include/bmmpy/core/detail/xor_basis.hpp:
+ * 123-160) in that order until t are chosen; the transform whose row i is
+ * chosen mask i replaces the window when the total weight does not grow
+ * (strictly drops when require_improvement).  Returns 0, OR_EINVAL for a
+ * mask bit at or beyond t, OR_ERANK when no full transform exists. */
+int or_global_greedy(uint64_t *rows, int t, int stride, int cols, const uint64_t *masks,
+                     int mask_words, const uint32_t *weights, size_t n_cand,
+                     int require_improvement, uint64_t *out_applied,
+                     uint64_t *out_improvement) {
+    *out_applied = 0;
+    *out_improvement = 0;
+    if (t == 0)
+        return 0;
+    const int mw = (t + 63) / 64, w = words_for(cols);
+    const size_t n_pool = n_cand + (size_t)t;
+    uint64_t *pm = calloc(n_pool * (size_t)mw, sizeof(uint64_t));
+    or_pool *pool = malloc(sizeof(or_pool) * n_pool);
+    uint64_t *piv = calloc((size_t)t * (size_t)mw, sizeof(uint64_t));
+    char *used = calloc((size_t)t, 1);
+    uint64_t *chosen = calloc((size_t)t * (size_t)mw, sizeof(uint64_t));
+    uint64_t *after = calloc((size_t)t * (size_t)stride, sizeof(uint64_t));
+    uint64_t *v = calloc((size_t)mw, sizeof(uint64_t));
+    int rc = 0;
+    for (size_t c = 0; c < n_cand && rc == 0; ++c)
+        for (int q = 0; q < mask_words; ++q) {
+            const uint64_t word = masks[c * (size_t)mask_words + q];
+            const int lo = q * 64;
+            const uint64_t allowed =
+                lo >= t ? 0ull : (t - lo >= 64 ? ~0ull : (1ull << (t - lo)) - 1ull);
+            if (word & ~allowed) {
+                rc = OR_EINVAL;
+                break;
+            }
+            if (q < mw)
+                pm[c * (size_t)mw + q] = word;
+        }
+    if (rc == 0) {
+        uint64_t old_weight = 0;
+        for (size_t c = 0; c < n_cand; ++c)
+            pool[c] = (or_pool){pm + c * (size_t)mw, weights[c], c};
+        for (int r = 0; r < t; ++r) {
+            uint64_t pc = 0;
+            for (int wi = 0; wi < w; ++wi)
+                pc += (uint64_t)__builtin_popcountll(rows[(size_t)r * stride + wi]);
+            old_weight += pc;
+            uint64_t *u = pm + (n_cand + (size_t)r) * (size_t)mw;
+            u[r / 64] = 1ull << (r % 64);
+            pool[n_cand + (size_t)r] = (or_pool){u, (uint32_t)pc, n_cand + (size_t)r};
+        }
+        qsort(pool, n_pool, sizeof(or_pool), pool_weight_stable);
+        int n_chosen = 0;
+        for (size_t p = 0; p < n_pool && n_chosen < t; ++p) {
+            memcpy(v, pool[p].mask, sizeof(uint64_t) * (size_t)mw);
+            int hb;
+            while ((hb = highest_bit_words(v, mw)) >= 0 && used[hb])
+                for (int q = 0; q < mw; ++q)
+                    v[q] ^= piv[(size_t)hb * mw + q];
+            if (hb < 0)
+                continue;
+            used[hb] = 1;
+            memcpy(piv + (size_t)hb * mw, v, sizeof(uint64_t) * (size_t)mw);
+            memcpy(chosen + (size_t)n_chosen * mw, pool[p].mask, sizeof(uint64_t) * (size_t)mw);
+            ++n_chosen;
+        }
+        if (n_chosen != t) {
+            rc = OR_ERANK;
+        } else {
+            uint64_t new_weight = 0;
+            for (int i = 0; i < t; ++i) {
+                uint64_t *o = after + (size_t)i * stride;
+                for (int k = 0; k < t; ++k)
+                    if ((chosen[(size_t)i * mw + k / 64] >> (k % 64)) & 1u)
+                        for (int wi = 0; wi < stride; ++wi)
+                            o[wi] ^= rows[(size_t)k * stride + wi];
+                for (int wi = 0; wi < w; ++wi)
+                    new_weight += (uint64_t)__builtin_popcountll(o[wi]);
+            }
+            if (!(new_weight > old_weight ||
+                  (require_improvement && new_weight == old_weight))) {
+                memcpy(rows, after, sizeof(uint64_t) * (size_t)t * (size_t)stride);
+                *out_applied = 1;
+                *out_improvement = old_weight - new_weight;
+            }
+        }
+    }
+    free(pm);
+    free(pool);
+    free(piv);
+    free(used);
+    free(chosen);
+    free(after);
+    free(v);
+    return rc;
 }

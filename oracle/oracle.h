@@ -101,6 +101,23 @@ int or_exact_basis(const uint64_t *base, int stride, int cols, const int *idx, i
 long long or_weight_order(const uint64_t *base, int stride, int cols, const int *idx, int t,
                           size_t k, uint32_t *out_masks, uint32_t *out_w);
 
+/* calc_scores_and_order (src/math/fwht.cpp:59-100); cnt and off hold n + 1
+ * entries, order m - 1.  Returns 0, -1 for n < 0, -2 (int16 form) for a
+ * score above 32767. */
+int or_scores_order_i32(const int32_t *h, size_t m, int32_t n, int32_t *s_by_mask,
+                        int32_t *order, int32_t *cnt, int32_t *off);
+int or_scores_order_i16(const int16_t *h, size_t m, int32_t n, int16_t *s_by_mask,
+                        int32_t *order, int32_t *cnt, int32_t *off);
+
+/* GlobalGreedyApplier::apply (src/apply/global_greedy_applier.cpp:15-105)
+ * on t rows (stride words) with n_cand candidates of mask_words words each.
+ * Rows replaced in place when applied.  Returns 0, -1 (mask bit >= t) or
+ * -4 (no full transform). */
+int or_global_greedy(uint64_t *rows, int t, int stride, int cols, const uint64_t *masks,
+                     int mask_words, const uint32_t *weights, size_t n_cand,
+                     int require_improvement, uint64_t *out_applied,
+                     uint64_t *out_improvement);
+
 #ifdef __cplusplus
 }
 #endif

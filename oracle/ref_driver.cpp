@@ -9,6 +9,7 @@
 // Return value: number of results written (>= 0), or -1 on a C++ exception
 // whose message is copied into `err` (up to err_len bytes).
 
+#include "bmmpy/apply/global_greedy_applier.hpp"
 #include "bmmpy/apply/greedy_applier.hpp"
 #include "bmmpy/core/bit_matrix.hpp"
 #include "bmmpy/core/row_window.hpp"
@@ -251,6 +252,58 @@ int ref_exact_basis(const std::uint64_t* rows, int t, int W, int cols, std::uint
         }
         *total = r.total_weight;
         return static_cast<int>(r.rank);
+    } catch (const std::exception& e) {
+        set_err(err, err_len, e.what());
+        return -1;
+    }
+}
+
+// calc_scores_and_order, both score types; returns 0.
+int ref_scores_order_i32(const std::int32_t* h, std::size_t m, std::int32_t n, std::int32_t* s,
+                         std::int32_t* order, std::int32_t* cnt, std::int32_t* off, char* err,
+                         int err_len) {
+    try {
+        bmmpy::calc_scores_and_order(h, m, n, s, order, cnt, off);
+        return 0;
+    } catch (const std::exception& e) {
+        set_err(err, err_len, e.what());
+        return -1;
+    }
+}
+
+int ref_scores_order_i16(const std::int16_t* h, std::size_t m, std::int32_t n, std::int16_t* s,
+                         std::int32_t* order, std::int32_t* cnt, std::int32_t* off, char* err,
+                         int err_len) {
+    try {
+        bmmpy::calc_scores_and_order(h, m, n, s, order, cnt, off);
+        return 0;
+    } catch (const std::exception& e) {
+        set_err(err, err_len, e.what());
+        return -1;
+    }
+}
+
+// GlobalGreedyApplier::apply over candidates of mask_words words each; rows
+// updated in place.
+int ref_global_greedy(std::uint64_t* rows, int t, int W, int cols, const std::uint64_t* masks,
+                      int mask_words, const std::uint32_t* weights, std::size_t n_cand,
+                      int require_improvement, std::uint64_t* out_applied,
+                      std::uint64_t* out_improvement, char* err, int err_len) {
+    try {
+        auto m = load(rows, t, W, cols);
+        auto win = m.row_window(all_rows(t));
+        std::vector<bmmpy::Candidate> cands;
+        for (std::size_t i = 0; i < n_cand; ++i)
+            cands.emplace_back(
+                bmmpy::Candidate::mask_type(masks + i * mask_words, masks + (i + 1) * mask_words),
+                weights[i]);
+        const auto res = bmmpy::GlobalGreedyApplier(require_improvement != 0).apply(win, cands);
+        *out_applied = res.applied_count;
+        *out_improvement = res.weight_improvement;
+        for (int rr = 0; rr < t; ++rr)
+            std::memcpy(rows + static_cast<std::size_t>(rr) * W, m.row_words(rr),
+                        sizeof(std::uint64_t) * W);
+        return 0;
     } catch (const std::exception& e) {
         set_err(err, err_len, e.what());
         return -1;

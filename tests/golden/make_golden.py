@@ -217,8 +217,120 @@ def main():
                              "seed": seed, "policy": policy, "expect": res})
     with open(os.path.join(HERE, "golden_sa.json"), "w") as f:
         json.dump({"cases": sa_cases}, f)
+    global_fixtures(ref, ora)
     print("wrote fixtures", flush=True)
 
 
+def combo_weights(mat, masks):
+    """True weights of the row combinations selected by multi-word masks."""
+    out = []
+    for mk in masks:
+        acc = np.zeros(mat.shape[1], np.uint64)
+        for r in range(mat.shape[0]):
+            if (int(mk[r // 64]) >> (r % 64)) & 1:
+                acc ^= mat[r]
+        out.append(sum(bin(int(v)).count("1") for v in acc))
+    return np.array(out, np.uint32)
+
+
+def gf2_inverse(M, n):
+    """Inverse of an invertible n x n GF(2) matrix (rows of mask words)."""
+    rows = [sum(int(M[i, q]) << (64 * q) for q in range(M.shape[1])) for i in range(n)]
+    inv = [1 << i for i in range(n)]
+    for c in range(n):
+        p = next(r for r in range(c, n) if (rows[r] >> c) & 1)
+        rows[c], rows[p], inv[c], inv[p] = rows[p], rows[c], inv[p], inv[c]
+        for r in range(n):
+            if r != c and (rows[r] >> c) & 1:
+                rows[r] ^= rows[c]
+                inv[r] ^= inv[c]
+    mw = (n + 63) // 64
+    return np.array([[(v >> (64 * q)) & (2**64 - 1) for q in range(mw)] for v in inv], np.uint64)
+
+
+def global_fixtures(ref, ora):
+    """calc_scores_and_order (src/math/fwht.cpp:59-126) and GlobalGreedyApplier
+    (src/apply/global_greedy_applier.cpp) -> golden_global.npz."""
+    out = {}
+    rng = np.random.default_rng(11)
+    score_specs = [("i32", 12, 37), ("i32", 13, 300), ("i32", 12, 0), ("i32", 14, 70000),
+                   ("i16", 12, 37), ("i16", 13, 5000), ("i16", 11, 30000)]
+    for q, (kind, nb, n) in enumerate(score_specs):
+        dt, lo, hi = (np.int32, -3 * n - 5, 3 * n + 5) if kind == "i32" else \
+            (np.int16, max(-32768, -n - 5), min(32767, n + 5))
+        h = rng.integers(lo, hi + 1, 1 << nb, dtype=np.int64).astype(dt)
+        s, order, cnt, off = ref.scores_order(h, n)
+        out[f"scores{q}_h"], out[f"scores{q}_n"] = h, np.array([n], np.int32)
+        out[f"scores{q}_s"], out[f"scores{q}_order"] = s, order
+        out[f"scores{q}_cnt"], out[f"scores{q}_off"] = cnt, off
+    # Spectrum of a real window: h = FWHT of the active-column histogram.
+    win = ora.bernoulli(51, 12, 640, 0.3)
+    hist = np.zeros(1 << 12, np.int32)
+    for c in range(640):
+        p = sum(((int(win[r, c // 64]) >> (c % 64)) & 1) << r for r in range(12))
+        hist[p] += 1 if p else 0
+    n_act = int(hist.sum())
+    h = ref.fwht_i32(hist)
+    s, order, cnt, off = ref.scores_order(h, n_act)
+    q = len(score_specs)
+    out[f"scores{q}_h"], out[f"scores{q}_n"] = h, np.array([n_act], np.int32)
+    out[f"scores{q}_s"], out[f"scores{q}_order"] = s, order
+    out[f"scores{q}_cnt"], out[f"scores{q}_off"] = cnt, off
+    out["scores_cases"] = np.array([q + 1], np.int32)
+
+    cases = 0
+    for t, cols, n_cand, seed in ((3, 64, 3, 61), (12, 256, 40, 62), (16, 300, 200, 63),
+                                  (40, 512, 120, 64), (70, 192, 150, 65), (24, 128, 0, 66)):
+        for req in (True, False):
+            mat = ora.bernoulli(seed, t, cols, 0.35)
+            mw = (t + 63) // 64
+            if t <= 16 and n_cand:
+                cands = ref.fwht_search(mat, cols, n_cand)
+                masks = np.array([[mk] for _, mk in cands], np.uint64).reshape(-1, 1)
+                ws = np.array([w for w, _ in cands], np.uint32)
+            else:
+                masks = rng.integers(0, 2**63, (n_cand, mw), dtype=np.int64).astype(np.uint64)
+                if t % 64:
+                    masks[:, -1] &= np.uint64((1 << (t % 64)) - 1)
+                sparse = rng.random((n_cand, t)) < 3.0 / t   # a few rows per mask
+                masks = np.zeros((n_cand, mw), np.uint64)
+                for i, j in zip(*np.nonzero(sparse)):
+                    masks[i, j // 64] |= np.uint64(1 << (j % 64))
+                ws = combo_weights(mat, masks)
+                # A few understated weights: the applier must still refuse a
+                # transform that does not lower the true weight.
+                ws[::7] = ws[::7] // 2
+            if n_cand == 0:
+                masks = np.zeros((0, mw), np.uint64)
+                ws = np.zeros(0, np.uint32)
+            res, ap, imp = ref.global_greedy(mat, cols, masks, ws, req)
+            out[f"gg{cases}_in"], out[f"gg{cases}_masks"], out[f"gg{cases}_w"] = mat, masks, ws
+            out[f"gg{cases}_meta"] = np.array([t, cols, int(req), ap, imp], np.int64)
+            out[f"gg{cases}_out"] = res
+            cases += 1
+    # Planted windows A = M S: the rows of M^-1 recover the sparse S, mixed
+    # with random sparse masks; weights are the true combination weights.
+    for t, cols, seed in ((40, 512, 71), (70, 192, 72), (64, 256, 73)):
+        S, A, M = ora.planted(seed, t, cols, 0.05)
+        inv = gf2_inverse(M, t)
+        mw = inv.shape[1]
+        extra = np.zeros((3 * t, mw), np.uint64)
+        for i, j in zip(*np.nonzero(rng.random((3 * t, t)) < 2.0 / t)):
+            extra[i, j // 64] |= np.uint64(1 << (j % 64))
+        masks = np.concatenate([extra[:t], inv, extra[t:]])
+        ws = combo_weights(A, masks)
+        for req in (True, False):
+            res, ap, imp = ref.global_greedy(A, cols, masks, ws, req)
+            out[f"gg{cases}_in"], out[f"gg{cases}_masks"], out[f"gg{cases}_w"] = A, masks, ws
+            out[f"gg{cases}_meta"] = np.array([t, cols, int(req), ap, imp], np.int64)
+            out[f"gg{cases}_out"] = res
+            cases += 1
+    out["gg_cases"] = np.array([cases], np.int32)
+    np.savez_compressed(os.path.join(HERE, "golden_global.npz"), **out)
+
+
 if __name__ == "__main__":
-    main()
+    if sys.argv[1:] == ["global"]:
+        global_fixtures(oracle.reference(), oracle.oracle())
+    else:
+        main()

@@ -34,12 +34,26 @@ def words_of(m: "bmm.BitMatrix"):
 def test_public_names():
     for name in ("BitMatrix", "RowWindow", "Candidate", "FwhtSearch", "MitmFwhtSearch",
                  "BruteforceSearch", "CudaMitmFwhtSearch", "CudaBruteforceSearch",
-                 "GreedyApplier", "ApplyResult", "search_apply", "matrix_from_rows",
+                 "GreedyApplier", "GlobalGreedyApplier", "ApplyResult", "search_apply",
+                 "matrix_from_rows", "SASelector", "solver",
                  "get_runtime_features", "get_version", "fwht_i16", "fwht_i32", "MatrixError"):
         assert hasattr(bmm, name), name
     # config structs stay hidden (tests/python/test_public_api.py:9-26)
     for hidden in ("FwhtSearchConfig", "MitmFwhtSearchConfig", "BruteforceSearchConfig"):
         assert not hasattr(bmm, hidden)
+
+
+def test_global_greedy_applier_surface():
+    ap = bmm.GlobalGreedyApplier()
+    assert ap.name() == "global_greedy" and ap.require_improvement is True
+    assert repr(bmm.GlobalGreedyApplier(require_improvement=False)) == \
+        "GlobalGreedyApplier(require_improvement=False)"
+    with pytest.raises(TypeError):
+        bmm.GlobalGreedyApplier(False)   # keyword-only, as bmmpy/apply.py:152
+    # An empty window returns before any device call.
+    m = bmm.BitMatrix(3, 5)
+    res = ap.apply(m.row_window([]), [])
+    assert (res.applied_count, res.weight_improvement) == (0, 0)
 
 
 def test_searcher_names_and_repr():
@@ -81,7 +95,9 @@ def test_bitmatrix_layout_and_ops():
     with pytest.raises(ValueError):
         bmm.matrix_from_rows(["10", "1"])
     i3 = bmm.BitMatrix.identity(3)
-    assert i3.rank() == 3 and (i3 @ i3).to_rows() == i3.to_rows()
+    assert i3.rank() == 3   # products run on the GPU (tests/test_gpu_global_apply.py)
+    with pytest.raises(bmm.MatrixError):   # shapes are checked before any device call
+        i3.mul(bmm.BitMatrix(2, 3))
 
 
 def test_candidate_api():

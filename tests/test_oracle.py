@@ -369,3 +369,107 @@ def test_exact_basis_matches_live_reference(ora):
                 if (int(mask) >> row) & 1:
                     acc ^= m[row]
             assert np.array_equal(acc, rb[q])
+
+
+# ------------------------------------------ scores / global greedy apply ---
+@pytest.fixture(scope="module")
+def gglobal():
+    return np.load(os.path.join(GOLD, "golden_global.npz"), allow_pickle=False)
+
+
+def test_scores_order_kats(ora):
+    # fwht_tests.cpp:72-107 (off: the end of each bucket after the scatter).
+    s, order, cnt, off = ora.scores_order(np.array([0, 0, 0, 4, 4], np.int32), 4)
+    assert s.tolist() == [0, 2, 2, 0, 0] and order.tolist() == [3, 4, 1, 2]
+    assert cnt.tolist() == [2, 0, 2, 0, 0] and off.tolist() == [2, 2, 4, 4, 4]
+    s, order, cnt, off = ora.scores_order(np.array([0, 4, 2, 0, -2], np.int16), 4)
+    assert s.dtype == np.int16 and s.tolist() == [0, 0, 1, 2, 3]
+    assert order.tolist() == [1, 2, 3, 4] and cnt.tolist() == [1, 1, 1, 1, 0]
+
+
+def test_scores_order_golden(ora, gglobal):
+    for q in range(int(gglobal["scores_cases"][0])):
+        h, n = gglobal[f"scores{q}_h"], int(gglobal[f"scores{q}_n"][0])
+        s, order, cnt, off = ora.scores_order(h, n)
+        for got, key in ((s, "s"), (order, "order"), (cnt, "cnt"), (off, "off")):
+            assert np.array_equal(got, gglobal[f"scores{q}_{key}"]), (q, key)
+
+
+@pytest.mark.parametrize("dt", [np.int32, np.int16])
+def test_scores_order_is_a_stable_sort(ora, dt):
+    """Closed forms: score = clamp(floor((n - h) / 2), 0, n); order = numpy's
+    stable argsort of the scores of masks 1..m-1; cnt = bincount; off =
+    cumulative cnt.  Covers m = 0 and 1 and n = 0."""
+    rng = np.random.default_rng(21)
+    for m, n in ((0, 3), (1, 5), (2, 0), (37, 0), (1000, 7), (4096, 300), (3000, 2000)):
+        lim = min(3 * n + 5, 32767) if dt == np.int16 else 3 * n + 5
+        h = rng.integers(-lim, lim + 1, m, dtype=np.int64).astype(dt)
+        s, order, cnt, off = ora.scores_order(h, n)
+        exp = np.clip(np.floor_divide(n - h.astype(np.int64), 2), 0, n)
+        if m:
+            exp[0] = 0
+        assert np.array_equal(s.astype(np.int64), exp)
+        assert np.array_equal(order, np.argsort(exp[1:], kind="stable") + 1)
+        assert np.array_equal(cnt, np.bincount(exp[1:], minlength=n + 1))
+        assert np.array_equal(off, np.cumsum(cnt) if m else np.zeros(n + 1, np.int32))
+
+
+def test_scores_order_matches_live_reference(ora, ref):
+    rng = np.random.default_rng(22)
+    for dt, m, n in ((np.int32, 5000, 999), (np.int32, 1 << 15, 70000), (np.int16, 4096, 1200),
+                     (np.int16, 777, 0)):
+        lim = min(2 * n + 9, 32767)
+        h = rng.integers(-lim, lim + 1, m, dtype=np.int64).astype(dt)
+        for a, b in zip(ora.scores_order(h, n), ref.scores_order(h, n)):
+            assert np.array_equal(a, b)
+
+
+def test_scores_order_rejects_negative_n(ora, ref):
+    with pytest.raises(oracle.OracleError):
+        ora.scores_order(np.zeros(4, np.int32), -1)
+    with pytest.raises(oracle.RefError, match="non-negative"):
+        ref.scores_order(np.zeros(4, np.int32), -1)
+
+
+def test_global_greedy_kat(ora):
+    # tests/python/test_matrix_api.py:66-105.
+    m, cols = oracle.matrix_from_rows(["111", "110", "001"])
+    out, ap, imp = ora.global_greedy(m, cols, [[0b011], [0b101], [0b110]], [1, 2, 3])
+    assert ap == 1 and imp > 0
+    assert ora.rank(out, cols) == ora.rank(m, cols)
+    eye, cols = oracle.matrix_from_rows(["10", "01"])
+    out, ap, imp = ora.global_greedy(eye, cols, np.zeros((0, 1), np.uint64), [])
+    assert (ap, imp) == (0, 0) and np.array_equal(out, eye)
+
+
+def test_global_greedy_golden(ora, gglobal):
+    for c in range(int(gglobal["gg_cases"][0])):
+        t, cols, req, ap, imp = (int(v) for v in gglobal[f"gg{c}_meta"])
+        out, a, i = ora.global_greedy(gglobal[f"gg{c}_in"], cols, gglobal[f"gg{c}_masks"],
+                                      gglobal[f"gg{c}_w"], bool(req))
+        assert (a, i) == (ap, imp), c
+        assert np.array_equal(out, gglobal[f"gg{c}_out"]), c
+
+
+def test_global_greedy_matches_live_reference(ora, ref):
+    rng = np.random.default_rng(23)
+    for t, cols in ((1, 64), (5, 70), (33, 130), (65, 64), (128, 200)):
+        mat = ora.bernoulli(90 + t, t, cols, 0.3)
+        mw = (t + 63) // 64
+        masks = np.zeros((2 * t, mw), np.uint64)
+        for i, j in zip(*np.nonzero(rng.random((2 * t, t)) < 2.5 / t)):
+            masks[i, j // 64] |= np.uint64(1 << (j % 64))
+        ws = rng.integers(0, cols // 3 + 1, 2 * t).astype(np.uint32)
+        for req in (True, False):
+            a = ora.global_greedy(mat, cols, masks, ws, req)
+            b = ref.global_greedy(mat, cols, masks, ws, req)
+            assert a[1:] == b[1:] and np.array_equal(a[0], b[0]), (t, req)
+
+
+def test_global_greedy_rejects_out_of_range_masks(ora, ref):
+    m, cols = oracle.matrix_from_rows(["10", "01"])
+    for masks in ([[0b100]], [[0b1, 0b1]]):
+        with pytest.raises(oracle.OracleError):
+            ora.global_greedy(m, cols, masks, [0])
+        with pytest.raises(oracle.RefError, match="outside the supported row range"):
+            ref.global_greedy(m, cols, masks, [0])
