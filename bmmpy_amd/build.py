@@ -60,7 +60,8 @@ def build_hip(jobs: int = 8) -> str:
 
 
 HOST_SOURCES = ["host/bit_matrix.cpp", "host/search.cpp", "host/ga.cpp", "host/apply.cpp",
-                "host/runtime.cpp", "host/solver.cpp", "host/sa_selector.cpp"]
+                "host/runtime.cpp", "host/solver.cpp", "host/sa_selector.cpp",
+                "host/matrix_io.cpp"]
 
 
 def build_host(jobs: int = 8) -> str | None:

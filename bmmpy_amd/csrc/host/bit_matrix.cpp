@@ -14,13 +14,13 @@ namespace bmmpy {
 
 namespace {
 const char* matrix_err_text(MatrixErr c) {
+    // src/core/bit_matrix.cpp:10-21.
     switch (c) {
-    case MatrixErr::OutOfRange: return "MatrixError: index out of range";
-    case MatrixErr::DimensionMismatch: return "MatrixError: dimension mismatch";
-    case MatrixErr::InvalidFormat: return "MatrixError: invalid format";
-    case MatrixErr::IoError: return "MatrixError: I/O error";
+    case MatrixErr::SizeTooLarge: return "Matrix dimensions too large";
+    case MatrixErr::AllocationFailed: return "Memory allocation failed";
+    case MatrixErr::DimensionMismatch: return "Matrix dimensions mismatch";
     }
-    return "MatrixError";
+    return "Unknown matrix error";
 }
 } // namespace
 
@@ -43,7 +43,7 @@ BitMatrix::BitMatrix(std::size_t rows, std::size_t cols) : _rows(rows), _cols(co
     const std::size_t words = (cols + 63) / 64;
     _stride = (words + 3) / 4 * 4;
     if (rows != 0 && _stride > SIZE_MAX / 8 / rows)
-        throw std::overflow_error("BitMatrix: size overflow");
+        throw MatrixError(MatrixErr::SizeTooLarge);
     allocate(_rows * _stride);
 }
 
@@ -54,7 +54,7 @@ void BitMatrix::allocate(std::size_t words) {
     void* p = std::aligned_alloc(k_alignment, (words * 8 + k_alignment - 1) / k_alignment *
                                                   k_alignment);
     if (!p)
-        throw std::bad_alloc();
+        throw MatrixError(MatrixErr::AllocationFailed);
     std::memset(p, 0, words * 8);
     _data = static_cast<std::uint64_t*>(p);
 }
@@ -216,6 +216,55 @@ BitMatrix BitMatrix::mul(const BitMatrix& other) const {
     for (std::size_t i = 0; i < _rows; ++i)
         std::memcpy(out.row_words(i), prod.data() + i * ow, ow * 8);
     return out;
+}
+
+void BitMatrix::xor_assign(const BitMatrix& other) {
+    if (_rows != other._rows || _cols != other._cols)
+        throw MatrixError(MatrixErr::DimensionMismatch);
+    for (std::size_t r = 0; r < _rows; ++r)
+        row_xor_from(r, other, r);
+}
+
+BitMatrix BitMatrix::add(const BitMatrix& other) const {
+    BitMatrix out(*this);
+    out.xor_assign(other);
+    return out;
+}
+
+BitMatrix BitMatrix::power(std::uint32_t exp) const {
+    if (_rows != _cols)
+        throw MatrixError(MatrixErr::DimensionMismatch);
+    BitMatrix result = identity(_rows), base(*this);
+    while (exp) {
+        if (exp & 1u)
+            result = result.mul(base);
+        exp >>= 1;
+        if (exp)
+            base = base.mul(base);
+    }
+    return result;
+}
+
+// src/core/bit_matrix_rows.cpp:97-131 (whole strides are copied).
+BitMatrix BitMatrix::extract_rows_by_indices(const std::vector<std::size_t>& indices) const {
+    BitMatrix out(indices.size(), _cols);
+    for (std::size_t d = 0; d < indices.size(); ++d) {
+        if (indices[d] >= _rows)
+            throw std::out_of_range("source row out of bounds");
+        std::memcpy(out._data + d * _stride, _data + indices[d] * _stride, _stride * 8);
+    }
+    return out;
+}
+
+void BitMatrix::insert_rows_by_indices(const BitMatrix& source,
+                                       const std::vector<std::size_t>& indices) {
+    if (source._rows != indices.size() || _cols != source._cols || _stride != source._stride)
+        throw MatrixError(MatrixErr::DimensionMismatch);
+    for (std::size_t s = 0; s < indices.size(); ++s) {
+        if (indices[s] >= _rows)
+            throw std::out_of_range("target row out of bounds");
+        std::memcpy(_data + indices[s] * _stride, source._data + s * _stride, _stride * 8);
+    }
 }
 
 BitMatrix BitMatrix::identity(std::size_t n) {

@@ -9,6 +9,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <iosfwd>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -17,7 +18,8 @@
 namespace bmmpy {
 
 // ------------------------------------------------------------- errors ----
-enum class MatrixErr { OutOfRange, DimensionMismatch, InvalidFormat, IoError };
+// include/bmmpy/core/bit_matrix.hpp:14.
+enum class MatrixErr { SizeTooLarge, AllocationFailed, DimensionMismatch };
 
 // RuntimeError subclass raised for matrix shape problems
 // (python/bindings/bit_matrix.cpp:94).
@@ -75,8 +77,30 @@ public:
     std::uint64_t row_popcount(std::size_t row) const;
     std::uint64_t weight() const;
     std::size_t rank() const;
+    std::size_t total_bytes() const noexcept { return total_words() * 8; }
     BitMatrix mul(const BitMatrix& other) const;
+    // XOR of equal shapes (src/core/bit_matrix_algebra.cpp:13-25).
+    void xor_assign(const BitMatrix& other);
+    BitMatrix add(const BitMatrix& other) const;
+    // Square-and-multiply over GPU products (bit_matrix_algebra.cpp:52-74).
+    BitMatrix power(std::uint32_t exp) const;
+    BitMatrix extract_rows_by_indices(const std::vector<std::size_t>& indices) const;
+    void insert_rows_by_indices(const BitMatrix& source, const std::vector<std::size_t>& indices);
     static BitMatrix identity(std::size_t n);
+
+    // Text and BMMPYB1 binary formats (src/core/bit_matrix_io.cpp) and the
+    // canonical SHA-256 (src/core/bit_matrix_hash.cpp): hex digest of
+    // le64(rows) | le64(cols) | every row's words little-endian, the last word
+    // masked to the logical columns.
+    void save_text(std::ostream& out) const;
+    void save_binary(std::ostream& out) const;
+    void save_text(const std::string& path) const;
+    void save_binary(const std::string& path) const;
+    static BitMatrix load_text(std::istream& in);
+    static BitMatrix load_binary(std::istream& in);
+    static BitMatrix load_text(const std::string& path);
+    static BitMatrix load_binary(const std::string& path);
+    std::string hash() const;
 
     RowWindow row_window(const std::vector<std::size_t>& rows);
     RowWindow row_window(const std::vector<std::size_t>& rows) const;

@@ -14,15 +14,22 @@ using namespace bmmpy;
 
 namespace {
 
+// The text format (src/core/bit_matrix_io.cpp:34-47), as the reference's
+// to_text / __str__.
 std::string matrix_to_text(const BitMatrix& m) {
-    std::string s;
-    for (std::size_t r = 0; r < m.rows(); ++r) {
-        for (std::size_t c = 0; c < m.cols(); ++c)
-            s.push_back(m.get(r, c) ? '1' : '0');
-        if (r + 1 < m.rows())
-            s.push_back('\n');
-    }
-    return s;
+    std::ostringstream out;
+    m.save_text(out);
+    return out.str();
+}
+
+BitMatrix matrix_from_text(const std::string& text) {
+    std::istringstream in(text);
+    return BitMatrix::load_text(in);
+}
+
+// str or os.PathLike -> filesystem path string.
+std::string fspath(const py::object& p) {
+    return py::module_::import("os").attr("fspath")(p).cast<std::string>();
 }
 
 std::vector<std::string> matrix_to_rows(const BitMatrix& m) {
@@ -41,12 +48,13 @@ BitMatrix matrix_from_rows(const std::vector<std::string>& rows) {
     BitMatrix m(rows.size(), cols);
     for (std::size_t r = 0; r < rows.size(); ++r) {
         if (rows[r].size() != cols)
-            throw std::invalid_argument("matrix_from_rows: rows must have equal width");
+            throw std::invalid_argument("BitMatrix.from_rows: all rows must have equal width");
         for (std::size_t c = 0; c < cols; ++c) {
             if (rows[r][c] == '1')
                 m.set(r, c, true);
             else if (rows[r][c] != '0')
-                throw std::invalid_argument("matrix_from_rows: rows may contain only 0 and 1");
+                throw std::invalid_argument(
+                    "BitMatrix.from_rows: rows must contain only '0' and '1'");
         }
     }
     return m;
@@ -130,10 +138,9 @@ PYBIND11_MODULE(_bmmpy, m) {
     });
 
     py::enum_<MatrixErr>(m, "MatrixErr")
-        .value("OutOfRange", MatrixErr::OutOfRange)
-        .value("DimensionMismatch", MatrixErr::DimensionMismatch)
-        .value("InvalidFormat", MatrixErr::InvalidFormat)
-        .value("IoError", MatrixErr::IoError);
+        .value("SizeTooLarge", MatrixErr::SizeTooLarge)
+        .value("AllocationFailed", MatrixErr::AllocationFailed)
+        .value("DimensionMismatch", MatrixErr::DimensionMismatch);
 
     py::class_<BitMatrix>(m, "BitMatrix")
         .def(py::init<>())
@@ -145,13 +152,36 @@ PYBIND11_MODULE(_bmmpy, m) {
         .def_property_readonly("stride_words", &BitMatrix::stride_words)
         .def_property_readonly("words_per_row", &BitMatrix::words_per_row)
         .def_property_readonly("total_words", &BitMatrix::total_words)
+        .def_property_readonly("total_bytes", &BitMatrix::total_bytes)
         .def("__len__", &BitMatrix::rows)
         .def("__str__", &matrix_to_text)
         .def("__repr__",
              [](const BitMatrix& b) {
                  return "BitMatrix(rows=" + std::to_string(b.rows()) +
-                        ", cols=" + std::to_string(b.cols()) + ")";
+                        ", cols=" + std::to_string(b.cols()) +
+                        ", weight=" + std::to_string(b.weight()) + ")";
              })
+        .def("copy", [](const BitMatrix& b) { return BitMatrix(b); })
+        .def(
+            "copy_from_words",
+            [](BitMatrix& b, const std::vector<std::uint64_t>& words) {
+                b.copy_from_words(words.data(), words.size());
+            },
+            py::arg("words"))
+        .def("add", &BitMatrix::add, py::arg("other"))
+        .def("power", &BitMatrix::power, py::arg("exp"))
+        .def("extract_rows_by_indices", &BitMatrix::extract_rows_by_indices, py::arg("indices"))
+        .def("insert_rows_by_indices", &BitMatrix::insert_rows_by_indices, py::arg("source"),
+             py::arg("indices"))
+        .def(
+            "save_text_file",
+            [](const BitMatrix& b, const py::object& path) { b.save_text(fspath(path)); },
+            py::arg("path"))
+        .def(
+            "save_binary_file",
+            [](const BitMatrix& b, const py::object& path) { b.save_binary(fspath(path)); },
+            py::arg("path"))
+        .def("hash", &BitMatrix::hash)
         .def("__getitem__",
              [](const BitMatrix& b, const py::tuple& t) {
                  auto [r, c] = index2(t);
@@ -182,6 +212,15 @@ PYBIND11_MODULE(_bmmpy, m) {
             py::arg("rows"), py::keep_alive<0, 1>())
         .def_static("from_rows", &matrix_from_rows, py::arg("rows"))
         .def_static("from_words", &matrix_from_words, py::arg("words"), py::arg("cols"))
+        .def_static("from_text", &matrix_from_text, py::arg("text"))
+        .def_static(
+            "load_text_file",
+            [](const py::object& path) { return BitMatrix::load_text(fspath(path)); },
+            py::arg("path"))
+        .def_static(
+            "load_binary_file",
+            [](const py::object& path) { return BitMatrix::load_binary(fspath(path)); },
+            py::arg("path"))
         .def_static("identity", &BitMatrix::identity, py::arg("n"));
 
     py::class_<RowWindow>(m, "RowWindow")

@@ -321,6 +321,11 @@ class Reference:
         L.ref_global_greedy.argtypes = [_u64p, C.c_int, C.c_int, C.c_int, _u64p, C.c_int, _u32p,
                                         C.c_size_t, C.c_int, C.POINTER(C.c_uint64),
                                         C.POINTER(C.c_uint64)] + ep
+        L.ref_matrix_io.argtypes = [_u64p, C.c_int, C.c_int, C.c_int, C.c_char_p, C.c_char_p,
+                                    C.c_size_t, C.POINTER(C.c_size_t), C.c_char_p, C.c_size_t,
+                                    C.POINTER(C.c_size_t)] + ep
+        L.ref_matrix_load.argtypes = [C.c_char_p, C.c_size_t, C.c_int, _u64p, C.c_size_t,
+                                      _u64p] + ep
 
     @staticmethod
     def _err():
@@ -442,6 +447,28 @@ class Reference:
     def scores_order(self, h, n):
         return _scores_order_call(self.lib.ref_scores_order_i32, self.lib.ref_scores_order_i16,
                                   h, n, self._run)
+
+    def matrix_io(self, m, cols):
+        """Reference (hash hex, save_binary bytes, save_text str) of rows m."""
+        m = self._rows(m)
+        hexbuf = C.create_string_buffer(65)
+        bl, tl = C.c_size_t(), C.c_size_t()
+        self._run(self.lib.ref_matrix_io, m, m.shape[0], m.shape[1], cols, hexbuf, None, 0,
+                  C.byref(bl), None, 0, C.byref(tl))
+        b = C.create_string_buffer(max(bl.value, 1))
+        t = C.create_string_buffer(max(tl.value, 1))
+        self._run(self.lib.ref_matrix_io, m, m.shape[0], m.shape[1], cols, hexbuf, b, bl.value,
+                  C.byref(bl), t, tl.value, C.byref(tl))
+        return hexbuf.value.decode(), b.raw[:bl.value], t.raw[:tl.value].decode()
+
+    def matrix_load(self, data: bytes, text: bool = False, cap: int = 1 << 20):
+        """Reference load_binary / load_text: (rows as words, rows, cols)."""
+        out = np.zeros(max(cap, 1), np.uint64)
+        shape = np.zeros(2, np.uint64)
+        self._run(self.lib.ref_matrix_load, data, len(data), int(text), out, cap, shape)
+        r, c = int(shape[0]), int(shape[1])
+        w = (c + 63) // 64
+        return out[:r * w].reshape(r, w), r, c
 
     def global_greedy(self, m, cols, masks, weights, require_improvement=True):
         m = np.array(m, dtype=np.uint64, copy=True, order="C")

@@ -26,6 +26,7 @@
 #include <cstring>
 #include <exception>
 #include <numeric>
+#include <sstream>
 #include <string>
 #include <vector>
 
@@ -303,6 +304,55 @@ int ref_global_greedy(std::uint64_t* rows, int t, int W, int cols, const std::ui
         for (int rr = 0; rr < t; ++rr)
             std::memcpy(rows + static_cast<std::size_t>(rr) * W, m.row_words(rr),
                         sizeof(std::uint64_t) * W);
+        return 0;
+    } catch (const std::exception& e) {
+        set_err(err, err_len, e.what());
+        return -1;
+    }
+}
+
+// BitMatrix::hash, save_binary and save_text of t rows (words as given, so
+// stray bits past `cols` reach the writers as they would in the reference).
+// Lengths are returned even when the buffers are too small (then nothing is
+// copied); returns 0.
+int ref_matrix_io(const std::uint64_t* rows, int t, int W, int cols, char* hex65,
+                  unsigned char* bin, std::size_t bin_cap, std::size_t* bin_len, char* text,
+                  std::size_t text_cap, std::size_t* text_len, char* err, int err_len) {
+    try {
+        auto m = load(rows, t, W, cols);
+        const std::string h = m.hash();
+        std::memcpy(hex65, h.c_str(), h.size() + 1);
+        std::ostringstream b, x;
+        m.save_binary(b);
+        m.save_text(x);
+        const std::string bs = b.str(), xs = x.str();
+        *bin_len = bs.size();
+        *text_len = xs.size();
+        if (bs.size() <= bin_cap)
+            std::memcpy(bin, bs.data(), bs.size());
+        if (xs.size() <= text_cap)
+            std::memcpy(text, xs.data(), xs.size());
+        return 0;
+    } catch (const std::exception& e) {
+        set_err(err, err_len, e.what());
+        return -1;
+    }
+}
+
+// BitMatrix::load_binary / load_text (kind 0 / 1) from a byte buffer; writes
+// rows x words_per_row words (up to cap) and the shape; returns 0.
+int ref_matrix_load(const unsigned char* buf, std::size_t len, int kind, std::uint64_t* out,
+                    std::size_t cap, std::uint64_t* shape, char* err, int err_len) {
+    try {
+        std::istringstream in(std::string(reinterpret_cast<const char*>(buf), len));
+        const bmmpy::BitMatrix m =
+            kind == 0 ? bmmpy::BitMatrix::load_binary(in) : bmmpy::BitMatrix::load_text(in);
+        shape[0] = m.rows();
+        shape[1] = m.cols();
+        const std::size_t w = m.words_per_row();
+        if (m.rows() * w <= cap)
+            for (std::size_t r = 0; r < m.rows(); ++r)
+                std::memcpy(out + r * w, m.row_words(r), sizeof(std::uint64_t) * w);
         return 0;
     } catch (const std::exception& e) {
         set_err(err, err_len, e.what());

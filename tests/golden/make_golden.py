@@ -218,6 +218,7 @@ def main():
     with open(os.path.join(HERE, "golden_sa.json"), "w") as f:
         json.dump({"cases": sa_cases}, f)
     global_fixtures(ref, ora)
+    io_fixtures(ref, ora)
     print("wrote fixtures", flush=True)
 
 
@@ -329,8 +330,52 @@ def global_fixtures(ref, ora):
     np.savez_compressed(os.path.join(HERE, "golden_global.npz"), **out)
 
 
+def io_fixtures(ref, ora):
+    """BitMatrix text / BMMPYB1 binary writers, readers and the canonical hash
+    (src/core/bit_matrix_io.cpp, bit_matrix_hash.cpp) -> golden_io.npz."""
+    out = {}
+    specs = [(np.zeros((0, 0), np.uint64), 0), (np.zeros((3, 0), np.uint64), 0),
+             (np.zeros((0, 1), np.uint64), 5),
+             # 1 x 65 with stray bits past column 64 (bit_matrix_hash_tests.cpp:11-31).
+             (np.array([[0x1, 0x1 | (1 << 17)]], np.uint64), 65),
+             (np.array([[0x1, 0x1]], np.uint64), 65),
+             (np.array([[0x3, 0x1]], np.uint64), 65),
+             (ora.bernoulli(101, 5, 130, 0.5), 130), (ora.bernoulli(102, 64, 1000, 0.3), 1000),
+             (ora.bernoulli(103, 7, 64, 0.5), 64)]
+    for q, (m, cols) in enumerate(specs):
+        h, b, t = ref.matrix_io(m, cols)
+        out[f"io{q}_in"], out[f"io{q}_cols"] = m, np.array([cols], np.int64)
+        out[f"io{q}_hash"] = np.array([h])
+        out[f"io{q}_bin"] = np.frombuffer(b, np.uint8)
+        out[f"io{q}_text"] = np.array([t])
+    out["io_cases"] = np.array([len(specs)], np.int32)
+    # Malformed inputs and the reference's messages.
+    good = ref.matrix_io(ora.bernoulli(104, 2, 70, 0.5), 70)
+    bad_bin = [b"", b"BMMPY", b"BMMPYB2\x00" + good[1][8:], good[1][:20], good[1][:-3],
+               good[1] + b"\x00\x00"]
+    bad_txt = [b"", b"2", b"2 3\n101\n01", b"1 3\n1x1\n", b"1 3\n101\n1", b"1 3 101 \n\t ",
+               b"2 2\n1 0\n0\r\n1\n"]
+    msgs = []
+    for kind, cases in ((0, bad_bin), (1, bad_txt)):
+        for data in cases:
+            try:
+                words, r, c = ref.matrix_load(data, text=bool(kind))
+                msgs.append((kind, data, "", words, r, c))
+            except oracle.RefError as e:
+                msgs.append((kind, data, str(e), np.zeros((0, 1), np.uint64), 0, 0))
+    for q, (kind, data, msg, words, r, c) in enumerate(msgs):
+        out[f"bad{q}_kind"] = np.array([kind], np.int32)
+        out[f"bad{q}_data"] = np.frombuffer(data, np.uint8)
+        out[f"bad{q}_msg"] = np.array([msg])
+        out[f"bad{q}_words"], out[f"bad{q}_shape"] = words, np.array([r, c], np.int64)
+    out["bad_cases"] = np.array([len(msgs)], np.int32)
+    np.savez_compressed(os.path.join(HERE, "golden_io.npz"), **out)
+
+
 if __name__ == "__main__":
     if sys.argv[1:] == ["global"]:
         global_fixtures(oracle.reference(), oracle.oracle())
+    elif sys.argv[1:] == ["io"]:
+        io_fixtures(oracle.reference(), oracle.oracle())
     else:
         main()
