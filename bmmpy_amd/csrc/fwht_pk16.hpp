@@ -14,7 +14,8 @@
 //
 // Build: the histogram is accumulated into biased halves (every half starts at
 // 0x8000), so a +-1 update of one half never borrows from or carries into the
-// other; the first transform round removes the bias.
+// other.  The bias needs no removal: the first stage adds or subtracts two
+// biased halves, and 0x8000 + 0x8000 = 0 and 0x8000 - 0x8000 = 0 modulo 2^16.
 #pragma once
 
 #include "fwht_block.hpp"
@@ -98,8 +99,8 @@ __device__ inline void pk_build_mem(uint32_t* s, const uint32_t* __restrict__ pa
 }
 
 // One radix-2^R round over word bits [LO, LO + R): one group per thread
-// (2^(13 - R) = 256 groups for R = 5).  The first round also removes the bias
-// and applies the bit-13 stage.
+// (2^(13 - R) = 256 groups for R = 5).  The first round also applies the
+// bit-13 stage (which cancels the build bias).
 template <int LO, int R, bool FIRST> __device__ inline void pk_round(uint32_t* s) {
     static_assert((1 << (kPkW - R)) == 256, "one group per thread");
     constexpr int E = 1 << R;
@@ -115,7 +116,7 @@ template <int LO, int R, bool FIRST> __device__ inline void pk_round(uint32_t* s
     if constexpr (FIRST) {
 #pragma unroll
         for (int e = 0; e < E; ++e)
-            v[e] = pk_half_stage(v[e] ^ kPkBias);
+            v[e] = pk_half_stage(v[e]);
     }
     pk_butterflies(v);
 #pragma unroll

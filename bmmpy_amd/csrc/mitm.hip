@@ -32,19 +32,30 @@ struct WinTarget {
     int32_t L;        // (total + 1) << L < 2^31 (packed block keys below)
     int64_t first_block;  // number of this target's block 0 in its kernel's queue
     int32_t slot;     // output index
-    int32_t pad;
+    int32_t top;      // packed targets: 1 when pattern >> 14 has 33 bits (47 rows)
+    int64_t lo_off;   // packed targets: WinPat8 list of the low-half entries
+    int64_t hi_off;   //   and of the high-half entries,
+    int32_t n_lo;     //   each padded to a multiple of 256 entries
+    int32_t n_hi;
 };
 
 constexpr int kWinMaxL = 13;
 
-// A window pattern prepared for packed-int16 blocks: the sign for block mh is
-// the parity of hi & mh; val (the signed multiplicity, shifted into the high
-// half for entries >= 2^13) is added to LDS word `addr` (padded index).
-struct WinPat {
-    uint64_t hi;    // pattern >> 14
-    uint32_t addr;  // fwht_pad(pattern mod 2^13)
-    int32_t val;    // mult or mult << 16
+// A window entry prepared for packed-int16 blocks.  The sign for block mh is
+// the parity of (pattern >> 14) & mh: `hi` keeps the low 32 bits of
+// pattern >> 14, and av bit 0 its bit 32 (47-row windows only; the LDS byte
+// address is a multiple of 4).  av = mult << 16 | LDS byte address | that bit:
+// a low-half entry adds mult (sign-extended) to the word, a high-half entry
+// mult << 16.  Padding entries add 0 at distinct addresses.
+struct WinPat8 {
+    uint32_t hi;
+    uint32_t av;
 };
+
+// Entries per thread and half held in registers across a target's blocks
+// (lists of at most 256 * 10 entries per half); longer lists are read from
+// memory for every block.
+constexpr int kWinRegSlots = 10;
 
 template <int L>
 __device__ inline void block_build_g_signed(int32_t* s, const uint64_t* __restrict__ pat,
@@ -111,26 +122,134 @@ __device__ void win_block(int32_t* s, const WinTarget& T, const uint64_t* pats,
     __syncthreads();
 }
 
-// Packed-int16 block of 2^14 masks (windows of >= 14 rows, at most 32767
-// active columns).  Word w = (q + 4 e) * 256 + t holds masks w and
-// w + 2^13.  Each entry becomes the key H * 2^16 | (65535 - 4 m') with
-// m' = ml - t (one v_lshl_or / v_and_or), so one max keeps the largest H and
-// the lowest ml among the thread's entries.
-__device__ void win_block_pk16(uint32_t* s, const WinTarget& T, const WinPat* __restrict__ wpats,
-                               uint64_t mh, unsigned long long* key) {
-    {
-        int4* s4 = reinterpret_cast<int4*>(s);
-        const int b = (int)kPkBias;
-        for (int i = threadIdx.x; i < (pk_lds_words() + 3) / 4; i += blockDim.x)
-            s4[i] = make_int4(b, b, b, b);
-        __syncthreads();
-        const WinPat* wp = wpats + T.pat_off;
-        for (int i = threadIdx.x; i < T.n_pat; i += blockDim.x) {
-            const WinPat P = wp[i];
-            const uint32_t v = (uint32_t)P.val;
-            atomicAdd(&s[P.addr], (__builtin_popcountll(P.hi & mh) & 1) ? 0u - v : v);
+// One entry of a packed block build: sign = parity(hi & mh) (+ av bit 0 &
+// mh bit 32 when TOP), applied as (v ^ m) - m with m = 0 or -1.
+template <bool TOP, bool HIGH>
+__device__ inline void win_add(char* sb, uint32_t hi, uint32_t av, uint32_t mh, uint32_t mh_top) {
+    const uint32_t par = __builtin_popcount(hi & mh) + (TOP ? (av & mh_top) : 0u);
+    const int32_t p = (int32_t)(par & 1u);
+    const int32_t v = HIGH ? (int32_t)(av & 0xffff0000u) : ((int32_t)av >> 16);
+    atomicAdd(reinterpret_cast<uint32_t*>(sb + (av & 0xfffcu)), (uint32_t)((v ^ -p) + p));
+}
+
+// The entries of one padded list from memory, four loads in flight.
+template <bool TOP, bool HIGH>
+__device__ inline void win_add_list(char* sb, const WinPat8* __restrict__ p, int n, uint32_t mh,
+                                    uint32_t mh_top) {
+    const WinPat8* q = p + threadIdx.x;
+    int i = 0;
+    for (; i + 1024 <= n; i += 1024) {
+        const WinPat8 P0 = q[i], P1 = q[i + 256], P2 = q[i + 512], P3 = q[i + 768];
+        win_add<TOP, HIGH>(sb, P0.hi, P0.av, mh, mh_top);
+        win_add<TOP, HIGH>(sb, P1.hi, P1.av, mh, mh_top);
+        win_add<TOP, HIGH>(sb, P2.hi, P2.av, mh, mh_top);
+        win_add<TOP, HIGH>(sb, P3.hi, P3.av, mh, mh_top);
+    }
+    for (; i < n; i += 256)
+        win_add<TOP, HIGH>(sb, q[i].hi, q[i].av, mh, mh_top);
+}
+
+// A target's entries held in registers: slot k < n_lo / 256 of the low half
+// is entry lo_off + 256 k + t, likewise the high half at kWinRegSlots + k.
+struct WinRegs {
+    uint32_t hi[2 * kWinRegSlots];
+    uint32_t av[2 * kWinRegSlots];
+    bool held;
+};
+
+__device__ inline void win_load_regs(WinRegs& R, const WinTarget& T,
+                                     const WinPat8* __restrict__ w8) {
+    R.held = T.n_lo <= 256 * kWinRegSlots && T.n_hi <= 256 * kWinRegSlots;
+    if (!R.held)
+        return;
+#pragma unroll
+    for (int k = 0; k < kWinRegSlots; ++k) {
+        if (256 * k < T.n_lo) {
+            const WinPat8 P = w8[T.lo_off + 256 * k + threadIdx.x];
+            R.hi[k] = P.hi;
+            R.av[k] = P.av;
+        }
+        if (256 * k < T.n_hi) {
+            const WinPat8 P = w8[T.hi_off + 256 * k + threadIdx.x];
+            R.hi[kWinRegSlots + k] = P.hi;
+            R.av[kWinRegSlots + k] = P.av;
         }
     }
+}
+
+template <bool TOP>
+__device__ inline void win_add_all(char* sb, const WinTarget& T, const WinRegs& R,
+                                   const WinPat8* __restrict__ w8, uint32_t mh, uint32_t mh_top) {
+    if (R.held) {
+#pragma unroll
+        for (int k = 0; k < kWinRegSlots; ++k)
+            if (256 * k < T.n_lo)
+                win_add<TOP, false>(sb, R.hi[k], R.av[k], mh, mh_top);
+#pragma unroll
+        for (int k = 0; k < kWinRegSlots; ++k)
+            if (256 * k < T.n_hi)
+                win_add<TOP, true>(sb, R.hi[kWinRegSlots + k], R.av[kWinRegSlots + k], mh, mh_top);
+    } else {
+        win_add_list<TOP, false>(sb, w8 + T.lo_off, T.n_lo, mh, mh_top);
+        win_add_list<TOP, true>(sb, w8 + T.hi_off, T.n_hi, mh, mh_top);
+    }
+}
+
+__device__ inline void win_build_pk16(uint32_t* s, const WinTarget& T, const WinRegs& R,
+                                      const WinPat8* __restrict__ w8, uint64_t mh) {
+    int4* s4 = reinterpret_cast<int4*>(s);
+    const int b = (int)kPkBias;
+    for (int i = threadIdx.x; i < (pk_lds_words() + 3) / 4; i += blockDim.x)
+        s4[i] = make_int4(b, b, b, b);
+    __syncthreads();
+    char* sb = reinterpret_cast<char*>(s);
+    if (T.top)
+        win_add_all<true>(sb, T, R, w8, (uint32_t)mh, (uint32_t)(mh >> 32) & 1u);
+    else
+        win_add_all<false>(sb, T, R, w8, (uint32_t)mh, 0u);
+}
+
+// Packed-int16 block of 2^14 masks (windows of >= 14 rows, at most 32767
+// active columns).  Word w = (q + 4 e) * 256 + t holds masks w and
+// w + 2^13.
+//
+// Pass 0 keeps only the block's largest H (one v_pk_max per word).  The
+// block's best key is at least (w << n | mh << 14) with w = (total - H) / 2;
+// when that floor is not below the target's current key (a relaxed read:
+// keys only decrease, so a stale value only sends more blocks on), no mask of
+// the block can win and the block ends.  Otherwise pass 1 rebuilds and
+// transforms the block and folds every entry into the key
+// H * 2^16 | (65535 - 4 m') with m' = ml - t, so one max keeps the largest H
+// and the lowest ml among the thread's entries.
+__device__ void win_block_pk16(uint32_t* s, const WinTarget& T, const WinRegs& R,
+                               const WinPat8* __restrict__ w8, uint64_t mh,
+                               unsigned long long* key) {
+    __shared__ int32_t wbest[4];
+    __shared__ int exact;
+    win_build_pk16(s, T, R, w8, mh);
+    {
+        s16x2 m = pk_as(0x80008000u);
+        pk_block_fwht_visit(s, [&](int, const uint32_t (&v)[8]) {
+#pragma unroll
+            for (int e = 0; e < 8; ++e)
+                m = pk_max(m, pk_as(v[e]));
+        });
+        const int32_t h = wave_max_i32(max(pk_lo(m), pk_hi(m)));
+        if (threadIdx.x % kWave == 0)
+            wbest[threadIdx.x / kWave] = h;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const int32_t hb = max(max(wbest[0], wbest[1]), max(wbest[2], wbest[3]));
+            const uint32_t w = (uint32_t)(T.total - hb) >> 1;
+            const unsigned long long floor_key =
+                ((unsigned long long)w << T.bits) | (mh << kPkL);
+            exact = floor_key < __atomic_load_n(key, __ATOMIC_RELAXED) ? 1 : 0;
+        }
+        __syncthreads();
+        if (!exact)
+            return;
+    }
+    win_build_pk16(s, T, R, w8, mh);
     int32_t best = INT32_MIN;
     pk_block_fwht_visit(s, [&](int q, const uint32_t (&v)[8]) {
 #pragma unroll
@@ -149,7 +268,6 @@ __device__ void win_block_pk16(uint32_t* s, const WinTarget& T, const WinPat* __
         best = (int32_t)(((uint32_t)h << 16) | (65535u - 4u * ml));
     }
     best = wave_max_i32(best);
-    __shared__ int32_t wbest[4];
     if (threadIdx.x % kWave == 0)
         wbest[threadIdx.x / kWave] = best;
     __syncthreads();
@@ -172,7 +290,7 @@ __global__ __launch_bounds__(256) void win_full_kernel(const WinTarget* __restri
                                                        int n_tgt, int64_t total_blocks,
                                                        const uint64_t* __restrict__ pats,
                                                        const int32_t* __restrict__ mults,
-                                                       const WinPat* __restrict__ wpats,
+                                                       const WinPat8* __restrict__ w8,
                                                        int chunk,
                                                        unsigned long long* __restrict__ counter,
                                                        unsigned long long* __restrict__ keys) {
@@ -180,6 +298,8 @@ __global__ __launch_bounds__(256) void win_full_kernel(const WinTarget* __restri
     __shared__ __align__(16) int32_t s[kWords + 4];
     __shared__ long long first;
     int ti = 0;
+    [[maybe_unused]] WinRegs R;
+    [[maybe_unused]] int held_ti = -1;
     for (;;) {
         if (threadIdx.x == 0)
             first = (long long)atomicAdd(counter, (unsigned long long)chunk);
@@ -194,9 +314,13 @@ __global__ __launch_bounds__(256) void win_full_kernel(const WinTarget* __restri
                 ++ti;
             const WinTarget T = tgts[ti];
             const uint64_t mh = (uint64_t)(g - T.first_block);
-            if constexpr (L == kPkL)
-                win_block_pk16(reinterpret_cast<uint32_t*>(s), T, wpats, mh, &keys[T.slot]);
-            else
+            if constexpr (L == kPkL) {
+                if (held_ti != ti) {   // a new target: reload its register-held entries
+                    win_load_regs(R, T, w8);
+                    held_ti = ti;
+                }
+                win_block_pk16(reinterpret_cast<uint32_t*>(s), T, R, w8, mh, &keys[T.slot]);
+            } else
                 win_block<L>(s, T, pats, mults, mh, &keys[T.slot]);
         }
     }
@@ -207,7 +331,7 @@ __global__ __launch_bounds__(256) void win_min_kernel(const WinTarget* __restric
                                                       int n_tgt, int64_t total_blocks,
                                                       const uint64_t* __restrict__ pats,
                                                       const int32_t* __restrict__ mults,
-                                                      const WinPat* __restrict__ /*wpats*/,
+                                                      const WinPat8* __restrict__ /*w8*/,
                                                       int chunk,
                                                       unsigned long long* __restrict__ counter,
                                                       unsigned long long* __restrict__ keys) {
@@ -242,7 +366,7 @@ __global__ __launch_bounds__(256) void win_min_kernel(const WinTarget* __restric
 
 namespace {
 struct WinWorkspace {
-    DevBuf tgts, pats, mults, wpats, keys, counter;
+    DevBuf tgts, pats, mults, w8, keys, counter;
 };
 WinWorkspace& win_ws() {
     thread_local WinWorkspace ws;
@@ -266,7 +390,7 @@ void window_coset_min(const uint64_t* rows, size_t n_rows, size_t words, size_t 
     std::vector<WinTarget> tg(n_targets);
     std::vector<uint64_t> all_p;
     std::vector<int32_t> all_m;
-    std::vector<WinPat> all_wp;   // same indexing, for packed-int16 targets
+    std::vector<WinPat8> all_w8;  // per-target lists for packed-int16 targets
     const uint64_t tail = cols % 64 ? ((1ull << (cols % 64)) - 1) : ~0ull;
     for (size_t i = 0; i < n_targets; ++i) {
         const int t = targets[i];
@@ -311,14 +435,36 @@ void window_coset_min(const uint64_t* rows, size_t n_rows, size_t words, size_t 
             if (m != 0) {
                 all_p.push_back(p);
                 all_m.push_back(m);
-                const uint32_t pl = (uint32_t)p & ((1u << kPkL) - 1u);
-                all_wp.push_back(WinPat{p >> kPkL,
-                                        (uint32_t)fwht_pad((int)(pl & ((1u << kPkW) - 1u))),
-                                        (pl >> kPkW) ? (int32_t)((uint32_t)m << 16) : m});
             }
             a = b;
         }
         T.n_pat = (int32_t)(all_p.size() - T.pat_off);
+        // Packed lists: low-half entries, then high-half entries, each padded
+        // to a multiple of 256 with zero entries at distinct words.
+        T.top = width > (size_t)kPkL + 32 ? 1 : 0;
+        T.lo_off = T.hi_off = 0;
+        T.n_lo = T.n_hi = 0;
+        if (width >= (size_t)kPkL && total <= kPkMaxTotal) {
+            for (uint32_t half = 0; half < 2; ++half) {
+                const int64_t off = (int64_t)all_w8.size();
+                for (int64_t k = T.pat_off; k < (int64_t)all_p.size(); ++k) {
+                    const uint32_t pl = (uint32_t)all_p[k] & ((1u << kPkL) - 1u);
+                    if ((pl >> kPkW) != half)
+                        continue;
+                    const uint64_t hi = all_p[k] >> kPkL;
+                    const uint32_t byte = 4u * (uint32_t)fwht_pad((int)(pl & ((1u << kPkW) - 1u)));
+                    all_w8.push_back(WinPat8{
+                        (uint32_t)hi, ((uint32_t)(uint16_t)(int16_t)all_m[k] << 16) | byte |
+                                          (uint32_t)((hi >> 32) & 1u)});
+                }
+                while ((all_w8.size() - off) % 256)
+                    all_w8.push_back(WinPat8{
+                        0u, 4u * (uint32_t)fwht_pad((int)((all_w8.size() - off) &
+                                                          ((1u << kPkW) - 1u)))});
+                (half ? T.hi_off : T.lo_off) = off;
+                (half ? T.n_hi : T.n_lo) = (int32_t)(all_w8.size() - off);
+            }
+        }
         T.bits = (int32_t)width;
         T.total = total;
         int L = (int)std::min<size_t>(width, kWinMaxL);
@@ -328,7 +474,6 @@ void window_coset_min(const uint64_t* rows, size_t n_rows, size_t words, size_t 
             L = kPkL;
         T.L = L;
         T.slot = (int32_t)i;
-        T.pad = 0;
     }
     if (n_targets == 0)
         return;
@@ -350,14 +495,14 @@ void window_coset_min(const uint64_t* rows, size_t n_rows, size_t words, size_t 
     WinTarget* d_t = ws.tgts.get<WinTarget>(n_targets);
     uint64_t* d_p = ws.pats.get<uint64_t>(std::max<size_t>(all_p.size(), 1));
     int32_t* d_m = ws.mults.get<int32_t>(std::max<size_t>(all_m.size(), 1));
-    WinPat* d_wp = ws.wpats.get<WinPat>(std::max<size_t>(all_wp.size(), 1));
+    WinPat8* d_w8 = ws.w8.get<WinPat8>(std::max<size_t>(all_w8.size(), 1));
     unsigned long long* d_k = ws.keys.get<unsigned long long>(n_targets);
     unsigned long long* d_c = ws.counter.get<unsigned long long>(3);
     BMM_HIP(hipMemcpy(d_t, tg.data(), sizeof(WinTarget) * n_targets, hipMemcpyHostToDevice));
     if (!all_p.empty()) {
         BMM_HIP(hipMemcpy(d_p, all_p.data(), all_p.size() * 8, hipMemcpyHostToDevice));
         BMM_HIP(hipMemcpy(d_m, all_m.data(), all_m.size() * 4, hipMemcpyHostToDevice));
-        BMM_HIP(hipMemcpy(d_wp, all_wp.data(), all_wp.size() * sizeof(WinPat),
+        BMM_HIP(hipMemcpy(d_w8, all_w8.data(), all_w8.size() * sizeof(WinPat8),
                           hipMemcpyHostToDevice));
     }
     fill_u64(reinterpret_cast<uint64_t*>(d_k), n_targets, ~0ull, nullptr);
@@ -379,7 +524,7 @@ void window_coset_min(const uint64_t* rows, size_t n_rows, size_t words, size_t 
             auto kern = qi == 0 ? win_full_kernel<kPkL>
                         : qi == 1 ? win_full_kernel<kWinMaxL> : win_min_kernel;
             hipLaunchKernelGGL(kern, dim3(grid), dim3(256), 0, nullptr, d_q[qi], (int)n_in[qi],
-                               q_blocks[qi], d_p, d_m, d_wp, chunk, d_c + qi, d_k);
+                               q_blocks[qi], d_p, d_m, d_w8, chunk, d_c + qi, d_k);
             BMM_HIP(hipGetLastError());
         }
     }
