@@ -430,7 +430,7 @@ def mode_line(args, r: dict, cpu: dict | None) -> dict:
     return out
 
 
-def ga_compare(budget_s: float, seed: int = 0x5EED0005) -> dict:
+def ga_compare(budget_s: float, seed: int = 0x5EED0005, isd_trials: int = 0) -> dict:
     """Planted 256 x 8192 matrix (Bernoulli(0.01) sparse S, random invertible
     256 x 256 transform M, A = M S); 64 islands x 256 individuals x 512 ops,
     ring migration every 20 generations.  Plain and learned proposal runs get
@@ -444,7 +444,7 @@ def ga_compare(budget_s: float, seed: int = 0x5EED0005) -> dict:
     target = int(planted * 1.01)
     res = {"planted_weight": planted, "start_weight": start_w, "target": target}
     for name, learned in (("plain", 0), ("learned", 1)):
-        p = _lib.ga_default_params(learned=learned, seed=seed)
+        p = _lib.ga_default_params(learned=learned, seed=seed, isd_trials=isd_trials)
         t0 = time.perf_counter()
         ga = _lib.GpuGA(A, 8192, p)
         setup = time.perf_counter() - t0
@@ -471,7 +471,8 @@ def ga_compare(budget_s: float, seed: int = 0x5EED0005) -> dict:
                      "evals_per_s": st.evaluations / max(elapsed, 1e-9),
                      "ms_per_generation": 1e3 * elapsed / max(int(st.generations), 1),
                      "train_loss": st.train_loss, "li_moves": int(st.li_moves),
-                     "window_moves": int(st.win_moves), "trajectory": traj[-50:]}
+                     "window_moves": int(st.win_moves), "isd_rows": int(st.isd_rows),
+                     "trajectory": traj[-50:]}
     return res
 
 
@@ -488,6 +489,9 @@ def main() -> int:
                          "FWHT coset-minimum workloads; ga: plain vs learned GA")
     ap.add_argument("--ga-budget", type=float, default=60.0,
                     help="wall-clock seconds per GA variant in --workload ga")
+    ap.add_argument("--ga-isd-trials", type=int, default=1,
+                    help="information-set trials per island per migration epoch in "
+                         "--workload ga (both variants; 0 = off)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="after the run, copy a buffer of this many MiB once (a plain "
                          "device copy of known size, to scale HBM byte counters)")
@@ -517,7 +521,7 @@ def main() -> int:
         atexit.register(_copy_once)
 
     if args.workload == "ga":
-        r = ga_compare(args.ga_budget)
+        r = ga_compare(args.ga_budget, isd_trials=args.ga_isd_trials)
         plain, learned = r["plain"], r["learned"]
         ratio = None
         if plain["time_to_target_s"] and learned["time_to_target_s"]:
@@ -529,7 +533,8 @@ def main() -> int:
             "scaling": "weak", "vs_baseline": None, "dtype": "u32", "data": "synthetic",
             "config": {"workload": "ga_island_planted_256x8192", "islands": 64,
                        "population": 256, "max_ops": 512, "migrate_every": 20,
-                       "window_rows": 16, "window_rounds_per_epoch": 2,
+                       "window_rows": 20, "window_rounds_per_epoch": 2,
+                       "isd_trials_per_epoch": args.ga_isd_trials,
                        "budget_s": args.ga_budget},
             "learned_over_plain_time_ratio": ratio, "ga": r}))
         return 0

@@ -59,7 +59,7 @@ class GaParams(C.Structure):
                 ("li_sweeps", C.c_int), ("rows", C.c_int), ("p_append", C.c_float),
                 ("p_replace", C.c_float), ("p_delete", C.c_float), ("learning_rate", C.c_float),
                 ("p_learned", C.c_float), ("seed", C.c_uint64), ("win_rows", C.c_int),
-                ("win_rounds", C.c_int), ("win_candidates", C.c_int), ("reserved", C.c_int)]
+                ("win_rounds", C.c_int), ("win_candidates", C.c_int), ("isd_trials", C.c_int)]
 
 
 class GaStatus(C.Structure):
@@ -67,7 +67,7 @@ class GaStatus(C.Structure):
                 ("best_weight", C.c_uint64), ("best_island", C.c_int),
                 ("reached_target", C.c_int), ("elapsed_s", C.c_double),
                 ("train_loss", C.c_double), ("li_moves", C.c_uint64),
-                ("win_moves", C.c_uint64)]
+                ("win_moves", C.c_uint64), ("isd_rows", C.c_uint64)]
 
 
 # name -> (argtypes, restype)

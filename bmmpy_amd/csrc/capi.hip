@@ -554,6 +554,7 @@ int bmm_hip_ga_default_params(bmm_ga_params* p) {
         p->win_rows = 20;
         p->win_rounds = 2;
         p->win_candidates = 32;
+        p->isd_trials = 1;
     });
 }
 

@@ -737,6 +737,248 @@ __global__ void ga_win_apply_kernel(uint64_t* __restrict__ base, int rows, int w
 }
 
 // best_w[isl] = total weight of the island base (after local improvement).
+// ------------------------------------------- information-set improvement --
+// Random information set per island (Prange's step of information-set
+// decoding): Gauss-Jordan elimination of the island base B on columns drawn
+// at random, pivoting on the lowest not-yet-pivoted row that has the column,
+// turns B into rows R = T B that each hold one pivot column alone.  Any
+// sparse vector of the row space that meets the chosen columns once is one
+// of these rows.  The rows of R (coefficient masks T) and the rows of B (unit
+// masks) then form the pool of GlobalGreedyApplier
+// (src/apply/global_greedy_applier.cpp:31-105): stably sorted by weight,
+// each entry kept when its mask is independent of the kept ones, until a
+// full basis is chosen.  The result is the lightest basis the pool holds, so
+// the island's weight never grows.
+//
+// One workgroup per island.  Thread r owns row r's mask (<= 4 words) in
+// registers; row XORs run over (row, word) pairs of the island's copy in
+// `rbuf` (L2-resident: rows x words u64 per island).
+constexpr int kIsdMaskWords = 4;   // rows <= 256
+
+__global__ __launch_bounds__(256) void ga_isd_elim_kernel(const uint64_t* __restrict__ base,
+                                                          int rows, int words, int cols,
+                                                          uint64_t seed,
+                                                          uint64_t* __restrict__ rbuf,
+                                                          uint64_t* __restrict__ tmask,
+                                                          uint32_t* __restrict__ rw) {
+    const int isl = blockIdx.x, tid = threadIdx.x;
+    const int wave = tid / kWave, lane = tid % kWave;
+    const size_t per = (size_t)rows * words;
+    const uint64_t* B = base + (size_t)isl * per;
+    uint64_t* Rb = rbuf + (size_t)isl * per;
+    for (size_t k = tid; k < per; k += blockDim.x)
+        Rb[k] = B[k];
+    uint64_t t[kIsdMaskWords] = {};
+    if (tid < rows)
+        t[tid / 64] = 1ull << (tid % 64);
+    __shared__ uint64_t piv_t[kIsdMaskWords];
+    __shared__ uint8_t hasbit[256], pivoted[256];
+    __shared__ int wmin[4];
+    __shared__ int s_col;
+    pivoted[tid] = 0;
+    hasbit[tid] = 0;
+    // Threads per row in the XOR sweep: a power of two covering the words
+    // (words is a multiple of 32), at most the workgroup.
+    int tpr = 32;
+    while (tpr < words && tpr < 256)
+        tpr <<= 1;
+    const int groups = 256 / tpr, sub = tid / tpr, w0 = tid % tpr;
+    uint64_t rng = seed ^ (0x9E3779B97F4A7C15ull * (uint64_t)(isl + 1));
+    if (rng == 0)
+        rng = 1;
+    __syncthreads();
+    int npiv = 0;
+    const int max_draws = 8 * cols + 64;
+    for (int draw = 0; draw < max_draws && npiv < rows; ++draw) {
+        if (tid == 0) {
+            rng ^= rng << 13;
+            rng ^= rng >> 7;
+            rng ^= rng << 17;
+            s_col = (int)(rng % (uint64_t)cols);
+        }
+        __syncthreads();
+        const int c = s_col;
+        const int bit = tid < rows ? (int)((Rb[(size_t)tid * words + (c >> 6)] >> (c & 63)) & 1ull)
+                                   : 0;
+        hasbit[tid] = (uint8_t)bit;
+        const uint64_t cand = __ballot(bit && !pivoted[tid]);
+        if (lane == 0)
+            wmin[wave] = cand ? wave * kWave + __builtin_ctzll(cand) : 1 << 30;
+        __syncthreads();
+        const int r = min(min(wmin[0], wmin[1]), min(wmin[2], wmin[3]));
+        if (r >= rows) {
+            __syncthreads();
+            continue;
+        }
+        if (tid == r) {
+            pivoted[r] = 1;
+#pragma unroll
+            for (int q = 0; q < kIsdMaskWords; ++q)
+                piv_t[q] = t[q];
+        }
+        __syncthreads();
+        if (bit && tid != r) {
+#pragma unroll
+            for (int q = 0; q < kIsdMaskWords; ++q)
+                t[q] ^= piv_t[q];
+        }
+        const uint64_t* P = Rb + (size_t)r * words;
+        for (int r2 = sub; r2 < rows; r2 += groups)
+            if (hasbit[r2] && r2 != r)
+                for (int w = w0; w < words; w += tpr)
+                    Rb[(size_t)r2 * words + w] ^= P[w];
+        __syncthreads();
+        ++npiv;
+    }
+    if (tid < rows) {
+        uint64_t* T = tmask + ((size_t)isl * rows + tid) * kIsdMaskWords;
+#pragma unroll
+        for (int q = 0; q < kIsdMaskWords; ++q)
+            T[q] = t[q];
+        uint32_t c = 0;
+        const uint64_t* row = Rb + (size_t)tid * words;
+        for (int w = 0; w < words; ++w)
+            c += (uint32_t)__builtin_popcountll(row[w]);
+        rw[(size_t)isl * rows + tid] = c;
+    }
+}
+
+// GlobalGreedyApplier over the pool {R rows (masks T)} + {B rows (unit
+// masks)}: bitonic sort of (weight, pool index) keys, then one entry at a
+// time against a fully reduced basis in LDS (reduce by the pivots set in the
+// mask, pivot = highest remaining bit, clear that bit from the other basis
+// rows).  chosen[k] = pool index of the k-th kept entry.
+__global__ __launch_bounds__(256) void ga_isd_select_kernel(int rows,
+                                                            const uint64_t* __restrict__ tmask,
+                                                            const uint32_t* __restrict__ rw,
+                                                            const uint32_t* __restrict__ roww,
+                                                            int16_t* __restrict__ chosen,
+                                                            unsigned int* __restrict__ n_isd) {
+    const int isl = blockIdx.x, tid = threadIdx.x;
+    const int wave = tid / kWave, lane = tid % kWave;
+    __shared__ unsigned long long key[512];
+    __shared__ uint64_t tm[256][kIsdMaskWords];
+    __shared__ uint64_t basis[256][kIsdMaskWords];
+    __shared__ uint8_t used[256];
+    __shared__ uint64_t red[4][kIsdMaskWords];
+    __shared__ uint64_t v_new[kIsdMaskWords];
+    __shared__ int s_piv, s_count, s_kept_isd;
+    const int pool = 2 * rows;
+    for (int k = tid; k < 512; k += blockDim.x) {
+        unsigned long long kk = ~0ull;
+        if (k < pool) {
+            const uint32_t w = k < rows ? rw[(size_t)isl * rows + k]
+                                        : roww[(size_t)isl * rows + (k - rows)];
+            kk = ((unsigned long long)w << 10) | (unsigned)k;
+        }
+        key[k] = kk;
+    }
+    for (int k = tid; k < rows; k += blockDim.x) {
+#pragma unroll
+        for (int q = 0; q < kIsdMaskWords; ++q) {
+            tm[k][q] = tmask[((size_t)isl * rows + k) * kIsdMaskWords + q];
+            basis[k][q] = 0;
+        }
+        used[k] = 0;
+    }
+    if (tid == 0) {
+        s_count = 0;
+        s_kept_isd = 0;
+    }
+    __syncthreads();
+    // Bitonic sort of 512 keys (ascending).
+    for (int size = 2; size <= 512; size <<= 1)
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            for (int i = tid; i < 256; i += blockDim.x) {
+                const int lo = 2 * i - (i & (stride - 1));
+                const int hi = lo + stride;
+                const bool up = (lo & size) == 0;
+                const unsigned long long a = key[lo], b = key[hi];
+                if ((a > b) == up) {
+                    key[lo] = b;
+                    key[hi] = a;
+                }
+            }
+            __syncthreads();
+        }
+    for (int e = 0; e < pool; ++e) {
+        if (s_count >= rows)
+            break;
+        const int idx = (int)(key[e] & 1023u);
+        uint64_t m[kIsdMaskWords];
+#pragma unroll
+        for (int q = 0; q < kIsdMaskWords; ++q)
+            m[q] = idx < rows ? tm[idx][q] : ((idx - rows) / 64 == q ? 1ull << ((idx - rows) % 64) : 0ull);
+        // XOR of the basis rows whose pivot bit is set in m.
+        uint64_t x[kIsdMaskWords] = {};
+        if (tid < rows && used[tid] && ((m[tid / 64] >> (tid % 64)) & 1ull)) {
+#pragma unroll
+            for (int q = 0; q < kIsdMaskWords; ++q)
+                x[q] = basis[tid][q];
+        }
+#pragma unroll
+        for (int q = 0; q < kIsdMaskWords; ++q) {
+            for (int o = kWave / 2; o > 0; o >>= 1)
+                x[q] ^= __shfl_xor(x[q], o, kWave);
+            if (lane == 0)
+                red[wave][q] = x[q];
+        }
+        __syncthreads();
+        if (tid == 0) {
+            int p = -1;
+            for (int q = kIsdMaskWords - 1; q >= 0; --q) {
+                const uint64_t v = m[q] ^ red[0][q] ^ red[1][q] ^ red[2][q] ^ red[3][q];
+                v_new[q] = v;
+                if (p < 0 && v)
+                    p = q * 64 + 63 - __builtin_clzll(v);
+            }
+            s_piv = p;
+            if (p >= 0) {
+                chosen[(size_t)isl * rows + s_count] = (int16_t)idx;
+                ++s_count;
+                if (idx < rows)
+                    ++s_kept_isd;
+            }
+        }
+        __syncthreads();
+        const int p = s_piv;
+        if (p >= 0) {
+            // Clear bit p from the kept rows (p is not yet a pivot, so row p
+            // is unused here), then install the new row.
+            if (tid < rows && used[tid] && ((basis[tid][p / 64] >> (p % 64)) & 1ull)) {
+#pragma unroll
+                for (int q = 0; q < kIsdMaskWords; ++q)
+                    basis[tid][q] ^= v_new[q];
+            }
+            __syncthreads();
+            if (tid == 0) {
+#pragma unroll
+                for (int q = 0; q < kIsdMaskWords; ++q)
+                    basis[p][q] = v_new[q];
+                used[p] = 1;
+            }
+        }
+        __syncthreads();
+    }
+    if (tid == 0 && s_kept_isd)
+        atomicAdd(n_isd, (unsigned)s_kept_isd);
+}
+
+// New base: row k = R row chosen[k] or B row chosen[k] - rows, into `out`.
+__global__ void ga_isd_apply_kernel(const uint64_t* __restrict__ base,
+                                    const uint64_t* __restrict__ rbuf, int rows, int words,
+                                    const int16_t* __restrict__ chosen,
+                                    uint64_t* __restrict__ out) {
+    const int isl = blockIdx.y, k = blockIdx.x;
+    const int c = chosen[(size_t)isl * rows + k];
+    const size_t per = (size_t)rows * words;
+    const uint64_t* src = c < rows ? rbuf + isl * per + (size_t)c * words
+                                   : base + isl * per + (size_t)(c - rows) * words;
+    uint64_t* dst = out + isl * per + (size_t)k * words;
+    for (int w = threadIdx.x; w < words; w += blockDim.x)
+        dst[w] = src[w];
+}
+
 __global__ void ga_base_weight_kernel(const uint32_t* __restrict__ roww, int rows,
                                       unsigned int* __restrict__ best_w) {
     const int isl = blockIdx.x;
@@ -766,8 +1008,11 @@ struct Ga {
     DevBuf pairs, pair_isl, feats, labels, prng;
     DevBuf li_move, li_w, li_changed;
     DevBuf win_cand, win, win_move, win_w;
+    DevBuf isd_r, isd_t, isd_w, isd_chosen, isd_count;
     uint64_t win_round = 0;
     uint64_t win_moves = 0;
+    uint64_t isd_round = 0;
+    uint64_t isd_rows = 0;
     Mlp* mlp = nullptr;
     int cur = 0;
     long generation = 0;
@@ -1021,6 +1266,45 @@ unsigned int window_improvement(Ga& G) {
     return n;
 }
 
+// Information-set trials on every island base (ga_isd_*_kernel); returns the
+// rows taken from the eliminated bases.
+unsigned int isd_improvement(Ga& G) {
+    const int I = G.P.islands, R = G.P.rows;
+    const size_t per = (size_t)R * G.words;
+    unsigned int* cnt = static_cast<unsigned int*>(G.isd_count.ptr);
+    fill_u32(cnt, 1, 0u, G.stream);
+    refresh_sketches(G);
+    for (int t = 0; t < G.P.isd_trials; ++t) {
+        const uint64_t seed = (uint64_t)G.P.seed ^ (0xA0761D6478BD642Full * (++G.isd_round));
+        hipLaunchKernelGGL(ga_isd_elim_kernel, dim3(I), dim3(256), 0, G.stream,
+                           static_cast<const uint64_t*>(G.base.ptr), R, G.words, G.words * 64,
+                           seed, static_cast<uint64_t*>(G.isd_r.ptr),
+                           static_cast<uint64_t*>(G.isd_t.ptr), static_cast<uint32_t*>(G.isd_w.ptr));
+        hipLaunchKernelGGL(ga_isd_select_kernel, dim3(I), dim3(256), 0, G.stream, R,
+                           static_cast<const uint64_t*>(G.isd_t.ptr),
+                           static_cast<const uint32_t*>(G.isd_w.ptr),
+                           static_cast<const uint32_t*>(G.roww.ptr),
+                           static_cast<int16_t*>(G.isd_chosen.ptr), cnt);
+        hipLaunchKernelGGL(ga_isd_apply_kernel, dim3(R, I), dim3(128), 0, G.stream,
+                           static_cast<const uint64_t*>(G.base.ptr),
+                           static_cast<const uint64_t*>(G.isd_r.ptr), R, G.words,
+                           static_cast<const int16_t*>(G.isd_chosen.ptr),
+                           static_cast<uint64_t*>(G.base_tmp.ptr));
+        BMM_HIP(hipGetLastError());
+        BMM_HIP(hipMemcpyAsync(G.base.ptr, G.base_tmp.ptr, (size_t)I * per * 8,
+                               hipMemcpyDeviceToDevice, G.stream));
+        refresh_sketches(G);
+    }
+    hipLaunchKernelGGL(ga_base_weight_kernel, dim3(I), dim3(256), 0, G.stream,
+                       static_cast<const uint32_t*>(G.roww.ptr), R,
+                       static_cast<unsigned int*>(G.best_w.ptr));
+    BMM_HIP(hipGetLastError());
+    unsigned int n = 0;
+    BMM_HIP(hipMemcpyAsync(&n, cnt, 4, hipMemcpyDeviceToHost, G.stream));
+    BMM_HIP(hipStreamSynchronize(G.stream));
+    return n;
+}
+
 void epoch(Ga& G) {
     const int I = G.P.islands;
     const int parity = G.cur;   // population about to be evaluated
@@ -1054,6 +1338,8 @@ void epoch(Ga& G) {
     }
     if (G.P.win_rows > 0 && G.P.win_rounds > 0)
         G.win_moves += window_improvement(G);
+    if (G.P.isd_trials > 0)
+        G.isd_rows += isd_improvement(G);
     update_proposals(G);
 }
 
@@ -1083,6 +1369,8 @@ Ga* ga_create(const uint64_t* matrix, size_t rows, size_t cols, const bmm_ga_par
         if (cols > 8192)
             P.win_rows = 0;   // one thread holds cols / 256 <= 32 column patterns
     }
+    if (P.isd_trials < 0 || P.isd_trials > 64)
+        fail(BMM_ERR_INVALID, "ga: isd_trials must be in [0, 64]");
     P.rows = (int)rows;
     G->words = (int)(cols / 64);
     G->W32 = (int)(cols / 32);
@@ -1148,6 +1436,13 @@ Ga* ga_create(const uint64_t* matrix, size_t rows, size_t cols, const bmm_ga_par
         G->win_move.get<uint32_t>((size_t)I * rows);
         G->win_w.get<uint32_t>((size_t)I * rows);
     }
+    if (P.isd_trials) {
+        G->isd_r.get<uint64_t>((size_t)I * rows * G->words);
+        G->isd_t.get<uint64_t>((size_t)I * rows * kIsdMaskWords);
+        G->isd_w.get<uint32_t>((size_t)I * rows);
+        G->isd_chosen.get<int16_t>((size_t)I * rows);
+        G->isd_count.get<uint32_t>(1);
+    }
     if (P.learned)
         G->mlp = mlp_create(P.seed ^ 0xA5A5A5A5ull);
     update_proposals(*G);
@@ -1200,6 +1495,7 @@ void ga_run(Ga* G, size_t max_generations, uint64_t target, double time_budget_s
     st->train_loss = G->train_loss;
     st->li_moves = G->li_moves;
     st->win_moves = G->win_moves;
+    st->isd_rows = G->isd_rows;
 }
 
 // Best matrix: the best island's base with its best sequence replayed.

@@ -273,7 +273,9 @@ typedef struct bmm_ga_params {
                            off for matrices wider than 8192 columns) */
     int win_rounds;     /* window rounds per epoch */
     int win_candidates; /* learned: candidates scored per target, best win_rows kept */
-    int reserved;
+    int isd_trials;     /* information-set trials per island per epoch (0 = off, <= 64):
+                           Gauss-Jordan elimination on random columns, then the lightest
+                           basis from those rows and the island's (GlobalGreedyApplier) */
 } bmm_ga_params;
 
 typedef struct bmm_ga_status {
@@ -287,6 +289,7 @@ typedef struct bmm_ga_status {
     double train_loss;    /* last MLP training loss (learned) */
     uint64_t li_moves;    /* local-improvement moves applied so far */
     uint64_t win_moves;   /* window-improvement moves applied so far */
+    uint64_t isd_rows;    /* rows taken from information-set trials so far */
 } bmm_ga_status;
 
 int bmm_hip_ga_default_params(bmm_ga_params *out);

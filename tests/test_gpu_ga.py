@@ -119,7 +119,8 @@ def test_ga_local_improvement_matches_restatement(lib):
     _say("ga local improvement")
     S, A, M = lib.gen_planted(0x5EED0005, 32, 2048, 0.05)
     p = lib.ga_default_params(islands=1, population=4, max_ops=8, migrate_every=1,
-                              mutations=0, li_sweeps=1, check_every=1, learned=0, win_rows=0)
+                              mutations=0, li_sweeps=1, check_every=1, learned=0, win_rows=0,
+                              isd_trials=0)
     ga = lib.GpuGA(A, 2048, p)
     st = ga.run(1)
     best, w = ga.best()
@@ -166,7 +167,7 @@ def test_ga_window_improvement_matches_restatement(lib, ora, learned, cols):
     rounds = 1 if learned else 2   # ties may pick other rows in model order
     p = lib.ga_default_params(islands=2, population=4, max_ops=8, migrate_every=1,
                               mutations=0, li_sweeps=0, check_every=1, learned=learned,
-                              win_rows=16, win_rounds=rounds, win_candidates=32)
+                              win_rows=16, win_rounds=rounds, win_candidates=32, isd_trials=0)
     ga = lib.GpuGA(A, cols, p)
     st = ga.run(1)
     best, w = ga.best()
@@ -179,3 +180,97 @@ def test_ga_window_improvement_matches_restatement(lib, ora, learned, cols):
     # Row operations only: same row space (full rank of [A; best] equals rank A).
     both = np.concatenate([A, best])
     assert ora.rank(both, cols) == ora.rank(A, cols) == 32
+
+
+def _isd_reference(A, seed, isl, cols):
+    """One information-set trial restated: column draws from XorShift64
+    seeded as the kernel seeds it, Gauss-Jordan elimination pivoting on the
+    lowest unpivoted row that has the column, then GlobalGreedyApplier's pool
+    (eliminated rows with their masks, then the island rows with unit masks),
+    stable by weight, kept while independent."""
+    M64 = (1 << 64) - 1
+    n = A.shape[0]
+    rows = [int(sum(int(A[i, k]) << (64 * k) for k in range(A.shape[1]))) for i in range(n)]
+    R = list(rows)
+    T = [1 << i for i in range(n)]
+    rng = (seed ^ ((0x9E3779B97F4A7C15 * (isl + 1)) & M64)) & M64 or 1
+    pivoted = [False] * n
+    npiv, draws = 0, 0
+    while npiv < n and draws < 8 * cols + 64:
+        draws += 1
+        rng ^= (rng << 13) & M64
+        rng ^= rng >> 7
+        rng ^= (rng << 17) & M64
+        c = rng % cols
+        has = [(R[i] >> c) & 1 for i in range(n)]
+        cand = [i for i in range(n) if has[i] and not pivoted[i]]
+        if not cand:
+            continue
+        r = cand[0]
+        pivoted[r] = True
+        npiv += 1
+        for i in range(n):
+            if has[i] and i != r:
+                R[i] ^= R[r]
+                T[i] ^= T[r]
+    pool = [(bin(R[i]).count("1"), i) for i in range(n)] + \
+           [(bin(rows[j]).count("1"), n + j) for j in range(n)]
+    pool.sort()
+    piv, out = {}, []
+    for _, idx in pool:
+        v = T[idx] if idx < n else 1 << (idx - n)
+        while v:
+            hb = v.bit_length() - 1
+            if hb in piv:
+                v ^= piv[hb]
+            else:
+                piv[hb] = v
+                out.append(R[idx] if idx < n else rows[idx - n])
+                break
+        if len(out) == n:
+            break
+    W = A.shape[1]
+    return np.array([[(r >> (64 * k)) & M64 for k in range(W)] for r in out], dtype=np.uint64)
+
+
+@pytest.mark.parametrize("islands", [1, 2])
+def test_ga_information_set_trial_matches_restatement(lib, ora, islands):
+    """One epoch with one information-set trial per island and nothing else
+    (no mutation, no pairwise or window improvement): each island base equals
+    the restated trial; the best island is reported."""
+    _say("ga information-set trial", islands)
+    cols = 2048
+    S, A, M = lib.gen_planted(0x5EED0008, 40, cols, 0.03)
+    p = lib.ga_default_params(islands=islands, population=4, max_ops=8, migrate_every=1,
+                              mutations=0, li_sweeps=0, check_every=1, learned=0, win_rows=0,
+                              isd_trials=1)
+    ga = lib.GpuGA(A, cols, p)
+    st = ga.run(1)
+    best, w = ga.best()
+    ga.close()
+    seed = (0x5EED0005 ^ 0xA0761D6478BD642F) & ((1 << 64) - 1)
+    exps = [_isd_reference(A, seed, i, cols) for i in range(islands)]
+    ws = [_weight(e) for e in exps]
+    k = ws.index(min(ws))
+    assert w == st.best_weight == ws[k] < _weight(A)
+    assert np.array_equal(best, exps[k])
+    assert st.isd_rows > 0
+    both = np.concatenate([A, best])
+    assert ora.rank(both, cols) == ora.rank(A, cols) == 40
+
+
+def test_ga_information_set_trials_recover_planted(lib):
+    """On a planted 64 x 2048 instance (Bernoulli(0.01) rows under a random
+    invertible 64 x 64 transform) information-set trials reach the planted
+    weight within a few epochs."""
+    _say("ga information-set planted")
+    S, A, M = lib.gen_planted(0x5EED0009, 64, 2048, 0.01)
+    planted = _weight(S)
+    p = lib.ga_default_params(islands=4, population=16, max_ops=16, migrate_every=2,
+                              check_every=2, learned=0, isd_trials=2)
+    ga = lib.GpuGA(A, 2048, p)
+    st = ga.run(40, target=int(planted * 1.01))
+    best, w = ga.best()
+    ga.close()
+    assert st.reached_target and w <= 1.01 * planted
+    assert _weight(best) == w
