@@ -394,11 +394,15 @@ __device__ void coset_block_phase2_pk16(uint32_t* s, const CosetMatrix& M, const
     __syncthreads();
 }
 
-// One workgroup per output slot (matrix, target).
+// One workgroup per output slot (matrix, target).  Each workgroup also
+// resets its key to ~0 after reading it, and workgroup 0 the work-queue
+// counter, so the next pass starts from the cleared state without fill
+// launches (the plan clears both once when it is built).
 __global__ __launch_bounds__(256) void coset_phase2_kernel(
     const CosetMatrix* __restrict__ mats, const int32_t* __restrict__ slot_matrix,
-    const uint32_t* __restrict__ pats, const unsigned long long* __restrict__ keys,
-    uint32_t* __restrict__ out_w, uint64_t* __restrict__ out_idx) {
+    const uint32_t* __restrict__ pats, unsigned long long* __restrict__ keys,
+    unsigned long long* __restrict__ counter, uint32_t* __restrict__ out_w,
+    uint64_t* __restrict__ out_idx) {
     constexpr int kWords = fwht_lds_words(kBatchMaxL) > pk_lds_words() ? fwht_lds_words(kBatchMaxL)
                                                                          : pk_lds_words();
     __shared__ __align__(16) int32_t s[kWords + 4];
@@ -407,6 +411,12 @@ __global__ __launch_bounds__(256) void coset_phase2_kernel(
     const CosetMatrix M = mats[slot_matrix[slot]];
     const int target = (int)(slot - M.out_off);
     const unsigned long long key = keys[slot];
+    __syncthreads();   // every thread has read the key before it is cleared
+    if (threadIdx.x == 0) {
+        keys[slot] = ~0ull;
+        if (slot == 0)
+            *counter = 0ull;
+    }
     const uint32_t* pat = pats + M.pat_off;
     switch (M.L) {
 #define BMM_CASE(LL)                                                                           \
@@ -520,8 +530,10 @@ void plan_upload(BatchPlan& P, int lmax) {
     int64_t* d_p = P.prefix.get<int64_t>(n + 1);
     int32_t* d_s = P.slot_matrix.get<int32_t>((size_t)P.n_slots);
     P.pats.get<uint32_t>((size_t)std::max<int64_t>(pat_off, 1));
-    P.keys.get<unsigned long long>((size_t)P.n_slots);
-    P.counter.get<unsigned long long>(1);
+    // Cleared once here; every pass's phase 2 leaves them cleared again.
+    fill_u64(reinterpret_cast<uint64_t*>(P.keys.get<unsigned long long>((size_t)P.n_slots)),
+             (size_t)P.n_slots, ~0ull, nullptr);
+    fill_u64(reinterpret_cast<uint64_t*>(P.counter.get<unsigned long long>(1)), 1, 0ull, nullptr);
     BMM_HIP(hipMemcpy(d_m, sorted.data(), sizeof(CosetMatrix) * n, hipMemcpyHostToDevice));
     BMM_HIP(hipMemcpy(d_p, prefix.data(), sizeof(int64_t) * (n + 1), hipMemcpyHostToDevice));
     BMM_HIP(hipMemcpy(d_s, slot_matrix.data(), sizeof(int32_t) * slot_matrix.size(),
@@ -535,8 +547,6 @@ void plan_enqueue(BatchPlan& P, hipStream_t s, bool timed) {
     unsigned long long* d_k = static_cast<unsigned long long*>(P.keys.ptr);
     unsigned long long* d_c = static_cast<unsigned long long*>(P.counter.ptr);
     int32_t* d_s = static_cast<int32_t*>(P.slot_matrix.ptr);
-    fill_u64(reinterpret_cast<uint64_t*>(d_k), (size_t)P.n_slots, ~0ull, s);
-    fill_u64(reinterpret_cast<uint64_t*>(d_c), 1, 0ull, s);
     hipLaunchKernelGGL(batch_patterns_kernel, dim3((unsigned)P.n_mat), dim3(256), 0, s, P.d_words,
                        d_m, d_pat);
     BMM_HIP(hipGetLastError());
@@ -554,7 +564,7 @@ void plan_enqueue(BatchPlan& P, hipStream_t s, bool timed) {
         BMM_HIP(hipGetLastError());
     }
     hipLaunchKernelGGL(coset_phase2_kernel, dim3((unsigned)P.n_slots), dim3(256), 0, s, d_m, d_s,
-                       d_pat, d_k, P.d_out_w, P.d_out_idx);
+                       d_pat, d_k, d_c, P.d_out_w, P.d_out_idx);
     BMM_HIP(hipGetLastError());
 }
 
