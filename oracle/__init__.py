@@ -326,6 +326,8 @@ class Reference:
                                     C.POINTER(C.c_size_t)] + ep
         L.ref_matrix_load.argtypes = [C.c_char_p, C.c_size_t, C.c_int, _u64p, C.c_size_t,
                                       _u64p] + ep
+        L.ref_mul_rank.argtypes = [_u64p, C.c_int, C.c_int, C.c_int, _u64p, C.c_int, C.c_int,
+                                   _u64p] + ep
 
     @staticmethod
     def _err():
@@ -447,6 +449,14 @@ class Reference:
     def scores_order(self, h, n):
         return _scores_order_call(self.lib.ref_scores_order_i32, self.lib.ref_scores_order_i16,
                                   h, n, self._run)
+
+    def mul_rank(self, a, a_cols, b, b_cols):
+        """Reference (a * b, rank(a))."""
+        a, b = self._rows(a), self._rows(b)
+        out = np.zeros((a.shape[0], b.shape[1]), np.uint64)
+        rank = self._run(self.lib.ref_mul_rank, a, a.shape[0], a.shape[1], a_cols, b, b.shape[1],
+                         b_cols, out)
+        return out, rank
 
     def matrix_io(self, m, cols):
         """Reference (hash hex, save_binary bytes, save_text str) of rows m."""

@@ -37,6 +37,9 @@
  *                      reference on random and rank-deficient windows;
  *                      the order against a sort of all masks by
  *                      (weight, inverse Gray index).
+ *   or_mul, or_rank    the live reference BitMatrix::mul / rank on random
+ *                      shapes (tests/test_oracle.py); identity and
+ *                      associativity; rank of random invertible products.
  *   generators         not reference behaviour (workload definitions); the
  *                      product generator must match them bit for bit, and
  *                      rank/inverse properties are checked directly (incl. a

@@ -360,4 +360,22 @@ int ref_matrix_load(const unsigned char* buf, std::size_t len, int kind, std::ui
     }
 }
 
+// BitMatrix::mul and rank: out = a (ar x ac) * b (ac x bc), rows of aw / bw
+// words; returns rank(a).
+int ref_mul_rank(const std::uint64_t* a, int ar, int aw, int ac, const std::uint64_t* b, int bw,
+                 int bc, std::uint64_t* out, char* err, int err_len) {
+    try {
+        auto A = load(a, ar, aw, ac);
+        auto B = load(b, ac, bw, bc);
+        const auto P = A.mul(B);
+        for (int r = 0; r < ar; ++r)
+            std::memcpy(out + static_cast<std::size_t>(r) * bw, P.row_words(r),
+                        sizeof(std::uint64_t) * bw);
+        return static_cast<int>(A.rank());
+    } catch (const std::exception& e) {
+        set_err(err, err_len, e.what());
+        return -1;
+    }
+}
+
 } // extern "C"

@@ -473,3 +473,25 @@ def test_global_greedy_rejects_out_of_range_masks(ora, ref):
             ora.global_greedy(m, cols, masks, [0])
         with pytest.raises(oracle.RefError, match="outside the supported row range"):
             ref.global_greedy(m, cols, masks, [0])
+
+
+def test_mul_and_rank_match_live_reference(ora, ref):
+    """or_mul / or_rank (the checkers of the GPU row combination) against the
+    reference BitMatrix::mul and rank, plus identity and associativity."""
+    for ar, ac, bc, p in ((1, 1, 1, 0.5), (7, 70, 130, 0.3), (64, 64, 64, 0.5), (33, 200, 65, 0.1)):
+        a = ora.bernoulli(700 + ar, ar, ac, p)
+        b = ora.bernoulli(800 + bc, ac, bc, p)
+        got = ora.mul(a, ac, b, bc)
+        exp, rank = ref.mul_rank(a, ac, b, bc)
+        assert np.array_equal(got, exp)
+        assert ora.rank(a, ac) == rank
+    n = 40
+    x = ora.bernoulli(1, n, n, 0.5)
+    y = ora.bernoulli(2, n, n, 0.5)
+    z = ora.bernoulli(3, n, 90, 0.5)
+    eye = np.zeros((n, 1), np.uint64)
+    for i in range(n):
+        eye[i, 0] = np.uint64(1 << i)
+    assert np.array_equal(ora.mul(eye, n, z, 90), z)
+    assert np.array_equal(ora.mul(ora.mul(x, n, y, n), n, z, 90),
+                          ora.mul(x, n, ora.mul(y, n, z, 90), 90))
