@@ -195,13 +195,19 @@ __device__ inline void win_add_all(char* sb, const WinTarget& T, const WinRegs& 
     }
 }
 
+// Bias-filled block plus the signed entries.  `after_clear` runs once every
+// thread is past the clear barrier (so past all reads of the previous
+// block's shared state).
+template <typename AfterClear>
 __device__ inline void win_build_pk16(uint32_t* s, const WinTarget& T, const WinRegs& R,
-                                      const WinPat8* __restrict__ w8, uint64_t mh) {
+                                      const WinPat8* __restrict__ w8, uint64_t mh,
+                                      AfterClear&& after_clear) {
     int4* s4 = reinterpret_cast<int4*>(s);
     const int b = (int)kPkBias;
     for (int i = threadIdx.x; i < (pk_lds_words() + 3) / 4; i += blockDim.x)
         s4[i] = make_int4(b, b, b, b);
     __syncthreads();
+    after_clear();
     char* sb = reinterpret_cast<char*>(s);
     if (T.top)
         win_add_all<true>(sb, T, R, w8, (uint32_t)mh, (uint32_t)(mh >> 32) & 1u);
@@ -225,8 +231,14 @@ __device__ void win_block_pk16(uint32_t* s, const WinTarget& T, const WinRegs& R
                                const WinPat8* __restrict__ w8, uint64_t mh,
                                unsigned long long* key) {
     __shared__ int32_t wbest[4];
-    __shared__ int exact;
-    win_build_pk16(s, T, R, w8, mh);
+    __shared__ unsigned long long s_key;
+    // The target's key is read (relaxed) right after the clear barrier and
+    // only used after the transform, so the load's latency overlaps the build.
+    unsigned long long key_seen = 0;
+    win_build_pk16(s, T, R, w8, mh, [&] {
+        if (threadIdx.x == 0)
+            key_seen = __atomic_load_n(key, __ATOMIC_RELAXED);
+    });
     {
         s16x2 m = pk_as(0x80008000u);
         pk_block_fwht_visit(s, [&](int, const uint32_t (&v)[8]) {
@@ -237,19 +249,17 @@ __device__ void win_block_pk16(uint32_t* s, const WinTarget& T, const WinRegs& R
         const int32_t h = wave_max_i32(max(pk_lo(m), pk_hi(m)));
         if (threadIdx.x % kWave == 0)
             wbest[threadIdx.x / kWave] = h;
+        if (threadIdx.x == 0)
+            s_key = key_seen;
         __syncthreads();
-        if (threadIdx.x == 0) {
-            const int32_t hb = max(max(wbest[0], wbest[1]), max(wbest[2], wbest[3]));
-            const uint32_t w = (uint32_t)(T.total - hb) >> 1;
-            const unsigned long long floor_key =
-                ((unsigned long long)w << T.bits) | (mh << kPkL);
-            exact = floor_key < __atomic_load_n(key, __ATOMIC_RELAXED) ? 1 : 0;
-        }
-        __syncthreads();
-        if (!exact)
+        // Every thread takes the same decision from the same shared values.
+        const int32_t hb = max(max(wbest[0], wbest[1]), max(wbest[2], wbest[3]));
+        const uint32_t w = (uint32_t)(T.total - hb) >> 1;
+        const unsigned long long floor_key = ((unsigned long long)w << T.bits) | (mh << kPkL);
+        if (!(floor_key < s_key))
             return;
     }
-    win_build_pk16(s, T, R, w8, mh);
+    win_build_pk16(s, T, R, w8, mh, [] {});
     int32_t best = INT32_MIN;
     pk_block_fwht_visit(s, [&](int q, const uint32_t (&v)[8]) {
 #pragma unroll
