@@ -822,11 +822,27 @@ __global__ __launch_bounds__(256) void ga_isd_elim_kernel(const uint64_t* __rest
             for (int q = 0; q < kIsdMaskWords; ++q)
                 t[q] ^= piv_t[q];
         }
+        // Rows with the bit ^= pivot row.  Eight rows per batch: all loads
+        // of a batch are issued before its stores, so they are in flight
+        // together (the rows of a batch are distinct, none is the pivot).
         const uint64_t* P = Rb + (size_t)r * words;
-        for (int r2 = sub; r2 < rows; r2 += groups)
-            if (hasbit[r2] && r2 != r)
-                for (int w = w0; w < words; w += tpr)
-                    Rb[(size_t)r2 * words + w] ^= P[w];
+        for (int w = w0; w < words; w += tpr) {
+            const uint64_t pw = P[w];
+            for (int r0 = sub; r0 < rows; r0 += 8 * groups) {
+                uint64_t x[8];
+                bool on[8];
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    const int r2 = r0 + j * groups;
+                    on[j] = r2 < rows && hasbit[r2] && r2 != r;
+                    x[j] = on[j] ? Rb[(size_t)r2 * words + w] : 0ull;
+                }
+#pragma unroll
+                for (int j = 0; j < 8; ++j)
+                    if (on[j])
+                        Rb[(size_t)(r0 + j * groups) * words + w] = x[j] ^ pw;
+            }
+        }
         __syncthreads();
         ++npiv;
     }

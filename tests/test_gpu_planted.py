@@ -85,3 +85,39 @@ def test_window_queues_mixed_in_one_call(lib, ora):
         assert (int(w[i]), int(m[i])) == _oracle_window(ora, A, 40000, t, windows[i]), t
     w3, m3 = lib.window_coset_min(A, 40000, [5], [windows[2][:10]])
     assert (int(w3[0]), int(m3[0])) == _oracle_window(ora, A, 40000, 5, windows[2][:10])
+
+
+@pytest.mark.parametrize("width", [15, 16])
+def test_window_coset_memory_lists_vs_oracle(lib, ora, width):
+    """6144 active columns over 16-17 row windows: each half of the packed
+    pattern list holds more than 2560 entries, so the blocks read their
+    entries from memory instead of registers."""
+    _say("window coset memory lists", width)
+    m = lib.gen_bernoulli(4242 + width, 24, 6144, 0.5)
+    targets = [0, 23]
+    windows = [[r for r in range(24) if r != t][:width] for t in targets]
+    w, mk = lib.window_coset_min(m, 6144, targets, windows)
+    for i, t in enumerate(targets):
+        assert (int(w[i]), int(mk[i])) == _oracle_window(ora, m, 6144, t, windows[i]), t
+
+
+def test_window_coset_47_rows_top_bit(lib):
+    """A 47-row window (33 high bits per pattern: the top one travels in the
+    entry's spare address bit): the target is planted as the XOR of ten
+    window rows, including the last one (mask bit 46), plus three flipped
+    bits, so the unique minimum is 3 at exactly that mask.  2^47 subsets
+    (about 45 s)."""
+    _say("window coset 47 rows")
+    rng = np.random.default_rng(47)
+    m = lib.gen_bernoulli(4747, 48, 4096, 0.5)
+    window = list(range(1, 48))
+    pick = [2, 5, 11, 17, 23, 30, 36, 41, 44, 46]   # window positions
+    t = np.zeros(m.shape[1], np.uint64)
+    for k in pick:
+        t ^= m[window[k]]
+    for c in rng.choice(4096, 3, replace=False):
+        t[c // 64] ^= np.uint64(1 << (int(c) % 64))
+    m[0] = t
+    w, mk = lib.window_coset_min(m, 4096, [0], [window])
+    assert int(w[0]) == 3
+    assert int(mk[0]) == sum(1 << k for k in pick)
