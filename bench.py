@@ -345,12 +345,12 @@ def run_basis(torch, steps: int, warmup: int) -> dict:
     # last-level cache, so it is not an HBM measurement.  The array FWHT's
     # HBM roofline is taken on 2^28 int32 (1 GiB).  Algorithmic bytes: 16 per
     # element (two out-of-LDS passes, each one read and one write of 4
-    # bytes).  The kernels move 8 bytes per element per pass: 13 index bits
-    # in one LDS pass, then 15 in three register passes (5 + 5 + 5).
+    # bytes).  The kernels move 8 bytes per element per pass: 14 index bits
+    # in one LDS pass, then 14 in two register passes (7 + 7).
     big = torch.zeros(1 << FWHT_HBM_BITS, dtype=torch.int32, device="cuda:0")
     stream = torch.cuda.current_stream()
     sp = stream.cuda_stream
-    fwht_passes = 1 + -(-(FWHT_HBM_BITS - 13) // 6)
+    fwht_passes = 1 + -(-(FWHT_HBM_BITS - 14) // 7)
     fwht_bytes = 16.0 * (1 << FWHT_HBM_BITS)
     fwht_ms = _time(torch, stream,
                     lambda: _lib.call("bmm_hip_fwht_i32_dev", big.data_ptr(), big.numel(), sp),
@@ -367,8 +367,8 @@ def run_basis(torch, steps: int, warmup: int) -> dict:
                      "frac": achieved / 8000.0,
                      "traffic": fwht_passes * 8.0 * (1 << FWHT_HBM_BITS),
                      "bytes_per_launch": fwht_bytes,
-                     "kernel": f"array FWHT of 2^{FWHT_HBM_BITS} int32: fwht_chunk_kernel<int,13> "
-                               f"+ {fwht_passes - 1} x fwht_stride_kernel<int,5>",
+                     "kernel": f"array FWHT of 2^{FWHT_HBM_BITS} int32: fwht_chunk_kernel<int,14> "
+                               f"+ {fwht_passes - 1} x fwht_stride_buf_kernel<int,7>",
                      "kernel_avg_ms": fwht_ms,
                      "solve_fwht_2^24_avg_ms": kernel_avg_ms, "solve_fwht_launches": nk,
                      "solve_ms": 1e3 * dt},

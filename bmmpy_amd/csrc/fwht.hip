@@ -2,8 +2,8 @@
 //
 // 1. fwht_chunk_kernel / fwht_stride_kernel: the stand-alone in-place
 //    transform of a device array (fwht_inplace, src/math/fwht.cpp:28-57):
-//    one LDS pass over the low 13 index bits, then register passes of 5-6
-//    bits with coalesced 256-byte wavefront accesses.
+//    one LDS pass over the low 14 index bits, then register passes of up to
+//    7 bits with coalesced 256-byte wavefront accesses.
 //
 // 2. column_patterns_kernel: per-column row patterns of a small matrix.
 //
@@ -24,13 +24,15 @@ namespace bmmhip {
 // fwht_inplace on a device array of 2^nb elements, wrapping arithmetic: the
 // radix-2 stages of the different index bits commute, so the transform is
 // applied as
-//   * one LDS pass over index bits [0, min(nb, 13)): each workgroup owns a
-//     contiguous chunk, loaded and stored as 16-byte vectors, transformed by
-//     block_fwht (radix-32 register rounds between LDS exchanges);
-//   * register passes over the remaining bits, 5 or 6 at a time: lane c of a
-//     wavefront owns column c of a 2^R x 64 tile at row stride 2^lo, so each
-//     of its 2^R loads and stores is one coalesced 256-byte wavefront access,
-//     and the R stages run in registers with no LDS or barriers.
+//   * one LDS pass over index bits [0, min(nb, 14)): each workgroup owns a
+//     contiguous chunk of 64 KiB (int32), loaded and stored as 16-byte
+//     vectors, transformed by block_fwht (radix-32 register rounds between
+//     LDS exchanges);
+//   * register passes over the remaining bits, at most 7 at a time, split
+//     evenly: lane c of a wavefront owns column c of a 2^R x 64 tile at row
+//     stride 2^lo, so each of its 2^R loads and stores is one coalesced
+//     256-byte wavefront access, and the R stages run in registers with no
+//     LDS or barriers.  2^28 int32 takes three passes (14 + 7 + 7 bits).
 // Each pass reads and writes the array once (8 bytes per int32 element).
 // int16 arrays are computed in 32-bit lanes and truncated on store: the
 // transform is a ring homomorphism, so the result equals int16 wrapping.
@@ -38,10 +40,13 @@ template <typename T> struct Unsigned;
 template <> struct Unsigned<int32_t> { using type = uint32_t; };
 template <> struct Unsigned<int16_t> { using type = uint16_t; };
 
-constexpr int kArrBlockBits = 13;
+constexpr int kArrBlockBits = 14;
+constexpr int kArrMaxR = 7;        // register bits per stride pass (2^7 values per lane)
+
+constexpr int kChunkThreads = 512;   // 16 waves per CU at two 64 KiB chunks per CU
 
 template <typename T, int L>
-__global__ __launch_bounds__(256) void fwht_chunk_kernel(T* __restrict__ data, int vec) {
+__global__ __launch_bounds__(kChunkThreads) void fwht_chunk_kernel(T* __restrict__ data, int vec) {
     __shared__ __align__(16) int32_t s[fwht_lds_words(L) + 4];
     constexpr int N = 1 << L;
     T* chunk = data + (size_t)blockIdx.x * N;
@@ -74,7 +79,7 @@ __global__ __launch_bounds__(256) void fwht_chunk_kernel(T* __restrict__ data, i
     }
 }
 
-// Index bits [lo, lo + R) with lo >= 13: thread (tile, c) transforms the 2^R
+// Index bits [lo, lo + R) with lo >= 14: thread (tile, c) transforms the 2^R
 // elements h 2^(lo+R) + e 2^lo + c, e < 2^R; 256 consecutive columns per
 // workgroup.
 template <typename T, int R>
@@ -102,17 +107,67 @@ __global__ __launch_bounds__(256) void fwht_stride_kernel(T* __restrict__ data, 
         p[(size_t)e << lo] = (T)(int32_t)v[e];
 }
 
+// The same pass through buffer loads and stores (arrays of at most 2 GiB):
+// the per-element stride goes in the scalar offset, so a lane keeps one
+// 32-bit base offset besides its 2^R values (the global-pointer form keeps a
+// 64-bit address per value, which caps the occupancy).
+template <typename T> __device__ inline uint32_t buf_load(__amdgpu_buffer_rsrc_t rs, uint32_t v,
+                                                          uint32_t s) {
+    if constexpr (sizeof(T) == 4)
+        return (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(rs, (int)v, (int)s, 0);
+    else
+        return (uint32_t)(int32_t)(int16_t)__builtin_amdgcn_raw_buffer_load_b16(rs, (int)v, (int)s,
+                                                                                0);
+}
+
+template <typename T> __device__ inline void buf_store(uint32_t x, __amdgpu_buffer_rsrc_t rs,
+                                                       uint32_t v, uint32_t s) {
+    if constexpr (sizeof(T) == 4)
+        __builtin_amdgcn_raw_buffer_store_b32(x, rs, (int)v, (int)s, 0);
+    else
+        __builtin_amdgcn_raw_buffer_store_b16((unsigned short)x, rs, (int)v, (int)s, 0);
+}
+
+template <typename T, int R>
+__global__ __launch_bounds__(256) void fwht_stride_buf_kernel(T* __restrict__ data,
+                                                              uint32_t bytes, int lo) {
+    const __amdgpu_buffer_rsrc_t rs =
+        __builtin_amdgcn_make_buffer_rsrc(data, 0, (int)bytes, 0x00020000);
+    const uint32_t col = blockIdx.x * 256u + threadIdx.x;   // over (h, c)
+    const uint32_t h = col >> lo, c = col & ((1u << lo) - 1u);
+    const uint32_t vbase = ((h << (lo + R)) + c) * (uint32_t)sizeof(T);
+    const uint32_t stride = (1u << lo) * (uint32_t)sizeof(T);
+    constexpr int E = 1 << R;
+    uint32_t v[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e)
+        v[e] = buf_load<T>(rs, vbase, (uint32_t)e * stride);
+#pragma unroll
+    for (int len = 1; len < E; len <<= 1)
+#pragma unroll
+        for (int i = 0; i < E; ++i)
+            if ((i & len) == 0) {
+                const uint32_t a = v[i], b = v[i + len];
+                v[i] = a + b;
+                v[i + len] = a - b;
+            }
+#pragma unroll
+    for (int e = 0; e < E; ++e)
+        buf_store<T>(v[e], rs, vbase, (uint32_t)e * stride);
+}
+
 template <typename T> void launch_chunk_pass(T* d, size_t n, int L, hipStream_t stream) {
     const unsigned grid = (unsigned)(n >> L);
     const int vec = ((uintptr_t)d % 16) == 0;   // 16-byte vectors only on aligned arrays
     switch (L) {
 #define BMM_L(LL)                                                                              \
     case LL:                                                                                   \
-        hipLaunchKernelGGL((fwht_chunk_kernel<T, LL>), dim3(grid), dim3(256), 0, stream, d,   \
+        hipLaunchKernelGGL((fwht_chunk_kernel<T, LL>), dim3(grid), dim3(kChunkThreads), 0,     \
+                           stream, d,                                                          \
                            vec);                                                               \
         break;
         BMM_L(1) BMM_L(2) BMM_L(3) BMM_L(4) BMM_L(5) BMM_L(6) BMM_L(7) BMM_L(8) BMM_L(9)
-        BMM_L(10) BMM_L(11) BMM_L(12) BMM_L(13)
+        BMM_L(10) BMM_L(11) BMM_L(12) BMM_L(13) BMM_L(14)
 #undef BMM_L
     default: fail(BMM_ERR_INVALID, "fwht_inplace: bad chunk size");
     }
@@ -129,22 +184,36 @@ template <typename T> void fwht_array_dev(T* d, size_t n, hipStream_t stream) {
         ++nb;
     const int L = std::min(nb, kArrBlockBits);
     launch_chunk_pass(d, n, L, stream);
-    // Remaining bits in the fewest register passes of at most 6 bits, split
-    // evenly (11 bits: 6 + 5).
+    // Remaining bits in the fewest register passes of at most kArrMaxR bits,
+    // split evenly (10 bits: 5 + 5; 14 bits: 7 + 7).
     int lo = L;
     while (lo < nb) {
         const int rem = nb - lo;
-        const int passes = (rem + 5) / 6;
+        const int passes = (rem + kArrMaxR - 1) / kArrMaxR;
         const int R = (rem + passes - 1) / passes;
         const unsigned grid = (unsigned)((n >> R) / 256);
-        switch (R) {
+        if (n * sizeof(T) <= ((size_t)1 << 31)) {
+            const uint32_t bytes = (uint32_t)(n * sizeof(T));
+            switch (R) {
+#define BMM_R(RR)                                                                              \
+    case RR:                                                                                   \
+        hipLaunchKernelGGL((fwht_stride_buf_kernel<T, RR>), dim3(grid), dim3(256), 0, stream, d, \
+                           bytes, lo);                                                         \
+        break;
+                BMM_R(1) BMM_R(2) BMM_R(3) BMM_R(4) BMM_R(5) BMM_R(6) BMM_R(7)
+#undef BMM_R
+            default: fail(BMM_ERR_INVALID, "fwht_inplace: bad stride pass");
+            }
+        } else {
+            switch (R) {
 #define BMM_R(RR)                                                                              \
     case RR:                                                                                   \
         hipLaunchKernelGGL((fwht_stride_kernel<T, RR>), dim3(grid), dim3(256), 0, stream, d, lo); \
         break;
-            BMM_R(1) BMM_R(2) BMM_R(3) BMM_R(4) BMM_R(5) BMM_R(6)
+                BMM_R(1) BMM_R(2) BMM_R(3) BMM_R(4) BMM_R(5) BMM_R(6) BMM_R(7)
 #undef BMM_R
-        default: fail(BMM_ERR_INVALID, "fwht_inplace: bad stride pass");
+            default: fail(BMM_ERR_INVALID, "fwht_inplace: bad stride pass");
+            }
         }
         BMM_HIP(hipGetLastError());
         lo += R;

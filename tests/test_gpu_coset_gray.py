@@ -18,7 +18,7 @@ def test_fwht_kats(lib):
     assert lib.fwht_i16([32767, 1]).tolist() == [-32768, 32766]
 
 
-@pytest.mark.parametrize("n_bits", [1, 5, 12, 13, 14, 18, 19, 21, 24, 25])
+@pytest.mark.parametrize("n_bits", [1, 5, 12, 13, 14, 15, 18, 19, 21, 22, 24, 25, 26])
 def test_fwht_i32_matches_oracle(lib, ora, n_bits):
     _say("fwht_i32", n_bits)
     rng = np.random.default_rng(n_bits)
@@ -26,7 +26,7 @@ def test_fwht_i32_matches_oracle(lib, ora, n_bits):
     assert np.array_equal(lib.fwht_i32(x), ora.fwht_i32(x))
 
 
-@pytest.mark.parametrize("n_bits", [3, 11, 12, 16, 19, 24])
+@pytest.mark.parametrize("n_bits", [3, 11, 12, 14, 16, 19, 21, 24])
 def test_fwht_i16_wraps_like_oracle(lib, ora, n_bits):
     _say("fwht_i16", n_bits)
     rng = np.random.default_rng(100 + n_bits)
