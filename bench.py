@@ -356,7 +356,11 @@ def run_basis(torch, steps: int, warmup: int) -> dict:
                     lambda: _lib.call("bmm_hip_fwht_i32_dev", big.data_ptr(), big.numel(), sp),
                     10)
     achieved = fwht_bytes / (fwht_ms / 1e3) / 1e9
-    del big
+    # Practical ceiling: a plain device copy of the same 1 GiB (read + write).
+    dst = torch.empty_like(big)
+    copy_ms = _time(torch, stream, lambda: dst.copy_(big), 10)
+    copy_gbps = 2.0 * 4 * (1 << FWHT_HBM_BITS) / (copy_ms / 1e3) / 1e9
+    del big, dst
     return {
         "ms": 1e3 * dt, "evals": float(n - 1),
         "results": [{"rank": int(res.rank), "total_weight": int(res.total_weight),
@@ -371,6 +375,7 @@ def run_basis(torch, steps: int, warmup: int) -> dict:
                                f"+ {fwht_passes - 1} x fwht_stride_buf_kernel<int,7>",
                      "kernel_avg_ms": fwht_ms,
                      "solve_fwht_2^24_avg_ms": kernel_avg_ms, "solve_fwht_launches": nk,
+                     "copy_1gib_gbps": copy_gbps,
                      "solve_ms": 1e3 * dt},
     }
 

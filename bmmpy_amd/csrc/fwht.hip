@@ -43,7 +43,7 @@ template <> struct Unsigned<int16_t> { using type = uint16_t; };
 constexpr int kArrBlockBits = 14;
 constexpr int kArrMaxR = 7;        // register bits per stride pass (2^7 values per lane)
 
-constexpr int kChunkThreads = 512;   // 16 waves per CU at two 64 KiB chunks per CU
+constexpr int kChunkThreads = 1024;   // 32 waves per CU at two 64 KiB chunks per CU
 
 template <typename T, int L>
 __global__ __launch_bounds__(kChunkThreads) void fwht_chunk_kernel(T* __restrict__ data, int vec) {
