@@ -1,7 +1,8 @@
 // Walsh-Hadamard kernels on gfx950.
 //
-// 1. fwht_chunk_kernel / fwht_stride_kernel: the stand-alone in-place
-//    transform of a device array (fwht_inplace, src/math/fwht.cpp:28-57):
+// 1. fwht_chunk_kernel / fwht_stride_buf_kernel (fwht_stride_kernel for
+//    arrays over 2 GiB): the stand-alone in-place transform of a device
+//    array (fwht_inplace, src/math/fwht.cpp:28-57):
 //    one LDS pass over the low 14 index bits, then register passes of up to
 //    7 bits with coalesced 256-byte wavefront accesses.
 //
