@@ -691,8 +691,8 @@ int bmm_hip_weight_order(const uint64_t* row_words, size_t t, size_t words, size
         if (!n_out || (t > 0 && words > 0 && !row_words) || (k > 0 && (!out_mask || !out_weight)))
             fail(BMM_ERR_INVALID, "weight_order: null pointer");
         *n_out = 0;
-        if (t > 28)
-            fail(BMM_ERR_INVALID, "weight_order: at most 28 rows");
+        if (t > 32)
+            fail(BMM_ERR_INVALID, "weight_order: at most 32 rows");
         if (cols > words * 64)
             fail(BMM_ERR_INVALID, "weight_order: cols exceeds words_per_row * 64");
         if (cols >= (size_t)1 << 31)

@@ -186,14 +186,18 @@ template <typename T> void fwht_array_dev(T* d, size_t n, hipStream_t stream) {
     const int L = std::min(nb, kArrBlockBits);
     launch_chunk_pass(d, n, L, stream);
     // Remaining bits in the fewest register passes of at most kArrMaxR bits,
-    // split evenly (10 bits: 5 + 5; 14 bits: 7 + 7).
+    // split evenly (10 bits: 5 + 5; 14 bits: 7 + 7).  Arrays over 2 GiB take
+    // the global-pointer passes, at most 5 bits each (a 64-bit address per
+    // value otherwise exhausts the registers).
+    const bool buf = n * sizeof(T) <= ((size_t)1 << 31);
+    const int max_r = buf ? kArrMaxR : 5;
     int lo = L;
     while (lo < nb) {
         const int rem = nb - lo;
-        const int passes = (rem + kArrMaxR - 1) / kArrMaxR;
+        const int passes = (rem + max_r - 1) / max_r;
         const int R = (rem + passes - 1) / passes;
         const unsigned grid = (unsigned)((n >> R) / 256);
-        if (n * sizeof(T) <= ((size_t)1 << 31)) {
+        if (buf) {
             const uint32_t bytes = (uint32_t)(n * sizeof(T));
             switch (R) {
 #define BMM_R(RR)                                                                              \

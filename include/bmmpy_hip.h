@@ -205,7 +205,7 @@ int bmm_hip_scores_order_i16(const int16_t *h, size_t m, int32_t n, int16_t *s_b
 
 /* ---- candidate stream of ExactBasisSolver (src/solver/exact_basis_solver.cpp:
  * 29-124): the first min(k, 2^t - 1) nonzero masks of a t-row window
- * (t <= 28) in the order (weight, Gray index), i.e. the reference's Gray
+ * (t <= 32) in the order (weight, Gray index), i.e. the reference's Gray
  * enumeration counting-sorted by weight; mask bit r = row r.
  * *n_out = number written. */
 int bmm_hip_weight_order(const uint64_t *row_words, size_t t, size_t words_per_row,

@@ -259,3 +259,26 @@ def test_batch_int16_blocks_refuse_wide_matrices(lib):
             lib.call("bmm_hip_batch_plan_set_shape", plan, 11, 0, 0)
     finally:
         lib.call("bmm_hip_batch_plan_destroy", plan)
+
+
+@pytest.mark.parametrize("n_bits", [29, 30])
+def test_fwht_large_sparse_closed_form(lib, n_bits):
+    """2 GiB (buffer-load passes) and 4 GiB (global-pointer passes) int32
+    arrays: a few spikes x_p = v transform to H[m] = sum_p v (-1)^|p & m|,
+    checked on 65536 random indices."""
+    _say("fwht large", n_bits)
+    rng = np.random.default_rng(900 + n_bits)
+    n = 1 << n_bits
+    pos = rng.choice(n, 5, replace=False).astype(np.int64)
+    val = rng.integers(-1000, 1000, 5).astype(np.int64)
+    x = np.zeros(n, np.int32)
+    x[pos] = val
+    h = lib.fwht_i32(x)
+    del x
+    m = rng.integers(0, n, 65536, dtype=np.int64)
+    exp = np.zeros(m.size, np.int64)
+    for p, v in zip(pos, val):
+        par = np.bitwise_count((m & p).astype(np.uint64)) & 1
+        exp += np.where(par == 1, -v, v)
+    assert np.array_equal(h[m].astype(np.int64), exp)
+    assert int(h[0]) == int(val.sum())

@@ -62,3 +62,41 @@ def test_exact_basis_matches_oracle(lib, ora, t, cols, p, mode):
             assert res.transform_matrix[q, r] == bool((int(mask) >> r) & 1)
     if mode == "kat":
         assert res.rank == 2 and res.total_weight == 3 and sorted(res.basis_weights) == [1, 2]
+
+
+def _gray_inverse(g):
+    s = 0
+    while g:
+        s ^= g
+        g >>= 1
+    return s
+
+
+@pytest.mark.parametrize("t", [29, 30])
+def test_weight_order_29_30_rows_properties(lib, ora, t):
+    """Windows past the oracle's enumeration range (2^29 and 2^30 states; the
+    2^30 transform takes the global-pointer passes): every returned weight is
+    the weight of its row combination, the stream is ordered by (weight, Gray
+    index), and no sampled mask lighter than the last returned weight is
+    missing from it."""
+    _say("weight order", t)
+    cols, k = 128, 3000
+    m = ora.bernoulli(7300 + t, t, cols, 0.3)
+    gm, gw = lib.weight_order(m, cols, k)
+    assert gm.size == k and len(set(gm.tolist())) == k and 0 not in set(gm.tolist())
+
+    def combo_weights(masks):
+        acc = np.zeros((masks.size, m.shape[1]), np.uint64)
+        for r in range(t):
+            sel = ((masks >> r) & 1).astype(bool)
+            acc[sel] ^= m[r]
+        return np.bitwise_count(acc).sum(axis=1)
+
+    assert np.array_equal(combo_weights(gm.astype(np.int64)), gw.astype(np.int64))
+    keys = [(int(w), _gray_inverse(int(x))) for x, w in zip(gm, gw)]
+    assert keys == sorted(keys)
+    rng = np.random.default_rng(t)
+    sample = rng.integers(1, 1 << t, 20000, dtype=np.int64)
+    sw = combo_weights(sample)
+    lighter = set(sample[sw < int(gw[-1])].tolist())
+    assert lighter <= set(gm.tolist())
