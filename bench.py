@@ -20,6 +20,13 @@ Other workloads (one JSON line each, same fields):
   --workload basis   ExactBasisSolver on a 24 x 1024 Bernoulli(0.3) window
                      (seed 0x5EED0007): all 2^24 - 1 row combinations weighed,
                      minimum-total-weight basis; a step is one solve.
+  --workload scores  calc_scores_and_order on 2^28 int32 entries uniform in
+                     [-1023, 1023] (seed 0x5EED0010), n = 1023: scores,
+                     histogram and stable order of 2^28 - 1 masks; a step is
+                     one call from host arrays, the roofline on the kernels.
+  --workload sa      SASelector's pairwise synergy table (pair_overlaps) on
+                     4096 x 8192 Bernoulli(0.3) (seed 0x5EED0011); value in
+                     row pairs/s.
   --workload ga      plain vs learned island GA on the planted 256 x 8192.
 
 A step is one full pass of the workload with the inputs already resident in
@@ -403,6 +410,130 @@ def basis_cpu_baseline(budget_s: float = 15.0) -> dict:
                       f"2^24 - 1 masks each)"}
 
 
+SCORES_BITS, SCORES_N, SCORES_SEED = 28, 1023, 0x5EED0010
+SA_ROWS, SA_COLS, SA_P, SA_SEED = 4096, 8192, 0.3, 0x5EED0011
+
+
+def run_scores(torch, steps: int, warmup: int) -> dict:
+    """calc_scores_and_order on a spectrum of 2^28 int32 entries (scores in
+    [0, 1023], the range of a 1023-column window): scores, score histogram
+    and the stable order of masks 1 .. m-1.  A step is one call; the roofline
+    is taken on the kernels' time.  Algorithmic bytes: read h, write the
+    scores and the order, 12 B per mask."""
+    from bmmpy_amd import _lib
+
+    rng = np.random.default_rng(SCORES_SEED)
+    m = 1 << SCORES_BITS
+    h = rng.integers(-SCORES_N, SCORES_N + 1, m, dtype=np.int32)
+    for _ in range(warmup):
+        _lib.calc_scores_and_order(h, SCORES_N)
+    _lib.timing_begin()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        s, order, cnt, off = _lib.calc_scores_and_order(h, SCORES_N)
+    dt = (time.perf_counter() - t0) / steps
+    kms, nk = _lib.timing_end()
+    kernel_ms = kms / max(steps, 1)   # the scores kernel and the radix passes of one call
+    alg = 12.0 * m
+    achieved = alg / (kernel_ms / 1e3) / 1e9
+    # Moved by the kernels (10-bit scores: two 5-bit digit passes): scores
+    # (read h, write s: 8 B, the first digit histogram on the side), first
+    # scatter (read the scores, write keys and masks: 12 B), second pass's
+    # histogram (read keys: 4 B) and scatter (read keys and masks, write the
+    # order: 12 B).
+    moved = (8.0 + 12 + 4 + 12) * m
+    return {
+        "ms": 1e3 * dt, "evals": float(m - 1),
+        "results": [{"cnt_head": [int(x) for x in cnt[:4]], "order_head": [int(x) for x in order[:4]],
+                     "off_last": int(off[-1])}],
+        "config": {"workload": "scores_order_2^28", "entries": m, "n": SCORES_N,
+                   "seed": hex(SCORES_SEED), "subset_evals_per_step": m - 1},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": 8000.0, "unit": "GB/s",
+                     "frac": achieved / 8000.0, "traffic": moved, "bytes_per_launch": alg,
+                     "kernel": "scores_kernel + radix_scatter_kernel + radix_hist_kernel "
+                               "+ radix_scatter_kernel (+ digit-table scans)",
+                     "kernel_avg_ms": kernel_ms, "host_call_ms": 1e3 * dt},
+    }
+
+
+def scores_cpu_baseline(budget_s: float = 10.0) -> dict:
+    """The reference calc_scores_and_order on 2^24-entry samples."""
+    import oracle  # checker / baseline only
+
+    ref = oracle.reference()
+    rng = np.random.default_rng(SCORES_SEED)
+    h = rng.integers(-SCORES_N, SCORES_N + 1, 1 << 24, dtype=np.int32)
+    done = 0
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < budget_s:
+        if ref is not None:
+            ref.scores_order(h, SCORES_N)
+        else:
+            oracle.oracle().scores_order(h, SCORES_N)
+        done += 1
+    dt = time.perf_counter() - t0
+    return {"value": done * float((1 << 24) - 1) / dt, "unit": "subset evals/s", "cores": 1,
+            "kind": "reference" if ref is not None else "port",
+            "sample": f"calc_scores_and_order on 2^24 entries, {done} calls"}
+
+
+def run_sa(torch, steps: int, warmup: int) -> dict:
+    """SASelector's pairwise synergy table on 4096 rows x 8192 columns
+    (Bernoulli(0.3)): popcount(r_i & r_j) for every pair.  Roofline: 4 int32
+    VALU ops per pair and 64-bit word (AND of two halves, two bit counts),
+    over the upper triangle."""
+    from bmmpy_amd import _lib
+
+    m = _lib.gen_bernoulli(SA_SEED, SA_ROWS, SA_COLS, SA_P)
+    for _ in range(warmup):
+        _lib.pair_overlaps(m)
+    _lib.timing_begin()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        out = _lib.pair_overlaps(m)
+    dt = (time.perf_counter() - t0) / steps
+    kms, nk = _lib.timing_end()
+    kernel_ms = kms / max(nk, 1)
+    pairs = SA_ROWS * (SA_ROWS + 1) / 2
+    ops = pairs * (SA_COLS // 64) * 4
+    achieved = ops / (kernel_ms / 1e3) / 1e12
+    return {
+        "ms": 1e3 * dt, "evals": pairs,
+        "results": [{"overlap_0_1": int(out[0, 1]), "trace": int(np.trace(out))}],
+        "config": {"workload": "pair_overlaps_4096x8192", "rows": SA_ROWS, "cols": SA_COLS,
+                   "p": SA_P, "seed": hex(SA_SEED)},
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": 39.3216,
+                     "unit": "TOPS (int32 VALU)", "frac": achieved / 39.3216, "traffic": None,
+                     "kernel": "overlap_kernel", "kernel_avg_ms": kernel_ms,
+                     "host_call_ms": 1e3 * dt},
+    }
+
+
+def sa_cpu_baseline(budget_s: float = 10.0) -> dict:
+    """The reference SASelector (scalar build), one restart of one iteration
+    on 1024 x 8192 samples: its time is the pairwise synergy table."""
+    import oracle  # checker / baseline only
+    from bmmpy_amd import _lib
+
+    ref = oracle.reference(scalar=True)
+    m = _lib.gen_bernoulli(SA_SEED, 1024, SA_COLS, SA_P)
+    done = 0
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < budget_s:
+        if ref is None:
+            break
+        ref.sa_select(m, SA_COLS, 16, iterations=1, restarts=1, seed=1,
+                      policy="pairwise_synergy")
+        done += 1
+    dt = time.perf_counter() - t0
+    if ref is None or done == 0:
+        return None
+    return {"value": done * 1024 * 1025 / 2 / dt, "unit": "row pairs/s", "cores": 1,
+            "kind": "reference",
+            "sample": f"reference SASelector.select on 1024 x 8192 (iterations=1, restarts=1: "
+                      f"one pairwise synergy table), {done} calls"}
+
+
 def secondary(torch, _lib, dev, stream) -> dict:
     """The FWHT coset-minimum workloads, timed beside the headline (graph
     replay, autotuned launch shape)."""
@@ -489,7 +620,7 @@ def main() -> int:
     ap.add_argument("--chunk-bits", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--workload", default="gray",
-                    choices=["gray", "coset", "batch", "window", "basis", "ga"],
+                    choices=["gray", "coset", "batch", "window", "basis", "scores", "sa", "ga"],
                     help="gray: the Gray coset sweep (default); coset / batch / window: the "
                          "FWHT coset-minimum workloads; ga: plain vs learned GA")
     ap.add_argument("--ga-budget", type=float, default=60.0,
@@ -545,9 +676,19 @@ def main() -> int:
         return 0
 
     dev = torch.device("cuda:0")
-    if args.workload in ("coset", "batch", "window", "basis"):
+    if args.workload in ("coset", "batch", "window", "basis", "scores", "sa"):
         stream = torch.cuda.current_stream(dev)
-        if args.workload == "basis":
+        if args.workload == "scores":
+            r = run_scores(torch, args.steps, args.warmup)
+            cpu = None if args.no_cpu_baseline else scores_cpu_baseline()
+        elif args.workload == "sa":
+            r = run_sa(torch, args.steps, args.warmup)
+            cpu = None if args.no_cpu_baseline else sa_cpu_baseline()
+            line = mode_line(args, r, cpu)
+            line["value"], line["unit"] = r["evals"] / (r["ms"] / 1e3), "row pairs/s"
+            print(json.dumps(line))
+            return 0
+        elif args.workload == "basis":
             r = run_basis(torch, args.steps, args.warmup)
             cpu = None if args.no_cpu_baseline else basis_cpu_baseline()
         elif args.workload == "window":

@@ -145,6 +145,28 @@ __device__ inline uint32_t wave_sum_u32(uint32_t v) {
     return v;
 }
 
+// Workgroups are dispatched to the 8 XCDs round-robin (workgroup b on XCD
+// b % 8), and each XCD has its own L2.  xcd_tile(b, G) renumbers the G
+// workgroups of a launch so that each XCD takes one contiguous range of
+// tiles: neighbouring tiles share an L2, where partial cache lines at tile
+// edges merge before they are written back.
+constexpr uint32_t kXcds = 8;
+__device__ inline uint32_t xcd_tile(uint32_t b, uint32_t G) {
+    const uint32_t x = b % kXcds, q = G / kXcds, r = G % kXcds;
+    return x * q + (x < r ? x : r) + b / kXcds;
+}
+
+// Mask of the lanes whose digit (dbits bits) equals this lane's, among the
+// lanes where `valid` holds.  Every lane of the wave must call it.
+__device__ inline uint64_t match_digit(uint32_t d, int dbits, bool valid) {
+    uint64_t match = __ballot(valid);
+    for (int b = 0; b < dbits; ++b) {
+        const uint64_t bal = __ballot((d >> b) & 1u);
+        match &= ((d >> b) & 1u) ? bal : ~bal;
+    }
+    return match;
+}
+
 // Optional HIP-event timing of the dominant kernel of each entry point
 // (bmm_hip_timing_begin / bmm_hip_timing_end).  When enabled, a
 // ScopedKernelTimer records an event pair on the launch stream around the

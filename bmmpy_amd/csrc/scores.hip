@@ -6,13 +6,17 @@
 // score, stably (ascending u within a score); off[v] ends as the end of
 // bucket v in order (the reference's offsets after its scatter).
 //
-// The stable counting sort is an LSD radix sort over 8-bit digits of the
-// score (ceil(bits(n) / 8) passes; one pass for n < 256).  Each pass: a
-// digit histogram per tile of 4096 masks, an exclusive scan of the
-// digit-major table (digit start + earlier tiles), and a scatter in which a
-// tile ranks its masks in index order: for each of its 16 strided slices the
-// waves match equal digits with 8 ballots, and per-wave digit counts in LDS
-// give the offsets of earlier waves and slices.
+// The stable counting sort is an LSD radix sort over digits of the score:
+// ceil(bits(n) / 8) passes with the bits split evenly (a 10-bit score: two
+// 5-bit digits).  The scores kernel also writes the first pass's digit
+// histogram per tile of 8192 masks; later passes count their digits in a
+// histogram kernel.  On the first pass the masks themselves are the values.  The digit-major table (digit, tile) is scanned
+// exclusively on the device, which gives each tile the global start of each
+// digit's run.  The scatter stages a tile in LDS: each wave ranks its 1024
+// contiguous masks (16 slices of 64; equal digits matched with one ballot per
+// digit bit, running per-wave digit counts in LDS), the tile is reordered by
+// digit in LDS, and the runs are written out with consecutive lanes on
+// consecutive addresses.
 #include "common.hpp"
 
 #include <algorithm>
@@ -22,40 +26,109 @@
 namespace bmmhip {
 namespace {
 
-constexpr int kRsTile = 4096;   // masks per tile: 16 slices of 256
+constexpr int kRsTile = 8192;   // masks per tile
+constexpr int kRsWaves = 8;     // waves of a scatter workgroup
+constexpr int kRsThreads = kRsWaves * kWave;
+constexpr int kRsSlices = kRsTile / kRsThreads;   // 16 slices of 64 per wave
 constexpr int kScanBlock = 1024;
 constexpr int kLdsBuckets = 16384;
 
 template <typename S>
+struct Vec4;
+template <>
+struct Vec4<int32_t> {
+    using T = int4;
+};
+template <>
+struct Vec4<int16_t> {
+    using T = short4;
+};
+
+// Tiles are ranges of kRsTile indices: masks u (the first pass, u = 1 .. m-1
+// valid) or positions in the previous pass's output (later passes).  A
+// workgroup reads 4 consecutive entries per lane and vector load.
+//
+// Scores of masks 1 .. m-1, s[0] = 0, the score histogram cnt (in LDS when
+// lds_hist, then one atomic per bucket and workgroup) and the first radix
+// pass's digit histogram per tile: hist0[d * n_tiles + tile] for the low
+// dbits bits of the score.  Tiles go round-robin over the workgroups, so the
+// grid sweeps the arrays front to back together.
+template <typename S>
 __global__ __launch_bounds__(256) void scores_kernel(const S* __restrict__ h, uint64_t m, int32_t n,
                                                      S* __restrict__ s_out,
-                                                     int32_t* __restrict__ keys,
                                                      unsigned int* __restrict__ cnt, int lds_hist,
-                                                     unsigned int* __restrict__ overflow) {
+                                                     unsigned int* __restrict__ overflow,
+                                                     uint32_t n_tiles, int dbits,
+                                                     uint32_t* __restrict__ hist0) {
+    using V = typename Vec4<S>::T;
+    constexpr int kVecs = kRsTile / (256 * 4);
     extern __shared__ unsigned int sh_cnt[];
+    __shared__ unsigned int dig[256];
     const int buckets = n + 1;
-    if (lds_hist) {
+    const uint32_t dmask = (1u << dbits) - 1u;
+    if (lds_hist)
         for (int q = threadIdx.x; q < buckets; q += blockDim.x)
             sh_cnt[q] = 0;
+    for (uint32_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
+        // Thread t alone reads dig[t] below and clears it here: no barrier
+        // between the two.
+        dig[threadIdx.x] = 0;
         __syncthreads();
-    }
-    for (uint64_t u = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; u < m;
-         u += (uint64_t)gridDim.x * blockDim.x) {
-        if (u == 0) {
-            s_out[0] = S(0);
-            continue;
+        const uint64_t base = (uint64_t)tile * kRsTile + 4u * threadIdx.x;
+        S hv[kVecs][4];
+#pragma unroll
+        for (int k = 0; k < kVecs; ++k) {
+            const uint64_t u = base + (uint64_t)k * 1024;
+            if (u + 4 <= m) {
+                const V v = *reinterpret_cast<const V*>(h + u);
+                hv[k][0] = v.x;
+                hv[k][1] = v.y;
+                hv[k][2] = v.z;
+                hv[k][3] = v.w;
+            } else {
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    hv[k][j] = u + j < m ? h[u + j] : S(0);
+            }
         }
-        // 64-bit difference: n - h[u] cannot overflow here.
-        int64_t su = ((int64_t)n - (int64_t)h[u]) >> 1;
-        su = su < 0 ? 0 : (su > n ? n : su);
-        if (std::is_same<S, int16_t>::value && su > 32767)
-            atomicOr(overflow, 1u);
-        s_out[u] = S(su);
-        keys[u - 1] = (int32_t)su;
-        if (lds_hist)
-            atomicAdd(&sh_cnt[su], 1u);
-        else
-            atomicAdd(&cnt[su], 1u);
+#pragma unroll
+        for (int k = 0; k < kVecs; ++k) {
+            const uint64_t u = base + (uint64_t)k * 1024;
+            S sv[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                // 64-bit difference: n - h[u] cannot overflow here.
+                int64_t su = ((int64_t)n - (int64_t)hv[k][j]) >> 1;
+                su = su < 0 ? 0 : (su > n ? n : su);
+                const bool valid = u + j >= 1 && u + j < m;
+                sv[j] = valid ? S(su) : S(0);   // s[0] = 0
+                if (!valid)
+                    continue;
+                if (std::is_same<S, int16_t>::value && su > 32767)
+                    atomicOr(overflow, 1u);
+                atomicAdd(&dig[(uint32_t)su & dmask], 1u);
+                if (lds_hist)
+                    atomicAdd(&sh_cnt[su], 1u);
+                else
+                    atomicAdd(&cnt[su], 1u);
+            }
+            if (u + 4 <= m) {
+                V v;
+                v.x = sv[0];
+                v.y = sv[1];
+                v.z = sv[2];
+                v.w = sv[3];
+                *reinterpret_cast<V*>(s_out + u) = v;
+            } else {
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    if (u + j < m)
+                        s_out[u + j] = sv[j];
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x <= dmask)
+            hist0[(size_t)threadIdx.x * n_tiles + tile] = dig[threadIdx.x];
     }
     if (lds_hist) {
         __syncthreads();
@@ -65,64 +138,172 @@ __global__ __launch_bounds__(256) void scores_kernel(const S* __restrict__ h, ui
     }
 }
 
-// hist[d * n_tiles + tile] = masks of the tile whose digit is d.
+// hist[d * n_tiles + tile] = entries of the tile (positions below N) whose
+// digit is d.
 __global__ __launch_bounds__(256) void radix_hist_kernel(const int32_t* __restrict__ keys,
-                                                         uint32_t N, int shift, uint32_t n_tiles,
+                                                         uint32_t N, int shift, int dbits,
+                                                         uint32_t n_tiles,
                                                          uint32_t* __restrict__ hist) {
+    constexpr int kVecs = kRsTile / (256 * 4);
     __shared__ unsigned int c[256];
+    const uint32_t dmask = (1u << dbits) - 1u;
     c[threadIdx.x] = 0;
-    __syncthreads();
-    const uint64_t base = (uint64_t)blockIdx.x * kRsTile + threadIdx.x;
-    for (int k = 0; k < kRsTile / 256; ++k) {
-        const uint64_t e = base + (uint64_t)k * 256;
-        if (e < N)
-            atomicAdd(&c[((uint32_t)keys[e] >> shift) & 255u], 1u);
+    const uint64_t base = (uint64_t)blockIdx.x * kRsTile + 4u * threadIdx.x;
+    int32_t kv[kVecs][4];
+#pragma unroll
+    for (int k = 0; k < kVecs; ++k) {
+        const uint64_t p = base + (uint64_t)k * 1024;
+        if (p + 4 <= N) {
+            const int4 v = *reinterpret_cast<const int4*>(keys + p);
+            kv[k][0] = v.x;
+            kv[k][1] = v.y;
+            kv[k][2] = v.z;
+            kv[k][3] = v.w;
+        } else {
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                kv[k][j] = p + j < N ? keys[p + j] : 0;
+        }
     }
     __syncthreads();
-    hist[(size_t)threadIdx.x * n_tiles + blockIdx.x] = c[threadIdx.x];
+#pragma unroll
+    for (int k = 0; k < kVecs; ++k)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            if (base + (uint64_t)k * 1024 + j < N)
+                atomicAdd(&c[((uint32_t)kv[k][j] >> shift) & dmask], 1u);
+    __syncthreads();
+    if (threadIdx.x <= dmask)
+        hist[(size_t)threadIdx.x * n_tiles + blockIdx.x] = c[threadIdx.x];
 }
 
-// Stable scatter of one digit pass; start = scanned hist.  vals_in == nullptr
-// stands for the identity values e + 1 (mask u = element e + 1).
-__global__ __launch_bounds__(256) void radix_scatter_kernel(
-    const int32_t* __restrict__ keys_in, const int32_t* __restrict__ vals_in, uint32_t N,
-    int shift, const uint32_t* __restrict__ start, uint32_t n_tiles,
+// Stable scatter of one digit pass over the entries lo <= i < hi (tile t:
+// i in [t kRsTile, (t + 1) kRsTile)); start = the scanned digit-major table.
+// vals_in == nullptr stands for the identity values i (the masks, on the
+// first pass); keys_out == nullptr on the last pass.
+template <typename KI>
+__global__ __launch_bounds__(kRsThreads) void radix_scatter_kernel(
+    const KI* __restrict__ keys_in, const int32_t* __restrict__ vals_in, uint64_t lo, uint64_t hi,
+    int shift, int dbits, const uint32_t* __restrict__ start, uint32_t n_tiles,
     int32_t* __restrict__ keys_out, int32_t* __restrict__ vals_out) {
-    __shared__ uint32_t next[256];      // next output slot of each digit
-    __shared__ uint32_t wcnt[4][256];   // this slice's digit counts per wave
+    __shared__ int32_t skey[kRsTile];
+    __shared__ int32_t sval[kRsTile];
+    __shared__ uint32_t run[kRsWaves][256];   // per-wave digit counts, then wave offsets
+    __shared__ uint32_t lstart[256];          // tile-local start of each digit
+    __shared__ uint32_t gstart[256];          // global start of each digit's run
+    __shared__ uint32_t wsum[kRsWaves];
     const int tid = threadIdx.x, wave = tid / kWave, lane = tid % kWave;
-    next[tid] = start[(size_t)tid * n_tiles + blockIdx.x];
-    wcnt[0][tid] = wcnt[1][tid] = wcnt[2][tid] = wcnt[3][tid] = 0;
+    const uint32_t dmask = (1u << dbits) - 1u;
+    // Neighbouring tiles write neighbouring parts of each digit's region:
+    // they go to one XCD.
+    const uint32_t tile = xcd_tile(blockIdx.x, n_tiles);
+    const uint64_t tile0 = (uint64_t)tile * kRsTile;
+    for (int i = tid; i < kRsWaves * 256; i += kRsThreads)
+        (&run[0][0])[i] = 0;
+    if (tid < 256)
+        gstart[tid] = (uint32_t)tid <= dmask ? start[(size_t)tid * n_tiles + tile] : 0u;
+    // Wave w owns the tile's entries [1024 w, 1024 w + 1024), slice k of
+    // them at lanes 0..63.
+    const uint64_t wbase = tile0 + (uint64_t)wave * (kRsSlices * kWave) + lane;
+    int32_t key[kRsSlices], val[kRsSlices];
+    uint32_t pos[kRsSlices];
+#pragma unroll
+    for (int k = 0; k < kRsSlices; ++k) {
+        const uint64_t i = wbase + (uint64_t)k * kWave;
+        key[k] = i >= lo && i < hi ? (int32_t)keys_in[i] : 0;
+    }
+    // The branch on vals_in stays outside the unrolled loops (a select per
+    // slice measured 1.7x slower).
+    if (vals_in) {
+#pragma unroll
+        for (int k = 0; k < kRsSlices; ++k) {
+            const uint64_t i = wbase + (uint64_t)k * kWave;
+            val[k] = i >= lo && i < hi ? vals_in[i] : 0;
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < kRsSlices; ++k)
+            val[k] = (int32_t)(wbase + (uint64_t)k * kWave);
+    }
     __syncthreads();
     const uint64_t below = (1ull << lane) - 1ull;
-    for (int k = 0; k < kRsTile / 256; ++k) {
-        const uint64_t e = (uint64_t)blockIdx.x * kRsTile + (uint64_t)k * 256 + tid;
-        const bool valid = e < N;
-        const int32_t key = valid ? keys_in[e] : 0;
-        const uint32_t d = ((uint32_t)key >> shift) & 255u;
-        uint64_t match = __ballot(valid);
 #pragma unroll
-        for (int b = 0; b < 8; ++b) {
-            const uint64_t bal = __ballot((d >> b) & 1u);
-            match &= ((d >> b) & 1u) ? bal : ~bal;
-        }
+    for (int k = 0; k < kRsSlices; ++k) {
+        const uint64_t i = wbase + (uint64_t)k * kWave;
+        const bool valid = i >= lo && i < hi;
+        const uint32_t d = ((uint32_t)key[k] >> shift) & dmask;
+        const uint64_t match = match_digit(d, dbits, valid);
         const uint32_t rank = (uint32_t)__popcll(match & below);
+        // All lanes read the wave's count before the one lane per digit
+        // updates it; a wave's LDS operations complete in issue order.
+        const uint32_t before = run[wave][d];
+        pos[k] = before + rank;
+        __builtin_amdgcn_wave_barrier();
         if (valid && rank == 0)
-            wcnt[wave][d] = (uint32_t)__popcll(match);
-        __syncthreads();
-        if (valid) {
-            uint32_t pos = next[d] + rank;
-            for (int q = 0; q < wave; ++q)
-                pos += wcnt[q][d];
-            if (keys_out)
-                keys_out[pos] = key;
-            vals_out[pos] = vals_in ? vals_in[e] : (int32_t)(e + 1);
-        }
-        __syncthreads();
-        next[tid] += wcnt[0][tid] + wcnt[1][tid] + wcnt[2][tid] + wcnt[3][tid];
-        wcnt[0][tid] = wcnt[1][tid] = wcnt[2][tid] = wcnt[3][tid] = 0;
-        __syncthreads();
+            run[wave][d] = before + (uint32_t)__popcll(match);
+        __builtin_amdgcn_wave_barrier();
     }
+    __syncthreads();
+    // Threads 0..255, digit tid: exclusive offsets of the waves, tile count
+    // c, then the exclusive scan of c over the digits.
+    uint32_t c = 0, x = 0;
+    if (tid < 256) {
+#pragma unroll
+        for (int w = 0; w < kRsWaves; ++w) {
+            const uint32_t r = run[w][tid];
+            run[w][tid] = c;
+            c += r;
+        }
+        x = c;
+        for (int o = 1; o < kWave; o <<= 1) {
+            const uint32_t y = __shfl_up(x, o, kWave);
+            if (lane >= o)
+                x += y;
+        }
+        if (lane == kWave - 1)
+            wsum[wave] = x;
+    }
+    __syncthreads();
+    if (tid < 256) {
+        uint32_t ex = x - c;
+        for (int q = 0; q < wave; ++q)
+            ex += wsum[q];
+        lstart[tid] = ex;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kRsSlices; ++k) {
+        const uint64_t i = wbase + (uint64_t)k * kWave;
+        if (i >= lo && i < hi) {
+            const uint32_t d = ((uint32_t)key[k] >> shift) & dmask;
+            const uint32_t slot = lstart[d] + run[wave][d] + pos[k];
+            skey[slot] = key[k];
+            sval[slot] = val[k];
+        }
+    }
+    __syncthreads();
+    // Valid entries of the tile: slots 0 .. tile_n-1.
+    const uint64_t tile1 = tile0 + kRsTile;
+    const uint32_t tile_n = (uint32_t)((hi < tile1 ? hi : tile1) - (lo > tile0 ? lo : tile0));
+    // Destinations of slots tid + kRsThreads q, then one output array at a
+    // time.
+    uint32_t g[kRsSlices];
+#pragma unroll
+    for (int q = 0; q < kRsSlices; ++q) {
+        const uint32_t j = tid + (uint32_t)kRsThreads * q;
+        const uint32_t d = ((uint32_t)skey[j < tile_n ? j : 0] >> shift) & dmask;
+        g[q] = gstart[d] + (j - lstart[d]);
+    }
+    if (keys_out) {
+#pragma unroll
+        for (int q = 0; q < kRsSlices; ++q)
+            if (tid + (uint32_t)kRsThreads * q < tile_n)
+                keys_out[g[q]] = skey[tid + kRsThreads * q];
+    }
+#pragma unroll
+    for (int q = 0; q < kRsSlices; ++q)
+        if (tid + (uint32_t)kRsThreads * q < tile_n)
+            vals_out[g[q]] = sval[tid + kRsThreads * q];
 }
 
 // Scan of blocks of 1024 entries in place (exclusive or inclusive);
@@ -205,52 +386,72 @@ void scores_order(const S* h, size_t m, int32_t n, S* s_by_mask, int32_t* order,
     ScoresWorkspace& ws = scores_ws();
     const size_t buckets = (size_t)n + 1;
     const uint64_t N = m - 1;   // masks 1 .. m-1
+    // First pass: tiles over the masks 0 .. m-1; later passes: over the
+    // positions 0 .. N-1.
+    const uint32_t tiles_u = (uint32_t)((m + kRsTile - 1) / kRsTile);
+    const uint32_t tiles_p = (uint32_t)((N + kRsTile - 1) / kRsTile);
+    // Digits: ceil(bits / 8) passes, bits split evenly.
+    const int bits = n == 0 ? 1 : 32 - __builtin_clz((uint32_t)n);
+    const int passes = (bits + 7) / 8;
+    const int dbits = (bits + passes - 1) / passes;
     S* d_h = ws.h.get<S>(m);
     S* d_s = ws.s.get<S>(m);
     uint32_t* d_cnt = ws.cnt.get<uint32_t>(buckets);
     uint32_t* d_off = ws.off.get<uint32_t>(buckets);
     uint32_t* d_ovf = ws.overflow.get<uint32_t>(1);
-    int32_t* keys[2] = {ws.keys[0].get<int32_t>(std::max<uint64_t>(N, 1)),
-                        ws.keys[1].get<int32_t>(std::max<uint64_t>(N, 1))};
+    uint32_t* d_hist = ws.hist.get<uint32_t>((size_t)256 * std::max(tiles_u, tiles_p));
     BMM_HIP(hipMemcpy(d_h, h, m * sizeof(S), hipMemcpyHostToDevice));
     fill_u32(d_cnt, buckets, 0, nullptr);
     fill_u32(d_ovf, 1, 0, nullptr);
     const bool lds = buckets <= (size_t)kLdsBuckets;
-    const unsigned grid = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>((m + 4095) / 4096, 1024));
-    hipLaunchKernelGGL(scores_kernel<S>, dim3(grid), dim3(256), lds ? buckets * 4 : 0, nullptr, d_h,
-                       (uint64_t)m, n, d_s, keys[0], d_cnt, lds ? 1 : 0, d_ovf);
-    BMM_HIP(hipGetLastError());
+    const unsigned grid = std::min(tiles_u, 2048u);
+    {
+        ScopedKernelTimer timer(nullptr);   // scores, counts, first digit histogram
+        hipLaunchKernelGGL(scores_kernel<S>, dim3(grid), dim3(256), lds ? buckets * 4 : 0, nullptr,
+                           d_h, (uint64_t)m, n, d_s, d_cnt, lds ? 1 : 0, d_ovf, tiles_u, dbits,
+                           d_hist);
+        BMM_HIP(hipGetLastError());
+    }
     uint32_t ovf = 0;
     BMM_HIP(hipMemcpy(&ovf, d_ovf, 4, hipMemcpyDeviceToHost));
     if (ovf)
         fail(BMM_ERR_OVERFLOW, "calc_scores_and_order: a score exceeds the int16 range");
+    int32_t* d_order = ws.order.get<int32_t>(std::max<uint64_t>(N, 1));
     if (N > 0) {
-        ScopedKernelTimer timer(nullptr);
-        const int bits = n == 0 ? 1 : 32 - __builtin_clz((uint32_t)n);
-        const int passes = (bits + 7) / 8;
-        const uint32_t n_tiles = (uint32_t)((N + kRsTile - 1) / kRsTile);
-        uint32_t* d_hist = ws.hist.get<uint32_t>((size_t)256 * n_tiles);
-        int32_t* d_order = ws.order.get<int32_t>((size_t)N);
+        ScopedKernelTimer timer(nullptr);   // the radix passes
+        int32_t* keys[2] = {ws.keys[0].get<int32_t>(passes > 1 ? (size_t)N : 1),
+                            ws.keys[1].get<int32_t>(passes > 2 ? (size_t)N : 1)};
         int32_t* vals[2] = {ws.vals[0].get<int32_t>(passes > 1 ? (size_t)N : 1),
                             ws.vals[1].get<int32_t>(passes > 2 ? (size_t)N : 1)};
-        const int32_t* kin = keys[0];
+        const int32_t* kin = nullptr;
         const int32_t* vin = nullptr;
         for (int p = 0; p < passes; ++p) {
+            const int shift = p * dbits, db = std::min(dbits, bits - shift);
             const bool last = p == passes - 1;
-            int32_t* kout = last ? nullptr : keys[(p + 1) & 1];
+            const uint32_t tiles = p == 0 ? tiles_u : tiles_p;
+            int32_t* kout = last ? nullptr : keys[p & 1];
             int32_t* vout = last ? d_order : vals[p & 1];
-            hipLaunchKernelGGL(radix_hist_kernel, dim3(n_tiles), dim3(256), 0, nullptr, kin,
-                               (uint32_t)N, 8 * p, n_tiles, d_hist);
-            BMM_HIP(hipGetLastError());
-            scan_u32(d_hist, (uint64_t)256 * n_tiles, false, ws.levels, 0, nullptr);
-            hipLaunchKernelGGL(radix_scatter_kernel, dim3(n_tiles), dim3(256), 0, nullptr, kin, vin,
-                               (uint32_t)N, 8 * p, d_hist, n_tiles, kout, vout);
+            if (p > 0) {
+                hipLaunchKernelGGL(radix_hist_kernel, dim3(tiles), dim3(256), 0, nullptr, kin,
+                                   (uint32_t)N, shift, db, tiles, d_hist);
+                BMM_HIP(hipGetLastError());
+            }
+            scan_u32(d_hist, ((uint64_t)1 << db) * tiles, false, ws.levels, 0, nullptr);
+            if (p == 0)   // the scores of masks 1 .. m-1, the masks as values
+                hipLaunchKernelGGL(radix_scatter_kernel<S>, dim3(tiles), dim3(kRsThreads), 0,
+                                   nullptr, d_s, nullptr, (uint64_t)1, (uint64_t)m, shift, db,
+                                   d_hist, tiles, kout, vout);
+            else
+                hipLaunchKernelGGL(radix_scatter_kernel<int32_t>, dim3(tiles), dim3(kRsThreads), 0,
+                                   nullptr, kin, vin, (uint64_t)0, N, shift, db, d_hist, tiles,
+                                   kout, vout);
             BMM_HIP(hipGetLastError());
             kin = kout;
             vin = vout;
         }
-        BMM_HIP(hipMemcpy(order, d_order, N * 4, hipMemcpyDeviceToHost));
     }
+    if (N > 0)
+        BMM_HIP(hipMemcpy(order, d_order, N * 4, hipMemcpyDeviceToHost));
     BMM_HIP(hipMemcpy(d_off, d_cnt, buckets * 4, hipMemcpyDeviceToDevice));
     scan_u32(d_off, buckets, true, ws.levels, 0, nullptr);
     BMM_HIP(hipMemcpy(s_by_mask, d_s, m * sizeof(S), hipMemcpyDeviceToHost));

@@ -46,9 +46,11 @@ def test_scores_order_golden(lib, gglobal):
 
 
 # n around the one-pass limit (255/256), the LDS-histogram limit (16383 /
-# 16384) and the two/three/four-pass digit counts.
+# 16384), the two/three/four-pass digit counts and uneven digit splits
+# (300: 5 + 4 bits; 1023: 5 + 5), with partial last tiles.
 @pytest.mark.parametrize("dt,m,n", [
     (np.int32, 1, 5), (np.int32, 2, 0), (np.int32, 4097, 0), (np.int32, 65539, 255),
+    (np.int32, 70001, 300), (np.int32, (1 << 20) + 3, 1023), (np.int16, 5000, 1),
     (np.int32, 65536, 256), (np.int32, 1 << 20, 16383), (np.int32, 1 << 20, 16384),
     (np.int32, (1 << 22) + 5, 65535), (np.int32, 1 << 20, 65536),
     (np.int32, 300000, (1 << 24) + 5), (np.int16, 1 << 20, 4000), (np.int16, 12345, 32767),
