@@ -124,16 +124,22 @@ template <int LO, int R, bool FIRST> __device__ inline void pk_round(uint32_t* s
         p[e * ST] = v[e];
 }
 
-// Transforms the block and hands the last round's words to visit(q, v)
-// instead of storing them: v[e] (e < 8) is word w = (q + 4 e) * 256 + t for
-// thread t, q < 4, i.e. entries w (low half) and w + 2^13 (high half).
-// Starts with a barrier; leaves LDS holding the pre-last-round words.
-template <typename Visit> __device__ inline void pk_block_fwht_visit(uint32_t* s, Visit&& visit) {
+// The first two rounds (word bits 0-9, with the bit-13 stage).  Starts and
+// ends with a barrier; LDS then holds the pre-last-round words.
+__device__ inline void pk_block_fwht_rounds(uint32_t* s) {
     __syncthreads();
     pk_round<0, 5, true>(s);
     __syncthreads();
     pk_round<5, 5, false>(s);
     __syncthreads();
+}
+
+// The last round's words of thread t, word w = (q + 4 e) * 256 + t for q < 4
+// (entries w in the low half and w + 2^13 in the high half), handed to
+// visit(q, v) after the last three stages (BUTTERFLY) or as loaded (the
+// pre-last-round words, whose group of 8 combines into v's 8 outputs).
+template <bool BUTTERFLY, typename Visit>
+__device__ inline void pk_last_round_visit(const uint32_t* s, Visit&& visit) {
     constexpr int ST = fwht_pad_stride<10>();
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
@@ -142,9 +148,50 @@ template <typename Visit> __device__ inline void pk_block_fwht_visit(uint32_t* s
 #pragma unroll
         for (int e = 0; e < 8; ++e)
             v[e] = p[e * ST];
-        pk_butterflies(v);
+        if constexpr (BUTTERFLY)
+            pk_butterflies(v);
         visit(q, v);
     }
+}
+
+// The stages on word bits 10-12 in place: 4 groups of 8 words per thread
+// (word (q + 4 e) * 256 + t, q < 4).
+__device__ inline void pk_round_top(uint32_t* s) {
+    constexpr int ST = fwht_pad_stride<10>();
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        uint32_t* p = s + fwht_pad((int)threadIdx.x + 256 * q);
+        uint32_t v[8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e)
+            v[e] = p[e * ST];
+        pk_butterflies(v);
+#pragma unroll
+        for (int e = 0; e < 8; ++e)
+            p[e * ST] = v[e];
+    }
+}
+
+// Thread t's group over word bits 5-9 (words (t >> 5) << 10 | e << 5 | (t & 31),
+// e < 32), as loaded.
+template <typename Visit> __device__ inline void pk_mid_group_visit(const uint32_t* s, Visit&& visit) {
+    constexpr int ST = fwht_pad_stride<5>();
+    const int g = (int)threadIdx.x;
+    const uint32_t* p = s + fwht_pad(((g >> 5) << 10) | (g & 31));
+    uint32_t v[32];
+#pragma unroll
+    for (int e = 0; e < 32; ++e)
+        v[e] = p[e * ST];
+    visit(v);
+}
+
+// Transforms the block and hands the last round's words to visit(q, v)
+// instead of storing them: v[e] (e < 8) is word w = (q + 4 e) * 256 + t for
+// thread t, q < 4, i.e. entries w (low half) and w + 2^13 (high half).
+// Starts with a barrier; leaves LDS holding the pre-last-round words.
+template <typename Visit> __device__ inline void pk_block_fwht_visit(uint32_t* s, Visit&& visit) {
+    pk_block_fwht_rounds(s);
+    pk_last_round_visit<true>(s, visit);
 }
 
 // Packed subset maxima (subset_max in fwht_block.hpp, both halves at once).

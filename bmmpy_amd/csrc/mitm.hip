@@ -219,18 +219,24 @@ __device__ inline void win_build_pk16(uint32_t* s, const WinTarget& T, const Win
 // active columns).  Word w = (q + 4 e) * 256 + t holds masks w and
 // w + 2^13.
 //
-// Pass 0 keeps only the block's largest H (one v_pk_max per word).  The
-// block's best key is at least (w << n | mh << 14) with w = (total - H) / 2;
-// when that floor is not below the target's current key (a relaxed read:
-// keys only decrease, so a stale value only sends more blocks on), no mask of
-// the block can win and the block ends.  Otherwise pass 1 rebuilds and
-// transforms the block and folds every entry into the key
+// The block's best key is at least (w << n | mh << 14) with
+// w = (total - max H) / 2; when that floor is not below the target's current
+// key (a relaxed read: keys only decrease, so a stale value only sends more
+// blocks on), no mask of the block can win and the block ends.  The test
+// runs first on an upper bound of max H: the stages on word bits 0-4 (with
+// the bit-13 stage) and 10-12 are done, and the 32 words x_0..x_31 of a
+// group over bits 5-9, each a signed sum over its own set of column
+// patterns, would combine into outputs x_0 + sum_e +-x_e <=
+// x_0 + sum_{e >= 1} |x_e| (at most total, so no int16 overflow).  A block the
+// bound cannot settle finishes those stages in registers and is tested on its
+// exact max H.  A block that still can win is rebuilt and transformed with
+// the exact keyed scan: every entry folds into the key
 // H * 2^16 | (65535 - 4 m') with m' = ml - t, so one max keeps the largest H
 // and the lowest ml among the thread's entries.
 __device__ void win_block_pk16(uint32_t* s, const WinTarget& T, const WinRegs& R,
                                const WinPat8* __restrict__ w8, uint64_t mh,
                                unsigned long long* key) {
-    __shared__ int32_t wbest[4];
+    __shared__ int32_t wbest[4], wexact[4];
     __shared__ unsigned long long s_key;
     // The target's key is read (relaxed) right after the clear barrier and
     // only used after the transform, so the load's latency overlaps the build.
@@ -239,26 +245,52 @@ __device__ void win_block_pk16(uint32_t* s, const WinTarget& T, const WinRegs& R
         if (threadIdx.x == 0)
             key_seen = __atomic_load_n(key, __ATOMIC_RELAXED);
     });
-    {
-        s16x2 m = pk_as(0x80008000u);
-        pk_block_fwht_visit(s, [&](int, const uint32_t (&v)[8]) {
-#pragma unroll
-            for (int e = 0; e < 8; ++e)
-                m = pk_max(m, pk_as(v[e]));
-        });
-        const int32_t h = wave_max_i32(max(pk_lo(m), pk_hi(m)));
+    __syncthreads();
+    pk_round<0, 5, true>(s);
+    __syncthreads();
+    pk_round_top(s);
+    __syncthreads();
+    if (threadIdx.x == 0)
+        s_key = key_seen;
+    // Block maximum of one value per thread; every thread takes the same
+    // decision from the same shared values.
+    auto cannot_win = [&](int32_t h, int32_t* ws) {
+        h = wave_max_i32(h);
         if (threadIdx.x % kWave == 0)
-            wbest[threadIdx.x / kWave] = h;
-        if (threadIdx.x == 0)
-            s_key = key_seen;
+            ws[threadIdx.x / kWave] = h;
         __syncthreads();
-        // Every thread takes the same decision from the same shared values.
-        const int32_t hb = max(max(wbest[0], wbest[1]), max(wbest[2], wbest[3]));
+        const int32_t hb = max(max(ws[0], ws[1]), max(ws[2], ws[3]));
         const uint32_t w = (uint32_t)(T.total - hb) >> 1;
         const unsigned long long floor_key = ((unsigned long long)w << T.bits) | (mh << kPkL);
-        if (!(floor_key < s_key))
-            return;
-    }
+        return !(floor_key < s_key);
+    };
+    int32_t hbound = 0;
+    pk_mid_group_visit(s, [&](const uint32_t (&v)[32]) {
+        s16x2 b = pk_as(v[0]);
+#pragma unroll
+        for (int e = 1; e < 32; ++e) {
+            const s16x2 x = pk_as(v[e]);
+            b = b + pk_max(x, -x);
+        }
+        hbound = max(pk_lo(b), pk_hi(b));
+    });
+    if (cannot_win(hbound, wbest))
+        return;
+    int32_t hexact = 0;
+    pk_mid_group_visit(s, [&](const uint32_t (&v)[32]) {
+        uint32_t y[32];
+#pragma unroll
+        for (int e = 0; e < 32; ++e)
+            y[e] = v[e];
+        pk_butterflies(y);
+        s16x2 m = pk_as(y[0]);
+#pragma unroll
+        for (int e = 1; e < 32; ++e)
+            m = pk_max(m, pk_as(y[e]));
+        hexact = max(pk_lo(m), pk_hi(m));
+    });
+    if (cannot_win(hexact, wexact))
+        return;
     win_build_pk16(s, T, R, w8, mh, [] {});
     int32_t best = INT32_MIN;
     pk_block_fwht_visit(s, [&](int q, const uint32_t (&v)[8]) {
