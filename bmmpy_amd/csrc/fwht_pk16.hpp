@@ -154,21 +154,30 @@ __device__ inline void pk_last_round_visit(const uint32_t* s, Visit&& visit) {
     }
 }
 
-// The stages on word bits 10-12 in place: 4 groups of 8 words per thread
-// (word (q + 4 e) * 256 + t, q < 4).
+// The stages on word bits 10-12 in place: thread t takes the 4 groups of 8
+// words 4 t + j + 1024 e (j < 4, e < 8).  The 4 words of one e lie in one
+// 32-word padding segment, so they are adjacent in LDS and go as paired
+// accesses (the stride between e, 1056 words, is too wide for paired
+// offsets); lanes 4 words apart hit distinct banks thanks to the padding.
 __device__ inline void pk_round_top(uint32_t* s) {
     constexpr int ST = fwht_pad_stride<10>();
+    uint32_t* p = s + fwht_pad(4 * (int)threadIdx.x);
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        uint32_t* p = s + fwht_pad((int)threadIdx.x + 256 * q);
-        uint32_t v[8];
-#pragma unroll
-        for (int e = 0; e < 8; ++e)
-            v[e] = p[e * ST];
-        pk_butterflies(v);
+    for (int j0 = 0; j0 < 4; j0 += 2) {   // two groups at a time (register budget)
+        uint32_t v[2][8];
 #pragma unroll
         for (int e = 0; e < 8; ++e)
-            p[e * ST] = v[e];
+#pragma unroll
+            for (int j = 0; j < 2; ++j)
+                v[j][e] = p[e * ST + j0 + j];
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+            pk_butterflies(v[j]);
+#pragma unroll
+        for (int e = 0; e < 8; ++e)
+#pragma unroll
+            for (int j = 0; j < 2; ++j)
+                p[e * ST + j0 + j] = v[j][e];
     }
 }
 
