@@ -78,8 +78,10 @@ __global__ __launch_bounds__(256) void batch_patterns_kernel(const uint64_t* __r
 // `overall` is the thread's maximum, mreg[j] its maximum over the elements
 // with ml bit 8 + j set (j < L - 8).  Target bits below 8 reduce row 0 over
 // the threads with that bit, bits 8..L-1 reduce their rows, bits >= L take
-// the block maximum when mh has the bit.
-template <int L>
+// the block maximum when mh has the bit.  With SHIFTED (the packed block's
+// paired last round) thread t's elements have ml bits 2..9 = t instead, and
+// mreg[0..1] stand for ml bits 0 and 1, mreg[2..] for bits 10 and up.
+template <int L, bool SHIFTED = false>
 __device__ void coset_block_finish(int32_t* s, const CosetMatrix& M, uint32_t mh,
                                    unsigned long long* keys, int32_t overall,
                                    const int32_t* mreg) {
@@ -113,7 +115,12 @@ __device__ void coset_block_finish(int32_t* s, const CosetMatrix& M, uint32_t mh
     __syncthreads();
     if (tid < M.rows) {
         const bool has = tid < L || ((mh >> (tid - L)) & 1u);
-        const int32_t h = res[tid < L ? tid : L];
+        // Row of target bit b: thread bit b (or b - 2), else the register
+        // bit's row 8 + its index.
+        int row = tid < L ? tid : L;
+        if (SHIFTED && tid < L)
+            row = tid < 2 ? 8 + tid : tid < 10 ? tid - 2 : tid;
+        const int32_t h = res[row];
         if (has && h != INT32_MIN) {
             // Keys only decrease, so a stale read never skips an improvement.
             const uint32_t w = (uint32_t)(M.total - h) >> 1;
@@ -178,8 +185,9 @@ __device__ void coset_block_phase1(int32_t* s, const CosetMatrix& M, uint32_t mh
 }
 
 // Packed-int16 block of 2^14 entries (matrices of >= 14 rows and at most
-// 32767 columns).  Word w = (q + 4 e) * 256 + t holds entries w and
-// w + 2^13: ml bits 8..12 are the register index q + 4 e, bit 13 the half.
+// 32767 columns).  The last round hands thread t the words 4 t + j + 1024 e,
+// each holding entries w and w + 2^13: ml bits 0-1 are j, bits 2-9 the
+// thread, bits 10-12 e and bit 13 the half.
 template <typename Build>
 __device__ void coset_block_pk16(uint32_t* s, const CosetMatrix& M, uint32_t mh,
                                  unsigned long long* keys, Build&& build) {
@@ -190,7 +198,7 @@ __device__ void coset_block_pk16(uint32_t* s, const CosetMatrix& M, uint32_t mh,
 #pragma unroll
     for (int i = 0; i < 5; ++i)
         mreg[i] = lowest;
-    pk_block_fwht_visit(s, [&](int q, const uint32_t (&v)[8]) {
+    pk_block_fwht_visit_paired(s, [&](int q, const uint32_t (&v)[8]) {
         s16x2 x[8], bm[3];
 #pragma unroll
         for (int e = 0; e < 8; ++e)
@@ -210,7 +218,7 @@ __device__ void coset_block_pk16(uint32_t* s, const CosetMatrix& M, uint32_t mh,
     for (int i = 0; i < 5; ++i)
         m[i] = max(pk_lo(mreg[i]), pk_hi(mreg[i]));
     m[5] = pk_hi(overall);
-    coset_block_finish<kPkL>(reinterpret_cast<int32_t*>(s), M, mh, keys,
+    coset_block_finish<kPkL, true>(reinterpret_cast<int32_t*>(s), M, mh, keys,
                              max(pk_lo(overall), pk_hi(overall)), m);
 }
 

@@ -194,6 +194,32 @@ template <typename Visit> __device__ inline void pk_mid_group_visit(const uint32
     visit(v);
 }
 
+// Transforms the block; the last round goes by thread t's words
+// 4 t + j + 1024 e (j < 4 the entry bits 0-1, t entry bits 2-9, e bits
+// 10-12), handed to visit(j, v) with v[e] after the last three stages.  The
+// two words j0, j0 + 1 of one e are adjacent in LDS (one padding segment),
+// so each pair of groups loads with paired accesses.  Starts with a barrier;
+// leaves LDS holding the pre-last-round words.
+template <typename Visit> __device__ inline void pk_block_fwht_visit_paired(uint32_t* s,
+                                                                           Visit&& visit) {
+    pk_block_fwht_rounds(s);
+    constexpr int ST = fwht_pad_stride<10>();
+    const uint32_t* p = s + fwht_pad(4 * (int)threadIdx.x);
+#pragma unroll
+    for (int j0 = 0; j0 < 4; j0 += 2) {
+        uint32_t v0[8], v1[8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            v0[e] = p[e * ST + j0];
+            v1[e] = p[e * ST + j0 + 1];
+        }
+        pk_butterflies(v0);
+        visit(j0, v0);
+        pk_butterflies(v1);
+        visit(j0 + 1, v1);
+    }
+}
+
 // Transforms the block and hands the last round's words to visit(q, v)
 // instead of storing them: v[e] (e < 8) is word w = (q + 4 e) * 256 + t for
 // thread t, q < 4, i.e. entries w (low half) and w + 2^13 (high half).

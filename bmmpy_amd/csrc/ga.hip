@@ -670,11 +670,13 @@ __global__ __launch_bounds__(256) void ga_win_kernel(const uint64_t* __restrict_
                 atomicAdd(&s[fwht_pad((int)(pl & ((1u << kPkW) - 1u)))], neg ? 0u - one : one);
             }
         }
+        // Thread t's entries: ml = 4 t + m' with m' = j + 1024 e (+ 2^13 for
+        // the high half), keyed H * 2^16 | (65535 - 4 m').
         int32_t best = INT32_MIN;
-        pk_block_fwht_visit(s, [&](int q, const uint32_t (&v)[8]) {
+        pk_block_fwht_visit_paired(s, [&](int j, const uint32_t (&v)[8]) {
 #pragma unroll
             for (int e = 0; e < 8; ++e) {
-                const uint32_t m_lo = (uint32_t)(256 * (q + 4 * e));
+                const uint32_t m_lo = (uint32_t)(j + 1024 * e);
                 best = max(best, (int32_t)((v[e] << 16) | (65535u - 4u * m_lo)));
                 best = max(best, (int32_t)((v[e] & 0xffff0000u) |
                                            (65535u - 4u * (m_lo + (1u << kPkW)))));
@@ -682,7 +684,7 @@ __global__ __launch_bounds__(256) void ga_win_kernel(const uint64_t* __restrict_
         });
         {
             const int32_t h = best >> 16;
-            const uint32_t ml = ((65535u - ((uint32_t)best & 0xffffu)) >> 2) + threadIdx.x;
+            const uint32_t ml = ((65535u - ((uint32_t)best & 0xffffu)) >> 2) + 4u * threadIdx.x;
             best = (int32_t)(((uint32_t)h << 16) | (65535u - 4u * ml));
         }
         best = wave_max_i32(best);
