@@ -79,8 +79,8 @@ __global__ __launch_bounds__(256) void batch_patterns_kernel(const uint64_t* __r
 // with ml bit 8 + j set (j < L - 8).  Target bits below 8 reduce row 0 over
 // the threads with that bit, bits 8..L-1 reduce their rows, bits >= L take
 // the block maximum when mh has the bit.  With SHIFTED (the packed block's
-// paired last round) thread t's elements have ml bits 2..9 = t instead, and
-// mreg[0..1] stand for ml bits 0 and 1, mreg[2..] for bits 10 and up.
+// paired last round) thread t's elements have ml bits 1..8 = t instead, and
+// mreg[0..1] stand for ml bits 0 and 9, mreg[2..] for bits 10 and up.
 template <int L, bool SHIFTED = false>
 __device__ void coset_block_finish(int32_t* s, const CosetMatrix& M, uint32_t mh,
                                    unsigned long long* keys, int32_t overall,
@@ -115,11 +115,11 @@ __device__ void coset_block_finish(int32_t* s, const CosetMatrix& M, uint32_t mh
     __syncthreads();
     if (tid < M.rows) {
         const bool has = tid < L || ((mh >> (tid - L)) & 1u);
-        // Row of target bit b: thread bit b (or b - 2), else the register
+        // Row of target bit b: thread bit b (or b - 1), else the register
         // bit's row 8 + its index.
         int row = tid < L ? tid : L;
         if (SHIFTED && tid < L)
-            row = tid < 2 ? 8 + tid : tid < 10 ? tid - 2 : tid;
+            row = tid == 0 ? 8 : tid < 9 ? tid - 1 : tid;
         const int32_t h = res[row];
         if (has && h != INT32_MIN) {
             // Keys only decrease, so a stale read never skips an improvement.
@@ -185,9 +185,10 @@ __device__ void coset_block_phase1(int32_t* s, const CosetMatrix& M, uint32_t mh
 }
 
 // Packed-int16 block of 2^14 entries (matrices of >= 14 rows and at most
-// 32767 columns).  The last round hands thread t the words 4 t + j + 1024 e,
-// each holding entries w and w + 2^13: ml bits 0-1 are j, bits 2-9 the
-// thread, bits 10-12 e and bit 13 the half.
+// 32767 columns).  The last round hands thread t the words
+// 2 t + j + 512 h + 1024 e, each holding entries w and w + 2^13: ml bit 0 is
+// j, bits 1-8 the thread, bit 9 h, bits 10-12 e and bit 13 the half; the
+// visit's q = j | h << 1.
 template <typename Build>
 __device__ void coset_block_pk16(uint32_t* s, const CosetMatrix& M, uint32_t mh,
                                  unsigned long long* keys, Build&& build) {

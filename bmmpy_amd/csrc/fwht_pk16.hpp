@@ -1,8 +1,8 @@
 // Packed-int16 Walsh-Hadamard blocks for one 256-thread workgroup.
 //
 // A block of 2^14 entries lives in LDS as 2^13 words: word w holds entry w in
-// its low half and entry w + 2^13 in its high half (padding as in
-// fwht_block.hpp: one word per 32).  Every radix-2 stage on entry bits 0..12
+// its low half and entry w + 2^13 in its high half (padded: 4 words after
+// every 32, see pk_pad).  Every radix-2 stage on entry bits 0..12
 // applies the same butterfly to both halves, so one v_pk_add_u16 /
 // v_pk_sub_i16 pair does two butterflies; the stage on entry bit 13 combines
 // the two halves of a word, (lo, hi) -> (lo + hi, lo - hi), in one
@@ -34,7 +34,18 @@ constexpr int kPkL = 14;                 // entries per block: 2^14
 constexpr int kPkW = kPkL - 1;           // word index bits
 constexpr uint32_t kPkBias = 0x80008000u;
 constexpr int kPkMaxTotal = 32767;
-__host__ __device__ constexpr int pk_lds_words() { return fwht_lds_words(kPkW); }
+
+// LDS layout of a packed block: 4 pad words after every 32 (rows of 36
+// words, 16-byte aligned).  The first round's 32 consecutive words per
+// thread are then eight 16-byte accesses; lanes 36 words apart start on
+// distinct bank groups, and 8-byte accesses of lanes 2 words apart within a
+// row cover distinct bank pairs.
+__host__ __device__ constexpr int pk_pad(int x) { return x + ((x >> 5) << 2); }
+template <int LO> __host__ __device__ constexpr int pk_pad_stride() {
+    static_assert(LO == 0 || LO >= 5, "strides inside a 32-word row are unpadded");
+    return pk_pad(1 << LO);
+}
+__host__ __device__ constexpr int pk_lds_words() { return pk_pad(1 << kPkW); }
 
 template <int E> __device__ inline void pk_butterflies(uint32_t (&v)[E]) {
 #pragma unroll
@@ -74,7 +85,7 @@ __device__ inline void pk_build(uint32_t* s, const uint32_t (&preg)[NREG], uint3
             const uint32_t pl = p & ((1u << kPkL) - 1u);
             const uint32_t one = (pl >> kPkW) ? 0x10000u : 1u;
             const int neg = __builtin_popcount((p >> kPkL) & mh) & 1;
-            atomicAdd(&s[fwht_pad((int)(pl & ((1u << kPkW) - 1u)))], neg ? 0u - one : one);
+            atomicAdd(&s[pk_pad((int)(pl & ((1u << kPkW) - 1u)))], neg ? 0u - one : one);
         }
     }
 }
@@ -94,7 +105,7 @@ __device__ inline void pk_build_mem(uint32_t* s, const uint32_t* __restrict__ pa
         const uint32_t pl = p & ((1u << kPkL) - 1u);
         const uint32_t one = (pl >> kPkW) ? 0x10000u : 1u;
         const int neg = __builtin_popcount((p >> kPkL) & mh) & 1;
-        atomicAdd(&s[fwht_pad((int)(pl & ((1u << kPkW) - 1u)))], neg ? 0u - one : one);
+        atomicAdd(&s[pk_pad((int)(pl & ((1u << kPkW) - 1u)))], neg ? 0u - one : one);
     }
 }
 
@@ -104,11 +115,11 @@ __device__ inline void pk_build_mem(uint32_t* s, const uint32_t* __restrict__ pa
 template <int LO, int R, bool FIRST> __device__ inline void pk_round(uint32_t* s) {
     static_assert((1 << (kPkW - R)) == 256, "one group per thread");
     constexpr int E = 1 << R;
-    constexpr int ST = fwht_pad_stride<LO>();
+    constexpr int ST = pk_pad_stride<LO>();
     const int g = (int)threadIdx.x;
     const int low = g & ((1 << LO) - 1);
     const int high = g >> LO;
-    uint32_t* p = s + fwht_pad((high << (LO + R)) | low);
+    uint32_t* p = s + pk_pad((high << (LO + R)) | low);
     uint32_t v[E];
 #pragma unroll
     for (int e = 0; e < E; ++e)
@@ -140,10 +151,10 @@ __device__ inline void pk_block_fwht_rounds(uint32_t* s) {
 // pre-last-round words, whose group of 8 combines into v's 8 outputs).
 template <bool BUTTERFLY, typename Visit>
 __device__ inline void pk_last_round_visit(const uint32_t* s, Visit&& visit) {
-    constexpr int ST = fwht_pad_stride<10>();
+    constexpr int ST = pk_pad_stride<10>();
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-        const uint32_t* p = s + fwht_pad((int)threadIdx.x + 256 * q);
+        const uint32_t* p = s + pk_pad((int)threadIdx.x + 256 * q);
         uint32_t v[8];
 #pragma unroll
         for (int e = 0; e < 8; ++e)
@@ -155,21 +166,21 @@ __device__ inline void pk_last_round_visit(const uint32_t* s, Visit&& visit) {
 }
 
 // The stages on word bits 10-12 in place: thread t takes the 4 groups of 8
-// words 4 t + j + 1024 e (j < 4, e < 8).  The 4 words of one e lie in one
-// 32-word padding segment, so they are adjacent in LDS and go as paired
-// accesses (the stride between e, 1056 words, is too wide for paired
-// offsets); lanes 4 words apart hit distinct banks thanks to the padding.
+// words c + 1024 e (e < 8) for c = 2 t + j + 512 h (j, h < 2).  The words
+// c, c + 1 of one e are adjacent and 8-byte aligned, so they go as one 8-byte
+// access (the stride between e is too wide for paired offsets), and 16 lanes
+// of one access cover distinct bank pairs.
 __device__ inline void pk_round_top(uint32_t* s) {
-    constexpr int ST = fwht_pad_stride<10>();
-    uint32_t* p = s + fwht_pad(4 * (int)threadIdx.x);
+    constexpr int ST = pk_pad_stride<10>();
 #pragma unroll
-    for (int j0 = 0; j0 < 4; j0 += 2) {   // two groups at a time (register budget)
+    for (int h = 0; h < 2; ++h) {   // two groups at a time (register budget)
+        uint32_t* p = s + pk_pad(2 * (int)threadIdx.x + 512 * h);
         uint32_t v[2][8];
 #pragma unroll
         for (int e = 0; e < 8; ++e)
 #pragma unroll
             for (int j = 0; j < 2; ++j)
-                v[j][e] = p[e * ST + j0 + j];
+                v[j][e] = p[e * ST + j];
 #pragma unroll
         for (int j = 0; j < 2; ++j)
             pk_butterflies(v[j]);
@@ -177,16 +188,16 @@ __device__ inline void pk_round_top(uint32_t* s) {
         for (int e = 0; e < 8; ++e)
 #pragma unroll
             for (int j = 0; j < 2; ++j)
-                p[e * ST + j0 + j] = v[j][e];
+                p[e * ST + j] = v[j][e];
     }
 }
 
 // Thread t's group over word bits 5-9 (words (t >> 5) << 10 | e << 5 | (t & 31),
 // e < 32), as loaded.
 template <typename Visit> __device__ inline void pk_mid_group_visit(const uint32_t* s, Visit&& visit) {
-    constexpr int ST = fwht_pad_stride<5>();
+    constexpr int ST = pk_pad_stride<5>();
     const int g = (int)threadIdx.x;
-    const uint32_t* p = s + fwht_pad(((g >> 5) << 10) | (g & 31));
+    const uint32_t* p = s + pk_pad(((g >> 5) << 10) | (g & 31));
     uint32_t v[32];
 #pragma unroll
     for (int e = 0; e < 32; ++e)
@@ -194,29 +205,28 @@ template <typename Visit> __device__ inline void pk_mid_group_visit(const uint32
     visit(v);
 }
 
-// Transforms the block; the last round goes by thread t's words
-// 4 t + j + 1024 e (j < 4 the entry bits 0-1, t entry bits 2-9, e bits
-// 10-12), handed to visit(j, v) with v[e] after the last three stages.  The
-// two words j0, j0 + 1 of one e are adjacent in LDS (one padding segment),
-// so each pair of groups loads with paired accesses.  Starts with a barrier;
-// leaves LDS holding the pre-last-round words.
+// Transforms the block; the last round goes by thread t's words c + 1024 e
+// for c = 2 t + j + 512 h (entry bit 0 = j, bits 1-8 = t, bit 9 = h, bits
+// 10-12 = e), handed to visit(j | h << 1, v) with v[e] after the last three
+// stages.  The words c, c + 1 of one e go as one 8-byte access.  Starts with
+// a barrier; leaves LDS holding the pre-last-round words.
 template <typename Visit> __device__ inline void pk_block_fwht_visit_paired(uint32_t* s,
                                                                            Visit&& visit) {
     pk_block_fwht_rounds(s);
-    constexpr int ST = fwht_pad_stride<10>();
-    const uint32_t* p = s + fwht_pad(4 * (int)threadIdx.x);
+    constexpr int ST = pk_pad_stride<10>();
 #pragma unroll
-    for (int j0 = 0; j0 < 4; j0 += 2) {
+    for (int h = 0; h < 2; ++h) {
+        const uint32_t* p = s + pk_pad(2 * (int)threadIdx.x + 512 * h);
         uint32_t v0[8], v1[8];
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
-            v0[e] = p[e * ST + j0];
-            v1[e] = p[e * ST + j0 + 1];
+            v0[e] = p[e * ST];
+            v1[e] = p[e * ST + 1];
         }
         pk_butterflies(v0);
-        visit(j0, v0);
+        visit(h << 1, v0);
         pk_butterflies(v1);
-        visit(j0 + 1, v1);
+        visit(1 | h << 1, v1);
     }
 }
 

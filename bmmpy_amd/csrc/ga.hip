@@ -591,7 +591,7 @@ __global__ void ga_win_rank_kernel(int islands, int rows, int C, int W, int bit,
 // as W-bit window patterns with the target bit at bit 31.  Blocks mh of 2^14
 // masks (2^(W-14) of them) are visited in order; the first block holding the
 // largest H keeps the lowest mask.  move[id] = the subset mask (0: none).
-__global__ __launch_bounds__(256) void ga_win_kernel(const uint64_t* __restrict__ base, int rows,
+__global__ __launch_bounds__(256, 2) void ga_win_kernel(const uint64_t* __restrict__ base, int rows,
                                                      int words, int W, int bit, int phase,
                                                      const int16_t* __restrict__ win,
                                                      const uint32_t* __restrict__ roww,
@@ -667,16 +667,16 @@ __global__ __launch_bounds__(256) void ga_win_kernel(const uint64_t* __restrict_
                 const uint32_t pl = p & ((1u << kPkL) - 1u);
                 const uint32_t one = (pl >> kPkW) ? 0x10000u : 1u;
                 const uint32_t neg = ((p >> 31) ^ __builtin_popcount((p >> kPkL) & hi_mask & mh)) & 1u;
-                atomicAdd(&s[fwht_pad((int)(pl & ((1u << kPkW) - 1u)))], neg ? 0u - one : one);
+                atomicAdd(&s[pk_pad((int)(pl & ((1u << kPkW) - 1u)))], neg ? 0u - one : one);
             }
         }
-        // Thread t's entries: ml = 4 t + m' with m' = j + 1024 e (+ 2^13 for
-        // the high half), keyed H * 2^16 | (65535 - 4 m').
+        // Thread t's entries: ml = 2 t + m' with m' = j + 512 h + 1024 e
+        // (+ 2^13 for the high half), keyed H * 2^16 | (65535 - 4 m').
         int32_t best = INT32_MIN;
-        pk_block_fwht_visit_paired(s, [&](int j, const uint32_t (&v)[8]) {
+        pk_block_fwht_visit_paired(s, [&](int jh, const uint32_t (&v)[8]) {
 #pragma unroll
             for (int e = 0; e < 8; ++e) {
-                const uint32_t m_lo = (uint32_t)(j + 1024 * e);
+                const uint32_t m_lo = (uint32_t)((jh & 1) + 512 * (jh >> 1) + 1024 * e);
                 best = max(best, (int32_t)((v[e] << 16) | (65535u - 4u * m_lo)));
                 best = max(best, (int32_t)((v[e] & 0xffff0000u) |
                                            (65535u - 4u * (m_lo + (1u << kPkW)))));
@@ -684,7 +684,7 @@ __global__ __launch_bounds__(256) void ga_win_kernel(const uint64_t* __restrict_
         });
         {
             const int32_t h = best >> 16;
-            const uint32_t ml = ((65535u - ((uint32_t)best & 0xffffu)) >> 2) + 4u * threadIdx.x;
+            const uint32_t ml = ((65535u - ((uint32_t)best & 0xffffu)) >> 2) + 2u * threadIdx.x;
             best = (int32_t)(((uint32_t)h << 16) | (65535u - 4u * ml));
         }
         best = wave_max_i32(best);

@@ -328,7 +328,7 @@ __device__ void win_block_pk16(uint32_t* s, const WinTarget& T, const WinRegs& R
 // Blocks of the targets whose blocks are 2^L wide (the persistent queue of
 // one kernel); targets in queue order, found by scanning forward.
 template <int L>
-__global__ __launch_bounds__(256) void win_full_kernel(const WinTarget* __restrict__ tgts,
+__global__ __launch_bounds__(256, 4) void win_full_kernel(const WinTarget* __restrict__ tgts,
                                                        int n_tgt, int64_t total_blocks,
                                                        const uint64_t* __restrict__ pats,
                                                        const int32_t* __restrict__ mults,
@@ -494,14 +494,14 @@ void window_coset_min(const uint64_t* rows, size_t n_rows, size_t words, size_t 
                     if ((pl >> kPkW) != half)
                         continue;
                     const uint64_t hi = all_p[k] >> kPkL;
-                    const uint32_t byte = 4u * (uint32_t)fwht_pad((int)(pl & ((1u << kPkW) - 1u)));
+                    const uint32_t byte = 4u * (uint32_t)pk_pad((int)(pl & ((1u << kPkW) - 1u)));
                     all_w8.push_back(WinPat8{
                         (uint32_t)hi, ((uint32_t)(uint16_t)(int16_t)all_m[k] << 16) | byte |
                                           (uint32_t)((hi >> 32) & 1u)});
                 }
                 while ((all_w8.size() - off) % 256)
                     all_w8.push_back(WinPat8{
-                        0u, 4u * (uint32_t)fwht_pad((int)((all_w8.size() - off) &
+                        0u, 4u * (uint32_t)pk_pad((int)((all_w8.size() - off) &
                                                           ((1u << kPkW) - 1u)))});
                 (half ? T.hi_off : T.lo_off) = off;
                 (half ? T.n_hi : T.n_lo) = (int32_t)(all_w8.size() - off);
