@@ -27,6 +27,8 @@ Other workloads (one JSON line each, same fields):
   --workload sa      SASelector's pairwise synergy table (pair_overlaps) on
                      4096 x 8192 Bernoulli(0.3) (seed 0x5EED0011); value in
                      row pairs/s.
+  --workload comb    fixed_weight_masks_u32(32, 16): 601,080,390 masks;
+                     value in masks/s from host arrays, roofline on the kernel.
   --workload ga      plain vs learned island GA on the planted 256 x 8192.
 
 A step is one full pass of the workload with the inputs already resident in
@@ -534,6 +536,58 @@ def sa_cpu_baseline(budget_s: float = 10.0) -> dict:
                       f"one pairwise synergy table), {done} calls"}
 
 
+COMB_N, COMB_K = 32, 16
+
+
+def run_comb(torch, steps: int, warmup: int) -> dict:
+    """fixed_weight_masks_u32(32, 16): the C(32, 16) = 601,080,390 masks of
+    32 bits with 16 set, increasing.  A step is one call from the host; the
+    roofline is the kernel's store of 4 bytes per mask."""
+    from bmmpy_amd import _lib
+
+    for _ in range(warmup):
+        _lib.fixed_weight_masks(COMB_N, COMB_K)
+    _lib.timing_begin()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        out = _lib.fixed_weight_masks(COMB_N, COMB_K)
+    dt = (time.perf_counter() - t0) / steps
+    kms, nk = _lib.timing_end()
+    kernel_ms = kms / max(nk, 1)
+    count = out.size
+    alg = 4.0 * count
+    achieved = alg / (kernel_ms / 1e3) / 1e9
+    return {
+        "ms": 1e3 * dt, "evals": float(count),
+        "results": [{"count": int(count), "first": int(out[0]), "last": int(out[-1])}],
+        "config": {"workload": "fixed_weight_masks_u32_32_16", "n": COMB_N, "k": COMB_K},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": 8000.0, "unit": "GB/s",
+                     "frac": achieved / 8000.0, "traffic": alg, "bytes_per_launch": alg,
+                     "kernel": "fixed_weight_kernel<uint32_t>", "kernel_avg_ms": kernel_ms,
+                     "host_call_ms": 1e3 * dt},
+    }
+
+
+def comb_cpu_baseline(budget_s: float = 10.0) -> dict:
+    """The reference fixed_weight_masks_u32 on C(28, 14) samples."""
+    import math
+    import oracle  # checker / baseline only
+
+    ref = oracle.reference()
+    if ref is None:
+        return None
+    done = 0
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < budget_s:
+        ref.fixed_weight_masks(28, 14)
+        done += 1
+    dt = time.perf_counter() - t0
+    return {"value": done * math.comb(28, 14) / dt, "unit": "masks/s", "cores": 1,
+            "kind": "reference",
+            "sample": f"reference fixed_weight_masks_u32(28, 14) (40,116,600 masks), {done} "
+                      f"calls, copied out as uint64"}
+
+
 def secondary(torch, _lib, dev, stream) -> dict:
     """The FWHT coset-minimum workloads, timed beside the headline (graph
     replay, autotuned launch shape)."""
@@ -620,7 +674,8 @@ def main() -> int:
     ap.add_argument("--chunk-bits", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--workload", default="gray",
-                    choices=["gray", "coset", "batch", "window", "basis", "scores", "sa", "ga"],
+                    choices=["gray", "coset", "batch", "window", "basis", "scores", "sa", "comb",
+                             "ga"],
                     help="gray: the Gray coset sweep (default); coset / batch / window: the "
                          "FWHT coset-minimum workloads; ga: plain vs learned GA")
     ap.add_argument("--ga-budget", type=float, default=60.0,
@@ -676,8 +731,15 @@ def main() -> int:
         return 0
 
     dev = torch.device("cuda:0")
-    if args.workload in ("coset", "batch", "window", "basis", "scores", "sa"):
+    if args.workload in ("coset", "batch", "window", "basis", "scores", "sa", "comb"):
         stream = torch.cuda.current_stream(dev)
+        if args.workload == "comb":
+            r = run_comb(torch, args.steps, args.warmup)
+            cpu = None if args.no_cpu_baseline else comb_cpu_baseline()
+            line = mode_line(args, r, cpu)
+            line["value"], line["unit"] = r["evals"] / (r["ms"] / 1e3), "masks/s"
+            print(json.dumps(line))
+            return 0
         if args.workload == "scores":
             r = run_scores(torch, args.steps, args.warmup)
             cpu = None if args.no_cpu_baseline else scores_cpu_baseline()

@@ -121,6 +121,10 @@ SIGNATURES = {
                             C.c_int),
     "bmm_hip_scores_order_i32": ([_i32p, _sz, C.c_int32, _i32p, _i32p, _i32p, _i32p], C.c_int),
     "bmm_hip_scores_order_i16": ([_i16p, _sz, C.c_int32, _i16p, _i32p, _i32p, _i32p], C.c_int),
+    "bmm_hip_fixed_weight_masks_u32": ([C.c_uint32, C.c_uint32, _vp, _sz, C.POINTER(_sz)],
+                                       C.c_int),
+    "bmm_hip_fixed_weight_masks_u64": ([C.c_uint32, C.c_uint32, _vp, _sz, C.POINTER(_sz)],
+                                       C.c_int),
     "bmm_hip_mlp_gradients": ([_vp, _f32p, _f32p, _sz, _f32p, C.POINTER(C.c_float)], C.c_int),
     "bmm_hip_mlp_train_step": ([_vp, _f32p, _f32p, _sz, C.c_float, C.POINTER(C.c_float)],
                                C.c_int),
@@ -306,6 +310,18 @@ def row_combine(rows, masks):
     call("bmm_hip_row_combine", rows.reshape(-1), rows.shape[0], rows.shape[1],
          masks.reshape(-1), masks.shape[1], masks.shape[0], out.reshape(-1), C.byref(wt))
     return out, wt.value
+
+
+def fixed_weight_masks(n: int, k: int, bits: int = 32) -> np.ndarray:
+    """fixed_weight_masks_u32 / _u64 as a numpy array (count query, then the
+    masks)."""
+    name = f"bmm_hip_fixed_weight_masks_u{bits}"
+    cnt = _sz()
+    call(name, n, k, None, 0, C.byref(cnt))
+    out = np.zeros(max(cnt.value, 1), np.uint32 if bits == 32 else np.uint64)
+    if cnt.value:
+        call(name, n, k, out.ctypes.data, out.size, C.byref(cnt))
+    return out[:cnt.value]
 
 
 def calc_scores_and_order(h, n: int):

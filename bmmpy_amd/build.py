@@ -25,7 +25,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 HIP_SOURCES = ["capi.hip", "gray.hip", "fwht.hip", "search.hip", "replay.hip", "mitm.hip", "bruteforce.hip",
                "batch.hip", "ga.hip", "mlp.hip", "workloads.hip", "order.hip",
-               "overlap.hip", "combine.hip", "scores.hip"]
+               "overlap.hip", "combine.hip", "scores.hip", "comb.hip"]
 HIP_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-Wall",
     "-Wno-unused-function",

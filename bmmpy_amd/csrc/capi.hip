@@ -93,6 +93,9 @@ void row_combine(const uint64_t* rows, size_t n_rows, size_t words, const uint64
 template <typename S>
 void scores_order(const S* h, size_t m, int32_t n, S* s_by_mask, int32_t* order, int32_t* cnt,
                   int32_t* off);
+uint64_t binom_u64(uint32_t n, uint32_t k);
+template <typename T>
+void fixed_weight_masks(uint32_t n, uint32_t k, T* out);
 void mlp_score_pairs_host(Mlp* M, const uint64_t* sketch, size_t rows, const uint16_t* pairs,
                           size_t n, float* y);
 float mlp_grads_host(Mlp* M, const float* x, const float* t, size_t B, float* grads);
@@ -134,6 +137,35 @@ template <typename T> void fwht_host(T* data, size_t n) {
     BMM_HIP(hipMemcpy(d, data, n * sizeof(T), hipMemcpyHostToDevice));
     fwht_array_dev<T>(d, n, nullptr);
     BMM_HIP(hipMemcpy(data, d, n * sizeof(T), hipMemcpyDeviceToHost));
+}
+
+// fixed_weight_masks_u32 / _u64 (src/math/comb.cpp:13-82): the reference's
+// checks in its order (k = 0 first, then the mask width, then k > n).
+template <typename T>
+int fixed_weight_entry(uint32_t n, uint32_t k, T* out, size_t capacity, size_t* count,
+                       const char* name) {
+    return guarded([&] {
+        constexpr uint32_t width = 8 * sizeof(T);
+        if (!count)
+            fail(BMM_ERR_INVALID, std::string(name) + ": null count");
+        if (k == 0) {
+            *count = 1;
+            if (out && capacity >= 1)
+                out[0] = 0;
+            return;
+        }
+        if (n > width)
+            fail(BMM_ERR_INVALID, std::string(name) + ": n exceeds " + std::to_string(width) +
+                                      "-bit mask width");
+        const uint64_t c = binom_u64(n, k);
+        *count = (size_t)c;
+        if (c == 0 || !out)
+            return;
+        if (capacity < c)
+            fail(BMM_ERR_INVALID, std::string(name) + ": output capacity below C(n, k)");
+        require_device();
+        fixed_weight_masks<T>(n, k, out);
+    });
 }
 
 template <typename S>
@@ -673,6 +705,16 @@ int bmm_hip_row_combine(const uint64_t* row_words, size_t n_rows, size_t words,
         require_device();
         row_combine(row_words, n_rows, words, masks, mask_words, n_out, out_words, out_weight);
     });
+}
+
+int bmm_hip_fixed_weight_masks_u32(uint32_t n, uint32_t k, uint32_t* out, size_t capacity,
+                                   size_t* count) {
+    return fixed_weight_entry<uint32_t>(n, k, out, capacity, count, "fixed_weight_masks_u32");
+}
+
+int bmm_hip_fixed_weight_masks_u64(uint32_t n, uint32_t k, uint64_t* out, size_t capacity,
+                                   size_t* count) {
+    return fixed_weight_entry<uint64_t>(n, k, out, capacity, count, "fixed_weight_masks_u64");
 }
 
 int bmm_hip_scores_order_i32(const int32_t* h, size_t m, int32_t n, int32_t* s_by_mask,

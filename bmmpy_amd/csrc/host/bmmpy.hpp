@@ -306,6 +306,11 @@ void calc_scores_and_order(const std::int32_t* h, std::size_t m, std::int32_t n,
                            std::int32_t* s_by_mask, std::int32_t* order, std::int32_t* cnt,
                            std::int32_t* off);
 
+// fixed_weight_masks_u32 / _u64 (include/bmmpy/math/comb.hpp:7-12): out =
+// every n-bit mask with k set bits, increasing (computed on the GPU).
+void fixed_weight_masks_u32(std::uint32_t n, std::uint32_t k, std::vector<std::uint32_t>& out);
+void fixed_weight_masks_u64(std::uint32_t n, std::uint32_t k, std::vector<std::uint64_t>& out);
+
 // ---------------------------------------------------------- appliers -----
 struct ApplyResult {
     std::size_t applied_count = 0;

@@ -418,4 +418,21 @@ PYBIND11_MODULE(_bmmpy, m) {
     m.def("add", [](int a, int b) { return a + b; }, py::arg("a"), py::arg("b"));
     m.def("fwht_i32", &fwht_copy<std::int32_t>, py::arg("values"));
     m.def("fwht_i16", &fwht_copy<std::int16_t>, py::arg("values"));
+    // Lists, as the reference's bindings return (python/bindings/math.cpp:14-34).
+    m.def(
+        "fixed_weight_masks_u32",
+        [](std::uint32_t n, std::uint32_t k) {
+            std::vector<std::uint32_t> out;
+            fixed_weight_masks_u32(n, k, out);
+            return out;
+        },
+        py::arg("n"), py::arg("k"));
+    m.def(
+        "fixed_weight_masks_u64",
+        [](std::uint32_t n, std::uint32_t k) {
+            std::vector<std::uint64_t> out;
+            fixed_weight_masks_u64(n, k, out);
+            return out;
+        },
+        py::arg("n"), py::arg("k"));
 }

@@ -44,4 +44,22 @@ void calc_scores_and_order(const std::int32_t* h, std::size_t m, std::int32_t n,
     throw_status(bmm_hip_scores_order_i32(h, m, n, s_by_mask, order, cnt, off));
 }
 
+void fixed_weight_masks_u32(std::uint32_t n, std::uint32_t k, std::vector<std::uint32_t>& out) {
+    out.clear();
+    std::size_t count = 0;
+    throw_status(bmm_hip_fixed_weight_masks_u32(n, k, nullptr, 0, &count));
+    out.resize(count);
+    if (count)
+        throw_status(bmm_hip_fixed_weight_masks_u32(n, k, out.data(), count, &count));
+}
+
+void fixed_weight_masks_u64(std::uint32_t n, std::uint32_t k, std::vector<std::uint64_t>& out) {
+    out.clear();
+    std::size_t count = 0;
+    throw_status(bmm_hip_fixed_weight_masks_u64(n, k, nullptr, 0, &count));
+    out.resize(count);
+    if (count)
+        throw_status(bmm_hip_fixed_weight_masks_u64(n, k, out.data(), count, &count));
+}
+
 } // namespace bmmpy

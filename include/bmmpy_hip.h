@@ -203,6 +203,18 @@ int bmm_hip_scores_order_i32(const int32_t *h, size_t m, int32_t n, int32_t *s_b
 int bmm_hip_scores_order_i16(const int16_t *h, size_t m, int32_t n, int16_t *s_by_mask,
                              int32_t *order, int32_t *cnt, int32_t *off);
 
+/* ---- fixed_weight_masks_u32 / _u64 (include/bmmpy/math/comb.hpp:7-12,
+ * src/math/comb.cpp:13-82): every n-bit mask with exactly k set bits, in
+ * increasing order.  *count = the number of masks (1 for k = 0, whose one
+ * mask is 0 whatever n; 0 for k > n; else C(n, k)); with out == NULL only
+ * *count is set, otherwise out must hold capacity >= *count masks.  n above
+ * the mask width (32 / 64) is BMM_ERR_INVALID, checked after k = 0 as in the
+ * reference. */
+int bmm_hip_fixed_weight_masks_u32(uint32_t n, uint32_t k, uint32_t *out, size_t capacity,
+                                   size_t *count);
+int bmm_hip_fixed_weight_masks_u64(uint32_t n, uint32_t k, uint64_t *out, size_t capacity,
+                                   size_t *count);
+
 /* ---- candidate stream of ExactBasisSolver (src/solver/exact_basis_solver.cpp:
  * 29-124): the first min(k, 2^t - 1) nonzero masks of a t-row window
  * (t <= 32) in the order (weight, Gray index), i.e. the reference's Gray

@@ -105,6 +105,8 @@ class Oracle:
                                           _i32p]
         L.or_scores_order_i16.argtypes = [_i16p, C.c_size_t, C.c_int32, _i16p, _i32p, _i32p,
                                           _i32p]
+        L.or_fixed_weight_masks.argtypes = [C.c_uint32, C.c_uint32, C.c_int, _u64p, C.c_size_t]
+        L.or_fixed_weight_masks.restype = C.c_longlong
         L.or_global_greedy.argtypes = [_u64p, C.c_int, C.c_int, C.c_int, _u64p, C.c_int, _u32p,
                                        C.c_size_t, C.c_int, C.POINTER(C.c_uint64),
                                        C.POINTER(C.c_uint64)]
@@ -268,6 +270,14 @@ class Oracle:
         return _scores_order_call(self.lib.or_scores_order_i32, self.lib.or_scores_order_i16,
                                   h, n, lambda f, *a: _check(f(*a), "scores_order"))
 
+    def fixed_weight_masks(self, n, k, width=32):
+        """fixed_weight_masks_u32 / _u64 as a uint64 array."""
+        cnt = self.lib.or_fixed_weight_masks(n, k, width, np.zeros(1, np.uint64), 0)
+        _check(min(cnt, 0), "fixed_weight_masks")
+        out = np.zeros(max(cnt, 1), np.uint64)
+        self.lib.or_fixed_weight_masks(n, k, width, out, out.size)
+        return out[:cnt]
+
     def global_greedy(self, m, cols, masks, weights, require_improvement=True):
         """GlobalGreedyApplier on all rows of m; masks: (n_cand, mask_words)
         uint64.  Returns (rows, applied, improvement)."""
@@ -328,6 +338,9 @@ class Reference:
                                       _u64p] + ep
         L.ref_mul_rank.argtypes = [_u64p, C.c_int, C.c_int, C.c_int, _u64p, C.c_int, C.c_int,
                                    _u64p] + ep
+        L.ref_fixed_weight_masks.argtypes = [C.c_uint32, C.c_uint32, C.c_int, _u64p,
+                                             C.c_size_t] + ep
+        L.ref_fixed_weight_masks.restype = C.c_longlong
 
     @staticmethod
     def _err():
@@ -449,6 +462,13 @@ class Reference:
     def scores_order(self, h, n):
         return _scores_order_call(self.lib.ref_scores_order_i32, self.lib.ref_scores_order_i16,
                                   h, n, self._run)
+
+    def fixed_weight_masks(self, n, k, width=32):
+        """Reference fixed_weight_masks_u32 / _u64 as a uint64 array."""
+        cnt = self._run(self.lib.ref_fixed_weight_masks, n, k, width, np.zeros(1, np.uint64), 0)
+        out = np.zeros(max(cnt, 1), np.uint64)
+        self._run(self.lib.ref_fixed_weight_masks, n, k, width, out, out.size)
+        return out[:cnt]
 
     def mul_rank(self, a, a_cols, b, b_cols):
         """Reference (a * b, rank(a))."""

@@ -1003,6 +1003,43 @@ int or_scores_order_i16(const int16_t *h, size_t m, int32_t n, int16_t *s_by_mas
     return scores_order_core(NULL, h, m, n, NULL, s_by_mask, order, cnt, off);
 }
 
+/* ------------------------------------------------ fixed-weight masks --- */
+/* fixed_weight_masks_u32 / _u64 (src/math/comb.cpp:13-82): k = 0 gives the
+ * single mask 0 before any width check; n above the width is an error; k > n
+ * gives none; else the n-bit masks with k set bits in increasing order,
+ * stepped by Gosper's successor m' = (((r ^ m) >> 2) / (m & -m)) | r with
+ * r = m + (m & -m) until m reaches 2^n (r wraps to 0 after the last mask
+ * when n = 64).  Pinned by the reference's known answers (n 5 k 3, n 6 k 2),
+ * by brute force over all masks below 2^n for n <= 16, and by the count
+ * C(n, k). */
+long long or_fixed_weight_masks(uint32_t n, uint32_t k, int width, uint64_t *out,
+                                size_t capacity) {
+    size_t count = 0;
+    if (k == 0) {
+        if (capacity)
+            out[0] = 0;
+        return 1;
+    }
+    if (n > (uint32_t)width)
+        return OR_EINVAL;
+    if (k > n)
+        return 0;
+    uint64_t m = (k == 64) ? ~(uint64_t)0 : (((uint64_t)1 << k) - 1);
+    for (;;) {
+        if (n < 64 && (m >> n) != 0)
+            break;
+        if (count < capacity)
+            out[count] = m;
+        ++count;
+        const uint64_t low = m & (~m + 1);
+        const uint64_t r = m + low;
+        if (r == 0)
+            break;
+        m = (((r ^ m) >> 2) / low) | r;
+    }
+    return (long long)count;
+}
+
 /* ------------------------------------------------ global greedy apply --- */
 typedef struct {
     const uint64_t *mask;

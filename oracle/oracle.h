@@ -113,6 +113,12 @@ int or_scores_order_i16(const int16_t *h, size_t m, int32_t n, int16_t *s_by_mas
  * on t rows (stride words) with n_cand candidates of mask_words words each.
  * Rows replaced in place when applied.  Returns 0, -1 (mask bit >= t) or
  * -4 (no full transform). */
+/* fixed_weight_masks_u32 / _u64 (src/math/comb.cpp:13-82) for a mask width
+ * of 32 or 64: writes up to `capacity` masks (n bits, k set, increasing) and
+ * returns their count; OR_EINVAL for n above the width when k > 0. */
+long long or_fixed_weight_masks(uint32_t n, uint32_t k, int width, uint64_t *out,
+                                size_t capacity);
+
 int or_global_greedy(uint64_t *rows, int t, int stride, int cols, const uint64_t *masks,
                      int mask_words, const uint32_t *weights, size_t n_cand,
                      int require_improvement, uint64_t *out_applied,

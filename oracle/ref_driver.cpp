@@ -13,6 +13,7 @@
 #include "bmmpy/apply/greedy_applier.hpp"
 #include "bmmpy/core/bit_matrix.hpp"
 #include "bmmpy/core/row_window.hpp"
+#include "bmmpy/math/comb.hpp"
 #include "bmmpy/math/fwht.hpp"
 #include "bmmpy/search/bruteforce_search.hpp"
 #include "bmmpy/search/fwht_search.hpp"
@@ -278,6 +279,28 @@ int ref_scores_order_i16(const std::int16_t* h, std::size_t m, std::int32_t n, s
     try {
         bmmpy::calc_scores_and_order(h, m, n, s, order, cnt, off);
         return 0;
+    } catch (const std::exception& e) {
+        set_err(err, err_len, e.what());
+        return -1;
+    }
+}
+
+// fixed_weight_masks_u32 / _u64: up to capacity masks into out (as u64);
+// returns the count.
+long long ref_fixed_weight_masks(std::uint32_t n, std::uint32_t k, int width, std::uint64_t* out,
+                                 std::size_t capacity, char* err, int err_len) {
+    try {
+        std::vector<std::uint64_t> v;
+        if (width == 32) {
+            std::vector<std::uint32_t> v32;
+            bmmpy::fixed_weight_masks_u32(n, k, v32);
+            v.assign(v32.begin(), v32.end());
+        } else {
+            bmmpy::fixed_weight_masks_u64(n, k, v);
+        }
+        for (std::size_t i = 0; i < v.size() && i < capacity; ++i)
+            out[i] = v[i];
+        return (long long)v.size();
     } catch (const std::exception& e) {
         set_err(err, err_len, e.what());
         return -1;
