@@ -18,6 +18,8 @@ from ._bmmpy import (  # noqa: F401
     RowWindow,
     RuntimeFeatures,
     add,
+    fixed_weight_masks_u32,
+    fixed_weight_masks_u64,
     fwht_i16,
     fwht_i32,
     get_runtime_features,
@@ -36,12 +38,27 @@ from .search import (
 )
 from .workflows import search_apply
 from . import solver  # noqa: E402  (bmm.solver.ExactBasisSolver, as in the reference)
+from . import ga  # noqa: E402
+from .ga import (  # noqa: E402
+    GeneticAlgorithm,
+    IslandModel,
+    IslandModelSnapshot,
+    IslandSnapshot,
+    IslandSpec,
+    MigrationPolicy,
+    OverflowPolicy,
+    RunStats,
+    StopCriteria,
+)
 
 __all__ = [
     "ApplyResult", "BitMatrix", "BruteforceSearch", "Candidate", "CudaBruteforceSearch",
     "CudaMitmFwhtSearch", "FwhtSearch", "GlobalGreedyApplier", "GreedyApplier", "MatrixErr", "MatrixError",
     "MitmFwhtSearch", "RowWindow", "SASelectionResult", "SASelector", "RuntimeFeatures", "add", "fwht_i16", "fwht_i32",
     "get_runtime_features", "get_version", "matrix_from_rows", "search_apply", "solver",
+    "fixed_weight_masks_u32", "fixed_weight_masks_u64", "ga", "GeneticAlgorithm", "IslandModel",
+    "IslandModelSnapshot", "IslandSnapshot", "IslandSpec", "MigrationPolicy", "OverflowPolicy",
+    "RunStats", "StopCriteria",
 ]
 
 __version__ = get_version()

@@ -107,6 +107,7 @@ SIGNATURES = {
     "bmm_hip_ga_create": ([_u64p, _sz, _sz, C.POINTER(GaParams), C.POINTER(C.c_void_p)], C.c_int),
     "bmm_hip_ga_run": ([_vp, _sz, C.c_uint64, C.c_double, C.POINTER(GaStatus)], C.c_int),
     "bmm_hip_ga_best": ([_vp, _u64p, C.POINTER(C.c_uint64)], C.c_int),
+    "bmm_hip_ga_island_best": ([_vp, C.c_int, _u64p, C.POINTER(C.c_uint64)], C.c_int),
     "bmm_hip_ga_destroy": ([_vp], C.c_int),
     "bmm_hip_mlp_num_params": ([], C.c_size_t),
     "bmm_hip_mlp_create": ([C.c_uint64, C.POINTER(C.c_void_p)], C.c_int),
@@ -416,6 +417,13 @@ class GpuGA:
         out = np.zeros((self.rows, self.cols // 64), np.uint64)
         w = C.c_uint64()
         call("bmm_hip_ga_best", self.h, out, C.byref(w))
+        return out, w.value
+
+    def island_best(self, island: int):
+        """Island `island`'s best matrix and weight."""
+        out = np.zeros((self.rows, self.cols // 64), np.uint64)
+        w = C.c_uint64()
+        call("bmm_hip_ga_island_best", self.h, island, out, C.byref(w))
         return out, w.value
 
     def close(self):

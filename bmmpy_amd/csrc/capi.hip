@@ -79,6 +79,7 @@ void ga_destroy(Ga* G);
 void ga_run(Ga* G, size_t max_generations, uint64_t target, double time_budget_s,
             bmm_ga_status* st);
 void ga_best(Ga* G, uint64_t* out, uint64_t* weight);
+void ga_island_best(Ga* G, int island, uint64_t* out, uint64_t* weight);
 struct Mlp;
 Mlp* mlp_create(uint64_t seed);
 void mlp_destroy(Mlp* M);
@@ -617,6 +618,14 @@ int bmm_hip_ga_best(void* ga, uint64_t* out_matrix, uint64_t* out_weight) {
         if (!ga || !out_matrix || !out_weight)
             fail(BMM_ERR_INVALID, "ga_best: null pointer");
         ga_best(static_cast<Ga*>(ga), out_matrix, out_weight);
+    });
+}
+
+int bmm_hip_ga_island_best(void* ga, int island, uint64_t* out_matrix, uint64_t* out_weight) {
+    return guarded([&] {
+        if (!ga || !out_matrix || !out_weight)
+            fail(BMM_ERR_INVALID, "ga_island_best: null pointer");
+        ga_island_best(static_cast<Ga*>(ga), island, out_matrix, out_weight);
     });
 }
 

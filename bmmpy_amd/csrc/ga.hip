@@ -1517,13 +1517,16 @@ void ga_run(Ga* G, size_t max_generations, uint64_t target, double time_budget_s
 }
 
 // Best matrix: the best island's base with its best sequence replayed.
-void ga_best(Ga* G, uint64_t* out, uint64_t* weight) {
-    bmm_ga_status st{};
+// Island `island`'s best individual (its base with its best op sequence
+// replayed); island < 0: the island with the lightest best.
+void ga_island_best(Ga* G, int island, uint64_t* out, uint64_t* weight) {
     const int I = G->P.islands;
+    if (island >= I)
+        fail(BMM_ERR_RANGE, "ga_island_best: island out of range");
     std::vector<unsigned int> bw(I);
     BMM_HIP(hipMemcpy(bw.data(), G->best_w.ptr, (size_t)I * 4, hipMemcpyDeviceToHost));
-    const int isl = (int)(std::min_element(bw.begin(), bw.end()) - bw.begin());
-    (void)st;
+    const int isl =
+        island >= 0 ? island : (int)(std::min_element(bw.begin(), bw.end()) - bw.begin());
     const size_t per = (size_t)G->P.rows * G->words;
     std::vector<uint64_t> m(per);
     BMM_HIP(hipMemcpy(m.data(), static_cast<uint64_t*>(G->base.ptr) + per * isl, per * 8,
@@ -1544,5 +1547,7 @@ void ga_best(Ga* G, uint64_t* out, uint64_t* weight) {
     std::memcpy(out, m.data(), per * 8);
     *weight = w;
 }
+
+void ga_best(Ga* G, uint64_t* out, uint64_t* weight) { ga_island_best(G, -1, out, weight); }
 
 } // namespace bmmhip

@@ -313,6 +313,11 @@ int bmm_hip_ga_run(void *ga, size_t max_generations, uint64_t target_weight,
                    double time_budget_s, bmm_ga_status *out);
 /* Best matrix found (rows x cols/64 words, host) and its weight. */
 int bmm_hip_ga_best(void *ga, uint64_t *out_matrix, uint64_t *out_weight);
+/* Island `island`'s best matrix and weight (island < 0: the lightest
+ * island's); the per-island snapshot of IslandModel
+ * (include/bmmpy/ga/types.hpp:45-53).  An island out of range is
+ * BMM_ERR_RANGE. */
+int bmm_hip_ga_island_best(void *ga, int island, uint64_t *out_matrix, uint64_t *out_weight);
 int bmm_hip_ga_destroy(void *ga);
 
 /* ---- proposal MLP (256 -> 256 -> 256 -> 1, ReLU, bf16 products with fp32
