@@ -65,6 +65,9 @@ TRAFFIC_BYTES = {
     "gray_coset_sweep": (192.5 * 2.0 + 160.0) * 1024,
     "coset_batch_4096": (53729.0 * 2.0 + 29953.0) * 1024,
     "window_coset_planted_48x4096": (118.4 * 2.0 + 131137.0) * 1024,
+    # The four kernels of one calc_scores_and_order call on 2^28 entries.
+    "scores_order_2p28": ((524576.3 + 537970.3 + 524304.9 + 526392.4) * 2.0
+                          + 1231680.9 + 1048635.2 + 34002.3 + 1048579.9) * 1024,
 }
 
 
@@ -438,12 +441,13 @@ def run_scores(torch, steps: int, warmup: int) -> dict:
     kernel_ms = kms / max(steps, 1)   # the scores kernel and the radix passes of one call
     alg = 12.0 * m
     achieved = alg / (kernel_ms / 1e3) / 1e9
-    # Moved by the kernels (10-bit scores: two 5-bit digit passes): scores
-    # (read h, write s: 8 B, the first digit histogram on the side), first
-    # scatter (read the scores, write keys and masks: 12 B), second pass's
-    # histogram (read keys: 4 B) and scatter (read keys and masks, write the
-    # order: 12 B).
-    moved = (8.0 + 12 + 4 + 12) * m
+    # Moved by the kernels (10-bit scores: a 6-bit and a 4-bit digit pass):
+    # scores (read h, write s: 8 B, the first digit histogram on the side),
+    # first scatter (read the scores, write mask | upper score bits: 8 B),
+    # second pass's histogram (read the packed words: 4 B) and scatter (read
+    # them, write the order: 8 B): 28 B per mask.  rocprof's FETCH_SIZE and
+    # WRITE_SIZE of the four kernels (copy-calibrated) give the figure used.
+    moved = TRAFFIC_BYTES["scores_order_2p28"]
     return {
         "ms": 1e3 * dt, "evals": float(m - 1),
         "results": [{"cnt_head": [int(x) for x in cnt[:4]], "order_head": [int(x) for x in order[:4]],

@@ -8,15 +8,23 @@
 //
 // The stable counting sort is an LSD radix sort over digits of the score:
 // ceil(bits(n) / 8) passes with the bits split evenly (a 10-bit score: two
-// 5-bit digits).  The scores kernel also writes the first pass's digit
-// histogram per tile of 8192 masks; later passes count their digits in a
-// histogram kernel.  On the first pass the masks themselves are the values.  The digit-major table (digit, tile) is scanned
-// exclusively on the device, which gives each tile the global start of each
-// digit's run.  The scatter stages a tile in LDS: each wave ranks its 1024
-// contiguous masks (16 slices of 64; equal digits matched with one ballot per
-// digit bit, running per-wave digit counts in LDS), the tile is reordered by
-// digit in LDS, and the runs are written out with consecutive lanes on
-// consecutive addresses.
+// 5-bit digits).  Two passes pack the mask and the score bits above the
+// first digit into one 32-bit word between them when both fit (2^28 masks
+// with 10-bit scores: a 6-bit first digit, 28 + 4 bits packed), which saves
+// 8 bytes of traffic per mask.  The scores kernel also writes the first
+// pass's digit histogram per tile of 4096 masks; later passes count their
+// digits in a histogram kernel.  On the first pass the masks themselves are
+// the values.  The digit-major table (digit, tile) is scanned exclusively on
+// the device, which gives each tile the global start of each digit's run.
+// The scatter stages a tile in LDS: each wave ranks its 512 contiguous masks
+// (8 slices of 64; equal digits matched with one ballot per digit bit,
+// running per-wave digit counts in LDS), the tile is reordered by digit in
+// LDS (one word per entry where the value can be rebuilt from it), and the
+// runs are written out with consecutive lanes on consecutive addresses.
+// Tiles of 4096 keep the scatter at 57-73 VGPRs and 26-43 KiB of LDS, so 3-4
+// workgroups share a CU and hide each other's load latency: 8192-mask tiles
+// (121 VGPRs, 2 workgroups per CU) took 1.53 ms for the two scatters, 4096
+// take 1.30 ms and 2048 (runs half as long, twice the digit table) 1.50 ms.
 #include "common.hpp"
 
 #include <algorithm>
@@ -26,10 +34,10 @@
 namespace bmmhip {
 namespace {
 
-constexpr int kRsTile = 8192;   // masks per tile
+constexpr int kRsTile = 4096;   // masks per tile
 constexpr int kRsWaves = 8;     // waves of a scatter workgroup
 constexpr int kRsThreads = kRsWaves * kWave;
-constexpr int kRsSlices = kRsTile / kRsThreads;   // 16 slices of 64 per wave
+constexpr int kRsSlices = kRsTile / kRsThreads;   // 8 slices of 64 per wave
 constexpr int kScanBlock = 1024;
 constexpr int kLdsBuckets = 16384;
 
@@ -177,23 +185,38 @@ __global__ __launch_bounds__(256) void radix_hist_kernel(const int32_t* __restri
         hist[(size_t)threadIdx.x * n_tiles + blockIdx.x] = c[threadIdx.x];
 }
 
+// Where a scatter pass takes its values from: the entry index i (the masks,
+// on the first pass), an array, or the low vbits bits of the key (a packed
+// word mask | rest-of-score << vbits, written by a packing first pass).
+enum : int { kValIndex = 0, kValArray = 1, kValInKey = 2 };
+constexpr int kRsTileBits = 12;   // log2 kRsTile
+static_assert((1 << kRsTileBits) == kRsTile, "tile size");
+
 // Stable scatter of one digit pass over the entries lo <= i < hi (tile t:
 // i in [t kRsTile, (t + 1) kRsTile)); start = the scanned digit-major table.
-// vals_in == nullptr stands for the identity values i (the masks, on the
-// first pass); keys_out == nullptr on the last pass.
-template <typename KI>
+// keys_out == nullptr writes values only (the last pass, or a packing pass).
+// pack_out: the value written is val | (key >> (shift + dbits)) << vbits,
+// the score bits later passes need stored above the mask.
+//
+// ONE: the tile is staged in LDS as one word per entry, which halves the
+// workgroup's LDS (more workgroups per CU hide the load and LDS latency):
+// the key itself when the value is part of it, or the entry's offset in the
+// tile | key << 12 for index values (keys below 2^20).
+template <typename KI, int VMODE, bool ONE>
 __global__ __launch_bounds__(kRsThreads) void radix_scatter_kernel(
     const KI* __restrict__ keys_in, const int32_t* __restrict__ vals_in, uint64_t lo, uint64_t hi,
     int shift, int dbits, const uint32_t* __restrict__ start, uint32_t n_tiles,
-    int32_t* __restrict__ keys_out, int32_t* __restrict__ vals_out) {
+    int32_t* __restrict__ keys_out, int32_t* __restrict__ vals_out, int vbits, int pack_out) {
+    static_assert(!ONE || VMODE != kValArray, "array values need their own stage");
     __shared__ int32_t skey[kRsTile];
-    __shared__ int32_t sval[kRsTile];
+    __shared__ int32_t sval[ONE ? 1 : kRsTile];
     __shared__ uint32_t run[kRsWaves][256];   // per-wave digit counts, then wave offsets
     __shared__ uint32_t lstart[256];          // tile-local start of each digit
     __shared__ uint32_t gstart[256];          // global start of each digit's run
     __shared__ uint32_t wsum[kRsWaves];
     const int tid = threadIdx.x, wave = tid / kWave, lane = tid % kWave;
     const uint32_t dmask = (1u << dbits) - 1u;
+    const uint32_t vmask = (uint32_t)((1ull << vbits) - 1ull);
     // Neighbouring tiles write neighbouring parts of each digit's region:
     // they go to one XCD.
     const uint32_t tile = xcd_tile(blockIdx.x, n_tiles);
@@ -202,35 +225,50 @@ __global__ __launch_bounds__(kRsThreads) void radix_scatter_kernel(
         (&run[0][0])[i] = 0;
     if (tid < 256)
         gstart[tid] = (uint32_t)tid <= dmask ? start[(size_t)tid * n_tiles + tile] : 0u;
-    // Wave w owns the tile's entries [1024 w, 1024 w + 1024), slice k of
-    // them at lanes 0..63.
-    const uint64_t wbase = tile0 + (uint64_t)wave * (kRsSlices * kWave) + lane;
+    // Wave w owns the tile's entries [512 w, 512 w + 512), slice k of
+    // them at lanes 0..63; offsets in the tile are 32-bit, valid in
+    // [r_lo, r_hi).
+    const uint32_t rbase = (uint32_t)wave * (kRsSlices * kWave) + lane;
+    const uint32_t r_lo = lo > tile0 ? (uint32_t)(lo - tile0) : 0u;
+    const uint32_t r_hi = hi - tile0 < (uint64_t)kRsTile ? (uint32_t)(hi - tile0) : kRsTile;
     int32_t key[kRsSlices], val[kRsSlices];
     uint32_t pos[kRsSlices];
 #pragma unroll
     for (int k = 0; k < kRsSlices; ++k) {
-        const uint64_t i = wbase + (uint64_t)k * kWave;
-        key[k] = i >= lo && i < hi ? (int32_t)keys_in[i] : 0;
+        const uint32_t r = rbase + k * kWave;
+        key[k] = r >= r_lo && r < r_hi ? (int32_t)keys_in[tile0 + r] : 0;
     }
-    // The branch on vals_in stays outside the unrolled loops (a select per
-    // slice measured 1.7x slower).
-    if (vals_in) {
+    // The value source is a template parameter: a select per slice inside
+    // the unrolled loops measured 1.7x slower.
+    if constexpr (!ONE) {
+        if constexpr (VMODE == kValArray) {
 #pragma unroll
-        for (int k = 0; k < kRsSlices; ++k) {
-            const uint64_t i = wbase + (uint64_t)k * kWave;
-            val[k] = i >= lo && i < hi ? vals_in[i] : 0;
+            for (int k = 0; k < kRsSlices; ++k) {
+                const uint32_t r = rbase + k * kWave;
+                val[k] = r >= r_lo && r < r_hi ? vals_in[tile0 + r] : 0;
+            }
+        } else if constexpr (VMODE == kValIndex) {
+#pragma unroll
+            for (int k = 0; k < kRsSlices; ++k)
+                val[k] = (int32_t)((uint32_t)tile0 + rbase + k * kWave);
+        } else if constexpr (VMODE == kValInKey) {
+#pragma unroll
+            for (int k = 0; k < kRsSlices; ++k)
+                val[k] = (int32_t)((uint32_t)key[k] & vmask);
         }
-    } else {
+        if (pack_out) {
 #pragma unroll
-        for (int k = 0; k < kRsSlices; ++k)
-            val[k] = (int32_t)(wbase + (uint64_t)k * kWave);
+            for (int k = 0; k < kRsSlices; ++k)
+                val[k] = (int32_t)((uint32_t)val[k] |
+                                   (((uint32_t)key[k] >> (shift + dbits)) << vbits));
+        }
     }
     __syncthreads();
     const uint64_t below = (1ull << lane) - 1ull;
 #pragma unroll
     for (int k = 0; k < kRsSlices; ++k) {
-        const uint64_t i = wbase + (uint64_t)k * kWave;
-        const bool valid = i >= lo && i < hi;
+        const uint32_t r = rbase + k * kWave;
+        const bool valid = r >= r_lo && r < r_hi;
         const uint32_t d = ((uint32_t)key[k] >> shift) & dmask;
         const uint64_t match = match_digit(d, dbits, valid);
         const uint32_t rank = (uint32_t)__popcll(match & below);
@@ -273,37 +311,48 @@ __global__ __launch_bounds__(kRsThreads) void radix_scatter_kernel(
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < kRsSlices; ++k) {
-        const uint64_t i = wbase + (uint64_t)k * kWave;
-        if (i >= lo && i < hi) {
+        const uint32_t r = rbase + k * kWave;
+        if (r >= r_lo && r < r_hi) {
             const uint32_t d = ((uint32_t)key[k] >> shift) & dmask;
             const uint32_t slot = lstart[d] + run[wave][d] + pos[k];
-            skey[slot] = key[k];
-            sval[slot] = val[k];
+            if constexpr (!ONE) {
+                skey[slot] = key[k];
+                sval[slot] = val[k];
+            } else if constexpr (VMODE == kValIndex) {
+                skey[slot] = (int32_t)(r | ((uint32_t)key[k] << kRsTileBits));
+            } else {
+                skey[slot] = key[k];
+            }
         }
     }
     __syncthreads();
     // Valid entries of the tile: slots 0 .. tile_n-1.
-    const uint64_t tile1 = tile0 + kRsTile;
-    const uint32_t tile_n = (uint32_t)((hi < tile1 ? hi : tile1) - (lo > tile0 ? lo : tile0));
-    // Destinations of slots tid + kRsThreads q, then one output array at a
-    // time.
-    uint32_t g[kRsSlices];
+    const uint32_t tile_n = r_hi - r_lo;
+    // Slot tid + kRsThreads q goes to the start of its digit's run plus its
+    // offset in the run: consecutive lanes write consecutive addresses.
 #pragma unroll
     for (int q = 0; q < kRsSlices; ++q) {
         const uint32_t j = tid + (uint32_t)kRsThreads * q;
-        const uint32_t d = ((uint32_t)skey[j < tile_n ? j : 0] >> shift) & dmask;
-        g[q] = gstart[d] + (j - lstart[d]);
+        if (j >= tile_n)
+            continue;
+        const uint32_t w = (uint32_t)skey[j];
+        const uint32_t k = ONE && VMODE == kValIndex ? w >> kRsTileBits : w;   // the key
+        const uint32_t d = (k >> shift) & dmask;
+        const uint32_t g = gstart[d] + (j - lstart[d]);
+        uint32_t v;
+        if constexpr (!ONE) {
+            v = (uint32_t)sval[j];
+        } else if constexpr (VMODE == kValIndex) {
+            v = (uint32_t)tile0 + (w & (kRsTile - 1));
+            if (pack_out)
+                v |= (k >> (shift + dbits)) << vbits;
+        } else {
+            v = w & vmask;
+        }
+        if (keys_out)
+            keys_out[g] = (int32_t)k;
+        vals_out[g] = (int32_t)v;
     }
-    if (keys_out) {
-#pragma unroll
-        for (int q = 0; q < kRsSlices; ++q)
-            if (tid + (uint32_t)kRsThreads * q < tile_n)
-                keys_out[g[q]] = skey[tid + kRsThreads * q];
-    }
-#pragma unroll
-    for (int q = 0; q < kRsSlices; ++q)
-        if (tid + (uint32_t)kRsThreads * q < tile_n)
-            vals_out[g[q]] = sval[tid + kRsThreads * q];
 }
 
 // Scan of blocks of 1024 entries in place (exclusive or inclusive);
@@ -390,10 +439,23 @@ void scores_order(const S* h, size_t m, int32_t n, S* s_by_mask, int32_t* order,
     // positions 0 .. N-1.
     const uint32_t tiles_u = (uint32_t)((m + kRsTile - 1) / kRsTile);
     const uint32_t tiles_p = (uint32_t)((N + kRsTile - 1) / kRsTile);
-    // Digits: ceil(bits / 8) passes, bits split evenly.
+    // Digits: ceil(bits / 8) passes, bits split evenly.  Two passes with
+    // packing when the mask (vbits bits) and the score bits above the first
+    // digit fit one 32-bit word: the first pass then writes
+    // mask | (score >> d0) << vbits and the second reads both from it (8
+    // bytes per mask fewer than separate keys and values).  The first digit
+    // widens from half the bits up to 8 to make room.
     const int bits = n == 0 ? 1 : 32 - __builtin_clz((uint32_t)n);
     const int passes = (bits + 7) / 8;
-    const int dbits = (bits + passes - 1) / passes;
+    const int vbits = N > 0 ? 32 - __builtin_clz((uint32_t)(m - 1)) : 0;
+    int dbits = (bits + passes - 1) / passes;
+    bool packed = false;
+    if (passes == 2)
+        for (int d = dbits; d <= 8 && !packed; ++d)
+            if (vbits + (bits - d) <= 32) {
+                dbits = d;
+                packed = true;
+            }
     S* d_h = ws.h.get<S>(m);
     S* d_s = ws.s.get<S>(m);
     uint32_t* d_cnt = ws.cnt.get<uint32_t>(buckets);
@@ -421,32 +483,48 @@ void scores_order(const S* h, size_t m, int32_t n, S* s_by_mask, int32_t* order,
         ScopedKernelTimer timer(nullptr);   // the radix passes
         int32_t* keys[2] = {ws.keys[0].get<int32_t>(passes > 1 ? (size_t)N : 1),
                             ws.keys[1].get<int32_t>(passes > 2 ? (size_t)N : 1)};
-        int32_t* vals[2] = {ws.vals[0].get<int32_t>(passes > 1 ? (size_t)N : 1),
+        int32_t* vals[2] = {ws.vals[0].get<int32_t>(passes > 1 && !packed ? (size_t)N : 1),
                             ws.vals[1].get<int32_t>(passes > 2 ? (size_t)N : 1)};
         const int32_t* kin = nullptr;
         const int32_t* vin = nullptr;
         for (int p = 0; p < passes; ++p) {
-            const int shift = p * dbits, db = std::min(dbits, bits - shift);
+            // Packed: the second digit sits at bit vbits of the packed word.
+            const int shift = packed ? (p == 0 ? 0 : vbits) : p * dbits;
+            const int db = packed ? (p == 0 ? dbits : bits - dbits)
+                                  : std::min(dbits, bits - shift);
             const bool last = p == passes - 1;
             const uint32_t tiles = p == 0 ? tiles_u : tiles_p;
-            int32_t* kout = last ? nullptr : keys[p & 1];
-            int32_t* vout = last ? d_order : vals[p & 1];
+            int32_t* kout = last || packed ? nullptr : keys[p & 1];
+            int32_t* vout = last ? d_order : packed ? keys[0] : vals[p & 1];
             if (p > 0) {
                 hipLaunchKernelGGL(radix_hist_kernel, dim3(tiles), dim3(256), 0, nullptr, kin,
                                    (uint32_t)N, shift, db, tiles, d_hist);
                 BMM_HIP(hipGetLastError());
             }
             scan_u32(d_hist, ((uint64_t)1 << db) * tiles, false, ws.levels, 0, nullptr);
-            if (p == 0)   // the scores of masks 1 .. m-1, the masks as values
-                hipLaunchKernelGGL(radix_scatter_kernel<S>, dim3(tiles), dim3(kRsThreads), 0,
-                                   nullptr, d_s, nullptr, (uint64_t)1, (uint64_t)m, shift, db,
-                                   d_hist, tiles, kout, vout);
+            // The scores of masks 1 .. m-1 with the masks as values (one LDS
+            // word per entry for scores below 2^20), the packed words, or
+            // keys and values.
+            if (p == 0 && bits <= 32 - kRsTileBits)
+                hipLaunchKernelGGL((radix_scatter_kernel<S, kValIndex, true>), dim3(tiles),
+                                   dim3(kRsThreads), 0, nullptr, d_s, nullptr, (uint64_t)1,
+                                   (uint64_t)m, shift, db, d_hist, tiles, kout, vout, vbits,
+                                   packed ? 1 : 0);
+            else if (p == 0)
+                hipLaunchKernelGGL((radix_scatter_kernel<S, kValIndex, false>), dim3(tiles),
+                                   dim3(kRsThreads), 0, nullptr, d_s, nullptr, (uint64_t)1,
+                                   (uint64_t)m, shift, db, d_hist, tiles, kout, vout, vbits,
+                                   packed ? 1 : 0);
+            else if (packed)
+                hipLaunchKernelGGL((radix_scatter_kernel<int32_t, kValInKey, true>), dim3(tiles),
+                                   dim3(kRsThreads), 0, nullptr, kin, nullptr, (uint64_t)0, N,
+                                   shift, db, d_hist, tiles, kout, vout, vbits, 0);
             else
-                hipLaunchKernelGGL(radix_scatter_kernel<int32_t>, dim3(tiles), dim3(kRsThreads), 0,
-                                   nullptr, kin, vin, (uint64_t)0, N, shift, db, d_hist, tiles,
-                                   kout, vout);
+                hipLaunchKernelGGL((radix_scatter_kernel<int32_t, kValArray, false>), dim3(tiles),
+                                   dim3(kRsThreads), 0, nullptr, kin, vin, (uint64_t)0, N, shift,
+                                   db, d_hist, tiles, kout, vout, vbits, 0);
             BMM_HIP(hipGetLastError());
-            kin = kout;
+            kin = packed ? vout : kout;
             vin = vout;
         }
     }
