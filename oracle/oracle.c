@@ -19,7 +19,7 @@
  *   or_bruteforce_topk reference golden (16x512/1024/4096, k=8/128); untruncated
  *                      tail popcount checked against a planted tail bit.
  *   or_coset_min       equals the first target-containing entry of the reference
- *                      FwhtSearch full ranking (C1 golden); brute force t<=12.
+ *                      FwhtSearch full ranking (20x128 golden); brute force t<=12.
  *   or_gray_min        min weight and lowest mask equal the reference FwhtSearch
  *                      ranking of [block..., target]; first Gray step equals the
  *                      minimum inverse-Gray index over all minimising masks.

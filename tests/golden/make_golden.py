@@ -122,7 +122,7 @@ def main():
                        "q": [int(v) for v in q], "r": [int(v) for v in r],
                        "mult": [int(v) for v in mult]})
 
-    # --- C1: 20x128 coset minimum from the full reference ranking -----------
+    # --- 20x128 coset minimum from the full reference ranking ---------------
     c1 = ora.bernoulli(SEED_C1, 20, 128, 0.5)
     ws, masks = ref.fwht_search_arrays(c1, 128, (1 << 20) - 1)
     c1_res = [first_containing(ws, masks, i) for i in range(20)]
@@ -130,7 +130,7 @@ def main():
     out["c1_weight"] = np.array([w for w, _ in c1_res], np.uint32)
     out["c1_mask"] = np.array([mk for _, mk in c1_res], np.uint64)
 
-    # --- C2: Gray sweep minima for four targets via the reference ranking ---
+    # --- Gray sweep minima for four targets via the reference ranking -------
     c2 = ora.bernoulli(SEED_C2, 32, 1024, 0.3)
     c2_t, c2_w, c2_s, c2_m = [], [], [], []
     for target in (0, 5, 24, 31):
@@ -152,7 +152,7 @@ def main():
     out["c2_first"] = np.array(c2_s, np.uint64)
     out["c2_mask"] = np.array(c2_m, np.uint64)
 
-    # --- C3: planted instance, reduced windows through reference MITM ------
+    # --- planted instance, reduced windows through reference MITM ----------
     S, A, M = ora.planted(SEED_C3, 48, 4096, 0.02)
     out["c3_sparse_weights"] = np.array([bin(int(x)).count("1") for r in S for x in r],
                                         np.uint32).reshape(48, -1).sum(1).astype(np.uint32)
@@ -165,7 +165,7 @@ def main():
         c3_cases.append({"target": target, "others": others,
                          "result": [[w, str(mk)] for w, mk in res]})
 
-    # --- C4: batched uneven workload, small matrices via reference ranking --
+    # --- batched uneven workload, small matrices via reference ranking ------
     rows, cols, dens, words, offs = ora.batch(SEED_C4, 64)
     c4 = []
     for mi in range(64):

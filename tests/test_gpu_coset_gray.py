@@ -106,8 +106,8 @@ def test_coset_zero_rows_and_duplicates(lib, ora):
 
 
 def test_coset_batch_matches_per_matrix(lib, ora):
-    """Batched uneven workload (C4 generator), matrices with <= 20 rows checked
-    one by one against the oracle."""
+    """Batched uneven workload (the 4096-matrix generator), matrices with
+    <= 20 rows checked one by one against the oracle."""
     _say("coset batch")
     rows, cols, dens, words, offs = lib.gen_batch(SEED_C4, 256)
     w, idx = lib.coset_min_batch(words, offs, rows, cols)

@@ -1,6 +1,6 @@
 """GPU checks of the BitMatrix surface that runs on the device (power through
 the GF(2) row-combination kernel) and of batched coset minima read from
-BMMPYB1 files: the same result as from the in-memory batch, which the C4
+BMMPYB1 files: the same result as from the in-memory batch, which the batch
 parity tests pin to the oracle."""
 import numpy as np
 import pytest
