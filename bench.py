@@ -51,7 +51,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "subset evals/s (FWHT, Gray); GA time-to-1%-of-planted, learned vs plain"
-SEED_C2 = 0x5EED0002
+SEED_GRAY_32X1024 = 0x5EED0002
 ROWS, COLS, P, BLOCK = 32, 1024, 0.3, 24
 GRAY_OPS_PER_EVAL = 66      # 32 v_xor_b32 + 32 v_bcnt_u32_b32 + 2 key minima
 VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12  # INT32 VALU, non-packed
@@ -71,7 +71,7 @@ TRAFFIC_BYTES = {
 }
 
 
-def c2_evals() -> int:
+def gray_sweep_evals() -> int:
     return sum((1 << (BLOCK - 1)) if t < BLOCK else (1 << BLOCK) for t in range(ROWS))
 
 
@@ -767,7 +767,7 @@ def main() -> int:
         print(json.dumps(mode_line(args, r, cpu)))
         return 0
 
-    matrix = _lib.gen_bernoulli(SEED_C2, ROWS, COLS, P)
+    matrix = _lib.gen_bernoulli(SEED_GRAY_32X1024, ROWS, COLS, P)
     d_rows = torch.from_numpy(matrix.view(np.int64).copy()).to(dev)
     targets = np.arange(ROWS, dtype=np.int32)
     d_first = torch.empty(ROWS, dtype=torch.int64, device=dev)
@@ -795,7 +795,7 @@ def main() -> int:
 
     # Results of the last pass (checked against the oracle by the tests).
     keys = d_first.cpu().numpy().view(np.uint64)
-    evals = c2_evals()
+    evals = gray_sweep_evals()
     value = evals / (ms / 1e3)
     achieved_tops = evals * GRAY_OPS_PER_EVAL / (kernel_avg_ms / 1e3) / 1e12
     out = {
@@ -812,7 +812,7 @@ def main() -> int:
         "dtype": "u32",
         "data": "synthetic",
         "config": {"workload": "gray_coset_sweep", "rows": ROWS, "cols": COLS, "p": P,
-                   "pivot_block": BLOCK, "targets": ROWS, "seed": hex(SEED_C2),
+                   "pivot_block": BLOCK, "targets": ROWS, "seed": hex(SEED_GRAY_32X1024),
                    "subset_evals_per_step": evals},
         "roofline": {"bound": "valu", "achieved": achieved_tops, "peak": VALU_PEAK_TOPS,
                      "unit": "TOPS (int32 VALU)", "frac": achieved_tops / VALU_PEAK_TOPS,

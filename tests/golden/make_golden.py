@@ -22,7 +22,7 @@ sys.path.insert(0, ROOT)
 
 import oracle  # noqa: E402
 
-SEED_C1, SEED_C2, SEED_C3, SEED_C4 = 0x5EED0001, 0x5EED0002, 0x5EED0003, 0x5EED0004
+SEED_COSET_20X128, SEED_GRAY_32X1024, SEED_PLANTED_48X4096, SEED_BATCH_4096 = 0x5EED0001, 0x5EED0002, 0x5EED0003, 0x5EED0004
 
 
 def ginv(g: int) -> int:
@@ -123,7 +123,7 @@ def main():
                        "mult": [int(v) for v in mult]})
 
     # --- 20x128 coset minimum from the full reference ranking ---------------
-    c1 = ora.bernoulli(SEED_C1, 20, 128, 0.5)
+    c1 = ora.bernoulli(SEED_COSET_20X128, 20, 128, 0.5)
     ws, masks = ref.fwht_search_arrays(c1, 128, (1 << 20) - 1)
     c1_res = [first_containing(ws, masks, i) for i in range(20)]
     out["c1_rows"] = c1
@@ -131,7 +131,7 @@ def main():
     out["c1_mask"] = np.array([mk for _, mk in c1_res], np.uint64)
 
     # --- Gray sweep minima for four targets via the reference ranking -------
-    c2 = ora.bernoulli(SEED_C2, 32, 1024, 0.3)
+    c2 = ora.bernoulli(SEED_GRAY_32X1024, 32, 1024, 0.3)
     c2_t, c2_w, c2_s, c2_m = [], [], [], []
     for target in (0, 5, 24, 31):
         blk = [r for r in range(24) if r != target]
@@ -153,7 +153,7 @@ def main():
     out["c2_mask"] = np.array(c2_m, np.uint64)
 
     # --- planted instance, reduced windows through reference MITM ----------
-    S, A, M = ora.planted(SEED_C3, 48, 4096, 0.02)
+    S, A, M = ora.planted(SEED_PLANTED_48X4096, 48, 4096, 0.02)
     out["c3_sparse_weights"] = np.array([bin(int(x)).count("1") for r in S for x in r],
                                         np.uint32).reshape(48, -1).sum(1).astype(np.uint32)
     out["c3_transform"] = M
@@ -166,7 +166,7 @@ def main():
                          "result": [[w, str(mk)] for w, mk in res]})
 
     # --- batched uneven workload, small matrices via reference ranking ------
-    rows, cols, dens, words, offs = ora.batch(SEED_C4, 64)
+    rows, cols, dens, words, offs = ora.batch(SEED_BATCH_4096, 64)
     c4 = []
     for mi in range(64):
         if rows[mi] > 16:

@@ -5,7 +5,7 @@ import pytest
 
 pytestmark = pytest.mark.gpu
 
-SEED_C1, SEED_C2, SEED_C4 = 0x5EED0001, 0x5EED0002, 0x5EED0004
+SEED_COSET_20X128, SEED_GRAY_32X1024, SEED_BATCH_4096 = 0x5EED0001, 0x5EED0002, 0x5EED0004
 
 
 def _say(*a):
@@ -55,7 +55,7 @@ def test_fwht_dev_unaligned_array(lib, ora, n_bits):
 def test_gray_c2_full_config(lib, ora):
     """32x1024 Bernoulli(0.3); every target against rows 0..23 minus itself."""
     _say("gray c2 full")
-    m = lib.gen_bernoulli(SEED_C2, 32, 1024, 0.3)
+    m = lib.gen_bernoulli(SEED_GRAY_32X1024, 32, 1024, 0.3)
     targets = np.arange(32, dtype=np.int32)
     w, first, low = lib.gray_coset_min(m, 24, targets)
     for t in range(32):
@@ -77,7 +77,7 @@ def test_gray_other_widths(lib, ora, rows, cols, p, block):
 def test_coset_c1_full_config(lib, ora):
     """20x128 Bernoulli(0.5): per-target min and argmin over the other 19 rows."""
     _say("coset c1")
-    m = lib.gen_bernoulli(SEED_C1, 20, 128, 0.5)
+    m = lib.gen_bernoulli(SEED_COSET_20X128, 20, 128, 0.5)
     w, idx = lib.coset_min(m, 128)
     ew, eidx = ora.coset_min(m, 128)
     assert np.array_equal(w, ew)
@@ -109,7 +109,7 @@ def test_coset_batch_matches_per_matrix(lib, ora):
     """Batched uneven workload (the 4096-matrix generator), matrices with
     <= 20 rows checked one by one against the oracle."""
     _say("coset batch")
-    rows, cols, dens, words, offs = lib.gen_batch(SEED_C4, 256)
+    rows, cols, dens, words, offs = lib.gen_batch(SEED_BATCH_4096, 256)
     w, idx = lib.coset_min_batch(words, offs, rows, cols)
     out_off = np.concatenate([[0], np.cumsum(rows)[:-1]])
     checked = 0
@@ -216,7 +216,7 @@ def test_batch_plan_block_sizes(lib, ora, bits):
     against the oracle on the matrices of <= 20 rows, and against the 2^13
     result on all matrices (up to 28 rows), eager and graph replay."""
     _say("batch block size", bits)
-    rows, cols, dens, words, offs = lib.gen_batch(SEED_C4, 160)
+    rows, cols, dens, words, offs = lib.gen_batch(SEED_BATCH_4096, 160)
     w, i = _plan_run(lib, words, offs, rows, cols, bits, graph=bits == 14)
     assert _check_small(ora, words, offs, rows, cols, w, i, 20) > 60
     w13, i13 = _plan_run(lib, words, offs, rows, cols, 13, graph=False)

@@ -7,7 +7,7 @@ import pytest
 
 pytestmark = pytest.mark.gpu
 
-SEED_C3 = 0x5EED0003
+SEED_PLANTED_48X4096 = 0x5EED0003
 
 
 def _say(*a):
@@ -23,7 +23,7 @@ def _oracle_window(ora, A, cols, target, window):
 @pytest.mark.parametrize("width", [1, 5, 13, 14, 18, 21])
 def test_window_coset_vs_oracle(lib, ora, width):
     _say("window coset", width)
-    S, A, M = lib.gen_planted(SEED_C3, 48, 4096, 0.02)
+    S, A, M = lib.gen_planted(SEED_PLANTED_48X4096, 48, 4096, 0.02)
     targets = [0, 17, 47]
     windows = [[r for r in range(48) if r != t][:width] for t in targets]
     w, m = lib.window_coset_min(A, 4096, targets, windows)
@@ -55,7 +55,7 @@ def test_planted_40_wide_properties(lib):
     the target's own weight (empty subset) and the returned mask reproduces
     the returned weight."""
     _say("planted 40 wide")
-    S, A, M = lib.gen_planted(SEED_C3, 48, 4096, 0.02)
+    S, A, M = lib.gen_planted(SEED_PLANTED_48X4096, 48, 4096, 0.02)
     targets = [0, 47]
     windows = [[r for r in range(48) if r != t][:40] for t in targets]
     w, mk = lib.window_coset_min(A, 4096, targets, windows)
