@@ -66,8 +66,8 @@ TRAFFIC_BYTES = {
     "coset_batch_4096": (53729.0 * 2.0 + 29953.0) * 1024,
     "window_coset_planted_48x4096": (118.4 * 2.0 + 131137.0) * 1024,
     # The four kernels of one calc_scores_and_order call on 2^28 entries.
-    "scores_order_2p28": ((524576.3 + 537970.3 + 524304.9 + 526392.4) * 2.0
-                          + 1231680.9 + 1048635.2 + 34002.3 + 1048579.9) * 1024,
+    "scores_order_2p28": ((524558.6 + 537199.4 + 524304.9 + 526393.4) * 2.0
+                          + 1091265.5 + 1048633.4 + 34645.5 + 1048580.2) * 1024,
 }
 
 

@@ -77,7 +77,15 @@ __global__ __launch_bounds__(256) void scores_kernel(const S* __restrict__ h, ui
     if (lds_hist)
         for (int q = threadIdx.x; q < buckets; q += blockDim.x)
             sh_cnt[q] = 0;
-    for (uint32_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
+    for (uint32_t v = blockIdx.x; v < n_tiles; v += gridDim.x) {
+        // The grid sweeps windows of gridDim.x tiles front to back; inside a
+        // full window each XCD takes a contiguous range of tiles, so the
+        // digit table's entries for neighbouring tiles (one cache line per 16
+        // tiles and digit) are written from one L2.
+        const uint32_t w0 = v - v % gridDim.x, b = v % gridDim.x;
+        const uint32_t tile = gridDim.x % kXcds == 0 && w0 + gridDim.x <= n_tiles
+                                  ? w0 + (b % kXcds) * (gridDim.x / kXcds) + b / kXcds
+                                  : v;
         // Thread t alone reads dig[t] below and clears it here: no barrier
         // between the two.
         dig[threadIdx.x] = 0;
