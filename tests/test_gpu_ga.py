@@ -3,6 +3,8 @@ the found transform, monotone progress, ring migration and determinism."""
 import numpy as np
 import pytest
 
+from oracle.ga_li import li_rounds
+
 pytestmark = pytest.mark.gpu
 
 
@@ -82,39 +84,6 @@ def test_ga_rejects_bad_shapes(lib):
         lib.GpuGA(np.zeros((300, 32), np.uint64), 2048, lib.ga_default_params())
 
 
-def _li_reference(rows_words, sweeps=1):
-    """Parallel-round pairwise improvement, restated in Python: per round the
-    rows with index bit b == phase are targets, the others sources; a target
-    i takes the source j minimising (wt_i + wt_j - 2 |r_i & r_j|, j) if that
-    beats wt_i (genetic_algorithm_operators.cpp:131-192 with rounds)."""
-    rows = [int.from_bytes(r.tobytes(), "little") for r in rows_words]
-    n = len(rows)
-    bits = max(1, (n - 1).bit_length())
-    pc = lambda x: bin(x).count("1")   # noqa: E731
-    for _ in range(sweeps):
-        for b in range(bits):
-            for phase in (0, 1):
-                w = [pc(r) for r in rows]
-                moves = {}
-                for i in range(n):
-                    if (i >> b) & 1 != phase or w[i] == 0:
-                        continue
-                    best = None
-                    for j in range(n):
-                        if (j >> b) & 1 == phase or w[j] >= 2 * w[i]:
-                            continue
-                        cw = w[i] + w[j] - 2 * pc(rows[i] & rows[j])
-                        if cw < w[i] and (best is None or (cw, j) < best):
-                            best = (cw, j)
-                    if best:
-                        moves[i] = best[1]
-                for i, j in moves.items():
-                    rows[i] ^= rows[j]
-    W = rows_words.shape[1]
-    return np.array([[(r >> (64 * k)) & ((1 << 64) - 1) for k in range(W)] for r in rows],
-                    dtype=np.uint64)
-
-
 def test_ga_local_improvement_matches_restatement(lib):
     _say("ga local improvement")
     S, A, M = lib.gen_planted(0x5EED0005, 32, 2048, 0.05)
@@ -125,7 +94,7 @@ def test_ga_local_improvement_matches_restatement(lib):
     st = ga.run(1)
     best, w = ga.best()
     ga.close()
-    exp = _li_reference(A, 1)
+    exp = li_rounds(A, 1)
     assert st.li_moves > 0
     assert np.array_equal(best, exp)
     assert w == st.best_weight == _weight(exp)
