@@ -9,7 +9,7 @@
 // rest with Gosper's successor,
 // m' = (((r ^ m) >> 2) >> ctz(m & -m)) | r with r = m + (m & -m).  The
 // masks of a workgroup go through LDS (runs padded by one entry against bank
-// conflicts) so that the stores are coalesced.
+// conflicts) so that the stores are coalesced, 16 bytes per lane.
 #include "common.hpp"
 
 #include <algorithm>
@@ -74,9 +74,27 @@ __global__ __launch_bounds__(kCombThreads) void fixed_weight_kernel(uint32_t n, 
         }
     }
     __syncthreads();
-    for (int q = threadIdx.x; q < kCombThreads * kCombRun; q += kCombThreads)
-        if (base + (uint64_t)q < count)
-            out[base + q] = stage[(q / kCombRun) * kRow + q % kCombRun];
+    // 16-byte stores of kV consecutive masks (they lie in one thread's run);
+    // a 16-byte aligned output takes them whole, the end of the range and
+    // an unaligned output go mask by mask.
+    constexpr int kV = 16 / (int)sizeof(T);
+    static_assert(kCombRun % kV == 0, "a 16-byte group stays in one run");
+    const bool vec = (reinterpret_cast<uintptr_t>(out) & 15u) == 0;
+    for (int q = threadIdx.x * kV; q < kCombThreads * kCombRun; q += kCombThreads * kV) {
+        const uint64_t o = base + (uint64_t)q;
+        const T* src = stage + (q / kCombRun) * kRow + q % kCombRun;
+        if (vec && o + kV <= count) {
+            T v[kV];
+#pragma unroll
+            for (int e = 0; e < kV; ++e)
+                v[e] = src[e];
+            *reinterpret_cast<uint4*>(out + o) = __builtin_bit_cast(uint4, v);
+        } else {
+            for (int e = 0; e < kV; ++e)
+                if (o + e < count)
+                    out[o + e] = src[e];
+        }
+    }
 }
 
 } // namespace
