@@ -51,14 +51,15 @@ def test_scores_order_golden(lib, gglobal):
 # pack the mask and the score bits above the first digit into one 32-bit
 # word between the passes: 2^28 masks with n = 1023 widen the first digit to
 # 6 bits so that 28 + 4 bits fill the word (its top bit set for scores of
-# 512 and above); (1 << 22) + 5 masks with n = 65535 pack 23 + 8 bits.
+# 512 and above); (1 << 22) + 5 masks with n = 65535 pack 23 + 8 bits, and
+# (1 << 25) + 3 masks cannot (26 + 8 bits): separate keys and values.
 @pytest.mark.parametrize("dt,m,n", [
     (np.int32, 1, 5), (np.int32, 2, 0), (np.int32, 4097, 0), (np.int32, 65539, 255),
     (np.int32, 70001, 300), (np.int32, (1 << 20) + 3, 1023), (np.int16, 5000, 1),
     (np.int32, 65536, 256), (np.int32, 1 << 20, 16383), (np.int32, 1 << 20, 16384),
     (np.int32, (1 << 22) + 5, 65535), (np.int32, 1 << 20, 65536),
     (np.int32, 300000, (1 << 24) + 5), (np.int16, 1 << 20, 4000), (np.int16, 12345, 32767),
-    (np.int16, 1 << 16, 60000), (np.int32, 1 << 28, 1023)])
+    (np.int16, 1 << 16, 60000), (np.int32, 1 << 28, 1023), (np.int32, (1 << 25) + 3, 65535)])
 def test_scores_order_matches_oracle(lib, ora, dt, m, n):
     _say("scores order", dt.__name__, m, n)
     rng = np.random.default_rng(m ^ n)
