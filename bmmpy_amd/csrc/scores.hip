@@ -25,6 +25,9 @@
 // workgroups share a CU and hide each other's load latency: 8192-mask tiles
 // (121 VGPRs, 2 workgroups per CU) took 1.53 ms for the two scatters, 4096
 // take 1.30 ms and 2048 (runs half as long, twice the digit table) 1.50 ms.
+// The one-word variants are held to 64 VGPRs (8 waves per SIMD, 4
+// workgroups per CU): the first pass needed 65, and the bound (2 spilled
+// dwords) took it from 741 to 686 us.
 #include "common.hpp"
 
 #include <algorithm>
@@ -211,7 +214,7 @@ static_assert((1 << kRsTileBits) == kRsTile, "tile size");
 // the key itself when the value is part of it, or the entry's offset in the
 // tile | key << 12 for index values (keys below 2^20).
 template <typename KI, int VMODE, bool ONE>
-__global__ __launch_bounds__(kRsThreads) void radix_scatter_kernel(
+__global__ __launch_bounds__(kRsThreads, ONE ? 8 : 4) void radix_scatter_kernel(
     const KI* __restrict__ keys_in, const int32_t* __restrict__ vals_in, uint64_t lo, uint64_t hi,
     int shift, int dbits, const uint32_t* __restrict__ start, uint32_t n_tiles,
     int32_t* __restrict__ keys_out, int32_t* __restrict__ vals_out, int vbits, int pack_out) {
