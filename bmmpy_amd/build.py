@@ -23,7 +23,7 @@ BUILD = os.path.join(HERE, "_build")
 HIP_LIB = os.path.join(HERE, "libbmmpy_hip.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-HIP_SOURCES = ["capi.hip", "gray.hip", "fwht.hip", "search.hip", "replay.hip", "mitm.hip", "bruteforce.hip",
+HIP_SOURCES = ["capi.hip", "gray.hip", "fwht.hip", "search.hip", "replay.hip", "mitm.hip",
                "batch.hip", "ga.hip", "mlp.hip", "workloads.hip", "order.hip",
                "overlap.hip", "combine.hip", "scores.hip", "comb.hip"]
 HIP_FLAGS = [
@@ -44,9 +44,18 @@ def _run(cmd):
     return out
 
 
+def _sources(names: list[str]) -> list[str]:
+    """The listed sources; a missing one is an error (a silently skipped
+    source would leave unresolved symbols that fail only at load time)."""
+    missing = [s for s in names if not os.path.exists(os.path.join(CSRC, s))]
+    if missing:
+        raise FileNotFoundError(f"listed sources missing from {CSRC}: {missing}")
+    return list(names)
+
+
 def build_hip(jobs: int = 8) -> str:
     os.makedirs(BUILD, exist_ok=True)
-    srcs = [s for s in HIP_SOURCES if os.path.exists(os.path.join(CSRC, s))]
+    srcs = _sources(HIP_SOURCES)
     objs = [os.path.join(BUILD, s.replace(".hip", ".o")) for s in srcs]
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
         futs = [ex.submit(_run, [HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, s), "-o", o])
@@ -59,16 +68,14 @@ def build_hip(jobs: int = 8) -> str:
     return HIP_LIB
 
 
-HOST_SOURCES = ["host/bit_matrix.cpp", "host/search.cpp", "host/ga.cpp", "host/apply.cpp",
+HOST_SOURCES = ["host/bit_matrix.cpp", "host/search.cpp", "host/apply.cpp",
                 "host/runtime.cpp", "host/solver.cpp", "host/sa_selector.cpp",
                 "host/matrix_io.cpp"]
 
 
-def build_host(jobs: int = 8) -> str | None:
-    """C++ host layer + pybind11 module (skipped if its sources are absent)."""
-    srcs = [s for s in HOST_SOURCES if os.path.exists(os.path.join(CSRC, s))]
-    if not srcs:
-        return None
+def build_host(jobs: int = 8) -> str:
+    """C++ host layer + pybind11 module."""
+    srcs = _sources(HOST_SOURCES)
     import pybind11
 
     ext = sysconfig.get_config_var("EXT_SUFFIX")
