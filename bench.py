@@ -75,45 +75,78 @@ def gray_sweep_evals() -> int:
     return sum((1 << (BLOCK - 1)) if t < BLOCK else (1 << BLOCK) for t in range(ROWS))
 
 
-def cpu_baseline(matrix: np.ndarray, budget_s: float = 12.0) -> dict:
-    """Reference CPU brute force (BruteforceSearch, compiled from the reference
-    sources into oracle/_ref) on windows [block rows..., target], k = 1: each
-    window visits all 2^25 masks.  Falls back to the project's C restatement
-    (oracle/liboracle.so) where the reference build is absent."""
+def host_cpu() -> dict:
+    """The host the CPU baseline ran on: CPU model (as lscpu prints it, from
+    /proc/cpuinfo), logical CPUs in the machine and in this process's
+    affinity mask."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"cpu_model": model, "nproc": len(os.sched_getaffinity(0)),
+            "logical_cpus": os.cpu_count()}
+
+
+def _omp_threads(n: int) -> None:
+    """Sets the OpenMP team size of the reference build (libgomp is shared by
+    the process)."""
+    import ctypes
+
+    ctypes.CDLL("libgomp.so.1").omp_set_num_threads(int(n))
+
+
+def cpu_baseline(matrix: np.ndarray, budget_s: float = 4.0, one_budget_s: float = 8.0) -> dict:
+    """Reference CPU brute force (BruteforceSearch, compiled from the
+    reference sources into oracle/_ref by oracle/Makefile) on windows
+    [block rows..., target], k = 1: each window visits all 2^25 masks, OpenMP
+    over prefixes.  Timed with all the process's cores (windows repeated for
+    about `budget_s` seconds) and with one thread (`one_budget_s`).  Where
+    the reference build is absent the project's C restatement (one thread)
+    runs instead, and `kind` / `path` say so."""
     import oracle  # checker / baseline only
 
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    host = host_cpu()
     ref = oracle.reference()
+    wins = [np.concatenate([matrix[:BLOCK], matrix[t:t + 1]]) for t in range(BLOCK, ROWS)]
     if ref is not None:
-        evals = 0
-        t0 = time.perf_counter()
-        done = 0
-        for target in range(BLOCK, ROWS):
-            win = np.concatenate([matrix[:BLOCK], matrix[target:target + 1]])
-            ref.bruteforce_search(win, COLS, 1)
-            evals += 1 << (BLOCK + 1)
-            done += 1
-            if time.perf_counter() - t0 > budget_s:
-                break
-        dt = time.perf_counter() - t0
-        return {"value": evals / dt, "unit": "subset evals/s", "cores": threads,
-                "kind": "reference",
-                "sample": f"reference BruteforceSearch(k=1) on {done} windows of 25 rows "
-                          f"(24-row pivot block + target) x 1024 cols, 2^25 masks each, "
-                          f"OpenMP over prefixes"}
+        def run(threads: int, budget: float):
+            _omp_threads(threads)
+            evals, done = 0, 0
+            t0 = time.perf_counter()
+            while time.perf_counter() - t0 < budget:
+                ref.bruteforce_search(wins[done % len(wins)], COLS, 1)
+                evals += 1 << (BLOCK + 1)
+                done += 1
+            return evals / (time.perf_counter() - t0), done
+
+        threads = host["nproc"]
+        v_all, n_all = run(threads, budget_s)
+        v_one, n_one = run(1, one_budget_s)
+        _omp_threads(threads)
+        return {"value": v_all, "unit": "subset evals/s", "cores": threads, "kind": "reference",
+                "path": "oracle/_ref/libbmmref.so: reference BruteforceSearch",
+                "sample": f"reference BruteforceSearch(k=1) on windows of 25 rows (24-row pivot "
+                          f"block + target) x 1024 cols, 2^25 masks each, OpenMP over prefixes: "
+                          f"{n_all} windows on {threads} threads, {n_one} on 1 thread",
+                "one_thread": {"value": v_one, "windows": n_one}, **host}
     ora = oracle.oracle()
     t0 = time.perf_counter()
     evals = 0
     done = 0
-    for target in range(BLOCK, ROWS):
-        ora.gray_min(matrix, BLOCK, target)
+    while time.perf_counter() - t0 < one_budget_s:
+        ora.gray_min(matrix, BLOCK, BLOCK + done % (ROWS - BLOCK))
         evals += 1 << BLOCK
         done += 1
-        if time.perf_counter() - t0 > budget_s:
-            break
     dt = time.perf_counter() - t0
     return {"value": evals / dt, "unit": "subset evals/s", "cores": 1, "kind": "port",
-            "sample": f"C restatement Gray sweep, {done} targets x 2^24 subsets"}
+            "path": "oracle/liboracle.so: C restatement (reference build absent on this host)",
+            "sample": f"C restatement Gray sweep, {done} targets x 2^24 subsets, one thread",
+            "one_thread": {"value": evals / dt, "windows": done}, **host}
 
 
 def _time(torch, stream, fn, reps):
@@ -593,9 +626,26 @@ def comb_cpu_baseline(budget_s: float = 10.0) -> dict:
 
 
 def secondary(torch, _lib, dev, stream) -> dict:
-    """The FWHT coset-minimum workloads, timed beside the headline (graph
-    replay, autotuned launch shape)."""
+    """The other configured workloads, timed beside the headline: the FWHT
+    coset minima (graph replay, autotuned launch shape), one 2^40 window of
+    the planted 48 x 4096 instance, and the plain and learned island GA on
+    the planted 256 x 8192 instance until 1.01 x the planted weight."""
     res = {}
+    r = run_window(torch, _lib, 1, 1)
+    res["window_48x4096"] = {"subset_evals": r["evals"], "ms_per_target": r["ms"],
+                             "subset_evals_per_s": r["evals"] / (r["ms"] / 1e3),
+                             "roofline_frac": r["roofline"]["frac"],
+                             "kernel_avg_ms": r["roofline"]["kernel_avg_ms"],
+                             "result": r["results"][0]}
+    g = ga_compare(20.0, isd_trials=1)
+    res["ga_256x8192"] = {
+        "planted_weight": g["planted_weight"], "start_weight": g["start_weight"],
+        "target": g["target"],
+        **{f"{k}_{f}": g[k][f] for k in ("plain", "learned")
+           for f in ("best_weight", "time_to_target_s", "generations", "ms_per_generation")},
+        "learned_over_plain_time_ratio":
+            (g["learned"]["time_to_target_s"] / g["plain"]["time_to_target_s"]
+             if g["plain"]["time_to_target_s"] and g["learned"]["time_to_target_s"] else None)}
     for name, key in (("coset", "coset_20x128"), ("batch", "batch_4096")):
         r = run_batch_plan(torch, _lib, dev, stream, name, 50 if name == "coset" else 5, 2)
         res[key] = {"subset_evals": r["evals"], "ms_per_pass": r["ms"],
@@ -624,21 +674,25 @@ def mode_line(args, r: dict, cpu: dict | None) -> dict:
     return out
 
 
-def ga_compare(budget_s: float, seed: int = 0x5EED0005, isd_trials: int = 0) -> dict:
+def ga_compare(budget_s: float, seed: int = 0x5EED0005, isd_trials: int = 0,
+               ga_seed: int | None = None, **overrides) -> dict:
     """Planted 256 x 8192 matrix (Bernoulli(0.01) sparse S, random invertible
-    256 x 256 transform M, A = M S); 64 islands x 256 individuals x 512 ops,
-    ring migration every 20 generations.  Plain and learned proposal runs get
-    the same wall-clock budget; reported: weight reached, time to reach
-    1.01 * wt(S) (null when not reached) and the trajectory."""
+    256 x 256 transform M, A = M S, drawn from `seed`); 64 islands x 256
+    individuals x 512 ops, ring migration every 20 generations, GA random
+    streams from `ga_seed` (default: `seed`).  Plain and learned proposal
+    runs get the same wall-clock budget; reported: weight reached, time to
+    reach 1.01 * wt(S) (null when not reached) and the trajectory."""
     from bmmpy_amd import _lib
 
     S, A, _ = _lib.gen_planted(seed, 256, 8192, 0.01)
+    ga_seed = seed if ga_seed is None else ga_seed
     planted = int(sum(bin(int(v)).count("1") for v in S.reshape(-1)))
     start_w = int(sum(bin(int(v)).count("1") for v in A.reshape(-1)))
     target = int(planted * 1.01)
     res = {"planted_weight": planted, "start_weight": start_w, "target": target}
     for name, learned in (("plain", 0), ("learned", 1)):
-        p = _lib.ga_default_params(learned=learned, seed=seed, isd_trials=isd_trials)
+        p = _lib.ga_default_params(learned=learned, seed=ga_seed, isd_trials=isd_trials,
+                                   **overrides)
         t0 = time.perf_counter()
         ga = _lib.GpuGA(A, 8192, p)
         setup = time.perf_counter() - t0

@@ -59,7 +59,17 @@ class GaParams(C.Structure):
                 ("li_sweeps", C.c_int), ("rows", C.c_int), ("p_append", C.c_float),
                 ("p_replace", C.c_float), ("p_delete", C.c_float), ("learning_rate", C.c_float),
                 ("p_learned", C.c_float), ("seed", C.c_uint64), ("win_rows", C.c_int),
-                ("win_rounds", C.c_int), ("win_candidates", C.c_int), ("isd_trials", C.c_int)]
+                ("win_rounds", C.c_int), ("win_candidates", C.c_int), ("isd_trials", C.c_int),
+                ("num_parents", C.c_int), ("num_offspring", C.c_int),
+                ("mutation_rate", C.c_float), ("catastrophe_threshold", C.c_int),
+                ("catastrophe_survival", C.c_float), ("restart_after", C.c_int),
+                ("restart_p", C.c_float)]
+
+
+class GaIslandState(C.Structure):
+    _fields_ = [("best_weight", C.c_uint32), ("best_len", C.c_int32),
+                ("no_improvement", C.c_uint32), ("catastrophe", C.c_uint32),
+                ("mutation_rate", C.c_float), ("migrate", C.c_int32)]
 
 
 class GaStatus(C.Structure):
@@ -109,6 +119,11 @@ SIGNATURES = {
     "bmm_hip_ga_best": ([_vp, _u64p, C.POINTER(C.c_uint64)], C.c_int),
     "bmm_hip_ga_island_best": ([_vp, C.c_int, _u64p, C.POINTER(C.c_uint64)], C.c_int),
     "bmm_hip_ga_destroy": ([_vp], C.c_int),
+    "bmm_hip_ga_configure_islands": ([_vp, _vp, _vp, _sz], C.c_int),
+    "bmm_hip_ga_best_ops": ([_vp, C.c_int, _u64p, _u16p, C.POINTER(_sz)], C.c_int),
+    "bmm_hip_ga_set_island_base": ([_vp, C.c_int, _u64p], C.c_int),
+    "bmm_hip_ga_population": ([_vp, C.c_int, _vp, _vp, _vp, _vp, _vp, _vp,
+                               C.POINTER(GaIslandState)], C.c_int),
     "bmm_hip_mlp_num_params": ([], C.c_size_t),
     "bmm_hip_mlp_create": ([C.c_uint64, C.POINTER(C.c_void_p)], C.c_int),
     "bmm_hip_mlp_destroy": ([_vp], C.c_int),
@@ -425,6 +440,47 @@ class GpuGA:
         w = C.c_uint64()
         call("bmm_hip_ga_island_best", self.h, island, out, C.byref(w))
         return out, w.value
+
+    def configure_islands(self, seeds=None, migrate=None):
+        """Per-island seeds and migration flags (bit 0 export, bit 1 import)."""
+        n = None
+        sp = mp = None
+        if seeds is not None:
+            sd = np.ascontiguousarray(np.asarray(seeds, dtype=np.uint64))
+            n, sp = sd.size, sd.ctypes.data
+        if migrate is not None:
+            mg = np.ascontiguousarray(np.asarray(migrate, dtype=np.int32))
+            n, mp = mg.size, mg.ctypes.data
+        call("bmm_hip_ga_configure_islands", self.h, sp, mp, n or 0)
+
+    def best_ops(self, island: int):
+        """(base matrix, best op sequence as (src, dst) rows) of an island."""
+        base = np.zeros((self.rows, self.cols // 64), np.uint64)
+        ops = np.zeros(4096, np.uint16)
+        n = _sz()
+        call("bmm_hip_ga_best_ops", self.h, island, base, ops, C.byref(n))
+        o = ops[:n.value]
+        return base, np.stack([o & 0xff, o >> 8], axis=1).astype(np.uint16)
+
+    def set_island_base(self, island: int, matrix):
+        call("bmm_hip_ga_set_island_base", self.h, island, _rows(matrix))
+
+    def population(self, island: int, pop: int, max_ops: int) -> dict:
+        """Next / previous population of an island, the previous one's
+        fitness, the individual random streams and the island counters."""
+        on = np.zeros((pop, max_ops), np.uint16)
+        ln = np.zeros(pop, np.int32)
+        op = np.zeros((pop, max_ops), np.uint16)
+        lp = np.zeros(pop, np.int32)
+        fp = np.zeros(pop, np.uint32)
+        rng = np.zeros(pop, np.uint64)
+        st = GaIslandState()
+        call("bmm_hip_ga_population", self.h, island, on.ctypes.data, ln.ctypes.data,
+             op.ctypes.data, lp.ctypes.data, fp.ctypes.data, rng.ctypes.data, C.byref(st))
+        return {"ops_next": on, "len_next": ln, "ops_prev": op, "len_prev": lp, "fit_prev": fp,
+                "rng": rng, "best_weight": st.best_weight, "best_len": st.best_len,
+                "no_improvement": st.no_improvement, "catastrophe": st.catastrophe,
+                "mutation_rate": st.mutation_rate, "migrate": st.migrate}
 
     def close(self):
         if self.h:

@@ -80,6 +80,11 @@ void ga_run(Ga* G, size_t max_generations, uint64_t target, double time_budget_s
             bmm_ga_status* st);
 void ga_best(Ga* G, uint64_t* out, uint64_t* weight);
 void ga_island_best(Ga* G, int island, uint64_t* out, uint64_t* weight);
+void ga_best_ops(Ga* G, int island, uint64_t* base, uint16_t* ops, size_t* n_ops);
+void ga_population(Ga* G, int island, uint16_t* ops_next, int32_t* len_next, uint16_t* ops_prev,
+                   int32_t* len_prev, uint32_t* fit_prev, uint64_t* rng, bmm_ga_island_state* st);
+void ga_set_island_base(Ga* G, int island, const uint64_t* matrix);
+void ga_configure_islands(Ga* G, const uint64_t* seeds, const int32_t* migrate, size_t n);
 struct Mlp;
 Mlp* mlp_create(uint64_t seed);
 void mlp_destroy(Mlp* M);
@@ -588,6 +593,13 @@ int bmm_hip_ga_default_params(bmm_ga_params* p) {
         p->win_rounds = 2;
         p->win_candidates = 32;
         p->isd_trials = 1;
+        p->num_parents = 2;
+        p->num_offspring = 1;
+        p->mutation_rate = 0.f;
+        p->catastrophe_threshold = 0;
+        p->catastrophe_survival = 0.2f;
+        p->restart_after = 0;
+        p->restart_p = 0.2f;
     });
 }
 
@@ -626,6 +638,43 @@ int bmm_hip_ga_island_best(void* ga, int island, uint64_t* out_matrix, uint64_t*
         if (!ga || !out_matrix || !out_weight)
             fail(BMM_ERR_INVALID, "ga_island_best: null pointer");
         ga_island_best(static_cast<Ga*>(ga), island, out_matrix, out_weight);
+    });
+}
+
+int bmm_hip_ga_configure_islands(void* ga, const uint64_t* seeds, const int32_t* migrate,
+                                 size_t n) {
+    return guarded([&] {
+        if (!ga)
+            fail(BMM_ERR_INVALID, "ga_configure_islands: null pointer");
+        ga_configure_islands(static_cast<Ga*>(ga), seeds, migrate, n);
+    });
+}
+
+int bmm_hip_ga_best_ops(void* ga, int island, uint64_t* out_base, uint16_t* out_ops,
+                        size_t* n_ops) {
+    return guarded([&] {
+        if (!ga || !out_base || !out_ops || !n_ops)
+            fail(BMM_ERR_INVALID, "ga_best_ops: null pointer");
+        ga_best_ops(static_cast<Ga*>(ga), island, out_base, out_ops, n_ops);
+    });
+}
+
+int bmm_hip_ga_set_island_base(void* ga, int island, const uint64_t* matrix) {
+    return guarded([&] {
+        if (!ga || !matrix)
+            fail(BMM_ERR_INVALID, "ga_set_island_base: null pointer");
+        ga_set_island_base(static_cast<Ga*>(ga), island, matrix);
+    });
+}
+
+int bmm_hip_ga_population(void* ga, int island, uint16_t* ops_next, int32_t* len_next,
+                          uint16_t* ops_prev, int32_t* len_prev, uint32_t* fit_prev,
+                          uint64_t* rng, bmm_ga_island_state* state) {
+    return guarded([&] {
+        if (!ga)
+            fail(BMM_ERR_INVALID, "ga_population: null pointer");
+        ga_population(static_cast<Ga*>(ga), island, ops_next, len_next, ops_prev, len_prev,
+                      fit_prev, rng, state);
     });
 }
 

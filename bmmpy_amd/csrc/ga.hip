@@ -91,17 +91,37 @@ __global__ __launch_bounds__(64) void ga_eval_kernel(const uint32_t* __restrict_
 }
 
 // ------------------------------------------------------------ breed ------
+constexpr int kMaxParents = 4;
+
 struct GaDev {
     int islands, pop, rows, max_ops, tournament, elite, n_cand, learned;
     float p_append, p_replace, p_delete, p_learned;
     int mutations;
+    int num_parents, num_offspring;
+    float mutation_rate;         // > 0: the reference's row-XOR mutation, adapted per generation
+    int catastrophe_threshold;   // 0: off
+    float catastrophe_survival;
+    int restart_after;           // 0: off
+    float restart_p;
 };
+
+// Per-island counters of the reference GA (genetic_algorithm.hpp: _no_improvement,
+// _catastrophe, the adapted mutation rate).
+struct GaIslandState {
+    unsigned int no_improvement;
+    unsigned int catastrophe;
+    float rate;
+    unsigned int pad;
+};
+
+__device__ inline float unit24(uint64_t& rng) {
+    return (float)(xs_next(rng) >> 40) * (1.0f / 16777216.0f);
+}
 
 __device__ inline uint16_t propose(const GaDev& P, uint64_t& rng, const uint16_t* cand) {
     // Learned: a ranked pair with probability p_learned, else uniform (the
     // plain proposal stays the fallback).
-    if (P.learned && P.n_cand > 0 &&
-        (float)(xs_next(rng) >> 40) * (1.0f / 16777216.0f) < P.p_learned)
+    if (P.learned && P.n_cand > 0 && unit24(rng) < P.p_learned)
         return cand[xs_next(rng) % (uint64_t)P.n_cand];
     const int r = P.rows;
     const int s = (int)(xs_next(rng) % (uint64_t)r);
@@ -111,96 +131,223 @@ __device__ inline uint16_t propose(const GaDev& P, uint64_t& rng, const uint16_t
     return (uint16_t)(s | (d << 8));
 }
 
+// A random individual: the identity followed by 3 * rows random row XORs
+// r_i ^= r_j, i == j skipped (genetic_algorithm_population.cpp:20-36), as
+// ops (src j, dst i), at most max_ops of them.
+__device__ inline int random_sequence(const GaDev& P, uint64_t& rng, uint16_t* child) {
+    int clen = 0;
+    for (int k = 0; k < 3 * P.rows && clen < P.max_ops; ++k) {
+        const int i = (int)(xs_next(rng) % (uint64_t)P.rows);
+        const int j = (int)(xs_next(rng) % (uint64_t)P.rows);
+        if (i != j)
+            child[clen++] = (uint16_t)(j | (i << 8));
+    }
+    return clen;
+}
+
+// Tournament of P.tournament uniform draws, lightest wins, first drawn on ties
+// (genetic_algorithm_operators.cpp:39-56).
+__device__ inline int tournament(const GaDev& P, uint64_t& rng, const unsigned int* sf) {
+    int best = (int)(xs_next(rng) % (uint64_t)P.pop);
+    for (int k = 1; k < P.tournament; ++k) {
+        const int c = (int)(xs_next(rng) % (uint64_t)P.pop);
+        if (sf[c] < sf[best])
+            best = c;
+    }
+    return best;
+}
+
+// One workgroup per island, one thread per child.  Order of one generation
+// (the reference's step(), genetic_algorithm_lifecycle.cpp:112-200, read from
+// the evaluation of the population that was just bred):
+//  1. rank the evaluated population by (fitness, index); a lighter best than
+//     the island's best is recorded and resets the stale counters, else they
+//     grow;
+//  2. catastrophe when its counter reaches the threshold
+//     (genetic_algorithm_operators.cpp:15-37): the floor(pop * survival) best
+//     survive, the rest are random individuals;
+//  3. otherwise the mutation rate adapts (genetic_algorithm_core.cpp:33-41),
+//     the elite_count best are copied, and the other children come in groups
+//     of num_offspring that share num_parents tournament winners and their
+//     cut points: child = parent 0 ops [0, c0) ++ middle parent p ops
+//     [s_p, e_p) ++ last parent ops [s, len) (two parents: one-point
+//     crossover of the op lists); after restart_after stale generations a
+//     child is instead a random individual with probability restart_p;
+//  4. each child is mutated on its own stream: mutation_rate > 0 appends
+//     max(1, floor(rows * rate)) row XORs r_i ^= r_j with i, j uniform and
+//     i == j skipped (:100-129; a full sequence takes the op at a random
+//     position instead); otherwise `mutations` draws of append / replace /
+//     delete.
+// LDS: fitness[pop] u32, order[pop] u16, then per group and parent (index,
+// start, end) u16.
 __global__ __launch_bounds__(1024) void ga_breed_kernel(
     GaDev P, const unsigned int* __restrict__ fit, const uint16_t* __restrict__ ops_cur,
     const int32_t* __restrict__ len_cur, uint16_t* __restrict__ ops_next,
     int32_t* __restrict__ len_next, uint64_t* __restrict__ rng_state,
     unsigned int* __restrict__ best_w, uint16_t* __restrict__ best_ops,
     int32_t* __restrict__ best_len, const uint16_t* __restrict__ cand,
-    unsigned long long* __restrict__ evals) {
-    extern __shared__ unsigned int sf[];  // pop fitness values
-    __shared__ unsigned long long bestkey;
+    unsigned long long* __restrict__ evals, GaIslandState* __restrict__ state) {
+    extern __shared__ unsigned int sf[];
+    uint16_t* order = reinterpret_cast<uint16_t*>(sf + P.pop);
+    uint16_t* gsel = order + ((P.pop + 1) & ~1);
+    __shared__ int s_mode;
+    __shared__ unsigned int s_noimp;
+    __shared__ float s_rate;
     const int isl = blockIdx.x;
     const int t = threadIdx.x;
     const size_t base_id = (size_t)isl * P.pop;
-    if (t == 0)
-        bestkey = ~0ull;
+    if (t < P.pop)
+        sf[t] = fit[base_id + t];
     __syncthreads();
     if (t < P.pop) {
-        sf[t] = fit[base_id + t];
-        atomicMin(&bestkey, ((unsigned long long)sf[t] << 32) | (unsigned)t);
+        const unsigned int f = sf[t];
+        int r = 0;
+        for (int j = 0; j < P.pop; ++j) {
+            const unsigned int g = sf[j];
+            r += (g < f) || (g == f && j < t);
+        }
+        order[r] = (uint16_t)t;
     }
     __syncthreads();
-    const int arg = (int)(bestkey & 0xffffffffu);
-    const unsigned int wbest = (unsigned int)(bestkey >> 32);
+    const int arg = order[0];
+    const unsigned int wbest = sf[arg];
     const bool improved = wbest < best_w[isl];
     __syncthreads();
     if (improved) {
         const uint16_t* src = ops_cur + (base_id + arg) * P.max_ops;
         for (int k = t; k < P.max_ops; k += blockDim.x)
             best_ops[(size_t)isl * P.max_ops + k] = src[k];
-        if (t == 0) {
+    }
+    if (t == 0) {
+        GaIslandState s = state[isl];
+        if (improved) {
             best_w[isl] = wbest;
             best_len[isl] = len_cur[base_id + arg];
+            s.no_improvement = 0;
+            s.catastrophe = 0;
+        } else {
+            ++s.no_improvement;
+            ++s.catastrophe;
+        }
+        int mode = 0;
+        if (P.catastrophe_threshold > 0 && s.catastrophe == (unsigned int)P.catastrophe_threshold) {
+            mode = 1;
+            s.catastrophe = 0;
+        }
+        if (P.mutation_rate > 0.f) {
+            if (s.no_improvement > 10)
+                s.rate = fminf(0.8f, s.rate * 1.5f);
+            else if (s.no_improvement > 5)
+                s.rate = fminf(0.6f, s.rate * 1.3f);
+            else
+                s.rate = fmaxf(0.3f, s.rate * 0.95f);
+        }
+        state[isl] = s;
+        s_mode = mode;
+        s_noimp = s.no_improvement;
+        s_rate = s.rate;
+        atomicAdd(evals, (unsigned long long)P.pop);
+    }
+    __syncthreads();
+    const int mode = s_mode;
+    const bool active = t < P.pop;
+    uint64_t rng = active ? rng_state[base_id + t] : 0;
+    uint16_t* child = ops_next + (base_id + (active ? t : 0)) * P.max_ops;
+    int clen = 0;
+    int kind = 0;   // 0 copy of order[t], 1 random, 2 crossover child
+    int src_copy = 0;
+    if (active) {
+        if (mode == 1) {
+            const int survivors = (int)((float)P.pop * P.catastrophe_survival);
+            kind = t < survivors ? 0 : 1;
+            src_copy = t;
+        } else if (t < P.elite) {
+            kind = 0;
+            src_copy = t;
+        } else if (P.restart_after > 0 && s_noimp > (unsigned int)P.restart_after &&
+                   unit24(rng) < P.restart_p) {
+            kind = 1;
+        } else {
+            kind = 2;
         }
     }
-    if (t == 0)
-        atomicAdd(evals, (unsigned long long)P.pop);
-    if (t >= P.pop)
-        return;
-    uint64_t rng = rng_state[base_id + t];
-    uint16_t* child = ops_next + (base_id + t) * P.max_ops;
-    int clen;
-    if (t < P.elite) {
-        // Elite: the generation's best (t == 0) or a tournament winner copy.
-        int pick = arg;
-        if (t > 0) {
-            pick = (int)(xs_next(rng) % (uint64_t)P.pop);
-            for (int k = 1; k < P.tournament; ++k) {
-                const int c = (int)(xs_next(rng) % (uint64_t)P.pop);
-                if (sf[c] < sf[pick])
-                    pick = c;
+    // Group leaders (the first non-elite child of each group, whatever its
+    // own kind) draw the parents and cut points of their group.
+    const int c = t - P.elite;
+    const int g = c >= 0 ? c / P.num_offspring : 0;
+    if (active && mode == 0 && c >= 0 && c % P.num_offspring == 0) {
+        uint16_t* gs = gsel + (size_t)g * P.num_parents * 3;
+        for (int p = 0; p < P.num_parents; ++p) {
+            const int par = tournament(P, rng, sf);
+            const int lp = len_cur[base_id + par];
+            int s0 = 0, e0 = lp;
+            if (P.num_parents == 1) {
+                // one parent: the child is its copy (then mutated)
+            } else if (p == 0) {
+                e0 = (int)(xs_next(rng) % (uint64_t)(lp + 1));
+            } else {
+                s0 = (int)(xs_next(rng) % (uint64_t)(lp + 1));
+                if (p + 1 < P.num_parents)
+                    e0 = s0 + (int)(xs_next(rng) % (uint64_t)(lp - s0 + 1));
             }
+            gs[p * 3] = (uint16_t)par;
+            gs[p * 3 + 1] = (uint16_t)s0;
+            gs[p * 3 + 2] = (uint16_t)e0;
         }
+    }
+    __syncthreads();
+    if (!active)
+        return;
+    if (kind == 0) {
+        const int pick = order[src_copy];
         const uint16_t* src = ops_cur + (base_id + pick) * P.max_ops;
         clen = len_cur[base_id + pick];
         for (int k = 0; k < clen; ++k)
             child[k] = src[k];
+    } else if (kind == 1) {
+        clen = random_sequence(P, rng, child);
     } else {
-        int pa = (int)(xs_next(rng) % (uint64_t)P.pop), pb = (int)(xs_next(rng) % (uint64_t)P.pop);
-        for (int k = 1; k < P.tournament; ++k) {
-            const int ca = (int)(xs_next(rng) % (uint64_t)P.pop);
-            const int cb = (int)(xs_next(rng) % (uint64_t)P.pop);
-            if (sf[ca] < sf[pa])
-                pa = ca;
-            if (sf[cb] < sf[pb])
-                pb = cb;
+        const uint16_t* gs = gsel + (size_t)g * P.num_parents * 3;
+        for (int p = 0; p < P.num_parents; ++p) {
+            const uint16_t* src = ops_cur + (base_id + gs[p * 3]) * P.max_ops;
+            for (int k = gs[p * 3 + 1]; k < gs[p * 3 + 2] && clen < P.max_ops; ++k)
+                child[clen++] = src[k];
         }
-        const int la = len_cur[base_id + pa], lb = len_cur[base_id + pb];
-        const int ca = (int)(xs_next(rng) % (uint64_t)(la + 1));
-        const int cb = (int)(xs_next(rng) % (uint64_t)(lb + 1));
-        const uint16_t* A = ops_cur + (base_id + pa) * P.max_ops;
-        const uint16_t* B = ops_cur + (base_id + pb) * P.max_ops;
-        clen = 0;
-        for (int k = 0; k < ca && clen < P.max_ops; ++k)
-            child[clen++] = A[k];
-        for (int k = cb; k < lb && clen < P.max_ops; ++k)
-            child[clen++] = B[k];
-        for (int m = 0; m < P.mutations; ++m) {
-            const float u = (float)(xs_next(rng) >> 40) * (1.0f / 16777216.0f);
-            if (u < P.p_append) {
+        const uint16_t* ic = cand + (size_t)isl * P.n_cand;
+        if (P.mutation_rate > 0.f) {
+            const int count = max(1, (int)((float)P.rows * s_rate));
+            for (int m = 0; m < count; ++m) {
+                uint16_t op;
+                if (P.learned && P.n_cand > 0 && unit24(rng) < P.p_learned) {
+                    op = ic[xs_next(rng) % (uint64_t)P.n_cand];
+                } else {
+                    const int i = (int)(xs_next(rng) % (uint64_t)P.rows);
+                    const int j = (int)(xs_next(rng) % (uint64_t)P.rows);
+                    if (i == j)
+                        continue;
+                    op = (uint16_t)(j | (i << 8));
+                }
                 if (clen < P.max_ops)
-                    child[clen++] = propose(P, rng, cand + (size_t)isl * P.n_cand);
-            } else if (u < P.p_append + P.p_replace) {
-                if (clen > 0)
-                    child[xs_next(rng) % (uint64_t)clen] =
-                        propose(P, rng, cand + (size_t)isl * P.n_cand);
-            } else if (u < P.p_append + P.p_replace + P.p_delete) {
-                if (clen > 0) {
-                    const int pos = (int)(xs_next(rng) % (uint64_t)clen);
-                    for (int k = pos; k + 1 < clen; ++k)
-                        child[k] = child[k + 1];
-                    --clen;
+                    child[clen++] = op;
+                else
+                    child[xs_next(rng) % (uint64_t)clen] = op;
+            }
+        } else {
+            for (int m = 0; m < P.mutations; ++m) {
+                const float u = unit24(rng);
+                if (u < P.p_append) {
+                    if (clen < P.max_ops)
+                        child[clen++] = propose(P, rng, ic);
+                } else if (u < P.p_append + P.p_replace) {
+                    if (clen > 0)
+                        child[xs_next(rng) % (uint64_t)clen] = propose(P, rng, ic);
+                } else if (u < P.p_append + P.p_replace + P.p_delete) {
+                    if (clen > 0) {
+                        const int pos = (int)(xs_next(rng) % (uint64_t)clen);
+                        for (int k = pos; k + 1 < clen; ++k)
+                            child[k] = child[k + 1];
+                        --clen;
+                    }
                 }
             }
         }
@@ -240,15 +387,19 @@ __global__ void ga_after_rebase_kernel(int32_t* __restrict__ best_len, int32_t* 
     }
 }
 
-// Ring migration: island i adopts island i-1's base when it is lighter.
+// Ring migration: island i adopts island i-1's base when it is lighter, if
+// island i-1 exports (mig bit 0) and island i imports (mig bit 1).  The
+// reference publishes each island's best alone (genetic_algorithm_lifecycle.cpp:
+// 76-81) and an import replaces the best only when lighter (:83-110).
 __global__ void ga_migrate_decide_kernel(const unsigned int* __restrict__ best_w, int islands,
+                                         const int32_t* __restrict__ mig,
                                          int32_t* __restrict__ flag,
                                          unsigned int* __restrict__ w_snap) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= islands)
         return;
     const int prev = (i + islands - 1) % islands;
-    flag[i] = best_w[prev] < best_w[i] ? 1 : 0;
+    flag[i] = ((mig[prev] & 1) && (mig[i] & 2) && best_w[prev] < best_w[i]) ? 1 : 0;
     w_snap[i] = best_w[prev];
 }
 
@@ -1027,6 +1178,7 @@ struct Ga {
     DevBuf li_move, li_w, li_changed;
     DevBuf win_cand, win, win_move, win_w;
     DevBuf isd_r, isd_t, isd_w, isd_chosen, isd_count;
+    DevBuf isl_state, mig;
     uint64_t win_round = 0;
     uint64_t win_moves = 0;
     uint64_t isd_round = 0;
@@ -1070,7 +1222,26 @@ GaDev dev_params(const Ga& G) {
     d.p_delete = G.P.p_delete;
     d.mutations = G.P.mutations;
     d.p_learned = G.P.p_learned;
+    d.num_parents = G.P.num_parents;
+    d.num_offspring = G.P.num_offspring;
+    d.mutation_rate = G.P.mutation_rate;
+    d.catastrophe_threshold = G.P.catastrophe_threshold;
+    d.catastrophe_survival = G.P.catastrophe_survival;
+    d.restart_after = G.P.restart_after;
+    d.restart_p = G.P.restart_p;
     return d;
+}
+
+size_t breed_lds_bytes(const bmm_ga_params& P) {
+    const size_t pop = (size_t)P.population;
+    return pop * 4 + ((pop + 1) & ~(size_t)1) * 2 + pop * (size_t)P.num_parents * 3 * 2;
+}
+
+uint64_t xs_host(uint64_t& s) {
+    s ^= s << 13;
+    s ^= s >> 7;
+    s ^= s << 17;
+    return s;
 }
 
 void enqueue_generation(Ga& G, int parity, hipStream_t s) {
@@ -1086,7 +1257,7 @@ void enqueue_generation(Ga& G, int parity, hipStream_t s) {
                        G.P.max_ops, fit);
     BMM_HIP(hipGetLastError());
     hipLaunchKernelGGL(ga_breed_kernel, dim3(I), dim3(std::max(64, (Pp + 63) / 64 * 64)),
-                       (size_t)Pp * 4, s, dev_params(G), fit,
+                       breed_lds_bytes(G.P), s, dev_params(G), fit,
                        static_cast<const uint16_t*>(G.ops[parity].ptr),
                        static_cast<const int32_t*>(G.len[parity].ptr),
                        static_cast<uint16_t*>(G.ops[1 - parity].ptr),
@@ -1095,7 +1266,8 @@ void enqueue_generation(Ga& G, int parity, hipStream_t s) {
                        static_cast<uint16_t*>(G.best_ops.ptr),
                        static_cast<int32_t*>(G.best_len.ptr),
                        static_cast<const uint16_t*>(G.cand.ptr),
-                       static_cast<unsigned long long*>(G.evals.ptr));
+                       static_cast<unsigned long long*>(G.evals.ptr),
+                       static_cast<GaIslandState*>(G.isl_state.ptr));
     BMM_HIP(hipGetLastError());
 }
 
@@ -1336,7 +1508,8 @@ void epoch(Ga& G) {
     const size_t per = (size_t)G.P.rows * G.words;
     hipLaunchKernelGGL(ga_migrate_decide_kernel, dim3((I + 63) / 64), dim3(64), 0, G.stream,
                        static_cast<const unsigned int*>(G.best_w.ptr), I,
-                       static_cast<int32_t*>(G.flag.ptr), static_cast<unsigned int*>(G.w_snap.ptr));
+                       static_cast<const int32_t*>(G.mig.ptr), static_cast<int32_t*>(G.flag.ptr),
+                       static_cast<unsigned int*>(G.w_snap.ptr));
     hipLaunchKernelGGL(ga_migrate_copy_kernel, dim3(64, I), dim3(256), 0, G.stream,
                        static_cast<const uint64_t*>(G.base.ptr),
                        static_cast<uint64_t*>(G.base_tmp.ptr), per,
@@ -1371,10 +1544,19 @@ Ga* ga_create(const uint64_t* matrix, size_t rows, size_t cols, const bmm_ga_par
         fail(BMM_ERR_INVALID, "ga: rows must be in [2, 256]");
     if (cols == 0 || cols % 2048 != 0)
         fail(BMM_ERR_INVALID, "ga: cols must be a positive multiple of 2048");
-    if (P.islands < 1 || P.population < 2 || P.population > 1024 || P.max_ops < 1 ||
+    if (P.islands < 1 || P.population < 1 || P.population > 1024 || P.max_ops < 1 ||
         P.max_ops > 4096 || P.tournament < 1 || P.elite < 0 || P.elite > P.population ||
         P.migrate_every < 1 || P.mutations < 0)
         fail(BMM_ERR_INVALID, "ga: parameter out of range");
+    if (P.num_parents < 1 || P.num_parents > kMaxParents || P.num_offspring < 1 ||
+        P.num_offspring > P.population)
+        fail(BMM_ERR_INVALID, "ga: num_parents must be in [1, 4] and num_offspring in "
+                              "[1, population]");
+    if (!(P.mutation_rate >= 0.f && P.mutation_rate <= 1000.f) || P.catastrophe_threshold < 0 ||
+        !(P.catastrophe_survival >= 0.f && P.catastrophe_survival <= 1.f) ||
+        P.restart_after < 0 || !(P.restart_p >= 0.f && P.restart_p <= 1.f))
+        fail(BMM_ERR_INVALID, "ga: mutation_rate must be in [0, 1000], catastrophe_survival and "
+                              "restart_p in [0, 1], catastrophe_threshold and restart_after >= 0");
     if (P.learned && (P.n_candidates < 1 || P.n_candidates > kScorePerIsland))
         fail(BMM_ERR_INVALID, "ga: n_candidates must be in [1, 1024] for the learned proposal");
     if (P.win_rows != 0) {
@@ -1428,16 +1610,17 @@ Ga* ga_create(const uint64_t* matrix, size_t rows, size_t cols, const bmm_ga_par
     const int n_pairs = std::max(kScorePerIsland, kTrainPerIsland) * I;
     std::vector<uint64_t> seeds((size_t)I * Pp + n_pairs);
     uint64_t s = P.seed == 0 ? 0x9E3779B97F4A7C15ull : P.seed;
-    for (auto& x : seeds) {
-        s ^= s << 13;
-        s ^= s >> 7;
-        s ^= s << 17;
-        x = s | 1ull;
-    }
+    for (auto& x : seeds)
+        x = xs_host(s) | 1ull;
     BMM_HIP(hipMemcpy(G->rng.get<uint64_t>((size_t)I * Pp), seeds.data(), (size_t)I * Pp * 8,
                       hipMemcpyHostToDevice));
     BMM_HIP(hipMemcpy(G->prng.get<uint64_t>(n_pairs), seeds.data() + (size_t)I * Pp,
                       (size_t)n_pairs * 8, hipMemcpyHostToDevice));
+    std::vector<GaIslandState> st0(I, GaIslandState{0u, 0u, P.mutation_rate, 0u});
+    BMM_HIP(hipMemcpy(G->isl_state.get<GaIslandState>(I), st0.data(), (size_t)I * sizeof(GaIslandState),
+                      hipMemcpyHostToDevice));
+    std::vector<int32_t> mig0(I, 3);
+    BMM_HIP(hipMemcpy(G->mig.get<int32_t>(I), mig0.data(), (size_t)I * 4, hipMemcpyHostToDevice));
     G->li_move.get<int32_t>((size_t)I * rows);
     G->li_w.get<uint32_t>((size_t)I * rows);
     G->li_changed.get<unsigned int>(1);
@@ -1549,5 +1732,114 @@ void ga_island_best(Ga* G, int island, uint64_t* out, uint64_t* weight) {
 }
 
 void ga_best(Ga* G, uint64_t* out, uint64_t* weight) { ga_island_best(G, -1, out, weight); }
+
+// Island `island`'s base matrix and best op sequence: replaying the ops on
+// the base gives ga_island_best's matrix.
+void ga_best_ops(Ga* G, int island, uint64_t* base, uint16_t* ops, size_t* n_ops) {
+    if (island < 0 || island >= G->P.islands)
+        fail(BMM_ERR_RANGE, "ga_best_ops: island out of range");
+    BMM_HIP(hipStreamSynchronize(G->stream));
+    const size_t per = (size_t)G->P.rows * G->words;
+    BMM_HIP(hipMemcpy(base, static_cast<uint64_t*>(G->base.ptr) + per * island, per * 8,
+                      hipMemcpyDeviceToHost));
+    int32_t n = 0;
+    BMM_HIP(hipMemcpy(&n, static_cast<int32_t*>(G->best_len.ptr) + island, 4,
+                      hipMemcpyDeviceToHost));
+    if (n > 0)
+        BMM_HIP(hipMemcpy(ops, static_cast<uint16_t*>(G->best_ops.ptr) + (size_t)island * G->P.max_ops,
+                          (size_t)n * 2, hipMemcpyDeviceToHost));
+    *n_ops = (size_t)n;
+}
+
+// Populations and counters of one island (see bmm_hip_ga_population).
+void ga_population(Ga* G, int island, uint16_t* ops_next, int32_t* len_next, uint16_t* ops_prev,
+                   int32_t* len_prev, uint32_t* fit_prev, uint64_t* rng, bmm_ga_island_state* st) {
+    if (island < 0 || island >= G->P.islands)
+        fail(BMM_ERR_RANGE, "ga_population: island out of range");
+    BMM_HIP(hipStreamSynchronize(G->stream));
+    const size_t Pp = (size_t)G->P.population, off = (size_t)island * Pp;
+    const size_t mo = (size_t)G->P.max_ops;
+    const int nx = G->cur, pv = 1 - G->cur;
+    if (ops_next)
+        BMM_HIP(hipMemcpy(ops_next, static_cast<uint16_t*>(G->ops[nx].ptr) + off * mo, Pp * mo * 2,
+                          hipMemcpyDeviceToHost));
+    if (len_next)
+        BMM_HIP(hipMemcpy(len_next, static_cast<int32_t*>(G->len[nx].ptr) + off, Pp * 4,
+                          hipMemcpyDeviceToHost));
+    if (ops_prev)
+        BMM_HIP(hipMemcpy(ops_prev, static_cast<uint16_t*>(G->ops[pv].ptr) + off * mo, Pp * mo * 2,
+                          hipMemcpyDeviceToHost));
+    if (len_prev)
+        BMM_HIP(hipMemcpy(len_prev, static_cast<int32_t*>(G->len[pv].ptr) + off, Pp * 4,
+                          hipMemcpyDeviceToHost));
+    if (fit_prev)
+        BMM_HIP(hipMemcpy(fit_prev, static_cast<unsigned int*>(G->fit.ptr) + off, Pp * 4,
+                          hipMemcpyDeviceToHost));
+    if (rng)
+        BMM_HIP(hipMemcpy(rng, static_cast<uint64_t*>(G->rng.ptr) + off, Pp * 8,
+                          hipMemcpyDeviceToHost));
+    if (st) {
+        GaIslandState s{};
+        BMM_HIP(hipMemcpy(&s, static_cast<GaIslandState*>(G->isl_state.ptr) + island, sizeof(s),
+                          hipMemcpyDeviceToHost));
+        BMM_HIP(hipMemcpy(&st->best_weight, static_cast<unsigned int*>(G->best_w.ptr) + island, 4,
+                          hipMemcpyDeviceToHost));
+        BMM_HIP(hipMemcpy(&st->best_len, static_cast<int32_t*>(G->best_len.ptr) + island, 4,
+                          hipMemcpyDeviceToHost));
+        BMM_HIP(hipMemcpy(&st->migrate, static_cast<int32_t*>(G->mig.ptr) + island, 4,
+                          hipMemcpyDeviceToHost));
+        st->no_improvement = s.no_improvement;
+        st->catastrophe = s.catastrophe;
+        st->mutation_rate = s.rate;
+    }
+}
+
+// Replaces island `island`'s base by `matrix` (rows x cols/64 words); the
+// island's best becomes that base (weight = its popcount, empty sequence).
+void ga_set_island_base(Ga* G, int island, const uint64_t* matrix) {
+    if (island < 0 || island >= G->P.islands)
+        fail(BMM_ERR_RANGE, "ga_set_island_base: island out of range");
+    const size_t per = (size_t)G->P.rows * G->words;
+    uint64_t w = 0;
+    for (size_t i = 0; i < per; ++i)
+        w += (uint64_t)__builtin_popcountll(matrix[i]);
+    if (w >= 0xffffffffull)
+        fail(BMM_ERR_OVERFLOW, "ga: matrix weight exceeds 32 bits");
+    BMM_HIP(hipStreamSynchronize(G->stream));
+    BMM_HIP(hipMemcpy(static_cast<uint64_t*>(G->base.ptr) + per * island, matrix, per * 8,
+                      hipMemcpyHostToDevice));
+    const unsigned int w32 = (unsigned int)w;
+    const int32_t zero = 0;
+    BMM_HIP(hipMemcpy(static_cast<unsigned int*>(G->best_w.ptr) + island, &w32, 4,
+                      hipMemcpyHostToDevice));
+    BMM_HIP(hipMemcpy(static_cast<int32_t*>(G->best_len.ptr) + island, &zero, 4,
+                      hipMemcpyHostToDevice));
+}
+
+// Per-island seeds (individual streams of island i restarted from seeds[i],
+// as the reference seeds each island's GA, bmmpy/ga.py:373) and migration
+// flags (bit 0 export, bit 1 import).  Either array may be null.
+void ga_configure_islands(Ga* G, const uint64_t* seeds, const int32_t* migrate, size_t n) {
+    const int I = G->P.islands;
+    if (n != (size_t)I)
+        fail(BMM_ERR_INVALID, "ga_configure_islands: one entry per island expected");
+    BMM_HIP(hipStreamSynchronize(G->stream));
+    const size_t Pp = (size_t)G->P.population;
+    if (seeds) {
+        std::vector<uint64_t> st(Pp * I);
+        for (int i = 0; i < I; ++i) {
+            uint64_t s = seeds[i] == 0 ? 0x9E3779B97F4A7C15ull : seeds[i];
+            for (size_t k = 0; k < Pp; ++k)
+                st[i * Pp + k] = xs_host(s) | 1ull;
+        }
+        BMM_HIP(hipMemcpy(G->rng.ptr, st.data(), st.size() * 8, hipMemcpyHostToDevice));
+    }
+    if (migrate) {
+        for (int i = 0; i < I; ++i)
+            if (migrate[i] < 0 || migrate[i] > 3)
+                fail(BMM_ERR_INVALID, "ga_configure_islands: migrate flags must be in [0, 3]");
+        BMM_HIP(hipMemcpy(G->mig.ptr, migrate, (size_t)I * 4, hipMemcpyHostToDevice));
+    }
+}
 
 } // namespace bmmhip

@@ -4,15 +4,22 @@ python/bindings/ga.cpp:82-198) on the GPU island GA (bmm_hip_ga_*).
 The names, keyword arguments, defaults, record types and lifecycle
 (initialize / step / done / optimize; start / request_stop / wait /
 snapshot / run_to_completion) are the reference's.  The search itself is the
-GPU GA: op-sequence individuals over island bases, tournament selection,
-elitism, one-point crossover and append / replace / delete mutations, exact
-local improvement and information-set trials every epoch, ring migration.
-Configuration fields with a counterpart there are used (population_size,
-elite_count, tournament_size, seed, the stop criteria, the migration
-interval); the others (num_parents, num_offspring, the catastrophe
-settings, mutation_rate, export / import counts, channel capacity and
-overflow policy) are validated and kept as given, as the reference's
-operators have no GPU counterpart for them.
+GPU GA: op-sequence individuals over island bases (an individual is a list
+of row XORs r_dst ^= r_src, its fitness the exact total weight after replay),
+exact local improvement and information-set trials every epoch, ring
+migration.  The reference's operator settings drive the GPU breeding step:
+elitism (the elite_count best), tournaments of tournament_size, num_parents
+parents per crossover and num_offspring children per parent set (op-list
+crossover), mutation as max(1, floor(rows * mutation_rate)) row XORs per
+child with the reference's rate adaptation, restarts after 15 stale
+generations with probability 0.2, the catastrophe when enabled, per-island
+seeds, and export / import counts (zero turns an island's export or import
+off).  In a ring each island receives at most one migrant per interval (its
+predecessor's best, the one individual the reference exports), so
+channel_capacity never binds and the overflow policy never applies; both are
+validated as in the reference.  Values the GPU GA cannot take raise
+ValueError (num_parents outside [1, 4], num_offspring < 1, a negative
+mutation_rate, islands that differ in their breeding settings).
 
 best_individual() is a list of one Candidate per window row: candidate i's
 mask selects the window rows whose XOR is output row i of the best transform
@@ -186,6 +193,33 @@ def _resolve_overflow_policy(policy) -> OverflowPolicy:
     return policy
 
 
+def _breeding_key(c: GeneticAlgorithmConfig) -> tuple:
+    return (c.population_size, c.elite_count, c.tournament_size, c.num_parents, c.num_offspring,
+            float(c.mutation_rate), bool(c.enable_catastrophe),
+            int(c.catastrophe_threshold) if c.enable_catastrophe else 0,
+            float(c.catastrophe_survival_rate) if c.enable_catastrophe else 0.0)
+
+
+def _check_gpu_config(configs: list[GeneticAlgorithmConfig]) -> None:
+    c0 = configs[0]
+    if any(_breeding_key(c) != _breeding_key(c0) for c in configs):
+        raise ValueError("IslandModel: the GPU islands share population_size, elite_count, "
+                         "tournament_size, num_parents, num_offspring, mutation_rate and the "
+                         "catastrophe settings")
+    if c0.population_size > 1024:
+        raise ValueError("GeneticAlgorithm: population_size must be at most 1024")
+    if not 1 <= c0.num_parents <= 4:
+        raise ValueError("GeneticAlgorithm: the GPU GA takes num_parents in [1, 4]")
+    if c0.num_offspring < 1:
+        raise ValueError("GeneticAlgorithm: num_offspring must be at least 1")
+    if not 0.0 <= float(c0.mutation_rate) <= 1000.0:
+        raise ValueError("GeneticAlgorithm: mutation_rate must be in [0, 1000]")
+    if c0.enable_catastrophe and not 0.0 <= float(c0.catastrophe_survival_rate) <= 1.0:
+        raise ValueError("GeneticAlgorithm: catastrophe_survival_rate must be in [0, 1]")
+    if c0.enable_catastrophe and c0.catastrophe_threshold < 0:
+        raise ValueError("GeneticAlgorithm: catastrophe_threshold must be non-negative")
+
+
 def _should_stop(cfg: GeneticAlgorithmConfig, st: RunStats, best: int) -> bool:
     # genetic_algorithm_internal.hpp:22-31.
     s = cfg.stop
@@ -267,7 +301,7 @@ class _Run:
     """One GPU GA over a window: `islands` islands with the given configs."""
 
     def __init__(self, window: RowWindow, configs: list[GeneticAlgorithmConfig],
-                 migrate_every: int) -> None:
+                 migrate_every: int, migrate_flags: list[int] | None = None) -> None:
         n, cols = len(window), window.cols
         if n == 0 or cols == 0:
             raise ValueError("GeneticAlgorithm::initialize: empty window is not supported")
@@ -277,13 +311,8 @@ class _Run:
                     "GeneticAlgorithm::initialize: population_size must be greater than zero")
         if n > 256:
             raise ValueError("GeneticAlgorithm: the GPU GA takes windows of at most 256 rows")
+        _check_gpu_config(configs)
         c0 = configs[0]
-        if any(c.population_size != c0.population_size or c.elite_count != c0.elite_count
-               or c.tournament_size != c0.tournament_size for c in configs):
-            raise ValueError("IslandModel: the GPU islands share population_size, elite_count "
-                             "and tournament_size")
-        if c0.population_size > 1024:
-            raise ValueError("GeneticAlgorithm: population_size must be at most 1024")
         self.words, self.cols = _window_words(window)
         self.n = n
         self.configs = configs
@@ -293,12 +322,24 @@ class _Run:
             pad = -(-cols // _GPU_COLS) * _GPU_COLS
             m = np.zeros((n, pad // 64), np.uint64)
             m[:, :self.words.shape[1]] = self.words
+            pop = c0.population_size
+            # A rate of 0 still mutates once per child and adapts to 0.3 at the
+            # first step (genetic_algorithm_core.cpp:33-41); 0 itself selects the
+            # GPU's own mutation draws, so the smallest positive rate stands in.
+            rate = max(float(c0.mutation_rate), 1e-30)
             p = _lib.ga_default_params(
-                islands=len(configs), population=max(2, c0.population_size),
-                tournament=max(1, c0.tournament_size),
-                elite=min(c0.elite_count, max(2, c0.population_size)),
-                migrate_every=max(1, migrate_every), seed=c0.seed & ((1 << 64) - 1))
+                islands=len(configs), population=pop,
+                tournament=max(1, c0.tournament_size), elite=min(c0.elite_count, pop),
+                migrate_every=max(1, migrate_every), seed=c0.seed & ((1 << 64) - 1),
+                num_parents=c0.num_parents, num_offspring=min(c0.num_offspring, pop),
+                mutation_rate=rate,
+                catastrophe_threshold=c0.catastrophe_threshold if c0.enable_catastrophe else 0,
+                catastrophe_survival=float(c0.catastrophe_survival_rate),
+                restart_after=15, restart_p=0.2)
             self.ga = _lib.GpuGA(m, pad, p)
+            self.ga.configure_islands(
+                seeds=[c.seed & ((1 << 64) - 1) for c in configs],
+                migrate=migrate_flags if migrate_flags is not None else [3] * len(configs))
         self.generations = 0
         self.best_score = self.start_weight
         self.stale = 0
@@ -517,11 +558,21 @@ class IslandModel:
         return all(_should_stop(c, st, self._run.best_score) for c in self._configs)
 
     def initialize(self, window: RowWindow) -> None:
-        self.wait()
+        if self._running:
+            raise RuntimeError("IslandModel::initialize: model is already running")
         if self._run is not None:
             self._run.close()
-        interval = min(max(1, int(s.migration.interval_generations)) for s in self._specs)
-        self._run = _Run(window, self._configs, interval)
+        # Epochs (migration, base folding, local improvement) run at the
+        # smallest positive interval; an island with interval 0 or a zero
+        # export_count exports nothing, one with a zero import_count imports
+        # nothing (src/ga/island.cpp:149-228).
+        intervals = [int(s.migration.interval_generations) for s in self._specs]
+        positive = [i for i in intervals if i > 0]
+        interval = min(positive) if positive else 20
+        flags = [(1 if (iv > 0 and s.migration.export_count > 0) else 0)
+                 | (2 if s.migration.import_count > 0 else 0)
+                 for iv, s in zip(intervals, self._specs)]
+        self._run = _Run(window, self._configs, interval, flags)
         self._stop = False
         self._finished = self._all_done()
 
@@ -531,15 +582,16 @@ class IslandModel:
                 with self._lock:
                     self._run.step(1)
         finally:
-            self._running = False
             self._finished = True
 
     def start(self) -> None:
+        # src/ga/island_model.cpp:56-72: running stays set until wait().
         if self._run is None:
-            raise RuntimeError("IslandModel::start: initialize() first")
+            raise RuntimeError("IslandModel::start: model must be initialized first")
         if self._running:
-            return
+            raise RuntimeError("IslandModel::start: model is already running")
         self._running = True
+        self._stop = False
         self._finished = False
         self._thread = threading.Thread(target=self._loop, daemon=True)
         self._thread.start()
@@ -551,6 +603,7 @@ class IslandModel:
         if self._thread is not None:
             self._thread.join()
             self._thread = None
+        self._running = False
 
     def running(self) -> bool:
         return self._running

@@ -288,6 +288,22 @@ typedef struct bmm_ga_params {
     int isd_trials;     /* information-set trials per island per epoch (0 = off, <= 64):
                            Gauss-Jordan elimination on random columns, then the lightest
                            basis from those rows and the island's (GlobalGreedyApplier) */
+    /* The reference GA's operator settings (GeneticAlgorithmConfig,
+     * include/bmmpy/ga/genetic_algorithm.hpp:17-29; src/ga/genetic_algorithm_*.cpp). */
+    int num_parents;          /* tournament winners per crossover, 1..4 (1: copy) */
+    int num_offspring;        /* children sharing one parent set and its cut points */
+    float mutation_rate;      /* > 0: each child gets max(1, floor(rows * rate)) row XORs
+                                 r_i ^= r_j (i, j uniform, i == j skipped), the rate adapted
+                                 every generation (genetic_algorithm_core.cpp:33-41);
+                                 0: `mutations` append / replace / delete draws */
+    int catastrophe_threshold;  /* 0 = off; else after this many stale generations all but
+                                   the best floor(population * catastrophe_survival) are
+                                   replaced by random individuals (:15-37) */
+    float catastrophe_survival;
+    int restart_after;        /* 0 = off; after this many stale generations a child is a
+                                 random individual with probability restart_p
+                                 (genetic_algorithm_lifecycle.cpp:118-142: 15 and 0.2) */
+    float restart_p;
 } bmm_ga_params;
 
 typedef struct bmm_ga_status {
@@ -319,6 +335,45 @@ int bmm_hip_ga_best(void *ga, uint64_t *out_matrix, uint64_t *out_weight);
  * BMM_ERR_RANGE. */
 int bmm_hip_ga_island_best(void *ga, int island, uint64_t *out_matrix, uint64_t *out_weight);
 int bmm_hip_ga_destroy(void *ga);
+
+/* Per-island settings of an island model (IslandSpec / GeneticAlgorithmConfig
+ * per island, bmmpy/ga.py:360-380): seeds[i] restarts island i's individual
+ * random streams (XorShift64, seed 0 -> 0x9E3779B97F4A7C15); migrate[i] bit 0
+ * = island i exports its best to its ring successor, bit 1 = it imports from
+ * its predecessor (a zero export_count / import_count or interval clears the
+ * bit, src/ga/island.cpp:149-228).  n = islands; either array may be NULL. */
+int bmm_hip_ga_configure_islands(void *ga, const uint64_t *seeds, const int32_t *migrate,
+                                 size_t n);
+
+/* Island `island`'s base matrix (rows x cols/64 words) and best op sequence
+ * (*n_ops ops of src | dst << 8, at most max_ops): replaying the ops on the
+ * base (RowWindow::row_xor, src/core/bit_matrix_rows.cpp:12-20) gives
+ * bmm_hip_ga_island_best's matrix. */
+int bmm_hip_ga_best_ops(void *ga, int island, uint64_t *out_base, uint16_t *out_ops,
+                        size_t *n_ops);
+
+/* Replaces island `island`'s base matrix; its best becomes that base. */
+int bmm_hip_ga_set_island_base(void *ga, int island, const uint64_t *matrix);
+
+/* Counters of one island (the reference GA's _best_score, _no_improvement,
+ * _catastrophe and adapted mutation rate, include/bmmpy/ga/genetic_algorithm.hpp). */
+typedef struct bmm_ga_island_state {
+    uint32_t best_weight;
+    int32_t best_len;
+    uint32_t no_improvement;
+    uint32_t catastrophe;
+    float mutation_rate;
+    int32_t migrate;
+} bmm_ga_island_state;
+
+/* Island `island`'s populations: "next" = the population the next generation
+ * evaluates (population x max_ops ops, lengths), "prev" = the population the
+ * last generation evaluated with its exact fitness, rng = the individual
+ * streams the next breeding step draws from, and the island counters.  Any
+ * output may be NULL.  An island out of range is BMM_ERR_RANGE. */
+int bmm_hip_ga_population(void *ga, int island, uint16_t *ops_next, int32_t *len_next,
+                          uint16_t *ops_prev, int32_t *len_prev, uint32_t *fit_prev,
+                          uint64_t *rng, bmm_ga_island_state *state);
 
 /* ---- proposal MLP (256 -> 256 -> 256 -> 1, ReLU, bf16 products with fp32
  * accumulation on MFMA, fp32 master weights, Adam).  Parameters flat in the

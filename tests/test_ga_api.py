@@ -104,3 +104,27 @@ def test_combination_masks_reconstruct_rows():
             assert np.array_equal(acc, t)
     with pytest.raises(RuntimeError, match="row space"):
         G._combination_masks(np.array([[1]], np.uint64), np.array([[2]], np.uint64))
+
+
+def test_unsupported_breeding_settings_are_refused():
+    """Settings the GPU breeding step cannot take raise ValueError before any
+    device work, instead of being ignored."""
+    w = bmm.matrix_from_rows(["1100", "0110", "0011"]).row_window([0, 1, 2])
+    for kw, msg in (({"num_parents": 5}, "num_parents in \\[1, 4\\]"),
+                    ({"num_parents": 0}, "num_parents in \\[1, 4\\]"),
+                    ({"num_offspring": 0}, "num_offspring must be at least 1"),
+                    ({"mutation_rate": -0.5}, "mutation_rate must be in"),
+                    ({"enable_catastrophe": True, "catastrophe_survival_rate": 1.5},
+                     "catastrophe_survival_rate"),
+                    ({"population_size": 2000}, "at most 1024")):
+        with pytest.raises(ValueError, match=msg):
+            bmm.GeneticAlgorithm(**kw).initialize(w)
+    spec = bmm.IslandSpec()
+    cfgs = [bmm.ga.GeneticAlgorithmConfig(), bmm.ga.GeneticAlgorithmConfig()]
+    cfgs[1].mutation_rate = 0.5
+    with pytest.raises(ValueError, match="share population_size"):
+        bmm.IslandModel(islands=[spec, spec], ga_configs=cfgs).initialize(w)
+    cfgs[1].mutation_rate = 0.3
+    cfgs[1].num_offspring = 2
+    with pytest.raises(ValueError, match="share population_size"):
+        bmm.IslandModel(islands=[spec, spec], ga_configs=cfgs).initialize(w)

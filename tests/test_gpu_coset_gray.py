@@ -1,5 +1,6 @@
 """GPU parity: Gray coset sweep, coset minimum (single and batched) and the
 stand-alone FWHT, against the oracle on identical seeded inputs."""
+import os
 import numpy as np
 import pytest
 
@@ -282,3 +283,46 @@ def test_fwht_large_sparse_closed_form(lib, n_bits):
         exp += np.where(par == 1, -v, v)
     assert np.array_equal(h[m].astype(np.int64), exp)
     assert int(h[0]) == int(val.sum())
+
+
+@pytest.mark.parametrize("bits,graph", [(13, False), (14, True), (0, True)])
+def test_batch_4096_large_matrices_vs_golden(lib, bits, graph):
+    """The whole 4096-matrix workload in one plan (block size 2^13 int32,
+    2^14 packed int16, or the autotuned shape), compared on the first matrix
+    of each row count 21 .. 28 (where nearly all the transform work is) with
+    golden minima and indices computed by the oracle's coset minimum
+    (tests/golden/make_golden_batch_large.py)."""
+    import ctypes as C
+    import torch
+    _say("batch 4096 large matrices vs golden", bits, graph)
+    g = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                             "golden_batch_large.npz"))
+    rows, cols, dens, words, offs = lib.gen_batch(SEED_BATCH_4096, 4096)
+    assert np.array_equal(rows[g["matrices"]], g["rows"])
+    assert np.array_equal(cols[g["matrices"]], g["cols"])
+    if bits:
+        w, i = _plan_run(lib, words, offs, rows, cols, bits, graph=graph)
+    else:
+        out_off = np.concatenate([[0], np.cumsum(rows)[:-1]]).astype(np.int64)
+        n = int(rows.sum())
+        dev = torch.device("cuda:0")
+        d_words = torch.from_numpy(words.view(np.int64).copy()).to(dev)
+        d_w = torch.full((n,), -1, dtype=torch.int32, device=dev)
+        d_i = torch.full((n,), -1, dtype=torch.int64, device=dev)
+        plan = C.c_void_p()
+        lib.call("bmm_hip_batch_plan_create", d_words.data_ptr(), offs, rows, cols, rows.size,
+                 out_off, d_w.data_ptr(), d_i.data_ptr(), 0, 1, C.byref(plan))
+        try:
+            lib.call("bmm_hip_batch_plan_run", plan, 1, None)
+            lib.call("bmm_hip_batch_plan_run", plan, 1, None)
+            torch.cuda.synchronize()
+            w, i = d_w.cpu().numpy().view(np.uint32), d_i.cpu().numpy().view(np.uint64)
+        finally:
+            lib.call("bmm_hip_batch_plan_destroy", plan)
+    out_off = np.concatenate([[0], np.cumsum(rows)[:-1]])
+    k = 0
+    for m, r in zip(g["matrices"], g["rows"]):
+        o = out_off[m]
+        assert np.array_equal(w[o:o + r], g["weights"][k:k + r]), (m, r)
+        assert np.array_equal(i[o:o + r], g["indices"][k:k + r]), (m, r)
+        k += r

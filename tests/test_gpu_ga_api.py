@@ -159,3 +159,38 @@ def test_single_row_dependent_rows_and_request_stop():
     model.wait()
     assert not model.running() and model.stop_requested()
     assert len(model.best_individual()) == 4
+
+
+def test_island_model_lifecycle_errors_and_island_seeds():
+    """initialize() or start() on a running model raise as the reference's
+    IslandModel (src/ga/island_model.cpp:32-35, 56-66); islands seeded
+    seed + i * seed_stride draw from their own streams."""
+    _say("ga api lifecycle")
+    w = make_ga_matrix().row_window([0, 1, 2, 3])
+    model = bmm.IslandModel(islands=[_spec(0, 5), _spec(1, 5)], population_size=8, seed=4)
+    model.initialize(w)
+    pops = [model._run.ga.population(i, 8, 512)["rng"] for i in range(2)]
+    assert not np.array_equal(pops[0], pops[1])
+    model.start()
+    try:
+        with pytest.raises(RuntimeError, match="model is already running"):
+            model.initialize(w)
+        with pytest.raises(RuntimeError, match="model is already running"):
+            model.start()
+    finally:
+        model.request_stop()
+        model.wait()
+    assert not model.running()
+    model.initialize(w)      # allowed again after wait()
+    same = bmm.IslandModel(islands=[_spec(0, 5), _spec(1, 5)], population_size=8, seed=4,
+                           seed_stride=0)
+    same.initialize(w)
+    pops = [same._run.ga.population(i, 8, 512)["rng"] for i in range(2)]
+    assert np.array_equal(pops[0], pops[1])
+    # export_count = 0 on island 0 and import_count = 0 on island 1 reach the device
+    s0, s1 = _spec(0, 5), _spec(1, 5)
+    s0.migration.export_count = 0
+    s1.migration.import_count = 0
+    flags = bmm.IslandModel(islands=[s0, s1], population_size=8)
+    flags.initialize(w)
+    assert [flags._run.ga.population(i, 8, 512)["migrate"] for i in range(2)] == [2, 1]

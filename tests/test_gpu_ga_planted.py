@@ -1,0 +1,40 @@
+"""The island GA at the configured workload shape: planted 256 x 8192
+(Bernoulli(0.01) rows under a random invertible 256 x 256 GF(2) transform,
+seed 0x5EED0005), 64 islands x 256 individuals x 512 ops, ring migration
+every 20 generations, plain and learned proposal.  Both must reach 1.01 x
+the planted weight; the reported weight is the popcount of the returned
+rows, and the returned rows span the same space as the input (reference
+GA semantics: the transform is invertible, src/ga/island.cpp:149-228)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+SEED_PLANTED_256X8192 = 0x5EED0005
+
+
+def _weight(m):
+    return int(np.unpackbits(np.ascontiguousarray(m).view(np.uint8)).sum())
+
+
+@pytest.mark.parametrize("learned", [0, 1])
+def test_island_ga_reaches_planted_at_workload_shape(lib, ora, learned):
+    print("ga planted 256 x 8192, learned", learned, flush=True)
+    S, A, M = lib.gen_planted(SEED_PLANTED_256X8192, 256, 8192, 0.01)
+    planted = _weight(S)
+    target = int(planted * 1.01)
+    p = lib.ga_default_params(learned=learned)
+    assert (p.islands, p.population, p.max_ops, p.migrate_every) == (64, 256, 512, 20)
+    ga = lib.GpuGA(A, 8192, p)
+    try:
+        st = ga.run(400, target=target)
+        best, w = ga.best()
+    finally:
+        ga.close()
+    print(f"  reached {w} ({w / planted:.4f} x planted {planted}) after "
+          f"{st.generations} generations, {st.elapsed_s:.3f} s", flush=True)
+    assert st.reached_target and w <= target
+    assert w == st.best_weight == _weight(best)
+    assert st.evaluations == st.generations * 64 * 256
+    assert ora.rank(best, 8192) == 256
+    assert ora.rank(np.concatenate([A, best]), 8192) == 256

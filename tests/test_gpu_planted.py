@@ -121,3 +121,34 @@ def test_window_coset_47_rows_top_bit(lib):
     w, mk = lib.window_coset_min(m, 4096, [0], [window])
     assert int(w[0]) == 3
     assert int(mk[0]) == sum(1 << k for k in pick)
+
+
+def _kat40(lib, target):
+    S, A, M = lib.gen_planted(SEED_PLANTED_48X4096, 48, 4096, 0.02)
+    window = [r for r in range(48) if r != target][:40]
+    rng = np.random.default_rng(4000 + target)
+    pick = sorted({0, 39} | set(int(x) for x in rng.choice(np.arange(1, 39), 15, replace=False)))
+    t = np.zeros(A.shape[1], np.uint64)
+    for k in pick:
+        t ^= A[window[k]]
+    for c in rng.choice(4096, 3, replace=False):
+        t[c // 64] ^= np.uint64(1 << (int(c) % 64))
+    m = A.copy()
+    m[target] = t
+    w, mk = lib.window_coset_min(m, 4096, [target], [window])
+    return int(w[0]), int(mk[0]), sum(1 << k for k in pick), len(pick)
+
+
+@pytest.mark.parametrize("target", [0, 47])
+def test_planted_40_wide_known_answer(lib, target):
+    """Known answer at the workload's full width on the planted 48 x 4096
+    instance: the target row is replaced by the XOR of 17 of its 40 window
+    rows (the lowest-index other rows, window positions 0 and 39 included)
+    plus three flipped bits.  The rows of the mixed matrix are independent,
+    so any other subset adds a nonzero word of the window's row space, whose
+    weight is far above 6; the unique minimum is 3 at exactly that mask
+    (2^40 subsets, about 0.3 s)."""
+    _say("planted 40 wide known answer", target)
+    w, mk, expect, n = _kat40(lib, target)
+    assert n == 17
+    assert w == 3 and mk == expect
