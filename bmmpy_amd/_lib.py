@@ -112,6 +112,8 @@ SIGNATURES = {
                                C.c_int),
     "bmm_hip_gray_coset_min_dev": ([_vp, _sz, _sz, _sz, _i32p, _sz, _sz, _vp, _vp, _vp],
                                    C.c_int),
+    "bmm_hip_gray_coset_min_noalloc": ([_vp, _sz, _sz, _sz, _i32p, _sz, _sz, _vp, _vp, _vp],
+                                       C.c_int),
     "bmm_hip_replay": ([_u64p, _sz, _sz, _u16p, _sz, C.POINTER(C.c_uint64)], C.c_int),
     "bmm_hip_ga_default_params": ([C.POINTER(GaParams)], C.c_int),
     "bmm_hip_ga_create": ([_u64p, _sz, _sz, C.POINTER(GaParams), C.POINTER(C.c_void_p)], C.c_int),

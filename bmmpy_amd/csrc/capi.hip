@@ -63,6 +63,10 @@ void gray_coset_min_dev(const uint64_t* d_rows, size_t n_rows, size_t words_per_
                         size_t block_rows, const int32_t* targets, size_t n_targets,
                         size_t chunk_bits, uint64_t* d_first_key, uint64_t* d_mask_key,
                         hipStream_t stream);
+void gray_coset_min_noalloc(const uint64_t* d_rows, size_t n_rows, size_t words_per_row,
+                        size_t block_rows, const int32_t* targets, size_t n_targets,
+                        size_t chunk_bits, uint64_t* d_first_key, uint64_t* d_mask_key,
+                        hipStream_t stream);
 void coset_min_batch_dev(const uint64_t* d_rows, const int64_t* word_offset, const int32_t* rows,
                          const int32_t* cols, size_t n_mat, const int64_t* out_offset,
                          uint32_t* d_out_w, uint64_t* d_out_idx, int persistent_wgs,
@@ -298,6 +302,16 @@ int bmm_hip_gray_coset_min_dev(const uint64_t* d_rows, size_t n_rows, size_t wor
     return guarded([&] {
         gray_coset_min_dev(d_rows, n_rows, words_per_row, block_rows, targets, n_targets,
                            chunk_bits, d_first_key, d_mask_key, (hipStream_t)stream);
+    });
+}
+
+int bmm_hip_gray_coset_min_noalloc(const uint64_t* d_rows, size_t n_rows, size_t words_per_row,
+                                   size_t block_rows, const int32_t* targets, size_t n_targets,
+                                   size_t chunk_bits, uint64_t* d_first_key,
+                                   uint64_t* d_mask_key, void* stream) {
+    return guarded([&] {
+        gray_coset_min_noalloc(d_rows, n_rows, words_per_row, block_rows, targets, n_targets,
+                               chunk_bits, d_first_key, d_mask_key, (hipStream_t)stream);
     });
 }
 

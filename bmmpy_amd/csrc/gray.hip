@@ -23,6 +23,8 @@
 
 #include <algorithm>
 #include <cstring>
+#include <mutex>
+#include <string>
 #include <vector>
 
 namespace bmmhip {
@@ -179,8 +181,8 @@ GrayWorkspace& gray_ws() {
 
 template <int NW, int RES>
 void launch_gray(const uint32_t* rows, const GrayTask* d_tasks, const int32_t* d_wg_task,
-                 int n_wg, int c, uint64_t* k1, uint64_t* k2, hipStream_t s) {
-    ScopedKernelTimer timer(s);
+                 int n_wg, int c, uint64_t* k1, uint64_t* k2, hipStream_t s, bool timed = true) {
+    ScopedKernelTimer timer(s, timed);
     hipLaunchKernelGGL((gray_coset_kernel<NW, RES>), dim3(n_wg), dim3(256), 0, s, rows, d_tasks,
                        d_wg_task, c, k1, k2);
     BMM_HIP(hipGetLastError());
@@ -307,6 +309,175 @@ void gray_coset_min_dev(const uint64_t* d_rows, size_t n_rows, size_t words_per_
                            d_first_key, d_mask_key, stream);
         return;
     }
+    }
+}
+
+// ------------------------------------------------ allocation-free form ----
+// The same sweep for callers that cannot allocate device memory (code under
+// HIP graph capture, tight serving loops): the task tables live in `scratch`,
+// a file-scope device array each GPU holds until the process ends, and are
+// built on the device by gray_setup_kernel from by-value kernel arguments, so
+// a captured launch sequence refers to no host memory.  Layout: GrayTask[64]
+// at 0, the workgroup -> task table at 16 KiB (61,440 entries), zero-padded
+// rows at 256 KiB (98,304 words).  Calls that overlap on one GPU share the
+// array and race.
+constexpr size_t kGrayScratchBytes = (size_t)1 << 20;
+constexpr size_t kGrayScratchWgOff = (size_t)16 << 10;
+constexpr size_t kGrayScratchRowsOff = (size_t)256 << 10;
+constexpr int kGrayNoAllocTargets = 64;
+constexpr size_t kGrayNoAllocWg = (kGrayScratchRowsOff - kGrayScratchWgOff) / 4;
+constexpr size_t kGrayNoAllocRowWords = (kGrayScratchBytes - kGrayScratchRowsOff) / 8;
+static_assert(kGrayNoAllocTargets * sizeof(GrayTask) <= kGrayScratchWgOff, "task table fits");
+
+static __device__ __align__(256) unsigned char scratch[kGrayScratchBytes];
+
+namespace {
+
+struct GraySetupArgs {
+    int32_t targets[kGrayNoAllocTargets];
+    int32_t n_targets, block_rows, c, n_wg;
+};
+
+// Task i (thread i < n_targets) and workgroup table entry k (thread k < n_wg),
+// equal to what gray_coset_min_dev uploads: block = rows [0, block_rows)
+// minus the target in ascending order, unused blk entries 0, each task's
+// 2^(nbits - c - 8) workgroups after the previous task's.
+__global__ void gray_setup_kernel(GraySetupArgs a, GrayTask* __restrict__ tasks,
+                                  int32_t* __restrict__ wg_task) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int first = 0;
+    for (int k = 0; k < a.n_targets; ++k) {
+        const int t = a.targets[k];
+        const int nbits = a.block_rows - (t < a.block_rows ? 1 : 0);
+        const int n_wg = 1 << (nbits - a.c - 8);
+        if (k == i) {
+            GrayTask g;
+            g.target = t;
+            g.nbits = nbits;
+            g.first_wg = first;
+            g.slot = k;
+            int b = 0;
+            for (int r = 0; r < a.block_rows; ++r)
+                if (r != t)
+                    g.blk[b++] = r;
+            for (; b < kGrayMaxBlock; ++b)
+                g.blk[b] = 0;
+            tasks[k] = g;
+        }
+        if (i >= first && i < first + n_wg)
+            wg_task[i] = k;
+        first += n_wg;
+    }
+}
+
+// Rows zero-padded from `words` to `pw` words per row.
+__global__ void gray_pad_kernel(const uint64_t* __restrict__ src, size_t words, size_t pw,
+                                size_t n, uint64_t* __restrict__ dst) {
+    const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    const size_t r = i / pw, w = i % pw;
+    dst[i] = w < words ? src[r * words + w] : 0ull;
+}
+
+std::mutex g_scratch_mutex;
+void* g_scratch_ptr[256] = {};
+
+} // namespace
+
+// Device address of `scratch` on the current device, looked up once per
+// device with that device current (the array's host-side name is not a
+// device address).
+unsigned char* gray_scratch_ptr() {
+    int dev = 0;
+    BMM_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 256)
+        fail(BMM_ERR_DEVICE, "gray_coset_min_noalloc: device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[dev]) {
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            fail(BMM_ERR_DEVICE, std::string("gray_coset_min_noalloc: hipGetSymbolAddress: ") +
+                                     hipGetErrorString(e));
+        g_scratch_ptr[dev] = ptr;
+    }
+    return static_cast<unsigned char*>(g_scratch_ptr[dev]);
+}
+
+void gray_coset_min_noalloc(const uint64_t* d_rows, size_t n_rows, size_t words_per_row,
+                            size_t block_rows, const int32_t* targets, size_t n_targets,
+                            size_t chunk_bits, uint64_t* d_first_key, uint64_t* d_mask_key,
+                            hipStream_t stream) {
+    if (!d_rows || !d_first_key || !d_mask_key || (n_targets && !targets))
+        fail(BMM_ERR_INVALID, "gray_coset_min_noalloc: null pointer");
+    gray_check_args(n_rows, words_per_row, block_rows, targets, n_targets, chunk_bits);
+    if (n_targets > (size_t)kGrayNoAllocTargets)
+        fail(BMM_ERR_INVALID, "gray_coset_min_noalloc: at most 64 targets per call");
+    if (n_targets == 0)
+        return;
+    GraySetupArgs a{};
+    int min_bits = 64;
+    for (size_t i = 0; i < n_targets; ++i) {
+        const int t = targets[i];
+        if (t < 0 || (size_t)t >= n_rows)
+            fail(BMM_ERR_RANGE, "gray_coset_min: target row out of range");
+        const int b = (int)block_rows - ((size_t)t < block_rows ? 1 : 0);
+        if (b > kGrayMaxBlock)
+            fail(BMM_ERR_INVALID, "gray_coset_min: block larger than 40 rows");
+        min_bits = std::min(min_bits, b);
+        a.targets[i] = t;
+    }
+    const int c = chunk_bits ? (int)chunk_bits : gray_auto_chunk(min_bits);
+    if (min_bits - c < 8 || c < 4 || c > 20)
+        fail(BMM_ERR_INVALID,
+             "gray_coset_min: block must have >= chunk_bits + 8 rows and chunk_bits in [4, 20]");
+    size_t n_wg = 0;
+    for (size_t i = 0; i < n_targets; ++i) {
+        const int b = (int)block_rows - ((size_t)a.targets[i] < block_rows ? 1 : 0);
+        n_wg += (size_t)1 << (b - c - 8);
+    }
+    if (n_wg > kGrayNoAllocWg)
+        fail(BMM_ERR_INVALID, "gray_coset_min_noalloc: more than 61440 workgroups (use fewer "
+                              "targets or a larger chunk_bits)");
+    size_t pw = 1;
+    while (pw < words_per_row)
+        pw <<= 1;
+    const bool pad = pw != words_per_row;
+    if (pad && n_rows * pw > kGrayNoAllocRowWords)
+        fail(BMM_ERR_INVALID, "gray_coset_min_noalloc: padded rows exceed the 768 KiB scratch "
+                              "region (use words_per_row 1, 2, 4, 8, 16 or 32)");
+    a.n_targets = (int32_t)n_targets;
+    a.block_rows = (int32_t)block_rows;
+    a.c = c;
+    a.n_wg = (int32_t)n_wg;
+    unsigned char* base = gray_scratch_ptr();
+    GrayTask* d_tasks = reinterpret_cast<GrayTask*>(base);
+    int32_t* d_wg = reinterpret_cast<int32_t*>(base + kGrayScratchWgOff);
+    const int setup_threads = (int)std::max(n_wg, n_targets);
+    hipLaunchKernelGGL(gray_setup_kernel, dim3((setup_threads + 255) / 256), dim3(256), 0, stream,
+                       a, d_tasks, d_wg);
+    BMM_HIP(hipGetLastError());
+    const uint64_t* rows = d_rows;
+    if (pad) {
+        uint64_t* padded = reinterpret_cast<uint64_t*>(base + kGrayScratchRowsOff);
+        const size_t n = n_rows * pw;
+        hipLaunchKernelGGL(gray_pad_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                           stream, d_rows, words_per_row, pw, n, padded);
+        BMM_HIP(hipGetLastError());
+        rows = padded;
+    }
+    fill_u64(d_first_key, n_targets, ~0ull, stream);
+    fill_u64(d_mask_key, n_targets, ~0ull, stream);
+    const uint32_t* rows32 = reinterpret_cast<const uint32_t*>(rows);
+    const int nw = (int)n_wg;
+    switch (pw) {
+    case 1: launch_gray<2, 1>(rows32, d_tasks, d_wg, nw, c, d_first_key, d_mask_key, stream, false); break;
+    case 2: launch_gray<4, 2>(rows32, d_tasks, d_wg, nw, c, d_first_key, d_mask_key, stream, false); break;
+    case 4: launch_gray<8, 3>(rows32, d_tasks, d_wg, nw, c, d_first_key, d_mask_key, stream, false); break;
+    case 8: launch_gray<16, 3>(rows32, d_tasks, d_wg, nw, c, d_first_key, d_mask_key, stream, false); break;
+    case 16: launch_gray<32, 3>(rows32, d_tasks, d_wg, nw, c, d_first_key, d_mask_key, stream, false); break;
+    default: launch_gray<64, 1>(rows32, d_tasks, d_wg, nw, c, d_first_key, d_mask_key, stream, false); break;
     }
 }
 

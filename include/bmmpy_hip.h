@@ -241,6 +241,22 @@ int bmm_hip_gray_coset_min_dev(const uint64_t *d_rows, size_t n_rows, size_t wor
                                size_t chunk_bits, uint64_t *d_first_key, uint64_t *d_mask_key,
                                void *stream);
 
+/* Allocation-free form of bmm_hip_gray_coset_min_dev (no reference
+ * counterpart): for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments and results; nothing is allocated or freed: the task tables
+ * (and rows zero-padded to a supported width) live in a 1 MiB device array
+ * each GPU holds for the life of the process, built on the device from
+ * by-value kernel arguments.  Limits: at most 64 targets, at most 61,440
+ * workgroups (sum over targets of 2^(block bits - chunk_bits - 8)), and for
+ * words_per_row other than 1, 2, 4, 8, 16, 32 at most 98,304 padded words;
+ * beyond them BMM_ERR_INVALID.  Calls that overlap on one GPU share the array
+ * and race: serialise them (one stream, or events). */
+int bmm_hip_gray_coset_min_noalloc(const uint64_t *d_rows, size_t n_rows, size_t words_per_row,
+                                   size_t block_rows, const int32_t *targets, size_t n_targets,
+                                   size_t chunk_bits, uint64_t *d_first_key,
+                                   uint64_t *d_mask_key, void *stream);
+
 /* ---- src/apply: replay of a row-XOR sequence (RowWindow::row_xor,
  * src/core/bit_matrix_rows.cpp:12-20, as driven by GreedyApplier::apply,
  * src/apply/greedy_applier.cpp:92-95).  ops holds n_ops (source, target)
