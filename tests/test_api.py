@@ -7,7 +7,7 @@ import random
 import numpy as np
 import pytest
 
-import bmmpy_amd as bmm
+import bmmpy as bmm
 import oracle
 
 ROWS_6X12 = ["110010101001", "101100101100", "011010011001", "111000010111", "000111100011",
@@ -293,3 +293,34 @@ def test_sa_selector_validation():
     r = bmm.SASelector(seed=0).select(mat, 0)
     assert list(r.rows) == [] and r.seed == 0x9E3779B97F4A7C15
     assert bmm.SASelector().name() == "sa"
+
+
+def test_reference_import_name():
+    """``import bmmpy`` and its submodules resolve to this package, with the
+    reference's public names (bmmpy/__init__.py:65-103, less the algebra
+    package, which this build leaves out)."""
+    import importlib
+
+    import bmmpy_amd
+
+    ref_all = {"ApplyResult", "BitMatrix", "BruteforceSearch", "Candidate", "FwhtSearch",
+               "GreedyApplier", "GlobalGreedyApplier", "MatrixErr", "MatrixError",
+               "MitmFwhtSearch", "CudaMitmFwhtSearch", "CudaBruteforceSearch", "RowWindow",
+               "RuntimeFeatures", "SASelectionResult", "SASelector", "GeneticAlgorithm",
+               "IslandModel", "IslandModelSnapshot", "IslandSnapshot", "IslandSpec",
+               "MigrationPolicy", "OverflowPolicy", "RunStats", "StopCriteria", "ga", "add",
+               "solver", "fixed_weight_masks_u32", "fixed_weight_masks_u64", "fwht_i16",
+               "fwht_i32", "get_runtime_features", "get_version", "matrix_from_rows",
+               "search_apply"}
+    assert ref_all <= set(bmm.__all__)
+    for name in ref_all:
+        assert getattr(bmm, name) is getattr(bmmpy_amd, name)
+    assert importlib.import_module("bmmpy._bmmpy") is bmmpy_amd._bmmpy
+    for mod in ("search", "solver", "ga", "apply", "matrix", "workflows"):
+        assert importlib.import_module(f"bmmpy.{mod}") is getattr(bmmpy_amd, mod)
+    from bmmpy._bmmpy import BitMatrix
+    from bmmpy.candidate import Candidate
+    from bmmpy.search import FwhtSearch
+    assert BitMatrix is bmm.BitMatrix and Candidate is bmm.Candidate
+    assert FwhtSearch is bmm.FwhtSearch
+    assert bmm.__version__ == bmmpy_amd.__version__

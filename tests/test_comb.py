@@ -8,8 +8,8 @@ import math
 import numpy as np
 import pytest
 
-import bmmpy_amd as bmm
-from bmmpy_amd import _bmmpy
+import bmmpy as bmm
+from bmmpy import _bmmpy
 
 
 def test_oracle_known_answers(ora):

@@ -5,8 +5,8 @@ into one Candidate per window row."""
 import numpy as np
 import pytest
 
-import bmmpy_amd as bmm
-from bmmpy_amd import ga as G
+import bmmpy as bmm
+from bmmpy import ga as G
 
 
 def test_records_and_defaults():

@@ -6,7 +6,7 @@ import math
 import numpy as np
 import pytest
 
-from bmmpy_amd import _bmmpy
+from bmmpy import _bmmpy
 
 pytestmark = pytest.mark.gpu
 

@@ -6,8 +6,8 @@ score never exceeds the window's weight."""
 import numpy as np
 import pytest
 
-import bmmpy_amd as bmm
-from bmmpy_amd import ga as G
+import bmmpy as bmm
+from bmmpy import ga as G
 
 pytestmark = pytest.mark.gpu
 

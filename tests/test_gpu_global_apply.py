@@ -120,7 +120,7 @@ def test_row_combine_matches_oracle_mul(lib, ora):
 
 
 def test_bitmatrix_mul_on_gpu(ora):
-    import bmmpy_amd as bmm
+    import bmmpy as bmm
     _say("BitMatrix.mul")
     i3 = bmm.BitMatrix.identity(3)
     assert (i3 @ i3).to_rows() == i3.to_rows()
@@ -141,7 +141,7 @@ def _candidates(bmm, masks, ws):
 
 def test_global_greedy_reference_kat():
     # tests/python/test_matrix_api.py:66-105.
-    import bmmpy_amd as bmm
+    import bmmpy as bmm
     _say("global greedy KAT")
     m = bmm.matrix_from_rows(["111", "110", "001"])
     rank = m.rank()
@@ -157,7 +157,7 @@ def test_global_greedy_reference_kat():
 
 
 def test_global_greedy_golden(gglobal):
-    import bmmpy_amd as bmm
+    import bmmpy as bmm
     for c in range(int(gglobal["gg_cases"][0])):
         t, cols, req, ap, imp = (int(v) for v in gglobal[f"gg{c}_meta"])
         _say("global greedy golden", c, t, cols, req)
@@ -174,7 +174,7 @@ def test_global_greedy_golden(gglobal):
 def test_global_greedy_window_of_larger_matrix(ora):
     """A window over unsorted rows of a bigger matrix: only those rows change,
     as the oracle applies the transform to the extracted rows."""
-    import bmmpy_amd as bmm
+    import bmmpy as bmm
     _say("global greedy sub-window")
     S, A, M = ora.planted(77, 30, 640, 0.05)
     big = np.concatenate([ora.bernoulli(78, 10, 640, 0.5), A])
@@ -195,7 +195,7 @@ def test_global_greedy_window_of_larger_matrix(ora):
 
 
 def test_global_greedy_rejects_out_of_range_mask():
-    import bmmpy_amd as bmm
+    import bmmpy as bmm
     _say("global greedy mask range")
     m = bmm.matrix_from_rows(["10", "01"])
     with pytest.raises(ValueError, match="outside the supported row range"):

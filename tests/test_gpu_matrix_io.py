@@ -13,7 +13,7 @@ def _say(*a):
 
 
 def test_power_matches_repeated_oracle_products(ora):
-    import bmmpy_amd as bmm
+    import bmmpy as bmm
     for n, e in ((1, 0), (5, 1), (16, 7), (70, 13), (128, 100)):
         _say("power", n, e)
         a = ora.bernoulli(500 + n, n, n, 0.2)

@@ -7,7 +7,7 @@ import pytest
 
 from test_api import _bitmatrix, _sa_cases
 
-import bmmpy_amd as bmm
+import bmmpy as bmm
 
 pytestmark = pytest.mark.gpu
 

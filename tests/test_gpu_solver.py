@@ -34,7 +34,7 @@ def _matrix(bmm, m, cols):
                                            (12, 150, 0.3, "deficient"), (16, 300, 0.1, "rand"),
                                            (20, 128, 0.4, "rand"), (24, 200, 0.2, "rand")])
 def test_exact_basis_matches_oracle(lib, ora, t, cols, p, mode):
-    import bmmpy_amd as bmm
+    import bmmpy as bmm
     _say("exact basis", t, cols, mode)
     if mode == "kat":   # the reference's own example (tests/python/test_matrix_api.py:45-65)
         mat = bmm.matrix_from_rows(["111", "110", "001"])

@@ -10,7 +10,7 @@ import struct
 import numpy as np
 import pytest
 
-import bmmpy_amd as bmm
+import bmmpy as bmm
 import oracle
 
 GOLD = os.path.join(os.path.dirname(__file__), "golden")
