@@ -625,6 +625,78 @@ def comb_cpu_baseline(budget_s: float = 10.0) -> dict:
                       f"calls, copied out as uint64"}
 
 
+def _window_of(m: np.ndarray, cols: int):
+    import bmmpy as bmm
+    rows = ["".join("1" if (int(r[c // 64]) >> (c % 64)) & 1 else "0" for c in range(cols))
+            for r in m]
+    return bmm.matrix_from_rows(rows).row_window(list(range(m.shape[0])))
+
+
+TOPK_L = 13   # fwht_topk_kernel: blocks of 2^13 masks, int32 transform in LDS
+
+
+def run_search(torch, steps: int, warmup: int) -> dict:
+    """The reference-API top-k searchers (FwhtSearch engine: persistent
+    workgroups over blocks of 2^13 masks, int32 transform in LDS, exact
+    per-workgroup top-K, one merge).  Cases: MitmFwhtSearch and
+    CudaMitmFwhtSearch on 28 x 96 Bernoulli(0.5) with k = 128 (2^28 masks),
+    MitmFwhtSearch with k = 64 on rows 0..39 of the planted 48 x 4096
+    instance (2^40 masks; the value), BruteforceSearch and
+    CudaBruteforceSearch on 16 x 4096 Bernoulli(0.5) with k = 128.  Per
+    case: host call time and kernel time (events around fwht_topk_kernel);
+    roofline: 13 int32 add/sub per mask (the block transform) against the
+    int32 VALU peak."""
+    import bmmpy as bmm
+    from bmmpy_amd import _lib
+
+    S, A, _ = _lib.gen_planted(WIN_SEED, WIN_ROWS, WIN_COLS, WIN_P)
+    cases = [
+        ("mitm_28x96_k128", bmm.MitmFwhtSearch(max_candidates=128),
+         _lib.gen_bernoulli(0x5EED0012, 28, 96, 0.5), 96, steps),
+        ("cuda_mitm_28x96_k128", bmm.CudaMitmFwhtSearch(max_candidates=128),
+         _lib.gen_bernoulli(0x5EED0012, 28, 96, 0.5), 96, steps),
+        ("mitm_planted_40x4096_k64", bmm.MitmFwhtSearch(max_candidates=64), A[:40], WIN_COLS, 1),
+        ("bruteforce_16x4096_k128", bmm.BruteforceSearch(max_candidates=128),
+         _lib.gen_bernoulli(0x5EED0013, 16, 4096, 0.5), 4096, steps),
+        ("cuda_bruteforce_16x4096_k128", bmm.CudaBruteforceSearch(max_candidates=128),
+         _lib.gen_bernoulli(0x5EED0013, 16, 4096, 0.5), 4096, steps),
+    ]
+    out = {}
+    for name, searcher, m, cols, reps in cases:
+        win = _window_of(m, cols)
+        for _ in range(min(warmup, 2) if reps > 1 else 0):
+            searcher.search(win)
+        _lib.timing_begin()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            res = searcher.search(win)
+        dt = (time.perf_counter() - t0) / reps
+        kms, nk = _lib.timing_end()
+        t = m.shape[0]
+        masks = float((1 << t) - 1)
+        kernel_ms = kms / max(nk, 1)
+        ops = float(1 << t) * min(t, TOPK_L)
+        print(f"search {name}: {1e3 * dt:.2f} ms per call, kernel {kernel_ms:.3f} ms",
+              file=sys.stderr, flush=True)
+        out[name] = {"rows": t, "cols": cols, "k": len(res), "masks": masks,
+                     "call_ms": 1e3 * dt, "masks_per_s": masks / dt,
+                     "kernel_avg_ms": kernel_ms, "kernel_launches": nk,
+                     "kernel_masks_per_s": masks / (kernel_ms / 1e3),
+                     "roofline_frac": ops / (kernel_ms / 1e3) / 1e12 / VALU_PEAK_TOPS,
+                     "best": [int(res[0].weight), int(res[0].mask_u64())] if res else None}
+    main = out["mitm_planted_40x4096_k64"]
+    return {
+        "ms": main["call_ms"], "evals": main["masks"], "results": [out],
+        "config": {"workload": "topk_searchers", "cases": list(out),
+                   "subset_evals_per_step": main["masks"]},
+        "roofline": {"bound": "valu", "achieved": main["roofline_frac"] * VALU_PEAK_TOPS,
+                     "peak": VALU_PEAK_TOPS, "unit": "TOPS (int32 VALU)",
+                     "frac": main["roofline_frac"], "traffic": None,
+                     "ops_per_eval": TOPK_L, "kernel": "fwht_topk_kernel<13, 2048>",
+                     "kernel_avg_ms": main["kernel_avg_ms"]},
+    }
+
+
 def secondary(torch, _lib, dev, stream) -> dict:
     """The other configured workloads, timed beside the headline: the FWHT
     coset minima (graph replay, autotuned launch shape), one 2^40 window of
@@ -733,7 +805,7 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--workload", default="gray",
                     choices=["gray", "coset", "batch", "window", "basis", "scores", "sa", "comb",
-                             "ga"],
+                             "ga", "search"],
                     help="gray: the Gray coset sweep (default); coset / batch / window: the "
                          "FWHT coset-minimum workloads; ga: plain vs learned GA")
     ap.add_argument("--ga-budget", type=float, default=60.0,
@@ -789,6 +861,10 @@ def main() -> int:
         return 0
 
     dev = torch.device("cuda:0")
+    if args.workload == "search":
+        r = run_search(torch, args.steps, args.warmup)
+        print(json.dumps(mode_line(args, r, None)))
+        return 0
     if args.workload in ("coset", "batch", "window", "basis", "scores", "sa", "comb"):
         stream = torch.cuda.current_stream(dev)
         if args.workload == "comb":

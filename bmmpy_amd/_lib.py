@@ -227,13 +227,18 @@ def fwht_i16(x) -> np.ndarray:
     return a
 
 
+def _cap(k: int, t: int) -> int:
+    """Result capacity: min(k, 2^t - 1) records (at least one)."""
+    return max(1, k if t >= 63 else min(k, (1 << t) - 1))
+
+
 def _entries(out, n):
     return [(int(out["weight"][i]), int(out["mask"][i])) for i in range(n)]
 
 
 def fwht_topk(rows, cols: int, k: int):
     rows = _rows(rows)
-    out = np.zeros(max(k, 1), dtype=TOPK_DTYPE)
+    out = np.zeros(_cap(k, rows.shape[0]), dtype=TOPK_DTYPE)
     n = _sz()
     call("bmm_hip_fwht_topk", rows, rows.shape[0], rows.shape[1], cols, k, out, C.byref(n))
     return _entries(out, n.value)
@@ -243,7 +248,7 @@ def mitm_topk(q, r, mult, low_bits: int, high_bits: int, total: int, k: int):
     q = np.ascontiguousarray(q, dtype=np.uint64)
     r = np.ascontiguousarray(r, dtype=np.uint64)
     mult = np.ascontiguousarray(mult, dtype=np.int32)
-    out = np.zeros(max(k, 1), dtype=TOPK_DTYPE)
+    out = np.zeros(_cap(k, low_bits + high_bits), dtype=TOPK_DTYPE)
     n = _sz()
     call("bmm_hip_mitm_topk", q, r, mult, q.size, low_bits, high_bits, total, k, out,
          C.byref(n))
@@ -252,7 +257,7 @@ def mitm_topk(q, r, mult, low_bits: int, high_bits: int, total: int, k: int):
 
 def bruteforce_topk(rows, k: int, chunk_bits: int = 0):
     rows = _rows(rows)
-    out = np.zeros(max(k, 1), dtype=TOPK_DTYPE)
+    out = np.zeros(_cap(k, rows.shape[0]), dtype=TOPK_DTYPE)
     n = _sz()
     call("bmm_hip_bruteforce_topk", rows, rows.shape[0], rows.shape[1], chunk_bits, k, out,
          C.byref(n))

@@ -58,6 +58,8 @@ void require_device() {
 }
 
 // Implemented in the kernel translation units.
+std::vector<bmm_topk_entry> ranked_topk(const uint32_t* pat, const int32_t* mult, size_t U, int t,
+                                        int32_t total, size_t k);
 template <typename T> void fwht_array_dev(T* d, size_t n, hipStream_t stream);
 void gray_coset_min_dev(const uint64_t* d_rows, size_t n_rows, size_t words_per_row,
                         size_t block_rows, const int32_t* targets, size_t n_targets,
@@ -478,6 +480,41 @@ int bmm_hip_coset_min(const uint64_t* row_words, size_t n_rows, size_t words_per
     return bmm_hip_coset_min_batch(row_words, &woff, &r, &c, 1, &ooff, out_weight, out_index);
 }
 
+// Column patterns of a window (bit r = row r) for the full-ranking path,
+// one entry per active column among the first `cols` bit positions.
+void window_patterns(const uint64_t* row_words, size_t rows, size_t words_per_row, size_t cols,
+                     std::vector<uint32_t>& pat, std::vector<int32_t>& mult) {
+    for (size_t wi = 0; wi < words_per_row && wi * 64 < cols; ++wi) {
+        uint64_t active = 0;
+        for (size_t r = 0; r < rows; ++r)
+            active |= row_words[r * words_per_row + wi];
+        const size_t left = cols - wi * 64;
+        if (left < 64)
+            active &= (1ull << left) - 1;
+        while (active) {
+            const int bit = __builtin_ctzll(active);
+            uint32_t p = 0;
+            for (size_t r = 0; r < rows; ++r)
+                p |= (uint32_t)((row_words[r * words_per_row + wi] >> bit) & 1u) << r;
+            pat.push_back(p);
+            active &= active - 1;
+        }
+    }
+    if (pat.size() > (size_t)INT32_MAX)
+        fail(BMM_ERR_OVERFLOW, "topk: total weight exceeds int32 range");
+    mult.assign(pat.size(), 1);
+}
+
+std::vector<bmm_topk_entry> ranked_rows(const uint64_t* row_words, size_t rows,
+                                        size_t words_per_row, size_t cols, size_t k) {
+    if (rows > 30)
+        fail(BMM_ERR_INVALID, "max_candidates above 4096 needs a window of at most 30 rows");
+    std::vector<uint32_t> pat;
+    std::vector<int32_t> mult;
+    window_patterns(row_words, rows, words_per_row, cols, pat, mult);
+    return ranked_topk(pat.data(), mult.data(), pat.size(), (int)rows, (int32_t)pat.size(), k);
+}
+
 int bmm_hip_fwht_topk(const uint64_t* row_words, size_t rows, size_t words_per_row,
                       size_t cols, size_t k, bmm_topk_entry* out, size_t* n_out) {
     return guarded([&] {
@@ -490,12 +527,14 @@ int bmm_hip_fwht_topk(const uint64_t* row_words, size_t rows, size_t words_per_r
             fail(BMM_ERR_INVALID, "fwht_topk: null rows");
         if (rows > 48)
             fail(BMM_ERR_INVALID, "fwht_topk: window rows must be <= 48");
-        if (k > BMM_TOPK_MAX)
-            fail(BMM_ERR_INVALID, "fwht_topk: max_candidates exceeds the GPU top-k capacity");
         if (cols > words_per_row * 64)
             fail(BMM_ERR_INVALID, "fwht_topk: cols exceeds words_per_row * 64");
+        if (k > BMM_TOPK_MAX && rows > 30)
+            fail(BMM_ERR_INVALID, "fwht_topk: max_candidates above 4096 needs a window of at "
+                                  "most 30 rows");
         require_device();
-        auto res = fwht_topk_rows(row_words, rows, words_per_row, cols, k);
+        auto res = k > BMM_TOPK_MAX ? ranked_rows(row_words, rows, words_per_row, cols, k)
+                                    : fwht_topk_rows(row_words, rows, words_per_row, cols, k);
         std::copy(res.begin(), res.end(), out);
         *n_out = res.size();
     });
@@ -515,8 +554,9 @@ int bmm_hip_mitm_topk(const uint64_t* q, const uint64_t* r, const int32_t* mult,
         const size_t t = low_bits + high_bits;
         if (t == 0 || t > 48 || low_bits > 48 || high_bits > 48)
             fail(BMM_ERR_INVALID, "mitm_topk: low_bits + high_bits must be in [1, 48]");
-        if (k > BMM_TOPK_MAX)
-            fail(BMM_ERR_INVALID, "mitm_topk: max_candidates exceeds the GPU top-k capacity");
+        if (k > BMM_TOPK_MAX && t > 30)
+            fail(BMM_ERR_INVALID, "mitm_topk: max_candidates above 4096 needs a window of at "
+                                  "most 30 rows");
         int64_t sum = 0;
         for (size_t i = 0; i < unique; ++i) {
             if ((low_bits < 64 && r[i] >> low_bits) || (high_bits < 64 && q[i] >> high_bits))
@@ -529,7 +569,15 @@ int bmm_hip_mitm_topk(const uint64_t* q, const uint64_t* r, const int32_t* mult,
         if (sum != total)
             fail(BMM_ERR_INVALID, "mitm_topk: total_weight differs from the multiplicity sum");
         require_device();
-        auto res = mitm_topk_split(q, r, mult, unique, low_bits, high_bits, total, k);
+        std::vector<bmm_topk_entry> res;
+        if (k > BMM_TOPK_MAX) {
+            std::vector<uint32_t> pat(unique);
+            for (size_t i = 0; i < unique; ++i)
+                pat[i] = (uint32_t)((q[i] << low_bits) | r[i]);
+            res = ranked_topk(pat.data(), mult, unique, (int)t, total, k);
+        } else {
+            res = mitm_topk_split(q, r, mult, unique, low_bits, high_bits, total, k);
+        }
         std::copy(res.begin(), res.end(), out);
         *n_out = res.size();
     });
@@ -549,11 +597,14 @@ int bmm_hip_bruteforce_topk(const uint64_t* row_words, size_t rows, size_t words
             fail(BMM_ERR_INVALID, "bruteforce_topk: window rows must be <= 48");
         if (chunk_bits != 0 && (chunk_bits > rows || chunk_bits >= 64))
             fail(BMM_ERR_INVALID, "bruteforce_topk: chunk_bits must be in [1, rows] or 0");
-        if (k > BMM_TOPK_MAX)
-            fail(BMM_ERR_INVALID, "bruteforce_topk: max_candidates exceeds the GPU top-k capacity");
+        if (k > BMM_TOPK_MAX && rows > 30)
+            fail(BMM_ERR_INVALID, "bruteforce_topk: max_candidates above 4096 needs a window of "
+                                  "at most 30 rows");
         require_device();
         // Popcount over all words_per_row words, tail bits included.
-        auto res = fwht_topk_rows(row_words, rows, words_per_row, words_per_row * 64, k);
+        auto res = k > BMM_TOPK_MAX
+                       ? ranked_rows(row_words, rows, words_per_row, words_per_row * 64, k)
+                       : fwht_topk_rows(row_words, rows, words_per_row, words_per_row * 64, k);
         std::copy(res.begin(), res.end(), out);
         *n_out = res.size();
     });

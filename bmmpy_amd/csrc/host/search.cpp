@@ -30,10 +30,11 @@ std::vector<Candidate> to_candidates(const std::vector<bmm_topk_entry>& e, std::
     return out;
 }
 
-void check_k(std::size_t k, const char* who) {
-    if (k > k_gpu_topk_max)
-        throw std::invalid_argument(std::string(who) +
-                                    ": max_candidates exceeds the GPU top-k capacity (4096)");
+// Result capacity: min(k, 2^t - 1) records.
+std::size_t out_capacity(std::size_t k, std::size_t t) {
+    if (t >= 63)
+        return k;
+    return std::min<std::size_t>(k, (std::size_t{1} << t) - 1);
 }
 
 } // namespace
@@ -50,9 +51,8 @@ std::vector<Candidate> FwhtSearch::search(const RowWindow& window) {
     if (window.cols() > (std::size_t)std::numeric_limits<std::int32_t>::max())
         throw std::overflow_error("FwhtSearch: total weight exceeds int32 range");
     const std::size_t k = _config.max_candidates;
-    check_k(k, "FwhtSearch");
     const auto rows = window.packed_rows();
-    std::vector<bmm_topk_entry> out(k);
+    std::vector<bmm_topk_entry> out(out_capacity(k, t));
     std::size_t n = 0;
     throw_status(bmm_hip_fwht_topk(rows.data(), t, window.words_per_row(), window.cols(), k,
                                    out.data(), &n));
@@ -109,7 +109,7 @@ CompactSplitWindow build_compact_split_window(const RowWindow& window, std::size
 
 namespace {
 std::vector<Candidate> run_mitm(const CompactSplitWindow& prep, std::size_t k) {
-    std::vector<bmm_topk_entry> out(k);
+    std::vector<bmm_topk_entry> out(out_capacity(k, prep.t));
     std::size_t n = 0;
     throw_status(bmm_hip_mitm_topk(prep.q.data(), prep.r.data(), prep.multiplicity.data(),
                                    prep.q.size(), prep.low_bits, prep.high_bits,
@@ -134,7 +134,6 @@ std::vector<Candidate> MitmFwhtSearch::search(const RowWindow& window) {
         throw std::invalid_argument("MitmFwhtSearch: low half exceeds supported width");
     if (low >= 31 || high >= 31)
         throw std::invalid_argument("MitmFwhtSearch: split dimensions must be < 31");
-    check_k(_config.max_candidates, "MitmFwhtSearch");
     const CompactSplitWindow prep = build_compact_split_window(window, low);
     if (prep.multiplicity.empty())
         return {};
@@ -158,9 +157,8 @@ std::vector<Candidate> BruteforceSearch::search(const RowWindow& window) {
         throw std::invalid_argument(
             "BruteforceSearch: chunk_bits must be in [1, min(t, 63)] or 0 for auto");
     const std::size_t k = _config.max_candidates;
-    check_k(k, "BruteforceSearch");
     const auto rows = window.packed_rows();
-    std::vector<bmm_topk_entry> out(k);
+    std::vector<bmm_topk_entry> out(out_capacity(k, t));
     std::size_t n = 0;
     throw_status(bmm_hip_bruteforce_topk(rows.data(), t, window.words_per_row(),
                                          _config.chunk_bits, k, out.data(), &n));

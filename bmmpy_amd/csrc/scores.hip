@@ -33,8 +33,20 @@
 #include <algorithm>
 #include <array>
 #include <type_traits>
+#include <vector>
 
 namespace bmmhip {
+
+template <typename T> void fwht_array_dev(T* d, size_t n, hipStream_t stream);
+
+// Device results of one scores-and-order call (this thread's workspace).
+template <typename S> struct ScoresDev {
+    S* s;
+    int32_t* order;
+    uint32_t* cnt;
+    uint32_t* off;
+};
+
 namespace {
 
 constexpr int kRsTile = 4096;   // masks per tile
@@ -440,9 +452,10 @@ void scan_u32(uint32_t* d, uint64_t L, bool inclusive, std::array<DevBuf, 6>& le
 
 } // namespace
 
+// Device-resident core: d_h (m entries) on the device; the results stay in
+// this thread's workspace (valid until the next call).
 template <typename S>
-void scores_order(const S* h, size_t m, int32_t n, S* s_by_mask, int32_t* order, int32_t* cnt,
-                  int32_t* off) {
+ScoresDev<S> scores_order_dev(const S* d_h, size_t m, int32_t n) {
     ScoresWorkspace& ws = scores_ws();
     const size_t buckets = (size_t)n + 1;
     const uint64_t N = m - 1;   // masks 1 .. m-1
@@ -467,13 +480,11 @@ void scores_order(const S* h, size_t m, int32_t n, S* s_by_mask, int32_t* order,
                 dbits = d;
                 packed = true;
             }
-    S* d_h = ws.h.get<S>(m);
     S* d_s = ws.s.get<S>(m);
     uint32_t* d_cnt = ws.cnt.get<uint32_t>(buckets);
     uint32_t* d_off = ws.off.get<uint32_t>(buckets);
     uint32_t* d_ovf = ws.overflow.get<uint32_t>(1);
     uint32_t* d_hist = ws.hist.get<uint32_t>((size_t)256 * std::max(tiles_u, tiles_p));
-    BMM_HIP(hipMemcpy(d_h, h, m * sizeof(S), hipMemcpyHostToDevice));
     fill_u32(d_cnt, buckets, 0, nullptr);
     fill_u32(d_ovf, 1, 0, nullptr);
     const bool lds = buckets <= (size_t)kLdsBuckets;
@@ -539,13 +550,89 @@ void scores_order(const S* h, size_t m, int32_t n, S* s_by_mask, int32_t* order,
             vin = vout;
         }
     }
-    if (N > 0)
-        BMM_HIP(hipMemcpy(order, d_order, N * 4, hipMemcpyDeviceToHost));
     BMM_HIP(hipMemcpy(d_off, d_cnt, buckets * 4, hipMemcpyDeviceToDevice));
     scan_u32(d_off, buckets, true, ws.levels, 0, nullptr);
-    BMM_HIP(hipMemcpy(s_by_mask, d_s, m * sizeof(S), hipMemcpyDeviceToHost));
-    BMM_HIP(hipMemcpy(cnt, d_cnt, buckets * 4, hipMemcpyDeviceToHost));
-    BMM_HIP(hipMemcpy(off, d_off, buckets * 4, hipMemcpyDeviceToHost));
+    return ScoresDev<S>{d_s, d_order, d_cnt, d_off};
+}
+
+template <typename S>
+void scores_order(const S* h, size_t m, int32_t n, S* s_by_mask, int32_t* order, int32_t* cnt,
+                  int32_t* off) {
+    ScoresWorkspace& ws = scores_ws();
+    S* d_h = ws.h.get<S>(m);
+    BMM_HIP(hipMemcpy(d_h, h, m * sizeof(S), hipMemcpyHostToDevice));
+    const ScoresDev<S> r = scores_order_dev<S>(d_h, m, n);
+    const size_t buckets = (size_t)n + 1;
+    if (m > 1)
+        BMM_HIP(hipMemcpy(order, r.order, (m - 1) * 4, hipMemcpyDeviceToHost));
+    BMM_HIP(hipMemcpy(s_by_mask, r.s, m * sizeof(S), hipMemcpyDeviceToHost));
+    BMM_HIP(hipMemcpy(cnt, r.cnt, buckets * 4, hipMemcpyDeviceToHost));
+    BMM_HIP(hipMemcpy(off, r.off, buckets * 4, hipMemcpyDeviceToHost));
+}
+
+namespace {
+// b[pattern] += multiplicity (the column-pattern histogram of a window).
+__global__ void pattern_hist_kernel(const uint32_t* __restrict__ pat,
+                                    const int32_t* __restrict__ mult, size_t U,
+                                    int32_t* __restrict__ b) {
+    const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (i < U)
+        atomicAdd(&b[pat[i]], mult[i]);
+}
+
+struct RankWorkspace {
+    DevBuf b, pat, mult;
+};
+
+RankWorkspace& rank_ws() {
+    thread_local RankWorkspace ws;
+    return ws;
+}
+} // namespace
+
+// The first min(k, 2^t - 1) masks of a t-row window in (weight, mask)
+// order, for any k: the column-pattern histogram (U distinct t-bit patterns
+// with multiplicities, total = their sum) is built on the device, transformed
+// in HBM (weight(m) = (total - H[m]) / 2, src/search/fwht_search.cpp:59-112),
+// and calc_scores_and_order's stable radix order by score = weight (ties by
+// ascending mask) is the reference's order (fwht_search.cpp:15-22).
+std::vector<bmm_topk_entry> ranked_topk(const uint32_t* pat, const int32_t* mult, size_t U, int t,
+                                        int32_t total, size_t k) {
+    if (t < 1 || t > 30)
+        fail(BMM_ERR_INVALID, "max_candidates above 4096 needs a window of 1 to 30 rows");
+    if (total < 0 || total > (1 << 28))
+        fail(BMM_ERR_INVALID, "max_candidates above 4096 needs a window of at most 2^28 columns");
+    const size_t m = (size_t)1 << t;
+    RankWorkspace& ws = rank_ws();
+    int32_t* d_b = ws.b.get<int32_t>(m);
+    fill_u32(reinterpret_cast<uint32_t*>(d_b), m, 0u, nullptr);
+    if (U > 0) {
+        uint32_t* d_pat = ws.pat.get<uint32_t>(U);
+        int32_t* d_mult = ws.mult.get<int32_t>(U);
+        BMM_HIP(hipMemcpy(d_pat, pat, U * 4, hipMemcpyHostToDevice));
+        BMM_HIP(hipMemcpy(d_mult, mult, U * 4, hipMemcpyHostToDevice));
+        hipLaunchKernelGGL(pattern_hist_kernel, dim3((unsigned)((U + 255) / 256)), dim3(256), 0,
+                           nullptr, d_pat, d_mult, U, d_b);
+        BMM_HIP(hipGetLastError());
+    }
+    fwht_array_dev<int32_t>(d_b, m, nullptr);
+    const ScoresDev<int32_t> r = scores_order_dev<int32_t>(d_b, m, total);
+    const size_t K = std::min(k, m - 1);
+    std::vector<int32_t> order(K);
+    std::vector<uint32_t> off((size_t)total + 1);
+    if (K)
+        BMM_HIP(hipMemcpy(order.data(), r.order, K * 4, hipMemcpyDeviceToHost));
+    BMM_HIP(hipMemcpy(off.data(), r.off, off.size() * 4, hipMemcpyDeviceToHost));
+    std::vector<bmm_topk_entry> out(K);
+    uint32_t v = 0;
+    for (size_t i = 0; i < K; ++i) {
+        while (off[v] <= i)
+            ++v;
+        out[i].mask = (uint64_t)(uint32_t)order[i];
+        out[i].weight = v;
+        out[i].pad = 0;
+    }
+    return out;
 }
 
 template void scores_order<int32_t>(const int32_t*, size_t, int32_t, int32_t*, int32_t*, int32_t*,

@@ -83,8 +83,13 @@ int bmm_hip_fwht_i16_dev(int16_t *d_data, size_t n, void *stream);
 
 /* ---- FwhtSearch::search inner loops (src/search/fwht_search.cpp:53-131) -
  * Histogram of active-column patterns, FWHT, exact top-k over masks
- * 1..2^rows-1.  Writes min(k, 2^rows-1) entries to `out` (capacity k) and the
- * count to *n_out.  rows <= 40, k <= BMM_TOPK_MAX. */
+ * 1..2^rows-1.  Writes min(k, 2^rows-1) entries to `out` (capacity at least
+ * that many) and the count to *n_out.  rows <= 48.  k <= BMM_TOPK_MAX runs the
+ * on-chip top-k; any larger k (the reference takes any max_candidates,
+ * src/search/fwht_search.cpp:95-131) ranks every mask through the column
+ * histogram's transform in HBM and calc_scores_and_order's stable order,
+ * for windows of at most 30 rows.  The same holds for bmm_hip_mitm_topk and
+ * bmm_hip_bruteforce_topk. */
 #define BMM_TOPK_MAX 4096
 int bmm_hip_fwht_topk(const uint64_t *row_words, size_t rows, size_t words_per_row,
                       size_t cols, size_t k, bmm_topk_entry *out, size_t *n_out);

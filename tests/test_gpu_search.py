@@ -105,3 +105,54 @@ def test_mitm_planted_reduced_windows(lib, ora, gold):
         others, t = case["others"], case["target"]
         win = np.concatenate([A[others], A[t:t + 1]])
         assert gpu_search(lib, ora, "mitm", win, 4096, 64) == res_of(case["result"])
+
+
+@pytest.mark.parametrize("kind", ["fwht", "mitm", "bruteforce"])
+@pytest.mark.parametrize("t,cols,k", [(14, 300, 5000), (16, 1000, 65535), (20, 700, 123457)])
+def test_large_k_full_ranking(lib, ora, kind, t, cols, k):
+    """max_candidates above the on-chip top-k capacity (the reference takes
+    any k): the whole (weight, mask) ranking from the HBM transform and the
+    stable score order, against the oracle; k = 2^16 - 1 is every mask."""
+    _say("large k", kind, t, cols, k)
+    m = lib.gen_bernoulli(31 * t + k % 97, t, cols, 0.3)
+    if kind == "bruteforce" and cols % 64:
+        m[:, -1] |= np.uint64(1) << np.uint64(63)          # tail bits count here
+    got = gpu_search(lib, ora, kind, m, cols, k)
+    exp = {"fwht": ora.fwht_topk, "mitm": ora.mitm_topk,
+           "bruteforce": ora.bruteforce_topk}[kind](m, cols, k)
+    assert len(got) == min(k, 2 ** t - 1)
+    assert got == exp
+
+
+def test_large_k_every_mask_sorted_by_brute_force(lib):
+    """All 2^12 - 1 masks of a 12-row window: equal to sorting (weight, mask)
+    of directly computed XOR popcounts."""
+    m = lib.gen_bernoulli(4242, 12, 200, 0.4)
+    got = lib.fwht_topk(m, 200, 10 ** 6)
+    exp = []
+    for mask in range(1, 1 << 12):
+        acc = np.zeros(m.shape[1], np.uint64)
+        for r in range(12):
+            if (mask >> r) & 1:
+                acc ^= m[r]
+        exp.append((int(np.unpackbits(acc.view(np.uint8)).sum()), mask))
+    assert got == sorted(exp)
+
+
+def test_large_k_limits_and_searcher_api(lib):
+    import bmmpy as bmm
+    m = lib.gen_bernoulli(5, 31, 128, 0.3)
+    with pytest.raises(ValueError, match="at most 30 rows"):
+        lib.fwht_topk(m, 128, 5000)
+    rows = ["".join("1" if (int(r[c // 64]) >> (c % 64)) & 1 else "0" for c in range(128))
+            for r in m[:18]]
+    win = bmm.matrix_from_rows(rows).row_window(list(range(18)))
+    a = bmm.FwhtSearch(max_rows=18, max_candidates=20000).search(win)
+    b = bmm.MitmFwhtSearch(max_candidates=20000).search(win)
+    c = bmm.BruteforceSearch(max_candidates=20000).search(win)
+    assert len(a) == 20000
+    assert [(x.weight, x.mask_u64()) for x in a] == [(x.weight, x.mask_u64()) for x in b]
+    assert [(x.weight, x.mask_u64()) for x in a] == [(x.weight, x.mask_u64()) for x in c]
+    assert [(x.weight, x.mask_u64()) for x in a[:4096]] == \
+        [(x.weight, x.mask_u64()) for x in bmm.FwhtSearch(max_rows=18,
+                                                            max_candidates=4096).search(win)]
