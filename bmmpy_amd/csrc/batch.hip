@@ -448,6 +448,156 @@ __global__ __launch_bounds__(256) void coset_phase2_kernel(
     }
 }
 
+// ----------------------------------------------- single-launch form -----
+// Small batches (every matrix at most kPatRegs * 256 columns, few blocks):
+// one workgroup per block of 2^L masks (L = min(rows, 13)), one launch in
+// all.  The workgroup reads its matrix's column patterns from the rows, builds
+// and transforms its block, and takes for every target the block's best
+// element among the masks that hold the target bit as the packed key
+// K = H * 2^L + (2^L - 1 - ml) (largest H, then lowest ml); the exact
+// (weight << 32 | mask) then goes to the target's key with one atomicMin, so
+// the key ends at the minimum weight and its lowest mask and no second pass
+// is needed.  The last workgroup to finish (completion counter) turns the
+// keys into the outputs and clears keys and counter for the next launch.
+constexpr int kFusedMaxRows = 31;
+constexpr int64_t kFusedMaxBlocks = 1024;
+constexpr int kFusedL = 13;   // 2^13-mask blocks (2^12: 256 workgroups, measured no faster)
+
+template <int L>
+__device__ void coset_block_exact(int32_t* s, const CosetMatrix& M, int32_t total, uint32_t mh,
+                                  const uint32_t (&preg)[kPatRegs],
+                                  unsigned long long* __restrict__ keys) {
+    constexpr int N = 1 << L;
+    constexpr int E = N >= 256 ? N / 256 : 1;
+    build_from_regs<L>(s, preg, mh);
+    block_fwht<L>(s);
+    const int tid = threadIdx.x;
+    int32_t best[kFusedMaxRows];
+#pragma unroll
+    for (int b = 0; b < kFusedMaxRows; ++b)
+        best[b] = INT32_MIN;
+    int32_t overall = INT32_MIN;
+    if (tid < N) {
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const int ml = e * 256 + tid;
+            const int32_t k = s[fwht_pad(ml)] * N + (N - 1 - ml);
+            overall = max(overall, k);
+#pragma unroll
+            for (int b = 0; b < L; ++b)
+                if ((ml >> b) & 1)
+                    best[b] = max(best[b], k);
+        }
+    }
+    __shared__ int32_t red[4][kFusedMaxRows + 1];
+    const int wv = tid / kWave, lane = tid % kWave;
+#pragma unroll
+    for (int b = 0; b < L; ++b) {
+        const int32_t v = wave_max_i32(best[b]);
+        if (lane == 0)
+            red[wv][b] = v;
+    }
+    {
+        const int32_t v = wave_max_i32(overall);
+        if (lane == 0)
+            red[wv][kFusedMaxRows] = v;
+    }
+    __syncthreads();
+    if (tid < M.rows) {
+        const int b = tid;
+        const int row = b < L ? b : kFusedMaxRows;
+        const bool has = b < L || ((mh >> (b - L)) & 1u);
+        const int32_t k = max(max(red[0][row], red[1][row]), max(red[2][row], red[3][row]));
+        if (has && k != INT32_MIN) {
+            const int32_t h = k >> L;   // k = h * 2^L + (N - 1 - ml), 0 <= N - 1 - ml < 2^L
+            const uint32_t ml = (uint32_t)(N - 1) - ((uint32_t)k & (uint32_t)(N - 1));
+            const uint32_t w = (uint32_t)(total - h) >> 1;
+            const unsigned long long key =
+                ((unsigned long long)w << 32) | (((unsigned long long)mh << L) | ml);
+            atomicMin(&keys[M.out_off + b], key);
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void coset_fused_kernel(
+    const uint64_t* __restrict__ words, const CosetMatrix* __restrict__ mats,
+    const int64_t* __restrict__ prefix, int n_mat, const int32_t* __restrict__ slot_matrix,
+    int n_slots, unsigned long long* __restrict__ keys, unsigned int* __restrict__ done,
+    uint32_t* __restrict__ out_w, uint64_t* __restrict__ out_idx) {
+    __shared__ __align__(16) int32_t s[fwht_lds_words(kBatchMaxL) + 4];
+    __shared__ uint32_t cnt[4];
+    __shared__ int last;
+    const int64_t g = blockIdx.x;
+    int lo = 0, hi = n_mat - 1;     // prefix[lo] <= g < prefix[lo + 1]
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (prefix[mid] <= g)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    const CosetMatrix M = mats[lo];
+    const uint32_t mh = (uint32_t)(g - prefix[lo]);
+    // The matrix (at most 31 x 32 words) goes to LDS in one coalesced pass,
+    // then each thread forms the patterns of its columns from there.
+    const uint64_t* rows = words + M.word_off;
+    uint64_t* sw = reinterpret_cast<uint64_t*>(s);
+    const int nw = M.rows * M.words;
+    for (int i = (int)threadIdx.x; i < nw; i += blockDim.x)
+        sw[i] = rows[i];
+    __syncthreads();
+    uint32_t preg[kPatRegs];
+    uint32_t active = 0;
+#pragma unroll
+    for (int k = 0; k < kPatRegs; ++k) {
+        const int c = (int)threadIdx.x + 256 * k;
+        uint32_t p = 0;
+        if (c < M.cols) {
+            const uint64_t* col = sw + (c >> 6);
+            for (int i = 0; i < M.rows; ++i)
+                p |= (uint32_t)((col[i * M.words] >> (c & 63)) & 1ull) << i;
+        }
+        preg[k] = p;
+        active += p != 0u;
+    }
+    active = wave_sum_u32(active);
+    if (threadIdx.x % kWave == 0)
+        cnt[threadIdx.x / kWave] = active;
+    __syncthreads();   // patterns read before the block build clears LDS
+    const int32_t total = (int32_t)(cnt[0] + cnt[1] + cnt[2] + cnt[3]);
+    switch (M.L) {
+#define BMM_CASE(LL)                                                                           \
+    case LL: coset_block_exact<LL>(s, M, total, mh, preg, keys); break;
+        BMM_CASE(1) BMM_CASE(2) BMM_CASE(3) BMM_CASE(4) BMM_CASE(5) BMM_CASE(6) BMM_CASE(7)
+        BMM_CASE(8) BMM_CASE(9) BMM_CASE(10) BMM_CASE(11) BMM_CASE(12) BMM_CASE(13)
+#undef BMM_CASE
+    default: break;
+    }
+    // Completion: each workgroup takes a ticket with release semantics after
+    // its key updates; the last one (acquire) sees all of them.
+    __syncthreads();
+    if (threadIdx.x == 0)
+        last = __hip_atomic_fetch_add(done, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) ==
+               gridDim.x - 1;
+    __syncthreads();
+    if (!last)
+        return;
+    for (int slot = threadIdx.x; slot < n_slots; slot += blockDim.x) {
+        const unsigned long long key =
+            __hip_atomic_load(&keys[slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const CosetMatrix& S = mats[slot_matrix[slot]];
+        const int target = (int)(slot - S.out_off);
+        const uint64_t mask = (uint32_t)key;
+        const uint64_t below = mask & ((1ull << target) - 1ull);
+        const uint64_t above = mask >> (target + 1);
+        out_w[slot] = (uint32_t)(key >> 32);
+        out_idx[slot] = below | (above << target);
+        keys[slot] = ~0ull;
+    }
+    if (threadIdx.x == 0)
+        *done = 0u;
+}
+
 int default_persistent_wgs();
 
 // ------------------------------------------------------------- plan ------
@@ -463,10 +613,11 @@ struct BatchPlan {
     int grid = 0;
     int chunk = 1;
     int lmax = kBatchMaxL;
+    bool fused = false;         // single-launch form (coset_fused_kernel)
     const uint64_t* d_words = nullptr;
     uint32_t* d_out_w = nullptr;
     uint64_t* d_out_idx = nullptr;
-    DevBuf mats, prefix, pats, keys, counter, slot_matrix;
+    DevBuf mats, prefix, pats, keys, counter, slot_matrix, done;
     std::vector<int32_t> rows, cols;
     std::vector<int64_t> word_off, out_off;
     hipStream_t cap_stream = nullptr;
@@ -556,6 +707,14 @@ void plan_enqueue(BatchPlan& P, hipStream_t s, bool timed) {
     unsigned long long* d_k = static_cast<unsigned long long*>(P.keys.ptr);
     unsigned long long* d_c = static_cast<unsigned long long*>(P.counter.ptr);
     int32_t* d_s = static_cast<int32_t*>(P.slot_matrix.ptr);
+    if (P.fused) {
+        ScopedKernelTimer timer(s, timed);
+        hipLaunchKernelGGL(coset_fused_kernel, dim3((unsigned)P.total_blocks), dim3(256), 0, s,
+                           P.d_words, d_m, d_p, (int)P.n_mat, d_s, (int)P.n_slots, d_k,
+                           static_cast<unsigned int*>(P.done.ptr), P.d_out_w, P.d_out_idx);
+        BMM_HIP(hipGetLastError());
+        return;
+    }
     hipLaunchKernelGGL(batch_patterns_kernel, dim3((unsigned)P.n_mat), dim3(256), 0, s, P.d_words,
                        d_m, d_pat);
     BMM_HIP(hipGetLastError());
@@ -690,6 +849,21 @@ BatchPlan* batch_plan_create(const uint64_t* d_words, const int64_t* word_offset
         sig.push_back(cols[m]);
     }
     BMM_HIP(hipGetDevice(&P->device));
+    // Small batches: the single-launch form (no tuning needed).
+    int64_t fused_blocks = 0;
+    bool narrow = true;
+    for (size_t m = 0; m < n_mat; ++m) {
+        fused_blocks += int64_t(1) << (rows[m] - std::min(rows[m], kFusedL));
+        narrow = narrow && cols[m] <= kPatRegs * 256;
+    }
+    if (narrow && fused_blocks <= kFusedMaxBlocks) {
+        plan_upload(*P, kFusedL);
+        fill_u32(P->done.get<uint32_t>(1), 1, 0u, nullptr);
+        P->fused = true;
+        P->grid = (int)P->total_blocks;
+        P->chunk = 1;
+        return P.release();
+    }
     auto it = tune_cache().find(sig);
     if (autotune && it == tune_cache().end()) {
         plan_autotune(*P);
@@ -734,6 +908,7 @@ void batch_plan_set_shape(BatchPlan* P, int max_block_bits, int persistent_wgs, 
              "batch_plan_set_shape: 14-bit (int16) blocks need at most 32767 columns");
     if (persistent_wgs < 0 || chunk < 0)
         fail(BMM_ERR_INVALID, "batch_plan_set_shape: negative launch size");
+    P->fused = false;
     if (P->exec) {
         BMM_HIP(hipStreamSynchronize(P->cap_stream));
         BMM_HIP(hipGraphExecDestroy(P->exec));

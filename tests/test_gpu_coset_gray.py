@@ -326,3 +326,69 @@ def test_batch_4096_large_matrices_vs_golden(lib, bits, graph):
         assert np.array_equal(w[o:o + r], g["weights"][k:k + r]), (m, r)
         assert np.array_equal(i[o:o + r], g["indices"][k:k + r]), (m, r)
         k += r
+
+
+def test_small_batch_single_launch_vs_oracle_and_staged(lib, ora):
+    """Small batches run as one launch (every workgroup one block, exact
+    per-target lowest mask, the last workgroup writes the outputs): against
+    the oracle, against the staged three-launch plan (set_shape), and over
+    repeated graph replays (the kernel clears its own state)."""
+    import ctypes as C
+    import torch
+    _say("small batch single launch")
+    rng = np.random.default_rng(2024)
+    shapes = [(2, 64), (3, 1), (8, 100), (13, 2048), (14, 700), (17, 130), (20, 128),
+              (21, 64), (9, 0), (12, 1999)]
+    rows = np.array([r for r, _ in shapes], np.int32)
+    cols = np.array([c for _, c in shapes], np.int32)
+    mats, offs, o = [], [], 0
+    for r, c in shapes:
+        W = (c + 63) // 64
+        m = rng.integers(0, 2 ** 63, size=(r, max(W, 1)), dtype=np.uint64)[:, :W]
+        if c % 64:
+            m[:, -1] &= np.uint64((1 << (c % 64)) - 1)
+        if r == 17:
+            m[5] = m[2] ^ m[9]          # a zero-weight coset
+        mats.append(np.ascontiguousarray(m))
+        offs.append(o)
+        o += r * W
+    words = np.concatenate([m.reshape(-1) for m in mats]).astype(np.uint64)
+    offs = np.array(offs, np.int64)
+    out_off = np.concatenate([[0], np.cumsum(rows)[:-1]]).astype(np.int64)
+    n = int(rows.sum())
+    dev = torch.device("cuda:0")
+    d_words = torch.from_numpy(words.view(np.int64).copy()).to(dev)
+    d_w = torch.full((n,), -1, dtype=torch.int32, device=dev)
+    d_i = torch.full((n,), -1, dtype=torch.int64, device=dev)
+    plan = C.c_void_p()
+    lib.call("bmm_hip_batch_plan_create", d_words.data_ptr(), offs, rows, cols, rows.size,
+             out_off, d_w.data_ptr(), d_i.data_ptr(), 0, 1, C.byref(plan))
+    try:
+        info = lib.BatchInfo()
+        lib.call("bmm_hip_batch_plan_info", plan, C.byref(info))
+        assert info.persistent_wgs == info.total_blocks <= 1024
+        results = []
+        for rep in range(4):
+            d_w.fill_(-1)
+            torch.cuda.synchronize()
+            lib.call("bmm_hip_batch_plan_run", plan, int(rep > 0), None)
+            torch.cuda.synchronize()
+            results.append((d_w.cpu().numpy().view(np.uint32).copy(),
+                            d_i.cpu().numpy().view(np.uint64).copy()))
+        lib.call("bmm_hip_batch_plan_set_shape", plan, 13, 0, 0)
+        lib.call("bmm_hip_batch_plan_run", plan, 0, None)
+        torch.cuda.synchronize()
+        staged = (d_w.cpu().numpy().view(np.uint32).copy(),
+                  d_i.cpu().numpy().view(np.uint64).copy())
+    finally:
+        lib.call("bmm_hip_batch_plan_destroy", plan)
+    for w, i in results[1:] + [staged]:
+        assert np.array_equal(w, results[0][0]) and np.array_equal(i, results[0][1])
+    w, i = results[0]
+    for k, (r, c) in enumerate(shapes):
+        ew, ei = ora.coset_min(mats[k].reshape(r, -1) if c else np.zeros((r, 1), np.uint64),
+                               c)
+        o = out_off[k]
+        assert np.array_equal(w[o:o + r], ew), (r, c)
+        assert np.array_equal(i[o:o + r], ei), (r, c)
+    assert w[out_off[5] + 5] == 0
