@@ -796,6 +796,70 @@ def ga_compare(budget_s: float, seed: int = 0x5EED0005, isd_trials: int = 0,
     return res
 
 
+def ga_seeds(budget_s: float, n_seeds: int, isd_trials: int, learned_overrides: dict,
+             marks=(40.0, 20.0, 1.01)) -> dict:
+    """Plain and learned GA over `n_seeds` GA seeds on the same planted
+    256 x 8192 instance: per variant the time to reach each multiple of the
+    planted weight in `marks` (None when not reached within the budget) and
+    the weight at equal generations; median and range over the seeds."""
+    from bmmpy_amd import _lib
+
+    S, A, _ = _lib.gen_planted(0x5EED0005, 256, 8192, 0.01)
+    planted = int(sum(bin(int(v)).count("1") for v in S.reshape(-1)))
+    runs = {"plain": [], "learned": []}
+    for k in range(n_seeds):
+        seed = 0x5EED0005 + 0x1000 * k
+        for name, learned in (("plain", 0), ("learned", 1)):
+            kw = dict(learned=learned, seed=seed, isd_trials=isd_trials)
+            if learned:
+                kw.update(learned_overrides)
+            p = _lib.ga_default_params(**kw)
+            ga = _lib.GpuGA(A, 8192, p)
+            hit = {m: None for m in marks}
+            elapsed, st, traj = 0.0, None, []
+            while elapsed < budget_s:
+                st = ga.run(20, target=int(planted * min(marks)), budget_s=0.0)
+                elapsed += st.elapsed_s
+                w = int(st.best_weight)
+                traj.append((int(st.generations), w))
+                for m in marks:
+                    if hit[m] is None and w <= planted * m:
+                        hit[m] = elapsed
+                if st.reached_target:
+                    break
+            ga.close()
+            runs[name].append({"seed": hex(seed), "time_to": hit, "best_weight": int(st.best_weight),
+                               "generations": int(st.generations), "trajectory": traj})
+            print(f"ga seeds {name} {k}: {hit} best {st.best_weight} gens {st.generations}",
+                  file=sys.stderr, flush=True)
+
+    def summary(name, m):
+        t = [r["time_to"][m] for r in runs[name]]
+        done = sorted(x for x in t if x is not None)
+        return {"reached": len(done), "of": len(t),
+                "median_s": float(np.median(done)) if len(done) == len(t) else None,
+                "min_s": done[0] if done else None, "max_s": done[-1] if done else None}
+
+    out = {"planted_weight": planted, "isd_trials": isd_trials, "budget_s": budget_s,
+           "learned_overrides": learned_overrides, "marks": list(marks)}
+    for m in marks:
+        a, b = summary("plain", m), summary("learned", m)
+        ratio = (b["median_s"] / a["median_s"]) if a["median_s"] and b["median_s"] else None
+        out[f"x{m}"] = {"plain": a, "learned": b, "median_ratio_learned_over_plain": ratio}
+    # weights at equal generations (the shorter run's last common point)
+    eq = []
+    for rp, rl in zip(runs["plain"], runs["learned"]):
+        gp, gl = dict(rp["trajectory"]), dict(rl["trajectory"])
+        common = sorted(set(gp) & set(gl))
+        if common:
+            g = common[-1]
+            eq.append({"generations": g, "plain": gp[g], "learned": gl[g]})
+    out["equal_generations"] = eq
+    out["runs"] = {k: [{kk: vv for kk, vv in r.items() if kk != "trajectory"} for r in v]
+                   for k, v in runs.items()}
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -813,6 +877,12 @@ def main() -> int:
     ap.add_argument("--ga-isd-trials", type=int, default=1,
                     help="information-set trials per island per migration epoch in "
                          "--workload ga (both variants; 0 = off)")
+    ap.add_argument("--ga-seeds", type=int, default=0,
+                    help="--workload ga: compare plain and learned over this many GA seeds "
+                         "(median and range of the times to 40x, 20x and 1.01x planted)")
+    ap.add_argument("--ga-win-candidates", type=int, default=0,
+                    help="--workload ga: window candidates the learned proposal ranks per "
+                         "target (default: the library's)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="after the run, copy a buffer of this many MiB once (a plain "
                          "device copy of known size, to scale HBM byte counters)")
@@ -841,6 +911,18 @@ def main() -> int:
 
         atexit.register(_copy_once)
 
+    if args.workload == "ga" and args.ga_seeds > 0:
+        ov = {"win_candidates": args.ga_win_candidates} if args.ga_win_candidates else {}
+        r = ga_seeds(args.ga_budget, args.ga_seeds, args.ga_isd_trials, ov)
+        print(json.dumps({"metric": METRIC, "value": r["x1.01"]["median_ratio_learned_over_plain"],
+                          "unit": "median time ratio learned / plain to 1.01 x planted",
+                          "n_gpus": 1, "steps": args.ga_seeds, "warmup": 0, "ms_per_step": None,
+                          "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+                          "dtype": "u32", "data": "synthetic",
+                          "config": {"workload": "ga_island_planted_256x8192_seeds",
+                                     "seeds": args.ga_seeds, "isd_trials": args.ga_isd_trials},
+                          "ga": r}))
+        return 0
     if args.workload == "ga":
         r = ga_compare(args.ga_budget, isd_trials=args.ga_isd_trials)
         plain, learned = r["plain"], r["learned"]

@@ -124,6 +124,7 @@ SIGNATURES = {
     "bmm_hip_ga_configure_islands": ([_vp, _vp, _vp, _sz], C.c_int),
     "bmm_hip_ga_best_ops": ([_vp, C.c_int, _u64p, _u16p, C.POINTER(_sz)], C.c_int),
     "bmm_hip_ga_set_island_base": ([_vp, C.c_int, _u64p], C.c_int),
+    "bmm_hip_ga_proposals": ([_vp, C.c_int, _u16p, _sz, C.POINTER(_sz)], C.c_int),
     "bmm_hip_ga_population": ([_vp, C.c_int, _vp, _vp, _vp, _vp, _vp, _vp,
                                C.POINTER(GaIslandState)], C.c_int),
     "bmm_hip_mlp_num_params": ([], C.c_size_t),
@@ -468,6 +469,13 @@ class GpuGA:
         call("bmm_hip_ga_best_ops", self.h, island, base, ops, C.byref(n))
         o = ops[:n.value]
         return base, np.stack([o & 0xff, o >> 8], axis=1).astype(np.uint16)
+
+    def proposals(self, island: int) -> np.ndarray:
+        """The island's ranked candidate pairs (src | dst << 8)."""
+        out = np.zeros(1024, np.uint16)
+        n = _sz()
+        call("bmm_hip_ga_proposals", self.h, island, out, out.size, C.byref(n))
+        return out[:n.value].copy()
 
     def set_island_base(self, island: int, matrix):
         call("bmm_hip_ga_set_island_base", self.h, island, _rows(matrix))

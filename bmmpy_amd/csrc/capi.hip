@@ -90,6 +90,7 @@ void ga_best_ops(Ga* G, int island, uint64_t* base, uint16_t* ops, size_t* n_ops
 void ga_population(Ga* G, int island, uint16_t* ops_next, int32_t* len_next, uint16_t* ops_prev,
                    int32_t* len_prev, uint32_t* fit_prev, uint64_t* rng, bmm_ga_island_state* st);
 void ga_set_island_base(Ga* G, int island, const uint64_t* matrix);
+void ga_proposals(Ga* G, int island, uint16_t* out, size_t cap, size_t* n);
 void ga_configure_islands(Ga* G, const uint64_t* seeds, const int32_t* migrate, size_t n);
 struct Mlp;
 Mlp* mlp_create(uint64_t seed);
@@ -721,6 +722,15 @@ int bmm_hip_ga_best_ops(void* ga, int island, uint64_t* out_base, uint16_t* out_
         if (!ga || !out_base || !out_ops || !n_ops)
             fail(BMM_ERR_INVALID, "ga_best_ops: null pointer");
         ga_best_ops(static_cast<Ga*>(ga), island, out_base, out_ops, n_ops);
+    });
+}
+
+int bmm_hip_ga_proposals(void* ga, int island, uint16_t* out_pairs, size_t capacity,
+                         size_t* n_pairs) {
+    return guarded([&] {
+        if (!ga || !n_pairs || (capacity && !out_pairs))
+            fail(BMM_ERR_INVALID, "ga_proposals: null pointer");
+        ga_proposals(static_cast<Ga*>(ga), island, out_pairs, capacity, n_pairs);
     });
 }
 

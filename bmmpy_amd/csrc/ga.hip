@@ -92,6 +92,11 @@ __global__ __launch_bounds__(64) void ga_eval_kernel(const uint32_t* __restrict_
 
 // ------------------------------------------------------------ breed ------
 constexpr int kMaxParents = 4;
+// Learned crossover: the start of a later parent's segment is the best of
+// kCutTries uniform draws by the number of ops among its next kCutSpan that
+// the proposal ranks as weight-reducing (the island's candidate pairs).
+constexpr int kCutTries = 4;
+constexpr int kCutSpan = 8;
 
 struct GaDev {
     int islands, pop, rows, max_ops, tournament, elite, n_cand, learned;
@@ -190,6 +195,7 @@ __global__ __launch_bounds__(1024) void ga_breed_kernel(
     extern __shared__ unsigned int sf[];
     uint16_t* order = reinterpret_cast<uint16_t*>(sf + P.pop);
     uint16_t* gsel = order + ((P.pop + 1) & ~1);
+    __shared__ uint32_t good[65536 / 32];   // learned: bitmap of the ranked pairs
     __shared__ int s_mode;
     __shared__ unsigned int s_noimp;
     __shared__ float s_rate;
@@ -271,6 +277,17 @@ __global__ __launch_bounds__(1024) void ga_breed_kernel(
             kind = 2;
         }
     }
+    const bool learned_cut = P.learned && P.n_cand > 0;
+    if (learned_cut) {
+        for (int k = t; k < 65536 / 32; k += blockDim.x)
+            good[k] = 0u;
+        __syncthreads();
+        for (int k = t; k < P.n_cand; k += blockDim.x) {
+            const uint32_t op = cand[(size_t)isl * P.n_cand + k];
+            atomicOr(&good[op >> 5], 1u << (op & 31u));
+        }
+        __syncthreads();
+    }
     // Group leaders (the first non-elite child of each group, whatever its
     // own kind) draw the parents and cut points of their group.
     const int c = t - P.elite;
@@ -286,7 +303,22 @@ __global__ __launch_bounds__(1024) void ga_breed_kernel(
             } else if (p == 0) {
                 e0 = (int)(xs_next(rng) % (uint64_t)(lp + 1));
             } else {
-                s0 = (int)(xs_next(rng) % (uint64_t)(lp + 1));
+                if (learned_cut) {
+                    const uint16_t* src = ops_cur + (base_id + par) * P.max_ops;
+                    int best_sc = -1;
+                    for (int j = 0; j < kCutTries; ++j) {
+                        const int cs = (int)(xs_next(rng) % (uint64_t)(lp + 1));
+                        int sc = 0;
+                        for (int k = cs; k < min(cs + kCutSpan, lp); ++k)
+                            sc += (int)((good[src[k] >> 5] >> (src[k] & 31u)) & 1u);
+                        if (sc > best_sc) {
+                            best_sc = sc;
+                            s0 = cs;
+                        }
+                    }
+                } else {
+                    s0 = (int)(xs_next(rng) % (uint64_t)(lp + 1));
+                }
                 if (p + 1 < P.num_parents)
                     e0 = s0 + (int)(xs_next(rng) % (uint64_t)(lp - s0 + 1));
             }
@@ -1792,6 +1824,20 @@ void ga_population(Ga* G, int island, uint16_t* ops_next, int32_t* len_next, uin
         st->catastrophe = s.catastrophe;
         st->mutation_rate = s.rate;
     }
+}
+
+// The island's ranked candidate pairs (learned proposal; empty otherwise).
+void ga_proposals(Ga* G, int island, uint16_t* out, size_t cap, size_t* n) {
+    if (island < 0 || island >= G->P.islands)
+        fail(BMM_ERR_RANGE, "ga_proposals: island out of range");
+    const size_t k = G->P.learned ? (size_t)G->P.n_candidates : 0;
+    if (k > cap)
+        fail(BMM_ERR_INVALID, "ga_proposals: output capacity below n_candidates");
+    BMM_HIP(hipStreamSynchronize(G->stream));
+    if (k)
+        BMM_HIP(hipMemcpy(out, static_cast<uint16_t*>(G->cand.ptr) + (size_t)island * k, k * 2,
+                          hipMemcpyDeviceToHost));
+    *n = k;
 }
 
 // Replaces island `island`'s base by `matrix` (rows x cols/64 words); the

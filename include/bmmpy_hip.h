@@ -373,6 +373,14 @@ int bmm_hip_ga_configure_islands(void *ga, const uint64_t *seeds, const int32_t 
 int bmm_hip_ga_best_ops(void *ga, int island, uint64_t *out_base, uint16_t *out_ops,
                         size_t *n_ops);
 
+/* The learned proposal's ranked candidate pairs of island `island`
+ * (src | dst << 8, *n_pairs = n_candidates; 0 without the learned proposal):
+ * mutations draw from them with probability p_learned, and a crossover's
+ * later parents start their segment at the best of 4 cut draws by how many
+ * of their next 8 ops are among them. */
+int bmm_hip_ga_proposals(void *ga, int island, uint16_t *out_pairs, size_t capacity,
+                         size_t *n_pairs);
+
 /* Replaces island `island`'s base matrix; its best becomes that base. */
 int bmm_hip_ga_set_island_base(void *ga, int island, const uint64_t *matrix);
 

@@ -23,6 +23,11 @@ fitness:
    i == j skipped (operators.cpp:100-129); otherwise `mutations` draws of
    append / replace / delete of one op.
 
+With the learned proposal, mutations take one of the island's ranked pairs
+with probability p_learned, and a later parent's segment starts at the best
+of 4 uniform cut draws by the number of its next 8 ops that are ranked pairs
+(first best on ties).
+
 A random individual is the identity followed by 3 * rows random row XORs
 (genetic_algorithm_population.cpp:20-36), at most max_ops of them.
 
@@ -102,7 +107,9 @@ def breed(p: dict, seqs: list[list[int]], fit, rng_states, state: dict, cand=Non
     restart_after, restart_p.  seqs: the evaluated population (op lists),
     fit: its fitness, rng_states: one stream state per child, state:
     best_weight, best_len, no_improvement, catastrophe, mutation_rate.
-    Returns (children, new stream states, new state, best index or None)."""
+    cand: the island's ranked pairs (learned proposal).  Returns (children,
+    new stream states, new state, best index or None, 1 for a catastrophe
+    step else 0)."""
     pop = p["pop"]
     order = sorted(range(pop), key=lambda i: (int(fit[i]), i))
     arg = order[0]
@@ -144,6 +151,7 @@ def breed(p: dict, seqs: list[list[int]], fit, rng_states, state: dict, cand=Non
         else:
             kinds[t] = 2
     # Group leaders draw parents and cut points.
+    good = set(int(x) for x in cand) if (p["learned"] and p["n_cand"] > 0) else None
     groups = {}
     for t in range(pop):
         c = t - p["elite"]
@@ -158,7 +166,15 @@ def breed(p: dict, seqs: list[list[int]], fit, rng_states, state: dict, cand=Non
                 elif q == 0:
                     e0 = rngs[t].index(lp + 1)
                 else:
-                    s0 = rngs[t].index(lp + 1)
+                    if good is not None:
+                        best_sc = -1
+                        for _ in range(4):
+                            cs = rngs[t].index(lp + 1)
+                            sc = sum(1 for k in range(cs, min(cs + 8, lp)) if seqs[par][k] in good)
+                            if sc > best_sc:
+                                best_sc, s0 = sc, cs
+                    else:
+                        s0 = rngs[t].index(lp + 1)
                     if q + 1 < p["num_parents"]:
                         e0 = s0 + rngs[t].index(lp - s0 + 1)
                 sel.append((par, s0, e0))

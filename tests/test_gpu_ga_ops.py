@@ -41,6 +41,9 @@ CONFIGS = {
     # catastrophe every 2 stale generations, one parent (copy + mutation)
     "catastrophe": dict(mutations=0, elite=1, mutation_rate=0.1, num_parents=1,
                         catastrophe_threshold=2, catastrophe_survival=0.25),
+    # learned proposal: ranked pairs in mutations and in the crossover cuts
+    "learned": dict(mutations=3, elite=2, learned=1, n_candidates=64, train_steps=2,
+                    p_learned=0.7, num_parents=3),
 }
 
 
@@ -55,6 +58,8 @@ def test_breeding_matches_restatement(lib, ora, name):
     S, A, M = lib.gen_planted(0x5EED0005, 32, COLS, 0.02)
     kw = dict(islands=2, population=48, max_ops=40, tournament=3, migrate_every=1000,
               check_every=1, learned=0, li_sweeps=0, win_rows=0, isd_trials=0)
+    if name == "learned":
+        S, _, _ = lib.gen_planted(0x5EED0006, 32, COLS, 0.05)   # room for improving moves
     kw.update(CONFIGS[name])
     p = lib.ga_default_params(**kw)
     ga = lib.GpuGA(S, COLS, p)
@@ -75,7 +80,9 @@ def test_breeding_matches_restatement(lib, ora, name):
                     assert int(b["fit_prev"][i]) == w, (gen, isl, i)
                 st0 = {k: a[k] for k in ("best_weight", "best_len", "no_improvement",
                                          "catastrophe", "mutation_rate")}
-                kids, rng, st, _, mode = breed(P, seqs, b["fit_prev"], a["rng"], st0)
+                cand = ga.proposals(isl)
+                assert cand.size == (64 if name == "learned" else 0)
+                kids, rng, st, _, mode = breed(P, seqs, b["fit_prev"], a["rng"], st0, cand)
                 modes.add(mode)
                 got = _seqs(b["ops_next"], b["len_next"])
                 assert got == kids, (name, gen, isl)

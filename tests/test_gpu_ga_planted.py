@@ -38,3 +38,28 @@ def test_island_ga_reaches_planted_at_workload_shape(lib, ora, learned):
     assert st.evaluations == st.generations * 64 * 256
     assert ora.rank(best, 8192) == 256
     assert ora.rank(np.concatenate([A, best]), 8192) == 256
+
+
+def test_plain_and_learned_at_equal_generations(lib, ora):
+    """Both variants at the workload shape for the same 60 generations (three
+    epochs, no early stop): the exact weights they reach, each equal to the
+    popcount of its rows and row-space preserving.  The learned proposal is
+    not required to be lighter; the weights are printed for the record."""
+    S, A, M = lib.gen_planted(SEED_PLANTED_256X8192, 256, 8192, 0.01)
+    planted = _weight(S)
+    got = {}
+    for learned in (0, 1):
+        ga = lib.GpuGA(A, 8192, lib.ga_default_params(learned=learned))
+        try:
+            st = ga.run(60)
+            best, w = ga.best()
+        finally:
+            ga.close()
+        assert st.generations == 60 and w == st.best_weight == _weight(best)
+        assert ora.rank(np.concatenate([A, best]), 8192) == 256
+        got["learned" if learned else "plain"] = (w, st.elapsed_s)
+    print(f"60 generations, planted {planted}: plain {got['plain'][0]} "
+          f"({got['plain'][0] / planted:.4f} x, {got['plain'][1]:.3f} s), learned "
+          f"{got['learned'][0]} ({got['learned'][0] / planted:.4f} x, {got['learned'][1]:.3f} s)",
+          flush=True)
+    assert max(got["plain"][0], got["learned"][0]) < _weight(A) // 4
