@@ -9,8 +9,9 @@
 // split m = mh << L | ml; each persistent workgroup takes blocks mh from a
 // device counter, builds g_mh in LDS (sign (-1)^popcount((p >> L) & mh) per
 // pattern), transforms 2^L entries on chip and offers the packed keys
-// (weight << t | mask) to its exact top-K (topk.hpp).  A final one-workgroup
-// merge folds the per-workgroup lists.  The split point does not change the
+// (weight << t | mask) to its exact top-K (topk.hpp).  A tree of parallel
+// merges (16 lists per workgroup and level) folds the per-workgroup lists.
+// The split point does not change the
 // result: the top-K under the strict (weight, mask) order is unique.
 #include "common.hpp"
 #include "fwht_block.hpp"
@@ -110,8 +111,25 @@ __global__ void pattern_from_split_kernel(const uint64_t* __restrict__ q,
 
 namespace {
 struct SearchWorkspace {
-    DevBuf rows, pat, mult, q, r, counter, lists, out;
+    DevBuf rows, pat, mult, q, r, counter, lists, lists2, out;
 };
+
+// Folds n_lists lists of K keys into one by a tree of parallel merges
+// (fan-in kMergeFan); returns the buffer holding the final list.
+constexpr int kMergeFan = 16;
+
+template <int CAP>
+uint64_t* merge_lists(uint64_t* a, uint64_t* b, int n_lists, int K, hipStream_t s) {
+    while (n_lists > 1) {
+        const int groups = (n_lists + kMergeFan - 1) / kMergeFan;
+        hipLaunchKernelGGL(topk_merge_group_kernel<CAP>, dim3(groups), dim3(256), 0, s, a,
+                           n_lists, kMergeFan, K, b);
+        BMM_HIP(hipGetLastError());
+        std::swap(a, b);
+        n_lists = groups;
+    }
+    return a;
+}
 SearchWorkspace& search_ws() {
     thread_local SearchWorkspace ws;
     return ws;
@@ -171,20 +189,18 @@ std::vector<bmm_topk_entry> fwht_topk_engine(const uint64_t* d_pat, const int32_
     SearchWorkspace& ws = search_ws();
     unsigned long long* d_counter = ws.counter.get<unsigned long long>(1);
     uint64_t* d_lists = ws.lists.get<uint64_t>((size_t)grid * K);
-    uint64_t* d_out = ws.out.get<uint64_t>((size_t)K);
+    uint64_t* d_lists2 = ws.lists2.get<uint64_t>((size_t)((grid + kMergeFan - 1) / kMergeFan) * K);
     fill_u64(reinterpret_cast<uint64_t*>(d_counter), 1, 0, stream);
+    uint64_t* d_out;
     if (K <= 512) {
         dispatch_fwht_topk<2048>(L, d_pat, d_mult, n_pat, t, (int32_t)total, n_blocks, K,
                                  d_counter, d_lists, grid, stream);
-        hipLaunchKernelGGL(topk_merge_kernel<2048>, dim3(1), dim3(256), 0, stream, d_lists,
-                           (size_t)grid * K, K, d_out);
+        d_out = merge_lists<2048>(d_lists, d_lists2, grid, K, stream);
     } else {
         dispatch_fwht_topk<8192>(L, d_pat, d_mult, n_pat, t, (int32_t)total, n_blocks, K,
                                  d_counter, d_lists, grid, stream);
-        hipLaunchKernelGGL(topk_merge_kernel<8192>, dim3(1), dim3(256), 0, stream, d_lists,
-                           (size_t)grid * K, K, d_out);
+        d_out = merge_lists<8192>(d_lists, d_lists2, grid, K, stream);
     }
-    BMM_HIP(hipGetLastError());
     std::vector<uint64_t> keys((size_t)K);
     BMM_HIP(hipMemcpyAsync(keys.data(), d_out, keys.size() * 8, hipMemcpyDeviceToHost, stream));
     BMM_HIP(hipStreamSynchronize(stream));

@@ -125,15 +125,19 @@ __device__ inline void topk_store(TopkShared<CAP>& S, uint64_t* out, int K) {
         out[i] = i < (int)S.n_list ? S.buf[i] : ~0ull;
 }
 
-// One workgroup merges n sorted-or-not keys from global memory into the
-// exact K smallest; writes them sorted to out[0, K) (padded with ~0).
+// Tree level of the merge: workgroup g folds lists [g F, min((g + 1) F,
+// n_lists)) of K keys each into the exact K smallest, out[g K, (g + 1) K).
 template <int CAP>
-__global__ __launch_bounds__(256) void topk_merge_kernel(const uint64_t* __restrict__ keys,
-                                                         size_t n, int K,
-                                                         uint64_t* __restrict__ out) {
+__global__ __launch_bounds__(256) void topk_merge_group_kernel(const uint64_t* __restrict__ lists,
+                                                               int n_lists, int F, int K,
+                                                               uint64_t* __restrict__ out) {
     __shared__ TopkShared<CAP> S;
     topk_init(S);
     constexpr int E = 8;
+    const int g = blockIdx.x;
+    const int first = g * F;
+    const size_t n = (size_t)min(F, n_lists - first) * K;
+    const uint64_t* keys = lists + (size_t)first * K;
     for (size_t base = 0; base < n; base += (size_t)E * blockDim.x) {
         uint64_t k[E];
         uint32_t valid = 0;
@@ -146,7 +150,7 @@ __global__ __launch_bounds__(256) void topk_merge_kernel(const uint64_t* __restr
         }
         topk_offer<CAP, E>(S, k, valid, K);
     }
-    topk_store(S, out, K);
+    topk_store(S, out + (size_t)g * K, K);
 }
 
 } // namespace bmmhip
