@@ -15,6 +15,7 @@
 // result: the top-K under the strict (weight, mask) order is unique.
 #include "common.hpp"
 #include "fwht_block.hpp"
+#include "fwht_pk16.hpp"
 #include "topk.hpp"
 
 #include <algorithm>
@@ -82,6 +83,135 @@ __global__ __launch_bounds__(256) void fwht_topk_kernel(const uint64_t* __restri
             const uint64_t w = (uint64_t)(uint32_t)(total - sp[e * 264]) >> 1;
             return (w << t) | (mh << L) | (uint64_t)(e * 256 + (int)threadIdx.x);
         });
+    }
+    topk_store(T, lists + (size_t)blockIdx.x * K, K);
+}
+
+// Packed pass over a block of 2^14 masks (windows of >= 14 rows and at most
+// 32767 active columns): g_mh built into biased int16 halves (a low-half
+// entry adds +-mult to the word, a high-half entry +-mult << 16; the bias
+// keeps every half in range, so no carry crosses halves) and transformed with
+// the packed radix rounds.  Each thread keeps, from its 64 final values, the
+// keys of the entries with H >= hmin (weight at most the list threshold's) in
+// up to kPkCand registers; the block falls back to the exact int32 pass when
+// any thread has more (warm-up, or a block crowded with light masks).
+constexpr int kPkCand = 4;
+
+__device__ inline bool pk_topk_block(uint32_t* s, const uint64_t* __restrict__ pat,
+                                     const int32_t* __restrict__ mult, int n, uint64_t mh,
+                                     int t, int32_t total, int32_t hmin,
+                                     uint64_t (&cand)[kPkCand], int& nc) {
+    int4* s4 = reinterpret_cast<int4*>(s);
+    const int b = (int)kPkBias;
+    for (int i = threadIdx.x; i < (pk_lds_words() + 3) / 4; i += blockDim.x)
+        s4[i] = make_int4(b, b, b, b);
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        const uint64_t p = pat[i];
+        const uint32_t pl = (uint32_t)p & ((1u << kPkL) - 1u);
+        const uint32_t m = (uint32_t)mult[i];
+        const uint32_t add = (pl >> kPkW) ? (m << 16) : m;
+        const int neg = __builtin_popcountll((p >> kPkL) & mh) & 1;
+        atomicAdd(&s[pk_pad((int)(pl & ((1u << kPkW) - 1u)))], neg ? 0u - add : add);
+    }
+    nc = 0;
+    bool over = false;
+    const uint64_t base = mh << kPkL;
+    auto take = [&](int32_t h, uint32_t ml) {
+        if (h >= hmin && (mh != 0 || ml != 0)) {
+            if (nc < kPkCand) {
+                const uint64_t key = ((uint64_t)(uint32_t)(total - h) >> 1 << t) | base | ml;
+#pragma unroll
+                for (int k = 0; k < kPkCand; ++k)   // static register slots
+                    if (k == nc)
+                        cand[k] = key;
+                ++nc;
+            } else {
+                over = true;
+            }
+        }
+    };
+    pk_block_fwht_visit_paired(s, [&](int q, const uint32_t (&v)[8]) {
+        const uint32_t w0 = 2u * threadIdx.x + (uint32_t)(q & 1) + 512u * (uint32_t)(q >> 1);
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const s16x2 x = pk_as(v[e]);
+            if (max(pk_lo(x), pk_hi(x)) >= hmin) {
+                const uint32_t w = w0 + 1024u * e;
+                take(pk_lo(x), w);
+                take(pk_hi(x), w + (1u << kPkW));
+            }
+        }
+    });
+    __shared__ int any_over;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        any_over = 0;
+    __syncthreads();
+    if (over)
+        any_over = 1;
+    __syncthreads();
+    return any_over != 0;
+}
+
+// Blocks of 2^14 masks: the packed pass offers its few candidates directly;
+// a block with more candidates than the registers hold is built again as two
+// exact int32 blocks of 2^13 and offered in full.
+template <int CAP>
+__global__ __launch_bounds__(256) void fwht_topk_pk_kernel(const uint64_t* __restrict__ pat,
+                                                           const int32_t* __restrict__ mult,
+                                                           int n_pat, int t, int32_t total,
+                                                           uint64_t n_blocks, int K,
+                                                           unsigned long long* __restrict__ counter,
+                                                           uint64_t* __restrict__ lists) {
+    constexpr int L = 13;
+    constexpr int kWords = pk_lds_words() > fwht_lds_words(L) ? pk_lds_words() : fwht_lds_words(L);
+    __shared__ __align__(16) int32_t s[kWords + 4];
+    __shared__ TopkShared<CAP> T;
+    __shared__ unsigned long long blk_id;
+    constexpr int E = (1 << L) / 256;
+    topk_init(T);
+    for (;;) {
+        if (threadIdx.x == 0)
+            blk_id = atomicAdd(counter, 1ull);
+        __syncthreads();
+        const uint64_t mh14 = blk_id;
+        if (mh14 >= n_blocks)
+            break;
+        const uint64_t thr = T.thr;
+        const int64_t wthr = thr == ~0ull ? INT64_MAX : (int64_t)(thr >> t);
+        const int32_t hmin = wthr >= (int64_t)total ? INT32_MIN
+                                                    : (int32_t)((int64_t)total - 2 * wthr);
+        uint64_t cand[kPkCand];
+        int nc = 0;
+        const bool over = pk_topk_block(reinterpret_cast<uint32_t*>(s), pat, mult, n_pat, mh14, t,
+                                        total, hmin, cand, nc);
+        if (!over) {
+            topk_offer_fn<CAP, kPkCand>(T, (1u << nc) - 1u, K, [&](int e) { return cand[e]; });
+            continue;
+        }
+        for (int j = 0; j < 2; ++j) {
+            const uint64_t mh = 2 * mh14 + j;
+            __syncthreads();
+            block_build_g64<L>(s, pat, mult, n_pat, mh);
+            block_fwht<L>(s);
+            const uint64_t thr2 = T.thr;
+            const int64_t wthr2 = thr2 == ~0ull ? INT64_MAX : (int64_t)(thr2 >> t);
+            const int32_t hmin2 = wthr2 >= (int64_t)total ? INT32_MIN
+                                                         : (int32_t)((int64_t)total - 2 * wthr2);
+            const int32_t* sp = s + fwht_pad((int)threadIdx.x);
+            uint32_t valid = 0;
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                const int ml = e * 256 + (int)threadIdx.x;
+                if ((mh != 0 || ml != 0) && sp[e * 264] >= hmin2)
+                    valid |= 1u << e;
+            }
+            topk_offer_fn<CAP, E>(T, valid, K, [&](int e) {
+                const uint64_t w = (uint64_t)(uint32_t)(total - sp[e * 264]) >> 1;
+                return (w << t) | (mh << L) | (uint64_t)(e * 256 + (int)threadIdx.x);
+            });
+        }
     }
     topk_store(T, lists + (size_t)blockIdx.x * K, K);
 }
@@ -192,7 +322,27 @@ std::vector<bmm_topk_entry> fwht_topk_engine(const uint64_t* d_pat, const int32_
     uint64_t* d_lists2 = ws.lists2.get<uint64_t>((size_t)((grid + kMergeFan - 1) / kMergeFan) * K);
     fill_u64(reinterpret_cast<uint64_t*>(d_counter), 1, 0, stream);
     uint64_t* d_out;
-    if (K <= 512) {
+    if (t >= kPkL && total <= kPkMaxTotal && (n_masks >> kPkL) >= 256) {
+        // Packed blocks of 2^14 masks offering their few candidates, exact
+        // int32 rescan of the crowded ones.  (Fewer than 256 such blocks
+        // keep the int32 blocks of 2^13: twice the workgroups.)
+        const uint64_t n14 = n_masks >> kPkL;
+        const int grid14 = (int)std::max<uint64_t>(1, std::min<uint64_t>(n14, (uint64_t)grid));
+        {
+            ScopedKernelTimer timer(stream);
+            if (K <= 512)
+                hipLaunchKernelGGL(fwht_topk_pk_kernel<2048>, dim3(grid14), dim3(256), 0, stream,
+                                   d_pat, d_mult, n_pat, t, (int32_t)total, n14, K, d_counter,
+                                   d_lists);
+            else
+                hipLaunchKernelGGL(fwht_topk_pk_kernel<8192>, dim3(grid14), dim3(256), 0, stream,
+                                   d_pat, d_mult, n_pat, t, (int32_t)total, n14, K, d_counter,
+                                   d_lists);
+            BMM_HIP(hipGetLastError());
+        }
+        d_out = K <= 512 ? merge_lists<2048>(d_lists, d_lists2, grid14, K, stream)
+                         : merge_lists<8192>(d_lists, d_lists2, grid14, K, stream);
+    } else if (K <= 512) {
         dispatch_fwht_topk<2048>(L, d_pat, d_mult, n_pat, t, (int32_t)total, n_blocks, K,
                                  d_counter, d_lists, grid, stream);
         d_out = merge_lists<2048>(d_lists, d_lists2, grid, K, stream);
