@@ -97,15 +97,34 @@ __global__ __launch_bounds__(256) void fwht_topk_kernel(const uint64_t* __restri
 // any thread has more (warm-up, or a block crowded with light masks).
 constexpr int kPkCand = 4;
 
-__device__ inline bool pk_topk_block(uint32_t* s, const uint64_t* __restrict__ pat,
-                                     const int32_t* __restrict__ mult, int n, uint64_t mh,
-                                     int t, int32_t total, int32_t hmin,
-                                     uint64_t (&cand)[kPkCand], int& nc) {
-    int4* s4 = reinterpret_cast<int4*>(s);
-    const int b = (int)kPkBias;
-    for (int i = threadIdx.x; i < (pk_lds_words() + 3) / 4; i += blockDim.x)
-        s4[i] = make_int4(b, b, b, b);
-    __syncthreads();
+// A column pattern prepared for register-held builds: hi = pattern >> 14
+// (windows of at most 46 rows), av = mult << 16 | LDS byte address of its
+// word | 1 for the high half (addresses are multiples of 4); av = 0 pads.
+__device__ inline void pk_prep_entry(uint64_t p, int32_t m, uint32_t& hi, uint32_t& av) {
+    const uint32_t pl = (uint32_t)p & ((1u << kPkL) - 1u);
+    hi = (uint32_t)(p >> kPkL);
+    av = ((uint32_t)m << 16) | ((uint32_t)pk_pad((int)(pl & ((1u << kPkW) - 1u))) * 4u) |
+         (pl >> kPkW);
+}
+
+template <int R>
+__device__ inline void pk_build_regs(uint32_t* s, const uint32_t (&hi)[R], const uint32_t (&av)[R],
+                                     uint32_t mh) {
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        const uint32_t a = av[k];
+        if (a != 0u) {
+            const uint32_t m = a >> 16;
+            const uint32_t add = (a & 1u) ? (m << 16) : m;
+            const int neg = __builtin_popcount(hi[k] & mh) & 1;
+            atomicAdd(reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(s) + (a & 0xfffcu)),
+                      neg ? 0u - add : add);
+        }
+    }
+}
+
+__device__ inline void pk_build_mem64(uint32_t* s, const uint64_t* __restrict__ pat,
+                                      const int32_t* __restrict__ mult, int n, uint64_t mh) {
     for (int i = threadIdx.x; i < n; i += blockDim.x) {
         const uint64_t p = pat[i];
         const uint32_t pl = (uint32_t)p & ((1u << kPkL) - 1u);
@@ -114,6 +133,17 @@ __device__ inline bool pk_topk_block(uint32_t* s, const uint64_t* __restrict__ p
         const int neg = __builtin_popcountll((p >> kPkL) & mh) & 1;
         atomicAdd(&s[pk_pad((int)(pl & ((1u << kPkW) - 1u)))], neg ? 0u - add : add);
     }
+}
+
+template <typename Build>
+__device__ inline bool pk_topk_block(uint32_t* s, Build&& build, uint64_t mh, int t, int32_t total,
+                                     int32_t hmin, uint64_t (&cand)[kPkCand], int& nc) {
+    int4* s4 = reinterpret_cast<int4*>(s);
+    const int b = (int)kPkBias;
+    for (int i = threadIdx.x; i < (pk_lds_words() + 3) / 4; i += blockDim.x)
+        s4[i] = make_int4(b, b, b, b);
+    __syncthreads();
+    build();
     nc = 0;
     bool over = false;
     const uint64_t base = mh << kPkL;
@@ -143,25 +173,24 @@ __device__ inline bool pk_topk_block(uint32_t* s, const uint64_t* __restrict__ p
             }
         }
     });
-    __shared__ int any_over;
-    __syncthreads();
-    if (threadIdx.x == 0)
-        any_over = 0;
-    __syncthreads();
-    if (over)
-        any_over = 1;
-    __syncthreads();
-    return any_over != 0;
+    return __syncthreads_or(over) != 0;
 }
 
-// Blocks of 2^14 masks: the packed pass offers its few candidates directly;
-// a block with more candidates than the registers hold is built again as two
-// exact int32 blocks of 2^13 and offered in full.
-template <int CAP>
-__global__ __launch_bounds__(256) void fwht_topk_pk_kernel(const uint64_t* __restrict__ pat,
+// Blocks of 2^14 masks, taken `chunk` at a time from the work counter (up to
+// kPkChunk, fewer when each workgroup gets only a few blocks): the
+// packed pass offers its few candidates directly (nothing when no thread has
+// one); a block with more candidates than the registers hold is built again
+// as two exact int32 blocks of 2^13 and offered in full.
+constexpr int kPkChunk = 16;
+
+// R > 0: each thread holds patterns tid + 256 k (k < R) in registers across
+// its blocks (windows of at most 46 rows and 256 R patterns); R = 0 reads
+// them from memory for every block.
+template <int CAP, int R>
+__global__ __launch_bounds__(256, 3) void fwht_topk_pk_kernel(const uint64_t* __restrict__ pat,
                                                            const int32_t* __restrict__ mult,
                                                            int n_pat, int t, int32_t total,
-                                                           uint64_t n_blocks, int K,
+                                                           uint64_t n_blocks, int K, int chunk,
                                                            unsigned long long* __restrict__ counter,
                                                            uint64_t* __restrict__ lists) {
     constexpr int L = 13;
@@ -170,24 +199,51 @@ __global__ __launch_bounds__(256) void fwht_topk_pk_kernel(const uint64_t* __res
     __shared__ TopkShared<CAP> T;
     __shared__ unsigned long long blk_id;
     constexpr int E = (1 << L) / 256;
+    constexpr int RR = R > 0 ? R : 1;
+    uint32_t hi[RR], av[RR];
+#pragma unroll
+    for (int k = 0; k < RR; ++k) {
+        const int i = (int)threadIdx.x + 256 * k;
+        hi[k] = 0u;
+        av[k] = 0u;
+        if (R > 0 && i < n_pat)
+            pk_prep_entry(pat[i], mult[i], hi[k], av[k]);
+    }
     topk_init(T);
+    uint64_t next = 0, stop = 0;
     for (;;) {
-        if (threadIdx.x == 0)
-            blk_id = atomicAdd(counter, 1ull);
-        __syncthreads();
-        const uint64_t mh14 = blk_id;
-        if (mh14 >= n_blocks)
-            break;
+        if (next == stop) {
+            if (threadIdx.x == 0)
+                blk_id = atomicAdd(counter, (unsigned long long)chunk);
+            __syncthreads();
+            next = blk_id;
+            __syncthreads();
+            if (next >= n_blocks)
+                break;
+            stop = min(next + (uint64_t)chunk, n_blocks);
+        }
+        const uint64_t mh14 = next++;
+        __syncthreads();   // the previous block's LDS reads are done
         const uint64_t thr = T.thr;
         const int64_t wthr = thr == ~0ull ? INT64_MAX : (int64_t)(thr >> t);
         const int32_t hmin = wthr >= (int64_t)total ? INT32_MIN
                                                     : (int32_t)((int64_t)total - 2 * wthr);
         uint64_t cand[kPkCand];
         int nc = 0;
-        const bool over = pk_topk_block(reinterpret_cast<uint32_t*>(s), pat, mult, n_pat, mh14, t,
-                                        total, hmin, cand, nc);
+        uint32_t* su = reinterpret_cast<uint32_t*>(s);
+        bool over;
+        if constexpr (R > 0)
+            over = pk_topk_block(
+                su, [&] { pk_build_regs<RR>(su, hi, av, (uint32_t)mh14); }, mh14, t, total, hmin,
+                cand, nc);
+        else
+            over = pk_topk_block(
+                su, [&] { pk_build_mem64(su, pat, mult, n_pat, mh14); }, mh14, t, total, hmin,
+                cand, nc);
         if (!over) {
-            topk_offer_fn<CAP, kPkCand>(T, (1u << nc) - 1u, K, [&](int e) { return cand[e]; });
+            if (__syncthreads_or(nc > 0))
+                topk_offer_fn<CAP, kPkCand>(T, (1u << nc) - 1u, K,
+                                            [&](int e) { return cand[e]; });
             continue;
         }
         for (int j = 0; j < 2; ++j) {
@@ -330,14 +386,17 @@ std::vector<bmm_topk_entry> fwht_topk_engine(const uint64_t* d_pat, const int32_
         const int grid14 = (int)std::max<uint64_t>(1, std::min<uint64_t>(n14, (uint64_t)grid));
         {
             ScopedKernelTimer timer(stream);
-            if (K <= 512)
-                hipLaunchKernelGGL(fwht_topk_pk_kernel<2048>, dim3(grid14), dim3(256), 0, stream,
-                                   d_pat, d_mult, n_pat, t, (int32_t)total, n14, K, d_counter,
-                                   d_lists);
-            else
-                hipLaunchKernelGGL(fwht_topk_pk_kernel<8192>, dim3(grid14), dim3(256), 0, stream,
-                                   d_pat, d_mult, n_pat, t, (int32_t)total, n14, K, d_counter,
-                                   d_lists);
+            const bool regs = t <= 46;
+            auto kern = K <= 512 ? (regs && n_pat <= 256 * 4    ? fwht_topk_pk_kernel<2048, 4>
+                                    : regs && n_pat <= 256 * 16 ? fwht_topk_pk_kernel<2048, 16>
+                                                                : fwht_topk_pk_kernel<2048, 0>)
+                                 : (regs && n_pat <= 256 * 4    ? fwht_topk_pk_kernel<8192, 4>
+                                    : regs && n_pat <= 256 * 16 ? fwht_topk_pk_kernel<8192, 16>
+                                                                : fwht_topk_pk_kernel<8192, 0>);
+            const int chunk = (int)std::min<uint64_t>(
+                kPkChunk, std::max<uint64_t>(1, n14 / ((uint64_t)grid14 * 16)));
+            hipLaunchKernelGGL(kern, dim3(grid14), dim3(256), 0, stream, d_pat, d_mult, n_pat, t,
+                               (int32_t)total, n14, K, chunk, d_counter, d_lists);
             BMM_HIP(hipGetLastError());
         }
         d_out = K <= 512 ? merge_lists<2048>(d_lists, d_lists2, grid14, K, stream)
