@@ -714,7 +714,8 @@ def secondary(torch, _lib, dev, stream) -> dict:
         "planted_weight": g["planted_weight"], "start_weight": g["start_weight"],
         "target": g["target"],
         **{f"{k}_{f}": g[k][f] for k in ("plain", "learned")
-           for f in ("best_weight", "time_to_target_s", "generations", "ms_per_generation")},
+           for f in ("best_weight", "time_to_target_s", "generations", "ms_per_generation",
+                     "evals_per_s", "replayed_ops_per_s")},
         "learned_over_plain_time_ratio":
             (g["learned"]["time_to_target_s"] / g["plain"]["time_to_target_s"]
              if g["plain"]["time_to_target_s"] and g["learned"]["time_to_target_s"] else None)}
@@ -789,6 +790,8 @@ def ga_compare(budget_s: float, seed: int = 0x5EED0005, isd_trials: int = 0,
                      "setup_s": setup, "generations": int(st.generations),
                      "evaluations": int(st.evaluations),
                      "evals_per_s": st.evaluations / max(elapsed, 1e-9),
+                     "replayed_ops": int(st.replayed_ops),
+                     "replayed_ops_per_s": st.replayed_ops / max(elapsed, 1e-9),
                      "ms_per_generation": 1e3 * elapsed / max(int(st.generations), 1),
                      "train_loss": st.train_loss, "li_moves": int(st.li_moves),
                      "window_moves": int(st.win_moves), "isd_rows": int(st.isd_rows),

@@ -77,7 +77,8 @@ class GaStatus(C.Structure):
                 ("best_weight", C.c_uint64), ("best_island", C.c_int),
                 ("reached_target", C.c_int), ("elapsed_s", C.c_double),
                 ("train_loss", C.c_double), ("li_moves", C.c_uint64),
-                ("win_moves", C.c_uint64), ("isd_rows", C.c_uint64)]
+                ("win_moves", C.c_uint64), ("isd_rows", C.c_uint64),
+                ("replayed_ops", C.c_uint64)]
 
 
 # name -> (argtypes, restype)

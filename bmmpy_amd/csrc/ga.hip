@@ -57,37 +57,142 @@ __device__ inline uint64_t xs_next(uint64_t& s) {
 }
 
 // ------------------------------------------------------------- eval ------
-// rows <= 256; u32 words per row = W32 (multiple of 64).
+// rows <= 256; u32 words per row = W32 (multiple of 64).  One wavefront per
+// (island, individual, 2048-column slice).  Exact fitness from the rows the
+// op list touches: fit (initialised to the base weight) += sum over those rows
+// of popcount(after) - popcount(before), so untouched rows cost nothing.  The
+// touched rows of the slice are staged in LDS slots (row -> slot table), the
+// ops applied lane-parallel over the slice's 64 words.
 __global__ __launch_bounds__(64) void ga_eval_kernel(const uint32_t* __restrict__ base,
                                                      const uint16_t* __restrict__ ops,
                                                      const int32_t* __restrict__ len, int pop,
                                                      int rows, int W32, int max_ops,
-                                                     unsigned int* __restrict__ fit) {
-    extern __shared__ uint32_t cols[];   // rows x 64
+                                                     unsigned int* __restrict__ fit,
+                                                     unsigned long long* __restrict__ n_ops) {
+    // Dynamic LDS: rows x 64 words for the touched rows, then the ops padded
+    // to a multiple of 4 (first as loaded, then as src slot | dst slot << 16).
+    extern __shared__ __align__(16) uint32_t cols[];
+    uint32_t* opsl = cols + rows * 64;
+    __shared__ uint16_t slot_of[256];    // 0: untouched, else slot + 1
+    __shared__ uint16_t row_of[256];
     const int slices = W32 / 64;
     const int task = blockIdx.x;
     const int slice = task % slices;
     const int ind = (task / slices) % pop;
     const int isl = task / (slices * pop);
     const int lane = threadIdx.x;
-    const uint32_t* b = base + (size_t)isl * rows * W32 + (size_t)slice * 64 + lane;
-    for (int r = 0; r < rows; ++r)
-        cols[r * 64 + lane] = b[(size_t)r * W32];
     const size_t id = (size_t)isl * pop + ind;
     const uint16_t* o = ops + id * max_ops;
     const int n = len[id];
-    for (int k = 0; k < n; ++k) {
+    for (int r = lane; r < 256; r += kWave)
+        slot_of[r] = 0;
+    __syncthreads();
+    for (int k = lane; k < n; k += kWave) {
         const uint32_t op = o[k];
         const int src = op & 0xff, dst = op >> 8;
-        if (src != dst && src < rows && dst < rows)
-            cols[dst * 64 + lane] ^= cols[src * 64 + lane];
+        opsl[k] = op;
+        if (src != dst && src < rows && dst < rows) {
+            slot_of[src] = 1;
+            slot_of[dst] = 1;
+        }
     }
-    uint32_t pc = 0;
-    for (int r = 0; r < rows; ++r)
-        pc += __builtin_popcount(cols[r * 64 + lane]);
-    pc = wave_sum_u32(pc);
-    if (lane == 0)
-        atomicAdd(&fit[id], pc);
+    for (int k = n + lane; k < ((n + 3) & ~3); k += kWave)
+        opsl[k] = 0u;   // pad to a multiple of 4 with no-ops (src == dst)
+    __syncthreads();
+    // Slots in row order: lane l owns rows 4l .. 4l + 3.
+    uint32_t f[4], cnt = 0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        f[q] = slot_of[4 * lane + q];
+        cnt += f[q];
+    }
+    uint32_t incl = cnt;   // inclusive scan over the wavefront
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+        const uint32_t v = __shfl_up(incl, off, kWave);
+        if (lane >= off)
+            incl += v;
+    }
+    const int n_touched = (int)__shfl(incl, kWave - 1, kWave);
+    uint32_t slot = incl - cnt;
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int r = 4 * lane + q;
+        if (f[q]) {
+            slot_of[r] = (uint16_t)(slot + 1);
+            row_of[slot] = (uint16_t)r;
+            ++slot;
+        }
+    }
+    __syncthreads();
+    // Ops as slot pairs (no-ops as 0xffffffff).
+    for (int k = lane; k < n; k += kWave) {
+        const uint32_t op = opsl[k];
+        const int src = op & 0xff, dst = op >> 8;
+        opsl[k] = (src != dst && src < rows && dst < rows)
+                      ? (uint32_t)(slot_of[src] - 1) | ((uint32_t)(slot_of[dst] - 1) << 16)
+                      : 0xffffffffu;
+    }
+    for (int k = n + lane; k < ((n + 3) & ~3); k += kWave)
+        opsl[k] = 0xffffffffu;
+    const uint32_t* b = base + (size_t)isl * rows * W32 + (size_t)slice * 64 + lane;
+    uint32_t before = 0;
+    int j = 0;
+    for (; j + 16 <= n_touched; j += 16) {   // sixteen row loads in flight
+        uint32_t v[16];
+#pragma unroll
+        for (int q = 0; q < 16; ++q)
+            v[q] = b[(size_t)row_of[j + q] * W32];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            cols[(j + q) * 64 + lane] = v[q];
+            before += __builtin_popcount(v[q]);
+        }
+    }
+    for (; j + 4 <= n_touched; j += 4) {
+        uint32_t v[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            v[q] = b[(size_t)row_of[j + q] * W32];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            cols[(j + q) * 64 + lane] = v[q];
+            before += __builtin_popcount(v[q]);
+        }
+    }
+    for (; j < n_touched; ++j) {
+        const uint32_t v = b[(size_t)row_of[j] * W32];
+        cols[j * 64 + lane] = v;
+        before += __builtin_popcount(v);
+    }
+    __syncthreads();
+    const uint4* op4 = reinterpret_cast<const uint4*>(opsl);
+    for (int k = 0; k < (n + 3) >> 2; ++k) {
+        const uint4 q = op4[k];   // four ops per LDS read
+        const uint32_t qs[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+            if (qs[e] != 0xffffffffu)
+                cols[(qs[e] >> 16) * 64 + lane] ^= cols[(qs[e] & 0xffffu) * 64 + lane];
+    }
+    uint32_t after = 0;
+    for (int t = 0; t < n_touched; ++t)
+        after += __builtin_popcount(cols[t * 64 + lane]);
+    const int32_t delta = (int32_t)wave_sum_u32(after) - (int32_t)wave_sum_u32(before);
+    if (lane == 0) {
+        atomicAdd(&fit[id], (unsigned int)delta);   // wraps back to the exact total
+        if (slice == 0)
+            atomicAdd(n_ops, (unsigned long long)n);   // ops replayed (one count per individual)
+    }
+}
+
+// fit[island, individual] = the island base's total weight.
+__global__ void ga_fit_init_kernel(unsigned int* __restrict__ fit,
+                                   const unsigned int* __restrict__ base_total, int pop, size_t n) {
+    const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (i < n)
+        fit[i] = base_total[i / pop];
 }
 
 // ------------------------------------------------------------ breed ------
@@ -834,10 +939,8 @@ __global__ __launch_bounds__(256, 2) void ga_win_kernel(const uint64_t* __restri
         wcount[threadIdx.x / kWave] = cnt;
     __syncthreads();
     const int32_t total = (int32_t)(wcount[0] + wcount[1] + wcount[2] + wcount[3]);
-    int32_t best_h = INT32_MIN;
-    uint32_t best_mask = 0;
     const uint32_t hi_mask = (1u << (W - kPkL)) - 1u;
-    for (uint32_t mh = 0; mh <= hi_mask; ++mh) {
+    auto build = [&](uint32_t mh) {
         int4* s4 = reinterpret_cast<int4*>(s);
         const int bias = (int)kPkBias;
         for (int q = threadIdx.x; q < (pk_lds_words() + 3) / 4; q += blockDim.x)
@@ -853,36 +956,68 @@ __global__ __launch_bounds__(256, 2) void ga_win_kernel(const uint64_t* __restri
                 atomicAdd(&s[pk_pad((int)(pl & ((1u << kPkW) - 1u)))], neg ? 0u - one : one);
             }
         }
-        // Thread t's entries: ml = 2 t + m' with m' = j + 512 h + 1024 e
-        // (+ 2^13 for the high half), keyed H * 2^16 | (65535 - 4 m').
-        int32_t best = INT32_MIN;
-        pk_block_fwht_visit_paired(s, [&](int jh, const uint32_t (&v)[8]) {
+    };
+    // Pass 1: the largest H of each block; the first block reaching the
+    // overall maximum holds the lowest mask reaching it.
+    __shared__ int32_t s_best_h;
+    __shared__ uint32_t s_best_mh;
+    int32_t best_h = INT32_MIN;   // thread 0's running values
+    uint32_t best_mh = 0;
+    for (uint32_t mh = 0; mh <= hi_mask; ++mh) {
+        build(mh);
+        s16x2 m = {INT16_MIN, INT16_MIN};
+        pk_block_fwht_visit_paired(s, [&](int, const uint32_t (&v)[8]) {
 #pragma unroll
-            for (int e = 0; e < 8; ++e) {
-                const uint32_t m_lo = (uint32_t)((jh & 1) + 512 * (jh >> 1) + 1024 * e);
-                best = max(best, (int32_t)((v[e] << 16) | (65535u - 4u * m_lo)));
-                best = max(best, (int32_t)((v[e] & 0xffff0000u) |
-                                           (65535u - 4u * (m_lo + (1u << kPkW)))));
-            }
+            for (int e = 0; e < 8; ++e)
+                m = pk_max(m, pk_as(v[e]));
         });
-        {
-            const int32_t h = best >> 16;
-            const uint32_t ml = ((65535u - ((uint32_t)best & 0xffffu)) >> 2) + 2u * threadIdx.x;
-            best = (int32_t)(((uint32_t)h << 16) | (65535u - 4u * ml));
-        }
-        best = wave_max_i32(best);
+        const int32_t bm = wave_max_i32(max(pk_lo(m), pk_hi(m)));
         if (threadIdx.x % kWave == 0)
-            wbest[threadIdx.x / kWave] = best;
+            wbest[threadIdx.x / kWave] = bm;
         __syncthreads();
         if (threadIdx.x == 0) {
-            const int32_t k = max(max(wbest[0], wbest[1]), max(wbest[2], wbest[3]));
-            const int32_t h = k >> 16;
+            const int32_t h = max(max(wbest[0], wbest[1]), max(wbest[2], wbest[3]));
             if (h > best_h) {
                 best_h = h;
-                best_mask = (mh << kPkL) | ((65535u - ((uint32_t)k & 0xffffu)) >> 2);
+                best_mh = mh;
             }
         }
         __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        s_best_h = best_h;
+        s_best_mh = best_mh;
+    }
+    __syncthreads();
+    best_h = s_best_h;
+    const uint32_t mh = s_best_mh;
+    // Pass 2: that block again, keyed H * 2^16 | (65535 - 4 m') per entry
+    // (thread t's entries: ml = 2 t + m' with m' = j + 512 h + 1024 e, + 2^13
+    // for the high half), so one max keeps the largest H and the lowest ml.
+    build(mh);
+    int32_t best = INT32_MIN;
+    pk_block_fwht_visit_paired(s, [&](int jh, const uint32_t (&v)[8]) {
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const uint32_t m_lo = (uint32_t)((jh & 1) + 512 * (jh >> 1) + 1024 * e);
+            best = max(best, (int32_t)((v[e] << 16) | (65535u - 4u * m_lo)));
+            best = max(best, (int32_t)((v[e] & 0xffff0000u) |
+                                       (65535u - 4u * (m_lo + (1u << kPkW)))));
+        }
+    });
+    {
+        const int32_t h = best >> 16;
+        const uint32_t ml = ((65535u - ((uint32_t)best & 0xffffu)) >> 2) + 2u * threadIdx.x;
+        best = (int32_t)(((uint32_t)h << 16) | (65535u - 4u * ml));
+    }
+    best = wave_max_i32(best);
+    if (threadIdx.x % kWave == 0)
+        wbest[threadIdx.x / kWave] = best;
+    __syncthreads();
+    uint32_t best_mask = 0;
+    if (threadIdx.x == 0) {
+        const int32_t k = max(max(wbest[0], wbest[1]), max(wbest[2], wbest[3]));
+        best_mask = (mh << kPkL) | ((65535u - ((uint32_t)k & 0xffffu)) >> 2);
     }
     if (threadIdx.x == 0) {
         const uint32_t w = (uint32_t)(total - best_h) >> 1;
@@ -1204,7 +1339,7 @@ struct Ga {
     bmm_ga_params P{};
     int W32 = 0, words = 0;
     DevBuf base, base_tmp, ops[2], len[2], fit, rng, best_w, best_ops, best_len, cand, flag,
-        w_snap, roww, sketch, evals;
+        w_snap, roww, sketch, evals, n_ops, base_total;
     // proposal model data
     DevBuf pairs, pair_isl, feats, labels, prng;
     DevBuf li_move, li_w, li_changed;
@@ -1279,14 +1414,16 @@ uint64_t xs_host(uint64_t& s) {
 void enqueue_generation(Ga& G, int parity, hipStream_t s) {
     const int I = G.P.islands, Pp = G.P.population;
     unsigned int* fit = static_cast<unsigned int*>(G.fit.ptr);
-    fill_u32(fit, (size_t)I * Pp, 0u, s);
+    const size_t n_ind = (size_t)I * Pp;
+    hipLaunchKernelGGL(ga_fit_init_kernel, dim3((unsigned)((n_ind + 255) / 256)), dim3(256), 0, s,
+                       fit, static_cast<const unsigned int*>(G.base_total.ptr), Pp, n_ind);
     const int slices = G.W32 / 64;
-    const size_t lds = (size_t)G.P.rows * 64 * 4;
+    const size_t lds = ((size_t)G.P.rows * 64 + (size_t)((G.P.max_ops + 3) & ~3)) * 4;
     hipLaunchKernelGGL(ga_eval_kernel, dim3((unsigned)(I * Pp * slices)), dim3(64), lds, s,
                        static_cast<const uint32_t*>(G.base.ptr),
                        static_cast<const uint16_t*>(G.ops[parity].ptr),
                        static_cast<const int32_t*>(G.len[parity].ptr), Pp, G.P.rows, G.W32,
-                       G.P.max_ops, fit);
+                       G.P.max_ops, fit, static_cast<unsigned long long*>(G.n_ops.ptr));
     BMM_HIP(hipGetLastError());
     hipLaunchKernelGGL(ga_breed_kernel, dim3(I), dim3(std::max(64, (Pp + 63) / 64 * 64)),
                        breed_lds_bytes(G.P), s, dev_params(G), fit,
@@ -1332,6 +1469,15 @@ void refresh_sketches(Ga& G) {
     BMM_HIP(hipGetLastError());
 }
 
+// Per island the base's total weight (from the row weights of the last
+// refresh), the starting fitness of every evaluation.
+void update_base_total(Ga& G) {
+    hipLaunchKernelGGL(ga_base_weight_kernel, dim3(G.P.islands), dim3(256), 0, G.stream,
+                       static_cast<const uint32_t*>(G.roww.ptr), G.P.rows,
+                       static_cast<unsigned int*>(G.base_total.ptr));
+    BMM_HIP(hipGetLastError());
+}
+
 // Samples n pairs per island into G.pairs / G.pair_isl.
 void sample_pairs(Ga& G, int per_island) {
     const int n = per_island * G.P.islands;
@@ -1359,6 +1505,7 @@ constexpr float kLabelScale = 64.f;
 void update_proposals(Ga& G) {
     const int I = G.P.islands;
     refresh_sketches(G);
+    update_base_total(G);
     if (!G.P.learned)
         return;
     // training batch
@@ -1625,10 +1772,12 @@ Ga* ga_create(const uint64_t* matrix, size_t rows, size_t cols, const bmm_ga_par
     G->flag.get<int32_t>(I);
     G->w_snap.get<unsigned int>(I);
     G->roww.get<uint32_t>((size_t)I * rows);
+    G->base_total.get<unsigned int>((size_t)I);
     G->sketch.get<uint64_t>((size_t)I * rows * 2);
     G->cand.get<uint16_t>((size_t)I * std::max(1, P.n_candidates));
     unsigned long long* ev = G->evals.get<unsigned long long>(1);
     BMM_HIP(hipMemset(ev, 0, 8));
+    BMM_HIP(hipMemset(G->n_ops.get<unsigned long long>(1), 0, 8));
     // base weight as the initial best
     uint64_t w0 = 0;
     for (size_t i = 0; i < per; ++i)
@@ -1714,8 +1863,9 @@ void ga_run(Ga* G, size_t max_generations, uint64_t target, double time_budget_s
     }
     BMM_HIP(hipMemcpyAsync(bw.data(), G->best_w.ptr, (size_t)I * 4, hipMemcpyDeviceToHost,
                            G->stream));
-    unsigned long long ev = 0;
+    unsigned long long ev = 0, nops = 0;
     BMM_HIP(hipMemcpyAsync(&ev, G->evals.ptr, 8, hipMemcpyDeviceToHost, G->stream));
+    BMM_HIP(hipMemcpyAsync(&nops, G->n_ops.ptr, 8, hipMemcpyDeviceToHost, G->stream));
     BMM_HIP(hipStreamSynchronize(G->stream));
     st->generations = (size_t)G->generation;
     st->ran = done;
@@ -1729,6 +1879,7 @@ void ga_run(Ga* G, size_t max_generations, uint64_t target, double time_budget_s
     st->li_moves = G->li_moves;
     st->win_moves = G->win_moves;
     st->isd_rows = G->isd_rows;
+    st->replayed_ops = nops;
 }
 
 // Best matrix: the best island's base with its best sequence replayed.
@@ -1860,6 +2011,9 @@ void ga_set_island_base(Ga* G, int island, const uint64_t* matrix) {
                       hipMemcpyHostToDevice));
     BMM_HIP(hipMemcpy(static_cast<int32_t*>(G->best_len.ptr) + island, &zero, 4,
                       hipMemcpyHostToDevice));
+    refresh_sketches(*G);
+    update_base_total(*G);
+    BMM_HIP(hipStreamSynchronize(G->stream));
 }
 
 // Per-island seeds (individual streams of island i restarted from seeds[i],

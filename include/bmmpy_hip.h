@@ -339,6 +339,7 @@ typedef struct bmm_ga_status {
     uint64_t li_moves;    /* local-improvement moves applied so far */
     uint64_t win_moves;   /* window-improvement moves applied so far */
     uint64_t isd_rows;    /* rows taken from information-set trials so far */
+    uint64_t replayed_ops; /* row XOR ops replayed by fitness evaluation so far */
 } bmm_ga_status;
 
 int bmm_hip_ga_default_params(bmm_ga_params *out);

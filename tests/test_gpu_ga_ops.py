@@ -66,10 +66,13 @@ def test_breeding_matches_restatement(lib, ora, name):
     try:
         ga.configure_islands(seeds=[11, 12])
         P = params_of(p, 32)
-        modes, restarts = set(), 0
+        modes, restarts, ops_seen = set(), 0, 0
         for gen in range(12):
             pre = [ga.population(i, 48, 40) for i in range(2)]
-            ga.run(1)
+            st_run = ga.run(1)
+            # every evaluated individual's ops are counted once
+            ops_seen += sum(int(p["len_next"].sum()) for p in pre)
+            assert st_run.replayed_ops == ops_seen
             for isl in range(2):
                 a, b = pre[isl], ga.population(isl, 48, 40)
                 assert np.array_equal(b["len_prev"], a["len_next"])
