@@ -408,7 +408,7 @@ __device__ void coset_block_phase2_pk16(uint32_t* s, const CosetMatrix& M, const
 // counter, so the next pass starts from the cleared state without fill
 // launches (the plan clears both once when it is built).
 __global__ __launch_bounds__(256) void coset_phase2_kernel(
-    const CosetMatrix* __restrict__ mats, const int32_t* __restrict__ slot_matrix,
+    const CosetMatrix* __restrict__ mats, const int32_t* __restrict__ slot_code,
     const uint32_t* __restrict__ pats, unsigned long long* __restrict__ keys,
     unsigned long long* __restrict__ counter, uint32_t* __restrict__ out_w,
     uint64_t* __restrict__ out_idx) {
@@ -417,8 +417,9 @@ __global__ __launch_bounds__(256) void coset_phase2_kernel(
     __shared__ __align__(16) int32_t s[kWords + 4];
     __shared__ uint32_t low;
     const int slot = blockIdx.x;
-    const CosetMatrix M = mats[slot_matrix[slot]];
-    const int target = (int)(slot - M.out_off);
+    const int32_t code = slot_code[slot];
+    const CosetMatrix M = mats[code >> 5];
+    const int target = code & 31;
     const unsigned long long key = keys[slot];
     __syncthreads();   // every thread has read the key before it is cleared
     if (threadIdx.x == 0) {
@@ -468,25 +469,45 @@ __device__ void coset_block_exact(int32_t* s, const CosetMatrix& M, int32_t tota
                                   const uint32_t (&preg)[kPatRegs],
                                   unsigned long long* __restrict__ keys) {
     constexpr int N = 1 << L;
-    constexpr int E = N >= 256 ? N / 256 : 1;
     build_from_regs<L>(s, preg, mh);
-    block_fwht<L>(s);
     const int tid = threadIdx.x;
     int32_t best[kFusedMaxRows];
 #pragma unroll
     for (int b = 0; b < kFusedMaxRows; ++b)
         best[b] = INT32_MIN;
     int32_t overall = INT32_MIN;
-    if (tid < N) {
+    if constexpr (L >= 11) {
+        // The last transform round stays in registers: entry ml = (q + 4 e) * 256
+        // + tid, so bits 0-7 of ml are the thread's own and bits 8.. follow q and e.
+        block_fwht_visit<L, 0>(s, [&](int q, const auto& v) {
 #pragma unroll
-        for (int e = 0; e < E; ++e) {
-            const int ml = e * 256 + tid;
-            const int32_t k = s[fwht_pad(ml)] * N + (N - 1 - ml);
-            overall = max(overall, k);
+            for (int e = 0; e < (1 << (L - 10)); ++e) {
+                const int ml = (q + 4 * e) * 256 + tid;
+                const int32_t k = (int32_t)v[e] * N + (N - 1 - ml);
+                overall = max(overall, k);
 #pragma unroll
-            for (int b = 0; b < L; ++b)
-                if ((ml >> b) & 1)
-                    best[b] = max(best[b], k);
+                for (int b = 8; b < L; ++b)
+                    if ((ml >> b) & 1)
+                        best[b] = max(best[b], k);
+            }
+        });
+#pragma unroll
+        for (int b = 0; b < 8; ++b)
+            best[b] = ((tid >> b) & 1) ? overall : INT32_MIN;
+    } else {
+        constexpr int E = N >= 256 ? N / 256 : 1;
+        block_fwht<L>(s);
+        if (tid < N) {
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                const int ml = e * 256 + tid;
+                const int32_t k = s[fwht_pad(ml)] * N + (N - 1 - ml);
+                overall = max(overall, k);
+#pragma unroll
+                for (int b = 0; b < L; ++b)
+                    if ((ml >> b) & 1)
+                        best[b] = max(best[b], k);
+            }
         }
     }
     __shared__ int32_t red[4][kFusedMaxRows + 1];
@@ -519,25 +540,32 @@ __device__ void coset_block_exact(int32_t* s, const CosetMatrix& M, int32_t tota
     }
 }
 
+// A one-matrix plan (n_mat == 1) takes its matrix from the argument `first`
+// and reads no descriptor from memory; otherwise a workgroup finds its
+// matrix in `mats` by bisecting the block prefix.
 __global__ __launch_bounds__(256) void coset_fused_kernel(
-    const uint64_t* __restrict__ words, const CosetMatrix* __restrict__ mats,
-    const int64_t* __restrict__ prefix, int n_mat, const int32_t* __restrict__ slot_matrix,
+    const uint64_t* __restrict__ words, CosetMatrix first, const CosetMatrix* __restrict__ mats,
+    const int64_t* __restrict__ prefix, int n_mat, const int32_t* __restrict__ slot_code,
     int n_slots, unsigned long long* __restrict__ keys, unsigned int* __restrict__ done,
     uint32_t* __restrict__ out_w, uint64_t* __restrict__ out_idx) {
     __shared__ __align__(16) int32_t s[fwht_lds_words(kBatchMaxL) + 4];
     __shared__ uint32_t cnt[4];
     __shared__ int last;
     const int64_t g = blockIdx.x;
-    int lo = 0, hi = n_mat - 1;     // prefix[lo] <= g < prefix[lo + 1]
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (prefix[mid] <= g)
-            lo = mid;
-        else
-            hi = mid - 1;
+    CosetMatrix M = first;
+    uint32_t mh = (uint32_t)g;
+    if (n_mat > 1) {
+        int lo = 0, hi = n_mat - 1;     // prefix[lo] <= g < prefix[lo + 1]
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (prefix[mid] <= g)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+        M = mats[lo];
+        mh = (uint32_t)(g - prefix[lo]);
     }
-    const CosetMatrix M = mats[lo];
-    const uint32_t mh = (uint32_t)(g - prefix[lo]);
     // The matrix (at most 31 x 32 words) goes to LDS in one coalesced pass,
     // then each thread forms the patterns of its columns from there.
     const uint64_t* rows = words + M.word_off;
@@ -585,8 +613,7 @@ __global__ __launch_bounds__(256) void coset_fused_kernel(
     for (int slot = threadIdx.x; slot < n_slots; slot += blockDim.x) {
         const unsigned long long key =
             __hip_atomic_load(&keys[slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        const CosetMatrix& S = mats[slot_matrix[slot]];
-        const int target = (int)(slot - S.out_off);
+        const int target = n_mat == 1 ? (int)(slot - first.out_off) : (slot_code[slot] & 31);
         const uint64_t mask = (uint32_t)key;
         const uint64_t below = mask & ((1ull << target) - 1ull);
         const uint64_t above = mask >> (target + 1);
@@ -617,7 +644,9 @@ struct BatchPlan {
     const uint64_t* d_words = nullptr;
     uint32_t* d_out_w = nullptr;
     uint64_t* d_out_idx = nullptr;
-    DevBuf mats, prefix, pats, keys, counter, slot_matrix, done;
+    // slot_code[slot] = (index of the slot's matrix in the sorted order) << 5 | target row
+    DevBuf mats, prefix, pats, keys, counter, slot_code, done;
+    CosetMatrix first{};        // sorted matrix 0 (a one-matrix plan passes it by value)
     std::vector<int32_t> rows, cols;
     std::vector<int64_t> word_off, out_off;
     hipStream_t cap_stream = nullptr;
@@ -675,20 +704,21 @@ void plan_upload(BatchPlan& P, int lmax) {
     P.n_full = n_full;
     P.full_blocks = prefix[n_full];
     P.total_blocks = prefix[n];
-    std::vector<int32_t> slot_matrix((size_t)P.n_slots, -1);
+    std::vector<int32_t> slot_code((size_t)P.n_slots, -1);
     for (size_t i = 0; i < n; ++i)
         for (int t = 0; t < sorted[i].rows; ++t) {
-            int32_t& sm = slot_matrix[(size_t)(sorted[i].out_off + t)];
+            int32_t& sm = slot_code[(size_t)(sorted[i].out_off + t)];
             if (sm != -1)
                 fail(BMM_ERR_INVALID, "coset_min: overlapping output offsets");
-            sm = (int32_t)i;
+            sm = (int32_t)(i << 5) | t;
         }
-    for (int32_t sm : slot_matrix)
+    P.first = sorted[0];
+    for (int32_t sm : slot_code)
         if (sm < 0)
             fail(BMM_ERR_INVALID, "coset_min: output offsets leave gaps");
     CosetMatrix* d_m = P.mats.get<CosetMatrix>(n);
     int64_t* d_p = P.prefix.get<int64_t>(n + 1);
-    int32_t* d_s = P.slot_matrix.get<int32_t>((size_t)P.n_slots);
+    int32_t* d_s = P.slot_code.get<int32_t>((size_t)P.n_slots);
     P.pats.get<uint32_t>((size_t)std::max<int64_t>(pat_off, 1));
     // Cleared once here; every pass's phase 2 leaves them cleared again.
     fill_u64(reinterpret_cast<uint64_t*>(P.keys.get<unsigned long long>((size_t)P.n_slots)),
@@ -696,7 +726,7 @@ void plan_upload(BatchPlan& P, int lmax) {
     fill_u64(reinterpret_cast<uint64_t*>(P.counter.get<unsigned long long>(1)), 1, 0ull, nullptr);
     BMM_HIP(hipMemcpy(d_m, sorted.data(), sizeof(CosetMatrix) * n, hipMemcpyHostToDevice));
     BMM_HIP(hipMemcpy(d_p, prefix.data(), sizeof(int64_t) * (n + 1), hipMemcpyHostToDevice));
-    BMM_HIP(hipMemcpy(d_s, slot_matrix.data(), sizeof(int32_t) * slot_matrix.size(),
+    BMM_HIP(hipMemcpy(d_s, slot_code.data(), sizeof(int32_t) * slot_code.size(),
                       hipMemcpyHostToDevice));
 }
 
@@ -706,11 +736,11 @@ void plan_enqueue(BatchPlan& P, hipStream_t s, bool timed) {
     uint32_t* d_pat = static_cast<uint32_t*>(P.pats.ptr);
     unsigned long long* d_k = static_cast<unsigned long long*>(P.keys.ptr);
     unsigned long long* d_c = static_cast<unsigned long long*>(P.counter.ptr);
-    int32_t* d_s = static_cast<int32_t*>(P.slot_matrix.ptr);
+    int32_t* d_s = static_cast<int32_t*>(P.slot_code.ptr);
     if (P.fused) {
         ScopedKernelTimer timer(s, timed);
         hipLaunchKernelGGL(coset_fused_kernel, dim3((unsigned)P.total_blocks), dim3(256), 0, s,
-                           P.d_words, d_m, d_p, (int)P.n_mat, d_s, (int)P.n_slots, d_k,
+                           P.d_words, P.first, d_m, d_p, (int)P.n_mat, d_s, (int)P.n_slots, d_k,
                            static_cast<unsigned int*>(P.done.ptr), P.d_out_w, P.d_out_idx);
         BMM_HIP(hipGetLastError());
         return;
@@ -884,7 +914,9 @@ BatchPlan* batch_plan_create(const uint64_t* d_words, const int64_t* word_offset
 }
 
 void batch_plan_run(BatchPlan* P, hipStream_t stream, bool use_graph) {
-    if (!use_graph) {
+    // A single-launch plan gains nothing from a one-node graph, whose launch
+    // costs more than the kernel's own (about 7 us per pass for 20 x 128).
+    if (!use_graph || P->fused) {
         plan_enqueue(*P, stream, true);
         return;
     }

@@ -139,7 +139,10 @@ int bmm_hip_coset_min_batch_dev(const uint64_t *d_row_words, const int64_t *word
  * descriptors and work queue are built once (optionally autotuning the launch
  * shape: workgroups, queue chunk, block size, timed on this batch), and
  * run() enqueues the whole sequence, as one captured hipGraph when use_graph
- * is set.  Pointers given at creation must stay valid until destroy. */
+ * is set.  Small batches (few blocks, at most 2048 columns per matrix) run
+ * as one kernel, which run() launches directly either way: a graph of one
+ * node adds only the graph launch's own cost.  Pointers given at creation
+ * must stay valid until destroy. */
 typedef struct bmm_batch_info {
     size_t n_matrices;
     size_t n_slots;
