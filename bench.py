@@ -65,9 +65,11 @@ TRAFFIC_BYTES = {
     "gray_coset_sweep": (192.5 * 2.0 + 160.0) * 1024,
     "coset_batch_4096": (53729.0 * 2.0 + 29953.0) * 1024,
     "window_coset_planted_48x4096": (118.4 * 2.0 + 131137.0) * 1024,
-    # The four kernels of one calc_scores_and_order call on 2^28 entries.
-    "scores_order_2p28": ((524558.6 + 537199.4 + 524304.9 + 526393.4) * 2.0
-                          + 1091265.5 + 1048633.4 + 34645.5 + 1048580.2) * 1024,
+    # The kernels of one calc_scores_and_order call on 2^28 entries (scores,
+    # the three table-scan kernels, the tile order), FETCH_SIZE x 2.0 plus
+    # WRITE_SIZE x 1.0 in KiB.
+    "scores_order_2p28": ((524422.2 + 32791.5 + 261.3 + 33051.0 + 557233.6) * 2.0
+                          + 1118208.0 + 512.0 + 512.0 + 65536.0 + 1079401.8) * 1024,
 }
 
 
@@ -210,6 +212,13 @@ def run_batch_plan(torch, _lib, dev, stream, name: str, steps: int, warmup: int,
         b.record(stream)
         torch.cuda.synchronize()
         ms = a.elapsed_time(b) / steps
+        # The same passes launched directly (no graph), back to back.
+        a.record(stream)
+        for _ in range(steps):
+            _lib.call("bmm_hip_batch_plan_run", plan, 0, sp)
+        b.record(stream)
+        torch.cuda.synchronize()
+        ms_eager = a.elapsed_time(b) / steps
         # Kernel-level time of the persistent transform kernel (eager launches,
         # events around that kernel only).
         _lib.timing_begin()
@@ -227,7 +236,7 @@ def run_batch_plan(torch, _lib, dev, stream, name: str, steps: int, warmup: int,
     packed = info.max_block_bits == 14
     peak = VALU_PK16_PEAK_TOPS if packed else VALU_PEAK_TOPS
     return {
-        "ms": ms, "evals": info.subset_evals, "weights": weights,
+        "ms": ms, "ms_eager": ms_eager, "evals": info.subset_evals, "weights": weights,
         "config": {"workload": "coset_20x128" if name == "coset" else "coset_batch_4096",
                    "matrices": int(rows.size), "rows_total": n_slots,
                    "seed": "0x5eed0001" if name == "coset" else "0x5eed0004",
@@ -471,15 +480,14 @@ def run_scores(torch, steps: int, warmup: int) -> dict:
         s, order, cnt, off = _lib.calc_scores_and_order(h, SCORES_N)
     dt = (time.perf_counter() - t0) / steps
     kms, nk = _lib.timing_end()
-    kernel_ms = kms / max(steps, 1)   # the scores kernel and the radix passes of one call
+    kernel_ms = kms / max(steps, 1)   # the kernels of one call (two timed regions)
     alg = 12.0 * m
     achieved = alg / (kernel_ms / 1e3) / 1e9
-    # Moved by the kernels (10-bit scores: a 6-bit and a 4-bit digit pass):
-    # scores (read h, write s: 8 B, the first digit histogram on the side),
-    # first scatter (read the scores, write mask | upper score bits: 8 B),
-    # second pass's histogram (read the packed words: 4 B) and scatter (read
-    # them, write the order: 8 B): 28 B per mask.  rocprof's FETCH_SIZE and
-    # WRITE_SIZE of the four kernels (copy-calibrated) give the figure used.
+    # Moved by the kernels (10-bit scores, one-pass tile order): scores (read
+    # h, write s: 8 B, plus the per-tile score histogram, 0.25 B), the table
+    # scan (0.75 B) and the tile order (read s, write the order: 8 B, plus the
+    # tile's table row): about 17.2 B per mask.  rocprof's FETCH_SIZE and
+    # WRITE_SIZE of the kernels (copy-calibrated) give the figure used.
     moved = TRAFFIC_BYTES["scores_order_2p28"]
     return {
         "ms": 1e3 * dt, "evals": float(m - 1),
@@ -489,8 +497,8 @@ def run_scores(torch, steps: int, warmup: int) -> dict:
                    "seed": hex(SCORES_SEED), "subset_evals_per_step": m - 1},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": 8000.0, "unit": "GB/s",
                      "frac": achieved / 8000.0, "traffic": moved, "bytes_per_launch": alg,
-                     "kernel": "scores_kernel + radix_scatter_kernel + radix_hist_kernel "
-                               "+ radix_scatter_kernel (+ digit-table scans)",
+                     "kernel": "scores_tile_kernel + col_sum/col_base/col_start_kernel "
+                               "+ order_tile_kernel",
                      "kernel_avg_ms": kernel_ms, "host_call_ms": 1e3 * dt},
     }
 
@@ -722,6 +730,7 @@ def secondary(torch, _lib, dev, stream) -> dict:
     for name, key in (("coset", "coset_20x128"), ("batch", "batch_4096")):
         r = run_batch_plan(torch, _lib, dev, stream, name, 50 if name == "coset" else 5, 2)
         res[key] = {"subset_evals": r["evals"], "ms_per_pass": r["ms"],
+                    "ms_per_pass_direct_launch": r["ms_eager"],
                     "subset_evals_per_s": r["evals"] / (r["ms"] / 1e3),
                     "transform_elements_per_s":
                         r["config"]["transform_elements_per_step"] / (r["ms"] / 1e3),
@@ -742,6 +751,8 @@ def mode_line(args, r: dict, cpu: dict | None) -> dict:
     }
     if "results" in r:
         out["results"] = r["results"]
+    if "ms_eager" in r:
+        out["ms_per_step_direct_launch"] = r["ms_eager"]
     if cpu is not None:
         out["cpu_baseline"] = cpu
     return out

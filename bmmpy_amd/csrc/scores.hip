@@ -6,28 +6,27 @@
 // score, stably (ascending u within a score); off[v] ends as the end of
 // bucket v in order (the reference's offsets after its scatter).
 //
-// The stable counting sort is an LSD radix sort over digits of the score:
-// ceil(bits(n) / 8) passes with the bits split evenly (a 10-bit score: two
-// 5-bit digits).  Two passes pack the mask and the score bits above the
-// first digit into one 32-bit word between them when both fit (2^28 masks
-// with 10-bit scores: a 6-bit first digit, 28 + 4 bits packed), which saves
-// 8 bytes of traffic per mask.  The scores kernel also writes the first
-// pass's digit histogram per tile of 4096 masks; later passes count their
-// digits in a histogram kernel.  On the first pass the masks themselves are
-// the values.  The digit-major table (digit, tile) is scanned exclusively on
-// the device, which gives each tile the global start of each digit's run.
-// The scatter stages a tile in LDS: each wave ranks its 512 contiguous masks
-// (8 slices of 64; equal digits matched with one ballot per digit bit,
-// running per-wave digit counts in LDS), the tile is reordered by digit in
-// LDS (one word per entry where the value can be rebuilt from it), and the
-// runs are written out with consecutive lanes on consecutive addresses.
-// Tiles of 4096 keep the scatter at 57-73 VGPRs and 26-43 KiB of LDS, so 3-4
-// workgroups share a CU and hide each other's load latency: 8192-mask tiles
-// (121 VGPRs, 2 workgroups per CU) took 1.53 ms for the two scatters, 4096
-// take 1.30 ms and 2048 (runs half as long, twice the digit table) 1.50 ms.
-// The one-word variants are held to 64 VGPRs (8 waves per SIMD, 4
-// workgroups per CU): the first pass needed 65, and the bound (2 spilled
-// dwords) took it from 741 to 686 us.
+// Scores of at most 10 bits (n < 1024) take the one-pass tile order (the
+// section "one-pass order" below): 2^14-mask tiles, each sorted stably in
+// LDS and written run by run from a scanned per-tile score histogram, 16 B
+// of traffic per mask in all (2^28 masks, n = 1023: 1.19 ms of kernels,
+// against 1.94 ms for the radix passes).
+//
+// Wider scores take an LSD radix sort over digits of the score:
+// ceil(bits(n) / 8) passes with the bits split evenly.  Two passes pack the
+// mask and the score bits above the first digit into one 32-bit word between
+// them when both fit, which saves 8 bytes of traffic per mask.  The scores
+// kernel also writes the first pass's digit histogram per tile of 4096
+// masks; later passes count their digits in a histogram kernel.  On the
+// first pass the masks themselves are the values.  The digit-major table
+// (digit, tile) is scanned exclusively on the device, which gives each tile
+// the global start of each digit's run.  The scatter stages a tile in LDS:
+// each wave ranks its 512 contiguous masks (8 slices of 64; equal digits
+// matched with one ballot per digit bit, running per-wave digit counts in
+// LDS), the tile is reordered by digit in LDS (one word per entry where the
+// value can be rebuilt from it), and the runs are written out with
+// consecutive lanes on consecutive addresses.  Tiles of 4096 keep the
+// scatter at 57-73 VGPRs and 26-43 KiB of LDS, so 3-4 workgroups share a CU.
 #include "common.hpp"
 
 #include <algorithm>
@@ -55,6 +54,9 @@ constexpr int kRsThreads = kRsWaves * kWave;
 constexpr int kRsSlices = kRsTile / kRsThreads;   // 8 slices of 64 per wave
 constexpr int kScanBlock = 1024;
 constexpr int kLdsBuckets = 16384;
+constexpr int kTileKeyBits = 10;      // one-pass order: scores below 2^10
+constexpr int kOrderTileBits = 14;    // masks per tile sort (2^14: 66 KiB of LDS, two tiles per CU)
+constexpr uint32_t kScoresGrid = 1024;
 
 template <typename S>
 struct Vec4;
@@ -422,8 +424,473 @@ __global__ __launch_bounds__(256) void scan_add_kernel(uint32_t* __restrict__ da
         data[i] += sums[i / kScanBlock];
 }
 
+// ---------------------------------------------------- one-pass order ----
+// Scores of at most 10 bits (n < 1024) are ordered in one pass over tiles
+// of 2^TB masks: the scores kernel writes each tile's score histogram as a
+// row of the table (table[tile * K + k]), a column scan in chunks of tiles
+// turns it into the global start of every (score, tile) run, and each tile
+// is sorted stably in LDS (two digit passes on the packed word
+// score << TB | offset; the offsets never leave LDS) and written out run by
+// run.  Per mask the
+// call then reads h and the scores once and writes the scores and the order
+// once (16 B), against 28 B for two global radix passes.  Mask 0 (score 0,
+// the first entry of tile 0) and the offsets past m (read as score 0, last
+// among the tile's score-0 entries) are sorted with the rest but not written.
+
+template <typename S, int TB>
+__global__ __launch_bounds__(256) void scores_tile_kernel(const S* __restrict__ h, uint64_t m,
+                                                          int32_t n, S* __restrict__ s_out,
+                                                          unsigned int* __restrict__ cnt,
+                                                          unsigned int* __restrict__ overflow,
+                                                          uint32_t n_tiles,
+                                                          uint32_t* __restrict__ table) {
+    using V = typename Vec4<S>::T;
+    constexpr int kVecs = 4;                          // 4 vectors of 4 entries per thread
+    constexpr int kSubs = (1 << TB) / (256 * 4 * kVecs);
+    extern __shared__ unsigned int sh[];
+    const int K = n + 1;
+    unsigned int* th = sh;          // this tile's counts
+    unsigned int* tot = sh + K;     // the workgroup's counts over all its tiles
+    for (int q = threadIdx.x; q < K; q += blockDim.x)
+        tot[q] = 0;
+    for (uint32_t v = blockIdx.x; v < n_tiles; v += gridDim.x) {
+        // Windows of gridDim.x tiles, front to back.
+        const uint32_t tile = v;
+        // Thread t reads and clears the same entries (q = t mod 256): the
+        // barrier after the clear is the only one the table write needs.
+        for (int q = threadIdx.x; q < K; q += blockDim.x)
+            th[q] = 0;
+        __syncthreads();
+        for (int sub = 0; sub < kSubs; ++sub) {
+            const uint64_t base =
+                ((uint64_t)tile << TB) + (uint64_t)sub * (256 * 4 * kVecs) + 4u * threadIdx.x;
+            S hv[kVecs][4];
+#pragma unroll
+            for (int k = 0; k < kVecs; ++k) {
+                const uint64_t u = base + (uint64_t)k * 1024;
+                if (u + 4 <= m) {
+                    const V x = *reinterpret_cast<const V*>(h + u);
+                    hv[k][0] = x.x;
+                    hv[k][1] = x.y;
+                    hv[k][2] = x.z;
+                    hv[k][3] = x.w;
+                } else {
+#pragma unroll
+                    for (int j = 0; j < 4; ++j)
+                        hv[k][j] = u + j < m ? h[u + j] : S(0);
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < kVecs; ++k) {
+                const uint64_t u = base + (uint64_t)k * 1024;
+                S sv[4];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    int64_t su = ((int64_t)n - (int64_t)hv[k][j]) >> 1;
+                    su = su < 0 ? 0 : (su > n ? n : su);
+                    const bool valid = u + j >= 1 && u + j < m;
+                    sv[j] = valid ? S(su) : S(0);   // s[0] = 0
+                    if (!valid)
+                        continue;
+                    if (std::is_same<S, int16_t>::value && su > 32767)
+                        atomicOr(overflow, 1u);
+                    atomicAdd(&th[su], 1u);
+                }
+                if (u + 4 <= m) {
+                    V x;
+                    x.x = sv[0];
+                    x.y = sv[1];
+                    x.z = sv[2];
+                    x.w = sv[3];
+                    *reinterpret_cast<V*>(s_out + u) = x;
+                } else {
+#pragma unroll
+                    for (int j = 0; j < 4; ++j)
+                        if (u + j < m)
+                            s_out[u + j] = sv[j];
+                }
+            }
+        }
+        __syncthreads();
+        const int Kp = (K + 3) & ~3;
+        for (int q = threadIdx.x; q < Kp; q += blockDim.x) {
+            const unsigned int c = q < K ? th[q] : 0u;
+            table[(size_t)tile * Kp + q] = c;
+            if (q < K)
+                tot[q] += c;
+        }
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < K; q += blockDim.x)
+        if (tot[q])
+            atomicAdd(&cnt[q], tot[q]);
+}
+
+// LDS layouts of a tile sort: data entry i at i + i/32 (rows of 32 with a
+// pad word, so a thread reading 32 consecutive entries and its neighbours
+// hit distinct banks); digit counters as rows of T thread columns, two
+// 16-bit counts per word (digits 2r and 2r + 1 in row r), one pad word per
+// 16 columns.
+// Column scan of the table (rows = tiles, Kp = K rounded up to 4 words per
+// row, 4 keys per thread as one 16-byte access) in chunks of kScanRows rows:
+// col_sum_kernel totals each chunk's columns, col_base_kernel turns the chunk
+// totals into chunk starts (key-major: all of key k before key k + 1), and
+// col_start_kernel walks each chunk's rows, replacing every count by its
+// run's global start.  Rows are read 16 at a time so the loads overlap.
+constexpr int kScanRows = 128;
+constexpr int kScanBatch = 16;
+
+__global__ __launch_bounds__(256) void col_sum_kernel(const uint32_t* __restrict__ table, int Kp,
+                                                      uint32_t n_rows,
+                                                      uint32_t* __restrict__ csum) {
+    const uint32_t r0 = blockIdx.x * kScanRows, r1 = min(r0 + kScanRows, n_rows);
+    const int k4 = (int)threadIdx.x * 4;
+    if (k4 >= Kp)
+        return;
+    uint4 acc = make_uint4(0, 0, 0, 0);
+    for (uint32_t r = r0; r < r1; r += kScanBatch) {
+        uint4 v[kScanBatch];
+#pragma unroll
+        for (int i = 0; i < kScanBatch; ++i)
+            v[i] = r + i < r1 ? *reinterpret_cast<const uint4*>(table + (size_t)(r + i) * Kp + k4)
+                              : make_uint4(0, 0, 0, 0);
+#pragma unroll
+        for (int i = 0; i < kScanBatch; ++i) {
+            acc.x += v[i].x;
+            acc.y += v[i].y;
+            acc.z += v[i].z;
+            acc.w += v[i].w;
+        }
+    }
+    *reinterpret_cast<uint4*>(csum + (size_t)blockIdx.x * Kp + k4) = acc;
+}
+
+// One workgroup of 1024 threads: thread (seg, g) holds keys 4g .. 4g+3 and
+// a quarter of the chunks.  Each sums its quarter, the quarters' sums give
+// each thread its prefix and the key totals, a scan over the keys gives the
+// key starts, and each thread writes its chunks' starts.
+__global__ __launch_bounds__(1024) void col_base_kernel(uint32_t* __restrict__ csum, int Kp,
+                                                        uint32_t n_chunks) {
+    __shared__ uint4 qsum[4][256];
+    __shared__ uint32_t kbase[1024];
+    __shared__ uint32_t wsum[4];
+    const int g = threadIdx.x % 256, seg = threadIdx.x / 256;
+    const int k4 = 4 * g;
+    const uint32_t per = (n_chunks + 3) / 4;
+    const uint32_t c0 = min((uint32_t)seg * per, n_chunks), c1 = min(c0 + per, n_chunks);
+    uint4 tot = make_uint4(0, 0, 0, 0);
+    if (k4 < Kp)
+        for (uint32_t c = c0; c < c1; c += kScanBatch) {
+            uint4 v[kScanBatch];
+#pragma unroll
+            for (int i = 0; i < kScanBatch; ++i)
+                v[i] = c + i < c1 ? *reinterpret_cast<const uint4*>(csum + (size_t)(c + i) * Kp + k4)
+                                  : make_uint4(0, 0, 0, 0);
+#pragma unroll
+            for (int i = 0; i < kScanBatch; ++i) {
+                tot.x += v[i].x;
+                tot.y += v[i].y;
+                tot.z += v[i].z;
+                tot.w += v[i].w;
+            }
+        }
+    qsum[seg][g] = tot;
+    __syncthreads();
+    uint4 pre = make_uint4(0, 0, 0, 0), all = make_uint4(0, 0, 0, 0);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const uint4 v = qsum[q][g];
+        if (q < seg) {
+            pre.x += v.x;
+            pre.y += v.y;
+            pre.z += v.z;
+            pre.w += v.w;
+        }
+        all.x += v.x;
+        all.y += v.y;
+        all.z += v.z;
+        all.w += v.w;
+    }
+    // Exclusive scan of the key totals, 4 per thread, over the threads of
+    // the first quarter (g = tid for them; the others compute the same
+    // values and drop them).
+    const uint32_t t4 = all.x + all.y + all.z + all.w;
+    const int wave = g / kWave, lane = g % kWave;
+    uint32_t inc = t4;
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+        const uint32_t y = __shfl_up(inc, o, kWave);
+        if (lane >= o)
+            inc += y;
+    }
+    if (seg == 0 && lane == kWave - 1)
+        wsum[wave] = inc;
+    __syncthreads();
+    if (seg == 0) {
+        uint32_t b = inc - t4;
+        for (int w = 0; w < wave; ++w)
+            b += wsum[w];
+        kbase[k4] = b;
+        kbase[k4 + 1] = b + all.x;
+        kbase[k4 + 2] = b + all.x + all.y;
+        kbase[k4 + 3] = b + all.x + all.y + all.z;
+    }
+    __syncthreads();
+    if (k4 >= Kp)
+        return;
+    uint4 run = make_uint4(kbase[k4] + pre.x, kbase[k4 + 1] + pre.y, kbase[k4 + 2] + pre.z,
+                           kbase[k4 + 3] + pre.w);
+    for (uint32_t c = c0; c < c1; c += kScanBatch) {
+        uint4 v[kScanBatch];
+#pragma unroll
+        for (int i = 0; i < kScanBatch; ++i)
+            v[i] = c + i < c1 ? *reinterpret_cast<const uint4*>(csum + (size_t)(c + i) * Kp + k4)
+                              : make_uint4(0, 0, 0, 0);
+#pragma unroll
+        for (int i = 0; i < kScanBatch; ++i)
+            if (c + i < c1) {
+                *reinterpret_cast<uint4*>(csum + (size_t)(c + i) * Kp + k4) = run;
+                run.x += v[i].x;
+                run.y += v[i].y;
+                run.z += v[i].z;
+                run.w += v[i].w;
+            }
+    }
+}
+
+__global__ __launch_bounds__(256) void col_start_kernel(uint32_t* __restrict__ table, int Kp,
+                                                        uint32_t n_rows,
+                                                        const uint32_t* __restrict__ csum) {
+    const uint32_t r0 = blockIdx.x * kScanRows, r1 = min(r0 + kScanRows, n_rows);
+    const int k4 = (int)threadIdx.x * 4;
+    if (k4 >= Kp)
+        return;
+    uint4 run = *reinterpret_cast<const uint4*>(csum + (size_t)blockIdx.x * Kp + k4);
+    for (uint32_t r = r0; r < r1; r += kScanBatch) {
+        uint4 v[kScanBatch];
+#pragma unroll
+        for (int i = 0; i < kScanBatch; ++i)
+            v[i] = r + i < r1 ? *reinterpret_cast<const uint4*>(table + (size_t)(r + i) * Kp + k4)
+                              : make_uint4(0, 0, 0, 0);
+#pragma unroll
+        for (int i = 0; i < kScanBatch; ++i)
+            if (r + i < r1) {
+                *reinterpret_cast<uint4*>(table + (size_t)(r + i) * Kp + k4) = run;
+                run.x += v[i].x;
+                run.y += v[i].y;
+                run.z += v[i].z;
+                run.w += v[i].w;
+            }
+    }
+}
+
+typedef int v4i32 __attribute__((ext_vector_type(4)));
+// LDS layouts of a tile sort: data entry i at tile_pad(i) = i + i/32 where
+// threads read 32 consecutive entries each (one pad word per 32 puts the
+// threads of a wave on distinct banks), at i where they read in stripes;
+// digit counters as rows of T thread columns (row = digit) with one pad word
+// per 16 columns.
+__device__ inline uint32_t tile_pad(uint32_t i) { return i + (i >> 5); }
+template <int T> __device__ inline int cnt_at(int r, int c) { return r * (T + T / 16) + c + (c >> 4); }
+constexpr int kMaxDigitBits = 5;
+
+// One stable digit pass over a tile of E T entries held E per thread in
+// blocked order (thread t: entries t E .. t E + E - 1): entry j of thread t
+// goes to rank = (entries of smaller digit) + (entries of its digit in
+// threads below t) + (its digit among x[0 .. j-1]), stored at
+// lds[tile_pad(rank)] when PAD (the next pass reads blocked) and at
+// lds[rank] otherwise.  Each thread counts its digits in its own counter
+// column; a raking scan (thread q owns R consecutive counter words of one
+// row, the row's threads scan their totals with lane shuffles) and a scan
+// of the digit totals turn the counts into starts, which the thread then
+// advances for its entries.  Counter updates are LDS atomics on the
+// thread's own column: uncontended, completed in issue order, so a
+// thread's updates go out back to back and the returned values are its
+// ranks.  The counters share LDS with the data: starts and ends with a
+// barrier.
+template <int E, int T, int DB, bool PAD>
+__device__ inline void tile_rank_pass(const uint32_t (&x)[E], int shift, uint32_t* lds,
+                                      uint32_t* rowtot, uint32_t* dbase) {
+    static_assert(DB >= 1 && DB <= kMaxDigitBits, "digits of 1 to 5 bits");
+    // Counter rows: one per digit, at least T / 64 so that a row's raking
+    // threads (G of them) share a wave.
+    constexpr int R = (1 << DB) > T / kWave ? (1 << DB) : T / kWave;
+    constexpr int G = T / R;
+    static_assert(G >= 1 && G <= kWave && (G & (G - 1)) == 0, "a row's threads share a wave");
+    constexpr uint32_t dmask = (1u << DB) - 1u;
+    const int t = threadIdx.x;
+    __syncthreads();   // the data in lds may still be read
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+        lds[cnt_at<T>(r, t)] = 0;
+    // Each thread counts in its own column: no barrier before the counts.
+#pragma unroll
+    for (int j = 0; j < E; ++j)
+        atomicAdd(&lds[cnt_at<T>((int)((x[j] >> shift) & dmask), t)], 1u);
+    __syncthreads();
+    const int row = t / G, g = t % G, c0 = R * g;
+    uint32_t v[R];
+    uint32_t sum = 0;
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        const uint32_t c = lds[cnt_at<T>(row, c0 + k)];
+        v[k] = sum;
+        sum += c;
+    }
+    uint32_t inc = sum;
+#pragma unroll
+    for (int o = 1; o < G; o <<= 1) {
+        const uint32_t y = __shfl_up(inc, o, G);
+        if (g >= o)
+            inc += y;
+    }
+    const uint32_t ex = inc - sum;
+    if (g == G - 1)
+        rowtot[row] = inc;
+    __syncthreads();
+    if (t < kWave) {
+        // lane = digit: exclusive start over the digit totals
+        const uint32_t c = t <= (int)dmask ? rowtot[t] : 0u;
+        uint32_t x2 = c;
+#pragma unroll
+        for (int o = 1; o < kWave; o <<= 1) {
+            const uint32_t y = __shfl_up(x2, o, kWave);
+            if (t >= o)
+                x2 += y;
+        }
+        dbase[t] = x2 - c;
+    }
+    __syncthreads();
+    const uint32_t base = row <= (int)dmask ? dbase[row] : 0u;
+#pragma unroll
+    for (int k = 0; k < R; ++k)
+        lds[cnt_at<T>(row, c0 + k)] = v[k] + ex + base;
+    __syncthreads();
+    // A fresh copy of the shift: the digits are recomputed here rather than
+    // kept live (in registers) from the counting sweep.
+    int sh2 = shift;
+    asm volatile("" : "+s"(sh2));
+    uint32_t pos[E];
+#pragma unroll
+    for (int j = 0; j < E; ++j)
+        pos[j] = atomicAdd(&lds[cnt_at<T>((int)((x[j] >> sh2) & dmask), t)], 1u);
+    __syncthreads();   // every start read before the data overwrites the counters
+#pragma unroll
+    for (int j = 0; j < E; ++j)
+        lds[PAD ? tile_pad(pos[j]) : pos[j]] = x[j];
+}
+
+// The order of one tile of 2^TB masks from the scores s (KB-bit keys);
+// start is the scanned table: start[tile * Kp + k] is the global position
+// of the first mask of key k from this tile (Kp = K rounded up to 4).
+template <typename S, int TB, int KB>
+__global__ __launch_bounds__((1 << TB) / 32) void order_tile_kernel(
+    const S* __restrict__ s, uint64_t m, int K, const uint32_t* __restrict__ start,
+    uint32_t n_tiles, int32_t* __restrict__ order) {
+    constexpr int E = 32;
+    constexpr int T = (1 << TB) / E;
+    constexpr uint32_t kOff = (1u << TB) - 1u;
+    constexpr int D1 = (KB + 1) / 2, D2 = KB - D1;
+    constexpr int kRows = (1 << kMaxDigitBits) > T / kWave ? (1 << kMaxDigitBits) : T / kWave;
+    constexpr int kData = (1 << TB) + (1 << TB) / 32;
+    constexpr int kCnt = kRows * (T + T / 16);
+    __shared__ uint32_t lds[kData > kCnt ? kData : kCnt];
+    __shared__ uint32_t gst[1 << kTileKeyBits];   // per key: global start - local start
+    __shared__ uint32_t rowtot[kRows];
+    __shared__ uint32_t dbase[kWave];
+    const int tid = threadIdx.x;
+    const uint32_t tile = xcd_tile(blockIdx.x, n_tiles);
+    const uint64_t tile0 = (uint64_t)tile << TB;
+    // Offsets lo <= p < lim hold masks 1 .. m-1; a buffer resource over the
+    // tile keeps the offsets 32-bit.  Offsets past m count as score 0: they
+    // come last among the tile's score-0 entries and are not written, nor is
+    // mask 0 (score 0, the first entry of tile 0).
+    const uint32_t lim = (uint32_t)std::min<uint64_t>((uint64_t)1 << TB, m - tile0);
+    const uint32_t lo = tile0 == 0 ? 1u : 0u;
+    // The range check is per dword: the range is rounded up to whole dwords
+    // (the scores array has room for it) and entries at or past lim are
+    // replaced below.
+    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<S*>(s + tile0), 0, (int)((lim * sizeof(S) + 3) & ~3u), 0x00020000);
+    const bool whole = lo == 0 && lim == (1u << TB);
+    const uint32_t p0 = (uint32_t)tid * E;
+    uint32_t x[E];
+    if constexpr (sizeof(S) == 4) {
+#pragma unroll
+        for (int q = 0; q < E / 4; ++q) {
+            const v4i32 a = __builtin_amdgcn_raw_buffer_load_b128(rs, (int)((p0 + 4 * q) * 4), 0, 0);
+            x[4 * q] = (uint32_t)a[0];
+            x[4 * q + 1] = (uint32_t)a[1];
+            x[4 * q + 2] = (uint32_t)a[2];
+            x[4 * q + 3] = (uint32_t)a[3];
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < E / 8; ++q) {
+            const v4i32 a = __builtin_amdgcn_raw_buffer_load_b128(rs, (int)((p0 + 8 * q) * 2), 0, 0);
+#pragma unroll
+            for (int e = 0; e < 8; ++e)
+                x[8 * q + e] = ((uint32_t)a[e >> 1] >> ((e & 1) * 16)) & 0xffffu;
+        }
+    }
+    if (whole) {
+#pragma unroll
+        for (int j = 0; j < E; ++j)
+            x[j] = (x[j] << TB) | (p0 + (uint32_t)j);
+    } else {
+#pragma unroll
+        for (int j = 0; j < E; ++j) {
+            const uint32_t p = p0 + (uint32_t)j;
+            x[j] = ((p < lim ? x[j] : 0u) << TB) | p;
+        }
+    }
+    // Global starts of the keys (the loads overlap the first digit pass).
+    for (int k = tid; k < K; k += T)
+        gst[k] = start[(size_t)tile * ((K + 3) & ~3) + k];
+    if constexpr (D2 > 0) {
+        tile_rank_pass<E, T, D1, true>(x, TB, lds, rowtot, dbase);
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < E; ++j)
+            x[j] = lds[tile_pad(p0 + (uint32_t)j)];
+        tile_rank_pass<E, T, D2, false>(x, TB + D1, lds, rowtot, dbase);
+    } else {
+        tile_rank_pass<E, T, D1, false>(x, TB, lds, rowtot, dbase);
+    }
+    __syncthreads();
+    // The first entry of each key's run turns its global start into global -
+    // local; in tile 0, mask 0 heads the run of score 0.
+    uint32_t w[E];
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+        const uint32_t i = (uint32_t)tid + (uint32_t)T * q;
+        w[q] = lds[i];
+        const uint32_t k = w[q] >> TB;
+        if (i == 0 || (lds[i - 1] >> TB) != k)
+            gst[k] -= i + (k == 0 ? lo : 0u);
+    }
+    __syncthreads();
+    // Entry i of the sorted tile: consecutive lanes write consecutive
+    // positions of a run.
+    if (whole) {
+#pragma unroll
+        for (int q = 0; q < E; ++q) {
+            const uint32_t i = (uint32_t)tid + (uint32_t)T * q;
+            order[gst[w[q] >> TB] + i] = (int32_t)(tile0 + (w[q] & kOff));
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < E; ++q) {
+            const uint32_t i = (uint32_t)tid + (uint32_t)T * q;
+            const uint32_t p = w[q] & kOff;
+            if (p >= lo && p < lim)
+                order[gst[w[q] >> TB] + i] = (int32_t)(tile0 + p);
+        }
+    }
+}
+
 struct ScoresWorkspace {
-    DevBuf h, s, cnt, off, overflow, hist, order;
+    DevBuf h, s, cnt, off, overflow, hist, order, table, csum;
     DevBuf keys[2], vals[2];
     std::array<DevBuf, 6> levels;
 };
@@ -480,10 +947,58 @@ ScoresDev<S> scores_order_dev(const S* d_h, size_t m, int32_t n) {
                 dbits = d;
                 packed = true;
             }
-    S* d_s = ws.s.get<S>(m);
+    S* d_s = ws.s.get<S>(m + 2);   // two spare entries: whole-dword reads of the last tile
     uint32_t* d_cnt = ws.cnt.get<uint32_t>(buckets);
     uint32_t* d_off = ws.off.get<uint32_t>(buckets);
     uint32_t* d_ovf = ws.overflow.get<uint32_t>(1);
+    // Scores of at most 10 bits take the one-pass tile order.
+    const int kbits = n == 0 ? 1 : 32 - __builtin_clz((uint32_t)n);
+    if (N > 0 && kbits <= kTileKeyBits && m <= ((uint64_t)1 << 31)) {
+        constexpr int TB = kOrderTileBits;
+        const uint32_t tiles = (uint32_t)((m + ((uint64_t)1 << TB) - 1) >> TB);
+        uint32_t* d_tab = ws.table.get<uint32_t>(((buckets + 3) & ~(size_t)3) * tiles);
+        fill_u32(d_cnt, buckets, 0, nullptr);
+        fill_u32(d_ovf, 1, 0, nullptr);
+        {
+            ScopedKernelTimer timer(nullptr);   // scores, counts, per-tile key histograms
+            hipLaunchKernelGGL((scores_tile_kernel<S, TB>), dim3(std::min(tiles, kScoresGrid)),
+                               dim3(256), 2 * buckets * 4, nullptr, d_h, (uint64_t)m, n, d_s,
+                               d_cnt, d_ovf, tiles, d_tab);
+            BMM_HIP(hipGetLastError());
+        }
+        uint32_t ovf = 0;
+        BMM_HIP(hipMemcpy(&ovf, d_ovf, 4, hipMemcpyDeviceToHost));
+        if (ovf)
+            fail(BMM_ERR_OVERFLOW, "calc_scores_and_order: a score exceeds the int16 range");
+        int32_t* d_order = ws.order.get<int32_t>((size_t)N);
+        {
+            ScopedKernelTimer timer(nullptr);   // scan of the table and the tile sorts
+            const uint32_t chunks = (tiles + kScanRows - 1) / kScanRows;
+            const int Kp = (int)((buckets + 3) & ~(size_t)3);
+            uint32_t* d_cs = ws.csum.get<uint32_t>((size_t)chunks * Kp);
+            hipLaunchKernelGGL(col_sum_kernel, dim3(chunks), dim3(256), 0, nullptr, d_tab, Kp,
+                               tiles, d_cs);
+            hipLaunchKernelGGL(col_base_kernel, dim3(1), dim3(1024), 0, nullptr, d_cs, Kp, chunks);
+            hipLaunchKernelGGL(col_start_kernel, dim3(chunks), dim3(256), 0, nullptr, d_tab, Kp,
+                               tiles, d_cs);
+            BMM_HIP(hipGetLastError());
+            switch (kbits) {
+#define BMM_KB(KB)                                                                             \
+    case KB:                                                                                   \
+        hipLaunchKernelGGL((order_tile_kernel<S, TB, KB>), dim3(tiles), dim3((1 << TB) / 32),  \
+                           0, nullptr, d_s, (uint64_t)m, (int)buckets, d_tab, tiles, d_order); \
+        break;
+                BMM_KB(1) BMM_KB(2) BMM_KB(3) BMM_KB(4) BMM_KB(5) BMM_KB(6) BMM_KB(7) BMM_KB(8)
+                BMM_KB(9) BMM_KB(10)
+#undef BMM_KB
+            default: fail(BMM_ERR_INVALID, "calc_scores_and_order: bad key width");
+            }
+            BMM_HIP(hipGetLastError());
+        }
+        BMM_HIP(hipMemcpy(d_off, d_cnt, buckets * 4, hipMemcpyDeviceToDevice));
+        scan_u32(d_off, buckets, true, ws.levels, 0, nullptr);
+        return ScoresDev<S>{d_s, d_order, d_cnt, d_off};
+    }
     uint32_t* d_hist = ws.hist.get<uint32_t>((size_t)256 * std::max(tiles_u, tiles_p));
     fill_u32(d_cnt, buckets, 0, nullptr);
     fill_u32(d_ovf, 1, 0, nullptr);

@@ -53,8 +53,14 @@ def test_scores_order_golden(lib, gglobal):
 # 6 bits so that 28 + 4 bits fill the word (its top bit set for scores of
 # 512 and above); (1 << 22) + 5 masks with n = 65535 pack 23 + 8 bits, and
 # (1 << 25) + 3 masks cannot (26 + 8 bits): separate keys and values.
+# Scores of at most 10 bits (n <= 1023) take the one-pass order over tiles
+# of 2^15 masks: one tile (m = 2, 3, 5000), partial last tiles, the largest
+# key range (n = 1023, also in int16), and the first sizes past it (n = 1024,
+# 2046) on the radix passes.
 @pytest.mark.parametrize("dt,m,n", [
     (np.int32, 1, 5), (np.int32, 2, 0), (np.int32, 4097, 0), (np.int32, 65539, 255),
+    (np.int32, 3, 7), (np.int32, 3 * 32768 + 17, 1023), (np.int16, 40000, 1023),
+    (np.int32, 3 * 32768 + 17, 2046), (np.int16, 40000, 1500), (np.int32, 32768, 1024),
     (np.int32, 70001, 300), (np.int32, (1 << 20) + 3, 1023), (np.int16, 5000, 1),
     (np.int32, 65536, 256), (np.int32, 1 << 20, 16383), (np.int32, 1 << 20, 16384),
     (np.int32, (1 << 22) + 5, 65535), (np.int32, 1 << 20, 65536),
