@@ -641,19 +641,27 @@ def _window_of(m: np.ndarray, cols: int):
 
 
 TOPK_L = 13   # fwht_topk_kernel: blocks of 2^13 masks, int32 transform in LDS
+TOPK_PK_L = 14   # fwht_topk_pk_kernel: blocks of 2^14 masks, packed int16
+
+
+def topk_packed(t: int, cols: int) -> bool:
+    """Whether the top-k engine runs a t-row window over `cols` columns as
+    packed-int16 blocks of 2^14 masks (at least 256 blocks, at most 32767
+    active columns) rather than int32 blocks of 2^13."""
+    return t >= 22 and cols <= 32767
 
 
 def run_search(torch, steps: int, warmup: int) -> dict:
     """The reference-API top-k searchers (FwhtSearch engine: persistent
-    workgroups over blocks of 2^13 masks, int32 transform in LDS, exact
-    per-workgroup top-K, one merge).  Cases: MitmFwhtSearch and
-    CudaMitmFwhtSearch on 28 x 96 Bernoulli(0.5) with k = 128 (2^28 masks),
-    MitmFwhtSearch with k = 64 on rows 0..39 of the planted 48 x 4096
-    instance (2^40 masks; the value), BruteforceSearch and
-    CudaBruteforceSearch on 16 x 4096 Bernoulli(0.5) with k = 128.  Per
-    case: host call time and kernel time (events around fwht_topk_kernel);
-    roofline: 13 int32 add/sub per mask (the block transform) against the
-    int32 VALU peak."""
+    workgroups over blocks of masks, exact per-workgroup top-K, a tree of
+    merges).  Cases: MitmFwhtSearch and CudaMitmFwhtSearch on 28 x 96
+    Bernoulli(0.5) with k = 128 (2^28 masks), MitmFwhtSearch with k = 64 on
+    rows 0..39 of the planted 48 x 4096 instance (2^40 masks; the value),
+    BruteforceSearch and CudaBruteforceSearch on 16 x 4096 Bernoulli(0.5)
+    with k = 128.  Per case: host call time and kernel time (events around
+    the block kernel); roofline: the block transform's add/sub per mask, 14
+    against the packed-int16 peak for windows run as packed blocks of 2^14,
+    min(t, 13) against the int32 peak otherwise."""
     import bmmpy as bmm
     from bmmpy_amd import _lib
 
@@ -683,24 +691,28 @@ def run_search(torch, steps: int, warmup: int) -> dict:
         t = m.shape[0]
         masks = float((1 << t) - 1)
         kernel_ms = kms / max(nk, 1)
-        ops = float(1 << t) * min(t, TOPK_L)
+        packed = topk_packed(t, cols)
+        ops = float(1 << t) * (TOPK_PK_L if packed else min(t, TOPK_L))
+        peak = VALU_PK16_PEAK_TOPS if packed else VALU_PEAK_TOPS
         print(f"search {name}: {1e3 * dt:.2f} ms per call, kernel {kernel_ms:.3f} ms",
               file=sys.stderr, flush=True)
         out[name] = {"rows": t, "cols": cols, "k": len(res), "masks": masks,
                      "call_ms": 1e3 * dt, "masks_per_s": masks / dt,
                      "kernel_avg_ms": kernel_ms, "kernel_launches": nk,
                      "kernel_masks_per_s": masks / (kernel_ms / 1e3),
-                     "roofline_frac": ops / (kernel_ms / 1e3) / 1e12 / VALU_PEAK_TOPS,
+                     "kernel": "fwht_topk_pk_kernel" if packed else "fwht_topk_kernel<13>",
+                     "roofline_peak_tops": peak,
+                     "roofline_frac": ops / (kernel_ms / 1e3) / 1e12 / peak,
                      "best": [int(res[0].weight), int(res[0].mask_u64())] if res else None}
     main = out["mitm_planted_40x4096_k64"]
     return {
         "ms": main["call_ms"], "evals": main["masks"], "results": [out],
         "config": {"workload": "topk_searchers", "cases": list(out),
                    "subset_evals_per_step": main["masks"]},
-        "roofline": {"bound": "valu", "achieved": main["roofline_frac"] * VALU_PEAK_TOPS,
-                     "peak": VALU_PEAK_TOPS, "unit": "TOPS (int32 VALU)",
+        "roofline": {"bound": "valu", "achieved": main["roofline_frac"] * VALU_PK16_PEAK_TOPS,
+                     "peak": VALU_PK16_PEAK_TOPS, "unit": "TOPS (packed int16 VALU)",
                      "frac": main["roofline_frac"], "traffic": None,
-                     "ops_per_eval": TOPK_L, "kernel": "fwht_topk_kernel<13, 2048>",
+                     "ops_per_eval": TOPK_PK_L, "kernel": "fwht_topk_pk_kernel<2048, 4>",
                      "kernel_avg_ms": main["kernel_avg_ms"]},
     }
 

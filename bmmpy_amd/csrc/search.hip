@@ -135,6 +135,13 @@ __device__ inline void pk_build_mem64(uint32_t* s, const uint64_t* __restrict__ 
     }
 }
 
+// One packed block of 2^14 masks: every stage but those on word bits 5-9
+// runs in LDS; each thread then holds one group of 32 words over bits 5-9,
+// whose outputs are signed sums x_0 +- x_1 ... +- x_31, so
+// x_0 + sum_{e >= 1} |x_e| bounds them (at most the active column count: no
+// int16 overflow).  A thread whose bound is below hmin has no candidate and
+// stops; the others finish the five stages in registers and take the entries
+// with H >= hmin.  Returns whether any thread had more than kPkCand.
 template <typename Build>
 __device__ inline bool pk_topk_block(uint32_t* s, Build&& build, uint64_t mh, int t, int32_t total,
                                      int32_t hmin, uint64_t (&cand)[kPkCand], int& nc) {
@@ -161,18 +168,40 @@ __device__ inline bool pk_topk_block(uint32_t* s, Build&& build, uint64_t mh, in
             }
         }
     };
-    pk_block_fwht_visit_paired(s, [&](int q, const uint32_t (&v)[8]) {
-        const uint32_t w0 = 2u * threadIdx.x + (uint32_t)(q & 1) + 512u * (uint32_t)(q >> 1);
+    __syncthreads();
+    pk_round<0, 5, true>(s);
+    __syncthreads();
+    pk_round_top(s);
+    __syncthreads();
+    {
+        // Thread t's group over word bits 5-9: words (t >> 5) << 10 | e << 5
+        // | (t & 31), e < 32.
+        constexpr int ST = pk_pad_stride<5>();
+        const uint32_t w0 = ((threadIdx.x >> 5) << 10) | (threadIdx.x & 31u);
+        const uint32_t* p = s + pk_pad((int)w0);
+        uint32_t v[32];
 #pragma unroll
-        for (int e = 0; e < 8; ++e) {
+        for (int e = 0; e < 32; ++e)
+            v[e] = p[e * ST];
+        s16x2 bound = pk_as(v[0]);
+#pragma unroll
+        for (int e = 1; e < 32; ++e) {
             const s16x2 x = pk_as(v[e]);
-            if (max(pk_lo(x), pk_hi(x)) >= hmin) {
-                const uint32_t w = w0 + 1024u * e;
-                take(pk_lo(x), w);
-                take(pk_hi(x), w + (1u << kPkW));
+            bound = bound + pk_max(x, -x);
+        }
+        if (max(pk_lo(bound), pk_hi(bound)) >= hmin) {
+            pk_butterflies(v);
+#pragma unroll
+            for (int e = 0; e < 32; ++e) {
+                const s16x2 x = pk_as(v[e]);
+                if (max(pk_lo(x), pk_hi(x)) >= hmin) {
+                    const uint32_t w = w0 | ((uint32_t)e << 5);
+                    take(pk_lo(x), w);
+                    take(pk_hi(x), w + (1u << kPkW));
+                }
             }
         }
-    });
+    }
     return __syncthreads_or(over) != 0;
 }
 
@@ -186,8 +215,13 @@ constexpr int kPkChunk = 16;
 // R > 0: each thread holds patterns tid + 256 k (k < R) in registers across
 // its blocks (windows of at most 46 rows and 256 R patterns); R = 0 reads
 // them from memory for every block.
+// Waves per SIMD the packed top-k kernel is compiled for: four with a
+// 256-key list workspace (39 KiB of LDS: 4 workgroups per CU), three
+// otherwise.
+template <int CAP> constexpr int kTopkPkWaves = CAP <= 256 ? 4 : 3;
+
 template <int CAP, int R>
-__global__ __launch_bounds__(256, 3) void fwht_topk_pk_kernel(const uint64_t* __restrict__ pat,
+__global__ __launch_bounds__(256, kTopkPkWaves<CAP>) void fwht_topk_pk_kernel(const uint64_t* __restrict__ pat,
                                                            const int32_t* __restrict__ mult,
                                                            int n_pat, int t, int32_t total,
                                                            uint64_t n_blocks, int K, int chunk,
@@ -256,17 +290,20 @@ __global__ __launch_bounds__(256, 3) void fwht_topk_pk_kernel(const uint64_t* __
             const int32_t hmin2 = wthr2 >= (int64_t)total ? INT32_MIN
                                                          : (int32_t)((int64_t)total - 2 * wthr2);
             const int32_t* sp = s + fwht_pad((int)threadIdx.x);
-            uint32_t valid = 0;
+            // Offered 8 entries per thread at a time (fewer live registers).
+            for (int c = 0; c < E; c += 8) {
+                uint32_t valid = 0;
 #pragma unroll
-            for (int e = 0; e < E; ++e) {
-                const int ml = e * 256 + (int)threadIdx.x;
-                if ((mh != 0 || ml != 0) && sp[e * 264] >= hmin2)
-                    valid |= 1u << e;
+                for (int e = 0; e < 8; ++e) {
+                    const int ml = (c + e) * 256 + (int)threadIdx.x;
+                    if ((mh != 0 || ml != 0) && sp[(c + e) * 264] >= hmin2)
+                        valid |= 1u << e;
+                }
+                topk_offer_fn<CAP, 8>(T, valid, K, [&](int e) {
+                    const uint64_t w = (uint64_t)(uint32_t)(total - sp[(c + e) * 264]) >> 1;
+                    return (w << t) | (mh << L) | (uint64_t)((c + e) * 256 + (int)threadIdx.x);
+                });
             }
-            topk_offer_fn<CAP, E>(T, valid, K, [&](int e) {
-                const uint64_t w = (uint64_t)(uint32_t)(total - sp[e * 264]) >> 1;
-                return (w << t) | (mh << L) | (uint64_t)(e * 256 + (int)threadIdx.x);
-            });
         }
     }
     topk_store(T, lists + (size_t)blockIdx.x * K, K);
@@ -386,13 +423,20 @@ std::vector<bmm_topk_entry> fwht_topk_engine(const uint64_t* d_pat, const int32_
         const int grid14 = (int)std::max<uint64_t>(1, std::min<uint64_t>(n14, (uint64_t)grid));
         {
             ScopedKernelTimer timer(stream);
-            const bool regs = t <= 46;
-            auto kern = K <= 512 ? (regs && n_pat <= 256 * 4    ? fwht_topk_pk_kernel<2048, 4>
-                                    : regs && n_pat <= 256 * 16 ? fwht_topk_pk_kernel<2048, 16>
-                                                                : fwht_topk_pk_kernel<2048, 0>)
-                                 : (regs && n_pat <= 256 * 4    ? fwht_topk_pk_kernel<8192, 4>
-                                    : regs && n_pat <= 256 * 16 ? fwht_topk_pk_kernel<8192, 16>
-                                                                : fwht_topk_pk_kernel<8192, 0>);
+            // Patterns held in registers: 4 or 16 per thread (windows of at
+            // most 46 rows), else read from memory per block.
+            const int R = t > 46 ? 0 : n_pat <= 256 * 4 ? 4 : n_pat <= 256 * 16 ? 16 : 0;
+            // Lists of up to 64 keys fit a 256-key workspace: four
+            // workgroups per CU instead of three (not with 16 registers of
+            // patterns, which would spill at that occupancy).
+            auto kern = K <= 64 && R != 16 ? (R == 4 ? fwht_topk_pk_kernel<256, 4>
+                                                     : fwht_topk_pk_kernel<256, 0>)
+                        : K <= 512 ? (R == 4    ? fwht_topk_pk_kernel<2048, 4>
+                                      : R == 16 ? fwht_topk_pk_kernel<2048, 16>
+                                                : fwht_topk_pk_kernel<2048, 0>)
+                                   : (R == 4    ? fwht_topk_pk_kernel<8192, 4>
+                                      : R == 16 ? fwht_topk_pk_kernel<8192, 16>
+                                                : fwht_topk_pk_kernel<8192, 0>);
             const int chunk = (int)std::min<uint64_t>(
                 kPkChunk, std::max<uint64_t>(1, n14 / ((uint64_t)grid14 * 16)));
             hipLaunchKernelGGL(kern, dim3(grid14), dim3(256), 0, stream, d_pat, d_mult, n_pat, t,
