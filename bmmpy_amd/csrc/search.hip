@@ -226,6 +226,7 @@ __global__ __launch_bounds__(256, kTopkPkWaves<CAP>) void fwht_topk_pk_kernel(co
                                                            int n_pat, int t, int32_t total,
                                                            uint64_t n_blocks, int K, int chunk,
                                                            unsigned long long* __restrict__ counter,
+                                                           const unsigned long long* __restrict__ bound,
                                                            uint64_t* __restrict__ lists) {
     constexpr int L = 13;
     constexpr int kWords = pk_lds_words() > fwht_lds_words(L) ? pk_lds_words() : fwht_lds_words(L);
@@ -243,7 +244,7 @@ __global__ __launch_bounds__(256, kTopkPkWaves<CAP>) void fwht_topk_pk_kernel(co
         if (R > 0 && i < n_pat)
             pk_prep_entry(pat[i], mult[i], hi[k], av[k]);
     }
-    topk_init(T);
+    topk_init(T, bound ? *bound : ~0ull);
     uint64_t next = 0, stop = 0;
     for (;;) {
         if (next == stop) {
@@ -309,6 +310,179 @@ __global__ __launch_bounds__(256, kTopkPkWaves<CAP>) void fwht_topk_pk_kernel(co
     topk_store(T, lists + (size_t)blockIdx.x * K, K);
 }
 
+// A first pass for searches whose workgroups see few blocks each (a window
+// of 22-30 rows over a persistent grid): every workgroup's list would start
+// empty and its first blocks would all be crowded.  Instead, the sampled
+// blocks 0, stride, 2 stride, ... are transformed (one per workgroup) and
+// their masks counted by weight bin (bin = weight >> shift, so the bins span
+// [0, total]); from the summed counts, with b the first bin at which the
+// running count reaches K, at least K masks have weight below (b + 1) <<
+// shift, so every key of the exact top-K is below ((b + 1) << shift) << t.
+// A workgroup counts only the kBoundSpan bins from its block's lightest mask
+// up (8-bit counters packed in one register per thread, summed per
+// wavefront): fewer counts only weaken the bound, never invalidate it.  Each
+// workgroup stores its counts in its own slot (same-address device atomics
+// from many workgroups serialize at memory), and topk_bound_finish_kernel
+// sums the slots.
+constexpr int kBoundBins = 1024;
+constexpr int kBoundSpan = 8;
+constexpr int kBoundSample = 1024;   // sampled blocks of 2^14 masks at most
+
+struct TopkBoundSlot {
+    int32_t b0;                    // bin of the block's lightest mask
+    uint32_t count[kBoundSpan];    // masks in bins b0 .. b0 + kBoundSpan - 1
+};
+
+struct TopkBoundState {   // header zeroed before every search
+    unsigned long long counter;   // work queue of the main pass
+    unsigned long long bound;     // written by topk_bound_finish_kernel
+    TopkBoundSlot slot[kBoundSample];
+};
+
+__global__ __launch_bounds__(256) void topk_bound_pk_kernel(const uint64_t* __restrict__ pat,
+                                                            const int32_t* __restrict__ mult,
+                                                            int n_pat, int32_t total,
+                                                            uint64_t stride, int shift,
+                                                            TopkBoundState* __restrict__ st) {
+    __shared__ __align__(16) uint32_t s[pk_lds_words() + 4];
+    __shared__ uint32_t lh[4][kBoundSpan];
+    __shared__ int32_t red[4];
+    const int tid = (int)threadIdx.x;
+    const uint64_t mh14 = (uint64_t)blockIdx.x * stride;
+    int4* s4 = reinterpret_cast<int4*>(s);
+    const int b = (int)kPkBias;
+    for (int i = tid; i < (pk_lds_words() + 3) / 4; i += blockDim.x)
+        s4[i] = make_int4(b, b, b, b);
+    __syncthreads();
+    pk_build_mem64(s, pat, mult, n_pat, mh14);
+    __syncthreads();
+    pk_round<0, 5, true>(s);
+    __syncthreads();
+    pk_round_top(s);
+    __syncthreads();
+    // Thread t's group over word bits 5-9, finished in registers: entries
+    // w and w + 2^13 for w = (t >> 5) << 10 | e << 5 | (t & 31).
+    uint32_t v[32];
+    pk_mid_group_visit(s, [&](const uint32_t(&x)[32]) {
+#pragma unroll
+        for (int e = 0; e < 32; ++e)
+            v[e] = x[e];
+    });
+    pk_butterflies(v);
+    const bool skip0 = mh14 == 0 && tid == 0;   // mask 0 (word 0, low half) is never a result
+    int32_t hmax = INT32_MIN;
+#pragma unroll
+    for (int e = 0; e < 32; ++e) {
+        const s16x2 x = pk_as(v[e]);
+        if (!(skip0 && e == 0))
+            hmax = max(hmax, pk_lo(x));
+        hmax = max(hmax, pk_hi(x));
+    }
+    hmax = wave_max_i32(hmax);
+    if (tid % kWave == 0)
+        red[tid / kWave] = hmax;
+    __syncthreads();
+    hmax = max(max(red[0], red[1]), max(red[2], red[3]));
+    const int32_t b0 = ((total - hmax) >> 1) >> shift;   // bin of the block's lightest mask
+    // Counter d (bits 8d..8d+7) counts the thread's masks in bin b0 + d; at
+    // most 64 masks per thread, so no counter overflows.
+    uint64_t acc = 0;
+    auto count = [&](int32_t h) {
+        const int32_t d = (((total - h) >> 1) >> shift) - b0;
+        if (d < kBoundSpan)
+            acc += 1ull << (8 * d);
+    };
+#pragma unroll
+    for (int e = 0; e < 32; ++e) {
+        const s16x2 x = pk_as(v[e]);
+        if (!(skip0 && e == 0))
+            count(pk_lo(x));
+        count(pk_hi(x));
+    }
+#pragma unroll
+    for (int d = 0; d < kBoundSpan; ++d) {
+        const uint32_t c = wave_sum_u32((uint32_t)(acc >> (8 * d)) & 0xffu);
+        if (tid % kWave == 0)
+            lh[tid / kWave][d] = c;
+    }
+    __syncthreads();
+    TopkBoundSlot& slot = st->slot[blockIdx.x];
+    if (tid == 0)
+        slot.b0 = b0;
+    if (tid < kBoundSpan)
+        slot.count[tid] = lh[0][tid] + lh[1][tid] + lh[2][tid] + lh[3][tid];
+}
+
+// One workgroup: sums the n_slots slots into bins and writes the key bound
+// (~0 when fewer than K masks were counted).
+__global__ __launch_bounds__(256) void topk_bound_finish_kernel(int n_slots, int shift, int t,
+                                                                int K,
+                                                                TopkBoundState* __restrict__ st) {
+    __shared__ uint32_t hist[kBoundBins];
+    __shared__ uint32_t scan[256];
+    __shared__ int32_t first_bin;
+    const int tid = (int)threadIdx.x;
+    for (int i = tid; i < kBoundBins; i += blockDim.x)
+        hist[i] = 0u;
+    if (tid == 0)
+        first_bin = INT32_MAX;
+    __syncthreads();
+    // Thread t takes slots t + 256 j (j < 4): all loads in flight at once.
+    constexpr int SPT = kBoundSample / 256;
+    int32_t b0[SPT];
+    uint32_t cnt[SPT][kBoundSpan];
+#pragma unroll
+    for (int j = 0; j < SPT; ++j) {
+        const int i = tid + 256 * j;
+        b0[j] = kBoundBins;   // nothing counted
+        if (i < n_slots) {
+            b0[j] = st->slot[i].b0;
+#pragma unroll
+            for (int d = 0; d < kBoundSpan; ++d)
+                cnt[j][d] = st->slot[i].count[d];
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < SPT; ++j)
+#pragma unroll
+        for (int d = 0; d < kBoundSpan; ++d)
+            if (b0[j] + d < kBoundBins && cnt[j][d] != 0u)
+                atomicAdd(&hist[b0[j] + d], cnt[j][d]);
+    __syncthreads();
+    // Running counts, four bins per thread.
+    constexpr int PER = kBoundBins / 256;
+    uint32_t c[PER], sum = 0;
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+        c[i] = hist[tid * PER + i];
+        sum += c[i];
+    }
+    scan[tid] = sum;
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {   // inclusive scan of the thread sums
+        const uint32_t add = tid >= off ? scan[tid - off] : 0u;
+        __syncthreads();
+        scan[tid] += add;
+        __syncthreads();
+    }
+    uint32_t run = scan[tid] - sum;   // count before this thread's bins
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+        run += c[i];
+        if (run >= (uint32_t)K) {
+            atomicMin(&first_bin, tid * PER + i);
+            break;
+        }
+    }
+    __syncthreads();
+    if (tid == 0) {
+        const int32_t bin = first_bin;
+        st->bound = bin == INT32_MAX
+                        ? ~0ull
+                        : (unsigned long long)((uint64_t)(bin + 1) << shift) << t;
+    }
+}
+
 // u64 column patterns of a t-row window (t <= 64) over `cols` columns.
 __global__ void column_patterns64_kernel(const uint64_t* __restrict__ rows, int t, int words,
                                          int cols, uint64_t* __restrict__ pat,
@@ -334,19 +508,31 @@ __global__ void pattern_from_split_kernel(const uint64_t* __restrict__ q,
 
 namespace {
 struct SearchWorkspace {
-    DevBuf rows, pat, mult, q, r, counter, lists, lists2, out;
+    DevBuf rows, pat, mult, q, r, counter, lists, lists2, out, bound;
 };
 
-// Folds n_lists lists of K keys into one by a tree of parallel merges
-// (fan-in kMergeFan); returns the buffer holding the final list.
+// Folds n_lists lists of K keys into one by a tree of parallel merges;
+// returns the buffer holding the final list.
 constexpr int kMergeFan = 16;
 
 template <int CAP>
 uint64_t* merge_lists(uint64_t* a, uint64_t* b, int n_lists, int K, hipStream_t s) {
+    // Groups of at most kMergeSortedKeys keys (at a power-of-two list stride)
+    // merge by pairwise rounds, up to 64 lists per group; longer lists by
+    // offers to a top-K list.
+    int lk = 0;
+    while ((1 << lk) < K)
+        ++lk;
+    const int sorted_fan = std::min(64, kMergeSortedKeys >> lk);
     while (n_lists > 1) {
-        const int groups = (n_lists + kMergeFan - 1) / kMergeFan;
-        hipLaunchKernelGGL(topk_merge_group_kernel<CAP>, dim3(groups), dim3(256), 0, s, a,
-                           n_lists, kMergeFan, K, b);
+        const int fan = sorted_fan >= 2 ? sorted_fan : kMergeFan;
+        const int groups = (n_lists + fan - 1) / fan;
+        if (sorted_fan >= 2)
+            hipLaunchKernelGGL(topk_merge_sorted_kernel, dim3(groups), dim3(kMergeSortedThreads),
+                               0, s, a, n_lists, fan, K, lk, b);
+        else
+            hipLaunchKernelGGL(topk_merge_group_kernel<CAP>, dim3(groups), dim3(256), 0, s, a,
+                               n_lists, fan, K, b);
         BMM_HIP(hipGetLastError());
         std::swap(a, b);
         n_lists = groups;
@@ -421,30 +607,62 @@ std::vector<bmm_topk_entry> fwht_topk_engine(const uint64_t* d_pat, const int32_
         // keep the int32 blocks of 2^13: twice the workgroups.)
         const uint64_t n14 = n_masks >> kPkL;
         const int grid14 = (int)std::max<uint64_t>(1, std::min<uint64_t>(n14, (uint64_t)grid));
+        // Patterns held in registers: 4 or 16 per thread (windows of at
+        // most 46 rows), else read from memory per block.
+        const int R = t > 46 ? 0 : n_pat <= 256 * 4 ? 4 : n_pat <= 256 * 16 ? 16 : 0;
+        // Few blocks per workgroup: a sampled first pass bounds the keys.
+        const unsigned long long* d_bound = nullptr;
+        unsigned long long* d_queue = d_counter;
+        int grid_pk = grid14;
+        if (n14 <= (uint64_t)grid14 * 64) {
+            TopkBoundState* st = ws.bound.get<TopkBoundState>(1);
+            // Only the header needs clearing: every slot read is written.
+            fill_u64(reinterpret_cast<uint64_t*>(st), 2, 0, stream);
+            const uint64_t sample = std::min<uint64_t>(n14, kBoundSample);
+            int shift = 0;
+            while ((total >> shift) >= kBoundBins)
+                ++shift;
+            ScopedKernelTimer timer(stream);
+            hipLaunchKernelGGL(topk_bound_pk_kernel, dim3((unsigned)sample), dim3(256), 0, stream,
+                               d_pat, d_mult, n_pat, (int32_t)total, n14 / sample, shift, st);
+            BMM_HIP(hipGetLastError());
+            hipLaunchKernelGGL(topk_bound_finish_kernel, dim3(1), dim3(256), 0, stream,
+                               (int)sample, shift, t, K, st);
+            BMM_HIP(hipGetLastError());
+            d_bound = &st->bound;
+            d_queue = &st->counter;
+            // Bounded lists need no warm-up: one workgroup per resident slot
+            // (fewer lists to merge).
+            const int per_cu = K <= 128 && R != 16 ? 4 : K <= 512 ? 3 : 1;
+            grid_pk = (int)std::min<uint64_t>(
+                (uint64_t)grid14, (uint64_t)(default_persistent_wgs() / 8 * per_cu));
+        }
         {
             ScopedKernelTimer timer(stream);
-            // Patterns held in registers: 4 or 16 per thread (windows of at
-            // most 46 rows), else read from memory per block.
-            const int R = t > 46 ? 0 : n_pat <= 256 * 4 ? 4 : n_pat <= 256 * 16 ? 16 : 0;
-            // Lists of up to 64 keys fit a 256-key workspace: four
+            // Lists of up to 128 keys use a 256-key workspace (at least
+            // 128 staging slots; larger offers take several rounds): four
             // workgroups per CU instead of three (not with 16 registers of
             // patterns, which would spill at that occupancy).
-            auto kern = K <= 64 && R != 16 ? (R == 4 ? fwht_topk_pk_kernel<256, 4>
-                                                     : fwht_topk_pk_kernel<256, 0>)
+            auto kern = K <= 128 && R != 16 ? (R == 4 ? fwht_topk_pk_kernel<256, 4>
+                                                      : fwht_topk_pk_kernel<256, 0>)
                         : K <= 512 ? (R == 4    ? fwht_topk_pk_kernel<2048, 4>
                                       : R == 16 ? fwht_topk_pk_kernel<2048, 16>
                                                 : fwht_topk_pk_kernel<2048, 0>)
                                    : (R == 4    ? fwht_topk_pk_kernel<8192, 4>
                                       : R == 16 ? fwht_topk_pk_kernel<8192, 16>
                                                 : fwht_topk_pk_kernel<8192, 0>);
-            const int chunk = (int)std::min<uint64_t>(
-                kPkChunk, std::max<uint64_t>(1, n14 / ((uint64_t)grid14 * 16)));
-            hipLaunchKernelGGL(kern, dim3(grid14), dim3(256), 0, stream, d_pat, d_mult, n_pat, t,
-                               (int32_t)total, n14, K, chunk, d_counter, d_lists);
+            // Bounded lists: one contiguous run of blocks per workgroup (one
+            // queue atomic each); otherwise up to kPkChunk blocks per atomic.
+            const int chunk =
+                d_bound ? (int)((n14 + (uint64_t)grid_pk - 1) / (uint64_t)grid_pk)
+                        : (int)std::min<uint64_t>(
+                              kPkChunk, std::max<uint64_t>(1, n14 / ((uint64_t)grid_pk * 16)));
+            hipLaunchKernelGGL(kern, dim3(grid_pk), dim3(256), 0, stream, d_pat, d_mult, n_pat, t,
+                               (int32_t)total, n14, K, chunk, d_queue, d_bound, d_lists);
             BMM_HIP(hipGetLastError());
         }
-        d_out = K <= 512 ? merge_lists<2048>(d_lists, d_lists2, grid14, K, stream)
-                         : merge_lists<8192>(d_lists, d_lists2, grid14, K, stream);
+        d_out = K <= 512 ? merge_lists<2048>(d_lists, d_lists2, grid_pk, K, stream)
+                         : merge_lists<8192>(d_lists, d_lists2, grid_pk, K, stream);
     } else if (K <= 512) {
         dispatch_fwht_topk<2048>(L, d_pat, d_mult, n_pat, t, (int32_t)total, n_blocks, K,
                                  d_counter, d_lists, grid, stream);

@@ -156,3 +156,35 @@ def test_large_k_limits_and_searcher_api(lib):
     assert [(x.weight, x.mask_u64()) for x in a[:4096]] == \
         [(x.weight, x.mask_u64()) for x in bmm.FwhtSearch(max_rows=18,
                                                             max_candidates=4096).search(win)]
+
+
+def _sampled_bound_window(lib, case):
+    """Windows of 22-28 rows, where the packed search first bounds the keys
+    from a sample of blocks (each workgroup sees only a few blocks)."""
+    if case == "random24":
+        return lib.gen_bernoulli(0x5EED0021, 24, 200, 0.3), 200
+    if case == "dup_rows23":
+        # Ten distinct rows and thirteen copies of them: most masks tie.
+        base = lib.gen_bernoulli(0x5EED0022, 10, 160, 0.4)
+        return np.concatenate([base, base[np.arange(13) % 10]]), 160
+    if case == "few_cols22":
+        # Five active columns: 2^17 - 1 masks of weight 0, the list fills
+        # with them and every crowded block is rescanned exactly.
+        m = lib.gen_bernoulli(0x5EED0023, 22, 5, 0.5)
+        return m, 5
+    if case == "sparse28":
+        return lib.gen_bernoulli(0x5EED0024, 28, 700, 0.05), 700
+    raise ValueError(case)
+
+
+@pytest.mark.parametrize("case,k", [("random24", 1), ("random24", 1000), ("dup_rows23", 300),
+                                    ("few_cols22", 128), ("few_cols22", 4096),
+                                    ("sparse28", 64), ("sparse28", 513)])
+def test_sampled_bound_windows(lib, ora, case, k):
+    """The sampled key bound never drops a key of the exact top-k: FWHT and
+    MITM searches equal the oracle, ties at the bound included."""
+    _say("sampled bound", case, k)
+    m, cols = _sampled_bound_window(lib, case)
+    exp = ora.fwht_topk(m, cols, k)
+    assert lib.fwht_topk(m, cols, k) == exp
+    assert gpu_search(lib, ora, "mitm", m, cols, k) == exp
