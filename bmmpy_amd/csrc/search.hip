@@ -45,13 +45,14 @@ __global__ __launch_bounds__(256) void fwht_topk_kernel(const uint64_t* __restri
                                                         int n_pat, int t, int32_t total,
                                                         uint64_t n_blocks, int K,
                                                         unsigned long long* __restrict__ counter,
+                                                        const unsigned long long* __restrict__ bound,
                                                         uint64_t* __restrict__ lists) {
     __shared__ int32_t s[fwht_lds_words(L)];
     __shared__ TopkShared<CAP> T;
     __shared__ unsigned long long blk_id;
     constexpr int N = 1 << L;
     constexpr int E = N >= 256 ? N / 256 : 1;
-    topk_init(T);
+    topk_init(T, bound ? *bound : ~0ull);
     for (;;) {
         if (threadIdx.x == 0)
             blk_id = atomicAdd(counter, 1ull);
@@ -339,14 +340,57 @@ struct TopkBoundState {   // header zeroed before every search
     TopkBoundSlot slot[kBoundSample];
 };
 
+// Counts one block's masks into its slot: h[i] for the NV values of this
+// thread with ok[i] set (mask 0 excluded by the caller).
+template <int NV>
+__device__ inline void bound_count_block(const int32_t (&h)[NV], const bool (&ok)[NV],
+                                         int32_t total, int shift,
+                                         TopkBoundSlot& __restrict__ slot) {
+    __shared__ uint32_t lh[4][kBoundSpan];
+    __shared__ int32_t red[4];
+    const int tid = (int)threadIdx.x;
+    int32_t hmax = INT32_MIN;
+#pragma unroll
+    for (int i = 0; i < NV; ++i)
+        if (ok[i])
+            hmax = max(hmax, h[i]);
+    hmax = wave_max_i32(hmax);
+    if (tid % kWave == 0)
+        red[tid / kWave] = hmax;
+    __syncthreads();
+    hmax = max(max(red[0], red[1]), max(red[2], red[3]));
+    // A block with no mask (only mask 0) counts nothing.
+    const int32_t b0 = hmax == INT32_MIN ? kBoundBins : ((total - hmax) >> 1) >> shift;
+    // Counter d (bits 8d..8d+7) counts the thread's masks in bin b0 + d; at
+    // most 64 masks per thread, so no counter overflows.
+    static_assert(NV <= 255, "8-bit counters");
+    uint64_t acc = 0;
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+        const int32_t d = (((total - h[i]) >> 1) >> shift) - b0;
+        if (ok[i] && d < kBoundSpan)
+            acc += 1ull << (8 * d);
+    }
+#pragma unroll
+    for (int d = 0; d < kBoundSpan; ++d) {
+        const uint32_t c = wave_sum_u32((uint32_t)(acc >> (8 * d)) & 0xffu);
+        if (tid % kWave == 0)
+            lh[tid / kWave][d] = c;
+    }
+    __syncthreads();
+    if (tid == 0)
+        slot.b0 = b0;
+    if (tid < kBoundSpan)
+        slot.count[tid] = lh[0][tid] + lh[1][tid] + lh[2][tid] + lh[3][tid];
+}
+
+// Packed blocks of 2^14 masks (the packed top-k path).
 __global__ __launch_bounds__(256) void topk_bound_pk_kernel(const uint64_t* __restrict__ pat,
                                                             const int32_t* __restrict__ mult,
                                                             int n_pat, int32_t total,
                                                             uint64_t stride, int shift,
                                                             TopkBoundState* __restrict__ st) {
     __shared__ __align__(16) uint32_t s[pk_lds_words() + 4];
-    __shared__ uint32_t lh[4][kBoundSpan];
-    __shared__ int32_t red[4];
     const int tid = (int)threadIdx.x;
     const uint64_t mh14 = (uint64_t)blockIdx.x * stride;
     int4* s4 = reinterpret_cast<int4*>(s);
@@ -369,48 +413,44 @@ __global__ __launch_bounds__(256) void topk_bound_pk_kernel(const uint64_t* __re
             v[e] = x[e];
     });
     pk_butterflies(v);
-    const bool skip0 = mh14 == 0 && tid == 0;   // mask 0 (word 0, low half) is never a result
-    int32_t hmax = INT32_MIN;
+    int32_t h[64];
+    bool ok[64];
 #pragma unroll
     for (int e = 0; e < 32; ++e) {
         const s16x2 x = pk_as(v[e]);
-        if (!(skip0 && e == 0))
-            hmax = max(hmax, pk_lo(x));
-        hmax = max(hmax, pk_hi(x));
+        h[2 * e] = pk_lo(x);
+        h[2 * e + 1] = pk_hi(x);
+        ok[2 * e] = !(mh14 == 0 && tid == 0 && e == 0);   // mask 0 is never a result
+        ok[2 * e + 1] = true;
     }
-    hmax = wave_max_i32(hmax);
-    if (tid % kWave == 0)
-        red[tid / kWave] = hmax;
-    __syncthreads();
-    hmax = max(max(red[0], red[1]), max(red[2], red[3]));
-    const int32_t b0 = ((total - hmax) >> 1) >> shift;   // bin of the block's lightest mask
-    // Counter d (bits 8d..8d+7) counts the thread's masks in bin b0 + d; at
-    // most 64 masks per thread, so no counter overflows.
-    uint64_t acc = 0;
-    auto count = [&](int32_t h) {
-        const int32_t d = (((total - h) >> 1) >> shift) - b0;
-        if (d < kBoundSpan)
-            acc += 1ull << (8 * d);
-    };
+    bound_count_block(h, ok, total, shift, st->slot[blockIdx.x]);
+}
+
+// int32 blocks of 2^L masks (the generic top-k path).
+template <int L>
+__global__ __launch_bounds__(256) void topk_bound_i32_kernel(const uint64_t* __restrict__ pat,
+                                                             const int32_t* __restrict__ mult,
+                                                             int n_pat, int32_t total,
+                                                             uint64_t stride, int shift,
+                                                             TopkBoundState* __restrict__ st) {
+    __shared__ int32_t s[fwht_lds_words(L)];
+    constexpr int N = 1 << L;
+    constexpr int E = N >= 256 ? N / 256 : 1;
+    const int tid = (int)threadIdx.x;
+    const uint64_t mh = (uint64_t)blockIdx.x * stride;
+    block_build_g64<L>(s, pat, mult, n_pat, mh);
+    block_fwht<L>(s);
+    // Entry ml = e * 256 + tid sits at padded index pad(tid) + 264 e.
+    const int32_t* sp = s + fwht_pad(tid);
+    int32_t h[E];
+    bool ok[E];
 #pragma unroll
-    for (int e = 0; e < 32; ++e) {
-        const s16x2 x = pk_as(v[e]);
-        if (!(skip0 && e == 0))
-            count(pk_lo(x));
-        count(pk_hi(x));
+    for (int e = 0; e < E; ++e) {
+        const int ml = e * 256 + tid;
+        ok[e] = ml < N && (mh != 0 || ml != 0);
+        h[e] = ml < N ? sp[e * 264] : 0;
     }
-#pragma unroll
-    for (int d = 0; d < kBoundSpan; ++d) {
-        const uint32_t c = wave_sum_u32((uint32_t)(acc >> (8 * d)) & 0xffu);
-        if (tid % kWave == 0)
-            lh[tid / kWave][d] = c;
-    }
-    __syncthreads();
-    TopkBoundSlot& slot = st->slot[blockIdx.x];
-    if (tid == 0)
-        slot.b0 = b0;
-    if (tid < kBoundSpan)
-        slot.count[tid] = lh[0][tid] + lh[1][tid] + lh[2][tid] + lh[3][tid];
+    bound_count_block(h, ok, total, shift, st->slot[blockIdx.x]);
 }
 
 // One workgroup: sums the n_slots slots into bins and writes the key bound
@@ -546,25 +586,41 @@ SearchWorkspace& search_ws() {
 
 constexpr int kTopkMaxL = 13;
 
+// The int32 block pass, after the sampled bound pass when the blocks are few
+// per workgroup (or all in the sample).
 template <int L, int CAP>
 void launch_fwht_topk(const uint64_t* pat, const int32_t* mult, int n_pat, int t, int32_t total,
                       uint64_t n_blocks, int K, unsigned long long* counter, uint64_t* lists,
-                      int grid, hipStream_t s) {
+                      int grid, TopkBoundState* st, hipStream_t s) {
     ScopedKernelTimer timer(s);
+    const unsigned long long* bound = nullptr;
+    if (st) {
+        const uint64_t sample = std::min<uint64_t>(n_blocks, kBoundSample);
+        int shift = 0;
+        while ((total >> shift) >= kBoundBins)
+            ++shift;
+        hipLaunchKernelGGL((topk_bound_i32_kernel<L>), dim3((unsigned)sample), dim3(256), 0, s,
+                           pat, mult, n_pat, total, n_blocks / sample, shift, st);
+        BMM_HIP(hipGetLastError());
+        hipLaunchKernelGGL(topk_bound_finish_kernel, dim3(1), dim3(256), 0, s, (int)sample, shift,
+                           t, K, st);
+        BMM_HIP(hipGetLastError());
+        bound = &st->bound;
+    }
     hipLaunchKernelGGL((fwht_topk_kernel<L, CAP>), dim3(grid), dim3(256), 0, s, pat, mult, n_pat,
-                       t, total, n_blocks, K, counter, lists);
+                       t, total, n_blocks, K, counter, bound, lists);
     BMM_HIP(hipGetLastError());
 }
 
 template <int CAP>
 void dispatch_fwht_topk(int L, const uint64_t* pat, const int32_t* mult, int n_pat, int t,
                         int32_t total, uint64_t n_blocks, int K, unsigned long long* counter,
-                        uint64_t* lists, int grid, hipStream_t s) {
+                        uint64_t* lists, int grid, TopkBoundState* st, hipStream_t s) {
     switch (L) {
 #define BMM_L(LL)                                                                              \
     case LL:                                                                                   \
         launch_fwht_topk<LL, CAP>(pat, mult, n_pat, t, total, n_blocks, K, counter, lists, grid, \
-                                  s);                                                          \
+                                  st, s);                                                      \
         break;
         BMM_L(1) BMM_L(2) BMM_L(3) BMM_L(4) BMM_L(5) BMM_L(6) BMM_L(7) BMM_L(8) BMM_L(9)
         BMM_L(10) BMM_L(11) BMM_L(12) BMM_L(13)
@@ -663,14 +719,25 @@ std::vector<bmm_topk_entry> fwht_topk_engine(const uint64_t* d_pat, const int32_
         }
         d_out = K <= 512 ? merge_lists<2048>(d_lists, d_lists2, grid_pk, K, stream)
                          : merge_lists<8192>(d_lists, d_lists2, grid_pk, K, stream);
-    } else if (K <= 512) {
-        dispatch_fwht_topk<2048>(L, d_pat, d_mult, n_pat, t, (int32_t)total, n_blocks, K,
-                                 d_counter, d_lists, grid, stream);
-        d_out = merge_lists<2048>(d_lists, d_lists2, grid, K, stream);
     } else {
-        dispatch_fwht_topk<8192>(L, d_pat, d_mult, n_pat, t, (int32_t)total, n_blocks, K,
-                                 d_counter, d_lists, grid, stream);
-        d_out = merge_lists<8192>(d_lists, d_lists2, grid, K, stream);
+        // Few blocks per workgroup (or all of them in the sample): a sampled
+        // first pass bounds the keys, as for the packed blocks.
+        TopkBoundState* st = nullptr;
+        unsigned long long* d_queue = d_counter;
+        if (n_blocks <= (uint64_t)grid * 64) {
+            st = ws.bound.get<TopkBoundState>(1);
+            fill_u64(reinterpret_cast<uint64_t*>(st), 2, 0, stream);
+            d_queue = &st->counter;
+        }
+        if (K <= 512) {
+            dispatch_fwht_topk<2048>(L, d_pat, d_mult, n_pat, t, (int32_t)total, n_blocks, K,
+                                     d_queue, d_lists, grid, st, stream);
+            d_out = merge_lists<2048>(d_lists, d_lists2, grid, K, stream);
+        } else {
+            dispatch_fwht_topk<8192>(L, d_pat, d_mult, n_pat, t, (int32_t)total, n_blocks, K,
+                                     d_queue, d_lists, grid, st, stream);
+            d_out = merge_lists<8192>(d_lists, d_lists2, grid, K, stream);
+        }
     }
     std::vector<uint64_t> keys((size_t)K);
     BMM_HIP(hipMemcpyAsync(keys.data(), d_out, keys.size() * 8, hipMemcpyDeviceToHost, stream));
