@@ -713,7 +713,7 @@ def run_search(torch, steps: int, warmup: int) -> dict:
         "roofline": {"bound": "valu", "achieved": main["roofline_frac"] * VALU_PK16_PEAK_TOPS,
                      "peak": VALU_PK16_PEAK_TOPS, "unit": "TOPS (packed int16 VALU)",
                      "frac": main["roofline_frac"], "traffic": None,
-                     "ops_per_eval": TOPK_PK_L, "kernel": "fwht_topk_pk_kernel<2048, 4>",
+                     "ops_per_eval": TOPK_PK_L, "kernel": "fwht_topk_pk_kernel<256, 4>",
                      "kernel_avg_ms": main["kernel_avg_ms"]},
     }
 
