@@ -730,10 +730,11 @@ def secondary(torch, _lib, dev, stream) -> dict:
                              "roofline_frac": r["roofline"]["frac"],
                              "kernel_avg_ms": r["roofline"]["kernel_avg_ms"],
                              "result": r["results"][0]}
-    g = ga_compare(20.0, isd_trials=1)
+    g = ga_compare(20.0)
     res["ga_256x8192"] = {
         "planted_weight": g["planted_weight"], "start_weight": g["start_weight"],
-        "target": g["target"],
+        "target": g["target"], "isd_trials_per_epoch": g["isd_trials_per_epoch"],
+        "window_rounds_per_epoch": g["window_rounds_per_epoch"],
         **{f"{k}_{f}": g[k][f] for k in ("plain", "learned")
            for f in ("best_weight", "time_to_target_s", "generations", "ms_per_generation",
                      "evals_per_s", "replayed_ops_per_s")},
@@ -771,14 +772,15 @@ def mode_line(args, r: dict, cpu: dict | None) -> dict:
     return out
 
 
-def ga_compare(budget_s: float, seed: int = 0x5EED0005, isd_trials: int = 0,
-               ga_seed: int | None = None, **overrides) -> dict:
+def ga_compare(budget_s: float, seed: int = 0x5EED0005, ga_seed: int | None = None,
+               **overrides) -> dict:
     """Planted 256 x 8192 matrix (Bernoulli(0.01) sparse S, random invertible
     256 x 256 transform M, A = M S, drawn from `seed`); 64 islands x 256
     individuals x 512 ops, ring migration every 20 generations, GA random
-    streams from `ga_seed` (default: `seed`).  Plain and learned proposal
-    runs get the same wall-clock budget; reported: weight reached, time to
-    reach 1.01 * wt(S) (null when not reached) and the trajectory."""
+    streams from `ga_seed` (default: `seed`); the library's GA parameters
+    with `overrides` on top.  Plain and learned proposal runs get the same
+    wall-clock budget; reported: weight reached, time to reach 1.01 * wt(S)
+    (null when not reached), the trajectory and the epoch settings used."""
     from bmmpy_amd import _lib
 
     S, A, _ = _lib.gen_planted(seed, 256, 8192, 0.01)
@@ -788,8 +790,9 @@ def ga_compare(budget_s: float, seed: int = 0x5EED0005, isd_trials: int = 0,
     target = int(planted * 1.01)
     res = {"planted_weight": planted, "start_weight": start_w, "target": target}
     for name, learned in (("plain", 0), ("learned", 1)):
-        p = _lib.ga_default_params(learned=learned, seed=ga_seed, isd_trials=isd_trials,
-                                   **overrides)
+        p = _lib.ga_default_params(learned=learned, seed=ga_seed, **overrides)
+        res["isd_trials_per_epoch"] = int(p.isd_trials)
+        res["window_rounds_per_epoch"] = int(p.win_rounds)
         t0 = time.perf_counter()
         ga = _lib.GpuGA(A, 8192, p)
         setup = time.perf_counter() - t0
@@ -823,7 +826,7 @@ def ga_compare(budget_s: float, seed: int = 0x5EED0005, isd_trials: int = 0,
     return res
 
 
-def ga_seeds(budget_s: float, n_seeds: int, isd_trials: int, learned_overrides: dict,
+def ga_seeds(budget_s: float, n_seeds: int, overrides: dict, learned_overrides: dict,
              marks=(40.0, 20.0, 1.01)) -> dict:
     """Plain and learned GA over `n_seeds` GA seeds on the same planted
     256 x 8192 instance: per variant the time to reach each multiple of the
@@ -837,7 +840,7 @@ def ga_seeds(budget_s: float, n_seeds: int, isd_trials: int, learned_overrides: 
     for k in range(n_seeds):
         seed = 0x5EED0005 + 0x1000 * k
         for name, learned in (("plain", 0), ("learned", 1)):
-            kw = dict(learned=learned, seed=seed, isd_trials=isd_trials)
+            kw = dict(learned=learned, seed=seed, **overrides)
             if learned:
                 kw.update(learned_overrides)
             p = _lib.ga_default_params(**kw)
@@ -867,7 +870,9 @@ def ga_seeds(budget_s: float, n_seeds: int, isd_trials: int, learned_overrides: 
                 "median_s": float(np.median(done)) if len(done) == len(t) else None,
                 "min_s": done[0] if done else None, "max_s": done[-1] if done else None}
 
-    out = {"planted_weight": planted, "isd_trials": isd_trials, "budget_s": budget_s,
+    p0 = _lib.ga_default_params(**overrides)
+    out = {"planted_weight": planted, "isd_trials": int(p0.isd_trials),
+           "window_rounds": int(p0.win_rounds), "budget_s": budget_s,
            "learned_overrides": learned_overrides, "marks": list(marks)}
     for m in marks:
         a, b = summary("plain", m), summary("learned", m)
@@ -901,9 +906,12 @@ def main() -> int:
                          "FWHT coset-minimum workloads; ga: plain vs learned GA")
     ap.add_argument("--ga-budget", type=float, default=60.0,
                     help="wall-clock seconds per GA variant in --workload ga")
-    ap.add_argument("--ga-isd-trials", type=int, default=1,
+    ap.add_argument("--ga-isd-trials", type=int, default=-1,
                     help="information-set trials per island per migration epoch in "
-                         "--workload ga (both variants; 0 = off)")
+                         "--workload ga (both variants; 0 = off; default: the library's)")
+    ap.add_argument("--ga-win-rounds", type=int, default=-1,
+                    help="exact 20-row window rounds per migration epoch in --workload ga "
+                         "(both variants; default: the library's)")
     ap.add_argument("--ga-seeds", type=int, default=0,
                     help="--workload ga: compare plain and learned over this many GA seeds "
                          "(median and range of the times to 40x, 20x and 1.01x planted)")
@@ -938,20 +946,26 @@ def main() -> int:
 
         atexit.register(_copy_once)
 
+    ga_ov = {}
+    if args.ga_isd_trials >= 0:
+        ga_ov["isd_trials"] = args.ga_isd_trials
+    if args.ga_win_rounds >= 0:
+        ga_ov["win_rounds"] = args.ga_win_rounds
     if args.workload == "ga" and args.ga_seeds > 0:
         ov = {"win_candidates": args.ga_win_candidates} if args.ga_win_candidates else {}
-        r = ga_seeds(args.ga_budget, args.ga_seeds, args.ga_isd_trials, ov)
+        r = ga_seeds(args.ga_budget, args.ga_seeds, ga_ov, ov)
         print(json.dumps({"metric": METRIC, "value": r["x1.01"]["median_ratio_learned_over_plain"],
                           "unit": "median time ratio learned / plain to 1.01 x planted",
                           "n_gpus": 1, "steps": args.ga_seeds, "warmup": 0, "ms_per_step": None,
                           "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
                           "dtype": "u32", "data": "synthetic",
                           "config": {"workload": "ga_island_planted_256x8192_seeds",
-                                     "seeds": args.ga_seeds, "isd_trials": args.ga_isd_trials},
+                                     "seeds": args.ga_seeds, "isd_trials": r["isd_trials"],
+                                     "window_rounds": r["window_rounds"]},
                           "ga": r}))
         return 0
     if args.workload == "ga":
-        r = ga_compare(args.ga_budget, isd_trials=args.ga_isd_trials)
+        r = ga_compare(args.ga_budget, **ga_ov)
         plain, learned = r["plain"], r["learned"]
         ratio = None
         if plain["time_to_target_s"] and learned["time_to_target_s"]:
@@ -963,8 +977,9 @@ def main() -> int:
             "scaling": "weak", "vs_baseline": None, "dtype": "u32", "data": "synthetic",
             "config": {"workload": "ga_island_planted_256x8192", "islands": 64,
                        "population": 256, "max_ops": 512, "migrate_every": 20,
-                       "window_rows": 20, "window_rounds_per_epoch": 2,
-                       "isd_trials_per_epoch": args.ga_isd_trials,
+                       "window_rows": 20,
+                       "window_rounds_per_epoch": r["window_rounds_per_epoch"],
+                       "isd_trials_per_epoch": r["isd_trials_per_epoch"],
                        "budget_s": args.ga_budget},
             "learned_over_plain_time_ratio": ratio, "ga": r}))
         return 0
