@@ -1070,112 +1070,160 @@ __global__ void ga_win_apply_kernel(uint64_t* __restrict__ base, int rows, int w
 // full basis is chosen.  The result is the lightest basis the pool holds, so
 // the island's weight never grows.
 //
-// One workgroup per island.  Thread r owns row r's mask (<= 4 words) in
-// registers; row XORs run over (row, word) pairs of the island's copy in
-// `rbuf` (L2-resident: rows x words u64 per island).
+// Elimination in two kernels.  The draws (column c = xorshift state mod
+// cols) do not depend on the matrix, and row r's current bit in column c is
+// parity(T_r & a(c)) with a(c) column c of B, so the pivoting runs on the
+// 256-bit coefficient masks alone (ga_isd_pivot_kernel: one workgroup per
+// island, thread r owns T_r, the columns of the next kIsdDraws draws are
+// gathered as 256-bit vectors first, one barrier per draw).  R = T B is then
+// one GF(2) product (ga_isd_rows_kernel), with the row weights.  Both apply
+// exactly the row operations of elimination on the full rows.
 constexpr int kIsdMaskWords = 4;   // rows <= 256
+constexpr int kIsdDraws = 512;     // draws whose columns are gathered at once
+constexpr int kIsdGather = 32;     // gather loads in flight per thread
 
-__global__ __launch_bounds__(256) void ga_isd_elim_kernel(const uint64_t* __restrict__ base,
-                                                          int rows, int words, int cols,
-                                                          uint64_t seed,
-                                                          uint64_t* __restrict__ rbuf,
-                                                          uint64_t* __restrict__ tmask,
-                                                          uint32_t* __restrict__ rw) {
+__global__ __launch_bounds__(256) void ga_isd_pivot_kernel(const uint64_t* __restrict__ base,
+                                                           int rows, int words, int cols,
+                                                           uint64_t seed,
+                                                           uint64_t* __restrict__ tmask) {
     const int isl = blockIdx.x, tid = threadIdx.x;
     const int wave = tid / kWave, lane = tid % kWave;
-    const size_t per = (size_t)rows * words;
-    const uint64_t* B = base + (size_t)isl * per;
-    uint64_t* Rb = rbuf + (size_t)isl * per;
-    for (size_t k = tid; k < per; k += blockDim.x)
-        Rb[k] = B[k];
+    const uint64_t* B = base + (size_t)isl * rows * words;
+    __shared__ uint64_t colv[kIsdDraws][kIsdMaskWords];   // a(c) of the chunk's draws
+    __shared__ int cidx[kIsdDraws];
+    __shared__ uint64_t state[kIsdDraws];
+    __shared__ uint64_t tm[256][kIsdMaskWords];           // every row's current mask
+    __shared__ uint64_t cand[2][kIsdMaskWords];           // rows that could pivot, per draw parity
     uint64_t t[kIsdMaskWords] = {};
     if (tid < rows)
         t[tid / 64] = 1ull << (tid % 64);
-    __shared__ uint64_t piv_t[kIsdMaskWords];
-    __shared__ uint8_t hasbit[256], pivoted[256];
-    __shared__ int wmin[4];
-    __shared__ int s_col;
-    pivoted[tid] = 0;
-    hasbit[tid] = 0;
-    // Threads per row in the XOR sweep: a power of two covering the words
-    // (words is a multiple of 32), at most the workgroup.
-    int tpr = 32;
-    while (tpr < words && tpr < 256)
-        tpr <<= 1;
-    const int groups = 256 / tpr, sub = tid / tpr, w0 = tid % tpr;
+#pragma unroll
+    for (int q = 0; q < kIsdMaskWords; ++q)
+        tm[tid][q] = t[q];
+    bool pivoted = false;
     uint64_t rng = seed ^ (0x9E3779B97F4A7C15ull * (uint64_t)(isl + 1));
     if (rng == 0)
         rng = 1;
-    __syncthreads();
     int npiv = 0;
     const int max_draws = 8 * cols + 64;
-    for (int draw = 0; draw < max_draws && npiv < rows; ++draw) {
-        if (tid == 0) {
-            rng ^= rng << 13;
-            rng ^= rng >> 7;
-            rng ^= rng << 17;
-            s_col = (int)(rng % (uint64_t)cols);
-        }
+    for (int d0 = 0; d0 < max_draws && npiv < rows; d0 += kIsdDraws) {
+        const int nd = min(kIsdDraws, max_draws - d0);
+        __syncthreads();   // the previous chunk's reads of cidx / colv are done
+        // Thread 0 steps the generator; the (64-bit, software) modulo runs
+        // on all threads.
+        if (tid == 0)
+            for (int d = 0; d < nd; ++d) {
+                rng ^= rng << 13;
+                rng ^= rng >> 7;
+                rng ^= rng << 17;
+                state[d] = rng;
+            }
         __syncthreads();
-        const int c = s_col;
-        const int bit = tid < rows ? (int)((Rb[(size_t)tid * words + (c >> 6)] >> (c & 63)) & 1ull)
-                                   : 0;
-        hasbit[tid] = (uint8_t)bit;
-        const uint64_t cand = __ballot(bit && !pivoted[tid]);
-        if (lane == 0)
-            wmin[wave] = cand ? wave * kWave + __builtin_ctzll(cand) : 1 << 30;
+        for (int d = tid; d < nd; d += blockDim.x)
+            cidx[d] = (int)(state[d] % (uint64_t)cols);
         __syncthreads();
-        const int r = min(min(wmin[0], wmin[1]), min(wmin[2], wmin[3]));
-        if (r >= rows) {
-            __syncthreads();
-            continue;
-        }
-        if (tid == r) {
-            pivoted[r] = 1;
+        // Gather: thread r reads its row's word of each drawn column
+        // (kIsdGather loads in flight), each wave ballots its 64 rows' bits.
+        const uint64_t* row = B + (size_t)min(tid, rows - 1) * words;
+        for (int d = 0; d < nd; d += kIsdGather) {
+            uint64_t w[kIsdGather];
 #pragma unroll
-            for (int q = 0; q < kIsdMaskWords; ++q)
-                piv_t[q] = t[q];
-        }
-        __syncthreads();
-        if (bit && tid != r) {
+            for (int j = 0; j < kIsdGather; ++j)
+                w[j] = row[cidx[min(d + j, nd - 1)] >> 6];
 #pragma unroll
-            for (int q = 0; q < kIsdMaskWords; ++q)
-                t[q] ^= piv_t[q];
-        }
-        // Rows with the bit ^= pivot row.  Eight rows per batch: all loads
-        // of a batch are issued before its stores, so they are in flight
-        // together (the rows of a batch are distinct, none is the pivot).
-        const uint64_t* P = Rb + (size_t)r * words;
-        for (int w = w0; w < words; w += tpr) {
-            const uint64_t pw = P[w];
-            for (int r0 = sub; r0 < rows; r0 += 8 * groups) {
-                uint64_t x[8];
-                bool on[8];
-#pragma unroll
-                for (int j = 0; j < 8; ++j) {
-                    const int r2 = r0 + j * groups;
-                    on[j] = r2 < rows && hasbit[r2] && r2 != r;
-                    x[j] = on[j] ? Rb[(size_t)r2 * words + w] : 0ull;
-                }
-#pragma unroll
-                for (int j = 0; j < 8; ++j)
-                    if (on[j])
-                        Rb[(size_t)(r0 + j * groups) * words + w] = x[j] ^ pw;
+            for (int j = 0; j < kIsdGather; ++j) {
+                const int c = cidx[min(d + j, nd - 1)];
+                const uint64_t bal = __ballot(tid < rows && ((w[j] >> (c & 63)) & 1ull));
+                if (lane == 0 && d + j < nd)
+                    colv[d + j][wave] = bal;
             }
         }
         __syncthreads();
-        ++npiv;
+        for (int d = 0; d < nd && npiv < rows; ++d) {
+            uint32_t par = 0;
+#pragma unroll
+            for (int q = 0; q < kIsdMaskWords; ++q)
+                par += (uint32_t)__builtin_popcountll(t[q] & colv[d][q]);
+            const bool bit = (par & 1u) != 0u;
+            const uint64_t c = __ballot(bit && !pivoted);
+            if (lane == 0)
+                cand[d & 1][wave] = c;
+            __syncthreads();   // cand written; masks of the previous draw written
+            int r = -1;
+#pragma unroll
+            for (int q = 0; q < kIsdMaskWords; ++q)
+                if (r < 0 && cand[d & 1][q] != 0ull)
+                    r = q * 64 + __builtin_ctzll(cand[d & 1][q]);
+            if (r < 0)
+                continue;   // no row can pivot on this column
+            ++npiv;
+            if (tid == r) {
+                pivoted = true;
+            } else if (bit) {
+#pragma unroll
+                for (int q = 0; q < kIsdMaskWords; ++q) {
+                    t[q] ^= tm[r][q];
+                    tm[tid][q] = t[q];
+                }
+            }
+        }
     }
     if (tid < rows) {
         uint64_t* T = tmask + ((size_t)isl * rows + tid) * kIsdMaskWords;
 #pragma unroll
         for (int q = 0; q < kIsdMaskWords; ++q)
             T[q] = t[q];
+    }
+}
+
+// R = T B for one island and a slice of kIsdSlice words (grid: slices x
+// islands): the slice of B staged in LDS, thread r XORs the staged rows its
+// mask selects (a uniform loop over rows: broadcast LDS reads), stores its
+// slice of R and adds the slice's popcount to rw (zeroed before).
+constexpr int kIsdSlice = 16;
+
+__global__ __launch_bounds__(256) void ga_isd_rows_kernel(const uint64_t* __restrict__ base,
+                                                          int rows, int words,
+                                                          const uint64_t* __restrict__ tmask,
+                                                          uint64_t* __restrict__ rbuf,
+                                                          uint32_t* __restrict__ rw) {
+    const int slice = blockIdx.x, isl = blockIdx.y, tid = threadIdx.x;
+    const size_t per = (size_t)rows * words;
+    const uint64_t* B = base + (size_t)isl * per;
+    __shared__ uint64_t a[256][kIsdSlice];
+    const int w0 = slice * kIsdSlice;
+    for (int i = tid; i < rows * kIsdSlice; i += blockDim.x)
+        a[i / kIsdSlice][i % kIsdSlice] = B[(size_t)(i / kIsdSlice) * words + w0 + i % kIsdSlice];
+    uint64_t t[kIsdMaskWords] = {};
+    if (tid < rows) {
+        const uint64_t* T = tmask + ((size_t)isl * rows + tid) * kIsdMaskWords;
+#pragma unroll
+        for (int q = 0; q < kIsdMaskWords; ++q)
+            t[q] = T[q];
+    }
+    __syncthreads();
+    uint64_t acc[kIsdSlice] = {};
+#pragma unroll
+    for (int q = 0; q < kIsdMaskWords; ++q) {   // rows 64 q .. 64 q + 63
+        const uint64_t tq = t[q];
+        const int n = min(64, rows - 64 * q);
+        for (int jj = 0; jj < n; ++jj) {
+            const uint64_t m = 0ull - ((tq >> jj) & 1ull);
+            const uint64_t* aj = a[64 * q + jj];
+#pragma unroll
+            for (int k = 0; k < kIsdSlice; ++k)
+                acc[k] ^= aj[k] & m;
+        }
+    }
+    if (tid < rows) {
+        uint64_t* R = rbuf + (size_t)isl * per + (size_t)tid * words + w0;
         uint32_t c = 0;
-        const uint64_t* row = Rb + (size_t)tid * words;
-        for (int w = 0; w < words; ++w)
-            c += (uint32_t)__builtin_popcountll(row[w]);
-        rw[(size_t)isl * rows + tid] = c;
+#pragma unroll
+        for (int k = 0; k < kIsdSlice; ++k) {
+            R[k] = acc[k];
+            c += (uint32_t)__builtin_popcountll(acc[k]);
+        }
+        atomicAdd(&rw[(size_t)isl * rows + tid], c);
     }
 }
 
@@ -1645,10 +1693,14 @@ unsigned int isd_improvement(Ga& G) {
     refresh_sketches(G);
     for (int t = 0; t < G.P.isd_trials; ++t) {
         const uint64_t seed = (uint64_t)G.P.seed ^ (0xA0761D6478BD642Full * (++G.isd_round));
-        hipLaunchKernelGGL(ga_isd_elim_kernel, dim3(I), dim3(256), 0, G.stream,
+        hipLaunchKernelGGL(ga_isd_pivot_kernel, dim3(I), dim3(256), 0, G.stream,
                            static_cast<const uint64_t*>(G.base.ptr), R, G.words, G.words * 64,
-                           seed, static_cast<uint64_t*>(G.isd_r.ptr),
-                           static_cast<uint64_t*>(G.isd_t.ptr), static_cast<uint32_t*>(G.isd_w.ptr));
+                           seed, static_cast<uint64_t*>(G.isd_t.ptr));
+        fill_u32(static_cast<uint32_t*>(G.isd_w.ptr), (size_t)I * R, 0u, G.stream);
+        hipLaunchKernelGGL(ga_isd_rows_kernel, dim3(G.words / kIsdSlice, I), dim3(256), 0,
+                           G.stream, static_cast<const uint64_t*>(G.base.ptr), R, G.words,
+                           static_cast<const uint64_t*>(G.isd_t.ptr),
+                           static_cast<uint64_t*>(G.isd_r.ptr), static_cast<uint32_t*>(G.isd_w.ptr));
         hipLaunchKernelGGL(ga_isd_select_kernel, dim3(I), dim3(256), 0, G.stream, R,
                            static_cast<const uint64_t*>(G.isd_t.ptr),
                            static_cast<const uint32_t*>(G.isd_w.ptr),
