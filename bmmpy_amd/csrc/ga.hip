@@ -76,7 +76,12 @@ __global__ __launch_bounds__(64) void ga_eval_kernel(const uint32_t* __restrict_
     __shared__ uint16_t slot_of[256];    // 0: untouched, else slot + 1
     __shared__ uint16_t row_of[256];
     const int slices = W32 / 64;
-    const int task = blockIdx.x;
+    // Workgroups go to the 8 XCDs in turn; each XCD takes a contiguous eighth
+    // of the tasks (whole islands when the island count is a multiple of 8),
+    // so the island bases it reads stay in its own L2.
+    int task = blockIdx.x;
+    if ((gridDim.x & 7u) == 0u)
+        task = (int)((blockIdx.x & 7u) * (gridDim.x >> 3) + (blockIdx.x >> 3));
     const int slice = task % slices;
     const int ind = (task / slices) % pop;
     const int isl = task / (slices * pop);
