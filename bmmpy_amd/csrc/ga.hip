@@ -48,6 +48,7 @@ void mlp_adam_dev(Mlp* M, float lr, hipStream_t s);
 
 void replay_host(uint64_t* rows, size_t n_rows, size_t words, const uint16_t* ops, size_t n_ops,
                  uint64_t* weight);
+int default_persistent_wgs();
 
 __device__ inline uint64_t xs_next(uint64_t& s) {
     s ^= s << 13;
@@ -63,25 +64,20 @@ __device__ inline uint64_t xs_next(uint64_t& s) {
 // of popcount(after) - popcount(before), so untouched rows cost nothing.  The
 // touched rows of the slice are staged in LDS slots (row -> slot table), the
 // ops applied lane-parallel over the slice's 64 words.
-__global__ __launch_bounds__(64) void ga_eval_kernel(const uint32_t* __restrict__ base,
-                                                     const uint16_t* __restrict__ ops,
-                                                     const int32_t* __restrict__ len, int pop,
-                                                     int rows, int W32, int max_ops,
-                                                     unsigned int* __restrict__ fit,
-                                                     unsigned long long* __restrict__ n_ops) {
-    // Dynamic LDS: rows x 64 words for the touched rows, then the ops padded
-    // to a multiple of 4 (first as loaded, then as src slot | dst slot << 16).
-    extern __shared__ __align__(16) uint32_t cols[];
-    uint32_t* opsl = cols + rows * 64;
-    __shared__ uint16_t slot_of[256];    // 0: untouched, else slot + 1
-    __shared__ uint16_t row_of[256];
+// One task (island, individual, slice) on one wavefront.  `slots` rows x 64
+// words of LDS at `cols` hold the touched rows; returns false, with nothing
+// computed, when the op list touches more rows than that (the task is then
+// left to the large tier).
+__device__ inline bool ga_eval_task(int task, int slots, uint32_t* __restrict__ cols,
+                                    uint16_t* __restrict__ slot_of, uint16_t* __restrict__ row_of,
+                                    const uint32_t* __restrict__ base,
+                                    const uint16_t* __restrict__ ops,
+                                    const int32_t* __restrict__ len, int pop, int rows, int W32,
+                                    int max_ops, unsigned int* __restrict__ fit) {
+    // LDS: slots x 64 words for the touched rows, then the ops padded to a
+    // multiple of 4 (first as loaded, then as src slot | dst slot << 16).
+    uint32_t* opsl = cols + slots * 64;
     const int slices = W32 / 64;
-    // Workgroups go to the 8 XCDs in turn; each XCD takes a contiguous eighth
-    // of the tasks (whole islands when the island count is a multiple of 8),
-    // so the island bases it reads stay in its own L2.
-    int task = blockIdx.x;
-    if ((gridDim.x & 7u) == 0u)
-        task = (int)((blockIdx.x & 7u) * (gridDim.x >> 3) + (blockIdx.x >> 3));
     const int slice = task % slices;
     const int ind = (task / slices) % pop;
     const int isl = task / (slices * pop);
@@ -89,6 +85,7 @@ __global__ __launch_bounds__(64) void ga_eval_kernel(const uint32_t* __restrict_
     const size_t id = (size_t)isl * pop + ind;
     const uint16_t* o = ops + id * max_ops;
     const int n = len[id];
+    __syncthreads();   // the previous task's reads of LDS are done
     for (int r = lane; r < 256; r += kWave)
         slot_of[r] = 0;
     __syncthreads();
@@ -119,6 +116,8 @@ __global__ __launch_bounds__(64) void ga_eval_kernel(const uint32_t* __restrict_
             incl += v;
     }
     const int n_touched = (int)__shfl(incl, kWave - 1, kWave);
+    if (n_touched > slots)
+        return false;
     uint32_t slot = incl - cnt;
     __syncthreads();
 #pragma unroll
@@ -185,10 +184,74 @@ __global__ __launch_bounds__(64) void ga_eval_kernel(const uint32_t* __restrict_
     for (int t = 0; t < n_touched; ++t)
         after += __builtin_popcount(cols[t * 64 + lane]);
     const int32_t delta = (int32_t)wave_sum_u32(after) - (int32_t)wave_sum_u32(before);
-    if (lane == 0) {
+    if (lane == 0)
         atomicAdd(&fit[id], (unsigned int)delta);   // wraps back to the exact total
-        if (slice == 0)
-            atomicAdd(n_ops, (unsigned long long)n);   // ops replayed (one count per individual)
+    return true;
+}
+
+// Task order: workgroups go to the 8 XCDs in turn; each XCD takes a
+// contiguous eighth of the tasks (whole islands when the island count is a
+// multiple of 8), so the island bases it reads stay in its own L2.
+__device__ inline int ga_eval_task_of(unsigned int i, unsigned int n) {
+    return (n & 7u) == 0u ? (int)((i & 7u) * (n >> 3) + (i >> 3)) : (int)i;
+}
+
+// Tiers of row slots in LDS: the first tier (one wavefront per task, 32
+// slots: many wavefronts per CU) takes every task whose ops touch at most 32
+// rows and flags the others 1; the second (128 slots) takes the tasks
+// flagged 1 and flags 2 those touching more; the last (every row) takes
+// those.  Early in an epoch op lists are long; once the information-set
+// trials have lightened the bases they stay a few ops long.
+constexpr int kGaEvalSlots0 = 32;
+constexpr int kGaEvalSlots1 = 128;
+
+__global__ __launch_bounds__(64) void ga_eval_kernel(const uint32_t* __restrict__ base,
+                                                     const uint16_t* __restrict__ ops,
+                                                     const int32_t* __restrict__ len, int pop,
+                                                     int rows, int W32, int max_ops,
+                                                     unsigned int* __restrict__ fit,
+                                                     int slots, uint8_t* __restrict__ tier) {
+    extern __shared__ __align__(16) uint32_t cols[];
+    __shared__ uint16_t slot_of[256];    // 0: untouched, else slot + 1
+    __shared__ uint16_t row_of[256];
+    const int task = ga_eval_task_of(blockIdx.x, gridDim.x);
+    const bool done = ga_eval_task(task, slots, cols, slot_of, row_of, base, ops, len, pop, rows,
+                                   W32, max_ops, fit);
+    if (threadIdx.x == 0)
+        tier[task] = done ? 0 : 1;
+}
+
+// A later tier: a fixed grid of wavefronts reading the flags 64 at a time
+// and evaluating the tasks flagged `level` with `slots` row slots (flagging
+// level + 1 a task that touches more rows).
+__global__ __launch_bounds__(64) void ga_eval_tier_kernel(const uint32_t* __restrict__ base,
+                                                          const uint16_t* __restrict__ ops,
+                                                          const int32_t* __restrict__ len,
+                                                          int pop, int rows, int W32,
+                                                          int max_ops, int n_tasks, int slots,
+                                                          int level,
+                                                          unsigned int* __restrict__ fit,
+                                                          uint8_t* __restrict__ tier) {
+    extern __shared__ __align__(16) uint32_t cols[];
+    __shared__ uint16_t slot_of[256];
+    __shared__ uint16_t row_of[256];
+    // Wavefront b takes launch indices b, b + G, b + 2 G, ... (G = the grid):
+    // the same XCD as the first tier gave those tasks, and long and short op
+    // lists spread over all wavefronts.
+    const int G = (int)gridDim.x;
+    for (int k0 = 0; (int)blockIdx.x + k0 * G < n_tasks; k0 += kWave) {
+        const int i = (int)blockIdx.x + (k0 + (int)threadIdx.x) * G;
+        const int ti = i < n_tasks ? ga_eval_task_of((unsigned int)i, (unsigned int)n_tasks) : 0;
+        uint64_t m = __ballot(i < n_tasks && tier[ti] == level);
+        while (m) {
+            const int lane_k = __builtin_ctzll(m);
+            m &= m - 1;
+            const int task = __shfl(ti, lane_k, kWave);
+            if (!ga_eval_task(task, slots, cols, slot_of, row_of, base, ops, len, pop, rows, W32,
+                              max_ops, fit) &&
+                threadIdx.x == 0)
+                tier[task] = (uint8_t)(level + 1);
+        }
     }
 }
 
@@ -301,20 +364,31 @@ __global__ __launch_bounds__(1024) void ga_breed_kernel(
     int32_t* __restrict__ len_next, uint64_t* __restrict__ rng_state,
     unsigned int* __restrict__ best_w, uint16_t* __restrict__ best_ops,
     int32_t* __restrict__ best_len, const uint16_t* __restrict__ cand,
-    unsigned long long* __restrict__ evals, GaIslandState* __restrict__ state) {
+    unsigned long long* __restrict__ evals, unsigned long long* __restrict__ n_ops,
+    GaIslandState* __restrict__ state) {
     extern __shared__ unsigned int sf[];
     uint16_t* order = reinterpret_cast<uint16_t*>(sf + P.pop);
     uint16_t* gsel = order + ((P.pop + 1) & ~1);
     __shared__ uint32_t good[65536 / 32];   // learned: bitmap of the ranked pairs
     __shared__ int s_mode;
+    __shared__ unsigned int s_ops;
     __shared__ unsigned int s_noimp;
     __shared__ float s_rate;
     const int isl = blockIdx.x;
     const int t = threadIdx.x;
     const size_t base_id = (size_t)isl * P.pop;
+    if (t == 0)
+        s_ops = 0u;
     if (t < P.pop)
         sf[t] = fit[base_id + t];
     __syncthreads();
+    {
+        // Ops the evaluation replayed on this island (one device atomic per
+        // island instead of one per individual).
+        const uint32_t c = wave_sum_u32(t < P.pop ? (uint32_t)len_cur[base_id + t] : 0u);
+        if (t % kWave == 0 && c != 0u)
+            atomicAdd(&s_ops, c);
+    }
     if (t < P.pop) {
         const unsigned int f = sf[t];
         int r = 0;
@@ -364,6 +438,9 @@ __global__ __launch_bounds__(1024) void ga_breed_kernel(
         s_rate = s.rate;
         atomicAdd(evals, (unsigned long long)P.pop);
     }
+    __syncthreads();
+    if (t == 0 && s_ops != 0u)
+        atomicAdd(n_ops, (unsigned long long)s_ops);
     __syncthreads();
     const int mode = s_mode;
     const bool active = t < P.pop;
@@ -1392,7 +1469,7 @@ struct Ga {
     bmm_ga_params P{};
     int W32 = 0, words = 0;
     DevBuf base, base_tmp, ops[2], len[2], fit, rng, best_w, best_ops, best_len, cand, flag,
-        w_snap, roww, sketch, evals, n_ops, base_total;
+        w_snap, roww, sketch, evals, n_ops, base_total, eval_tier;
     // proposal model data
     DevBuf pairs, pair_isl, feats, labels, prng;
     DevBuf li_move, li_w, li_changed;
@@ -1471,13 +1548,33 @@ void enqueue_generation(Ga& G, int parity, hipStream_t s) {
     hipLaunchKernelGGL(ga_fit_init_kernel, dim3((unsigned)((n_ind + 255) / 256)), dim3(256), 0, s,
                        fit, static_cast<const unsigned int*>(G.base_total.ptr), Pp, n_ind);
     const int slices = G.W32 / 64;
-    const size_t lds = ((size_t)G.P.rows * 64 + (size_t)((G.P.max_ops + 3) & ~3)) * 4;
-    hipLaunchKernelGGL(ga_eval_kernel, dim3((unsigned)(I * Pp * slices)), dim3(64), lds, s,
+    const int n_tasks = I * Pp * slices;
+    const size_t ops_words = (size_t)((G.P.max_ops + 3) & ~3);
+    uint8_t* tier = static_cast<uint8_t*>(G.eval_tier.ptr);
+    auto lds = [&](int slots) { return ((size_t)slots * 64 + ops_words) * 4; };
+    const int s0 = std::min(G.P.rows, kGaEvalSlots0);
+    hipLaunchKernelGGL(ga_eval_kernel, dim3((unsigned)n_tasks), dim3(64), lds(s0), s,
                        static_cast<const uint32_t*>(G.base.ptr),
                        static_cast<const uint16_t*>(G.ops[parity].ptr),
                        static_cast<const int32_t*>(G.len[parity].ptr), Pp, G.P.rows, G.W32,
-                       G.P.max_ops, fit, static_cast<unsigned long long*>(G.n_ops.ptr));
+                       G.P.max_ops, fit, s0, tier);
     BMM_HIP(hipGetLastError());
+    // Later tiers on fixed grids: four (128 slots) and two (every row)
+    // wavefronts per CU.
+    const int cus = default_persistent_wgs() / 8;
+    const int levels[2] = {1, 2};
+    const int slots[2] = {std::min(G.P.rows, kGaEvalSlots1), G.P.rows};
+    const int per_cu[2] = {4, 2};
+    for (int k = 0; k < 2; ++k) {
+        if (slots[k] <= (k == 0 ? s0 : slots[0]))
+            break;   // every task fits an earlier tier
+        hipLaunchKernelGGL(ga_eval_tier_kernel, dim3((unsigned)std::min(n_tasks, cus * per_cu[k])),
+                           dim3(64), lds(slots[k]), s, static_cast<const uint32_t*>(G.base.ptr),
+                           static_cast<const uint16_t*>(G.ops[parity].ptr),
+                           static_cast<const int32_t*>(G.len[parity].ptr), Pp, G.P.rows, G.W32,
+                           G.P.max_ops, n_tasks, slots[k], levels[k], fit, tier);
+        BMM_HIP(hipGetLastError());
+    }
     hipLaunchKernelGGL(ga_breed_kernel, dim3(I), dim3(std::max(64, (Pp + 63) / 64 * 64)),
                        breed_lds_bytes(G.P), s, dev_params(G), fit,
                        static_cast<const uint16_t*>(G.ops[parity].ptr),
@@ -1489,6 +1586,7 @@ void enqueue_generation(Ga& G, int parity, hipStream_t s) {
                        static_cast<int32_t*>(G.best_len.ptr),
                        static_cast<const uint16_t*>(G.cand.ptr),
                        static_cast<unsigned long long*>(G.evals.ptr),
+                       static_cast<unsigned long long*>(G.n_ops.ptr),
                        static_cast<GaIslandState*>(G.isl_state.ptr));
     BMM_HIP(hipGetLastError());
 }
@@ -1823,6 +1921,8 @@ Ga* ga_create(const uint64_t* matrix, size_t rows, size_t cols, const bmm_ga_par
         BMM_HIP(hipMemset(G->len[b].ptr, 0, (size_t)I * Pp * 4));
     }
     G->fit.get<unsigned int>((size_t)I * Pp);
+    // One tier flag per fitness task (written by the first tier).
+    G->eval_tier.get<uint8_t>((size_t)I * Pp * (size_t)(G->W32 / 64));
     G->best_ops.get<uint16_t>((size_t)I * P.max_ops);
     int32_t* bl = G->best_len.get<int32_t>(I);
     BMM_HIP(hipMemset(bl, 0, (size_t)I * 4));

@@ -198,3 +198,40 @@ def test_island_configuration_errors(lib):
         lib.GpuGA(A, COLS, lib.ga_default_params(num_parents=5))
     with pytest.raises(ValueError):
         lib.GpuGA(A, COLS, lib.ga_default_params(num_offspring=0))
+
+
+def test_fitness_exact_in_both_eval_tiers(lib, ora):
+    """Fitness of every individual equals the oracle's replay of its ops on a
+    128-row window whose op lists grow from a few ops to well over 64
+    touched rows: individuals touching at most 64 rows are evaluated by the
+    small-LDS tier, the others by the large tier; both occur."""
+    _say("ga fitness, both evaluation tiers")
+    S, A, M = lib.gen_planted(0x5EED000A, 128, COLS, 0.03)
+    p = lib.ga_default_params(islands=2, population=32, max_ops=200, tournament=3, elite=2,
+                              migrate_every=1000, check_every=1, learned=0, li_sweeps=0,
+                              win_rows=0, isd_trials=0, mutations=12, p_append=1.0,
+                              p_replace=0.0, p_delete=0.0)
+    ga = lib.GpuGA(A, COLS, p)
+    small = large = 0
+    try:
+        for gen in range(10):
+            pre = [ga.population(i, 32, 200) for i in range(2)]
+            ga.run(1)
+            for isl in range(2):
+                a, b = pre[isl], ga.population(isl, 32, 200)
+                seqs = _seqs(a["ops_next"], a["len_next"])
+                base, _ = ga.best_ops(isl)
+                for i in range(32):
+                    pr = _pairs(seqs[i])
+                    moved = pr[pr[:, 0] != pr[:, 1]] if len(pr) else pr
+                    touched = len(set(moved.reshape(-1).tolist()))
+                    if touched <= 64:
+                        small += 1
+                    else:
+                        large += 1
+                    _, w = ora.replay(base, COLS, pr)
+                    assert int(b["fit_prev"][i]) == w, (gen, isl, i, touched)
+    finally:
+        ga.close()
+    print(f"  evaluations: {small} small-tier, {large} large-tier", flush=True)
+    assert small > 0 and large > 0
