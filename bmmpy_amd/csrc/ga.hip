@@ -779,54 +779,94 @@ __global__ void ga_select_kernel(const float* __restrict__ score,
 // chosen moves are independent and apply in parallel; rounds over every index
 // bit and both phases cover every pair.  One workgroup per (island, row):
 // target rows score each source row by AND-popcount over the row words.
-__global__ __launch_bounds__(256) void ga_li_score_kernel(const uint64_t* __restrict__ base,
-                                                          int rows, int words,
-                                                          const uint32_t* __restrict__ roww,
-                                                          int bit, int phase,
-                                                          int32_t* __restrict__ move,
-                                                          uint32_t* __restrict__ neww) {
-    extern __shared__ uint64_t ri[];
-    __shared__ unsigned long long best;
-    const int isl = blockIdx.x / rows;
-    const int i = blockIdx.x % rows;
-    if (((i >> bit) & 1) != phase)
-        return;
-    const uint64_t* b = base + (size_t)isl * rows * words;
-    for (int w = threadIdx.x; w < words; w += blockDim.x)
-        ri[w] = b[(size_t)i * words + w];
-    if (threadIdx.x == 0)
-        best = ~0ull;
-    __syncthreads();
-    const uint32_t wi = roww[(size_t)isl * rows + i];
-    for (int j = threadIdx.x; j < rows; j += blockDim.x) {
-        if (((j >> bit) & 1) == phase)
-            continue;
-        const uint32_t wj = roww[(size_t)isl * rows + j];
-        if (wi == 0 || wj >= 2 * wi)
-            continue;
-        const uint64_t* rj = b + (size_t)j * words;
-        uint32_t ov = 0;
-        for (int w = 0; w < words; ++w)
-            ov += __builtin_popcountll(ri[w] & rj[w]);
-        const uint32_t cw = wi + wj - 2 * ov;
-        if (cw < wi)
-            atomicMin(&best, ((unsigned long long)cw << 32) | (unsigned)j);
+// Scores of one round as tiles of 32 targets x 32 sources per workgroup
+// (grid: 16 tile pairs x islands; the k-th target of a round is row k with
+// `phase` inserted at bit `bit`, the k-th source likewise with the other
+// phase).  Word chunks of both row sets are staged in LDS word-major and
+// each thread forms 2 x 2 overlaps pop(r_i & r_j); a pair with
+// cw = wt(r_i) + wt(r_j) - 2 pop(r_i & r_j) below wt(r_i) offers the key
+// cw << 32 | j, and atomicMin per target keeps the lightest, lowest-index
+// source over all tiles (keys[] stays ~0 when none improves; the apply
+// kernel consumes and resets it).
+constexpr int kLiTile = 32;
+constexpr int kLiChunk = 32;   // u64 words per LDS stage
+
+__global__ __launch_bounds__(256) void ga_li_tile_kernel(const uint64_t* __restrict__ base,
+                                                         int rows, int words,
+                                                         const uint32_t* __restrict__ roww,
+                                                         int bit, int phase,
+                                                         unsigned long long* __restrict__ keys) {
+    const int isl = blockIdx.y;
+    const int tt = blockIdx.x / 4, st = blockIdx.x % 4;
+    const int tid = threadIdx.x;
+    __shared__ uint64_t A[kLiChunk][kLiTile + 1];
+    __shared__ uint64_t B[kLiChunk][kLiTile + 1];
+    __shared__ int ra[kLiTile], rb[kLiTile];
+    __shared__ uint32_t wa[kLiTile], wb[kLiTile];
+    const uint32_t lowm = (1u << bit) - 1u;
+    if (tid < 2 * kLiTile) {
+        const int t = tid % kLiTile;
+        const bool tgt = tid < kLiTile;
+        const int k = (tgt ? tt : st) * kLiTile + t;
+        const int r = (int)((((uint32_t)k >> bit) << (bit + 1)) |
+                            ((uint32_t)(tgt ? phase : 1 - phase) << bit) | ((uint32_t)k & lowm));
+        const bool ok = r < rows;
+        (tgt ? ra : rb)[t] = ok ? r : -1;
+        (tgt ? wa : wb)[t] = ok ? roww[(size_t)isl * rows + r] : 0u;
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        const size_t id = (size_t)isl * rows + i;
-        if (best != ~0ull) {
-            move[id] = (int32_t)(best & 0xffffffffu);
-            neww[id] = (uint32_t)(best >> 32);
-        } else {
-            move[id] = -1;
+    if (ra[0] < 0 || rb[0] < 0)
+        return;   // rows grow with k: an empty first slot means an empty tile
+    const uint64_t* b = base + (size_t)isl * rows * words;
+    const int tx = tid % 16, ty = tid / 16;
+    uint32_t acc[2][2] = {};
+    for (int w0 = 0; w0 < words; w0 += kLiChunk) {
+        const int cw = min(kLiChunk, words - w0);
+        for (int q = tid; q < kLiTile * kLiChunk; q += blockDim.x) {
+            const int r = q / kLiChunk, w = q % kLiChunk;
+            A[w][r] = (w < cw && ra[r] >= 0) ? b[(size_t)ra[r] * words + w0 + w] : 0ull;
+            B[w][r] = (w < cw && rb[r] >= 0) ? b[(size_t)rb[r] * words + w0 + w] : 0ull;
         }
+        __syncthreads();
+#pragma unroll 4
+        for (int w = 0; w < cw; ++w) {
+            const uint64_t a0 = A[w][ty], a1 = A[w][ty + 16];
+            const uint64_t b0 = B[w][tx], b1 = B[w][tx + 16];
+            const uint64_t x00 = a0 & b0, x01 = a0 & b1, x10 = a1 & b0, x11 = a1 & b1;
+            acc[0][0] = bcnt_acc((uint32_t)(x00 >> 32), bcnt_acc((uint32_t)x00, acc[0][0]));
+            acc[0][1] = bcnt_acc((uint32_t)(x01 >> 32), bcnt_acc((uint32_t)x01, acc[0][1]));
+            acc[1][0] = bcnt_acc((uint32_t)(x10 >> 32), bcnt_acc((uint32_t)x10, acc[1][0]));
+            acc[1][1] = bcnt_acc((uint32_t)(x11 >> 32), bcnt_acc((uint32_t)x11, acc[1][1]));
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        const int a = ty + 16 * i;
+        unsigned long long key = ~0ull;
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const int c = tx + 16 * j;
+            const uint32_t wi = wa[a], wj = wb[c];
+            if (ra[a] >= 0 && rb[c] >= 0 && wi > 0u) {
+                const uint32_t cw = wi + wj - 2u * acc[i][j];
+                if (cw < wi)
+                    key = min(key, ((unsigned long long)cw << 32) | (unsigned)rb[c]);
+            }
+        }
+        // The 16 threads of one target row are 16 consecutive lanes.
+#pragma unroll
+        for (int off = 1; off < 16; off <<= 1)
+            key = min(key, (unsigned long long)__shfl_xor(key, off, kWave));
+        if (tx == 0 && key != ~0ull)
+            atomicMin(&keys[(size_t)isl * rows + ra[a]], key);
     }
 }
 
+// Applies a round's moves: r_i ^= r_j for each target i whose key names a
+// source j, with the new weight from the key; clears the key.
 __global__ void ga_li_apply_kernel(uint64_t* __restrict__ base, int rows, int words,
-                                   const int32_t* __restrict__ move,
-                                   const uint32_t* __restrict__ neww,
+                                   unsigned long long* __restrict__ keys,
                                    uint32_t* __restrict__ roww, int bit, int phase,
                                    unsigned int* __restrict__ changed) {
     const int isl = blockIdx.x / rows;
@@ -834,14 +874,17 @@ __global__ void ga_li_apply_kernel(uint64_t* __restrict__ base, int rows, int wo
     if (((i >> bit) & 1) != phase)
         return;
     const size_t id = (size_t)isl * rows + i;
-    const int j = move[id];
-    if (j < 0)
+    const unsigned long long key = keys[id];
+    if (key == ~0ull)
         return;
+    const int j = (int)(key & 0xffffffffu);
     uint64_t* b = base + (size_t)isl * rows * words;
     for (int w = threadIdx.x; w < words; w += blockDim.x)
         b[(size_t)i * words + w] ^= b[(size_t)j * words + w];
+    __syncthreads();   // every thread has read the key
     if (threadIdx.x == 0) {
-        roww[id] = neww[id];
+        roww[id] = (uint32_t)(key >> 32);
+        keys[id] = ~0ull;
         atomicAdd(changed, 1u);
     }
 }
@@ -1472,7 +1515,7 @@ struct Ga {
         w_snap, roww, sketch, evals, n_ops, base_total, eval_tier;
     // proposal model data
     DevBuf pairs, pair_isl, feats, labels, prng;
-    DevBuf li_move, li_w, li_changed;
+    DevBuf li_key, li_changed;
     DevBuf win_cand, win, win_move, win_w;
     DevBuf isd_r, isd_t, isd_w, isd_chosen, isd_count;
     DevBuf isl_state, mig;
@@ -1706,16 +1749,14 @@ unsigned int local_improvement(Ga& G) {
     for (int sweep = 0; sweep < G.P.li_sweeps; ++sweep)
         for (int bit = 0; bit < bits; ++bit)
             for (int phase = 0; phase < 2; ++phase) {
-                hipLaunchKernelGGL(ga_li_score_kernel, dim3((unsigned)(I * R)), dim3(256),
-                                   (size_t)G.words * 8, G.stream,
-                                   static_cast<const uint64_t*>(G.base.ptr), R, G.words,
+                // Up to 128 targets and 128 sources: 4 x 4 tiles of 32.
+                hipLaunchKernelGGL(ga_li_tile_kernel, dim3(16, (unsigned)I), dim3(256), 0,
+                                   G.stream, static_cast<const uint64_t*>(G.base.ptr), R, G.words,
                                    static_cast<const uint32_t*>(G.roww.ptr), bit, phase,
-                                   static_cast<int32_t*>(G.li_move.ptr),
-                                   static_cast<uint32_t*>(G.li_w.ptr));
+                                   static_cast<unsigned long long*>(G.li_key.ptr));
                 hipLaunchKernelGGL(ga_li_apply_kernel, dim3((unsigned)(I * R)), dim3(128),
                                    0, G.stream, static_cast<uint64_t*>(G.base.ptr), R, G.words,
-                                   static_cast<const int32_t*>(G.li_move.ptr),
-                                   static_cast<const uint32_t*>(G.li_w.ptr),
+                                   static_cast<unsigned long long*>(G.li_key.ptr),
                                    static_cast<uint32_t*>(G.roww.ptr), bit, phase, changed);
             }
     hipLaunchKernelGGL(ga_base_weight_kernel, dim3(I), dim3(256), 0, G.stream,
@@ -1959,8 +2000,9 @@ Ga* ga_create(const uint64_t* matrix, size_t rows, size_t cols, const bmm_ga_par
                       hipMemcpyHostToDevice));
     std::vector<int32_t> mig0(I, 3);
     BMM_HIP(hipMemcpy(G->mig.get<int32_t>(I), mig0.data(), (size_t)I * 4, hipMemcpyHostToDevice));
-    G->li_move.get<int32_t>((size_t)I * rows);
-    G->li_w.get<uint32_t>((size_t)I * rows);
+    // Pairwise-improvement keys, ~0 between rounds (the apply kernel resets them).
+    fill_u64(reinterpret_cast<uint64_t*>(G->li_key.get<unsigned long long>((size_t)I * rows)),
+             (size_t)I * rows, ~0ull, G->stream);
     G->li_changed.get<unsigned int>(1);
     const size_t n_win = P.win_rows && P.learned ? (size_t)I * rows * P.win_candidates : 0;
     const size_t n_pair_buf = std::max((size_t)n_pairs, n_win);
