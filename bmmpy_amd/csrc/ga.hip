@@ -1354,9 +1354,10 @@ __global__ __launch_bounds__(256) void ga_isd_rows_kernel(const uint64_t* __rest
 
 // GlobalGreedyApplier over the pool {R rows (masks T)} + {B rows (unit
 // masks)}: bitonic sort of (weight, pool index) keys, then one entry at a
-// time against a fully reduced basis in LDS (reduce by the pivots set in the
-// mask, pivot = highest remaining bit, clear that bit from the other basis
-// rows).  chosen[k] = pool index of the k-th kept entry.
+// time against a fully reduced basis held in one wavefront's registers
+// (reduce by the pivots set in the mask, pivot = highest remaining bit,
+// clear that bit from the other basis rows).  chosen[k] = pool index of the
+// k-th kept entry.
 __global__ __launch_bounds__(256) void ga_isd_select_kernel(int rows,
                                                             const uint64_t* __restrict__ tmask,
                                                             const uint32_t* __restrict__ rw,
@@ -1367,11 +1368,6 @@ __global__ __launch_bounds__(256) void ga_isd_select_kernel(int rows,
     const int wave = tid / kWave, lane = tid % kWave;
     __shared__ unsigned long long key[512];
     __shared__ uint64_t tm[256][kIsdMaskWords];
-    __shared__ uint64_t basis[256][kIsdMaskWords];
-    __shared__ uint8_t used[256];
-    __shared__ uint64_t red[4][kIsdMaskWords];
-    __shared__ uint64_t v_new[kIsdMaskWords];
-    __shared__ int s_piv, s_count, s_kept_isd;
     const int pool = 2 * rows;
     for (int k = tid; k < 512; k += blockDim.x) {
         unsigned long long kk = ~0ull;
@@ -1384,15 +1380,8 @@ __global__ __launch_bounds__(256) void ga_isd_select_kernel(int rows,
     }
     for (int k = tid; k < rows; k += blockDim.x) {
 #pragma unroll
-        for (int q = 0; q < kIsdMaskWords; ++q) {
+        for (int q = 0; q < kIsdMaskWords; ++q)
             tm[k][q] = tmask[((size_t)isl * rows + k) * kIsdMaskWords + q];
-            basis[k][q] = 0;
-        }
-        used[k] = 0;
-    }
-    if (tid == 0) {
-        s_count = 0;
-        s_kept_isd = 0;
     }
     __syncthreads();
     // Bitonic sort of 512 keys (ascending).
@@ -1410,67 +1399,73 @@ __global__ __launch_bounds__(256) void ga_isd_select_kernel(int rows,
             }
             __syncthreads();
         }
-    for (int e = 0; e < pool; ++e) {
-        if (s_count >= rows)
-            break;
+    // Selection on one wavefront, the basis in registers: lane l holds the
+    // rows whose pivot is bit 64 s + l (s < 4).  The basis stays fully
+    // reduced (no row holds another row's pivot bit), so an entry's mask m
+    // reduces to m ^ (XOR of the rows whose pivot bit m has) in one step.
+    if (wave != 0)
+        return;
+    uint64_t bas[kIsdMaskWords][kIsdMaskWords] = {};   // [slot][word]
+    bool used_s[kIsdMaskWords] = {};
+    int count = 0, kept_isd = 0;
+    for (int e = 0; e < pool && count < rows; ++e) {
         const int idx = (int)(key[e] & 1023u);
         uint64_t m[kIsdMaskWords];
 #pragma unroll
         for (int q = 0; q < kIsdMaskWords; ++q)
-            m[q] = idx < rows ? tm[idx][q] : ((idx - rows) / 64 == q ? 1ull << ((idx - rows) % 64) : 0ull);
-        // XOR of the basis rows whose pivot bit is set in m.
+            m[q] = idx < rows ? tm[idx][q]
+                              : ((idx - rows) / 64 == q ? 1ull << ((idx - rows) % 64) : 0ull);
         uint64_t x[kIsdMaskWords] = {};
-        if (tid < rows && used[tid] && ((m[tid / 64] >> (tid % 64)) & 1ull)) {
 #pragma unroll
-            for (int q = 0; q < kIsdMaskWords; ++q)
-                x[q] = basis[tid][q];
-        }
+        for (int sl = 0; sl < kIsdMaskWords; ++sl)
+            if (used_s[sl] && ((m[sl] >> lane) & 1ull)) {
+#pragma unroll
+                for (int q = 0; q < kIsdMaskWords; ++q)
+                    x[q] ^= bas[sl][q];
+            }
+        uint64_t v[kIsdMaskWords];
 #pragma unroll
         for (int q = 0; q < kIsdMaskWords; ++q) {
             for (int o = kWave / 2; o > 0; o >>= 1)
                 x[q] ^= __shfl_xor(x[q], o, kWave);
-            if (lane == 0)
-                red[wave][q] = x[q];
+            v[q] = m[q] ^ x[q];
         }
-        __syncthreads();
-        if (tid == 0) {
-            int p = -1;
-            for (int q = kIsdMaskWords - 1; q >= 0; --q) {
-                const uint64_t v = m[q] ^ red[0][q] ^ red[1][q] ^ red[2][q] ^ red[3][q];
-                v_new[q] = v;
-                if (p < 0 && v)
-                    p = q * 64 + 63 - __builtin_clzll(v);
-            }
-            s_piv = p;
-            if (p >= 0) {
-                chosen[(size_t)isl * rows + s_count] = (int16_t)idx;
-                ++s_count;
-                if (idx < rows)
-                    ++s_kept_isd;
-            }
-        }
-        __syncthreads();
-        const int p = s_piv;
-        if (p >= 0) {
-            // Clear bit p from the kept rows (p is not yet a pivot, so row p
-            // is unused here), then install the new row.
-            if (tid < rows && used[tid] && ((basis[tid][p / 64] >> (p % 64)) & 1ull)) {
+        int pv = -1;   // highest set bit of v (the same on every lane)
+#pragma unroll
+        for (int q = kIsdMaskWords - 1; q >= 0; --q)
+            if (pv < 0 && v[q])
+                pv = q * 64 + 63 - __builtin_clzll(v[q]);
+        if (pv < 0)
+            continue;   // dependent on the kept entries
+        const int ps = pv / 64, pl = pv % 64;
+        // Clear bit pv from the kept rows (pv is not yet a pivot), then
+        // install v as the row of pivot pv.
+#pragma unroll
+        for (int sl = 0; sl < kIsdMaskWords; ++sl) {
+            uint64_t hb = 0;
+#pragma unroll
+            for (int q = 0; q < kIsdMaskWords; ++q)
+                if (q == ps)
+                    hb = bas[sl][q];
+            if (used_s[sl] && ((hb >> pl) & 1ull)) {
 #pragma unroll
                 for (int q = 0; q < kIsdMaskWords; ++q)
-                    basis[tid][q] ^= v_new[q];
+                    bas[sl][q] ^= v[q];
             }
-            __syncthreads();
-            if (tid == 0) {
+            if (sl == ps && lane == pl) {
 #pragma unroll
                 for (int q = 0; q < kIsdMaskWords; ++q)
-                    basis[p][q] = v_new[q];
-                used[p] = 1;
+                    bas[sl][q] = v[q];
+                used_s[sl] = true;
             }
         }
-        __syncthreads();
+        if (lane == 0)
+            chosen[(size_t)isl * rows + count] = (int16_t)idx;
+        ++count;
+        kept_isd += idx < rows ? 1 : 0;
     }
-    if (tid == 0 && s_kept_isd)
-        atomicAdd(n_isd, (unsigned)s_kept_isd);
+    if (lane == 0 && kept_isd)
+        atomicAdd(n_isd, (unsigned)kept_isd);
 }
 
 // New base: row k = R row chosen[k] or B row chosen[k] - rows, into `out`.
