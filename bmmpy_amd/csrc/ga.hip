@@ -198,12 +198,14 @@ __device__ inline int ga_eval_task_of(unsigned int i, unsigned int n) {
 
 // Tiers of row slots in LDS: the first tier (one wavefront per task, 32
 // slots: many wavefronts per CU) takes every task whose ops touch at most 32
-// rows and flags the others 1; the second (128 slots) takes the tasks
-// flagged 1 and flags 2 those touching more; the last (every row) takes
-// those.  Early in an epoch op lists are long; once the information-set
-// trials have lightened the bases they stay a few ops long.
+// rows and flags the others 1; each later tier takes the tasks flagged with
+// its level and flags one level up those touching more rows than it holds;
+// the last holds every row.  Early in an epoch op lists are long; once the
+// information-set trials have lightened the bases they stay a few ops long.
 constexpr int kGaEvalSlots0 = 32;
-constexpr int kGaEvalSlots1 = 128;
+constexpr int kGaEvalTiers = 4;   // later tiers: 64, 128, 192 and all rows
+constexpr int kGaEvalSlots[kGaEvalTiers] = {64, 128, 192, 256};
+constexpr int kGaEvalPerCu[kGaEvalTiers] = {8, 4, 3, 2};   // wavefronts per CU (LDS)
 
 __global__ __launch_bounds__(64) void ga_eval_kernel(const uint32_t* __restrict__ base,
                                                      const uint16_t* __restrict__ ops,
@@ -1597,21 +1599,23 @@ void enqueue_generation(Ga& G, int parity, hipStream_t s) {
                        static_cast<const int32_t*>(G.len[parity].ptr), Pp, G.P.rows, G.W32,
                        G.P.max_ops, fit, s0, tier);
     BMM_HIP(hipGetLastError());
-    // Later tiers on fixed grids: four (128 slots) and two (every row)
-    // wavefronts per CU.
+    // Later tiers on fixed grids, as many wavefronts per CU as their LDS
+    // allows; a tier no larger than the previous one is not needed.
     const int cus = default_persistent_wgs() / 8;
-    const int levels[2] = {1, 2};
-    const int slots[2] = {std::min(G.P.rows, kGaEvalSlots1), G.P.rows};
-    const int per_cu[2] = {4, 2};
-    for (int k = 0; k < 2; ++k) {
-        if (slots[k] <= (k == 0 ? s0 : slots[0]))
-            break;   // every task fits an earlier tier
-        hipLaunchKernelGGL(ga_eval_tier_kernel, dim3((unsigned)std::min(n_tasks, cus * per_cu[k])),
-                           dim3(64), lds(slots[k]), s, static_cast<const uint32_t*>(G.base.ptr),
+    int prev = s0, level = 1;
+    for (int k = 0; k < kGaEvalTiers && prev < G.P.rows; ++k) {
+        const int slots = std::min(G.P.rows, kGaEvalSlots[k]);
+        if (slots <= prev)
+            continue;
+        hipLaunchKernelGGL(ga_eval_tier_kernel,
+                           dim3((unsigned)std::min(n_tasks, cus * kGaEvalPerCu[k])), dim3(64),
+                           lds(slots), s, static_cast<const uint32_t*>(G.base.ptr),
                            static_cast<const uint16_t*>(G.ops[parity].ptr),
                            static_cast<const int32_t*>(G.len[parity].ptr), Pp, G.P.rows, G.W32,
-                           G.P.max_ops, n_tasks, slots[k], levels[k], fit, tier);
+                           G.P.max_ops, n_tasks, slots, level, fit, tier);
         BMM_HIP(hipGetLastError());
+        prev = slots;
+        ++level;
     }
     hipLaunchKernelGGL(ga_breed_kernel, dim3(I), dim3(std::max(64, (Pp + 63) / 64 * 64)),
                        breed_lds_bytes(G.P), s, dev_params(G), fit,
