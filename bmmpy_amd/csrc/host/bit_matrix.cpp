@@ -335,6 +335,27 @@ std::uint64_t RowWindow::weight() const {
     return n;
 }
 
+std::uint64_t RowWindow::total_weight() const {
+    const std::size_t W = words_per_row();
+    const std::size_t tail = cols() % 64;
+    std::uint64_t n = 0;
+    for (std::size_t w = 0; w < W; ++w) {
+        std::uint64_t any = 0;
+        for (std::size_t r : _rows)
+            any |= _matrix->row_words(r)[w];
+        if (w + 1 == W && tail)
+            any &= (1ull << tail) - 1;
+        n += (std::uint64_t)__builtin_popcountll(any);
+    }
+    return n;
+}
+
+void RowWindow::assign_materialized(const BitMatrix& rows) {
+    if (rows.rows() != _rows.size())
+        throw MatrixError(MatrixErr::DimensionMismatch);
+    require_mutable().insert_rows_by_indices(rows, _rows);
+}
+
 bool RowWindow::get(std::size_t local_row, std::size_t col) const {
     return _matrix->get(global_row(local_row), col);
 }

@@ -134,6 +134,12 @@ public:
     const std::uint64_t* row_words(std::size_t local_row) const;
     std::uint64_t row_popcount(std::size_t local_row) const;
     std::uint64_t weight() const;
+    // Columns set in at least one row of the window: the popcount of the OR
+    // of its rows (src/core/row_window.cpp:67-88).
+    std::uint64_t total_weight() const;
+    // Writes rows.rows() == size() rows over the window's rows, in order
+    // (src/core/row_window.cpp:109-115).
+    void assign_materialized(const BitMatrix& rows);
     bool get(std::size_t local_row, std::size_t col) const;
     void set(std::size_t local_row, std::size_t col, bool value);
     void row_xor(std::size_t target_local_row, std::size_t source_local_row);

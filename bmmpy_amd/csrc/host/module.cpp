@@ -247,7 +247,15 @@ PYBIND11_MODULE(_bmmpy, m) {
         .def("global_row", &RowWindow::global_row, py::arg("local_row"))
         .def("row_popcount", &RowWindow::row_popcount, py::arg("local_row"))
         .def("weight", &RowWindow::weight)
-        .def("total_weight", &RowWindow::weight)
+        .def_property_readonly("total_weight", &RowWindow::total_weight)
+        .def_property_readonly("shape",
+                               [](const RowWindow& w) { return py::make_tuple(w.size(), w.cols()); })
+        .def("__repr__",
+             [](const RowWindow& w) {
+                 return "RowWindow(size=" + std::to_string(w.size()) +
+                        ", cols=" + std::to_string(w.cols()) + ")";
+             })
+        .def("assign_materialized", &RowWindow::assign_materialized, py::arg("rows"))
         .def("get", &RowWindow::get, py::arg("local_row"), py::arg("col"))
         .def("set", &RowWindow::set, py::arg("local_row"), py::arg("col"), py::arg("value"))
         .def("row_xor", &RowWindow::row_xor, py::arg("target_local_row"),
@@ -269,7 +277,16 @@ PYBIND11_MODULE(_bmmpy, m) {
         .def("mask_popcount", &Candidate::mask_popcount)
         .def("mask_u64", &Candidate::mask_u64)
         .def("selected_rows", &Candidate::selected_rows)
-        .def_static("from_u64", &Candidate::from_u64, py::arg("mask"), py::arg("weight"));
+        .def("__iter__",
+             [](const Candidate& c) {
+                 return py::iter(py::cast(c.selected_rows()));
+             })
+        .def_static("from_u64", &Candidate::from_u64, py::arg("mask"), py::arg("weight"))
+        .def_static("from_words",
+                    [](const Candidate::mask_type& mask, std::uint32_t weight) {
+                        return Candidate(mask, weight);
+                    },
+                    py::arg("mask"), py::arg("weight"));
 
     py::class_<FwhtSearchConfig>(m, "FwhtSearchConfig")
         .def(py::init<>())
@@ -399,6 +416,7 @@ PYBIND11_MODULE(_bmmpy, m) {
     py::class_<RuntimeFeatures>(m, "RuntimeFeatures")
         .def_readonly("avx2_compiled", &RuntimeFeatures::avx2_compiled)
         .def_readonly("avx2_supported", &RuntimeFeatures::avx2_supported)
+        .def_readonly("avx2_available", &RuntimeFeatures::avx2_supported)
         .def_readonly("avx2_enabled", &RuntimeFeatures::avx2_enabled)
         .def_readonly("cuda_compiled", &RuntimeFeatures::cuda_compiled)
         .def_readonly("cuda_available", &RuntimeFeatures::cuda_available)

@@ -11,6 +11,9 @@ namespace bmmpy {
 RuntimeFeatures get_runtime_features() {
     RuntimeFeatures f;
     f.max_threads = (int)std::max(1u, std::thread::hardware_concurrency());
+    // What the host CPU offers (src/stub.cpp:24); no host path here uses it.
+    __builtin_cpu_init();
+    f.avx2_supported = __builtin_cpu_supports("avx2") != 0;
     bmm_device_info info{};
     if (bmm_hip_device_info(&info) == BMM_OK && info.device_count > 0) {
         f.hip_available = true;

@@ -70,6 +70,7 @@ def test_runtime_features():
     f = bmm.get_runtime_features()
     assert f.hip_compiled and not f.cuda_compiled and not f.cuda_available
     assert f.hip_available == (f.device_count > 0)
+    assert isinstance(f.avx2_available, bool) and not f.avx2_compiled
 
 
 def test_bitmatrix_layout_and_ops():
@@ -100,9 +101,39 @@ def test_bitmatrix_layout_and_ops():
         i3.mul(bmm.BitMatrix(2, 3))
 
 
+def test_row_window_metadata_and_assignment():
+    """total_weight counts the columns any row of the window sets (an OR of
+    the rows, src/core/row_window.cpp:67-88), not the sum of row weights;
+    assign_materialized writes rows back in window order (:109-115)."""
+    m = bmm.BitMatrix(4, 130)
+    for r, c in [(0, 0), (0, 129), (1, 129), (1, 64), (2, 3), (3, 100)]:
+        m[r, c] = True
+    w = m.row_window([1, 0])
+    assert [w.row_popcount(i) for i in range(2)] == [2, 2] and w.weight() == 4
+    assert w.total_weight == 3                 # columns 0, 64, 129
+    assert m.row_window([0, 1, 2, 3]).total_weight == 5
+    assert m.row_window([]).total_weight == 0
+    assert w.shape == (2, 130) and repr(w) == "RowWindow(size=2, cols=130)"
+    new = bmm.BitMatrix(2, 130)
+    new[0, 5] = True                           # goes to global row 1
+    new[1, 7] = True                           # goes to global row 0
+    w.assign_materialized(new)
+    assert m.to_rows()[1][5] == "1" and m.row_popcount(1) == 1
+    assert m.to_rows()[0][7] == "1" and m.row_popcount(0) == 1
+    assert w.materialize().to_rows() == new.to_rows()
+    assert m.row_popcount(2) == 1 and m.row_popcount(3) == 1   # untouched
+    with pytest.raises(bmm.MatrixError):
+        w.assign_materialized(bmm.BitMatrix(3, 130))
+    with pytest.raises(bmm.MatrixError):
+        w.assign_materialized(bmm.BitMatrix(2, 129))
+
+
 def test_candidate_api():
     c = bmm.Candidate.from_u64(0b1011, 7)
     assert c.rows == [0, 1, 3] and len(c) == 3 and 3 in c and 2 not in c
+    assert list(c) == [0, 1, 3] and sorted(iter(c)) == c.selected_rows()
+    wide = bmm.Candidate.from_words([0b10, 1 << 5], 9)
+    assert list(wide) == [1, 69] and wide.weight == 9 and 69 in wide and 68 not in wide
     big = bmm.Candidate([1, 2], 3)
     with pytest.raises(RuntimeError):
         big.mask_u64()
