@@ -68,8 +68,8 @@ TRAFFIC_BYTES = {
     # The kernels of one calc_scores_and_order call on 2^28 entries (scores,
     # the three table-scan kernels, the tile order), FETCH_SIZE x 2.0 plus
     # WRITE_SIZE x 1.0 in KiB.
-    "scores_order_2p28": ((524422.2 + 32791.5 + 261.3 + 33051.0 + 557233.6) * 2.0
-                          + 1118208.0 + 512.0 + 512.0 + 65536.0 + 1079401.8) * 1024,
+    "scores_order_2p28": ((524412.9 + 16406.5 + 261.3 + 16670.0 + 557224.6) * 2.0
+                          + 1085440.1 + 512.0 + 512.0 + 65536.0 + 1076899.8) * 1024,
 }
 
 
@@ -484,10 +484,11 @@ def run_scores(torch, steps: int, warmup: int) -> dict:
     alg = 12.0 * m
     achieved = alg / (kernel_ms / 1e3) / 1e9
     # Moved by the kernels (10-bit scores, one-pass tile order): scores (read
-    # h, write s: 8 B, plus the per-tile score histogram, 0.25 B), the table
-    # scan (0.75 B) and the tile order (read s, write the order: 8 B, plus the
-    # tile's table row): about 17.2 B per mask.  rocprof's FETCH_SIZE and
-    # WRITE_SIZE of the kernels (copy-calibrated) give the figure used.
+    # h, write s: 8 B, plus the per-tile 16-bit score counts, 0.125 B), the
+    # table scan (read the counts twice, write the 32-bit starts: 0.5 B) and
+    # the tile order (read s, write the order: 8 B, plus the tile's starts):
+    # about 17 B per mask.  rocprof's FETCH_SIZE and WRITE_SIZE of the
+    # kernels (copy-calibrated) give the figure used.
     moved = TRAFFIC_BYTES["scores_order_2p28"]
     return {
         "ms": 1e3 * dt, "evals": float(m - 1),

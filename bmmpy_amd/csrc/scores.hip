@@ -427,8 +427,9 @@ __global__ __launch_bounds__(256) void scan_add_kernel(uint32_t* __restrict__ da
 // ---------------------------------------------------- one-pass order ----
 // Scores of at most 10 bits (n < 1024) are ordered in one pass over tiles
 // of 2^TB masks: the scores kernel writes each tile's score histogram as a
-// row of the table (table[tile * K + k]), a column scan in chunks of tiles
-// turns it into the global start of every (score, tile) run, and each tile
+// row of 16-bit counts (counts[tile * Kp + k]; a tile holds at most 2^TB <=
+// 2^16 masks of a score), a column scan in chunks of tiles turns them into
+// the 32-bit global start of every (score, tile) run, and each tile
 // is sorted stably in LDS (two digit passes on the packed word
 // score << TB | offset; the offsets never leave LDS) and written out run by
 // run.  Per mask the
@@ -443,7 +444,8 @@ __global__ __launch_bounds__(256) void scores_tile_kernel(const S* __restrict__ 
                                                           unsigned int* __restrict__ cnt,
                                                           unsigned int* __restrict__ overflow,
                                                           uint32_t n_tiles,
-                                                          uint32_t* __restrict__ table) {
+                                                          uint16_t* __restrict__ counts) {
+    static_assert(TB <= 16, "a tile's count of one score must fit 16 bits");
     using V = typename Vec4<S>::T;
     constexpr int kVecs = 4;                          // 4 vectors of 4 entries per thread
     constexpr int kSubs = (1 << TB) / (256 * 4 * kVecs);
@@ -515,7 +517,7 @@ __global__ __launch_bounds__(256) void scores_tile_kernel(const S* __restrict__ 
         const int Kp = (K + 3) & ~3;
         for (int q = threadIdx.x; q < Kp; q += blockDim.x) {
             const unsigned int c = q < K ? th[q] : 0u;
-            table[(size_t)tile * Kp + q] = c;
+            counts[(size_t)tile * Kp + q] = (uint16_t)c;
             if (q < K)
                 tot[q] += c;
         }
@@ -531,16 +533,28 @@ __global__ __launch_bounds__(256) void scores_tile_kernel(const S* __restrict__ 
 // hit distinct banks); digit counters as rows of T thread columns, two
 // 16-bit counts per word (digits 2r and 2r + 1 in row r), one pad word per
 // 16 columns.
-// Column scan of the table (rows = tiles, Kp = K rounded up to 4 words per
-// row, 4 keys per thread as one 16-byte access) in chunks of kScanRows rows:
-// col_sum_kernel totals each chunk's columns, col_base_kernel turns the chunk
-// totals into chunk starts (key-major: all of key k before key k + 1), and
-// col_start_kernel walks each chunk's rows, replacing every count by its
-// run's global start.  Rows are read 16 at a time so the loads overlap.
+// Column scan of the counts (rows = tiles, Kp = K rounded up to 4 entries
+// per row, 4 keys per thread as one 8-byte access) in chunks of kScanRows
+// rows: col_sum_kernel totals each chunk's columns, col_base_kernel turns the
+// chunk totals into chunk starts (key-major: all of key k before key k + 1),
+// and col_start_kernel walks each chunk's rows, writing every count's run
+// start (32-bit, one 16-byte access per 4 keys) to the start table.  Rows are
+// read 16 at a time so the loads overlap.
 constexpr int kScanRows = 128;
 constexpr int kScanBatch = 16;
 
-__global__ __launch_bounds__(256) void col_sum_kernel(const uint32_t* __restrict__ table, int Kp,
+// Four 16-bit counts: one 8-byte access, then as 32-bit values (zero when
+// the row is not taken).
+__device__ inline uint2 load_counts2(const uint16_t* p) {
+    return *reinterpret_cast<const uint2*>(p);
+}
+__device__ inline uint4 unpack_counts4(uint2 v, bool take) {
+    if (!take)
+        v = make_uint2(0, 0);
+    return make_uint4(v.x & 0xffffu, v.x >> 16, v.y & 0xffffu, v.y >> 16);
+}
+
+__global__ __launch_bounds__(256) void col_sum_kernel(const uint16_t* __restrict__ counts, int Kp,
                                                       uint32_t n_rows,
                                                       uint32_t* __restrict__ csum) {
     const uint32_t r0 = blockIdx.x * kScanRows, r1 = min(r0 + kScanRows, n_rows);
@@ -549,17 +563,19 @@ __global__ __launch_bounds__(256) void col_sum_kernel(const uint32_t* __restrict
         return;
     uint4 acc = make_uint4(0, 0, 0, 0);
     for (uint32_t r = r0; r < r1; r += kScanBatch) {
-        uint4 v[kScanBatch];
+        // Unconditional loads at clamped rows (all 16 in flight), rows past
+        // the chunk's end masked afterwards.
+        uint2 v[kScanBatch];
 #pragma unroll
         for (int i = 0; i < kScanBatch; ++i)
-            v[i] = r + i < r1 ? *reinterpret_cast<const uint4*>(table + (size_t)(r + i) * Kp + k4)
-                              : make_uint4(0, 0, 0, 0);
+            v[i] = load_counts2(counts + (size_t)min(r + i, r1 - 1) * Kp + k4);
 #pragma unroll
         for (int i = 0; i < kScanBatch; ++i) {
-            acc.x += v[i].x;
-            acc.y += v[i].y;
-            acc.z += v[i].z;
-            acc.w += v[i].w;
+            const uint4 c = unpack_counts4(v[i], r + i < r1);
+            acc.x += c.x;
+            acc.y += c.y;
+            acc.z += c.z;
+            acc.w += c.w;
         }
     }
     *reinterpret_cast<uint4*>(csum + (size_t)blockIdx.x * Kp + k4) = acc;
@@ -658,28 +674,29 @@ __global__ __launch_bounds__(1024) void col_base_kernel(uint32_t* __restrict__ c
     }
 }
 
-__global__ __launch_bounds__(256) void col_start_kernel(uint32_t* __restrict__ table, int Kp,
-                                                        uint32_t n_rows,
-                                                        const uint32_t* __restrict__ csum) {
+__global__ __launch_bounds__(256) void col_start_kernel(const uint16_t* __restrict__ counts,
+                                                        int Kp, uint32_t n_rows,
+                                                        const uint32_t* __restrict__ csum,
+                                                        uint32_t* __restrict__ start) {
     const uint32_t r0 = blockIdx.x * kScanRows, r1 = min(r0 + kScanRows, n_rows);
     const int k4 = (int)threadIdx.x * 4;
     if (k4 >= Kp)
         return;
     uint4 run = *reinterpret_cast<const uint4*>(csum + (size_t)blockIdx.x * Kp + k4);
     for (uint32_t r = r0; r < r1; r += kScanBatch) {
-        uint4 v[kScanBatch];
+        uint2 v[kScanBatch];
 #pragma unroll
         for (int i = 0; i < kScanBatch; ++i)
-            v[i] = r + i < r1 ? *reinterpret_cast<const uint4*>(table + (size_t)(r + i) * Kp + k4)
-                              : make_uint4(0, 0, 0, 0);
+            v[i] = load_counts2(counts + (size_t)min(r + i, r1 - 1) * Kp + k4);
 #pragma unroll
         for (int i = 0; i < kScanBatch; ++i)
             if (r + i < r1) {
-                *reinterpret_cast<uint4*>(table + (size_t)(r + i) * Kp + k4) = run;
-                run.x += v[i].x;
-                run.y += v[i].y;
-                run.z += v[i].z;
-                run.w += v[i].w;
+                *reinterpret_cast<uint4*>(start + (size_t)(r + i) * Kp + k4) = run;
+                const uint4 c = unpack_counts4(v[i], true);
+                run.x += c.x;
+                run.y += c.y;
+                run.z += c.z;
+                run.w += c.w;
             }
     }
 }
@@ -890,7 +907,7 @@ __global__ __launch_bounds__((1 << TB) / 32) void order_tile_kernel(
 }
 
 struct ScoresWorkspace {
-    DevBuf h, s, cnt, off, overflow, hist, order, table, csum;
+    DevBuf h, s, cnt, off, overflow, hist, order, table, counts, csum;
     DevBuf keys[2], vals[2];
     std::array<DevBuf, 6> levels;
 };
@@ -957,13 +974,14 @@ ScoresDev<S> scores_order_dev(const S* d_h, size_t m, int32_t n) {
         constexpr int TB = kOrderTileBits;
         const uint32_t tiles = (uint32_t)((m + ((uint64_t)1 << TB) - 1) >> TB);
         uint32_t* d_tab = ws.table.get<uint32_t>(((buckets + 3) & ~(size_t)3) * tiles);
+        uint16_t* d_tc = ws.counts.get<uint16_t>(((buckets + 3) & ~(size_t)3) * tiles);
         fill_u32(d_cnt, buckets, 0, nullptr);
         fill_u32(d_ovf, 1, 0, nullptr);
         {
             ScopedKernelTimer timer(nullptr);   // scores, counts, per-tile key histograms
             hipLaunchKernelGGL((scores_tile_kernel<S, TB>), dim3(std::min(tiles, kScoresGrid)),
                                dim3(256), 2 * buckets * 4, nullptr, d_h, (uint64_t)m, n, d_s,
-                               d_cnt, d_ovf, tiles, d_tab);
+                               d_cnt, d_ovf, tiles, d_tc);
             BMM_HIP(hipGetLastError());
         }
         uint32_t ovf = 0;
@@ -976,11 +994,11 @@ ScoresDev<S> scores_order_dev(const S* d_h, size_t m, int32_t n) {
             const uint32_t chunks = (tiles + kScanRows - 1) / kScanRows;
             const int Kp = (int)((buckets + 3) & ~(size_t)3);
             uint32_t* d_cs = ws.csum.get<uint32_t>((size_t)chunks * Kp);
-            hipLaunchKernelGGL(col_sum_kernel, dim3(chunks), dim3(256), 0, nullptr, d_tab, Kp,
+            hipLaunchKernelGGL(col_sum_kernel, dim3(chunks), dim3(256), 0, nullptr, d_tc, Kp,
                                tiles, d_cs);
             hipLaunchKernelGGL(col_base_kernel, dim3(1), dim3(1024), 0, nullptr, d_cs, Kp, chunks);
-            hipLaunchKernelGGL(col_start_kernel, dim3(chunks), dim3(256), 0, nullptr, d_tab, Kp,
-                               tiles, d_cs);
+            hipLaunchKernelGGL(col_start_kernel, dim3(chunks), dim3(256), 0, nullptr, d_tc, Kp,
+                               tiles, d_cs, d_tab);
             BMM_HIP(hipGetLastError());
             switch (kbits) {
 #define BMM_KB(KB)                                                                             \
