@@ -399,12 +399,13 @@ def run_basis(torch, steps: int, warmup: int) -> dict:
     # last-level cache, so it is not an HBM measurement.  The array FWHT's
     # HBM roofline is taken on 2^28 int32 (1 GiB).  Algorithmic bytes: 16 per
     # element (two out-of-LDS passes, each one read and one write of 4
-    # bytes).  The kernels move 8 bytes per element per pass: 14 index bits
-    # in one LDS pass, then 14 in two register passes (7 + 7).
+    # bytes).  The kernels take exactly two passes at this size (index bits
+    # 0-15 in 2^16-element register tiles, then bits 16-27), 8 bytes per
+    # element each.
     big = torch.zeros(1 << FWHT_HBM_BITS, dtype=torch.int32, device="cuda:0")
     stream = torch.cuda.current_stream()
     sp = stream.cuda_stream
-    fwht_passes = 1 + -(-(FWHT_HBM_BITS - 14) // 7)
+    fwht_passes = 2
     fwht_bytes = 16.0 * (1 << FWHT_HBM_BITS)
     fwht_ms = _time(torch, stream,
                     lambda: _lib.call("bmm_hip_fwht_i32_dev", big.data_ptr(), big.numel(), sp),
@@ -425,8 +426,8 @@ def run_basis(torch, steps: int, warmup: int) -> dict:
                      "frac": achieved / 8000.0,
                      "traffic": fwht_passes * 8.0 * (1 << FWHT_HBM_BITS),
                      "bytes_per_launch": fwht_bytes,
-                     "kernel": f"array FWHT of 2^{FWHT_HBM_BITS} int32: fwht_chunk_kernel<int,14> "
-                               f"+ {fwht_passes - 1} x fwht_stride_buf_kernel<int,7>",
+                     "kernel": f"array FWHT of 2^{FWHT_HBM_BITS} int32: fwht_lo16_kernel "
+                               "+ fwht_hi12_kernel",
                      "kernel_avg_ms": fwht_ms,
                      "solve_fwht_2^24_avg_ms": kernel_avg_ms, "solve_fwht_launches": nk,
                      "copy_1gib_gbps": copy_gbps,

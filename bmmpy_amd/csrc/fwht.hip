@@ -4,7 +4,9 @@
 //    arrays over 2 GiB): the stand-alone in-place transform of a device
 //    array (fwht_inplace, src/math/fwht.cpp:28-57):
 //    one LDS pass over the low 14 index bits, then register passes of up to
-//    7 bits with coalesced 256-byte wavefront accesses.
+//    7 bits with coalesced 256-byte wavefront accesses.  Aligned 2^28 int32
+//    arrays take two passes of 2^16-element register tiles instead
+//    (fwht_lo16_kernel, fwht_hi12_kernel).
 //
 // 2. column_patterns_kernel: per-column row patterns of a small matrix.
 //
@@ -33,7 +35,8 @@ namespace bmmhip {
 //     evenly: lane c of a wavefront owns column c of a 2^R x 64 tile at row
 //     stride 2^lo, so each of its 2^R loads and stores is one coalesced
 //     256-byte wavefront access, and the R stages run in registers with no
-//     LDS or barriers.  2^28 int32 takes three passes (14 + 7 + 7 bits).
+//     LDS or barriers.  2^27 int32 takes three passes (14 + 7 + 6 bits);
+//     2^28 int32 takes the two-pass tile kernels below.
 // Each pass reads and writes the array once (8 bytes per int32 element).
 // int16 arrays are computed in 32-bit lanes and truncated on store: the
 // transform is a ring homomorphism, so the result equals int16 wrapping.
@@ -157,6 +160,175 @@ __global__ __launch_bounds__(256) void fwht_stride_buf_kernel(T* __restrict__ da
         buf_store<T>(v[e], rs, vbase, (uint32_t)e * stride);
 }
 
+// ------------------------------------------- two passes for 2^28 int32 --
+// A 1 GiB int32 array in two passes of 2^16-element tiles held in registers
+// (1024 threads x 64 values = 256 KiB, one workgroup per CU), each tile
+// reshuffled between register layouts through LDS in rounds of 128 KiB:
+//   * fwht_lo16_kernel: index bits 0-15 of a contiguous 256 KiB tile
+//     (16-byte loads, 1 KiB per wavefront access; 4-byte stores, 256 bytes
+//     per wavefront access), three register layouts;
+//   * fwht_hi12_kernel: index bits 16-27 of a slice of 16 columns (64-byte
+//     row segments; the two slices of a 128-byte line are neighbouring tiles
+//     on one XCD); stages on row bits 6-11 in the load layout and 0-5 after
+//     the exchange.
+// Each pass reads and writes the array once: 16 B per element in all,
+// against 24 B for the LDS pass plus two register passes.  With one
+// workgroup per CU the transform work of a tile does not overlap its own
+// loads; the CUs drift apart, and those in their memory phase keep HBM busy.
+typedef int i32x4 __attribute__((ext_vector_type(4)));
+constexpr int kTileThreads = 1024;
+
+// Butterflies over the index bits of v[] that are set in MASK.
+template <int MASK, int E> __device__ inline void reg_fwht_bits(uint32_t (&v)[E]) {
+#pragma unroll
+    for (int len = 1; len < E; len <<= 1)
+        if (MASK & len)
+#pragma unroll
+            for (int i = 0; i < E; ++i)
+                if ((i & len) == 0) {
+                    const uint32_t a = v[i], b = v[i + len];
+                    v[i] = a + b;
+                    v[i + len] = a - b;
+                }
+}
+
+// Pass 1: element e of tile t is data[t 2^16 + e].  Three register
+// layouts, 64 values per thread:
+//   A: v[q + 4 j] = e = q + 4 l + 256 w + 4096 j   (stages on bits 0, 1, 12-15)
+//   B: b[k] = (l & 3) + 4 (k & 31) + 128 (l >> 2) + 2048 w + 32768 (k >> 5)
+//                                                 (stages on bits 2-6)
+//   C: c[m] = l + 64 m + 4096 w                    (stages on bits 7-11)
+// (l = lane, w = wave).  Each exchange runs in two rounds on a bit held in
+// registers on both sides (A -> B on bit 15, B -> C on bit 6), so in a
+// round every thread writes 32 values and reads 32: at most 64 values are
+// live.  A round's 2^15 elements sit at word e' + 4 (e' >> 7) (first
+// exchange) or e' + 4 (e' >> 6) (second), e' = e without the round bit:
+// the B-layout accesses (lanes over bits 0, 1, 7-10) then fall on 32
+// distinct banks modulo 2, the A writes are 16-byte aligned, the C reads
+// consecutive.  Every slot is a per-thread base plus a constant.
+constexpr int kLo16LdsWords = 32768 + 2048;
+
+__global__ __launch_bounds__(kTileThreads) void fwht_lo16_kernel(int32_t* __restrict__ data,
+                                                                 uint32_t bytes) {
+    __shared__ __align__(16) uint32_t lds[kLo16LdsWords];
+    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(data, 0, (int)bytes,
+                                                                        0x00020000);
+    const uint32_t l = threadIdx.x & 63u, w = threadIdx.x >> 6;
+    const uint32_t base = xcd_tile(blockIdx.x, gridDim.x) << 16;
+    uint32_t v[64];
+    const uint32_t va = (base + (l << 2) + (w << 8)) * 4u;
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+        const i32x4 a = __builtin_amdgcn_raw_buffer_load_b128(rs, (int)va, j << 14, 0);
+        v[4 * j] = (uint32_t)a[0];
+        v[4 * j + 1] = (uint32_t)a[1];
+        v[4 * j + 2] = (uint32_t)a[2];
+        v[4 * j + 3] = (uint32_t)a[3];
+    }
+    reg_fwht_bits<63>(v);
+    const uint32_t lb = (l >> 2) & 15u;
+    uint32_t b[64];
+    {
+        uint32_t* const sa = lds + (l << 2) + 264u * w + 4u * (l >> 5);   // + 4224 (j & 7)
+        const uint32_t* const sb = lds + (l & 3u) + (lb << 7) + (w << 11) + 4u * (lb | (w << 4));
+#pragma unroll
+        for (int rho = 0; rho < 2; ++rho) {
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const int jj = j + 8 * rho;
+                *reinterpret_cast<uint4*>(sa + 4224 * j) =
+                    make_uint4(v[4 * jj], v[4 * jj + 1], v[4 * jj + 2], v[4 * jj + 3]);
+            }
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < 32; ++k)
+                b[k + 32 * rho] = sb[4 * k];
+            __syncthreads();
+        }
+    }
+    reg_fwht_bits<31>(b);
+    uint32_t c[64];
+    {
+        uint32_t* const sb2 = lds + (l & 3u) + (lb << 6) + (w << 10) + 4u * (lb | (w << 4));
+        const uint32_t* const sc = lds + l + (w << 11) + (w << 7);   // + 68 (m >> 1)
+#pragma unroll
+        for (int rho = 0; rho < 2; ++rho) {
+#pragma unroll
+            for (int k = 0; k < 64; ++k)
+                if (((k >> 4) & 1) == rho)
+                    sb2[4 * (k & 15) + 17408 * (k >> 5)] = b[k];
+            __syncthreads();
+#pragma unroll
+            for (int mh = 0; mh < 32; ++mh)
+                c[2 * mh + rho] = sc[68 * mh];
+            if (rho == 0)
+                __syncthreads();
+        }
+    }
+    reg_fwht_bits<62>(c);
+    const uint32_t vc = (base + l + (w << 12)) * 4u;
+#pragma unroll
+    for (int m = 0; m < 64; ++m)
+        __builtin_amdgcn_raw_buffer_store_b32((int)c[m], rs, (int)vc, m << 8, 0);
+}
+
+// Pass 2: tile s holds rows r < 2^12 (index bits 16-27) of columns
+// 16 s + col (col < 16): element data[r 2^16 + 16 s + col].  A round of the
+// exchange holds the rows with one value of row bit 5 at word
+// x = e' + 16 (e' >> 9), e' = col + 16 (r & 31) + 512 (r >> 6): the reads of
+// the second layout (lanes over col and row bits 6, 7) fall on 32 distinct
+// banks modulo 2.
+constexpr int kHi12LdsWords = 32768 + 1024;
+
+__device__ inline uint32_t hi12_slot(uint32_t col, uint32_t r0_4, uint32_t r6_11) {
+    const uint32_t x = col | (r0_4 << 4) | (r6_11 << 9);
+    return x + 16u * (x >> 9);
+}
+
+__global__ __launch_bounds__(kTileThreads) void fwht_hi12_kernel(int32_t* __restrict__ data,
+                                                                 uint32_t bytes) {
+    __shared__ uint32_t lds[kHi12LdsWords];
+    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(data, 0, (int)bytes,
+                                                                        0x00020000);
+    const uint32_t l = threadIdx.x & 63u, w = threadIdx.x >> 6;
+    const uint32_t col = l & 15u;
+    const uint32_t s = xcd_tile(blockIdx.x, gridDim.x);
+    // Layout A: v[j] = row rt + 64 j with rt = (l >> 4) + 4 w (row bits 0-5
+    // from the thread, 6-11 from j); one wavefront access covers 4 rows of
+    // 64 bytes.
+    const uint32_t rt = (l >> 4) | (w << 2);
+    uint32_t v[64];
+    const uint32_t va = ((rt << 16) + (s << 4) + col) * 4u;
+#pragma unroll
+    for (int j = 0; j < 64; ++j)
+        v[j] = (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(rs, (int)va, j << 24, 0);
+    reg_fwht_bits<63>(v);   // row bits 6-11
+    // Layout B: b[k] = row k + 64 rt (row bits 0-5 from k, 6-11 from the
+    // thread).  Rounds on row bit 5 (a wave bit of A, bit 5 of k).
+    uint32_t b[64];
+    uint32_t* const sa = lds + hi12_slot(col, rt & 31u, 0);   // + 528 j
+    const uint32_t* const sb = lds + hi12_slot(col, 0, rt);   // + 16 k
+#pragma unroll
+    for (int rho = 0; rho < 2; ++rho) {
+        if ((w >> 3) == (uint32_t)rho) {
+#pragma unroll
+            for (int j = 0; j < 64; ++j)
+                sa[528 * j] = v[j];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < 32; ++k)
+            b[k + 32 * rho] = sb[16 * k];
+        if (rho == 0)
+            __syncthreads();
+    }
+    reg_fwht_bits<63>(b);   // row bits 0-5
+    const uint32_t vb = ((rt << 22) + (s << 4) + col) * 4u;
+#pragma unroll
+    for (int k = 0; k < 64; ++k)
+        __builtin_amdgcn_raw_buffer_store_b32((int)b[k], rs, (int)vb, k << 18, 0);
+}
+
 template <typename T> void launch_chunk_pass(T* d, size_t n, int L, hipStream_t stream) {
     const unsigned grid = (unsigned)(n >> L);
     const int vec = ((uintptr_t)d % 16) == 0;   // 16-byte vectors only on aligned arrays
@@ -183,6 +355,18 @@ template <typename T> void fwht_array_dev(T* d, size_t n, hipStream_t stream) {
     int nb = 0;
     while (((size_t)1 << nb) < n)
         ++nb;
+    if constexpr (sizeof(T) == 4) {
+        if (nb == 28 && (uintptr_t)d % 16 == 0) {
+            const unsigned tiles = 1u << 12;
+            const uint32_t bytes = (uint32_t)(n * sizeof(T));
+            hipLaunchKernelGGL(fwht_lo16_kernel, dim3(tiles), dim3(kTileThreads), 0, stream,
+                               reinterpret_cast<int32_t*>(d), bytes);
+            hipLaunchKernelGGL(fwht_hi12_kernel, dim3(tiles), dim3(kTileThreads), 0, stream,
+                               reinterpret_cast<int32_t*>(d), bytes);
+            BMM_HIP(hipGetLastError());
+            return;
+        }
+    }
     const int L = std::min(nb, kArrBlockBits);
     launch_chunk_pass(d, n, L, stream);
     // Remaining bits in the fewest register passes of at most kArrMaxR bits,

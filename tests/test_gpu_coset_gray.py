@@ -19,8 +19,11 @@ def test_fwht_kats(lib):
     assert lib.fwht_i16([32767, 1]).tolist() == [-32768, 32766]
 
 
-@pytest.mark.parametrize("n_bits", [1, 5, 12, 13, 14, 15, 18, 19, 21, 22, 24, 25, 26])
+@pytest.mark.parametrize("n_bits", [1, 5, 12, 13, 14, 15, 18, 19, 21, 22, 24, 25, 26, 27, 28])
 def test_fwht_i32_matches_oracle(lib, ora, n_bits):
+    """Dense random int32 arrays (wrapping sums).  2^28 takes the two-pass
+    register-tile path (16 + 12 index bits), the other sizes the LDS pass and
+    register passes."""
     _say("fwht_i32", n_bits)
     rng = np.random.default_rng(n_bits)
     x = rng.integers(-2**31, 2**31, size=1 << n_bits, dtype=np.int64).astype(np.int32)
@@ -35,10 +38,11 @@ def test_fwht_i16_wraps_like_oracle(lib, ora, n_bits):
     assert np.array_equal(lib.fwht_i16(x), ora.fwht_i16(x))
 
 
-@pytest.mark.parametrize("n_bits", [12, 20])
+@pytest.mark.parametrize("n_bits", [12, 20, 28])
 def test_fwht_dev_unaligned_array(lib, ora, n_bits):
     """A device array 4 bytes off a 16-byte boundary takes the element-wise
-    load path of the first pass."""
+    load path of the first pass (at 2^28 instead of the two-pass path, whose
+    16-byte accesses need an aligned array)."""
     import torch
     _say("fwht unaligned", n_bits)
     rng = np.random.default_rng(50 + n_bits)
