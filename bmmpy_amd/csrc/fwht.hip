@@ -168,9 +168,8 @@ __global__ __launch_bounds__(256) void fwht_stride_buf_kernel(T* __restrict__ da
 //     (16-byte loads, 1 KiB per wavefront access; 4-byte stores, 256 bytes
 //     per wavefront access), three register layouts;
 //   * fwht_hi12_kernel: index bits 16-27 of a slice of 16 columns (64-byte
-//     row segments; the two slices of a 128-byte line are neighbouring tiles
-//     on one XCD); stages on row bits 6-11 in the load layout and 0-5 after
-//     the exchange.
+//     row segments, 16-byte accesses; the two slices of a 128-byte line are
+//     neighbouring tiles on one XCD), three register layouts.
 // Each pass reads and writes the array once: 16 B per element in all,
 // against 24 B for the LDS pass plus two register passes.  With one
 // workgroup per CU the transform work of a tile does not overlap its own
@@ -273,60 +272,87 @@ __global__ __launch_bounds__(kTileThreads) void fwht_lo16_kernel(int32_t* __rest
 }
 
 // Pass 2: tile s holds rows r < 2^12 (index bits 16-27) of columns
-// 16 s + col (col < 16): element data[r 2^16 + 16 s + col].  A round of the
-// exchange holds the rows with one value of row bit 5 at word
-// x = e' + 16 (e' >> 9), e' = col + 16 (r & 31) + 512 (r >> 6): the reads of
-// the second layout (lanes over col and row bits 6, 7) fall on 32 distinct
-// banks modulo 2.
-constexpr int kHi12LdsWords = 32768 + 1024;
-
-__device__ inline uint32_t hi12_slot(uint32_t col, uint32_t r0_4, uint32_t r6_11) {
-    const uint32_t x = col | (r0_4 << 4) | (r6_11 << 9);
-    return x + 16u * (x >> 9);
-}
+// 16 s + c (c < 16): element data[r 2^16 + 16 s + c].  Each lane owns four
+// consecutive columns (16-byte loads and stores, one wavefront access = 16
+// rows of 64 bytes); with cq = l & 3 and the row bits spread over lane bits
+// 2-5, waves and registers, three layouts:
+//   A: v[4 j + q] = row (l >> 2) + 16 w + 256 j   (stages on row bits 8-11)
+//   B: b[4 k + q] = row (l >> 2) + 16 k + 256 w   (stages on row bits 4-7)
+//   C: c[4 m + q] = row m + 16 (l >> 2) + 256 w   (stages on row bits 0-3)
+// all at column 4 cq + q.  The column bits stay in registers, so each
+// exchange runs in two rounds on column bit 1 with 32 values written and 32
+// read per thread (8-byte LDS accesses).  A round's element (r, c) sits at
+// word e + 8 (e >> 7), e = (c & 1) + 2 (c >> 2) + 8 r: every access of 16
+// lanes covers 32 distinct banks.
+constexpr int kHi12LdsWords = 32768 + 2048;
 
 __global__ __launch_bounds__(kTileThreads) void fwht_hi12_kernel(int32_t* __restrict__ data,
                                                                  uint32_t bytes) {
-    __shared__ uint32_t lds[kHi12LdsWords];
+    __shared__ __align__(16) uint32_t lds[kHi12LdsWords];
     const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(data, 0, (int)bytes,
                                                                         0x00020000);
     const uint32_t l = threadIdx.x & 63u, w = threadIdx.x >> 6;
-    const uint32_t col = l & 15u;
+    const uint32_t cq = l & 3u, lr = l >> 2;
     const uint32_t s = xcd_tile(blockIdx.x, gridDim.x);
-    // Layout A: v[j] = row rt + 64 j with rt = (l >> 4) + 4 w (row bits 0-5
-    // from the thread, 6-11 from j); one wavefront access covers 4 rows of
-    // 64 bytes.
-    const uint32_t rt = (l >> 4) | (w << 2);
     uint32_t v[64];
-    const uint32_t va = ((rt << 16) + (s << 4) + col) * 4u;
+    const uint32_t va = (((lr + 16u * w) << 16) + (s << 4) + 4u * cq) * 4u;
 #pragma unroll
-    for (int j = 0; j < 64; ++j)
-        v[j] = (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(rs, (int)va, j << 24, 0);
-    reg_fwht_bits<63>(v);   // row bits 6-11
-    // Layout B: b[k] = row k + 64 rt (row bits 0-5 from k, 6-11 from the
-    // thread).  Rounds on row bit 5 (a wave bit of A, bit 5 of k).
+    for (int j = 0; j < 16; ++j) {
+        const i32x4 a = __builtin_amdgcn_raw_buffer_load_b128(rs, (int)va, j << 26, 0);
+        v[4 * j] = (uint32_t)a[0];
+        v[4 * j + 1] = (uint32_t)a[1];
+        v[4 * j + 2] = (uint32_t)a[2];
+        v[4 * j + 3] = (uint32_t)a[3];
+    }
+    reg_fwht_bits<60>(v);   // row bits 8-11
+    uint32_t* const sa = lds + 2u * cq + 8u * lr + 136u * w;    // A: + 2176 j
+    uint32_t* const sb = lds + 2u * cq + 8u * lr + 2176u * w;   // B: + 136 k
+    const uint32_t* const sc = lds + 2u * cq + 136u * lr + 2176u * w;   // C: + 8 m
     uint32_t b[64];
-    uint32_t* const sa = lds + hi12_slot(col, rt & 31u, 0);   // + 528 j
-    const uint32_t* const sb = lds + hi12_slot(col, 0, rt);   // + 16 k
 #pragma unroll
     for (int rho = 0; rho < 2; ++rho) {
-        if ((w >> 3) == (uint32_t)rho) {
 #pragma unroll
-            for (int j = 0; j < 64; ++j)
-                sa[528 * j] = v[j];
-        }
+        for (int j = 0; j < 16; ++j)
+            *reinterpret_cast<uint2*>(sa + 2176 * j) =
+                make_uint2(v[4 * j + 2 * rho], v[4 * j + 2 * rho + 1]);
         __syncthreads();
 #pragma unroll
-        for (int k = 0; k < 32; ++k)
-            b[k + 32 * rho] = sb[16 * k];
+        for (int k = 0; k < 16; ++k) {
+            const uint2 x = *reinterpret_cast<const uint2*>(sb + 136 * k);
+            b[4 * k + 2 * rho] = x.x;
+            b[4 * k + 2 * rho + 1] = x.y;
+        }
+        __syncthreads();
+    }
+    reg_fwht_bits<60>(b);   // row bits 4-7
+    uint32_t c[64];
+#pragma unroll
+    for (int rho = 0; rho < 2; ++rho) {
+#pragma unroll
+        for (int k = 0; k < 16; ++k)
+            *reinterpret_cast<uint2*>(sb + 136 * k) =
+                make_uint2(b[4 * k + 2 * rho], b[4 * k + 2 * rho + 1]);
+        __syncthreads();
+#pragma unroll
+        for (int m = 0; m < 16; ++m) {
+            const uint2 x = *reinterpret_cast<const uint2*>(sc + 8 * m);
+            c[4 * m + 2 * rho] = x.x;
+            c[4 * m + 2 * rho + 1] = x.y;
+        }
         if (rho == 0)
             __syncthreads();
     }
-    reg_fwht_bits<63>(b);   // row bits 0-5
-    const uint32_t vb = ((rt << 22) + (s << 4) + col) * 4u;
+    reg_fwht_bits<60>(c);   // row bits 0-3
+    const uint32_t vc = (((16u * lr + 256u * w) << 16) + (s << 4) + 4u * cq) * 4u;
 #pragma unroll
-    for (int k = 0; k < 64; ++k)
-        __builtin_amdgcn_raw_buffer_store_b32((int)b[k], rs, (int)vb, k << 18, 0);
+    for (int m = 0; m < 16; ++m) {
+        i32x4 a;
+        a[0] = (int)c[4 * m];
+        a[1] = (int)c[4 * m + 1];
+        a[2] = (int)c[4 * m + 2];
+        a[3] = (int)c[4 * m + 3];
+        __builtin_amdgcn_raw_buffer_store_b128(a, rs, (int)vc, m << 18, 0);
+    }
 }
 
 template <typename T> void launch_chunk_pass(T* d, size_t n, int L, hipStream_t stream) {
