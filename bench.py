@@ -68,6 +68,8 @@ TRAFFIC_BYTES = {
     # The kernels of one calc_scores_and_order call on 2^28 entries (scores,
     # the three table-scan kernels, the tile order), FETCH_SIZE x 2.0 plus
     # WRITE_SIZE x 1.0 in KiB.
+    # The array FWHT of 2^28 int32 (fwht_lo16_kernel + fwht_hi12_kernel).
+    "fwht_array_2p28": ((524350.0 + 527800.0) * 2.0 + 1048576.0 + 1057000.0) * 1024,
     "scores_order_2p28": ((524412.9 + 16406.5 + 261.3 + 16670.0 + 557224.6) * 2.0
                           + 1085440.1 + 512.0 + 512.0 + 65536.0 + 1076899.8) * 1024,
 }
@@ -405,7 +407,6 @@ def run_basis(torch, steps: int, warmup: int) -> dict:
     big = torch.zeros(1 << FWHT_HBM_BITS, dtype=torch.int32, device="cuda:0")
     stream = torch.cuda.current_stream()
     sp = stream.cuda_stream
-    fwht_passes = 2
     fwht_bytes = 16.0 * (1 << FWHT_HBM_BITS)
     fwht_ms = _time(torch, stream,
                     lambda: _lib.call("bmm_hip_fwht_i32_dev", big.data_ptr(), big.numel(), sp),
@@ -424,7 +425,7 @@ def run_basis(torch, steps: int, warmup: int) -> dict:
                    "p": BASIS_P, "seed": hex(BASIS_SEED), "subset_evals_per_step": n - 1},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": 8000.0, "unit": "GB/s",
                      "frac": achieved / 8000.0,
-                     "traffic": fwht_passes * 8.0 * (1 << FWHT_HBM_BITS),
+                     "traffic": TRAFFIC_BYTES["fwht_array_2p28"],
                      "bytes_per_launch": fwht_bytes,
                      "kernel": f"array FWHT of 2^{FWHT_HBM_BITS} int32: fwht_lo16_kernel "
                                "+ fwht_hi12_kernel",
