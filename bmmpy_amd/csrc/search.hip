@@ -184,21 +184,37 @@ __device__ inline bool pk_topk_block(uint32_t* s, Build&& build, uint64_t mh, in
 #pragma unroll
         for (int e = 0; e < 32; ++e)
             v[e] = p[e * ST];
-        s16x2 bound = pk_as(v[0]);
+        // x_0 + sum |x_e| bounds every output of the group (four partial
+        // sums: independent chains).
+        s16x2 acc[4] = {pk_as(v[0]), {0, 0}, {0, 0}, {0, 0}};
 #pragma unroll
         for (int e = 1; e < 32; ++e) {
             const s16x2 x = pk_as(v[e]);
-            bound = bound + pk_max(x, -x);
+            acc[e & 3] = acc[e & 3] + pk_max(x, -x);
         }
+        const s16x2 bound = (acc[0] + acc[1]) + (acc[2] + acc[3]);
         if (max(pk_lo(bound), pk_hi(bound)) >= hmin) {
             pk_butterflies(v);
+            // The group's largest output (a packed max tree) decides whether
+            // any entry needs the per-entry test.
+            s16x2 m[16];
 #pragma unroll
-            for (int e = 0; e < 32; ++e) {
-                const s16x2 x = pk_as(v[e]);
-                if (max(pk_lo(x), pk_hi(x)) >= hmin) {
-                    const uint32_t w = w0 | ((uint32_t)e << 5);
-                    take(pk_lo(x), w);
-                    take(pk_hi(x), w + (1u << kPkW));
+            for (int e = 0; e < 16; ++e)
+                m[e] = pk_max(pk_as(v[e]), pk_as(v[e + 16]));
+#pragma unroll
+            for (int h = 8; h > 0; h >>= 1)
+#pragma unroll
+                for (int e = 0; e < h; ++e)
+                    m[e] = pk_max(m[e], m[e + h]);
+            if (max(pk_lo(m[0]), pk_hi(m[0])) >= hmin) {
+#pragma unroll
+                for (int e = 0; e < 32; ++e) {
+                    const s16x2 x = pk_as(v[e]);
+                    if (max(pk_lo(x), pk_hi(x)) >= hmin) {
+                        const uint32_t w = w0 | ((uint32_t)e << 5);
+                        take(pk_lo(x), w);
+                        take(pk_hi(x), w + (1u << kPkW));
+                    }
                 }
             }
         }
