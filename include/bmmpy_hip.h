@@ -77,7 +77,10 @@ int bmm_hip_device_info(bmm_device_info *out);
  * arithmetic; n must be a power of two (n <= 1 is a no-op). Host pointers. */
 int bmm_hip_fwht_i32(int32_t *data, size_t n);
 int bmm_hip_fwht_i16(int16_t *data, size_t n);
-/* Same on device memory, enqueued on `stream`. */
+/* Same on device memory, enqueued on `stream`.  An int32 array of 2^28
+ * elements on a 16-byte boundary takes two passes over HBM (other sizes and
+ * alignments take three or fewer passes of another form; the result is the
+ * same). */
 int bmm_hip_fwht_i32_dev(int32_t *d_data, size_t n, void *stream);
 int bmm_hip_fwht_i16_dev(int16_t *d_data, size_t n, void *stream);
 
