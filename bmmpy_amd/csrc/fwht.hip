@@ -286,6 +286,63 @@ __global__ __launch_bounds__(kTileThreads) void fwht_lo16_kernel(int32_t* __rest
 // lanes covers 32 distinct banks.
 constexpr int kHi12LdsWords = 32768 + 2048;
 
+// The loads of band j (rows 256 j .. 256 j + 255) and the stores of row
+// group m are issued in the order j = i ^ ROT, m = i ^ ROT (i = 0 .. 15),
+// with ROT = (s >> 1) mod 16: neighbouring line pairs start on different
+// DRAM rows instead of all tiles in flight opening the same ones (a probe of
+// the loads and stores alone ran 8% faster so; the kernel 1-2%).  One code
+// path per ROT keeps every register index a constant, and a scheduling
+// barrier after each access keeps the issue order.
+template <int ROT>
+__device__ inline void hi12_load_bands(__amdgpu_buffer_rsrc_t rs, uint32_t va, uint32_t (&v)[64]) {
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        const int j = i ^ ROT;
+        const i32x4 a = __builtin_amdgcn_raw_buffer_load_b128(rs, (int)va, j << 26, 0);
+        v[4 * j] = (uint32_t)a[0];
+        v[4 * j + 1] = (uint32_t)a[1];
+        v[4 * j + 2] = (uint32_t)a[2];
+        v[4 * j + 3] = (uint32_t)a[3];
+        __builtin_amdgcn_sched_barrier(0);   // keep the issue order
+    }
+}
+
+template <int ROT>
+__device__ inline void hi12_store_groups(__amdgpu_buffer_rsrc_t rs, uint32_t vc,
+                                         const uint32_t (&c)[64]) {
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        const int m = i ^ ROT;
+        i32x4 a;
+        a[0] = (int)c[4 * m];
+        a[1] = (int)c[4 * m + 1];
+        a[2] = (int)c[4 * m + 2];
+        a[3] = (int)c[4 * m + 3];
+        __builtin_amdgcn_raw_buffer_store_b128(a, rs, (int)vc, m << 18, 0);
+        __builtin_amdgcn_sched_barrier(0);
+    }
+}
+
+#define BMM_ROT16(F, ...)                                                                      \
+    switch (rot) {                                                                             \
+    case 0: F<0>(__VA_ARGS__); break;                                                          \
+    case 1: F<1>(__VA_ARGS__); break;                                                          \
+    case 2: F<2>(__VA_ARGS__); break;                                                          \
+    case 3: F<3>(__VA_ARGS__); break;                                                          \
+    case 4: F<4>(__VA_ARGS__); break;                                                          \
+    case 5: F<5>(__VA_ARGS__); break;                                                          \
+    case 6: F<6>(__VA_ARGS__); break;                                                          \
+    case 7: F<7>(__VA_ARGS__); break;                                                          \
+    case 8: F<8>(__VA_ARGS__); break;                                                          \
+    case 9: F<9>(__VA_ARGS__); break;                                                          \
+    case 10: F<10>(__VA_ARGS__); break;                                                        \
+    case 11: F<11>(__VA_ARGS__); break;                                                        \
+    case 12: F<12>(__VA_ARGS__); break;                                                        \
+    case 13: F<13>(__VA_ARGS__); break;                                                        \
+    case 14: F<14>(__VA_ARGS__); break;                                                        \
+    default: F<15>(__VA_ARGS__); break;                                                        \
+    }
+
 __global__ __launch_bounds__(kTileThreads) void fwht_hi12_kernel(int32_t* __restrict__ data,
                                                                  uint32_t bytes) {
     __shared__ __align__(16) uint32_t lds[kHi12LdsWords];
@@ -294,16 +351,10 @@ __global__ __launch_bounds__(kTileThreads) void fwht_hi12_kernel(int32_t* __rest
     const uint32_t l = threadIdx.x & 63u, w = threadIdx.x >> 6;
     const uint32_t cq = l & 3u, lr = l >> 2;
     const uint32_t s = xcd_tile(blockIdx.x, gridDim.x);
+    const uint32_t rot = (s >> 1) & 15u;
     uint32_t v[64];
     const uint32_t va = (((lr + 16u * w) << 16) + (s << 4) + 4u * cq) * 4u;
-#pragma unroll
-    for (int j = 0; j < 16; ++j) {
-        const i32x4 a = __builtin_amdgcn_raw_buffer_load_b128(rs, (int)va, j << 26, 0);
-        v[4 * j] = (uint32_t)a[0];
-        v[4 * j + 1] = (uint32_t)a[1];
-        v[4 * j + 2] = (uint32_t)a[2];
-        v[4 * j + 3] = (uint32_t)a[3];
-    }
+    BMM_ROT16(hi12_load_bands, rs, va, v)
     reg_fwht_bits<60>(v);   // row bits 8-11
     uint32_t* const sa = lds + 2u * cq + 8u * lr + 136u * w;    // A: + 2176 j
     uint32_t* const sb = lds + 2u * cq + 8u * lr + 2176u * w;   // B: + 136 k
@@ -344,16 +395,9 @@ __global__ __launch_bounds__(kTileThreads) void fwht_hi12_kernel(int32_t* __rest
     }
     reg_fwht_bits<60>(c);   // row bits 0-3
     const uint32_t vc = (((16u * lr + 256u * w) << 16) + (s << 4) + 4u * cq) * 4u;
-#pragma unroll
-    for (int m = 0; m < 16; ++m) {
-        i32x4 a;
-        a[0] = (int)c[4 * m];
-        a[1] = (int)c[4 * m + 1];
-        a[2] = (int)c[4 * m + 2];
-        a[3] = (int)c[4 * m + 3];
-        __builtin_amdgcn_raw_buffer_store_b128(a, rs, (int)vc, m << 18, 0);
-    }
+    BMM_ROT16(hi12_store_groups, rs, vc, c)
 }
+#undef BMM_ROT16
 
 template <typename T> void launch_chunk_pass(T* d, size_t n, int L, hipStream_t stream) {
     const unsigned grid = (unsigned)(n >> L);
