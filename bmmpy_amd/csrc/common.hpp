@@ -92,6 +92,35 @@ struct DevBuf {
     }
 };
 
+// Grow-only page-locked host memory (hipHostMalloc): copies between it and
+// the device are asynchronous.  Its contents must not be rewritten before
+// the copies that read them have completed.
+struct HostBuf {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+    ~HostBuf() {
+        if (ptr)
+            (void)hipHostFree(ptr);
+    }
+    template <typename T> T* get(size_t count) {
+        size_t need = count * sizeof(T);
+        if (need < 256)
+            need = 256;
+        if (need > bytes) {
+            if (ptr)
+                BMM_HIP(hipHostFree(ptr));
+            ptr = nullptr;
+            bytes = 0;
+            size_t cap = 256;
+            while (cap < need)
+                cap <<= 1;
+            BMM_HIP(hipHostMalloc(&ptr, cap, hipHostMallocDefault));
+            bytes = cap;
+        }
+        return static_cast<T*>(ptr);
+    }
+};
+
 // ------------------------------------------------------------ bit helpers --
 __host__ __device__ inline uint64_t gray_of(uint64_t s) { return s ^ (s >> 1); }
 

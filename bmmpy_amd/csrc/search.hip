@@ -19,6 +19,7 @@
 #include "topk.hpp"
 
 #include <algorithm>
+#include <cstring>
 #include <vector>
 
 namespace bmmhip {
@@ -554,17 +555,10 @@ __global__ void column_patterns64_kernel(const uint64_t* __restrict__ rows, int 
     mult[c] = p != 0 ? 1 : 0;
 }
 
-__global__ void pattern_from_split_kernel(const uint64_t* __restrict__ q,
-                                          const uint64_t* __restrict__ r, int low_bits, int n,
-                                          uint64_t* __restrict__ pat) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n)
-        pat[i] = (low_bits >= 64 ? 0ull : (q[i] << low_bits)) | r[i];
-}
-
 namespace {
 struct SearchWorkspace {
-    DevBuf rows, pat, mult, q, r, counter, lists, lists2, out, bound;
+    DevBuf rows, pat, mult, counter, lists, lists2, out, bound;
+    HostBuf stage, keys;   // page-locked: inputs sent and the list received asynchronously
 };
 
 // Folds n_lists lists of K keys into one by a tree of parallel merges;
@@ -671,7 +665,6 @@ std::vector<bmm_topk_entry> fwht_topk_engine(const uint64_t* d_pat, const int32_
     unsigned long long* d_counter = ws.counter.get<unsigned long long>(1);
     uint64_t* d_lists = ws.lists.get<uint64_t>((size_t)grid * K);
     uint64_t* d_lists2 = ws.lists2.get<uint64_t>((size_t)((grid + kMergeFan - 1) / kMergeFan) * K);
-    fill_u64(reinterpret_cast<uint64_t*>(d_counter), 1, 0, stream);
     uint64_t* d_out;
     if (t >= kPkL && total <= kPkMaxTotal && (n_masks >> kPkL) >= 256) {
         // Packed blocks of 2^14 masks offering their few candidates, exact
@@ -729,6 +722,8 @@ std::vector<bmm_topk_entry> fwht_topk_engine(const uint64_t* d_pat, const int32_
                 d_bound ? (int)((n14 + (uint64_t)grid_pk - 1) / (uint64_t)grid_pk)
                         : (int)std::min<uint64_t>(
                               kPkChunk, std::max<uint64_t>(1, n14 / ((uint64_t)grid_pk * 16)));
+            if (!d_bound)   // the bounded pass takes the counter cleared with its header
+                fill_u64(reinterpret_cast<uint64_t*>(d_counter), 1, 0, stream);
             hipLaunchKernelGGL(kern, dim3(grid_pk), dim3(256), 0, stream, d_pat, d_mult, n_pat, t,
                                (int32_t)total, n14, K, chunk, d_queue, d_bound, d_lists);
             BMM_HIP(hipGetLastError());
@@ -744,6 +739,8 @@ std::vector<bmm_topk_entry> fwht_topk_engine(const uint64_t* d_pat, const int32_
             st = ws.bound.get<TopkBoundState>(1);
             fill_u64(reinterpret_cast<uint64_t*>(st), 2, 0, stream);
             d_queue = &st->counter;
+        } else {
+            fill_u64(reinterpret_cast<uint64_t*>(d_counter), 1, 0, stream);
         }
         if (K <= 512) {
             dispatch_fwht_topk<2048>(L, d_pat, d_mult, n_pat, t, (int32_t)total, n_blocks, K,
@@ -755,12 +752,13 @@ std::vector<bmm_topk_entry> fwht_topk_engine(const uint64_t* d_pat, const int32_
             d_out = merge_lists<8192>(d_lists, d_lists2, grid, K, stream);
         }
     }
-    std::vector<uint64_t> keys((size_t)K);
-    BMM_HIP(hipMemcpyAsync(keys.data(), d_out, keys.size() * 8, hipMemcpyDeviceToHost, stream));
+    uint64_t* keys = ws.keys.get<uint64_t>((size_t)K);
+    BMM_HIP(hipMemcpyAsync(keys, d_out, (size_t)K * 8, hipMemcpyDeviceToHost, stream));
     BMM_HIP(hipStreamSynchronize(stream));
     std::vector<bmm_topk_entry> out;
     const uint64_t mmask = t >= 64 ? ~0ull : ((1ull << t) - 1);
-    for (uint64_t key : keys) {
+    for (int i = 0; i < K; ++i) {
+        const uint64_t key = keys[i];
         if (key == ~0ull)
             break;
         out.push_back(bmm_topk_entry{key & mmask, (uint32_t)(key >> t), 0});
@@ -769,7 +767,9 @@ std::vector<bmm_topk_entry> fwht_topk_engine(const uint64_t* d_pat, const int32_
 }
 
 // FwhtSearch over host rows: patterns of the first `cols` columns (tail masked
-// as the reference does, src/search/fwht_search.cpp:64-65).
+// as the reference does, src/search/fwht_search.cpp:64-65).  The rows go up
+// in one asynchronous copy from page-locked memory; the active-column count
+// (the number of nonzero patterns) is taken on the host from the rows' OR.
 std::vector<bmm_topk_entry> fwht_topk_rows(const uint64_t* rows, size_t t, size_t words,
                                            size_t cols, size_t k) {
     if (cols > (size_t)words * 64)
@@ -780,41 +780,45 @@ std::vector<bmm_topk_entry> fwht_topk_rows(const uint64_t* rows, size_t t, size_
     uint64_t* d_rows = ws.rows.get<uint64_t>(t * words);
     uint64_t* d_pat = ws.pat.get<uint64_t>(std::max<size_t>(cols, 1));
     int32_t* d_mult = ws.mult.get<int32_t>(std::max<size_t>(cols, 1));
-    BMM_HIP(hipMemcpy(d_rows, rows, t * words * 8, hipMemcpyHostToDevice));
-    // Active-column count = the number of nonzero patterns.
     int64_t total = 0;
     if (cols) {
+        uint64_t* h_rows = ws.stage.get<uint64_t>(t * words);
+        std::memcpy(h_rows, rows, t * words * 8);
+        BMM_HIP(hipMemcpyAsync(d_rows, h_rows, t * words * 8, hipMemcpyHostToDevice, nullptr));
+        for (size_t wi = 0; wi * 64 < cols; ++wi) {
+            uint64_t any = 0;
+            for (size_t i = 0; i < t; ++i)
+                any |= rows[i * words + wi];
+            if (cols - wi * 64 < 64)
+                any &= (1ull << (cols - wi * 64)) - 1;
+            total += __builtin_popcountll(any);
+        }
         hipLaunchKernelGGL(column_patterns64_kernel, dim3((unsigned)((cols + 255) / 256)),
                            dim3(256), 0, nullptr, d_rows, (int)t, (int)words, (int)cols, d_pat,
                            d_mult);
         BMM_HIP(hipGetLastError());
-        std::vector<int32_t> m(cols);
-        BMM_HIP(hipMemcpy(m.data(), d_mult, cols * 4, hipMemcpyDeviceToHost));
-        for (int32_t v : m)
-            total += v;
     }
     return fwht_topk_engine(d_pat, d_mult, (int)cols, (int)t, total, k, nullptr);
 }
 
+// The split form (patterns q << low_bits | r with multiplicities): patterns
+// and multiplicities are formed on the host and sent in one asynchronous copy.
 std::vector<bmm_topk_entry> mitm_topk_split(const uint64_t* q, const uint64_t* r,
                                             const int32_t* mult, size_t unique, size_t low_bits,
                                             size_t high_bits, int32_t total, size_t k) {
     SearchWorkspace& ws = search_ws();
     const size_t n = std::max<size_t>(unique, 1);
-    uint64_t* d_q = ws.q.get<uint64_t>(n);
-    uint64_t* d_r = ws.r.get<uint64_t>(n);
-    uint64_t* d_pat = ws.pat.get<uint64_t>(n);
-    int32_t* d_mult = ws.mult.get<int32_t>(n);
+    const size_t words = n + (n + 1) / 2;   // patterns, then the int32 multiplicities
+    uint64_t* d_in = ws.pat.get<uint64_t>(words);
     if (unique) {
-        BMM_HIP(hipMemcpy(d_q, q, unique * 8, hipMemcpyHostToDevice));
-        BMM_HIP(hipMemcpy(d_r, r, unique * 8, hipMemcpyHostToDevice));
-        BMM_HIP(hipMemcpy(d_mult, mult, unique * 4, hipMemcpyHostToDevice));
-        hipLaunchKernelGGL(pattern_from_split_kernel, dim3((unsigned)((unique + 255) / 256)),
-                           dim3(256), 0, nullptr, d_q, d_r, (int)low_bits, (int)unique, d_pat);
-        BMM_HIP(hipGetLastError());
+        uint64_t* h_in = ws.stage.get<uint64_t>(words);
+        for (size_t i = 0; i < unique; ++i)
+            h_in[i] = (low_bits >= 64 ? 0ull : (q[i] << low_bits)) | r[i];
+        std::memcpy(h_in + n, mult, unique * 4);
+        BMM_HIP(hipMemcpyAsync(d_in, h_in, words * 8, hipMemcpyHostToDevice, nullptr));
     }
-    return fwht_topk_engine(d_pat, d_mult, (int)unique, (int)(low_bits + high_bits), total, k,
-                            nullptr);
+    return fwht_topk_engine(d_in, reinterpret_cast<const int32_t*>(d_in + n), (int)unique,
+                            (int)(low_bits + high_bits), total, k, nullptr);
 }
 
 } // namespace bmmhip
