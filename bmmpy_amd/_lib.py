@@ -63,7 +63,7 @@ class GaParams(C.Structure):
                 ("num_parents", C.c_int), ("num_offspring", C.c_int),
                 ("mutation_rate", C.c_float), ("catastrophe_threshold", C.c_int),
                 ("catastrophe_survival", C.c_float), ("restart_after", C.c_int),
-                ("restart_p", C.c_float)]
+                ("restart_p", C.c_float), ("init_rounds", C.c_int)]
 
 
 class GaIslandState(C.Structure):

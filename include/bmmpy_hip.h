@@ -331,6 +331,9 @@ typedef struct bmm_ga_params {
                                  random individual with probability restart_p
                                  (genetic_algorithm_lifecycle.cpp:118-142: 15 and 0.2) */
     float restart_p;
+    int init_rounds;          /* 1: before the first generation the island bases go through the
+                                 epoch's improvement steps (li_sweeps, win_rounds, isd_trials);
+                                 0: the first generation starts from the input */
 } bmm_ga_params;
 
 typedef struct bmm_ga_status {

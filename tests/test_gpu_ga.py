@@ -84,17 +84,21 @@ def test_ga_rejects_bad_shapes(lib):
         lib.GpuGA(np.zeros((300, 32), np.uint64), 2048, lib.ga_default_params())
 
 
-def test_ga_local_improvement_matches_restatement(lib):
-    _say("ga local improvement")
+@pytest.mark.parametrize("init_rounds", [0, 1])
+def test_ga_local_improvement_matches_restatement(lib, init_rounds):
+    """One generation with an epoch after it: one pairwise sweep at the
+    epoch, and with init_rounds one more before the first generation (no
+    mutations, so the generation itself changes nothing)."""
+    _say("ga local improvement", init_rounds)
     S, A, M = lib.gen_planted(0x5EED0005, 32, 2048, 0.05)
     p = lib.ga_default_params(islands=1, population=4, max_ops=8, migrate_every=1,
                               mutations=0, li_sweeps=1, check_every=1, learned=0, win_rows=0,
-                              isd_trials=0)
+                              isd_trials=0, init_rounds=init_rounds)
     ga = lib.GpuGA(A, 2048, p)
     st = ga.run(1)
     best, w = ga.best()
     ga.close()
-    exp = li_rounds(A, 1)
+    exp = li_rounds(A, 1 + init_rounds)
     assert st.li_moves > 0
     assert np.array_equal(best, exp)
     assert w == st.best_weight == _weight(exp)
@@ -136,7 +140,8 @@ def test_ga_window_improvement_matches_restatement(lib, ora, learned, cols):
     rounds = 1 if learned else 2   # ties may pick other rows in model order
     p = lib.ga_default_params(islands=2, population=4, max_ops=8, migrate_every=1,
                               mutations=0, li_sweeps=0, check_every=1, learned=learned,
-                              win_rows=16, win_rounds=rounds, win_candidates=32, isd_trials=0)
+                              win_rows=16, win_rounds=rounds, win_candidates=32, isd_trials=0,
+                              init_rounds=0)
     ga = lib.GpuGA(A, cols, p)
     st = ga.run(1)
     best, w = ga.best()
@@ -202,17 +207,20 @@ def _isd_reference(A, seed, isl, cols):
     return np.array([[(r >> (64 * k)) & M64 for k in range(W)] for r in out], dtype=np.uint64)
 
 
-@pytest.mark.parametrize("islands", [1, 2])
-def test_ga_information_set_trial_matches_restatement(lib, ora, islands):
-    """One epoch with one information-set trial per island and nothing else
-    (no mutation, no pairwise or window improvement): each island base equals
-    the restated trial; the best island is reported."""
-    _say("ga information-set trial", islands)
+@pytest.mark.parametrize("islands,init_rounds", [(1, 0), (2, 0), (1, 1), (2, 1)])
+def test_ga_information_set_trial_matches_restatement(lib, ora, islands, init_rounds):
+    """One information-set trial per island and nothing else (no mutation,
+    no pairwise or window improvement), either at the epoch after one
+    generation or, with init_rounds and no epoch, before the first
+    generation: each island base equals the restated trial; the best island
+    is reported."""
+    _say("ga information-set trial", islands, init_rounds)
     cols = 2048
     S, A, M = lib.gen_planted(0x5EED0008, 40, cols, 0.03)
-    p = lib.ga_default_params(islands=islands, population=4, max_ops=8, migrate_every=1,
-                              mutations=0, li_sweeps=0, check_every=1, learned=0, win_rows=0,
-                              isd_trials=1)
+    p = lib.ga_default_params(islands=islands, population=4, max_ops=8,
+                              migrate_every=1000 if init_rounds else 1, mutations=0,
+                              li_sweeps=0, check_every=1, learned=0, win_rows=0, isd_trials=1,
+                              init_rounds=init_rounds)
     ga = lib.GpuGA(A, cols, p)
     st = ga.run(1)
     best, w = ga.best()
