@@ -738,6 +738,7 @@ def secondary(torch, _lib, dev, stream) -> dict:
         "planted_weight": g["planted_weight"], "start_weight": g["start_weight"],
         "target": g["target"], "isd_trials_per_epoch": g["isd_trials_per_epoch"],
         "window_rounds_per_epoch": g["window_rounds_per_epoch"],
+        "init_rounds": g["init_rounds"],
         **{f"{k}_{f}": g[k][f] for k in ("plain", "learned")
            for f in ("best_weight", "time_to_target_s", "generations", "ms_per_generation",
                      "evals_per_s", "replayed_ops_per_s")},
@@ -796,6 +797,7 @@ def ga_compare(budget_s: float, seed: int = 0x5EED0005, ga_seed: int | None = No
         p = _lib.ga_default_params(learned=learned, seed=ga_seed, **overrides)
         res["isd_trials_per_epoch"] = int(p.isd_trials)
         res["window_rounds_per_epoch"] = int(p.win_rounds)
+        res["init_rounds"] = int(p.init_rounds)
         t0 = time.perf_counter()
         ga = _lib.GpuGA(A, 8192, p)
         setup = time.perf_counter() - t0
@@ -983,6 +985,7 @@ def main() -> int:
                        "window_rows": 20,
                        "window_rounds_per_epoch": r["window_rounds_per_epoch"],
                        "isd_trials_per_epoch": r["isd_trials_per_epoch"],
+                       "init_rounds": r["init_rounds"],
                        "budget_s": args.ga_budget},
             "learned_over_plain_time_ratio": ratio, "ga": r}))
         return 0
