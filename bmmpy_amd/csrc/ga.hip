@@ -1208,12 +1208,27 @@ __global__ void ga_win_apply_kernel(uint64_t* __restrict__ base, int rows, int w
 constexpr int kIsdMaskWords = 4;   // rows <= 256
 constexpr int kIsdDraws = 512;     // draws whose columns are gathered at once
 constexpr int kIsdGather = 32;     // gather loads in flight per thread
+// Trials of one round run side by side from the same base (grid y) and are
+// selected from together.  A trial's eliminated rows depend only on the row
+// space and its draws (they are the rows reduced on its pivot columns), and
+// the lightest basis of a union is the lightest basis of one part's
+// lightest basis and the rest (matroid greedy), so a round of trials gives a
+// basis of the same weight as the trials one after another.
+constexpr int kIsdBatch = 3;       // trials per round: a pool of at most 1024 keys
+
+// Seed of trial `round` (counted from 1): the same stream the trials have
+// taken one after another.
+__host__ __device__ inline uint64_t isd_seed(uint64_t seed, uint64_t round) {
+    return seed ^ (0xA0761D6478BD642Full * round);
+}
 
 __global__ __launch_bounds__(256) void ga_isd_pivot_kernel(const uint64_t* __restrict__ base,
                                                            int rows, int words, int cols,
-                                                           uint64_t seed,
-                                                           uint64_t* __restrict__ tmask) {
+                                                           uint64_t seed0, uint64_t round0,
+                                                           uint64_t* __restrict__ tmask_all) {
     const int isl = blockIdx.x, tid = threadIdx.x;
+    const uint64_t seed = isd_seed(seed0, round0 + blockIdx.y);
+    uint64_t* tmask = tmask_all + (size_t)blockIdx.y * gridDim.x * rows * kIsdMaskWords;
     const int wave = tid / kWave, lane = tid % kWave;
     const uint64_t* B = base + (size_t)isl * rows * words;
     __shared__ uint64_t colv[kIsdDraws][kIsdMaskWords];   // a(c) of the chunk's draws
@@ -1311,11 +1326,15 @@ constexpr int kIsdSlice = 16;
 
 __global__ __launch_bounds__(256) void ga_isd_rows_kernel(const uint64_t* __restrict__ base,
                                                           int rows, int words,
-                                                          const uint64_t* __restrict__ tmask,
-                                                          uint64_t* __restrict__ rbuf,
-                                                          uint32_t* __restrict__ rw) {
+                                                          const uint64_t* __restrict__ tmask_all,
+                                                          uint64_t* __restrict__ rbuf_all,
+                                                          uint32_t* __restrict__ rw_all) {
     const int slice = blockIdx.x, isl = blockIdx.y, tid = threadIdx.x;
     const size_t per = (size_t)rows * words;
+    const size_t n_isl = gridDim.y;   // trial blockIdx.z's buffers follow the earlier trials'
+    const uint64_t* tmask = tmask_all + (size_t)blockIdx.z * n_isl * rows * kIsdMaskWords;
+    uint64_t* rbuf = rbuf_all + (size_t)blockIdx.z * n_isl * per;
+    uint32_t* rw = rw_all + (size_t)blockIdx.z * n_isl * rows;
     const uint64_t* B = base + (size_t)isl * per;
     __shared__ uint64_t a[256][kIsdSlice];
     const int w0 = slice * kIsdSlice;
@@ -1354,13 +1373,14 @@ __global__ __launch_bounds__(256) void ga_isd_rows_kernel(const uint64_t* __rest
     }
 }
 
-// GlobalGreedyApplier over the pool {R rows (masks T)} + {B rows (unit
-// masks)}: bitonic sort of (weight, pool index) keys, then one entry at a
-// time against a fully reduced basis held in one wavefront's registers
-// (reduce by the pivots set in the mask, pivot = highest remaining bit,
-// clear that bit from the other basis rows).  chosen[k] = pool index of the
-// k-th kept entry.
-__global__ __launch_bounds__(256) void ga_isd_select_kernel(int rows,
+// GlobalGreedyApplier over the pool {R rows of each trial (masks T)} + {B
+// rows (unit masks)}, pool index = trial * rows + row, then rows + B row:
+// bitonic sort of (weight, pool index) keys, then one entry at a time
+// against a fully reduced basis held in one wavefront's registers (reduce
+// by the pivots set in the mask, pivot = highest remaining bit, clear that
+// bit from the other basis rows).  chosen[k] = pool index of the k-th kept
+// entry.
+__global__ __launch_bounds__(256) void ga_isd_select_kernel(int rows, int n_trials,
                                                             const uint64_t* __restrict__ tmask,
                                                             const uint32_t* __restrict__ rw,
                                                             const uint32_t* __restrict__ roww,
@@ -1368,28 +1388,34 @@ __global__ __launch_bounds__(256) void ga_isd_select_kernel(int rows,
                                                             unsigned int* __restrict__ n_isd) {
     const int isl = blockIdx.x, tid = threadIdx.x;
     const int wave = tid / kWave, lane = tid % kWave;
-    __shared__ unsigned long long key[512];
-    __shared__ uint64_t tm[256][kIsdMaskWords];
-    const int pool = 2 * rows;
-    for (int k = tid; k < 512; k += blockDim.x) {
+    const size_t n_isl = gridDim.x;
+    __shared__ unsigned long long key[1024];
+    __shared__ uint64_t tm[kIsdBatch * 256][kIsdMaskWords];
+    const int n_r = n_trials * rows;   // pool entries from the trials
+    const int pool = n_r + rows;
+    int np2 = 512;
+    while (np2 < pool)
+        np2 <<= 1;
+    for (int k = tid; k < np2; k += blockDim.x) {
         unsigned long long kk = ~0ull;
         if (k < pool) {
-            const uint32_t w = k < rows ? rw[(size_t)isl * rows + k]
-                                        : roww[(size_t)isl * rows + (k - rows)];
+            const uint32_t w = k < n_r ? rw[((size_t)(k / rows) * n_isl + isl) * rows + k % rows]
+                                       : roww[(size_t)isl * rows + (k - n_r)];
             kk = ((unsigned long long)w << 10) | (unsigned)k;
         }
         key[k] = kk;
     }
-    for (int k = tid; k < rows; k += blockDim.x) {
+    for (int k = tid; k < n_r; k += blockDim.x) {
 #pragma unroll
         for (int q = 0; q < kIsdMaskWords; ++q)
-            tm[k][q] = tmask[((size_t)isl * rows + k) * kIsdMaskWords + q];
+            tm[k][q] = tmask[(((size_t)(k / rows) * n_isl + isl) * rows + k % rows) *
+                                 kIsdMaskWords + q];
     }
     __syncthreads();
-    // Bitonic sort of 512 keys (ascending).
-    for (int size = 2; size <= 512; size <<= 1)
+    // Bitonic sort of np2 keys (ascending).
+    for (int size = 2; size <= np2; size <<= 1)
         for (int stride = size >> 1; stride > 0; stride >>= 1) {
-            for (int i = tid; i < 256; i += blockDim.x) {
+            for (int i = tid; i < np2 / 2; i += blockDim.x) {
                 const int lo = 2 * i - (i & (stride - 1));
                 const int hi = lo + stride;
                 const bool up = (lo & size) == 0;
@@ -1415,8 +1441,8 @@ __global__ __launch_bounds__(256) void ga_isd_select_kernel(int rows,
         uint64_t m[kIsdMaskWords];
 #pragma unroll
         for (int q = 0; q < kIsdMaskWords; ++q)
-            m[q] = idx < rows ? tm[idx][q]
-                              : ((idx - rows) / 64 == q ? 1ull << ((idx - rows) % 64) : 0ull);
+            m[q] = idx < n_r ? tm[idx][q]
+                             : ((idx - n_r) / 64 == q ? 1ull << ((idx - n_r) % 64) : 0ull);
         uint64_t x[kIsdMaskWords] = {};
 #pragma unroll
         for (int sl = 0; sl < kIsdMaskWords; ++sl)
@@ -1464,22 +1490,25 @@ __global__ __launch_bounds__(256) void ga_isd_select_kernel(int rows,
         if (lane == 0)
             chosen[(size_t)isl * rows + count] = (int16_t)idx;
         ++count;
-        kept_isd += idx < rows ? 1 : 0;
+        kept_isd += idx < n_r ? 1 : 0;
     }
     if (lane == 0 && kept_isd)
         atomicAdd(n_isd, (unsigned)kept_isd);
 }
 
-// New base: row k = R row chosen[k] or B row chosen[k] - rows, into `out`.
+// New base: row k = row c % rows of trial c / rows's R (c = chosen[k] <
+// n_trials * rows) or B row c - n_trials * rows, into `out`.
 __global__ void ga_isd_apply_kernel(const uint64_t* __restrict__ base,
                                     const uint64_t* __restrict__ rbuf, int rows, int words,
-                                    const int16_t* __restrict__ chosen,
+                                    int n_trials, const int16_t* __restrict__ chosen,
                                     uint64_t* __restrict__ out) {
     const int isl = blockIdx.y, k = blockIdx.x;
     const int c = chosen[(size_t)isl * rows + k];
+    const int n_r = n_trials * rows;
     const size_t per = (size_t)rows * words;
-    const uint64_t* src = c < rows ? rbuf + isl * per + (size_t)c * words
-                                   : base + isl * per + (size_t)(c - rows) * words;
+    const uint64_t* src = c < n_r ? rbuf + ((size_t)(c / rows) * gridDim.y + isl) * per +
+                                        (size_t)(c % rows) * words
+                                  : base + isl * per + (size_t)(c - n_r) * words;
     uint64_t* dst = out + isl * per + (size_t)k * words;
     for (int w = threadIdx.x; w < words; w += blockDim.x)
         dst[w] = src[w];
@@ -1835,24 +1864,28 @@ unsigned int isd_improvement(Ga& G) {
     unsigned int* cnt = static_cast<unsigned int*>(G.isd_count.ptr);
     fill_u32(cnt, 1, 0u, G.stream);
     refresh_sketches(G);
-    for (int t = 0; t < G.P.isd_trials; ++t) {
-        const uint64_t seed = (uint64_t)G.P.seed ^ (0xA0761D6478BD642Full * (++G.isd_round));
-        hipLaunchKernelGGL(ga_isd_pivot_kernel, dim3(I), dim3(256), 0, G.stream,
+    // Rounds of up to kIsdBatch trials side by side (same base, one
+    // selection over all their rows and the island's).
+    for (int t0 = 0; t0 < G.P.isd_trials; t0 += kIsdBatch) {
+        const int nb = std::min(kIsdBatch, G.P.isd_trials - t0);
+        const uint64_t round0 = G.isd_round + 1;
+        G.isd_round += (uint64_t)nb;
+        hipLaunchKernelGGL(ga_isd_pivot_kernel, dim3(I, nb), dim3(256), 0, G.stream,
                            static_cast<const uint64_t*>(G.base.ptr), R, G.words, G.words * 64,
-                           seed, static_cast<uint64_t*>(G.isd_t.ptr));
-        fill_u32(static_cast<uint32_t*>(G.isd_w.ptr), (size_t)I * R, 0u, G.stream);
-        hipLaunchKernelGGL(ga_isd_rows_kernel, dim3(G.words / kIsdSlice, I), dim3(256), 0,
+                           (uint64_t)G.P.seed, round0, static_cast<uint64_t*>(G.isd_t.ptr));
+        fill_u32(static_cast<uint32_t*>(G.isd_w.ptr), (size_t)nb * I * R, 0u, G.stream);
+        hipLaunchKernelGGL(ga_isd_rows_kernel, dim3(G.words / kIsdSlice, I, nb), dim3(256), 0,
                            G.stream, static_cast<const uint64_t*>(G.base.ptr), R, G.words,
                            static_cast<const uint64_t*>(G.isd_t.ptr),
                            static_cast<uint64_t*>(G.isd_r.ptr), static_cast<uint32_t*>(G.isd_w.ptr));
-        hipLaunchKernelGGL(ga_isd_select_kernel, dim3(I), dim3(256), 0, G.stream, R,
+        hipLaunchKernelGGL(ga_isd_select_kernel, dim3(I), dim3(256), 0, G.stream, R, nb,
                            static_cast<const uint64_t*>(G.isd_t.ptr),
                            static_cast<const uint32_t*>(G.isd_w.ptr),
                            static_cast<const uint32_t*>(G.roww.ptr),
                            static_cast<int16_t*>(G.isd_chosen.ptr), cnt);
         hipLaunchKernelGGL(ga_isd_apply_kernel, dim3(R, I), dim3(128), 0, G.stream,
                            static_cast<const uint64_t*>(G.base.ptr),
-                           static_cast<const uint64_t*>(G.isd_r.ptr), R, G.words,
+                           static_cast<const uint64_t*>(G.isd_r.ptr), R, G.words, nb,
                            static_cast<const int16_t*>(G.isd_chosen.ptr),
                            static_cast<uint64_t*>(G.base_tmp.ptr));
         BMM_HIP(hipGetLastError());
@@ -2046,9 +2079,10 @@ Ga* ga_create(const uint64_t* matrix, size_t rows, size_t cols, const bmm_ga_par
         G->win_w.get<uint32_t>((size_t)I * rows);
     }
     if (P.isd_trials) {
-        G->isd_r.get<uint64_t>((size_t)I * rows * G->words);
-        G->isd_t.get<uint64_t>((size_t)I * rows * kIsdMaskWords);
-        G->isd_w.get<uint32_t>((size_t)I * rows);
+        const size_t nb = (size_t)std::min(kIsdBatch, P.isd_trials);
+        G->isd_r.get<uint64_t>(nb * I * rows * G->words);
+        G->isd_t.get<uint64_t>(nb * I * rows * kIsdMaskWords);
+        G->isd_w.get<uint32_t>(nb * I * rows);
         G->isd_chosen.get<int16_t>((size_t)I * rows);
         G->isd_count.get<uint32_t>(1);
     }

@@ -314,7 +314,9 @@ typedef struct bmm_ga_params {
     int win_candidates; /* learned: candidates scored per target, best win_rows kept */
     int isd_trials;     /* information-set trials per island per epoch (0 = off, <= 64):
                            Gauss-Jordan elimination on random columns, then the lightest
-                           basis from those rows and the island's (GlobalGreedyApplier) */
+                           basis from those rows and the island's (GlobalGreedyApplier);
+                           up to 3 trials run side by side from one base and are selected
+                           from together */
     /* The reference GA's operator settings (GeneticAlgorithmConfig,
      * include/bmmpy/ga/genetic_algorithm.hpp:17-29; src/ga/genetic_algorithm_*.cpp). */
     int num_parents;          /* tournament winners per crossover, 1..4 (1: copy) */
