@@ -1427,36 +1427,38 @@ __global__ __launch_bounds__(256) void ga_isd_select_kernel(int rows, int n_tria
             }
             __syncthreads();
         }
-    // Selection on one wavefront, the basis in registers: lane l holds the
-    // rows whose pivot is bit 64 s + l (s < 4).  The basis stays fully
-    // reduced (no row holds another row's pivot bit), so an entry's mask m
-    // reduces to m ^ (XOR of the rows whose pivot bit m has) in one step.
+    // Selection on one wavefront, the basis held by columns: basis row r is
+    // the row whose pivot is bit r, and lane l holds, for its bit positions
+    // j = 64 s + l (s < 4), the 256-bit column col[s] (bit r = row r's bit
+    // j).  The basis stays fully reduced (no row holds another row's pivot
+    // bit), so an entry's mask m reduces to v = m ^ x with x_j =
+    // parity(col_j & m & used): four parities per lane and four ballots, no
+    // cross-lane reductions.  A new row v of pivot pv is XORed into the rows
+    // holding bit pv (column pv, broadcast from its lane) and installed as
+    // row pv.
     if (wave != 0)
         return;
-    uint64_t bas[kIsdMaskWords][kIsdMaskWords] = {};   // [slot][word]
-    bool used_s[kIsdMaskWords] = {};
+    uint64_t col[kIsdMaskWords][kIsdMaskWords] = {};   // [slot s][word of r]
+    uint64_t used[kIsdMaskWords] = {};                  // pivot rows taken (uniform)
     int count = 0, kept_isd = 0;
     for (int e = 0; e < pool && count < rows; ++e) {
         const int idx = (int)(key[e] & 1023u);
-        uint64_t m[kIsdMaskWords];
-#pragma unroll
-        for (int q = 0; q < kIsdMaskWords; ++q)
-            m[q] = idx < n_r ? tm[idx][q]
-                             : ((idx - n_r) / 64 == q ? 1ull << ((idx - n_r) % 64) : 0ull);
-        uint64_t x[kIsdMaskWords] = {};
-#pragma unroll
-        for (int sl = 0; sl < kIsdMaskWords; ++sl)
-            if (used_s[sl] && ((m[sl] >> lane) & 1ull)) {
-#pragma unroll
-                for (int q = 0; q < kIsdMaskWords; ++q)
-                    x[q] ^= bas[sl][q];
-            }
-        uint64_t v[kIsdMaskWords];
+        uint64_t m[kIsdMaskWords], mu[kIsdMaskWords];
 #pragma unroll
         for (int q = 0; q < kIsdMaskWords; ++q) {
-            for (int o = kWave / 2; o > 0; o >>= 1)
-                x[q] ^= __shfl_xor(x[q], o, kWave);
-            v[q] = m[q] ^ x[q];
+            m[q] = idx < n_r ? tm[idx][q]
+                             : ((idx - n_r) / 64 == q ? 1ull << ((idx - n_r) % 64) : 0ull);
+            mu[q] = m[q] & used[q];
+        }
+        uint64_t v[kIsdMaskWords];
+#pragma unroll
+        for (int sl = 0; sl < kIsdMaskWords; ++sl) {
+            uint64_t a = 0;
+#pragma unroll
+            for (int q = 0; q < kIsdMaskWords; ++q)
+                a ^= col[sl][q] & mu[q];
+            const bool xb = (__builtin_popcountll(a) & 1) != 0;
+            v[sl] = m[sl] ^ (uint64_t)__ballot(xb);
         }
         int pv = -1;   // highest set bit of v (the same on every lane)
 #pragma unroll
@@ -1466,27 +1468,34 @@ __global__ __launch_bounds__(256) void ga_isd_select_kernel(int rows, int n_tria
         if (pv < 0)
             continue;   // dependent on the kept entries
         const int ps = pv / 64, pl = pv % 64;
-        // Clear bit pv from the kept rows (pv is not yet a pivot), then
-        // install v as the row of pivot pv.
+        // Column pv: the rows that hold bit pv.
+        uint64_t cp[kIsdMaskWords];
 #pragma unroll
-        for (int sl = 0; sl < kIsdMaskWords; ++sl) {
-            uint64_t hb = 0;
+        for (int q = 0; q < kIsdMaskWords; ++q) {
+            uint64_t src = 0;
 #pragma unroll
-            for (int q = 0; q < kIsdMaskWords; ++q)
-                if (q == ps)
-                    hb = bas[sl][q];
-            if (used_s[sl] && ((hb >> pl) & 1ull)) {
-#pragma unroll
-                for (int q = 0; q < kIsdMaskWords; ++q)
-                    bas[sl][q] ^= v[q];
-            }
-            if (sl == ps && lane == pl) {
-#pragma unroll
-                for (int q = 0; q < kIsdMaskWords; ++q)
-                    bas[sl][q] = v[q];
-                used_s[sl] = true;
-            }
+            for (int sl = 0; sl < kIsdMaskWords; ++sl)
+                if (sl == ps)
+                    src = col[sl][q];
+            const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)src, pl);
+            const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(src >> 32), pl);
+            cp[q] = ((uint64_t)hi << 32) | lo;
         }
+        // Rows holding bit pv take v (clearing their bit pv); row pv = v.
+#pragma unroll
+        for (int sl = 0; sl < kIsdMaskWords; ++sl)
+            if ((v[sl] >> lane) & 1ull) {
+#pragma unroll
+                for (int q = 0; q < kIsdMaskWords; ++q) {
+                    col[sl][q] ^= cp[q];
+                    if (q == ps)
+                        col[sl][q] |= 1ull << pl;
+                }
+            }
+#pragma unroll
+        for (int q = 0; q < kIsdMaskWords; ++q)
+            if (q == ps)
+                used[q] |= 1ull << pl;
         if (lane == 0)
             chosen[(size_t)isl * rows + count] = (int16_t)idx;
         ++count;
