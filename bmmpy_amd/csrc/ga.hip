@@ -1201,13 +1201,13 @@ __global__ void ga_win_apply_kernel(uint64_t* __restrict__ base, int rows, int w
 // cols) do not depend on the matrix, and row r's current bit in column c is
 // parity(T_r & a(c)) with a(c) column c of B, so the pivoting runs on the
 // 256-bit coefficient masks alone (ga_isd_pivot_kernel: one workgroup per
-// island, thread r owns T_r, the columns of the next kIsdDraws draws are
-// gathered as 256-bit vectors first, one barrier per draw).  R = T B is then
+// island and trial gathers the columns of the next kIsdDraws draws as
+// 256-bit vectors, then one wavefront runs the draws with every mask in
+// registers).  R = T B is then
 // one GF(2) product (ga_isd_rows_kernel), with the row weights.  Both apply
 // exactly the row operations of elimination on the full rows.
 constexpr int kIsdMaskWords = 4;   // rows <= 256
 constexpr int kIsdDraws = 512;     // draws whose columns are gathered at once
-constexpr int kIsdGather = 32;     // gather loads in flight per thread
 // Trials of one round run side by side from the same base (grid y) and are
 // selected from together.  A trial's eliminated rows depend only on the row
 // space and its draws (they are the rows reduced on its pivot columns), and
@@ -1222,7 +1222,38 @@ __host__ __device__ inline uint64_t isd_seed(uint64_t seed, uint64_t round) {
     return seed ^ (0xA0761D6478BD642Full * round);
 }
 
-__global__ __launch_bounds__(256) void ga_isd_pivot_kernel(const uint64_t* __restrict__ base,
+// The draw chain runs on one wavefront with every mask in registers: lane
+// l holds T_r for the rows r = 64 s + l (s < 4), a draw's candidate rows
+// are four ballots, and the pivot row's mask is broadcast with readlane, so
+// a draw takes no barrier and no LDS round trip (all 256 threads gather the
+// drawn columns first).
+// The base's columns as 256-bit vectors, colmaj[isl][c][q] bit i = row 64 q
+// + i's bit in column c (one round of trials reads a drawn column as one
+// 32-byte vector instead of a word from each row).  Grid: words x islands;
+// thread r holds row r's word, each wavefront ballots its 64 rows per bit.
+__global__ __launch_bounds__(256) void ga_isd_transpose_kernel(const uint64_t* __restrict__ base,
+                                                               int rows, int words,
+                                                               uint64_t* __restrict__ colmaj) {
+    const int w = blockIdx.x, isl = blockIdx.y, tid = threadIdx.x;
+    const int wave = tid / kWave, lane = tid % kWave;
+    const uint64_t x =
+        tid < rows ? base[((size_t)isl * rows + tid) * words + w] : 0ull;
+    uint64_t* out = colmaj + ((size_t)isl * words * 64 + (size_t)w * 64) * kIsdMaskWords;
+#pragma unroll 8
+    for (int b = 0; b < 64; ++b) {
+        const uint64_t bal = __ballot((x >> b) & 1ull);
+        if (lane == 0)
+            out[(size_t)b * kIsdMaskWords + wave] = bal;
+    }
+}
+
+__device__ inline uint64_t readlane_u64(uint64_t x, int lane) {
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)x, lane);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(x >> 32), lane);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+__global__ __launch_bounds__(256) void ga_isd_pivot_kernel(const uint64_t* __restrict__ colmaj,
                                                            int rows, int words, int cols,
                                                            uint64_t seed0, uint64_t round0,
                                                            uint64_t* __restrict__ tmask_all) {
@@ -1230,19 +1261,22 @@ __global__ __launch_bounds__(256) void ga_isd_pivot_kernel(const uint64_t* __res
     const uint64_t seed = isd_seed(seed0, round0 + blockIdx.y);
     uint64_t* tmask = tmask_all + (size_t)blockIdx.y * gridDim.x * rows * kIsdMaskWords;
     const int wave = tid / kWave, lane = tid % kWave;
-    const uint64_t* B = base + (size_t)isl * rows * words;
+    const uint64_t* A = colmaj + (size_t)isl * words * 64 * kIsdMaskWords;   // columns
     __shared__ uint64_t colv[kIsdDraws][kIsdMaskWords];   // a(c) of the chunk's draws
     __shared__ int cidx[kIsdDraws];
     __shared__ uint64_t state[kIsdDraws];
-    __shared__ uint64_t tm[256][kIsdMaskWords];           // every row's current mask
-    __shared__ uint64_t cand[2][kIsdMaskWords];           // rows that could pivot, per draw parity
-    uint64_t t[kIsdMaskWords] = {};
-    if (tid < rows)
-        t[tid / 64] = 1ull << (tid % 64);
+    __shared__ int s_npiv;
+    // Wave 0's registers: t[s] = T of row 64 s + lane; unpiv = rows not yet
+    // pivoted (bit l of word s: row 64 s + l; the same on every lane).
+    uint64_t t[kIsdMaskWords][kIsdMaskWords] = {};
+    uint64_t unpiv[kIsdMaskWords] = {};
 #pragma unroll
-    for (int q = 0; q < kIsdMaskWords; ++q)
-        tm[tid][q] = t[q];
-    bool pivoted = false;
+    for (int sl = 0; sl < kIsdMaskWords; ++sl) {
+        if (64 * sl + lane < rows)
+            t[sl][sl] = 1ull << lane;
+        const int n = min(64, max(0, rows - 64 * sl));
+        unpiv[sl] = n >= 64 ? ~0ull : ((1ull << n) - 1ull);
+    }
     uint64_t rng = seed ^ (0x9E3779B97F4A7C15ull * (uint64_t)(isl + 1));
     if (rng == 0)
         rng = 1;
@@ -1264,57 +1298,75 @@ __global__ __launch_bounds__(256) void ga_isd_pivot_kernel(const uint64_t* __res
         for (int d = tid; d < nd; d += blockDim.x)
             cidx[d] = (int)(state[d] % (uint64_t)cols);
         __syncthreads();
-        // Gather: thread r reads its row's word of each drawn column
-        // (kIsdGather loads in flight), each wave ballots its 64 rows' bits.
-        const uint64_t* row = B + (size_t)min(tid, rows - 1) * words;
-        for (int d = 0; d < nd; d += kIsdGather) {
-            uint64_t w[kIsdGather];
-#pragma unroll
-            for (int j = 0; j < kIsdGather; ++j)
-                w[j] = row[cidx[min(d + j, nd - 1)] >> 6];
-#pragma unroll
-            for (int j = 0; j < kIsdGather; ++j) {
-                const int c = cidx[min(d + j, nd - 1)];
-                const uint64_t bal = __ballot(tid < rows && ((w[j] >> (c & 63)) & 1ull));
-                if (lane == 0 && d + j < nd)
-                    colv[d + j][wave] = bal;
-            }
-        }
+        // The drawn columns: one 32-byte vector each.
+        for (int i = tid; i < nd * kIsdMaskWords; i += blockDim.x)
+            colv[i / kIsdMaskWords][i % kIsdMaskWords] =
+                A[(size_t)cidx[i / kIsdMaskWords] * kIsdMaskWords + i % kIsdMaskWords];
         __syncthreads();
-        for (int d = 0; d < nd && npiv < rows; ++d) {
-            uint32_t par = 0;
+        if (wave == 0) {
+            for (int d = 0; d < nd && npiv < rows; ++d) {
+                uint64_t a[kIsdMaskWords];
 #pragma unroll
-            for (int q = 0; q < kIsdMaskWords; ++q)
-                par += (uint32_t)__builtin_popcountll(t[q] & colv[d][q]);
-            const bool bit = (par & 1u) != 0u;
-            const uint64_t c = __ballot(bit && !pivoted);
-            if (lane == 0)
-                cand[d & 1][wave] = c;
-            __syncthreads();   // cand written; masks of the previous draw written
-            int r = -1;
+                for (int q = 0; q < kIsdMaskWords; ++q)
+                    a[q] = colv[d][q];
+                // bal[s] bit l: row 64 s + l has the column's bit.
+                uint64_t bal[kIsdMaskWords];
 #pragma unroll
-            for (int q = 0; q < kIsdMaskWords; ++q)
-                if (r < 0 && cand[d & 1][q] != 0ull)
-                    r = q * 64 + __builtin_ctzll(cand[d & 1][q]);
-            if (r < 0)
-                continue;   // no row can pivot on this column
-            ++npiv;
-            if (tid == r) {
-                pivoted = true;
-            } else if (bit) {
+                for (int sl = 0; sl < kIsdMaskWords; ++sl) {
+                    uint32_t par = 0;
+#pragma unroll
+                    for (int q = 0; q < kIsdMaskWords; ++q)
+                        par += (uint32_t)__builtin_popcountll(t[sl][q] & a[q]);
+                    bal[sl] = __ballot((par & 1u) != 0u);
+                }
+                int r = -1;   // lowest row not yet pivoted that has the bit
+#pragma unroll
+                for (int sl = 0; sl < kIsdMaskWords; ++sl)
+                    if (r < 0 && (bal[sl] & unpiv[sl]) != 0ull)
+                        r = 64 * sl + __builtin_ctzll(bal[sl] & unpiv[sl]);
+                if (r < 0)
+                    continue;   // no row can pivot on this column
+                ++npiv;
+                const int rs = r / 64, rl = r % 64;
+                uint64_t tr[kIsdMaskWords];
 #pragma unroll
                 for (int q = 0; q < kIsdMaskWords; ++q) {
-                    t[q] ^= tm[r][q];
-                    tm[tid][q] = t[q];
+                    uint64_t src = 0;
+#pragma unroll
+                    for (int sl = 0; sl < kIsdMaskWords; ++sl)
+                        if (sl == rs)
+                            src = t[sl][q];
+                    tr[q] = readlane_u64(src, rl);
+                }
+#pragma unroll
+                for (int sl = 0; sl < kIsdMaskWords; ++sl) {
+                    if (sl == rs)
+                        unpiv[sl] &= ~(1ull << rl);
+                    // Every other row with the bit takes the pivot row.
+                    if (((bal[sl] >> lane) & 1ull) && !(sl == rs && lane == rl)) {
+#pragma unroll
+                        for (int q = 0; q < kIsdMaskWords; ++q)
+                            t[sl][q] ^= tr[q];
+                    }
                 }
             }
+            if (lane == 0)
+                s_npiv = npiv;
         }
+        __syncthreads();
+        npiv = s_npiv;
     }
-    if (tid < rows) {
-        uint64_t* T = tmask + ((size_t)isl * rows + tid) * kIsdMaskWords;
+    if (wave == 0) {
 #pragma unroll
-        for (int q = 0; q < kIsdMaskWords; ++q)
-            T[q] = t[q];
+        for (int sl = 0; sl < kIsdMaskWords; ++sl) {
+            const int r = 64 * sl + lane;
+            if (r < rows) {
+                uint64_t* T = tmask + ((size_t)isl * rows + r) * kIsdMaskWords;
+#pragma unroll
+                for (int q = 0; q < kIsdMaskWords; ++q)
+                    T[q] = t[sl][q];
+            }
+        }
     }
 }
 
@@ -1552,7 +1604,7 @@ struct Ga {
     DevBuf pairs, pair_isl, feats, labels, prng;
     DevBuf li_key, li_changed;
     DevBuf win_cand, win, win_move, win_w;
-    DevBuf isd_r, isd_t, isd_w, isd_chosen, isd_count;
+    DevBuf isd_r, isd_t, isd_w, isd_chosen, isd_count, isd_cols;
     DevBuf isl_state, mig;
     uint64_t win_round = 0;
     uint64_t win_moves = 0;
@@ -1879,9 +1931,13 @@ unsigned int isd_improvement(Ga& G) {
         const int nb = std::min(kIsdBatch, G.P.isd_trials - t0);
         const uint64_t round0 = G.isd_round + 1;
         G.isd_round += (uint64_t)nb;
+        hipLaunchKernelGGL(ga_isd_transpose_kernel, dim3(G.words, I), dim3(256), 0, G.stream,
+                           static_cast<const uint64_t*>(G.base.ptr), R, G.words,
+                           static_cast<uint64_t*>(G.isd_cols.ptr));
         hipLaunchKernelGGL(ga_isd_pivot_kernel, dim3(I, nb), dim3(256), 0, G.stream,
-                           static_cast<const uint64_t*>(G.base.ptr), R, G.words, G.words * 64,
-                           (uint64_t)G.P.seed, round0, static_cast<uint64_t*>(G.isd_t.ptr));
+                           static_cast<const uint64_t*>(G.isd_cols.ptr), R, G.words,
+                           G.words * 64, (uint64_t)G.P.seed, round0,
+                           static_cast<uint64_t*>(G.isd_t.ptr));
         fill_u32(static_cast<uint32_t*>(G.isd_w.ptr), (size_t)nb * I * R, 0u, G.stream);
         hipLaunchKernelGGL(ga_isd_rows_kernel, dim3(G.words / kIsdSlice, I, nb), dim3(256), 0,
                            G.stream, static_cast<const uint64_t*>(G.base.ptr), R, G.words,
@@ -2094,6 +2150,7 @@ Ga* ga_create(const uint64_t* matrix, size_t rows, size_t cols, const bmm_ga_par
         G->isd_w.get<uint32_t>(nb * I * rows);
         G->isd_chosen.get<int16_t>((size_t)I * rows);
         G->isd_count.get<uint32_t>(1);
+        G->isd_cols.get<uint64_t>((size_t)I * G->words * 64 * kIsdMaskWords);
     }
     if (P.learned)
         G->mlp = mlp_create(P.seed ^ 0xA5A5A5A5ull);
