@@ -1304,11 +1304,18 @@ __global__ __launch_bounds__(256) void ga_isd_pivot_kernel(const uint64_t* __res
                 A[(size_t)cidx[i / kIsdMaskWords] * kIsdMaskWords + i % kIsdMaskWords];
         __syncthreads();
         if (wave == 0) {
+            uint64_t cv[kIsdMaskWords];   // lane l: the column of draw 64 (d / 64) + l
             for (int d = 0; d < nd && npiv < rows; ++d) {
+                if ((d & 63) == 0) {
+                    const int dd = min(d + lane, nd - 1);
+#pragma unroll
+                    for (int q = 0; q < kIsdMaskWords; ++q)
+                        cv[q] = colv[dd][q];
+                }
                 uint64_t a[kIsdMaskWords];
 #pragma unroll
                 for (int q = 0; q < kIsdMaskWords; ++q)
-                    a[q] = colv[d][q];
+                    a[q] = readlane_u64(cv[q], d & 63);
                 // bal[s] bit l: row 64 s + l has the column's bit.
                 uint64_t bal[kIsdMaskWords];
 #pragma unroll
@@ -1493,13 +1500,21 @@ __global__ __launch_bounds__(256) void ga_isd_select_kernel(int rows, int n_tria
     uint64_t col[kIsdMaskWords][kIsdMaskWords] = {};   // [slot s][word of r]
     uint64_t used[kIsdMaskWords] = {};                  // pivot rows taken (uniform)
     int count = 0, kept_isd = 0;
+    int il = 0;                    // lane l: pool index of entry 64 (e / 64) + l
+    uint64_t ml[kIsdMaskWords];    // and its mask
     for (int e = 0; e < pool && count < rows; ++e) {
-        const int idx = (int)(key[e] & 1023u);
+        if ((e & 63) == 0) {
+            il = (int)(key[min(e + lane, pool - 1)] & 1023u);
+#pragma unroll
+            for (int q = 0; q < kIsdMaskWords; ++q)
+                ml[q] = il < n_r ? tm[il][q]
+                                 : ((il - n_r) / 64 == q ? 1ull << ((il - n_r) % 64) : 0ull);
+        }
+        const int idx = __builtin_amdgcn_readlane(il, e & 63);
         uint64_t m[kIsdMaskWords], mu[kIsdMaskWords];
 #pragma unroll
         for (int q = 0; q < kIsdMaskWords; ++q) {
-            m[q] = idx < n_r ? tm[idx][q]
-                             : ((idx - n_r) / 64 == q ? 1ull << ((idx - n_r) % 64) : 0ull);
+            m[q] = readlane_u64(ml[q], e & 63);
             mu[q] = m[q] & used[q];
         }
         uint64_t v[kIsdMaskWords];
