@@ -11,6 +11,7 @@
 // products (weight gradients, backward activations) need no copies.
 #include "common.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <memory>
 #include <vector>
@@ -28,37 +29,59 @@ __device__ inline __bf16 to_bf16(float x) { return (__bf16)x; }
 
 // C[m][n] = sum_k A(m, k) B(k, n) (+ bias[n]); optional ReLU.  Writes fp32
 // (pre-activation) to Cf and/or bf16 (post-activation) to Cb.  M, N multiples
-// of 32, K a multiple of 16.
+// of 32, K a multiple of 16.  AK / BK: A / B contiguous along k (stride 1,
+// 16-byte aligned rows), read as one 8-element vector per lane and step.
+// With gridDim.y > 1 the K range is cut into gridDim.y slices of k_chunk:
+// slice y writes its raw partial tile to part[y][M][N] and
+// gemm_reduce_kernel adds the slices in order (no bias, no ReLU, fp32 only).
+template <bool AK, bool BK>
 __global__ __launch_bounds__(64) void gemm_bf16_kernel(
     const __bf16* __restrict__ A, long sam, long sak, const __bf16* __restrict__ B, long sbk,
-    long sbn, int M, int N, int K, const float* __restrict__ bias, int relu,
-    float* __restrict__ Cf, long ldcf, __bf16* __restrict__ Cb, long ldcb, float beta) {
+    long sbn, int M, int N, int K, int k_chunk, const float* __restrict__ bias, int relu,
+    float* __restrict__ Cf, long ldcf, __bf16* __restrict__ Cb, long ldcb,
+    float* __restrict__ part) {
     const int tiles_n = N / 32;
     const int tm = blockIdx.x / tiles_n, tn = blockIdx.x % tiles_n;
     const int lane = threadIdx.x;
     const int r = lane % 32, half = lane / 32;
     const long am = (long)(tm * 32 + r);
     const long bn = (long)(tn * 32 + r);
+    const int kb = blockIdx.y * k_chunk;
+    const int ke = min(K, kb + k_chunk);
     f32x16 acc = {};
-    for (int k0 = 0; k0 < K; k0 += 16) {
+    for (int k0 = kb; k0 < ke; k0 += 16) {
+        const long kh = k0 + 8 * half;
         bf16x8 va, vb;
+        if constexpr (AK) {
+            va = *reinterpret_cast<const bf16x8*>(A + am * sam + kh);
+        } else {
 #pragma unroll
-        for (int i = 0; i < 8; ++i) {
-            const long k = k0 + 8 * half + i;
-            va[i] = A[am * sam + k * sak];
-            vb[i] = B[k * sbk + bn * sbn];
+            for (int i = 0; i < 8; ++i)
+                va[i] = A[am * sam + (kh + i) * sak];
+        }
+        if constexpr (BK) {
+            vb = *reinterpret_cast<const bf16x8*>(B + kh + bn * sbn);
+        } else {
+#pragma unroll
+            for (int i = 0; i < 8; ++i)
+                vb[i] = B[(kh + i) * sbk + bn * sbn];
         }
         acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(va, vb, acc, 0, 0, 0);
     }
     const int col = tn * 32 + r;
+    if (gridDim.y > 1) {
+        float* P = part + (size_t)blockIdx.y * M * N;
+#pragma unroll
+        for (int j = 0; j < 16; ++j)
+            P[(long)(tm * 32 + 8 * (j / 4) + 4 * half + (j % 4)) * N + col] = acc[j];
+        return;
+    }
 #pragma unroll
     for (int j = 0; j < 16; ++j) {
         const int row = tm * 32 + 8 * (j / 4) + 4 * half + (j % 4);
         float v = acc[j] + (bias ? bias[col] : 0.f);
-        if (Cf) {
-            float* p = &Cf[(long)row * ldcf + col];
-            *p = beta != 0.f ? beta * *p + v : v;
-        }
+        if (Cf)
+            Cf[(long)row * ldcf + col] = v;
         if (relu)
             v = v > 0.f ? v : 0.f;
         if (Cb)
@@ -66,14 +89,58 @@ __global__ __launch_bounds__(64) void gemm_bf16_kernel(
     }
 }
 
+// Cf[m][n] = sum over s of part[s][m][n], s in increasing order.
+__global__ void gemm_reduce_kernel(const float* __restrict__ part, int slices, int M, int N,
+                                   float* __restrict__ Cf, long ldcf) {
+    const size_t MN = (size_t)M * N;
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < MN;
+         e += (size_t)gridDim.x * blockDim.x) {
+        float v = 0.f;
+        for (int s = 0; s < slices; ++s)
+            v += part[s * MN + e];
+        Cf[(e / N) * ldcf + e % N] = v;
+    }
+}
+
+// Launches the product.  A product with fewer than 256 output tiles, a long
+// K (>= 1024) and only an fp32 output (the weight gradients, K = the batch)
+// is cut along K into up to 16 slices of at least 256 so that the launch has
+// enough wavefronts; `part` must then hold slices * M * N floats (pass null
+// to keep one slice).  The slice sums are added in a fixed order, so results
+// are deterministic for a given K.
 void gemm_bf16(const __bf16* A, long sam, long sak, const __bf16* B, long sbk, long sbn, int M,
                int N, int K, const float* bias, bool relu, float* Cf, long ldcf, __bf16* Cb,
-               long ldcb, hipStream_t s, float beta = 0.f) {
+               long ldcb, hipStream_t s, DevBuf* part = nullptr) {
     if (M % 32 || N % 32 || K % 16)
         fail(BMM_ERR_INVALID, "gemm_bf16: shapes must be multiples of 32 x 32 x 16");
-    hipLaunchKernelGGL(gemm_bf16_kernel, dim3((M / 32) * (N / 32)), dim3(64), 0, s, A, sam, sak,
-                       B, sbk, sbn, M, N, K, bias, relu ? 1 : 0, Cf, ldcf, Cb, ldcb, beta);
+    const int tiles = (M / 32) * (N / 32);
+    int slices = 1;
+    if (part && tiles < 256 && K >= 1024 && !bias && !relu && !Cb && Cf)
+        slices = std::min(16, K / 256);
+    const int chunk = (K / 16 + slices - 1) / slices * 16;
+    float* P = slices > 1 ? part->get<float>((size_t)slices * M * N) : nullptr;
+    const auto al16 = [](const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; };
+    const bool ak = sak == 1 && sam % 8 == 0 && al16(A);
+    const bool bk = sbk == 1 && sbn % 8 == 0 && al16(B);
+    const dim3 grid((unsigned)tiles, (unsigned)slices);
+#define BMM_GEMM_LAUNCH(AKV, BKV)                                                              \
+    hipLaunchKernelGGL((gemm_bf16_kernel<AKV, BKV>), grid, dim3(64), 0, s, A, sam, sak, B, sbk,  \
+                       sbn, M, N, K, chunk, bias, relu ? 1 : 0, Cf, ldcf, Cb, ldcb, P)
+    if (ak && bk)
+        BMM_GEMM_LAUNCH(true, true);
+    else if (ak)
+        BMM_GEMM_LAUNCH(true, false);
+    else if (bk)
+        BMM_GEMM_LAUNCH(false, true);
+    else
+        BMM_GEMM_LAUNCH(false, false);
+#undef BMM_GEMM_LAUNCH
     BMM_HIP(hipGetLastError());
+    if (slices > 1) {
+        hipLaunchKernelGGL(gemm_reduce_kernel, dim3(256), dim3(256), 0, s, P, slices, M, N, Cf,
+                           ldcf);
+        BMM_HIP(hipGetLastError());
+    }
 }
 
 __global__ void f32_to_bf16_kernel(const float* __restrict__ x, __bf16* __restrict__ y, size_t n) {
@@ -292,6 +359,7 @@ struct Mlp {
     DevBuf wbt;   // bf16 transposes W1^T | W2^T ([out][in]) for the scoring kernel
     DevBuf score; // outputs of mlp_score_pairs_dev
     DevBuf act;   // activations / scratch for the current batch
+    DevBuf part;  // K-slice partial tiles of the weight-gradient products
     int cap_batch = 0;
     long step = 0;
     float beta1 = 0.9f, beta2 = 0.999f, eps = 1e-8f;
@@ -460,7 +528,7 @@ float* mlp_backward_dev(Mlp* M, const __bf16* d_x, const float* d_t, int B, int 
     BMM_HIP(hipGetLastError());
     // dW2 = A1^T dZ2 ; db2 = colsum(dZ2)
     gemm_bf16(a.A1, 1, kHid, a.dZ2b, kHid, 1, kHid, kHid, B, nullptr, false, g + kOff.W2, kHid,
-              nullptr, 0, s);
+              nullptr, 0, s, &M->part);
     hipLaunchKernelGGL(colsum_kernel, dim3(kHid), dim3(256), 0, s, a.dZ2f, B, g + kOff.b2);
     // dA1 = dZ2 W2^T, masked by Z1 > 0
     gemm_bf16(a.dZ2b, kHid, 1, W2, 1, kHid, B, kHid, kHid, nullptr, false, a.dZ1f, kHid,
@@ -469,7 +537,7 @@ float* mlp_backward_dev(Mlp* M, const __bf16* d_x, const float* d_t, int B, int 
                        (size_t)B * kHid);
     // dW1 = X^T dZ1 ; db1 = colsum(dZ1)
     gemm_bf16(d_x, 1, kIn, a.dZ1b, kHid, 1, kIn, kHid, B, nullptr, false, g + kOff.W1, kHid,
-              nullptr, 0, s);
+              nullptr, 0, s, &M->part);
     hipLaunchKernelGGL(colsum_kernel, dim3(kHid), dim3(256), 0, s, a.dZ1f, B, g + kOff.b1);
     BMM_HIP(hipGetLastError());
     return a.loss;

@@ -1,7 +1,9 @@
 """GPU proposal MLP (MFMA bf16 products, fp32 accumulation) against the
 float64 restatement with the same bf16 rounding points.  Tolerances:
 forward max |diff| <= 1e-3 * (1 + |y|); gradients relative L2 error <= 2e-3
-(fp32 accumulation order; ReLU masks at exactly zero may flip)."""
+(fp32 accumulation order; ReLU masks at exactly zero may flip).  Batches of
+1024 and more take the weight-gradient products cut along the batch into
+K slices (2080: slices of unequal length)."""
 import numpy as np
 import pytest
 
@@ -39,7 +41,7 @@ def test_forward_matches_oracle(lib, model, batch):
     assert np.all(np.abs(y - ref) <= 1e-3 * (1 + np.abs(ref))), np.max(np.abs(y - ref))
 
 
-@pytest.mark.parametrize("batch", [32, 96, 256])
+@pytest.mark.parametrize("batch", [32, 96, 256, 1024, 2080, 4096])
 def test_gradients_match_oracle(lib, model, batch):
     _say("mlp grads", batch)
     rng = np.random.default_rng(100 + batch)
