@@ -655,13 +655,12 @@ int bmm_hip_ga_default_params(bmm_ga_params* p) {
         p->p_delete = 0.15f;
         p->learning_rate = 1e-3f;
         p->seed = 0x5EED0005ull;
-        // Six information-set trials (two rounds of three side by side) and
-        // no window rounds per epoch: on the planted 256 x 8192 instance the
-        // trials set the number of epochs (one with five or more, two with
-        // three or four, two or three with two; median time to 1.01 x
-        // planted 0.0120 s with six, 0.0115 with five on the edge of one
-        // epoch, 0.0138 with three, 0.0167 with two, 0.0149 with nine), and
-        // window rounds (8.6 ms each) never saved one.
+        // Six information-set trials (one round, side by side) and no window
+        // rounds per epoch: on the planted 256 x 8192 instance the trials set
+        // the number of epochs (one with five or more, two with three or
+        // four, two or three with two; median time to 1.01 x planted
+        // 0.0076 s with six or five, 0.0116 with four), and window rounds
+        // (8.6 ms each) never saved one.
         p->win_rows = 20;
         p->win_rounds = 0;
         p->win_candidates = 32;

@@ -1214,7 +1214,10 @@ constexpr int kIsdDraws = 512;     // draws whose columns are gathered at once
 // the lightest basis of a union is the lightest basis of one part's
 // lightest basis and the rest (matroid greedy), so a round of trials gives a
 // basis of the same weight as the trials one after another.
-constexpr int kIsdBatch = 3;       // trials per round: a pool of at most 1024 keys
+constexpr int kIsdBatch = 6;       // trials per round
+constexpr int kIsdKeyBits = 11;    // pool index bits of a selection key
+constexpr int kIsdPool = 1 << kIsdKeyBits;   // (kIsdBatch + 1) * 256 entries fit
+static_assert((kIsdBatch + 1) * 256 <= kIsdPool, "selection pool too small");
 
 // Seed of trial `round` (counted from 1): the same stream the trials have
 // taken one after another.
@@ -1448,7 +1451,7 @@ __global__ __launch_bounds__(256) void ga_isd_select_kernel(int rows, int n_tria
     const int isl = blockIdx.x, tid = threadIdx.x;
     const int wave = tid / kWave, lane = tid % kWave;
     const size_t n_isl = gridDim.x;
-    __shared__ unsigned long long key[1024];
+    __shared__ unsigned long long key[kIsdPool];
     __shared__ uint64_t tm[kIsdBatch * 256][kIsdMaskWords];
     const int n_r = n_trials * rows;   // pool entries from the trials
     const int pool = n_r + rows;
@@ -1460,7 +1463,7 @@ __global__ __launch_bounds__(256) void ga_isd_select_kernel(int rows, int n_tria
         if (k < pool) {
             const uint32_t w = k < n_r ? rw[((size_t)(k / rows) * n_isl + isl) * rows + k % rows]
                                        : roww[(size_t)isl * rows + (k - n_r)];
-            kk = ((unsigned long long)w << 10) | (unsigned)k;
+            kk = ((unsigned long long)w << kIsdKeyBits) | (unsigned)k;
         }
         key[k] = kk;
     }
@@ -1504,7 +1507,7 @@ __global__ __launch_bounds__(256) void ga_isd_select_kernel(int rows, int n_tria
     uint64_t ml[kIsdMaskWords];    // and its mask
     for (int e = 0; e < pool && count < rows; ++e) {
         if ((e & 63) == 0) {
-            il = (int)(key[min(e + lane, pool - 1)] & 1023u);
+            il = (int)(key[min(e + lane, pool - 1)] & (kIsdPool - 1u));
 #pragma unroll
             for (int q = 0; q < kIsdMaskWords; ++q)
                 ml[q] = il < n_r ? tm[il][q]

@@ -315,7 +315,7 @@ typedef struct bmm_ga_params {
     int isd_trials;     /* information-set trials per island per epoch (0 = off, <= 64):
                            Gauss-Jordan elimination on random columns, then the lightest
                            basis from those rows and the island's (GlobalGreedyApplier);
-                           up to 3 trials run side by side from one base and are selected
+                           up to 6 trials run side by side from one base and are selected
                            from together */
     /* The reference GA's operator settings (GeneticAlgorithmConfig,
      * include/bmmpy/ga/genetic_algorithm.hpp:17-29; src/ga/genetic_algorithm_*.cpp). */
