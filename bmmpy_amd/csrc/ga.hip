@@ -791,7 +791,7 @@ __global__ void ga_select_kernel(const float* __restrict__ score,
 // source over all tiles (keys[] stays ~0 when none improves; the apply
 // kernel consumes and resets it).
 constexpr int kLiTile = 32;
-constexpr int kLiChunk = 32;   // u64 words per LDS stage
+constexpr int kLiChunk = 64;   // u64 words per LDS stage
 
 __global__ __launch_bounds__(256) void ga_li_tile_kernel(const uint64_t* __restrict__ base,
                                                          int rows, int words,
@@ -866,29 +866,42 @@ __global__ __launch_bounds__(256) void ga_li_tile_kernel(const uint64_t* __restr
 }
 
 // Applies a round's moves: r_i ^= r_j for each target i whose key names a
-// source j, with the new weight from the key; clears the key.
-__global__ void ga_li_apply_kernel(uint64_t* __restrict__ base, int rows, int words,
-                                   unsigned long long* __restrict__ keys,
-                                   uint32_t* __restrict__ roww, int bit, int phase,
-                                   unsigned int* __restrict__ changed) {
-    const int isl = blockIdx.x / rows;
-    const int i = blockIdx.x % rows;
-    if (((i >> bit) & 1) != phase)
-        return;
+// source j, with the new weight from the key; clears the key and counts the
+// moves in changed[island] (one atomic per workgroup: thousands of moves on
+// one address serialise).  One wavefront per target of the round (grid:
+// targets / 8 x islands; the k-th target is row k with `phase` inserted at
+// bit `bit`).
+constexpr int kLiApplyWaves = 8;
+__global__ __launch_bounds__(kLiApplyWaves * kWave) void ga_li_apply_kernel(uint64_t* __restrict__ base, int rows,
+                                                          int words,
+                                                          unsigned long long* __restrict__ keys,
+                                                          uint32_t* __restrict__ roww, int bit,
+                                                          int phase,
+                                                          unsigned int* __restrict__ changed) {
+    const int isl = blockIdx.y, lane = threadIdx.x % kWave;
+    const uint32_t k = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+    const uint32_t lowm = (1u << bit) - 1u;
+    const int i = (int)(((k >> bit) << (bit + 1)) | ((uint32_t)phase << bit) | (k & lowm));
+    __shared__ uint32_t s_moves;
+    if (threadIdx.x == 0)
+        s_moves = 0;
+    __syncthreads();
     const size_t id = (size_t)isl * rows + i;
-    const unsigned long long key = keys[id];
-    if (key == ~0ull)
-        return;
-    const int j = (int)(key & 0xffffffffu);
-    uint64_t* b = base + (size_t)isl * rows * words;
-    for (int w = threadIdx.x; w < words; w += blockDim.x)
-        b[(size_t)i * words + w] ^= b[(size_t)j * words + w];
-    __syncthreads();   // every thread has read the key
-    if (threadIdx.x == 0) {
-        roww[id] = (uint32_t)(key >> 32);
-        keys[id] = ~0ull;
-        atomicAdd(changed, 1u);
+    const unsigned long long key = i < rows ? keys[id] : ~0ull;
+    if (key != ~0ull) {
+        const int j = (int)(key & 0xffffffffu);
+        uint64_t* b = base + (size_t)isl * rows * words;
+        for (int w = lane; w < words; w += kWave)
+            b[(size_t)i * words + w] ^= b[(size_t)j * words + w];
+        if (lane == 0) {   // the wavefront's key load came before this store
+            roww[id] = (uint32_t)(key >> 32);
+            keys[id] = ~0ull;
+            atomicAdd(&s_moves, 1u);
+        }
     }
+    __syncthreads();
+    if (threadIdx.x == 0 && s_moves)
+        atomicAdd(&changed[isl], s_moves);
 }
 
 // Window local improvement: for each target row i of a round (index bit `bit`
@@ -1853,7 +1866,7 @@ unsigned int local_improvement(Ga& G) {
     while ((1 << bits) < R)
         ++bits;
     unsigned int* changed = static_cast<unsigned int*>(G.li_changed.ptr);
-    fill_u32(changed, 1, 0u, G.stream);
+    fill_u32(changed, (size_t)I, 0u, G.stream);
     for (int sweep = 0; sweep < G.P.li_sweeps; ++sweep)
         for (int bit = 0; bit < bits; ++bit)
             for (int phase = 0; phase < 2; ++phase) {
@@ -1862,7 +1875,13 @@ unsigned int local_improvement(Ga& G) {
                                    G.stream, static_cast<const uint64_t*>(G.base.ptr), R, G.words,
                                    static_cast<const uint32_t*>(G.roww.ptr), bit, phase,
                                    static_cast<unsigned long long*>(G.li_key.ptr));
-                hipLaunchKernelGGL(ga_li_apply_kernel, dim3((unsigned)(I * R)), dim3(128),
+                int targets = 0;   // rows whose index bit `bit` equals `phase`
+                for (int i = 0; i < R; ++i)
+                    targets += ((i >> bit) & 1) == phase;
+                hipLaunchKernelGGL(ga_li_apply_kernel,
+                                   dim3((unsigned)((targets + kLiApplyWaves - 1) / kLiApplyWaves),
+                                        (unsigned)I),
+                                   dim3(kLiApplyWaves * kWave),
                                    0, G.stream, static_cast<uint64_t*>(G.base.ptr), R, G.words,
                                    static_cast<unsigned long long*>(G.li_key.ptr),
                                    static_cast<uint32_t*>(G.roww.ptr), bit, phase, changed);
@@ -1871,9 +1890,12 @@ unsigned int local_improvement(Ga& G) {
                        static_cast<const uint32_t*>(G.roww.ptr), R,
                        static_cast<unsigned int*>(G.best_w.ptr));
     BMM_HIP(hipGetLastError());
-    unsigned int n = 0;
-    BMM_HIP(hipMemcpyAsync(&n, changed, 4, hipMemcpyDeviceToHost, G.stream));
+    std::vector<unsigned int> per(I);
+    BMM_HIP(hipMemcpyAsync(per.data(), changed, (size_t)I * 4, hipMemcpyDeviceToHost, G.stream));
     BMM_HIP(hipStreamSynchronize(G.stream));
+    unsigned int n = 0;
+    for (unsigned int c : per)
+        n += c;
     return n;
 }
 
@@ -2147,7 +2169,7 @@ Ga* ga_create(const uint64_t* matrix, size_t rows, size_t cols, const bmm_ga_par
     // Pairwise-improvement keys, ~0 between rounds (the apply kernel resets them).
     fill_u64(reinterpret_cast<uint64_t*>(G->li_key.get<unsigned long long>((size_t)I * rows)),
              (size_t)I * rows, ~0ull, G->stream);
-    G->li_changed.get<unsigned int>(1);
+    G->li_changed.get<unsigned int>((size_t)P.islands);   // per-island move counts
     const size_t n_win = P.win_rows && P.learned ? (size_t)I * rows * P.win_candidates : 0;
     const size_t n_pair_buf = std::max((size_t)n_pairs, n_win);
     G->pairs.get<uint16_t>(n_pair_buf + 32);
