@@ -1239,10 +1239,12 @@ __host__ __device__ inline uint64_t isd_seed(uint64_t seed, uint64_t round) {
 }
 
 // The draw chain runs on one wavefront with every mask in registers: lane
-// l holds T_r for the rows r = 64 s + l (s < 4), a draw's candidate rows
-// are four ballots, and the pivot row's mask is broadcast with readlane, so
-// a draw takes no barrier and no LDS round trip (all 256 threads gather the
-// drawn columns first).
+// l holds T_r for the rows r = 64 s + l (s < 4).  The drawn columns are
+// reduced against T 64 at a time by all four wavefronts and kept current
+// through the draws by one rank-1 update per pivot, and the pivot row's mask
+// is broadcast with readlane, so a draw takes no barrier, no LDS round trip
+// and no parity work of its own (all 256 threads gather the drawn columns
+// first).
 // The base's columns as 256-bit vectors, colmaj[isl][c][q] bit i = row 64 q
 // + i's bit in column c (one round of trials reads a drawn column as one
 // 32-byte vector instead of a word from each row).  Grid: words x islands;
@@ -1276,11 +1278,15 @@ __global__ __launch_bounds__(256) void ga_isd_pivot_kernel(const uint64_t* __res
     const int isl = blockIdx.x, tid = threadIdx.x;
     const uint64_t seed = isd_seed(seed0, round0 + blockIdx.y);
     uint64_t* tmask = tmask_all + (size_t)blockIdx.y * gridDim.x * rows * kIsdMaskWords;
-    const int wave = tid / kWave, lane = tid % kWave;
+    // The wavefront index as a uniform value: the draw loop under
+    // `wave == 0` then keeps its pivot search and counters in scalar registers.
+    const int wave = __builtin_amdgcn_readfirstlane(tid / kWave), lane = tid % kWave;
     const uint64_t* A = colmaj + (size_t)isl * words * 64 * kIsdMaskWords;   // columns
     __shared__ uint64_t colv[kIsdDraws][kIsdMaskWords];   // a(c) of the chunk's draws
     __shared__ int cidx[kIsdDraws];
     __shared__ uint64_t state[kIsdDraws];
+    __shared__ uint64_t tl[kIsdMaskWords * 64][kIsdMaskWords];   // T at a sub-batch start
+    __shared__ uint64_t vl[kWave][kIsdMaskWords];   // the sub-batch's reduced columns
     __shared__ int s_npiv;
     // Wave 0's registers: t[s] = T of row 64 s + lane; unpiv = rows not yet
     // pivoted (bit l of word s: row 64 s + l; the same on every lane).
@@ -1319,65 +1325,100 @@ __global__ __launch_bounds__(256) void ga_isd_pivot_kernel(const uint64_t* __res
             colv[i / kIsdMaskWords][i % kIsdMaskWords] =
                 A[(size_t)cidx[i / kIsdMaskWords] * kIsdMaskWords + i % kIsdMaskWords];
         __syncthreads();
-        if (wave == 0) {
-            uint64_t cv[kIsdMaskWords];   // lane l: the column of draw 64 (d / 64) + l
-            for (int d = 0; d < nd && npiv < rows; ++d) {
-                if ((d & 63) == 0) {
-                    const int dd = min(d + lane, nd - 1);
+        // Sub-batches of 64 draws.  All four wavefronts reduce the
+        // sub-batch's columns against T as it stands (lane l of wavefront w:
+        // word w of v_l = T a(c_l), the parities of rows 64 w + i), then
+        // wavefront 0 runs the draws in order: draw j pivots on the lowest
+        // unpivoted row r of v_j, rows i with v_j bit i (i != r) take T_r,
+        // and every later column follows: v_l bit i ^= v_j bit i & v_l bit r
+        // (i != r), so v_l stays T a(c_l) for the current T.
+        for (int e0 = 0; e0 < nd && npiv < rows; e0 += kWave) {
+            if (wave == 0) {
+#pragma unroll
+                for (int sl = 0; sl < kIsdMaskWords; ++sl)
 #pragma unroll
                     for (int q = 0; q < kIsdMaskWords; ++q)
-                        cv[q] = colv[dd][q];
-                }
+                        tl[64 * sl + lane][q] = t[sl][q];
+            }
+            __syncthreads();
+            {
+                const int dd = min(e0 + lane, nd - 1);
                 uint64_t a[kIsdMaskWords];
 #pragma unroll
                 for (int q = 0; q < kIsdMaskWords; ++q)
-                    a[q] = readlane_u64(cv[q], d & 63);
-                // bal[s] bit l: row 64 s + l has the column's bit.
-                uint64_t bal[kIsdMaskWords];
+                    a[q] = colv[dd][q];
+                uint32_t half[2] = {0u, 0u};
 #pragma unroll
-                for (int sl = 0; sl < kIsdMaskWords; ++sl) {
-                    uint32_t par = 0;
+                for (int h = 0; h < 2; ++h)
 #pragma unroll
-                    for (int q = 0; q < kIsdMaskWords; ++q)
-                        par += (uint32_t)__builtin_popcountll(t[sl][q] & a[q]);
-                    bal[sl] = __ballot((par & 1u) != 0u);
-                }
-                int r = -1;   // lowest row not yet pivoted that has the bit
-#pragma unroll
-                for (int sl = 0; sl < kIsdMaskWords; ++sl)
-                    if (r < 0 && (bal[sl] & unpiv[sl]) != 0ull)
-                        r = 64 * sl + __builtin_ctzll(bal[sl] & unpiv[sl]);
-                if (r < 0)
-                    continue;   // no row can pivot on this column
-                ++npiv;
-                const int rs = r / 64, rl = r % 64;
-                uint64_t tr[kIsdMaskWords];
-#pragma unroll
-                for (int q = 0; q < kIsdMaskWords; ++q) {
-                    uint64_t src = 0;
-#pragma unroll
-                    for (int sl = 0; sl < kIsdMaskWords; ++sl)
-                        if (sl == rs)
-                            src = t[sl][q];
-                    tr[q] = readlane_u64(src, rl);
-                }
-#pragma unroll
-                for (int sl = 0; sl < kIsdMaskWords; ++sl) {
-                    if (sl == rs)
-                        unpiv[sl] &= ~(1ull << rl);
-                    // Every other row with the bit takes the pivot row.
-                    if (((bal[sl] >> lane) & 1ull) && !(sl == rs && lane == rl)) {
+                    for (int i = 0; i < 32; ++i) {
+                        const uint64_t* tr = tl[64 * wave + 32 * h + i];
+                        uint32_t p = 0;
 #pragma unroll
                         for (int q = 0; q < kIsdMaskWords; ++q)
-                            t[sl][q] ^= tr[q];
+                            p += (uint32_t)__builtin_popcountll(tr[q] & a[q]);
+                        half[h] |= (p & 1u) << i;
+                    }
+                vl[lane][wave] = ((uint64_t)half[1] << 32) | half[0];
+            }
+            __syncthreads();
+            if (wave == 0) {
+                uint64_t vr[kIsdMaskWords];   // lane l: v of draw e0 + l
+#pragma unroll
+                for (int q = 0; q < kIsdMaskWords; ++q)
+                    vr[q] = vl[lane][q];
+                const int ne = min(kWave, nd - e0);
+                for (int j = 0; j < ne && npiv < rows; ++j) {
+                    // vj[s] bit l: row 64 s + l has the column's bit.
+                    uint64_t vj[kIsdMaskWords];
+#pragma unroll
+                    for (int q = 0; q < kIsdMaskWords; ++q)
+                        vj[q] = readlane_u64(vr[q], j);
+                    int r = -1;   // lowest row not yet pivoted that has the bit
+#pragma unroll
+                    for (int sl = 0; sl < kIsdMaskWords; ++sl)
+                        if (r < 0 && (vj[sl] & unpiv[sl]) != 0ull)
+                            r = 64 * sl + __builtin_ctzll(vj[sl] & unpiv[sl]);
+                    if (r < 0)
+                        continue;   // no row can pivot on this column
+                    ++npiv;
+                    const int rs = r / 64, rl = r % 64;
+                    uint64_t tr[kIsdMaskWords];
+#pragma unroll
+                    for (int q = 0; q < kIsdMaskWords; ++q) {
+                        uint64_t src = 0;
+#pragma unroll
+                        for (int sl = 0; sl < kIsdMaskWords; ++sl)
+                            if (sl == rs)
+                                src = t[sl][q];
+                        tr[q] = readlane_u64(src, rl);
+                    }
+                    uint64_t vrs = 0;   // this lane's column: row r's bit
+#pragma unroll
+                    for (int sl = 0; sl < kIsdMaskWords; ++sl) {
+                        if (sl == rs) {
+                            unpiv[sl] &= ~(1ull << rl);
+                            vrs = vr[sl];
+                        }
+                        // Every other row with the bit takes the pivot row.
+                        if (((vj[sl] >> lane) & 1ull) && !(sl == rs && lane == rl)) {
+#pragma unroll
+                            for (int q = 0; q < kIsdMaskWords; ++q)
+                                t[sl][q] ^= tr[q];
+                        }
+                    }
+                    if ((vrs >> rl) & 1ull) {
+#pragma unroll
+                        for (int q = 0; q < kIsdMaskWords; ++q)
+                            vr[q] ^= q == rs ? vj[q] & ~(1ull << rl) : vj[q];
                     }
                 }
+                if (lane == 0)
+                    s_npiv = npiv;
             }
-            if (lane == 0)
-                s_npiv = npiv;
+            __syncthreads();
+            npiv = s_npiv;
         }
-        __syncthreads();
-        npiv = s_npiv;
     }
     if (wave == 0) {
 #pragma unroll
@@ -1462,7 +1503,7 @@ __global__ __launch_bounds__(256) void ga_isd_select_kernel(int rows, int n_tria
                                                             int16_t* __restrict__ chosen,
                                                             unsigned int* __restrict__ n_isd) {
     const int isl = blockIdx.x, tid = threadIdx.x;
-    const int wave = tid / kWave, lane = tid % kWave;
+    const int wave = __builtin_amdgcn_readfirstlane(tid / kWave), lane = tid % kWave;
     const size_t n_isl = gridDim.x;
     __shared__ unsigned long long key[kIsdPool];
     __shared__ uint64_t tm[kIsdBatch * 256][kIsdMaskWords];
