@@ -1320,10 +1320,24 @@ __global__ __launch_bounds__(256) void ga_isd_pivot_kernel(const uint64_t* __res
         for (int d = tid; d < nd; d += blockDim.x)
             cidx[d] = (int)(state[d] % (uint64_t)cols);
         __syncthreads();
-        // The drawn columns: one 32-byte vector each.
-        for (int i = tid; i < nd * kIsdMaskWords; i += blockDim.x)
-            colv[i / kIsdMaskWords][i % kIsdMaskWords] =
-                A[(size_t)cidx[i / kIsdMaskWords] * kIsdMaskWords + i % kIsdMaskWords];
+        // The drawn columns: one 32-byte vector each, all of a thread's
+        // loads in flight at once (clamped indices, stores masked after).
+        {
+            constexpr int kPer = kIsdDraws * kIsdMaskWords / 256;
+            const int last = nd * kIsdMaskWords - 1;
+            uint64_t g[kPer];
+#pragma unroll
+            for (int k = 0; k < kPer; ++k) {
+                const int i = min(tid + 256 * k, last);
+                g[k] = A[(size_t)cidx[i / kIsdMaskWords] * kIsdMaskWords + i % kIsdMaskWords];
+            }
+#pragma unroll
+            for (int k = 0; k < kPer; ++k) {
+                const int i = tid + 256 * k;
+                if (i <= last)
+                    colv[i / kIsdMaskWords][i % kIsdMaskWords] = g[k];
+            }
+        }
         __syncthreads();
         // Sub-batches of 64 draws.  All four wavefronts reduce the
         // sub-batch's columns against T as it stands (lane l of wavefront w:
