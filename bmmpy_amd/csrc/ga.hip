@@ -1505,11 +1505,11 @@ __global__ __launch_bounds__(256) void ga_isd_rows_kernel(const uint64_t* __rest
 
 // GlobalGreedyApplier over the pool {R rows of each trial (masks T)} + {B
 // rows (unit masks)}, pool index = trial * rows + row, then rows + B row:
-// bitonic sort of (weight, pool index) keys, then one entry at a time
+// bitonic sort of (weight, pool index) keys, then the entries in order
 // against a fully reduced basis held in one wavefront's registers (reduce
 // by the pivots set in the mask, pivot = highest remaining bit, clear that
-// bit from the other basis rows).  chosen[k] = pool index of the k-th kept
-// entry.
+// bit from the other basis rows; entries reduced 64 at a time by all
+// wavefronts).  chosen[k] = pool index of the k-th kept entry.
 __global__ __launch_bounds__(256) void ga_isd_select_kernel(int rows, int n_trials,
                                                             const uint64_t* __restrict__ tmask,
                                                             const uint32_t* __restrict__ rw,
@@ -1557,89 +1557,137 @@ __global__ __launch_bounds__(256) void ga_isd_select_kernel(int rows, int n_tria
             }
             __syncthreads();
         }
-    // Selection on one wavefront, the basis held by columns: basis row r is
-    // the row whose pivot is bit r, and lane l holds, for its bit positions
+    // Selection with the basis held by columns: basis row r is the row whose
+    // pivot is bit r, and wavefront 0's lane l holds, for its bit positions
     // j = 64 s + l (s < 4), the 256-bit column col[s] (bit r = row r's bit
     // j).  The basis stays fully reduced (no row holds another row's pivot
     // bit), so an entry's mask m reduces to v = m ^ x with x_j =
-    // parity(col_j & m & used): four parities per lane and four ballots, no
-    // cross-lane reductions.  A new row v of pivot pv is XORed into the rows
-    // holding bit pv (column pv, broadcast from its lane) and installed as
-    // row pv.
-    if (wave != 0)
-        return;
-    uint64_t col[kIsdMaskWords][kIsdMaskWords] = {};   // [slot s][word of r]
+    // parity(col_j & m & used).  Entries go in batches of 64: all four
+    // wavefronts reduce the batch against the basis as it stands (the
+    // columns staged in LDS; wavefront w forms word w of every entry's v),
+    // then wavefront 0 takes the entries in order.  An entry with v = 0 is
+    // dependent; otherwise v (pivot pv = its highest bit) is XORed into the
+    // rows holding bit pv (column pv, broadcast from its lane) and installed
+    // as row pv, and every later entry of the batch follows with
+    // v_l ^= v_l bit pv ? v : 0, which is its reduction against the new
+    // basis.
+    __shared__ uint64_t cl[kIsdMaskWords * 64][kIsdMaskWords];   // basis columns
+    __shared__ uint64_t s_used[kIsdMaskWords];
+    __shared__ uint64_t vb[kWave][kIsdMaskWords];                // the batch's v
+    __shared__ int s_count;
+    uint64_t col[kIsdMaskWords][kIsdMaskWords] = {};   // [slot s][word of r] (wavefront 0)
     uint64_t used[kIsdMaskWords] = {};                  // pivot rows taken (uniform)
     int count = 0, kept_isd = 0;
-    int il = 0;                    // lane l: pool index of entry 64 (e / 64) + l
-    uint64_t ml[kIsdMaskWords];    // and its mask
-    for (int e = 0; e < pool && count < rows; ++e) {
-        if ((e & 63) == 0) {
-            il = (int)(key[min(e + lane, pool - 1)] & (kIsdPool - 1u));
-#pragma unroll
-            for (int q = 0; q < kIsdMaskWords; ++q)
-                ml[q] = il < n_r ? tm[il][q]
-                                 : ((il - n_r) / 64 == q ? 1ull << ((il - n_r) % 64) : 0ull);
-        }
-        const int idx = __builtin_amdgcn_readlane(il, e & 63);
-        uint64_t m[kIsdMaskWords], mu[kIsdMaskWords];
-#pragma unroll
-        for (int q = 0; q < kIsdMaskWords; ++q) {
-            m[q] = readlane_u64(ml[q], e & 63);
-            mu[q] = m[q] & used[q];
-        }
-        uint64_t v[kIsdMaskWords];
-#pragma unroll
-        for (int sl = 0; sl < kIsdMaskWords; ++sl) {
-            uint64_t a = 0;
-#pragma unroll
-            for (int q = 0; q < kIsdMaskWords; ++q)
-                a ^= col[sl][q] & mu[q];
-            const bool xb = (__builtin_popcountll(a) & 1) != 0;
-            v[sl] = m[sl] ^ (uint64_t)__ballot(xb);
-        }
-        int pv = -1;   // highest set bit of v (the same on every lane)
-#pragma unroll
-        for (int q = kIsdMaskWords - 1; q >= 0; --q)
-            if (pv < 0 && v[q])
-                pv = q * 64 + 63 - __builtin_clzll(v[q]);
-        if (pv < 0)
-            continue;   // dependent on the kept entries
-        const int ps = pv / 64, pl = pv % 64;
-        // Column pv: the rows that hold bit pv.
-        uint64_t cp[kIsdMaskWords];
-#pragma unroll
-        for (int q = 0; q < kIsdMaskWords; ++q) {
-            uint64_t src = 0;
+    for (int e0 = 0; e0 < pool && count < rows; e0 += kWave) {
+        if (wave == 0) {
 #pragma unroll
             for (int sl = 0; sl < kIsdMaskWords; ++sl)
-                if (sl == ps)
-                    src = col[sl][q];
-            const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)src, pl);
-            const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(src >> 32), pl);
-            cp[q] = ((uint64_t)hi << 32) | lo;
-        }
-        // Rows holding bit pv take v (clearing their bit pv); row pv = v.
 #pragma unroll
-        for (int sl = 0; sl < kIsdMaskWords; ++sl)
-            if ((v[sl] >> lane) & 1ull) {
+                for (int q = 0; q < kIsdMaskWords; ++q)
+                    cl[64 * sl + lane][q] = col[sl][q];
+            if (lane < kIsdMaskWords) {
+                uint64_t u = 0;
+#pragma unroll
+                for (int q = 0; q < kIsdMaskWords; ++q)
+                    if (q == lane)
+                        u = used[q];
+                s_used[lane] = u;
+            }
+        }
+        __syncthreads();
+        // Entry e0 + l of the sorted pool: pool index and mask.
+        const int il = (int)(key[min(e0 + lane, pool - 1)] & (kIsdPool - 1u));
+        {
+            uint64_t mu[kIsdMaskWords], mw = 0;
+#pragma unroll
+            for (int q = 0; q < kIsdMaskWords; ++q) {
+                const uint64_t m = il < n_r ? tm[il][q]
+                                            : ((il - n_r) / 64 == q ? 1ull << ((il - n_r) % 64) : 0ull);
+                mu[q] = m & s_used[q];
+                if (q == wave)
+                    mw = m;
+            }
+            uint32_t half[2] = {0u, 0u};
+#pragma unroll
+            for (int h = 0; h < 2; ++h)
+#pragma unroll
+                for (int i = 0; i < 32; ++i) {
+                    const uint64_t* cj = cl[64 * wave + 32 * h + i];
+                    uint64_t a = 0;
+#pragma unroll
+                    for (int q = 0; q < kIsdMaskWords; ++q)
+                        a ^= cj[q] & mu[q];
+                    half[h] |= ((uint32_t)__builtin_popcountll(a) & 1u) << i;
+                }
+            vb[lane][wave] = mw ^ (((uint64_t)half[1] << 32) | half[0]);
+        }
+        __syncthreads();
+        if (wave == 0) {
+            uint64_t vr[kIsdMaskWords];   // lane l: v of entry e0 + l
+#pragma unroll
+            for (int q = 0; q < kIsdMaskWords; ++q)
+                vr[q] = vb[lane][q];
+            const int ne = min(kWave, pool - e0);
+            for (int j = 0; j < ne && count < rows; ++j) {
+                uint64_t v[kIsdMaskWords];
+#pragma unroll
+                for (int q = 0; q < kIsdMaskWords; ++q)
+                    v[q] = readlane_u64(vr[q], j);
+                int pv = -1;   // highest set bit of v (the same on every lane)
+#pragma unroll
+                for (int q = kIsdMaskWords - 1; q >= 0; --q)
+                    if (pv < 0 && v[q])
+                        pv = q * 64 + 63 - __builtin_clzll(v[q]);
+                if (pv < 0)
+                    continue;   // dependent on the kept entries
+                const int idx = __builtin_amdgcn_readlane(il, j);
+                const int ps = pv / 64, pl = pv % 64;
+                // Column pv: the rows that hold bit pv.
+                uint64_t cp[kIsdMaskWords];
 #pragma unroll
                 for (int q = 0; q < kIsdMaskWords; ++q) {
-                    col[sl][q] ^= cp[q];
-                    if (q == ps)
-                        col[sl][q] |= 1ull << pl;
-                }
-            }
+                    uint64_t src = 0;
 #pragma unroll
-        for (int q = 0; q < kIsdMaskWords; ++q)
-            if (q == ps)
-                used[q] |= 1ull << pl;
-        if (lane == 0)
-            chosen[(size_t)isl * rows + count] = (int16_t)idx;
-        ++count;
-        kept_isd += idx < n_r ? 1 : 0;
+                    for (int sl = 0; sl < kIsdMaskWords; ++sl)
+                        if (sl == ps)
+                            src = col[sl][q];
+                    cp[q] = readlane_u64(src, pl);
+                }
+                // Rows holding bit pv take v (clearing their bit pv); row pv = v.
+#pragma unroll
+                for (int sl = 0; sl < kIsdMaskWords; ++sl)
+                    if ((v[sl] >> lane) & 1ull) {
+#pragma unroll
+                        for (int q = 0; q < kIsdMaskWords; ++q) {
+                            col[sl][q] ^= cp[q];
+                            if (q == ps)
+                                col[sl][q] |= 1ull << pl;
+                        }
+                    }
+                uint64_t vps = 0;   // this lane's entry: bit pv
+#pragma unroll
+                for (int q = 0; q < kIsdMaskWords; ++q)
+                    if (q == ps) {
+                        used[q] |= 1ull << pl;
+                        vps = vr[q];
+                    }
+                if ((vps >> pl) & 1ull) {
+#pragma unroll
+                    for (int q = 0; q < kIsdMaskWords; ++q)
+                        vr[q] ^= v[q];
+                }
+                if (lane == 0)
+                    chosen[(size_t)isl * rows + count] = (int16_t)idx;
+                ++count;
+                kept_isd += idx < n_r ? 1 : 0;
+            }
+            if (lane == 0)
+                s_count = count;
+        }
+        __syncthreads();
+        count = s_count;
     }
-    if (lane == 0 && kept_isd)
+    if (wave == 0 && lane == 0 && kept_isd)
         atomicAdd(n_isd, (unsigned)kept_isd);
 }
 
