@@ -824,10 +824,23 @@ __global__ __launch_bounds__(256) void ga_li_tile_kernel(const uint64_t* __restr
     uint32_t acc[2][2] = {};
     for (int w0 = 0; w0 < words; w0 += kLiChunk) {
         const int cw = min(kLiChunk, words - w0);
-        for (int q = tid; q < kLiTile * kLiChunk; q += blockDim.x) {
-            const int r = q / kLiChunk, w = q % kLiChunk;
-            A[w][r] = (w < cw && ra[r] >= 0) ? b[(size_t)ra[r] * words + w0 + w] : 0ull;
-            B[w][r] = (w < cw && rb[r] >= 0) ? b[(size_t)rb[r] * words + w0 + w] : 0ull;
+        {
+            // Every load of the thread in flight at once: clamped rows and
+            // words, masked when stored.
+            constexpr int kPer = kLiTile * kLiChunk / 256;
+            uint64_t xa[kPer], xb[kPer];
+#pragma unroll
+            for (int k = 0; k < kPer; ++k) {
+                const int q = tid + 256 * k, r = q / kLiChunk, w = min(q % kLiChunk, cw - 1);
+                xa[k] = b[(size_t)max(ra[r], 0) * words + w0 + w];
+                xb[k] = b[(size_t)max(rb[r], 0) * words + w0 + w];
+            }
+#pragma unroll
+            for (int k = 0; k < kPer; ++k) {
+                const int q = tid + 256 * k, r = q / kLiChunk, w = q % kLiChunk;
+                A[w][r] = (w < cw && ra[r] >= 0) ? xa[k] : 0ull;
+                B[w][r] = (w < cw && rb[r] >= 0) ? xb[k] : 0ull;
+            }
         }
         __syncthreads();
 #pragma unroll 4
