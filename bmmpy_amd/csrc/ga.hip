@@ -1462,10 +1462,14 @@ __global__ __launch_bounds__(256) void ga_isd_pivot_kernel(const uint64_t* __res
 }
 
 // R = T B for one island and a slice of kIsdSlice words (grid: slices x
-// islands): the slice of B staged in LDS, thread r XORs the staged rows its
-// mask selects (a uniform loop over rows: broadcast LDS reads), stores its
-// slice of R and adds the slice's popcount to rw (zeroed before).
-constexpr int kIsdSlice = 16;
+// islands x trials), by the method of four Russians: the base rows in
+// groups of four, each group's 16 subset XORs of the slice built once in LDS
+// (entries padded to 80 bytes, so the 16 entries of a group fall on
+// different banks), then thread r XORs, for each group, the entry its mask
+// nibble names (64 table reads instead of a masked XOR per base row),
+// stores its slice of R and adds the slice's popcount to rw (zeroed before).
+constexpr int kIsdSlice = 8;
+constexpr int kIsdTabStride = kIsdSlice + 2;   // u64 per table entry
 
 __global__ __launch_bounds__(256) void ga_isd_rows_kernel(const uint64_t* __restrict__ base,
                                                           int rows, int words,
@@ -1479,10 +1483,30 @@ __global__ __launch_bounds__(256) void ga_isd_rows_kernel(const uint64_t* __rest
     uint64_t* rbuf = rbuf_all + (size_t)blockIdx.z * n_isl * per;
     uint32_t* rw = rw_all + (size_t)blockIdx.z * n_isl * rows;
     const uint64_t* B = base + (size_t)isl * per;
-    __shared__ uint64_t a[256][kIsdSlice];
+    __shared__ __align__(16) uint64_t tab[64 * 16 * kIsdTabStride];
     const int w0 = slice * kIsdSlice;
-    for (int i = tid; i < rows * kIsdSlice; i += blockDim.x)
-        a[i / kIsdSlice][i % kIsdSlice] = B[(size_t)(i / kIsdSlice) * words + w0 + i % kIsdSlice];
+    // Thread p builds word k of group g's 16 entries (p = 8 g + k, two
+    // (g, k) pairs per thread): loads at clamped rows, masked after.
+#pragma unroll
+    for (int h = 0; h < 64 * kIsdSlice / 256; ++h) {
+        const int p = tid + 256 * h, g = p / kIsdSlice, k = p % kIsdSlice;
+        uint64_t x[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int row = 4 * g + r;
+            const uint64_t v = B[(size_t)min(row, rows - 1) * words + w0 + k];
+            x[r] = row < rows ? v : 0ull;
+        }
+        uint64_t e[16];
+        e[0] = 0ull;
+#pragma unroll
+        for (int m = 1; m < 16; ++m)
+            e[m] = e[m & (m - 1)] ^ x[__builtin_ctz(m)];
+        uint64_t* T = tab + (size_t)g * 16 * kIsdTabStride + k;
+#pragma unroll
+        for (int m = 0; m < 16; ++m)
+            T[m * kIsdTabStride] = e[m];
+    }
     uint64_t t[kIsdMaskWords] = {};
     if (tid < rows) {
         const uint64_t* T = tmask + ((size_t)isl * rows + tid) * kIsdMaskWords;
@@ -1493,15 +1517,15 @@ __global__ __launch_bounds__(256) void ga_isd_rows_kernel(const uint64_t* __rest
     __syncthreads();
     uint64_t acc[kIsdSlice] = {};
 #pragma unroll
-    for (int q = 0; q < kIsdMaskWords; ++q) {   // rows 64 q .. 64 q + 63
+    for (int q = 0; q < kIsdMaskWords; ++q) {   // groups 16 q .. 16 q + 15
         const uint64_t tq = t[q];
-        const int n = min(64, rows - 64 * q);
-        for (int jj = 0; jj < n; ++jj) {
-            const uint64_t m = 0ull - ((tq >> jj) & 1ull);
-            const uint64_t* aj = a[64 * q + jj];
+#pragma unroll 4
+        for (int gg = 0; gg < 16; ++gg) {
+            const int nib = (int)((tq >> (4 * gg)) & 15ull);
+            const uint64_t* en = tab + (size_t)((16 * q + gg) * 16 + nib) * kIsdTabStride;
 #pragma unroll
             for (int k = 0; k < kIsdSlice; ++k)
-                acc[k] ^= aj[k] & m;
+                acc[k] ^= en[k];
         }
     }
     if (tid < rows) {
