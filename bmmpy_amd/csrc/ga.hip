@@ -1395,7 +1395,22 @@ __global__ __launch_bounds__(256) void ga_isd_pivot_kernel(const uint64_t* __res
                 for (int q = 0; q < kIsdMaskWords; ++q)
                     vr[q] = vl[lane][q];
                 const int ne = min(kWave, nd - e0);
-                for (int j = 0; j < ne && npiv < rows; ++j) {
+                // Draws of the sub-batch not yet taken.  A draw whose column
+                // has no bit in an unpivoted row never gains one (unpiv only
+                // shrinks, and the column changes only when it holds the new
+                // pivot's bit), so the loop jumps from one pivoting draw to
+                // the next: the draws that would be skipped in order.
+                uint64_t rem = ne >= kWave ? ~0ull : ((1ull << ne) - 1ull);
+                while (npiv < rows) {
+                    uint64_t any = 0;
+#pragma unroll
+                    for (int q = 0; q < kIsdMaskWords; ++q)
+                        any |= vr[q] & unpiv[q];
+                    const uint64_t live = (uint64_t)__ballot(any != 0ull) & rem;
+                    if (live == 0ull)
+                        break;
+                    const int j = __builtin_ctzll(live);
+                    rem &= j == kWave - 1 ? 0ull : ~((2ull << j) - 1ull);
                     // vj[s] bit l: row 64 s + l has the column's bit.
                     uint64_t vj[kIsdMaskWords];
 #pragma unroll
@@ -1406,8 +1421,6 @@ __global__ __launch_bounds__(256) void ga_isd_pivot_kernel(const uint64_t* __res
                     for (int sl = 0; sl < kIsdMaskWords; ++sl)
                         if (r < 0 && (vj[sl] & unpiv[sl]) != 0ull)
                             r = 64 * sl + __builtin_ctzll(vj[sl] & unpiv[sl]);
-                    if (r < 0)
-                        continue;   // no row can pivot on this column
                     ++npiv;
                     const int rs = r / 64, rl = r % 64;
                     uint64_t tr[kIsdMaskWords];
@@ -1665,7 +1678,17 @@ __global__ __launch_bounds__(256) void ga_isd_select_kernel(int rows, int n_tria
             for (int q = 0; q < kIsdMaskWords; ++q)
                 vr[q] = vb[lane][q];
             const int ne = min(kWave, pool - e0);
-            for (int j = 0; j < ne && count < rows; ++j) {
+            // Entries of the batch not yet taken.  A dependent entry (v = 0)
+            // stays dependent (the update adds v only where v_l has the new
+            // pivot's bit), so the loop jumps from one kept entry to the next.
+            uint64_t rem = ne >= kWave ? ~0ull : ((1ull << ne) - 1ull);
+            while (count < rows) {
+                const uint64_t live =
+                    (uint64_t)__ballot((vr[0] | vr[1] | vr[2] | vr[3]) != 0ull) & rem;
+                if (live == 0ull)
+                    break;
+                const int j = __builtin_ctzll(live);
+                rem &= j == kWave - 1 ? 0ull : ~((2ull << j) - 1ull);
                 uint64_t v[kIsdMaskWords];
 #pragma unroll
                 for (int q = 0; q < kIsdMaskWords; ++q)
@@ -1675,8 +1698,6 @@ __global__ __launch_bounds__(256) void ga_isd_select_kernel(int rows, int n_tria
                 for (int q = kIsdMaskWords - 1; q >= 0; --q)
                     if (pv < 0 && v[q])
                         pv = q * 64 + 63 - __builtin_clzll(v[q]);
-                if (pv < 0)
-                    continue;   // dependent on the kept entries
                 const int idx = __builtin_amdgcn_readlane(il, j);
                 const int ps = pv / 64, pl = pv % 64;
                 // Column pv: the rows that hold bit pv.
