@@ -1270,12 +1270,14 @@ __global__ __launch_bounds__(256) void ga_isd_transpose_kernel(const uint64_t* _
     const uint64_t x =
         tid < rows ? base[((size_t)isl * rows + tid) * words + w] : 0ull;
     uint64_t* out = colmaj + ((size_t)isl * words * 64 + (size_t)w * 64) * kIsdMaskWords;
+    uint64_t mine = 0;   // lane b keeps bit b's ballot: one store per lane
 #pragma unroll 8
     for (int b = 0; b < 64; ++b) {
         const uint64_t bal = __ballot((x >> b) & 1ull);
-        if (lane == 0)
-            out[(size_t)b * kIsdMaskWords + wave] = bal;
+        if (lane == b)
+            mine = bal;
     }
+    out[(size_t)lane * kIsdMaskWords + wave] = mine;
 }
 
 __device__ inline uint64_t readlane_u64(uint64_t x, int lane) {
